@@ -1,0 +1,219 @@
+"""Synthetic workloads: meshes, tracers, velocities and time steps for the benchmark configurations.
+
+A *case* bundles one mesh, its element velocities ``uv(2, nl-1, elem)`` (stored ``(T, nl-1, 2)``),
+vertical velocities ``w, w_i, w_e (nl, nod2D)`` (stored ``(N, nl)``), the tracers
+``values(nl-1, nod2D)`` (stored ``(ntr, N, nl-1)``), the per-tracer scheme selection and ``dt``.
+
+Workloads (sizes fixed here; seeds make every array reproducible bit for bit):
+
+* ``analytic20``  20x20 periodic cells, 2000 km, 10 levels of 10 m, T Gaussian 0-20, S 34-36,
+  psi = A sin sin with max|u| = 0.1 m/s, w = 0, MFCT/QR4C/FCT, CFL 0.5, 10 steps.
+* ``mesh1000``    1000x1000 periodic, 48 levels stretched 5-250 m, smooth random bathymetry with
+  10-48 levels per element, randomly permuted numbering, velocity decaying as exp(-|z|/500 m),
+  2 tracers, 100 steps (FP64).
+* ``mesh500x64``  500x500 periodic, 48 levels, bathymetry as above, 64 tracers (T, S and 62 passive
+  tracers made of 4 random Gaussian blobs with amplitudes in [0, 1]), 200 steps.
+* ``multimesh32`` 32 independent periodic meshes, nx = ny log-uniform in [64, 512], nl uniform in
+  [10, 80], psi amplitude U[0.05, 0.5] m/s, own bathymetry, 4 tracers each, 100 steps.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from .mesh import Mesh, periodic_mesh, smooth_bathymetry_levels, stretched_layers
+
+# Scheme names as the reference spells them (lib/oce_adv_tra_driver.F90:244-282,286).
+HOR_SCHEMES = ("UPW1", "MUSCL", "MFCT")
+VER_SCHEMES = ("UPW1", "QR4C", "CDIFF", "PPM")
+LIM_SCHEMES = ("NONE", "FCT")
+
+
+@dataclasses.dataclass
+class Scheme:
+    hor: str = "MFCT"
+    ver: str = "QR4C"
+    lim: str = "FCT"
+    ph: float = 1.0
+    pv: float = 1.0
+
+    def __post_init__(self):
+        if self.hor not in HOR_SCHEMES:
+            raise ValueError(f"unknown horizontal advection type {self.hor!r}")
+        if self.ver not in VER_SCHEMES:
+            raise ValueError(f"unknown vertical advection type {self.ver!r}")
+        if self.lim not in LIM_SCHEMES:
+            raise ValueError(f"unknown limiter {self.lim!r}")
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    mesh: Mesh
+    uv: np.ndarray          # (T, nl-1, 2)
+    w: np.ndarray           # (N, nl)
+    wi: np.ndarray
+    we: np.ndarray
+    tracers: np.ndarray     # (ntr, N, nl-1)
+    schemes: List[Scheme]
+    dt: float
+    nsteps: int
+
+    @property
+    def ntracers(self) -> int:
+        return int(self.tracers.shape[0])
+
+    def node_level_updates_per_step(self) -> int:
+        return self.ntracers * self.mesh.n_node_levels
+
+
+def node_xy_m(mesh: Mesh) -> np.ndarray:
+    return mesh.xy_m
+
+
+def _periodic_dist2(mesh: Mesh, x0: float, y0: float) -> np.ndarray:
+    xy = mesh.xy_m
+    dx = xy[:, 0] - x0
+    dy = xy[:, 1] - y0
+    dx = dx - mesh.Lx * np.round(dx / mesh.Lx)
+    dy = dy - mesh.Ly * np.round(dy / mesh.Ly)
+    return dx * dx + dy * dy
+
+
+def wet_mask(mesh: Mesh) -> np.ndarray:
+    """(N, nl-1) True on wet node layers."""
+    k = np.arange(1, mesh.nl)[None, :]
+    return (k >= mesh.ulevels_nod2D[:, None]) & (k <= (mesh.nlevels_nod2D - 1)[:, None])
+
+
+def gaussian_tracer(mesh: Mesh, amp: float, sigma: float, x0: float, y0: float,
+                    base: float = 0.0) -> np.ndarray:
+    """base + amp*exp(-r^2/(2 sigma^2)) in every wet layer, 0 below the bottom."""
+    g = base + amp * np.exp(-_periodic_dist2(mesh, x0, y0) / (2.0 * sigma * sigma))
+    vals = np.broadcast_to(g[:, None], (mesh.nod2D, mesh.nl - 1)).copy()
+    return np.where(wet_mask(mesh), vals, 0.0)
+
+
+def linear_salinity(mesh: Mesh) -> np.ndarray:
+    s = 34.0 + 2.0 * mesh.xy_m[:, 1] / mesh.Ly
+    vals = np.broadcast_to(s[:, None], (mesh.nod2D, mesh.nl - 1)).copy()
+    return np.where(wet_mask(mesh), vals, 0.0)
+
+
+def blob_tracer(mesh: Mesh, rng: np.random.Generator, n_blobs: int = 4) -> np.ndarray:
+    g = np.zeros(mesh.nod2D)
+    for _ in range(n_blobs):
+        x0 = rng.uniform(0, mesh.Lx)
+        y0 = rng.uniform(0, mesh.Ly)
+        sig = rng.uniform(0.04, 0.15) * mesh.Lx
+        g += rng.uniform(0.0, 1.0) * np.exp(-_periodic_dist2(mesh, x0, y0) / (2 * sig * sig))
+    vals = np.broadcast_to(g[:, None], (mesh.nod2D, mesh.nl - 1)).copy()
+    return np.where(wet_mask(mesh), vals, 0.0)
+
+
+def streamfunction_velocity(mesh: Mesh, umax: float,
+                            depth_scale: Optional[float] = None) -> np.ndarray:
+    """Element velocities (u, v) = (-d psi/dy, d psi/dx) of the P1 interpolant of
+    psi = A sin(2 pi x/Lx) sin(2 pi y/Ly), A chosen so the largest element speed is ``umax``.
+
+    The discrete velocity is divergence free on the median-dual cells (the flux through each dual
+    segment telescopes to psi differences), so a uniform tracer stays uniform where all levels are
+    wet.  ``depth_scale`` multiplies layer k by exp(-|Z_k|/depth_scale).
+    """
+    xy = mesh.xy_m
+    psi = np.sin(2 * math.pi * xy[:, 0] / mesh.Lx) * np.sin(2 * math.pi * xy[:, 1] / mesh.Ly)
+    en = mesh.elem2D_nodes - 1
+    g = mesh.gradient_sca
+    dpsidx = g[:, 0] * psi[en[:, 0]] + g[:, 1] * psi[en[:, 1]] + g[:, 2] * psi[en[:, 2]]
+    dpsidy = g[:, 3] * psi[en[:, 0]] + g[:, 4] * psi[en[:, 1]] + g[:, 5] * psi[en[:, 2]]
+    u = -dpsidy
+    v = dpsidx
+    A = umax / np.max(np.hypot(u, v))
+    fac = np.ones(mesh.nl - 1)
+    if depth_scale is not None:
+        fac = np.exp(-np.abs(mesh.Z) / depth_scale)
+    uv = np.empty((mesh.elem2D, mesh.nl - 1, 2), dtype=np.float64)
+    uv[:, :, 0] = (A * u)[:, None] * fac[None, :]
+    uv[:, :, 1] = (A * v)[:, None] * fac[None, :]
+    return uv
+
+
+def cfl_dt(mesh: Mesh, umax: float, cfl: float = 0.5) -> float:
+    dx = min(mesh.Lx / mesh.nx, mesh.Ly / mesh.ny)
+    return cfl * dx / umax
+
+
+def analytic20(scheme: Optional[Scheme] = None, nsteps: int = 10) -> Case:
+    """20x20 cells, 2000 km square, 10 levels x 10 m, T Gaussian + S linear, psi flow, w = 0."""
+    Lx = Ly = 2.0e6
+    mesh = periodic_mesh(20, 20, Lx, Ly, np.full(9, 10.0))
+    umax = 0.1
+    uv = streamfunction_velocity(mesh, umax)
+    zeros = np.zeros((mesh.nod2D, mesh.nl))
+    T = gaussian_tracer(mesh, 20.0, Lx / 10.0, 0.5 * Lx, 0.5 * Ly)
+    S = linear_salinity(mesh)
+    sc = scheme or Scheme()
+    return Case("analytic20", mesh, uv, zeros, zeros.copy(), zeros.copy(), np.stack([T, S]),
+                [sc, dataclasses.replace(sc)], cfl_dt(mesh, umax), nsteps)
+
+
+def bathymetry_case(name: str, n: int, nl: int, seed: int, ntracers: int = 2,
+                    permute: bool = True, nsteps: int = 100, umax: float = 0.1,
+                    depth_scale: Optional[float] = 500.0, lev_lo: int = 10,
+                    lev_hi: Optional[int] = None, L: float = 2.0e6,
+                    scheme: Optional[Scheme] = None, w_amp: float = 0.0) -> Case:
+    """n x n periodic mesh with nl levels stretched 5-250 m and smooth random bathymetry."""
+    ss = np.random.SeedSequence(seed)
+    s_bath, s_perm, s_tr, s_w = ss.spawn(4)
+    layer_h = stretched_layers(nl - 1, 5.0, 250.0)
+    hi = nl if lev_hi is None else lev_hi
+    lev = smooth_bathymetry_levels(n, n, nl, min(lev_lo, hi), hi,
+                                   int(s_bath.generate_state(1)[0]))
+    mesh = periodic_mesh(n, n, L, L, layer_h, elem_nlevels=lev,
+                         seed=int(s_perm.generate_state(1)[0]), permute=permute)
+    uv = streamfunction_velocity(mesh, umax, depth_scale)
+    w = np.zeros((mesh.nod2D, mesh.nl))
+    if w_amp:
+        # Interior-only vertical velocity (surface and bottom interfaces stay 0), smooth in x, y.
+        rng = np.random.default_rng(s_w)
+        xy = mesh.xy_m
+        ph = rng.uniform(0, 2 * math.pi, 2)
+        horiz = np.sin(2 * math.pi * xy[:, 0] / L + ph[0]) * np.cos(2 * math.pi * xy[:, 1] / L + ph[1])
+        k = np.arange(1, mesh.nl + 1)[None, :]
+        interior = (k > mesh.ulevels_nod2D[:, None]) & (k < mesh.nlevels_nod2D[:, None])
+        prof = np.sin(math.pi * (k - 1) / (mesh.nl - 1))
+        w = np.where(interior, w_amp * horiz[:, None] * prof, 0.0)
+    rng = np.random.default_rng(s_tr)
+    tr = [gaussian_tracer(mesh, 20.0, L / 10.0, 0.5 * L, 0.5 * L), linear_salinity(mesh)]
+    while len(tr) < ntracers:
+        tr.append(blob_tracer(mesh, rng))
+    tracers = np.stack(tr[:ntracers])
+    sc = scheme or Scheme()
+    return Case(name, mesh, uv, w, w.copy(), w.copy(), tracers,
+                [dataclasses.replace(sc) for _ in range(ntracers)], cfl_dt(mesh, umax), nsteps)
+
+
+def mesh1000(nsteps: int = 100) -> Case:
+    return bathymetry_case("mesh1000", 1000, 48, seed=1, ntracers=2, nsteps=nsteps)
+
+
+def mesh500x64(nsteps: int = 200) -> Case:
+    return bathymetry_case("mesh500x64", 500, 48, seed=3, ntracers=64, nsteps=nsteps)
+
+
+def multimesh32(nsteps: int = 100, count: int = 32, seed: int = 4,
+                size_range: Sequence[int] = (64, 512)) -> List[Case]:
+    rng = np.random.default_rng(seed)
+    cases = []
+    lo, hi = size_range
+    for i in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        nl = int(rng.integers(10, 81))
+        umax = float(rng.uniform(0.05, 0.5))
+        cases.append(bathymetry_case(f"multimesh32[{i}]", n, nl, seed=1000 + i, ntracers=4,
+                                     nsteps=nsteps, umax=umax, lev_lo=min(10, nl), lev_hi=nl))
+    return cases
