@@ -1,0 +1,1075 @@
+// C ABI of the tracer advection library (declared in include/fesomtra.h).
+//
+// Mirrors lib/tracer_c_interface.F90: one process-wide instance, int status codes, copy-in /
+// copy-out of Fortran-ordered arrays with 1-based indices.  At tracer_init_complete the mesh is
+// validated, renumbered along a Hilbert curve (nodes) with elements and edges following their
+// nodes, and uploaded to HBM together with the CSR node->edge lists (kept in the reference's
+// edge order so every gathered sum is added in the same order as the reference's scatter).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/fesomtra.h"
+#include "fesomtra_device.hpp"
+#include "fesomtra_kernels.hpp"
+
+using namespace fesomtra;
+
+namespace {
+
+struct SchemeSel {
+    int hor = HOR_UPW1, ver = VER_UPW1;
+    bool fct = false;
+    double ph = 1.0, pv = 1.0;
+};
+
+struct State {
+    int nl = 0, N = 0, E = 0, T = 0, max_adj = 0;
+    bool dims = false;
+    std::vector<double> coord;
+    std::vector<int> enodes, edges, etri, nie, nie_num;
+    std::vector<int> nlev_n, ulev_n, nlev_e, ulev_e, nbl, updn;
+    std::vector<double> elem_area, area, areasvol, edxdy, ecd, elem_cos, gsca;
+    std::vector<double> hnode, hnode_new, helem, Z, zbar3, zbar;
+    std::vector<double> uv, w, wi, we;
+    bool have_coord = false, have_conn = false, have_nlev = false, have_elev = false;
+    bool have_geom = false, have_vert = false, have_hnode = false, have_muscl = false;
+    bool have_vel = false, have_zbar = false;
+    int myDim = -1;
+    int ntr = 0;
+    std::vector<std::vector<double>> values;
+    std::vector<SchemeSel> schemes;
+    bool initialized = false;  // tracer_init_complete / reference "tracer_initialized"
+
+    // device side
+    bool on_device = false;
+    hipStream_t stream = nullptr;
+    std::vector<int> perm_n, inv_n, perm_t, inv_t, perm_e, inv_e;
+    MeshDev md{};
+    std::vector<void *> allocs;
+    std::vector<double *> dT, dTAB;
+    double *flo = nullptr, *adfh = nullptr, *LO = nullptr, *Rp = nullptr, *Rm = nullptr;
+    double *adfv = nullptr, *acc_h = nullptr, *acc_v = nullptr, *dbg_h = nullptr, *dbg_v = nullptr;
+    int *d_perm_n = nullptr, *d_inv_n = nullptr;
+    double4 *part = nullptr;
+    bool last_fct = false;
+    double last_ms = 0.0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+State *g = nullptr;
+std::string g_err;
+constexpr int kStatsBlocks = 1024;
+
+State &S() {
+    if (!g) g = new State();
+    return *g;
+}
+
+int fail(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    fprintf(stderr, "### error: %s\n", buf);
+    return -1;
+}
+
+#define HIPCHK(call)                                                                  \
+    do {                                                                              \
+        hipError_t _e = (call);                                                       \
+        if (_e != hipSuccess) return fail("%s failed: %s", #call, hipGetErrorString(_e)); \
+    } while (0)
+
+template <class T>
+int dalloc(State &s, T **p, size_t count) {
+    void *q = nullptr;
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    s.allocs.push_back(q);
+    *p = static_cast<T *>(q);
+    return 0;
+}
+
+template <class T>
+int upload(State &s, T **p, const std::vector<T> &h) {
+    if (dalloc(s, p, h.size())) return -1;
+    if (!h.empty()) HIPCHK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+void release_device(State &s) {
+    if (s.stream) hipStreamSynchronize(s.stream);
+    for (void *p : s.allocs) hipFree(p);
+    s.allocs.clear();
+    s.dT.clear();
+    s.dTAB.clear();
+    s.on_device = false;
+    s.md = MeshDev{};
+}
+
+int check_dims(const State &s, int nl, int nod2D, const char *who) {
+    if (!s.dims) return fail("%s: call tracer_set_mesh_dims first", who);
+    if (nl != s.nl) return fail("%s: nl=%d does not match mesh nl=%d", who, nl, s.nl);
+    if (nod2D != s.N) return fail("%s: nod2D=%d does not match mesh nod2D=%d", who, nod2D, s.N);
+    return 0;
+}
+
+bool parse_hor(const char *h, int &v) {
+    if (!strcmp(h, "UPW1")) v = HOR_UPW1;
+    else if (!strcmp(h, "MUSCL")) v = HOR_MUSCL;
+    else if (!strcmp(h, "MFCT")) v = HOR_MFCT;
+    else return false;
+    return true;
+}
+bool parse_ver(const char *h, int &v) {
+    if (!strcmp(h, "UPW1")) v = VER_UPW1;
+    else if (!strcmp(h, "QR4C")) v = VER_QR4C;
+    else if (!strcmp(h, "CDIFF")) v = VER_CDIFF;
+    else if (!strcmp(h, "PPM")) v = VER_PPM;
+    else return false;
+    return true;
+}
+
+// Hilbert index of (x, y) on a 2^16 grid.
+uint64_t hilbert(uint32_t x, uint32_t y) {
+    const uint32_t n = 1u << 16;
+    uint64_t d = 0;
+    for (uint32_t sbit = n / 2; sbit > 0; sbit /= 2) {
+        const uint32_t rx = (x & sbit) > 0, ry = (y & sbit) > 0;
+        d += (uint64_t)sbit * sbit * ((3 * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = n - 1 - x;
+                y = n - 1 - y;
+            }
+            std::swap(x, y);
+        }
+    }
+    return d;
+}
+
+// Node order: Hilbert curve over the coordinates, else breadth-first over the edge graph.
+std::vector<int> node_order(const State &s) {
+    const int N = s.N;
+    std::vector<int> order(N);
+    std::iota(order.begin(), order.end(), 0);
+    if (s.have_coord) {
+        double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+        for (int n = 0; n < N; ++n) {
+            x0 = std::min(x0, s.coord[2 * n]);
+            x1 = std::max(x1, s.coord[2 * n]);
+            y0 = std::min(y0, s.coord[2 * n + 1]);
+            y1 = std::max(y1, s.coord[2 * n + 1]);
+        }
+        const double sx = x1 > x0 ? 65535.0 / (x1 - x0) : 0.0, sy = y1 > y0 ? 65535.0 / (y1 - y0) : 0.0;
+        std::vector<uint64_t> key(N);
+        for (int n = 0; n < N; ++n)
+            key[n] = hilbert((uint32_t)((s.coord[2 * n] - x0) * sx), (uint32_t)((s.coord[2 * n + 1] - y0) * sy));
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+        return order;
+    }
+    std::vector<int> ptr(N + 1, 0), adj(2 * (size_t)s.E);
+    for (int e = 0; e < s.E; ++e) {
+        ptr[s.edges[2 * e] + 1]++;
+        ptr[s.edges[2 * e + 1] + 1]++;
+    }
+    for (int n = 0; n < N; ++n) ptr[n + 1] += ptr[n];
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int e = 0; e < s.E; ++e) {
+        adj[fill[s.edges[2 * e]]++] = s.edges[2 * e + 1];
+        adj[fill[s.edges[2 * e + 1]]++] = s.edges[2 * e];
+    }
+    std::vector<char> seen(N, 0);
+    order.clear();
+    for (int start = 0; start < N; ++start) {
+        if (seen[start]) continue;
+        seen[start] = 1;
+        size_t head = order.size();
+        order.push_back(start);
+        while (head < order.size()) {
+            const int u = order[head++];
+            for (int i = ptr[u]; i < ptr[u + 1]; ++i)
+                if (!seen[adj[i]]) {
+                    seen[adj[i]] = 1;
+                    order.push_back(adj[i]);
+                }
+        }
+    }
+    return order;
+}
+
+int validate(State &s) {
+    if (!s.dims) return fail("mesh dimensions not set (tracer_set_mesh_dims)");
+    if (!s.have_conn) return fail("connectivity not set (tracer_set_connectivity)");
+    if (!s.have_nlev) return fail("node levels not set (tracer_set_levels)");
+    if (!s.have_elev) return fail("element levels not set (tracer_set_elem_levels)");
+    if (!s.have_geom) return fail("geometry not set (tracer_set_geometry)");
+    if (!s.have_hnode) return fail("layer thickness not set (tracer_set_vertical_grid / tracer_set_thickness)");
+    if (!s.have_vert) return fail("vertical grid Z_3d_n / zbar_3d_n / helem not set (tracer_set_vertical_grid)");
+    if (s.myDim >= 0 && s.myDim != s.N) return fail("myDim_nod2D=%d must equal nod2D=%d", s.myDim, s.N);
+    if (s.nl > 0xffff) return fail("nl=%d too large", s.nl);
+    if (s.nl - 1 > 256) return fail("nl-1=%d layers exceed the 256-thread column tile", s.nl - 1);
+    for (int n = 0; n < s.N; ++n) {
+        if (s.ulev_n[n] != 1) return fail("node %d: ulevels_nod2D=%d; cavities (ulevels > 1) are not supported", n + 1, s.ulev_n[n]);
+        if (s.nlev_n[n] < 3 || s.nlev_n[n] > s.nl) return fail("node %d: nlevels_nod2D=%d outside [3, nl]", n + 1, s.nlev_n[n]);
+    }
+    for (int t = 0; t < s.T; ++t) {
+        if (s.ulev_e[t] != 1) return fail("element %d: ulevels=%d; cavities are not supported", t + 1, s.ulev_e[t]);
+        if (s.nlev_e[t] < 2 || s.nlev_e[t] > s.nl) return fail("element %d: nlevels=%d outside [2, nl]", t + 1, s.nlev_e[t]);
+        for (int c = 0; c < 3; ++c) {
+            const int n = s.enodes[3 * t + c];
+            if (s.nlev_n[n] < s.nlev_e[t])
+                return fail("node %d has fewer levels (%d) than element %d (%d)", n + 1, s.nlev_n[n], t + 1, s.nlev_e[t]);
+        }
+    }
+    for (int e = 0; e < s.E; ++e)
+        if (s.etri[2 * e] < 0) return fail("edge %d: edge_tri(1) must be a valid element", e + 1);
+    return 0;
+}
+
+// nboundary_lay as muscl_adv_init derives it (lib/oce_muscl_adv.F90:96-159).
+void derive_nboundary_lay(State &s) {
+    s.nbl.assign(s.N, s.nl - 1);
+    for (int e = 0; e < s.E; ++e) {
+        const int a = s.edges[2 * e], b = s.edges[2 * e + 1];
+        const int t1 = s.etri[2 * e], t2 = s.etri[2 * e + 1];
+        if (t1 < 0 || t2 < 0) {
+            s.nbl[a] = 0;
+            s.nbl[b] = 0;
+        } else {
+            const int lev = std::min(s.nlev_e[t1], s.nlev_e[t2]) - 1;
+            s.nbl[a] = std::min(s.nbl[a], lev);
+            s.nbl[b] = std::min(s.nbl[b], lev);
+        }
+    }
+}
+
+int build_device(State &s) {
+    if (validate(s)) return -1;
+    for (const auto &sc : s.schemes)
+        if ((sc.hor == HOR_MUSCL || sc.hor == HOR_MFCT) && !s.have_muscl)
+            return fail("MUSCL/MFCT need edge_up_dn_tri (tracer_set_muscl_support)");
+    if (!s.have_muscl) derive_nboundary_lay(s);
+    release_device(s);
+    if (!s.stream) HIPCHK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    if (!s.ev0) HIPCHK(hipEventCreate(&s.ev0));
+    if (!s.ev1) HIPCHK(hipEventCreate(&s.ev1));
+    const int N = s.N, T = s.T, E = s.E, nl = s.nl, L = nl - 1, A = s.max_adj;
+
+    // Renumbering: nodes along the curve, elements / edges by their smallest new node.
+    s.perm_n = node_order(s);
+    s.inv_n.assign(N, 0);
+    for (int i = 0; i < N; ++i) s.inv_n[s.perm_n[i]] = i;
+    s.perm_t.resize(T);
+    std::iota(s.perm_t.begin(), s.perm_t.end(), 0);
+    {
+        std::vector<int64_t> key(T);
+        for (int t = 0; t < T; ++t) {
+            int a = s.inv_n[s.enodes[3 * t]], b = s.inv_n[s.enodes[3 * t + 1]], c = s.inv_n[s.enodes[3 * t + 2]];
+            key[t] = (int64_t)std::min(a, std::min(b, c)) * 4 + 0;
+        }
+        std::stable_sort(s.perm_t.begin(), s.perm_t.end(), [&](int a, int b) { return key[a] < key[b]; });
+    }
+    s.inv_t.assign(T, 0);
+    for (int i = 0; i < T; ++i) s.inv_t[s.perm_t[i]] = i;
+    s.perm_e.resize(E);
+    std::iota(s.perm_e.begin(), s.perm_e.end(), 0);
+    {
+        std::vector<int64_t> key(E);
+        for (int e = 0; e < E; ++e) {
+            const int a = s.inv_n[s.edges[2 * e]], b = s.inv_n[s.edges[2 * e + 1]];
+            key[e] = (int64_t)std::min(a, b) * (int64_t)N + std::max(a, b);
+        }
+        std::stable_sort(s.perm_e.begin(), s.perm_e.end(), [&](int a, int b) { return key[a] < key[b]; });
+    }
+    s.inv_e.assign(E, 0);
+    for (int i = 0; i < E; ++i) s.inv_e[s.perm_e[i]] = i;
+
+    MeshDev &m = s.md;
+    m.N = N;
+    m.T = T;
+    m.E = E;
+    m.nl = nl;
+    m.L = L;
+    m.max_adj = A;
+
+    // nodes
+    std::vector<int> i1(N), i2(N), i3(N), i4(N), i5(N);
+    std::vector<int> nlev_min(N), ulev_max(N);
+    for (int o = 0; o < N; ++o) {  // lib/oce_mesh.F90:1522-1560
+        int mn = 1 << 30, mx = -(1 << 30);
+        for (int k = 0; k < s.nie_num[o]; ++k) {
+            const int t = s.nie[(size_t)o * A + k];
+            mn = std::min(mn, s.nlev_e[t]);
+            mx = std::max(mx, s.ulev_e[t]);
+        }
+        nlev_min[o] = mn;
+        ulev_max[o] = mx;
+    }
+    for (int n = 0; n < N; ++n) {
+        const int o = s.perm_n[n];
+        i1[n] = s.nlev_n[o];
+        i2[n] = s.ulev_n[o];
+        i3[n] = nlev_min[o];
+        i4[n] = ulev_max[o];
+        i5[n] = s.nbl[o];
+    }
+    int *p_i;
+    if (upload(s, &p_i, i1)) return -1; m.nlev_n = p_i;
+    if (upload(s, &p_i, i2)) return -1; m.ulev_n = p_i;
+    if (upload(s, &p_i, i3)) return -1; m.nlev_min = p_i;
+    if (upload(s, &p_i, i4)) return -1; m.ulev_max = p_i;
+    if (upload(s, &p_i, i5)) return -1; m.nbl = p_i;
+
+    auto rows_d = [&](const std::vector<double> &src, int ld, const std::vector<int> &perm, const double **dst) {
+        std::vector<double> h((size_t)perm.size() * ld);
+        for (size_t r = 0; r < perm.size(); ++r)
+            std::memcpy(&h[r * ld], &src[(size_t)perm[r] * ld], sizeof(double) * ld);
+        double *p;
+        if (upload(s, &p, h)) return -1;
+        *dst = p;
+        return 0;
+    };
+    if (rows_d(s.hnode, L, s.perm_n, &m.hnode)) return -1;
+    if (rows_d(s.hnode_new, L, s.perm_n, &m.hnode_new)) return -1;
+    if (rows_d(s.Z, L, s.perm_n, &m.Z)) return -1;
+    if (rows_d(s.area, nl, s.perm_n, &m.area)) return -1;
+    if (rows_d(s.areasvol, nl, s.perm_n, &m.areasvol)) return -1;
+    if (rows_d(s.zbar3, nl, s.perm_n, &m.zbar)) return -1;
+    if (rows_d(s.w, nl, s.perm_n, &m.w)) return -1;
+    m.wi = m.w;
+    m.we = m.w;
+    if (s.wi != s.w && rows_d(s.wi, nl, s.perm_n, &m.wi)) return -1;
+    if (s.we != s.w && rows_d(s.we, nl, s.perm_n, &m.we)) return -1;
+
+    // nod_in_elem2D in the reference's list order, element ids renumbered
+    {
+        std::vector<int> h((size_t)N * A, 0), cnt(N);
+        for (int n = 0; n < N; ++n) {
+            const int o = s.perm_n[n];
+            cnt[n] = s.nie_num[o];
+            for (int k = 0; k < cnt[n]; ++k) h[(size_t)n * A + k] = s.inv_t[s.nie[(size_t)o * A + k]];
+        }
+        if (upload(s, &p_i, h)) return -1; m.nie = p_i;
+        if (upload(s, &p_i, cnt)) return -1; m.nie_num = p_i;
+    }
+    // elements
+    {
+        std::vector<int> en((size_t)3 * T), nle(T), ule(T);
+        std::vector<double> ar(T), ec(T), gs((size_t)6 * T);
+        for (int t = 0; t < T; ++t) {
+            const int o = s.perm_t[t];
+            for (int c = 0; c < 3; ++c) en[3 * t + c] = s.inv_n[s.enodes[3 * o + c]];
+            nle[t] = s.nlev_e[o];
+            ule[t] = s.ulev_e[o];
+            ar[t] = s.elem_area[o];
+            ec[t] = s.elem_cos[o];
+            for (int c = 0; c < 6; ++c) gs[6 * t + c] = s.gsca[6 * (size_t)o + c];
+        }
+        if (upload(s, &p_i, en)) return -1; m.enodes = p_i;
+        if (upload(s, &p_i, nle)) return -1; m.nlev_e = p_i;
+        if (upload(s, &p_i, ule)) return -1; m.ulev_e = p_i;
+        double *p;
+        if (upload(s, &p, ar)) return -1; m.elem_area = p;
+        if (upload(s, &p, ec)) return -1; m.elem_cos = p;
+        if (upload(s, &p, gs)) return -1; m.gsca = p;
+        if (rows_d(s.helem, L, s.perm_t, &m.helem)) return -1;
+        if (rows_d(s.uv, 2 * L, s.perm_t, &m.uv)) return -1;
+    }
+    // edges
+    std::vector<int> nu12(E), nl12(E);
+    {
+        std::vector<int4> ne(E);
+        std::vector<double4> cd(E);
+        std::vector<double2> dxy(E);
+        std::vector<int2> ud(E);
+        for (int e = 0; e < E; ++e) {
+            const int o = s.perm_e[e];
+            const int t1 = s.etri[2 * o], t2 = s.etri[2 * o + 1];
+            ne[e] = make_int4(s.inv_n[s.edges[2 * o]], s.inv_n[s.edges[2 * o + 1]], s.inv_t[t1], t2 >= 0 ? s.inv_t[t2] : -1);
+            cd[e] = make_double4(s.ecd[4 * (size_t)o], s.ecd[4 * (size_t)o + 1], s.ecd[4 * (size_t)o + 2], s.ecd[4 * (size_t)o + 3]);
+            dxy[e] = make_double2(s.edxdy[2 * (size_t)o], s.edxdy[2 * (size_t)o + 1]);
+            if (s.have_muscl) {
+                const int u = s.updn[2 * o], d = s.updn[2 * o + 1];
+                ud[e] = make_int2(u >= 0 ? s.inv_t[u] : -1, d >= 0 ? s.inv_t[d] : -1);
+            } else {
+                ud[e] = make_int2(-1, -1);
+            }
+            // union level range of the scatters (lib/oce_adv_tra_driver.F90:125-136)
+            const int l1 = s.nlev_e[t1] - 1, u1 = s.ulev_e[t1];
+            int l2 = 0, u2 = 0;
+            if (t2 >= 0) {
+                l2 = s.nlev_e[t2] - 1;
+                u2 = s.ulev_e[t2];
+            }
+            nl12[o] = std::max(l1, l2);
+            nu12[o] = u2 > 0 ? std::min(u1, u2) : u1;
+        }
+        int4 *p4;
+        if (upload(s, &p4, ne)) return -1; m.edge_ne = p4;
+        double4 *pd4;
+        if (upload(s, &pd4, cd)) return -1; m.ecd = pd4;
+        double2 *pd2;
+        if (upload(s, &pd2, dxy)) return -1; m.edxdy = pd2;
+        int2 *p2;
+        if (upload(s, &p2, ud)) return -1; m.updn = p2;
+    }
+    // CSR node -> incident edges, ascending reference edge number (= the serial scatter order)
+    {
+        std::vector<int> ptr(N + 1, 0);
+        for (int o = 0; o < E; ++o) {
+            ptr[s.inv_n[s.edges[2 * o]] + 1]++;
+            ptr[s.inv_n[s.edges[2 * o + 1]] + 1]++;
+        }
+        for (int n = 0; n < N; ++n) ptr[n + 1] += ptr[n];
+        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+        std::vector<int2> list(ptr[N]);
+        for (int o = 0; o < E; ++o) {  // ascending reference edge number
+            const int packed = (nu12[o] << 16) | nl12[o];
+            const int a = s.inv_n[s.edges[2 * o]], b = s.inv_n[s.edges[2 * o + 1]];
+            list[fill[a]++] = make_int2(s.inv_e[o] * 2 + 0, packed);
+            list[fill[b]++] = make_int2(s.inv_e[o] * 2 + 1, packed);
+        }
+        if (upload(s, &p_i, ptr)) return -1; m.ne_ptr = p_i;
+        int2 *p2;
+        if (upload(s, &p2, list)) return -1; m.ne_list = p2;
+    }
+    if (upload(s, &s.d_perm_n, s.perm_n)) return -1;
+    if (upload(s, &s.d_inv_n, s.inv_n)) return -1;
+
+    // work arrays
+    const size_t NLn = (size_t)N * L, ELn = (size_t)E * L, NIn = (size_t)N * nl;
+    if (dalloc(s, &s.flo, ELn) || dalloc(s, &s.adfh, ELn) || dalloc(s, &s.LO, NLn) ||
+        dalloc(s, &s.Rp, NLn) || dalloc(s, &s.Rm, NLn) || dalloc(s, &s.adfv, NIn) ||
+        dalloc(s, &s.part, (size_t)kStatsBlocks))
+        return -1;
+    // tracers
+    s.dT.assign(s.ntr, nullptr);
+    s.dTAB.assign(s.ntr, nullptr);
+    for (int i = 0; i < s.ntr; ++i) {
+        if (rows_d(s.values[i], L, s.perm_n, const_cast<const double **>(&s.dT[i]))) return -1;
+    }
+    HIPCHK(hipDeviceSynchronize());
+    s.on_device = true;
+    return 0;
+}
+
+int ensure_device(State &s) {
+    if (!s.initialized) return fail("tracer not initialized. Call tracer_init_complete first.");
+    if (s.on_device) return 0;
+    return build_device(s);
+}
+
+int ensure_dropin_buffers(State &s) {
+    const size_t NLn = (size_t)s.N * (s.nl - 1), ELn = (size_t)s.E * (s.nl - 1), NIn = (size_t)s.N * s.nl;
+    if (!s.acc_h) {
+        if (dalloc(s, &s.acc_h, NLn) || dalloc(s, &s.acc_v, NLn) || dalloc(s, &s.dbg_h, ELn) ||
+            dalloc(s, &s.dbg_v, NIn))
+            return -1;
+        HIPCHK(hipMemsetAsync(s.acc_h, 0, NLn * sizeof(double), s.stream));
+        HIPCHK(hipMemsetAsync(s.acc_v, 0, NLn * sizeof(double), s.stream));
+    }
+    return 0;
+}
+
+StepArgs step_args(State &s, int i, double dt) {
+    const SchemeSel &sc = s.schemes[i];
+    StepArgs a{};
+    a.hor = sc.hor;
+    a.ver = sc.ver;
+    a.fct = sc.fct;
+    a.ph = sc.ph;
+    a.pv = sc.pv;
+    a.dt = dt;
+    a.T = s.dT[i];
+    a.TAB = s.dT[i];
+    a.flo = s.flo;
+    a.adfh = s.adfh;
+    a.LO = s.LO;
+    a.Rp = s.Rp;
+    a.Rm = s.Rm;
+    a.adfv = s.adfv;
+    return a;
+}
+
+int advect(State &s, int nsteps, double dt) {
+    if (ensure_device(s)) return -1;
+    if (s.ntr == 0) return fail("no tracers allocated (tracer_allocate_tracers)");
+    for (int i = 0; i < s.ntr; ++i) s.dTAB[i] = nullptr;  // valuesAB := values
+    HIPCHK(hipEventRecord(s.ev0, s.stream));
+    for (int st = 0; st < nsteps; ++st)
+        for (int i = 0; i < s.ntr; ++i) {
+            StepArgs a = step_args(s, i, dt);
+            hipError_t e = launch_fct_step(s.md, a, s.stream);
+            if (e != hipSuccess) return fail("kernel launch failed: %s", hipGetErrorString(e));
+            s.last_fct = a.fct;
+        }
+    HIPCHK(hipEventRecord(s.ev1, s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
+    s.last_ms = ms;
+    return 0;
+}
+
+// Copies a device (rows, ld) array in device numbering to host reference numbering.
+int download_rows(State &s, const double *d, size_t rows, int ld, const std::vector<int> &perm,
+                  double *out) {
+    std::vector<double> h(rows * (size_t)ld);
+    HIPCHK(hipStreamSynchronize(s.stream));
+    HIPCHK(hipMemcpy(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t r = 0; r < rows; ++r)
+        std::memcpy(out + (size_t)perm[r] * ld, &h[r * ld], sizeof(double) * ld);
+    return 0;
+}
+
+int stats(State &s, int id, int all_levels, double *mn, double *mx, double *sm, double *vs) {
+    if (ensure_device(s)) return -1;
+    if (id < 1 || id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", id, s.ntr);
+    hipError_t e = launch_stats(s.md, s.dT[id - 1], all_levels, s.part, kStatsBlocks, s.stream);
+    if (e != hipSuccess) return fail("stats launch failed: %s", hipGetErrorString(e));
+    std::vector<double4> h(kStatsBlocks);
+    HIPCHK(hipStreamSynchronize(s.stream));
+    HIPCHK(hipMemcpy(h.data(), s.part, sizeof(double4) * kStatsBlocks, hipMemcpyDeviceToHost));
+    double a = INFINITY, b = -INFINITY, c = 0.0, d = 0.0;
+    for (const auto &p : h) {
+        a = std::min(a, p.x);
+        b = std::max(b, p.y);
+        c += p.z;
+        d += p.w;
+    }
+    if (mn) *mn = a;
+    if (mx) *mx = b;
+    if (sm) *sm = c;
+    if (vs) *vs = d;
+    return 0;
+}
+
+}  // namespace
+
+// =============================================================================================
+extern "C" {
+
+int tracer_init_mpi(int mpi_comm) {
+    (void)mpi_comm;
+    S();
+    return 0;
+}
+
+int tracer_finalize_mpi(void) { return 0; }
+
+int tracer_init(const char *restart_dir) {
+    return fail("tracer_init(%s): derived-type restart files are not supported; initialise from "
+                "arrays (tracer_set_* then tracer_init_complete)", restart_dir ? restart_dir : "");
+}
+
+int tracer_advect_step(double dt) {
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized. Call tracer_init first.");
+    return advect(s, 1, dt);
+}
+
+int tracer_advect_steps(int nsteps, double dt) {
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized. Call tracer_init first.");
+    if (nsteps < 0) return fail("nsteps must be >= 0");
+    return advect(s, nsteps, dt);
+}
+
+int tracer_get_stats(double *tmin, double *tmax, double *tsum) {
+    State &s = S();
+    if (!s.initialized) {
+        if (tmin) *tmin = 0.0;
+        if (tmax) *tmax = 0.0;
+        if (tsum) *tsum = 0.0;
+        return fail("tracer not initialized.");
+    }
+    return stats(s, 1, 1, tmin, tmax, tsum, nullptr);
+}
+
+int tracer_get_stats_by_id(int tracer_id, double *tmin, double *tmax, double *tsum, double *vsum) {
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    return stats(s, tracer_id, 0, tmin, tmax, tsum, vsum);
+}
+
+int tracer_get_size(int *nz, int *nn) {
+    State &s = S();
+    if (!s.initialized) {
+        if (nz) *nz = 0;
+        if (nn) *nn = 0;
+        return fail("tracer not initialized.");
+    }
+    *nz = s.nl - 1;
+    *nn = s.N;
+    return 0;
+}
+
+int tracer_get_values_by_id(int tracer_id, double *values, int nz, int nn) {
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
+    if (nz != s.nl - 1 || nn != s.N) return fail("tracer_get_values: expected (%d, %d), got (%d, %d)", s.nl - 1, s.N, nz, nn);
+    if (!s.on_device) {
+        std::memcpy(values, s.values[tracer_id - 1].data(), sizeof(double) * (size_t)nz * nn);
+        return 0;
+    }
+    return download_rows(s, s.dT[tracer_id - 1], s.N, s.nl - 1, s.perm_n, values);
+}
+
+int tracer_get_values(double *values, int nz, int nn) { return tracer_get_values_by_id(1, values, nz, nn); }
+
+int tracer_finalize(const char *output_dir) {
+    return fail("tracer_finalize(%s): restart output is not supported", output_dir ? output_dir : "");
+}
+
+int tracer_run_workflow(const char *restart_dir, int nsteps, double dt) {
+    (void)nsteps;
+    (void)dt;
+    return tracer_init(restart_dir);
+}
+
+int tracer_cleanup(void) {
+    if (g) {
+        release_device(*g);
+        if (g->ev0) hipEventDestroy(g->ev0);
+        if (g->ev1) hipEventDestroy(g->ev1);
+        if (g->stream) hipStreamDestroy(g->stream);
+        delete g;
+        g = nullptr;
+    }
+    g_err.clear();
+    return 0;
+}
+
+int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D) {
+    if (nl < 3 || nod2D < 1 || edge2D < 1 || elem2D < 1)
+        return fail("tracer_set_mesh_dims: invalid dims nl=%d nod2D=%d edge2D=%d elem2D=%d", nl, nod2D, edge2D, elem2D);
+    State &s = S();
+    release_device(s);
+    const hipStream_t keep = s.stream;
+    hipEvent_t k0 = s.ev0, k1 = s.ev1;
+    s = State();
+    s.stream = keep;
+    s.ev0 = k0;
+    s.ev1 = k1;
+    s.nl = nl;
+    s.N = nod2D;
+    s.E = edge2D;
+    s.T = elem2D;
+    s.dims = true;
+    return 0;
+}
+
+int tracer_set_levels(const int *nlevels, const int *ulevels, int nod2D) {
+    State &s = S();
+    if (!s.dims) return fail("tracer_set_levels: call tracer_set_mesh_dims first");
+    if (nod2D != s.N) return fail("tracer_set_levels: nod2D=%d != %d", nod2D, s.N);
+    s.nlev_n.assign(nlevels, nlevels + nod2D);
+    s.ulev_n.assign(ulevels, ulevels + nod2D);
+    s.have_nlev = true;
+    s.on_device = false;
+    return 0;
+}
+
+int tracer_set_elem_levels(const int *nlevels, const int *ulevels, int elem2D) {
+    State &s = S();
+    if (!s.dims) return fail("tracer_set_elem_levels: call tracer_set_mesh_dims first");
+    if (elem2D != s.T) return fail("tracer_set_elem_levels: elem2D=%d != %d", elem2D, s.T);
+    s.nlev_e.assign(nlevels, nlevels + elem2D);
+    s.ulev_e.assign(ulevels, ulevels + elem2D);
+    s.have_elev = true;
+    s.on_device = false;
+    return 0;
+}
+
+int tracer_set_thickness(const double *hnode, int nl, int nod2D) {
+    State &s = S();
+    if (check_dims(s, nl, nod2D, "tracer_set_thickness")) return -1;
+    s.hnode.assign(hnode, hnode + (size_t)(nl - 1) * nod2D);
+    s.hnode_new = s.hnode;
+    s.have_hnode = true;
+    s.on_device = false;
+    return 0;
+}
+
+int tracer_set_partition(int myDim_nod2D, int eDim_nod2D) {
+    State &s = S();
+    if (eDim_nod2D != 0) return fail("tracer_set_partition: eDim_nod2D=%d; one partition without halo is supported", eDim_nod2D);
+    if (s.dims && myDim_nod2D != s.N) return fail("tracer_set_partition: myDim_nod2D=%d != nod2D=%d", myDim_nod2D, s.N);
+    s.myDim = myDim_nod2D;
+    return 0;
+}
+
+int tracer_allocate_tracers(int num_tracers, int nl, int nod2D, int AB_order) {
+    State &s = S();
+    if (check_dims(s, nl, nod2D, "tracer_allocate_tracers")) return -1;
+    if (num_tracers < 1) return fail("tracer_allocate_tracers: num_tracers=%d", num_tracers);
+    if (AB_order < 1) return fail("tracer_allocate_tracers: AB_order=%d", AB_order);
+    s.ntr = num_tracers;
+    s.values.assign(num_tracers, std::vector<double>((size_t)(nl - 1) * nod2D, 0.0));
+    s.schemes.assign(num_tracers, SchemeSel());
+    s.on_device = false;
+    return 0;
+}
+
+int tracer_set_values(int tracer_id, const double *values, int nl, int nod2D) {
+    State &s = S();
+    if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
+    if (check_dims(s, nl, nod2D, "tracer_set_values")) return -1;
+    std::vector<double> &v = s.values[tracer_id - 1];
+    v.assign(values, values + (size_t)(nl - 1) * nod2D);
+    if (s.on_device) {
+        const int L = nl - 1;
+        std::vector<double> h((size_t)L * nod2D);
+        for (int r = 0; r < nod2D; ++r) std::memcpy(&h[(size_t)r * L], &v[(size_t)s.perm_n[r] * L], sizeof(double) * L);
+        HIPCHK(hipMemcpy(s.dT[tracer_id - 1], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        s.dTAB[tracer_id - 1] = nullptr;
+    }
+    return 0;
+}
+
+int tracer_set_values_ab(int tracer_id, const double *values, int nl, int nod2D) {
+    State &s = S();
+    if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
+    if (check_dims(s, nl, nod2D, "tracer_set_values_ab")) return -1;
+    if (ensure_device(s)) return -1;
+    const int L = nl - 1;
+    std::vector<double> h((size_t)L * nod2D);
+    for (int r = 0; r < nod2D; ++r) std::memcpy(&h[(size_t)r * L], &values[(size_t)s.perm_n[r] * L], sizeof(double) * L);
+    double *p = s.dTAB[tracer_id - 1];
+    if (!p && dalloc(s, &p, h.size())) return -1;
+    HIPCHK(hipMemcpy(p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    s.dTAB[tracer_id - 1] = p;
+    return 0;
+}
+
+int tracer_set_velocity_zero(int nl, int nod2D, int edge2D) {
+    State &s = S();
+    if (check_dims(s, nl, nod2D, "tracer_set_velocity_zero")) return -1;
+    if (edge2D != s.E) return fail("tracer_set_velocity_zero: edge2D=%d != %d", edge2D, s.E);
+    s.uv.assign((size_t)2 * (nl - 1) * s.T, 0.0);
+    s.w.assign((size_t)nl * nod2D, 0.0);
+    s.wi = s.w;
+    s.we = s.w;
+    s.have_vel = true;
+    s.on_device = false;
+    return 0;
+}
+
+int tracer_set_velocity(const double *uv, const double *w, int nl, int nod2D, int edge2D) {
+    State &s = S();
+    if (check_dims(s, nl, nod2D, "tracer_set_velocity")) return -1;
+    for (size_t i = 0; i < (size_t)(nl - 1) * edge2D * 2; ++i)
+        if (uv[i] != 0.0)
+            return fail("tracer_set_velocity: uv(nl-1, edge2D, 2) is not the element velocity "
+                        "vel(2, nl-1, elem2D) that do_oce_adv_tra reads; use tracer_set_velocity_elem");
+    s.uv.assign((size_t)2 * (nl - 1) * s.T, 0.0);
+    s.w.assign(w, w + (size_t)nl * nod2D);
+    s.wi = s.w;
+    s.we = s.w;
+    s.have_vel = true;
+    s.on_device = false;
+    return 0;
+}
+
+int tracer_set_velocity_elem(const double *uv, const double *w, const double *w_i, const double *w_e,
+                             int nl, int nod2D, int elem2D) {
+    State &s = S();
+    if (check_dims(s, nl, nod2D, "tracer_set_velocity_elem")) return -1;
+    if (elem2D != s.T) return fail("tracer_set_velocity_elem: elem2D=%d != %d", elem2D, s.T);
+    s.uv.assign(uv, uv + (size_t)2 * (nl - 1) * elem2D);
+    s.w.assign(w, w + (size_t)nl * nod2D);
+    s.wi = w_i ? std::vector<double>(w_i, w_i + (size_t)nl * nod2D) : s.w;
+    s.we = w_e ? std::vector<double>(w_e, w_e + (size_t)nl * nod2D) : s.w;
+    s.have_vel = true;
+    s.on_device = false;
+    return 0;
+}
+
+int tracer_init_complete(void) {
+    State &s = S();
+    if (!s.have_vel && s.dims) {
+        s.uv.assign((size_t)2 * (s.nl - 1) * s.T, 0.0);
+        s.w.assign((size_t)s.nl * s.N, 0.0);
+        s.wi = s.w;
+        s.we = s.w;
+        s.have_vel = true;
+    }
+    if (validate(s)) return -1;
+    s.initialized = true;
+    return build_device(s);
+}
+
+int tracer_load_mesh_partition(const char *restart_dir) {
+    return fail("tracer_load_mesh_partition(%s): partition files are not supported", restart_dir ? restart_dir : "");
+}
+
+int tracer_load_mesh_from_files(const char *mesh_path) {
+    return fail("tracer_load_mesh_from_files(%s): mesh files are not supported", mesh_path ? mesh_path : "");
+}
+
+int tracer_set_zbar(const double *zbar, int nl) {
+    State &s = S();
+    if (!s.dims || nl != s.nl) return fail("tracer_set_zbar: nl=%d does not match the mesh", nl);
+    s.zbar.assign(zbar, zbar + nl);
+    s.have_zbar = true;
+    return 0;
+}
+
+int tracer_init_ale_arrays(void) {
+    State &s = S();
+    if (!s.dims) return fail("tracer_init_ale_arrays called before initialization");
+    if (!s.have_zbar) return fail("tracer_init_ale_arrays: zbar not set (tracer_set_zbar)");
+    const int L = s.nl - 1;
+    s.hnode.assign((size_t)L * s.N, 0.0);
+    s.helem.assign((size_t)L * s.T, 0.0);
+    s.Z.assign((size_t)L * s.N, 0.0);
+    s.zbar3.assign((size_t)s.nl * s.N, 0.0);
+    for (int n = 0; n < s.N; ++n) {
+        for (int k = 0; k < L; ++k) {
+            s.hnode[(size_t)n * L + k] = s.zbar[k] - s.zbar[k + 1];
+            s.Z[(size_t)n * L + k] = 0.5 * (s.zbar[k] + s.zbar[k + 1]);
+        }
+        for (int k = 0; k < s.nl; ++k) s.zbar3[(size_t)n * s.nl + k] = s.zbar[k];
+    }
+    for (int t = 0; t < s.T; ++t)
+        for (int k = 0; k < L; ++k) s.helem[(size_t)t * L + k] = s.zbar[k] - s.zbar[k + 1];
+    s.hnode_new = s.hnode;
+    s.have_hnode = true;
+    s.have_vert = true;
+    s.on_device = false;
+    return 0;
+}
+
+int tracer_init_for_advection(void) {
+    State &s = S();
+    if (!s.initialized) return fail("tracer_init_for_advection called before mesh loaded");
+    return ensure_device(s);
+}
+
+int tracer_set_coordinates(const double *coord_nod2D, int nod2D) {
+    State &s = S();
+    if (!s.dims || nod2D != s.N) return fail("tracer_set_coordinates: nod2D=%d does not match", nod2D);
+    s.coord.assign(coord_nod2D, coord_nod2D + 2 * (size_t)nod2D);
+    s.have_coord = true;
+    s.on_device = false;
+    return 0;
+}
+
+int tracer_set_connectivity(const int *elem2D_nodes, const int *edges, const int *edge_tri,
+                            const int *nod_in_elem2D, const int *nod_in_elem2D_num, int max_adj,
+                            int nod2D, int elem2D, int edge2D) {
+    State &s = S();
+    if (!s.dims) return fail("tracer_set_connectivity: call tracer_set_mesh_dims first");
+    if (nod2D != s.N || elem2D != s.T || edge2D != s.E)
+        return fail("tracer_set_connectivity: dims (%d, %d, %d) do not match the mesh", nod2D, elem2D, edge2D);
+    if (max_adj < 1) return fail("tracer_set_connectivity: max_adj=%d", max_adj);
+    s.max_adj = max_adj;
+    s.enodes.resize((size_t)3 * elem2D);
+    for (size_t i = 0; i < s.enodes.size(); ++i) {
+        const int v = elem2D_nodes[i];
+        if (v < 1 || v > nod2D) return fail("elem2D_nodes(%zu, %zu)=%d out of range", i % 3 + 1, i / 3 + 1, v);
+        s.enodes[i] = v - 1;
+    }
+    s.edges.resize((size_t)2 * edge2D);
+    s.etri.resize((size_t)2 * edge2D);
+    for (size_t i = 0; i < s.edges.size(); ++i) {
+        const int v = edges[i], t = edge_tri[i];
+        if (v < 1 || v > nod2D) return fail("edges(%zu, %zu)=%d out of range", i % 2 + 1, i / 2 + 1, v);
+        if (t < 0 || t > elem2D) return fail("edge_tri(%zu, %zu)=%d out of range", i % 2 + 1, i / 2 + 1, t);
+        s.edges[i] = v - 1;
+        s.etri[i] = t - 1;
+    }
+    s.nie.assign((size_t)max_adj * nod2D, 0);
+    s.nie_num.assign(nod_in_elem2D_num, nod_in_elem2D_num + nod2D);
+    for (int n = 0; n < nod2D; ++n) {
+        if (s.nie_num[n] < 1 || s.nie_num[n] > max_adj)
+            return fail("nod_in_elem2D_num(%d)=%d outside [1, %d]", n + 1, s.nie_num[n], max_adj);
+        for (int k = 0; k < s.nie_num[n]; ++k) {
+            const int t = nod_in_elem2D[(size_t)n * max_adj + k];
+            if (t < 1 || t > elem2D) return fail("nod_in_elem2D(%d, %d)=%d out of range", k + 1, n + 1, t);
+            s.nie[(size_t)n * max_adj + k] = t - 1;
+        }
+    }
+    s.have_conn = true;
+    s.on_device = false;
+    return 0;
+}
+
+int tracer_set_geometry(const double *elem_area, const double *area, const double *areasvol,
+                        const double *edge_dxdy, const double *edge_cross_dxdy,
+                        const double *elem_cos, const double *gradient_sca, int nl, int nod2D,
+                        int elem2D, int edge2D) {
+    State &s = S();
+    if (check_dims(s, nl, nod2D, "tracer_set_geometry")) return -1;
+    if (elem2D != s.T || edge2D != s.E) return fail("tracer_set_geometry: elem2D/edge2D do not match");
+    s.elem_area.assign(elem_area, elem_area + elem2D);
+    s.area.assign(area, area + (size_t)nl * nod2D);
+    s.areasvol.assign(areasvol, areasvol + (size_t)nl * nod2D);
+    s.edxdy.assign(edge_dxdy, edge_dxdy + (size_t)2 * edge2D);
+    s.ecd.assign(edge_cross_dxdy, edge_cross_dxdy + (size_t)4 * edge2D);
+    s.elem_cos.assign(elem_cos, elem_cos + elem2D);
+    s.gsca.assign(gradient_sca, gradient_sca + (size_t)6 * elem2D);
+    s.have_geom = true;
+    s.on_device = false;
+    return 0;
+}
+
+int tracer_set_vertical_grid(const double *hnode, const double *hnode_new, const double *helem,
+                             const double *Z_3d_n, const double *zbar_3d_n, int nl, int nod2D,
+                             int elem2D) {
+    State &s = S();
+    if (check_dims(s, nl, nod2D, "tracer_set_vertical_grid")) return -1;
+    if (elem2D != s.T) return fail("tracer_set_vertical_grid: elem2D=%d != %d", elem2D, s.T);
+    const size_t NL = (size_t)(nl - 1) * nod2D;
+    s.hnode.assign(hnode, hnode + NL);
+    s.hnode_new.assign(hnode_new ? hnode_new : hnode, (hnode_new ? hnode_new : hnode) + NL);
+    s.helem.assign(helem, helem + (size_t)(nl - 1) * elem2D);
+    s.Z.assign(Z_3d_n, Z_3d_n + NL);
+    s.zbar3.assign(zbar_3d_n, zbar_3d_n + (size_t)nl * nod2D);
+    s.have_hnode = true;
+    s.have_vert = true;
+    s.on_device = false;
+    return 0;
+}
+
+int tracer_set_muscl_support(const int *edge_up_dn_tri, const int *nboundary_lay, int nod2D, int edge2D) {
+    State &s = S();
+    if (!s.dims || nod2D != s.N || edge2D != s.E) return fail("tracer_set_muscl_support: dims do not match");
+    s.updn.resize((size_t)2 * edge2D);
+    for (size_t i = 0; i < s.updn.size(); ++i) {
+        const int t = edge_up_dn_tri[i];
+        if (t < 0 || t > s.T) return fail("edge_up_dn_tri(%zu, %zu)=%d out of range", i % 2 + 1, i / 2 + 1, t);
+        s.updn[i] = t - 1;
+    }
+    s.nbl.assign(nboundary_lay, nboundary_lay + nod2D);
+    s.have_muscl = true;
+    s.on_device = false;
+    return 0;
+}
+
+int tracer_set_scheme(int tracer_id, const char *hor, const char *ver, const char *lim, double ph, double pv) {
+    State &s = S();
+    if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
+    SchemeSel sc;
+    if (!hor || !parse_hor(hor, sc.hor)) return fail("Unknown horizontal advection type %s! Check your namelists!", hor ? hor : "(null)");
+    if (!ver || !parse_ver(ver, sc.ver)) return fail("Unknown vertical advection type %s! Check your namelists!", ver ? ver : "(null)");
+    if (!lim || (strcmp(lim, "FCT") && strcmp(lim, "NONE"))) return fail("Unknown limiter %s", lim ? lim : "(null)");
+    sc.fct = !strcmp(lim, "FCT");
+    sc.ph = ph;
+    sc.pv = pv;
+    if ((sc.hor == HOR_MUSCL || sc.hor == HOR_MFCT) && !s.have_muscl)
+        return fail("tracer %d: %s needs edge_up_dn_tri (tracer_set_muscl_support)", tracer_id, hor);
+    s.schemes[tracer_id - 1] = sc;
+    return 0;
+}
+
+int tracer_zero_tendencies(void) {
+    State &s = S();
+    if (ensure_device(s) || ensure_dropin_buffers(s)) return -1;
+    const size_t NLn = (size_t)s.N * (s.nl - 1);
+    HIPCHK(hipMemsetAsync(s.acc_h, 0, NLn * sizeof(double), s.stream));
+    HIPCHK(hipMemsetAsync(s.acc_v, 0, NLn * sizeof(double), s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+int tracer_do_oce_adv_tra(double dt, int tracer_id) {
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
+    if (ensure_device(s) || ensure_dropin_buffers(s)) return -1;
+    const int i = tracer_id - 1;
+    const size_t L = s.nl - 1;
+    // the reference zeroes its flux / low-order work arrays at every call
+    HIPCHK(hipMemsetAsync(s.dbg_h, 0, (size_t)s.E * L * sizeof(double), s.stream));
+    HIPCHK(hipMemsetAsync(s.dbg_v, 0, (size_t)s.N * s.nl * sizeof(double), s.stream));
+    HIPCHK(hipMemsetAsync(s.adfh, 0, (size_t)s.E * L * sizeof(double), s.stream));
+    HIPCHK(hipMemsetAsync(s.adfv, 0, (size_t)s.N * s.nl * sizeof(double), s.stream));
+    HIPCHK(hipMemsetAsync(s.LO, 0, (size_t)s.N * L * sizeof(double), s.stream));
+    StepArgs a = step_args(s, i, dt);
+    if (s.dTAB[i]) a.TAB = s.dTAB[i];
+    a.acc_h = s.acc_h;
+    a.acc_v = s.acc_v;
+    a.dbg_h = s.dbg_h;
+    a.dbg_v = s.dbg_v;
+    hipError_t e = launch_fct_step(s.md, a, s.stream);
+    if (e != hipSuccess) return fail("kernel launch failed: %s", hipGetErrorString(e));
+    s.last_fct = a.fct;
+    HIPCHK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+int tracer_get_tendencies(double *dttf_h, double *dttf_v, int nz, int nn) {
+    State &s = S();
+    if (!s.on_device || !s.acc_h) return fail("tracer_get_tendencies: no tendencies computed yet");
+    if (nz != s.nl - 1 || nn != s.N) return fail("tracer_get_tendencies: expected (%d, %d)", s.nl - 1, s.N);
+    if (dttf_h && download_rows(s, s.acc_h, s.N, nz, s.perm_n, dttf_h)) return -1;
+    if (dttf_v && download_rows(s, s.acc_v, s.N, nz, s.perm_n, dttf_v)) return -1;
+    return 0;
+}
+
+int tracer_get_work(const char *name, double *out, size_t count) {
+    State &s = S();
+    if (!s.on_device) return fail("tracer_get_work: nothing computed yet");
+    const size_t L = s.nl - 1;
+    if (!strcmp(name, "fct_LO")) {
+        if (count != L * s.N) return fail("tracer_get_work(fct_LO): count must be %zu", L * s.N);
+        return download_rows(s, s.LO, s.N, (int)L, s.perm_n, out);
+    }
+    if (!strcmp(name, "adv_flux_hor")) {
+        if (count != L * s.E) return fail("tracer_get_work(adv_flux_hor): count must be %zu", L * s.E);
+        return download_rows(s, s.last_fct ? s.dbg_h : s.adfh, s.E, (int)L, s.perm_e, out);
+    }
+    if (!strcmp(name, "adv_flux_ver")) {
+        if (count != (size_t)s.nl * s.N) return fail("tracer_get_work(adv_flux_ver): count must be %zu", (size_t)s.nl * s.N);
+        return download_rows(s, s.last_fct ? s.dbg_v : s.adfv, s.N, s.nl, s.perm_n, out);
+    }
+    return fail("tracer_get_work: unknown array %s", name);
+}
+
+int tracer_set_values_device(int tracer_id, const void *values, int nl, int nod2D) {
+    State &s = S();
+    if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
+    if (check_dims(s, nl, nod2D, "tracer_set_values_device")) return -1;
+    if (ensure_device(s)) return -1;
+    hipError_t e = launch_gather_rows(static_cast<const double *>(values), s.dT[tracer_id - 1], s.d_perm_n, s.N, nl - 1, s.stream);
+    if (e != hipSuccess) return fail("gather launch failed: %s", hipGetErrorString(e));
+    s.dTAB[tracer_id - 1] = nullptr;
+    HIPCHK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+int tracer_get_values_device(int tracer_id, void *values, int nz, int nn) {
+    State &s = S();
+    if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
+    if (nz != s.nl - 1 || nn != s.N) return fail("tracer_get_values_device: expected (%d, %d)", s.nl - 1, s.N);
+    if (ensure_device(s)) return -1;
+    hipError_t e = launch_gather_rows(s.dT[tracer_id - 1], static_cast<double *>(values), s.d_inv_n, s.N, nz, s.stream);
+    if (e != hipSuccess) return fail("gather launch failed: %s", hipGetErrorString(e));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+int tracer_last_elapsed_ms(double *ms) {
+    *ms = S().last_ms;
+    return 0;
+}
+
+const char *tracer_last_error(void) { return g_err.c_str(); }
+
+const char *tracer_version(void) { return "fesomtra 0.1 (gfx950)"; }
+
+}  // extern "C"

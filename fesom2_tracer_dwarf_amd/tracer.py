@@ -1,0 +1,181 @@
+"""Host-side interface to the GPU tracer advection, mirroring the reference's Python wrapper.
+
+``FESOMTracerAdvection`` keeps the method names of python/demo_tracer.py:40-208
+(``initialize_mpi``, ``advect_step``, ``get_tracer_values``, ``get_statistics``, ``finalize``,
+``finalize_mpi``) and adds array-based initialisation from a :class:`~.configs.Case`, the
+do_oce_adv_tra drop-in call, and PyTorch device-tensor exchange.  Every call goes through the C ABI
+of libfesomtra.so; there is no CPU path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import abi
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class FESOMTracerAdvection:
+    """Process-wide tracer advection instance (the library holds one, as the reference does)."""
+
+    def __init__(self):
+        self.lib = abi.load()
+        self.nl = self.nod2D = self.elem2D = self.edge2D = 0
+        self.num_tracers = 0
+
+    # --- reference wrapper API (python/demo_tracer.py) ------------------------------------------
+    def initialize_mpi(self, mpi_comm: int = 0) -> int:
+        return self.lib.tracer_init_mpi(mpi_comm)
+
+    def initialize(self, restart_path: str = "./fesom_bin_restart") -> int:
+        return self.lib.tracer_init(restart_path.encode())
+
+    def advect_step(self, dt: float = 1.0e-3) -> int:
+        return self.lib.tracer_advect_step(dt)
+
+    def get_tracer_values(self, tracer_id: int = 1) -> np.ndarray:
+        nz, nn = ctypes.c_int(), ctypes.c_int()
+        abi.check(self.lib.tracer_get_size(ctypes.byref(nz), ctypes.byref(nn)), "tracer_get_size")
+        out = np.zeros((nn.value, nz.value), dtype=np.float64)  # == Fortran (nz, nn)
+        abi.check(self.lib.tracer_get_values_by_id(tracer_id, abi.dptr(out), nz.value, nn.value),
+                  "tracer_get_values")
+        return out
+
+    def get_statistics(self) -> Dict[str, float]:
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        abi.check(self.lib.tracer_get_stats(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+                  "tracer_get_stats")
+        return {"min": a.value, "max": b.value, "sum": c.value}
+
+    def finalize(self, output_path: str = "./fesom_bin_restart") -> int:
+        return self.lib.tracer_finalize(output_path.encode())
+
+    def finalize_mpi(self) -> int:
+        return self.lib.tracer_finalize_mpi()
+
+    # --- array-based initialisation ---------------------------------------------------------------
+    def setup_case(self, case, upload: bool = True) -> None:
+        """Hand a generated case (mesh, velocities, tracers, schemes) to the library."""
+        L = self.lib
+        m = case.mesh
+        self.nl, self.nod2D, self.elem2D, self.edge2D = m.nl, m.nod2D, m.elem2D, m.edge2D
+        self.num_tracers = case.ntracers
+        abi.check(L.tracer_init_mpi(0), "tracer_init_mpi")
+        abi.check(L.tracer_set_mesh_dims(m.nl, m.nod2D, m.edge2D, m.elem2D), "tracer_set_mesh_dims")
+        abi.check(L.tracer_set_partition(m.nod2D, 0), "tracer_set_partition")
+        self._keep = []
+
+        def I(a):
+            a = _i32(a)
+            self._keep.append(a)
+            return abi.iptr(a)
+
+        def D(a):
+            a = _f64(a)
+            self._keep.append(a)
+            return abi.dptr(a)
+
+        abi.check(L.tracer_set_coordinates(D(m.coord_nod2D), m.nod2D), "tracer_set_coordinates")
+        abi.check(L.tracer_set_connectivity(I(m.elem2D_nodes), I(m.edges), I(m.edge_tri),
+                                            I(m.nod_in_elem2D), I(m.nod_in_elem2D_num), m.max_adj,
+                                            m.nod2D, m.elem2D, m.edge2D), "tracer_set_connectivity")
+        abi.check(L.tracer_set_levels(I(m.nlevels_nod2D), I(m.ulevels_nod2D), m.nod2D), "tracer_set_levels")
+        abi.check(L.tracer_set_elem_levels(I(m.nlevels), I(m.ulevels), m.elem2D), "tracer_set_elem_levels")
+        abi.check(L.tracer_set_geometry(D(m.elem_area), D(m.area), D(m.areasvol), D(m.edge_dxdy),
+                                        D(m.edge_cross_dxdy), D(m.elem_cos), D(m.gradient_sca),
+                                        m.nl, m.nod2D, m.elem2D, m.edge2D), "tracer_set_geometry")
+        abi.check(L.tracer_set_vertical_grid(D(m.hnode), D(m.hnode_new), D(m.helem), D(m.Z_3d_n),
+                                             D(m.zbar_3d_n), m.nl, m.nod2D, m.elem2D),
+                  "tracer_set_vertical_grid")
+        abi.check(L.tracer_set_muscl_support(I(m.edge_up_dn_tri), I(m.nboundary_lay), m.nod2D,
+                                             m.edge2D), "tracer_set_muscl_support")
+        abi.check(L.tracer_set_velocity_elem(D(case.uv), D(case.w), D(case.wi), D(case.we), m.nl,
+                                             m.nod2D, m.elem2D), "tracer_set_velocity_elem")
+        abi.check(L.tracer_allocate_tracers(case.ntracers, m.nl, m.nod2D, 2), "tracer_allocate_tracers")
+        for i in range(case.ntracers):
+            abi.check(L.tracer_set_values(i + 1, D(case.tracers[i]), m.nl, m.nod2D), "tracer_set_values")
+            sc = case.schemes[i]
+            abi.check(L.tracer_set_scheme(i + 1, sc.hor.encode(), sc.ver.encode(), sc.lim.encode(),
+                                          float(sc.ph), float(sc.pv)), "tracer_set_scheme")
+        if upload:
+            abi.check(L.tracer_init_complete(), "tracer_init_complete")
+        self._keep = []
+
+    # --- stepping ---------------------------------------------------------------------------------
+    def advect_steps(self, nsteps: int, dt: float) -> float:
+        """Advance every tracer nsteps times; returns the device time in ms."""
+        abi.check(self.lib.tracer_advect_steps(nsteps, dt), "tracer_advect_steps")
+        ms = ctypes.c_double()
+        self.lib.tracer_last_elapsed_ms(ctypes.byref(ms))
+        return ms.value
+
+    def do_oce_adv_tra(self, dt: float, tracer_id: int) -> None:
+        abi.check(self.lib.tracer_do_oce_adv_tra(dt, tracer_id), "tracer_do_oce_adv_tra")
+
+    def zero_tendencies(self) -> None:
+        abi.check(self.lib.tracer_zero_tendencies(), "tracer_zero_tendencies")
+
+    def set_values(self, tracer_id: int, values: np.ndarray) -> None:
+        v = _f64(values)
+        abi.check(self.lib.tracer_set_values(tracer_id, abi.dptr(v), self.nl, self.nod2D), "tracer_set_values")
+
+    def set_values_ab(self, tracer_id: int, values: np.ndarray) -> None:
+        v = _f64(values)
+        abi.check(self.lib.tracer_set_values_ab(tracer_id, abi.dptr(v), self.nl, self.nod2D),
+                  "tracer_set_values_ab")
+
+    def get_tendencies(self):
+        h = np.zeros((self.nod2D, self.nl - 1))
+        v = np.zeros((self.nod2D, self.nl - 1))
+        abi.check(self.lib.tracer_get_tendencies(abi.dptr(h), abi.dptr(v), self.nl - 1, self.nod2D),
+                  "tracer_get_tendencies")
+        return h, v
+
+    def get_work(self, name: str) -> np.ndarray:
+        shapes = {"fct_LO": (self.nod2D, self.nl - 1), "adv_flux_hor": (self.edge2D, self.nl - 1),
+                  "adv_flux_ver": (self.nod2D, self.nl)}
+        out = np.zeros(shapes[name])
+        abi.check(self.lib.tracer_get_work(name.encode(), abi.dptr(out), out.size), "tracer_get_work")
+        return out
+
+    def get_all_values(self) -> np.ndarray:
+        return np.stack([self.get_tracer_values(i + 1) for i in range(self.num_tracers)])
+
+    def stats(self, tracer_id: int) -> Dict[str, float]:
+        a, b, c, d = (ctypes.c_double() for _ in range(4))
+        abi.check(self.lib.tracer_get_stats_by_id(tracer_id, ctypes.byref(a), ctypes.byref(b),
+                                                  ctypes.byref(c), ctypes.byref(d)),
+                  "tracer_get_stats_by_id")
+        return {"min": a.value, "max": b.value, "sum": c.value, "volume_sum": d.value}
+
+    # --- PyTorch device tensors -------------------------------------------------------------------
+    def set_values_tensor(self, tracer_id: int, t) -> None:
+        """Copy a float64 CUDA/HIP tensor of shape (nod2D, nl-1) (reference numbering) in."""
+        import torch
+        assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+        assert tuple(t.shape) == (self.nod2D, self.nl - 1)
+        torch.cuda.current_stream().synchronize()
+        abi.check(self.lib.tracer_set_values_device(tracer_id, ctypes.c_void_p(t.data_ptr()),
+                                                    self.nl, self.nod2D), "tracer_set_values_device")
+
+    def get_values_tensor(self, tracer_id: int, out=None):
+        import torch
+        if out is None:
+            out = torch.empty((self.nod2D, self.nl - 1), dtype=torch.float64, device="cuda")
+        torch.cuda.current_stream().synchronize()
+        abi.check(self.lib.tracer_get_values_device(tracer_id, ctypes.c_void_p(out.data_ptr()),
+                                                    self.nl - 1, self.nod2D), "tracer_get_values_device")
+        return out
+
+    def cleanup(self) -> None:
+        self.lib.tracer_cleanup()

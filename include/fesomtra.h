@@ -1,0 +1,157 @@
+/*
+ * fesomtra: MI355X (gfx950) tracer advection for FESOM2 meshes -- C ABI.
+ *
+ * Drop-in for the cross-language interface of sbrdar/fesom2_tracer_dwarf
+ * (lib/tracer_c_interface.F90): same symbol names, argument meaning, int status codes
+ * (0 = OK, -1 = state or argument error, message on stderr), Fortran column-major arrays with
+ * 1-based index values, copy-in / copy-out semantics and one process-wide instance.  Unlike the
+ * reference, the library owns device (HBM) copies of everything; the horizontal, vertical and
+ * limiter stages of do_oce_adv_tra (lib/oce_adv_tra_driver.F90:23-294) run as HIP kernels.
+ *
+ * Conventions (as the reference): a(d1, d2) means a Fortran array, i.e. a[(j-1)*d1 + (i-1)];
+ * node / element / edge numbers are 1-based; "nl" is the number of levels (interfaces), so
+ * tracers have nl-1 layers.
+ *
+ * Entry points marked [new] have no counterpart in the reference's C interface: the reference
+ * can only advect after a restart read, so the connectivity, geometry and scheme setters that a
+ * mesh needs are added here (they fill what lib/MOD_MESH.F90 T_MESH and lib/MOD_TRACER.F90
+ * hold).  Not thread-safe; call from one thread.
+ */
+#ifndef FESOMTRA_H
+#define FESOMTRA_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference entry points (lib/tracer_c_interface.F90) ---------------------------------- */
+
+/* :46  Records the communicator; one GPU, one partition: no MPI is initialised. */
+int tracer_init_mpi(int mpi_comm);
+/* :89 */
+int tracer_finalize_mpi(void);
+/* :113 Restart-file initialisation: the derived-type restart format is not supported;
+ *      returns -1 with a message. */
+int tracer_init(const char *restart_dir);
+/* :163 One time step of every tracer: zero tendencies, do_oce_adv_tra, values += (dttf_h +
+ *      dttf_v)/hnode_new, with valuesAB := values.  (The reference advects tracer 1 only and
+ *      never zeroes del_ttf; both are corrected here.)  Returns after the device finished. */
+int tracer_advect_step(double dt);
+/* :228 min / max / plain sum over all (nl-1) x nod2D entries of tracer 1. */
+int tracer_get_stats(double *tmin, double *tmax, double *tsum);
+/* :255 */
+int tracer_get_size(int *nz, int *nn);
+/* :280 Copies tracer 1 out, values(nz=nl-1, nn=nod2D). */
+int tracer_get_values(double *values, int nz, int nn);
+/* :308 Writes restart files in the reference: unsupported, returns -1. */
+int tracer_finalize(const char *output_dir);
+/* :356 Restart-based workflow: unsupported, returns -1. */
+int tracer_run_workflow(const char *restart_dir, int nsteps, double dt);
+/* :391 Releases all device and host state. */
+int tracer_cleanup(void);
+/* :415 */
+int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D);
+/* :441 nlevels_nod2D, ulevels_nod2D (nod2D). */
+int tracer_set_levels(const int *nlevels, const int *ulevels, int nod2D);
+/* :477 hnode(nl-1, nod2D); sets hnode and hnode_new (the reference sets hnode_new only). */
+int tracer_set_thickness(const double *hnode, int nl, int nod2D);
+/* :510 eDim_nod2D must be 0 (single partition; the periodic wrap lives in the connectivity). */
+int tracer_set_partition(int myDim_nod2D, int eDim_nod2D);
+/* :539 Schemes default to UPW1/UPW1/NONE, ph = pv = 1 (as the reference). */
+int tracer_allocate_tracers(int num_tracers, int nl, int nod2D, int AB_order);
+/* :613 values(nl-1, nod2D), tracer_id 1-based. */
+int tracer_set_values(int tracer_id, const double *values, int nl, int nod2D);
+/* :652 */
+int tracer_set_velocity_zero(int nl, int nod2D, int edge2D);
+/* :694 The reference stores uv as (nl-1, edge2D, 2) although the kernels read element velocities
+ *      vel(2, nl-1, elem2D); this entry point accepts only an all-zero uv (sets w) and otherwise
+ *      returns -1 pointing to tracer_set_velocity_elem. */
+int tracer_set_velocity(const double *uv, const double *w, int nl, int nod2D, int edge2D);
+/* :749 Validates the mesh, reorders it along a space-filling curve and uploads it to HBM. */
+int tracer_init_complete(void);
+/* :769, :830 File-mesh readers: unsupported, return -1. */
+int tracer_load_mesh_partition(const char *restart_dir);
+int tracer_load_mesh_from_files(const char *mesh_path);
+/* :875 hnode = hnode_new = helem = zbar(k) - zbar(k+1) from tracer_set_zbar. */
+int tracer_init_ale_arrays(void);
+/* :931 Allocates the advection work arrays (done lazily otherwise). */
+int tracer_init_for_advection(void);
+
+/* ---- [new] mesh, dynamics and scheme setters ------------------------------------------------ */
+
+/* coord_nod2D(2, nod2D) in radians (lib/MOD_MESH.F90 coord_nod2D); used for the node order. */
+int tracer_set_coordinates(const double *coord_nod2D, int nod2D);
+/* elem2D_nodes(3, elem2D) clockwise, edges(2, edge2D), edge_tri(2, edge2D) with edge_tri(1) the
+ * element left of edges(1)->edges(2) and 0 for none, nod_in_elem2D(max_adj, nod2D),
+ * nod_in_elem2D_num(nod2D). */
+int tracer_set_connectivity(const int *elem2D_nodes, const int *edges, const int *edge_tri,
+                            const int *nod_in_elem2D, const int *nod_in_elem2D_num, int max_adj,
+                            int nod2D, int elem2D, int edge2D);
+/* nlevels, ulevels of elements (elem2D). */
+int tracer_set_elem_levels(const int *nlevels, const int *ulevels, int elem2D);
+/* elem_area(elem2D), area(nl, nod2D), areasvol(nl, nod2D), edge_dxdy(2, edge2D) [radians],
+ * edge_cross_dxdy(4, edge2D) [m], elem_cos(elem2D), gradient_sca(6, elem2D)
+ * (lib/oce_mesh.F90 mesh_areas / mesh_auxiliary_arrays). */
+int tracer_set_geometry(const double *elem_area, const double *area, const double *areasvol,
+                        const double *edge_dxdy, const double *edge_cross_dxdy,
+                        const double *elem_cos, const double *gradient_sca, int nl, int nod2D,
+                        int elem2D, int edge2D);
+/* hnode, hnode_new, Z_3d_n (nl-1, nod2D), helem (nl-1, elem2D), zbar_3d_n (nl, nod2D). */
+int tracer_set_vertical_grid(const double *hnode, const double *hnode_new, const double *helem,
+                             const double *Z_3d_n, const double *zbar_3d_n, int nl, int nod2D,
+                             int elem2D);
+/* zbar(nl): reference interface depths, for tracer_init_ale_arrays. */
+int tracer_set_zbar(const double *zbar, int nl);
+/* MUSCL support (lib/oce_muscl_adv.F90): edge_up_dn_tri(2, edge2D), nboundary_lay(nod2D).
+ * Without it the library derives nboundary_lay itself and MUSCL/MFCT are unavailable. */
+int tracer_set_muscl_support(const int *edge_up_dn_tri, const int *nboundary_lay, int nod2D,
+                             int edge2D);
+/* Element velocities vel(2, nl-1, elem2D) and w, w_i, w_e (nl, nod2D), the arrays
+ * do_oce_adv_tra reads (lib/oce_adv_tra_driver.F90:42-45).  w_i / w_e may be NULL (= w). */
+int tracer_set_velocity_elem(const double *uv, const double *w, const double *w_i,
+                             const double *w_e, int nl, int nod2D, int elem2D);
+/* Scheme of one tracer with the reference's names: hor "UPW1" | "MUSCL" | "MFCT",
+ * ver "UPW1" | "QR4C" | "CDIFF" | "PPM", lim "NONE" | "FCT"; ph, pv as tra_adv_ph / tra_adv_pv
+ * (lib/MOD_TRACER.F90:18-22).  Unknown names return -1 (the reference aborts). */
+int tracer_set_scheme(int tracer_id, const char *hor, const char *ver, const char *lim,
+                      double ph, double pv);
+
+/* ---- [new] stepping, drop-in and inspection ------------------------------------------------- */
+
+/* nsteps calls of tracer_advect_step without host synchronisation in between. */
+int tracer_advect_steps(int nsteps, double dt);
+/* The do_oce_adv_tra contract for one tracer (lib/oce_adv_tra_driver.F90:23-294): accumulates
+ * into the device tendencies del_ttf_advhoriz / del_ttf_advvert (zeroed by
+ * tracer_zero_tendencies, as the caller zeroes them) from values and valuesAB of tracer_id. */
+int tracer_do_oce_adv_tra(double dt, int tracer_id);
+int tracer_zero_tendencies(void);
+/* Copies del_ttf_advhoriz / del_ttf_advvert (nz=nl-1, nn=nod2D) out. */
+int tracer_get_tendencies(double *dttf_h, double *dttf_v, int nz, int nn);
+/* valuesAB of one tracer (nl-1, nod2D), for tracer_do_oce_adv_tra; tracer_set_values and
+ * tracer_advect_step reset it to values. */
+int tracer_set_values_ab(int tracer_id, const double *values, int nl, int nod2D);
+int tracer_get_values_by_id(int tracer_id, double *values, int nz, int nn);
+/* Over the wet layers of tracer_id: min, max, plain sum and sum of T*hnode*areasvol (FP64). */
+int tracer_get_stats_by_id(int tracer_id, double *tmin, double *tmax, double *tsum,
+                           double *vsum);
+/* Work arrays of the last tracer_do_oce_adv_tra / advect call for inspection: name is
+ * "fct_LO" (nl-1, nod2D), "adv_flux_hor" (nl-1, edge2D) or "adv_flux_ver" (nl, nod2D), the
+ * latter two after limiting as the reference leaves them.  count = number of doubles. */
+int tracer_get_work(const char *name, double *out, size_t count);
+/* Device-pointer variants of tracer_set_values / tracer_get_values for callers whose tracers
+ * already live in HBM (e.g. PyTorch tensors): values(nl-1, nod2D) float64 in reference order. */
+int tracer_set_values_device(int tracer_id, const void *values, int nl, int nod2D);
+int tracer_get_values_device(int tracer_id, void *values, int nz, int nn);
+/* Wall time of the last tracer_advect_steps call measured with HIP events, in ms. */
+int tracer_last_elapsed_ms(double *ms);
+/* Last error message (empty if none). */
+const char *tracer_last_error(void);
+/* Library version string. */
+const char *tracer_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FESOMTRA_H */

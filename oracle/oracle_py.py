@@ -185,3 +185,36 @@ class Oracle:
         self.step(values, sc, dt)
         self._captured = self.work_arrays()
         self._captured["edge_up_dn_grad"] = egrad[: m.edge2D]
+
+
+def _work_ptrs(w: FoWork):
+    return w
+
+
+class DropIn:
+    """do_oce_adv_tra on the CPU restatement with independent values / valuesAB."""
+
+    def __init__(self, oracle: Oracle):
+        self.o = oracle
+
+    def run(self, values: np.ndarray, values_ab: np.ndarray, sc, dt: float):
+        o = self.o
+        m = o.case.mesh
+        L = m.nl - 1
+        ttf = np.ascontiguousarray(values, dtype=np.float64)
+        tab = np.ascontiguousarray(values_ab, dtype=np.float64)
+        wk = o.work
+        lib().fo_tracer_gradients(ctypes.byref(o.mesh), _p(tab, ctypes.c_double), wk.tr_xy)
+        lib().fo_fill_up_dn_grad(ctypes.byref(o.mesh), wk.tr_xy, wk.edge_up_dn_grad)
+        for name in ("dttf_h", "dttf_v"):
+            arr = np.ctypeslib.as_array(getattr(wk, name), shape=(m.nod2D * L,))
+            arr[:] = 0.0
+        s = o.scheme(sc)
+        rc = lib().fo_do_oce_adv_tra(ctypes.byref(o.mesh), ctypes.byref(s), dt,
+                                     _p(o.uv, ctypes.c_double), _p(o.w, ctypes.c_double),
+                                     _p(o.wi, ctypes.c_double), _p(o.we, ctypes.c_double),
+                                     _p(ttf, ctypes.c_double), _p(tab, ctypes.c_double),
+                                     ctypes.byref(wk))
+        if rc != 0:
+            raise RuntimeError("oracle do_oce_adv_tra failed")
+        return o.work_arrays()
