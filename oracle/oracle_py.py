@@ -187,10 +187,6 @@ class Oracle:
         self._captured["edge_up_dn_grad"] = egrad[: m.edge2D]
 
 
-def _work_ptrs(w: FoWork):
-    return w
-
-
 class DropIn:
     """do_oce_adv_tra on the CPU restatement with independent values / valuesAB."""
 

@@ -1,0 +1,82 @@
+"""Writes the golden fixtures of tests/golden from the compiled reference (oracle/_ref/ref_driver).
+
+Inputs are regenerated bit for bit from seeds by fesom2_tracer_dwarf_amd.configs; each fixture
+stores the reference's outputs plus a SHA-256 of the inputs it was made from, so a drift of the
+generator is detected instead of silently comparing different cases.
+
+    python tests/golden/make_golden.py        (needs oracle/_ref built by oracle/ref/build_ref.sh)
+"""
+
+from __future__ import annotations
+
+import hashlib
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+REPO = HERE.parents[1]
+sys.path.insert(0, str(REPO))
+sys.path.insert(0, str(REPO / "oracle" / "ref"))
+
+import bundle  # noqa: E402
+from fesom2_tracer_dwarf_amd import configs  # noqa: E402
+from fesom2_tracer_dwarf_amd.configs import Scheme  # noqa: E402
+
+SCHEME_GRID = [(h, v, l) for h in ("UPW1", "MUSCL", "MFCT") for v in ("UPW1", "QR4C", "CDIFF", "PPM")
+               for l in ("NONE", "FCT")]
+
+
+def case_hash(case) -> str:
+    h = hashlib.sha256()
+    m = case.mesh
+    for name in ("elem2D_nodes", "edges", "edge_tri", "nod_in_elem2D", "nlevels", "nlevels_nod2D",
+                 "edge_up_dn_tri", "nboundary_lay", "area", "areasvol", "edge_dxdy",
+                 "edge_cross_dxdy", "gradient_sca", "hnode", "helem", "Z_3d_n", "zbar_3d_n"):
+        h.update(np.ascontiguousarray(getattr(m, name)).tobytes())
+    for a in (case.uv, case.w, case.tracers):
+        h.update(np.ascontiguousarray(a).tobytes())
+    h.update(repr([(s.hor, s.ver, s.lim, s.ph, s.pv) for s in case.schemes]).encode())
+    h.update(repr(case.dt).encode())
+    return h.hexdigest()
+
+
+def combo_case(hor, ver, lim):
+    return configs.bathymetry_case("combo8", 8, 9, seed=21, nsteps=3, lev_lo=4,
+                                   scheme=Scheme(hor, ver, lim, 0.8, 0.7), w_amp=3e-4)
+
+
+def analytic_cases():
+    return {"analytic20_mfct_qr4c_fct": configs.analytic20(),
+            "analytic20_upw1_upw1_none": configs.analytic20(Scheme("UPW1", "UPW1", "NONE"))}
+
+
+def main():
+    if not bundle.ref_available():
+        raise SystemExit("oracle/_ref/ref_driver not built: run oracle/ref/build_ref.sh")
+    with tempfile.TemporaryDirectory() as td:
+        for name, case in analytic_cases().items():
+            p = bundle.write_bundle(Path(td) / name, case, save_steps=[1, 2, 10], save_work=True)
+            bundle.run_ref(p)
+            out = {f"values_s{s:04d}": bundle.read_values(p, case, s) for s in (1, 2, 10)}
+            for k, v in bundle.read_work(p, case).items():
+                out[f"work_{k}"] = v
+            np.savez_compressed(HERE / f"{name}.npz", input_sha256=case_hash(case), **out)
+            print("wrote", name)
+        out = {}
+        hashes = []
+        for hor, ver, lim in SCHEME_GRID:
+            case = combo_case(hor, ver, lim)
+            p = bundle.write_bundle(Path(td) / f"c_{hor}_{ver}_{lim}", case, save_steps=[3])
+            bundle.run_ref(p)
+            out[f"{hor}_{ver}_{lim}"] = bundle.read_values(p, case, 3)
+            hashes.append(case_hash(case))
+        np.savez_compressed(HERE / "combo8_schemes.npz",
+                            input_sha256=hashlib.sha256("".join(hashes).encode()).hexdigest(), **out)
+        print("wrote combo8_schemes")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,127 @@
+"""Host mesh preprocessing (geometry, levels, MUSCL support) against geometric facts."""
+import numpy as np
+import pytest
+
+from fesom2_tracer_dwarf_amd import configs
+from fesom2_tracer_dwarf_amd.mesh import periodic_mesh, stretched_layers
+
+
+def _mi(d, L):
+    return d - L * np.round(d / L)
+
+
+@pytest.fixture(scope="module", params=[False, True], ids=["canonical", "permuted"])
+def mesh(request):
+    lev = np.random.default_rng(0).integers(4, 9, size=2 * 9 * 7)
+    return periodic_mesh(9, 7, 9.0e5, 7.0e5, stretched_layers(7, 5.0, 50.0), elem_nlevels=lev,
+                         seed=5, permute=request.param)
+
+
+def test_areas_tile_the_domain(mesh):
+    m = mesh
+    assert np.isclose(m.elem_area.sum(), m.Lx * m.Ly, rtol=1e-13)
+    assert np.isclose(m.area[:, 0].sum(), m.Lx * m.Ly, rtol=1e-13)
+    # below the surface, the area at a level is one third of the wet element area per node
+    for k in range(m.nl - 1):
+        wet = m.nlevels - 1 >= k + 1
+        assert np.isclose(m.area[:, k].sum(), m.elem_area[wet].sum(), rtol=1e-13)
+
+
+def test_clockwise_triangles_and_left_element(mesh):
+    m = mesh
+    xy = m.xy_m
+    en = m.elem2D_nodes - 1
+    d1 = np.stack([_mi(xy[en[:, 1], 0] - xy[en[:, 0], 0], m.Lx), _mi(xy[en[:, 1], 1] - xy[en[:, 0], 1], m.Ly)], 1)
+    d2 = np.stack([_mi(xy[en[:, 2], 0] - xy[en[:, 0], 0], m.Lx), _mi(xy[en[:, 2], 1] - xy[en[:, 0], 1], m.Ly)], 1)
+    assert np.all(d1[:, 0] * d2[:, 1] - d1[:, 1] * d2[:, 0] < 0)
+    # edge_tri(1) contains both edge nodes and its third vertex lies left of n1 -> n2
+    e = m.edges - 1
+    for s, sign in ((0, 1.0), (1, -1.0)):
+        t = m.edge_tri[:, s] - 1
+        verts = en[t]
+        assert np.all((verts == e[:, :1]).any(1) & (verts == e[:, 1:]).any(1))
+        third = np.where((verts != e[:, :1]) & (verts != e[:, 1:]), verts, -1).max(1)
+        a = np.stack([_mi(xy[e[:, 1], 0] - xy[e[:, 0], 0], m.Lx), _mi(xy[e[:, 1], 1] - xy[e[:, 0], 1], m.Ly)], 1)
+        b = np.stack([_mi(xy[third, 0] - xy[e[:, 0], 0], m.Lx), _mi(xy[third, 1] - xy[e[:, 0], 1], m.Ly)], 1)
+        assert np.all(sign * (a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]) > 0)
+
+
+def test_gradient_exact_for_linear_functions(mesh):
+    m = mesh
+    xy = m.xy_m
+    en = m.elem2D_nodes - 1
+    a, b = 2.5, -1.25
+    base = xy[en[:, 0]]
+    for c in range(3):
+        pass
+    loc = np.stack([np.stack([_mi(xy[en[:, c], 0] - base[:, 0], m.Lx), _mi(xy[en[:, c], 1] - base[:, 1], m.Ly)], 1)
+                    for c in range(3)], 1)
+    f = a * loc[:, :, 0] + b * loc[:, :, 1] + 3.0
+    gx = (m.gradient_sca[:, 0:3] * f).sum(1)
+    gy = (m.gradient_sca[:, 3:6] * f).sum(1)
+    assert np.allclose(gx, a, rtol=1e-12) and np.allclose(gy, b, rtol=1e-12)
+
+
+def test_nod_in_elem_complete_and_ordered(mesh):
+    m = mesh
+    en = m.elem2D_nodes - 1
+    assert np.all(m.nod_in_elem2D_num == 6)
+    for n in range(m.nod2D):
+        lst = m.nod_in_elem2D[n, : m.nod_in_elem2D_num[n]] - 1
+        assert np.all(np.diff(lst) > 0)
+        assert set(lst) == set(np.nonzero((en == n).any(1))[0])
+
+
+def test_levels_follow_the_max_rule(mesh):
+    m = mesh
+    en = m.elem2D_nodes - 1
+    for n in range(m.nod2D):
+        els = np.nonzero((en == n).any(1))[0]
+        assert m.nlevels_nod2D[n] == m.nlevels[els].max()
+        assert m.nlevels_nod2D_min[n] == m.nlevels[els].min()
+    assert np.all((m.areasvol[:, : m.nl - 1] > 0) == configs.wet_mask(m))
+
+
+def test_up_down_triangles_contain_the_ray(mesh):
+    """The upwind triangle of n1->n2 contains the point a little beyond n1 away from n2 (and the
+    downwind one the point beyond n2), checked with barycentric coordinates."""
+    m = mesh
+    xy = m.xy_m
+    en = m.elem2D_nodes - 1
+    e = m.edges - 1
+    d = np.stack([_mi(xy[e[:, 1], 0] - xy[e[:, 0], 0], m.Lx), _mi(xy[e[:, 1], 1] - xy[e[:, 0], 1], m.Ly)], 1)
+    for s, (node, sgn) in enumerate(((e[:, 0], -1.0), (e[:, 1], 1.0))):
+        t = m.edge_up_dn_tri[:, s] - 1
+        p = xy[node] + sgn * 1e-3 * d
+        v = np.stack([xy[en[t, c]] for c in range(3)], 1)
+        rel = np.stack([np.stack([_mi(v[:, c, 0] - p[:, 0], m.Lx), _mi(v[:, c, 1] - p[:, 1], m.Ly)], 1) for c in range(3)], 1)
+        # point inside the triangle: the signed areas (p, vi, vj) all have one sign
+        sa = [rel[:, i, 0] * rel[:, j, 1] - rel[:, i, 1] * rel[:, j, 0] for i, j in ((0, 1), (1, 2), (2, 0))]
+        sa = np.stack(sa, 1)
+        tol = 1e-9 * (m.Lx / m.nx) ** 2
+        assert np.all((sa <= tol).all(1) | (sa >= -tol).all(1))
+
+
+def test_nboundary_lay_brute_force(mesh):
+    m = mesh
+    want = np.full(m.nod2D, m.nl - 1)
+    for ed in range(m.edge2D):
+        lev = min(m.nlevels[m.edge_tri[ed, 0] - 1], m.nlevels[m.edge_tri[ed, 1] - 1]) - 1
+        for n in m.edges[ed] - 1:
+            want[n] = min(want[n], lev)
+    assert np.array_equal(m.nboundary_lay, want)
+
+
+def test_streamfunction_flow_is_divergence_free():
+    case = configs.analytic20()
+    m = case.mesh
+    # net volume flux out of every dual cell (sum over edges, both segments) vanishes
+    t1, t2 = m.edge_tri[:, 0] - 1, m.edge_tri[:, 1] - 1
+    cd = m.edge_cross_dxdy
+    v = ((-case.uv[t1, 0, 1] * cd[:, 0] + case.uv[t1, 0, 0] * cd[:, 1])
+         + (case.uv[t2, 0, 1] * cd[:, 2] - case.uv[t2, 0, 0] * cd[:, 3]))
+    div = np.zeros(m.nod2D)
+    np.add.at(div, m.edges[:, 0] - 1, v)
+    np.add.at(div, m.edges[:, 1] - 1, -v)
+    assert np.max(np.abs(div)) <= 1e-12 * np.max(np.abs(v))
+    assert np.isclose(np.max(np.hypot(case.uv[:, 0, 0], case.uv[:, 0, 1])), 0.1, rtol=1e-12)
