@@ -66,6 +66,8 @@ _SIGNATURES = {
     "tracer_set_values_device": (_c_int, [_c_int, _vp, _c_int, _c_int]),
     "tracer_get_values_device": (_c_int, [_c_int, _vp, _c_int, _c_int]),
     "tracer_last_elapsed_ms": (_c_int, [_pd]),
+    "tracer_set_kernel_timing": (_c_int, [_c_int]),
+    "tracer_get_kernel_times": (_c_int, [_pd, ctypes.POINTER(ctypes.c_long)]),
     "tracer_last_error": (_c_char_p, []),
     "tracer_version": (_c_char_p, []),
 }

@@ -62,6 +62,9 @@ struct State {
     bool last_fct = false;
     double last_ms = 0.0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool kernel_timing = false;
+    double stage_ms[4] = {0, 0, 0, 0};
+    long stage_count = 0;
 };
 
 State *g = nullptr;
@@ -507,10 +510,16 @@ int advect(State &s, int nsteps, double dt) {
     if (ensure_device(s)) return -1;
     if (s.ntr == 0) return fail("no tracers allocated (tracer_allocate_tracers)");
     for (int i = 0; i < s.ntr; ++i) s.dTAB[i] = nullptr;  // valuesAB := values
+    std::vector<hipEvent_t> evs;
+    if (s.kernel_timing) {
+        evs.resize((size_t)nsteps * s.ntr * 5);
+        for (auto &e : evs) HIPCHK(hipEventCreate(&e));
+    }
     HIPCHK(hipEventRecord(s.ev0, s.stream));
     for (int st = 0; st < nsteps; ++st)
         for (int i = 0; i < s.ntr; ++i) {
             StepArgs a = step_args(s, i, dt);
+            if (s.kernel_timing) a.ev = &evs[((size_t)st * s.ntr + i) * 5];
             hipError_t e = launch_fct_step(s.md, a, s.stream);
             if (e != hipSuccess) return fail("kernel launch failed: %s", hipGetErrorString(e));
             s.last_fct = a.fct;
@@ -520,6 +529,16 @@ int advect(State &s, int nsteps, double dt) {
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
     s.last_ms = ms;
+    if (s.kernel_timing) {
+        for (size_t b = 0; b + 4 < evs.size(); b += 5)
+            for (int k = 0; k < 4; ++k) {
+                float d = 0.f;
+                HIPCHK(hipEventElapsedTime(&d, evs[b + k], evs[b + k + 1]));
+                s.stage_ms[k] += d;
+            }
+        s.stage_count += (long)(evs.size() / 5);
+        for (auto &e : evs) hipEventDestroy(e);
+    }
     return 0;
 }
 
@@ -1060,6 +1079,21 @@ int tracer_get_values_device(int tracer_id, void *values, int nz, int nn) {
     hipError_t e = launch_gather_rows(s.dT[tracer_id - 1], static_cast<double *>(values), s.d_inv_n, s.N, nz, s.stream);
     if (e != hipSuccess) return fail("gather launch failed: %s", hipGetErrorString(e));
     HIPCHK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+int tracer_set_kernel_timing(int enable) {
+    State &s = S();
+    s.kernel_timing = enable != 0;
+    for (double &v : s.stage_ms) v = 0.0;
+    s.stage_count = 0;
+    return 0;
+}
+
+int tracer_get_kernel_times(double *stage_ms, long *count) {
+    State &s = S();
+    for (int k = 0; k < 4; ++k) stage_ms[k] = s.stage_count ? s.stage_ms[k] / s.stage_count : 0.0;
+    *count = s.stage_count;
     return 0;
 }
 

@@ -332,6 +332,14 @@ hipError_t launch_fct_step(const MeshDev &m, const StepArgs &a, hipStream_t s) {
     const int cols = cols_for(m.L, B);
     const dim3 ge((m.E + cols - 1) / cols), gn((m.N + cols - 1) / cols);
     const bool fct = a.fct;
+#define MARK(i)                                            \
+    do {                                                   \
+        if (a.ev) {                                        \
+            hipError_t _r = hipEventRecord(a.ev[i], s);    \
+            if (_r != hipSuccess) return _r;               \
+        }                                                  \
+    } while (0)
+    MARK(0);
     switch (a.hor * 2 + (fct ? 1 : 0)) {
     case HOR_UPW1 * 2 + 0: k_edge<HOR_UPW1, false><<<ge, B, 0, s>>>(m, cols, a.T, a.TAB, a.ph, a.flo, a.adfh); break;
     case HOR_UPW1 * 2 + 1: k_edge<HOR_UPW1, true><<<ge, B, 0, s>>>(m, cols, a.T, a.TAB, a.ph, a.flo, a.adfh); break;
@@ -342,6 +350,7 @@ hipError_t launch_fct_step(const MeshDev &m, const StepArgs &a, hipStream_t s) {
     default: return hipErrorInvalidValue;
     }
     LAUNCH_CHECK();
+    MARK(1);
     if (fct) {
         switch (a.ver) {
         case VER_UPW1: k_node_lo<VER_UPW1><<<gn, B, 0, s>>>(m, cols, a.T, a.TAB, a.pv, a.dt, a.flo, a.LO, a.adfv); break;
@@ -351,8 +360,10 @@ hipError_t launch_fct_step(const MeshDev &m, const StepArgs &a, hipStream_t s) {
         default: return hipErrorInvalidValue;
         }
         LAUNCH_CHECK();
+        MARK(2);
         k_bounds<<<gn, B, 0, s>>>(m, cols, a.T, a.LO, a.adfh, a.adfv, a.dt, a.Rp, a.Rm);
         LAUNCH_CHECK();
+        MARK(3);
         k_final<true><<<gn, B, 0, s>>>(m, cols, a.T, a.LO, a.adfh, a.adfv, a.Rp, a.Rm, a.dt,
                                        a.acc_h, a.acc_v, a.dbg_h, a.dbg_v);
     } else {
@@ -364,10 +375,14 @@ hipError_t launch_fct_step(const MeshDev &m, const StepArgs &a, hipStream_t s) {
         default: return hipErrorInvalidValue;
         }
         LAUNCH_CHECK();
+        MARK(2);
+        MARK(3);
         k_final<false><<<gn, B, 0, s>>>(m, cols, a.T, nullptr, a.adfh, a.adfv, nullptr, nullptr,
                                         a.dt, a.acc_h, a.acc_v, nullptr, nullptr);
     }
     LAUNCH_CHECK();
+    MARK(4);
+#undef MARK
     return hipSuccess;
 }
 

@@ -19,6 +19,7 @@ struct StepArgs {
     double *adfv;        // (N, NL) work
     double *acc_h, *acc_v;  // do_oce_adv_tra accumulators or null
     double *dbg_h, *dbg_v;  // limited fluxes for inspection or null
+    hipEvent_t *ev;         // optional: 5 events recorded around the 4 stages, or null
 };
 
 hipError_t launch_fct_step(const MeshDev &m, const StepArgs &a, hipStream_t s);

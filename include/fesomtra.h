@@ -146,6 +146,12 @@ int tracer_set_values_device(int tracer_id, const void *values, int nl, int nod2
 int tracer_get_values_device(int tracer_id, void *values, int nz, int nn);
 /* Wall time of the last tracer_advect_steps call measured with HIP events, in ms. */
 int tracer_last_elapsed_ms(double *ms);
+/* Per-stage HIP-event timing of the advect calls that follow (enable = 1) on the library's
+ * stream: stage 0 edge fluxes, 1 low-order solution / vertical fluxes, 2 limiter bounds, 3
+ * limiting + update.  tracer_get_kernel_times returns the mean ms per tracer-step of each stage
+ * and the number of tracer-steps averaged. */
+int tracer_set_kernel_timing(int enable);
+int tracer_get_kernel_times(double *stage_ms, long *count);
 /* Last error message (empty if none). */
 const char *tracer_last_error(void);
 /* Library version string. */
