@@ -17,10 +17,11 @@ CSRC = PKG / "csrc"
 REPO = PKG.parent
 LIB = PKG / "libfesomtra.so"
 SOURCES = ["fesomtra_kernels.hip", "fesomtra_abi.cpp"]
-HEADERS = ["fesomtra_device.hpp", "fesomtra_kernels.hpp"]
+HEADERS = ["fesomtra_types.hpp", "fesomtra_column.hpp", "fesomtra_kernels.hpp"]
 
 # -ffp-contract=off: the FP64 path keeps every a*b+c unfused, as the reference's CPU build does,
-# so results are bit-identical to it.  Packed FP32 instructions are disabled.
+# so results are bit-identical to it.  Packed FP32 (v_pk_*_f32) instructions are disabled: some of
+# their forms have produced silently wrong results on this platform.
 HIPFLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-x", "hip",
     "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

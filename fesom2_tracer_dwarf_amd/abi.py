@@ -68,6 +68,10 @@ _SIGNATURES = {
     "tracer_last_elapsed_ms": (_c_int, [_pd]),
     "tracer_set_kernel_timing": (_c_int, [_c_int]),
     "tracer_get_kernel_times": (_c_int, [_pd, ctypes.POINTER(ctypes.c_long)]),
+    "tracer_set_precision": (_c_int, [_c_int]),
+    "tracer_set_tracers_per_pass": (_c_int, [_c_int]),
+    "tracer_set_launch_config": (_c_int, [_c_int, _c_int, _c_int, _c_int]),
+    "tracer_get_launch_config": (_c_int, [_pi]),
     "tracer_last_error": (_c_char_p, []),
     "tracer_version": (_c_char_p, []),
 }

@@ -2,9 +2,10 @@
 //
 // Mirrors lib/tracer_c_interface.F90: one process-wide instance, int status codes, copy-in /
 // copy-out of Fortran-ordered arrays with 1-based indices.  At tracer_init_complete the mesh is
-// validated, renumbered along a Hilbert curve (nodes) with elements and edges following their
-// nodes, and uploaded to HBM together with the CSR node->edge lists (kept in the reference's
-// edge order so every gathered sum is added in the same order as the reference's scatter).
+// validated, renumbered (nodes along a Hilbert curve, elements and edges after their first
+// node), transposed to the level-major device layout and uploaded to HBM together with the CSR
+// node->edge lists, which keep the reference's edge order so every gathered sum is added in the
+// order of the reference's serial scatter.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,7 +18,6 @@
 #include <vector>
 
 #include "../../include/fesomtra.h"
-#include "fesomtra_device.hpp"
 #include "fesomtra_kernels.hpp"
 
 using namespace fesomtra;
@@ -28,6 +28,9 @@ struct SchemeSel {
     int hor = HOR_UPW1, ver = VER_UPW1;
     bool fct = false;
     double ph = 1.0, pv = 1.0;
+    bool operator==(const SchemeSel &o) const {
+        return hor == o.hor && ver == o.ver && fct == o.fct && ph == o.ph && pv == o.pv;
+    }
 };
 
 struct State {
@@ -46,18 +49,26 @@ struct State {
     int ntr = 0;
     std::vector<std::vector<double>> values;
     std::vector<SchemeSel> schemes;
-    bool initialized = false;  // tracer_init_complete / reference "tracer_initialized"
+    bool initialized = false;  // the reference's "tracer_initialized"
+    int prec = PREC_F64;
+    int kt_max = 1;
+    LaunchCfg cfg{};
 
     // device side
     bool on_device = false;
     hipStream_t stream = nullptr;
     std::vector<int> perm_n, inv_n, perm_t, inv_t, perm_e, inv_e;
-    MeshDev md{};
+    size_t Np = 0, Tp = 0, Ep = 0;
+    MeshCol<double> m64{};
+    MeshCol<float> m32{};
     std::vector<void *> allocs;
-    std::vector<double *> dT, dTAB;
-    double *flo = nullptr, *adfh = nullptr, *LO = nullptr, *Rp = nullptr, *Rm = nullptr;
-    double *adfv = nullptr, *acc_h = nullptr, *acc_v = nullptr, *dbg_h = nullptr, *dbg_v = nullptr;
-    int *d_perm_n = nullptr, *d_inv_n = nullptr;
+    std::vector<void *> dT, dTAB;  // per tracer, storage type of the precision mode
+    void *flo[kMaxKT] = {}, *adfh[kMaxKT] = {}, *LO[kMaxKT] = {}, *Rp[kMaxKT] = {}, *Rm[kMaxKT] = {};
+    void *adfv[kMaxKT] = {};
+    double *acc_h = nullptr, *acc_v = nullptr;
+    void *dbg_h = nullptr, *dbg_v = nullptr;
+    double *stage = nullptr;  // reference-layout staging buffer (N * L doubles)
+    int *d_perm_n = nullptr;
     double4 *part = nullptr;
     bool last_fct = false;
     double last_ms = 0.0;
@@ -87,27 +98,37 @@ int fail(const char *fmt, ...) {
     return -1;
 }
 
-#define HIPCHK(call)                                                                  \
-    do {                                                                              \
-        hipError_t _e = (call);                                                       \
+#define HIPCHK(call)                                                                      \
+    do {                                                                                  \
+        hipError_t _e = (call);                                                           \
         if (_e != hipSuccess) return fail("%s failed: %s", #call, hipGetErrorString(_e)); \
     } while (0)
 
-template <class T>
-int dalloc(State &s, T **p, size_t count) {
+size_t storage_bytes(int prec) { return prec == PREC_F64 ? 8 : (prec == PREC_F32 ? 4 : 2); }
+size_t compute_bytes(int prec) { return prec == PREC_F64 ? 8 : 4; }
+
+int dalloc_bytes(State &s, void **p, size_t bytes) {
     void *q = nullptr;
-    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    bytes = std::max<size_t>(bytes, 256);
     hipError_t e = hipMalloc(&q, bytes);
     if (e != hipSuccess) return fail("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
     s.allocs.push_back(q);
+    *p = q;
+    return 0;
+}
+template <class T>
+int dalloc(State &s, T **p, size_t count) {
+    void *q;
+    if (dalloc_bytes(s, &q, count * sizeof(T))) return -1;
     *p = static_cast<T *>(q);
     return 0;
 }
-
 template <class T>
-int upload(State &s, T **p, const std::vector<T> &h) {
-    if (dalloc(s, p, h.size())) return -1;
-    if (!h.empty()) HIPCHK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+int upload(State &s, const T **p, const std::vector<T> &h) {
+    T *q;
+    if (dalloc(s, &q, h.size())) return -1;
+    if (!h.empty()) HIPCHK(hipMemcpy(q, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    *p = q;
     return 0;
 }
 
@@ -117,8 +138,15 @@ void release_device(State &s) {
     s.allocs.clear();
     s.dT.clear();
     s.dTAB.clear();
+    for (int i = 0; i < kMaxKT; ++i) s.flo[i] = s.adfh[i] = s.LO[i] = s.Rp[i] = s.Rm[i] = s.adfv[i] = nullptr;
+    s.acc_h = s.acc_v = nullptr;
+    s.dbg_h = s.dbg_v = nullptr;
+    s.stage = nullptr;
+    s.d_perm_n = nullptr;
+    s.part = nullptr;
     s.on_device = false;
-    s.md = MeshDev{};
+    s.m64 = MeshCol<double>{};
+    s.m32 = MeshCol<float>{};
 }
 
 int check_dims(const State &s, int nl, int nod2D, const char *who) {
@@ -221,8 +249,6 @@ int validate(State &s) {
     if (!s.have_hnode) return fail("layer thickness not set (tracer_set_vertical_grid / tracer_set_thickness)");
     if (!s.have_vert) return fail("vertical grid Z_3d_n / zbar_3d_n / helem not set (tracer_set_vertical_grid)");
     if (s.myDim >= 0 && s.myDim != s.N) return fail("myDim_nod2D=%d must equal nod2D=%d", s.myDim, s.N);
-    if (s.nl > 0xffff) return fail("nl=%d too large", s.nl);
-    if (s.nl - 1 > 256) return fail("nl-1=%d layers exceed the 256-thread column tile", s.nl - 1);
     for (int n = 0; n < s.N; ++n) {
         if (s.ulev_n[n] != 1) return fail("node %d: ulevels_nod2D=%d; cavities (ulevels > 1) are not supported", n + 1, s.ulev_n[n]);
         if (s.nlev_n[n] < 3 || s.nlev_n[n] > s.nl) return fail("node %d: nlevels_nod2D=%d outside [3, nl]", n + 1, s.nlev_n[n]);
@@ -258,6 +284,90 @@ void derive_nboundary_lay(State &s) {
     }
 }
 
+inline size_t pad64(size_t n) { return (n + 63) / 64 * 64; }
+
+// Device copy of a reference (rows, ld) array, transposed to [ld][rows_padded] in device order.
+template <class M>
+std::vector<M> level_major(const std::vector<double> &src, int ld, const std::vector<int> &perm, size_t pad) {
+    std::vector<M> h((size_t)ld * pad, M(0));
+    for (size_t r = 0; r < perm.size(); ++r) {
+        const double *row = &src[(size_t)perm[r] * ld];
+        for (int k = 0; k < ld; ++k) h[(size_t)k * pad + r] = (M)row[k];
+    }
+    return h;
+}
+// Device copy of a reference (rows, ld) array kept entity-major, rows in device order.
+template <class M>
+std::vector<M> rows_of(const std::vector<double> &src, int ld, const std::vector<int> &perm) {
+    std::vector<M> h((size_t)ld * perm.size());
+    for (size_t r = 0; r < perm.size(); ++r)
+        for (int k = 0; k < ld; ++k) h[r * ld + k] = (M)src[(size_t)perm[r] * ld + k];
+    return h;
+}
+
+template <class M>
+int upload_mesh(State &s, MeshCol<M> &m, const std::vector<int> &ints_n1, const std::vector<int> &ints_n2,
+                const std::vector<int> &ints_n3) {
+    const int nl = s.nl, L = nl - 1;
+    m.N = s.N;
+    m.T = s.T;
+    m.E = s.E;
+    m.nl = nl;
+    m.L = L;
+    m.Np = (int)s.Np;
+    m.Tp = (int)s.Tp;
+    m.Ep = (int)s.Ep;
+    m.max_adj = s.max_adj;
+    if (upload(s, &m.nlev_n, ints_n1) || upload(s, &m.nlev_min, ints_n2) || upload(s, &m.nbl, ints_n3)) return -1;
+    if (upload(s, &m.hnode, level_major<M>(s.hnode, L, s.perm_n, s.Np))) return -1;
+    if (upload(s, &m.hnode_new, level_major<M>(s.hnode_new, L, s.perm_n, s.Np))) return -1;
+    if (upload(s, &m.Z, level_major<M>(s.Z, L, s.perm_n, s.Np))) return -1;
+    if (upload(s, &m.area, level_major<M>(s.area, nl, s.perm_n, s.Np))) return -1;
+    if (upload(s, &m.areasvol, level_major<M>(s.areasvol, nl, s.perm_n, s.Np))) return -1;
+    if (upload(s, &m.zbar, level_major<M>(s.zbar3, nl, s.perm_n, s.Np))) return -1;
+    if (upload(s, &m.w, level_major<M>(s.w, nl, s.perm_n, s.Np))) return -1;
+    m.we = m.w;
+    if (s.we != s.w && upload(s, &m.we, level_major<M>(s.we, nl, s.perm_n, s.Np))) return -1;
+    // elements
+    std::vector<double> ea(s.T), ec(s.T);
+    for (int t = 0; t < s.T; ++t) {
+        ea[t] = s.elem_area[s.perm_t[t]];
+        ec[t] = s.elem_cos[s.perm_t[t]];
+    }
+    if (upload(s, &m.elem_area, std::vector<M>(ea.begin(), ea.end()))) return -1;
+    if (upload(s, &m.elem_cos, std::vector<M>(ec.begin(), ec.end()))) return -1;
+    if (upload(s, &m.gsca, rows_of<M>(s.gsca, 6, s.perm_t))) return -1;
+    if (upload(s, &m.helem, level_major<M>(s.helem, L, s.perm_t, s.Tp))) return -1;
+    {
+        std::vector<M> h((size_t)L * s.Tp * 2, M(0));
+        for (int t = 0; t < s.T; ++t) {
+            const double *row = &s.uv[(size_t)s.perm_t[t] * L * 2];
+            for (int k = 0; k < L; ++k) {
+                h[((size_t)k * s.Tp + t) * 2] = (M)row[2 * k];
+                h[((size_t)k * s.Tp + t) * 2 + 1] = (M)row[2 * k + 1];
+            }
+        }
+        if (upload(s, &m.uv, h)) return -1;
+    }
+    if (upload(s, &m.ecd, rows_of<M>(s.ecd, 4, s.perm_e))) return -1;
+    if (upload(s, &m.edxdy, rows_of<M>(s.edxdy, 2, s.perm_e))) return -1;
+    return 0;
+}
+
+template <class M>
+void copy_int_mesh(MeshCol<M> &dst, const MeshCol<double> &src) {
+    dst.ne_ptr = src.ne_ptr;
+    dst.ne_e2 = src.ne_e2;
+    dst.ne_top = src.ne_top;
+    dst.ne_other = src.ne_other;
+    dst.nie = src.nie;
+    dst.nie_num = src.nie_num;
+    dst.enodes = src.enodes;
+    dst.nlev_e = src.nlev_e;
+    dst.edge_ne = src.edge_ne;
+    dst.updn = src.updn;
+}
+
 int build_device(State &s) {
     if (validate(s)) return -1;
     for (const auto &sc : s.schemes)
@@ -269,94 +379,51 @@ int build_device(State &s) {
     if (!s.ev0) HIPCHK(hipEventCreate(&s.ev0));
     if (!s.ev1) HIPCHK(hipEventCreate(&s.ev1));
     const int N = s.N, T = s.T, E = s.E, nl = s.nl, L = nl - 1, A = s.max_adj;
+    s.Np = pad64(N);
+    s.Tp = pad64(T);
+    s.Ep = pad64(E);
 
     // Renumbering: nodes along the curve, elements / edges by their smallest new node.
     s.perm_n = node_order(s);
     s.inv_n.assign(N, 0);
     for (int i = 0; i < N; ++i) s.inv_n[s.perm_n[i]] = i;
-    s.perm_t.resize(T);
-    std::iota(s.perm_t.begin(), s.perm_t.end(), 0);
     {
         std::vector<int64_t> key(T);
         for (int t = 0; t < T; ++t) {
-            int a = s.inv_n[s.enodes[3 * t]], b = s.inv_n[s.enodes[3 * t + 1]], c = s.inv_n[s.enodes[3 * t + 2]];
-            key[t] = (int64_t)std::min(a, std::min(b, c)) * 4 + 0;
+            const int a = s.inv_n[s.enodes[3 * t]], b = s.inv_n[s.enodes[3 * t + 1]], c = s.inv_n[s.enodes[3 * t + 2]];
+            key[t] = std::min(a, std::min(b, c));
         }
+        s.perm_t.resize(T);
+        std::iota(s.perm_t.begin(), s.perm_t.end(), 0);
         std::stable_sort(s.perm_t.begin(), s.perm_t.end(), [&](int a, int b) { return key[a] < key[b]; });
     }
     s.inv_t.assign(T, 0);
     for (int i = 0; i < T; ++i) s.inv_t[s.perm_t[i]] = i;
-    s.perm_e.resize(E);
-    std::iota(s.perm_e.begin(), s.perm_e.end(), 0);
     {
         std::vector<int64_t> key(E);
         for (int e = 0; e < E; ++e) {
             const int a = s.inv_n[s.edges[2 * e]], b = s.inv_n[s.edges[2 * e + 1]];
             key[e] = (int64_t)std::min(a, b) * (int64_t)N + std::max(a, b);
         }
+        s.perm_e.resize(E);
+        std::iota(s.perm_e.begin(), s.perm_e.end(), 0);
         std::stable_sort(s.perm_e.begin(), s.perm_e.end(), [&](int a, int b) { return key[a] < key[b]; });
     }
     s.inv_e.assign(E, 0);
     for (int i = 0; i < E; ++i) s.inv_e[s.perm_e[i]] = i;
 
-    MeshDev &m = s.md;
-    m.N = N;
-    m.T = T;
-    m.E = E;
-    m.nl = nl;
-    m.L = L;
-    m.max_adj = A;
-
-    // nodes
-    std::vector<int> i1(N), i2(N), i3(N), i4(N), i5(N);
-    std::vector<int> nlev_min(N), ulev_max(N);
-    for (int o = 0; o < N; ++o) {  // lib/oce_mesh.F90:1522-1560
-        int mn = 1 << 30, mx = -(1 << 30);
-        for (int k = 0; k < s.nie_num[o]; ++k) {
-            const int t = s.nie[(size_t)o * A + k];
-            mn = std::min(mn, s.nlev_e[t]);
-            mx = std::max(mx, s.ulev_e[t]);
-        }
-        nlev_min[o] = mn;
-        ulev_max[o] = mx;
-    }
+    // node integers (lib/oce_mesh.F90:1522-1560 for nlevels_nod2D_min)
+    std::vector<int> i_nlev(N), i_nmin(N), i_nbl(N);
     for (int n = 0; n < N; ++n) {
         const int o = s.perm_n[n];
-        i1[n] = s.nlev_n[o];
-        i2[n] = s.ulev_n[o];
-        i3[n] = nlev_min[o];
-        i4[n] = ulev_max[o];
-        i5[n] = s.nbl[o];
+        int mn = 1 << 30;
+        for (int k = 0; k < s.nie_num[o]; ++k) mn = std::min(mn, s.nlev_e[s.nie[(size_t)o * A + k]]);
+        i_nlev[n] = s.nlev_n[o];
+        i_nmin[n] = mn;
+        i_nbl[n] = s.nbl[o];
     }
-    int *p_i;
-    if (upload(s, &p_i, i1)) return -1; m.nlev_n = p_i;
-    if (upload(s, &p_i, i2)) return -1; m.ulev_n = p_i;
-    if (upload(s, &p_i, i3)) return -1; m.nlev_min = p_i;
-    if (upload(s, &p_i, i4)) return -1; m.ulev_max = p_i;
-    if (upload(s, &p_i, i5)) return -1; m.nbl = p_i;
-
-    auto rows_d = [&](const std::vector<double> &src, int ld, const std::vector<int> &perm, const double **dst) {
-        std::vector<double> h((size_t)perm.size() * ld);
-        for (size_t r = 0; r < perm.size(); ++r)
-            std::memcpy(&h[r * ld], &src[(size_t)perm[r] * ld], sizeof(double) * ld);
-        double *p;
-        if (upload(s, &p, h)) return -1;
-        *dst = p;
-        return 0;
-    };
-    if (rows_d(s.hnode, L, s.perm_n, &m.hnode)) return -1;
-    if (rows_d(s.hnode_new, L, s.perm_n, &m.hnode_new)) return -1;
-    if (rows_d(s.Z, L, s.perm_n, &m.Z)) return -1;
-    if (rows_d(s.area, nl, s.perm_n, &m.area)) return -1;
-    if (rows_d(s.areasvol, nl, s.perm_n, &m.areasvol)) return -1;
-    if (rows_d(s.zbar3, nl, s.perm_n, &m.zbar)) return -1;
-    if (rows_d(s.w, nl, s.perm_n, &m.w)) return -1;
-    m.wi = m.w;
-    m.we = m.w;
-    if (s.wi != s.w && rows_d(s.wi, nl, s.perm_n, &m.wi)) return -1;
-    if (s.we != s.w && rows_d(s.we, nl, s.perm_n, &m.we)) return -1;
-
-    // nod_in_elem2D in the reference's list order, element ids renumbered
+    MeshCol<double> &m = s.m64;
+    // connectivity (shared by both precisions)
     {
         std::vector<int> h((size_t)N * A, 0), cnt(N);
         for (int n = 0; n < N; ++n) {
@@ -364,71 +431,37 @@ int build_device(State &s) {
             cnt[n] = s.nie_num[o];
             for (int k = 0; k < cnt[n]; ++k) h[(size_t)n * A + k] = s.inv_t[s.nie[(size_t)o * A + k]];
         }
-        if (upload(s, &p_i, h)) return -1; m.nie = p_i;
-        if (upload(s, &p_i, cnt)) return -1; m.nie_num = p_i;
+        if (upload(s, &m.nie, h) || upload(s, &m.nie_num, cnt)) return -1;
     }
-    // elements
     {
-        std::vector<int> en((size_t)3 * T), nle(T), ule(T);
-        std::vector<double> ar(T), ec(T), gs((size_t)6 * T);
+        std::vector<int> en((size_t)3 * T), nle(T);
         for (int t = 0; t < T; ++t) {
             const int o = s.perm_t[t];
             for (int c = 0; c < 3; ++c) en[3 * t + c] = s.inv_n[s.enodes[3 * o + c]];
             nle[t] = s.nlev_e[o];
-            ule[t] = s.ulev_e[o];
-            ar[t] = s.elem_area[o];
-            ec[t] = s.elem_cos[o];
-            for (int c = 0; c < 6; ++c) gs[6 * t + c] = s.gsca[6 * (size_t)o + c];
         }
-        if (upload(s, &p_i, en)) return -1; m.enodes = p_i;
-        if (upload(s, &p_i, nle)) return -1; m.nlev_e = p_i;
-        if (upload(s, &p_i, ule)) return -1; m.ulev_e = p_i;
-        double *p;
-        if (upload(s, &p, ar)) return -1; m.elem_area = p;
-        if (upload(s, &p, ec)) return -1; m.elem_cos = p;
-        if (upload(s, &p, gs)) return -1; m.gsca = p;
-        if (rows_d(s.helem, L, s.perm_t, &m.helem)) return -1;
-        if (rows_d(s.uv, 2 * L, s.perm_t, &m.uv)) return -1;
+        if (upload(s, &m.enodes, en) || upload(s, &m.nlev_e, nle)) return -1;
     }
-    // edges
-    std::vector<int> nu12(E), nl12(E);
+    std::vector<int> top_of(E);
     {
         std::vector<int4> ne(E);
-        std::vector<double4> cd(E);
-        std::vector<double2> dxy(E);
         std::vector<int2> ud(E);
         for (int e = 0; e < E; ++e) {
             const int o = s.perm_e[e];
             const int t1 = s.etri[2 * o], t2 = s.etri[2 * o + 1];
             ne[e] = make_int4(s.inv_n[s.edges[2 * o]], s.inv_n[s.edges[2 * o + 1]], s.inv_t[t1], t2 >= 0 ? s.inv_t[t2] : -1);
-            cd[e] = make_double4(s.ecd[4 * (size_t)o], s.ecd[4 * (size_t)o + 1], s.ecd[4 * (size_t)o + 2], s.ecd[4 * (size_t)o + 3]);
-            dxy[e] = make_double2(s.edxdy[2 * (size_t)o], s.edxdy[2 * (size_t)o + 1]);
             if (s.have_muscl) {
                 const int u = s.updn[2 * o], d = s.updn[2 * o + 1];
                 ud[e] = make_int2(u >= 0 ? s.inv_t[u] : -1, d >= 0 ? s.inv_t[d] : -1);
             } else {
                 ud[e] = make_int2(-1, -1);
             }
-            // union level range of the scatters (lib/oce_adv_tra_driver.F90:125-136)
-            const int l1 = s.nlev_e[t1] - 1, u1 = s.ulev_e[t1];
-            int l2 = 0, u2 = 0;
-            if (t2 >= 0) {
-                l2 = s.nlev_e[t2] - 1;
-                u2 = s.ulev_e[t2];
-            }
-            nl12[o] = std::max(l1, l2);
-            nu12[o] = u2 > 0 ? std::min(u1, u2) : u1;
+            // deepest level of the scatters' range (lib/oce_adv_tra_driver.F90:125-136)
+            top_of[o] = std::max(s.nlev_e[t1] - 1, t2 >= 0 ? s.nlev_e[t2] - 1 : 0);
         }
-        int4 *p4;
-        if (upload(s, &p4, ne)) return -1; m.edge_ne = p4;
-        double4 *pd4;
-        if (upload(s, &pd4, cd)) return -1; m.ecd = pd4;
-        double2 *pd2;
-        if (upload(s, &pd2, dxy)) return -1; m.edxdy = pd2;
-        int2 *p2;
-        if (upload(s, &p2, ud)) return -1; m.updn = p2;
+        if (upload(s, &m.edge_ne, ne) || upload(s, &m.updn, ud)) return -1;
     }
-    // CSR node -> incident edges, ascending reference edge number (= the serial scatter order)
+    // CSR node -> incident edges in ascending reference edge number (the serial scatter order)
     {
         std::vector<int> ptr(N + 1, 0);
         for (int o = 0; o < E; ++o) {
@@ -437,48 +470,108 @@ int build_device(State &s) {
         }
         for (int n = 0; n < N; ++n) ptr[n + 1] += ptr[n];
         std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-        std::vector<int2> list(ptr[N]);
-        for (int o = 0; o < E; ++o) {  // ascending reference edge number
-            const int packed = (nu12[o] << 16) | nl12[o];
+        std::vector<int> e2(ptr[N]), top(ptr[N]), oth(ptr[N]);
+        for (int o = 0; o < E; ++o) {
             const int a = s.inv_n[s.edges[2 * o]], b = s.inv_n[s.edges[2 * o + 1]];
-            list[fill[a]++] = make_int2(s.inv_e[o] * 2 + 0, packed);
-            list[fill[b]++] = make_int2(s.inv_e[o] * 2 + 1, packed);
+            int i = fill[a]++;
+            e2[i] = s.inv_e[o] * 2 + 0;
+            top[i] = top_of[o];
+            oth[i] = b;
+            i = fill[b]++;
+            e2[i] = s.inv_e[o] * 2 + 1;
+            top[i] = top_of[o];
+            oth[i] = a;
         }
-        if (upload(s, &p_i, ptr)) return -1; m.ne_ptr = p_i;
-        int2 *p2;
-        if (upload(s, &p2, list)) return -1; m.ne_list = p2;
+        if (upload(s, &m.ne_ptr, ptr) || upload(s, &m.ne_e2, e2) || upload(s, &m.ne_top, top) ||
+            upload(s, &m.ne_other, oth))
+            return -1;
     }
-    if (upload(s, &s.d_perm_n, s.perm_n)) return -1;
-    if (upload(s, &s.d_inv_n, s.inv_n)) return -1;
-
+    if (s.prec == PREC_F64) {
+        if (upload_mesh<double>(s, s.m64, i_nlev, i_nmin, i_nbl)) return -1;
+    } else {
+        copy_int_mesh(s.m32, s.m64);
+        if (upload_mesh<float>(s, s.m32, i_nlev, i_nmin, i_nbl)) return -1;
+    }
+    {
+        const int *p;
+        if (upload(s, &p, s.perm_n)) return -1;
+        s.d_perm_n = const_cast<int *>(p);
+    }
     // work arrays
-    const size_t NLn = (size_t)N * L, ELn = (size_t)E * L, NIn = (size_t)N * nl;
-    if (dalloc(s, &s.flo, ELn) || dalloc(s, &s.adfh, ELn) || dalloc(s, &s.LO, NLn) ||
-        dalloc(s, &s.Rp, NLn) || dalloc(s, &s.Rm, NLn) || dalloc(s, &s.adfv, NIn) ||
-        dalloc(s, &s.part, (size_t)kStatsBlocks))
-        return -1;
+    const size_t cb = compute_bytes(s.prec), sb = storage_bytes(s.prec);
+    const size_t NLn = (size_t)L * s.Np, ELn = (size_t)L * s.Ep, NIn = (size_t)nl * s.Np;
+    for (int i = 0; i < kMaxKT; ++i) {
+        if (dalloc_bytes(s, &s.flo[i], ELn * cb) || dalloc_bytes(s, &s.adfh[i], ELn * cb) ||
+            dalloc_bytes(s, &s.LO[i], NLn * cb) || dalloc_bytes(s, &s.Rp[i], NLn * cb) ||
+            dalloc_bytes(s, &s.Rm[i], NLn * cb) || dalloc_bytes(s, &s.adfv[i], NIn * cb))
+            return -1;
+    }
+    if (dalloc(s, &s.part, (size_t)kStatsBlocks) || dalloc(s, &s.stage, (size_t)N * L)) return -1;
     // tracers
     s.dT.assign(s.ntr, nullptr);
     s.dTAB.assign(s.ntr, nullptr);
     for (int i = 0; i < s.ntr; ++i) {
-        if (rows_d(s.values[i], L, s.perm_n, const_cast<const double **>(&s.dT[i]))) return -1;
+        if (dalloc_bytes(s, &s.dT[i], NLn * sb)) return -1;
+        HIPCHK(hipMemset(s.dT[i], 0, NLn * sb));
     }
     HIPCHK(hipDeviceSynchronize());
     s.on_device = true;
     return 0;
 }
 
+// reference-layout host values -> device tracer storage (through the staging buffer)
+int to_device(State &s, const double *host_or_dev, bool is_device, void *dst) {
+    const int L = s.nl - 1;
+    const double *src = host_or_dev;
+    if (!is_device) {
+        HIPCHK(hipMemcpyAsync(s.stage, host_or_dev, sizeof(double) * (size_t)L * s.N, hipMemcpyHostToDevice, s.stream));
+        src = s.stage;
+    }
+    hipError_t e;
+    switch (s.prec) {
+    case PREC_F64: e = launch_relayout<double>(src, (double *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, 0, s.stream); break;
+    case PREC_F32: e = launch_relayout<float>(src, (float *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, 0, s.stream); break;
+    case PREC_F32_F16: e = launch_relayout<f16_t>(src, (f16_t *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, 0, s.stream); break;
+    default: e = launch_relayout<bf16_t>(src, (bf16_t *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, 0, s.stream); break;
+    }
+    if (e != hipSuccess) return fail("relayout launch failed: %s", hipGetErrorString(e));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+// device tracer storage -> reference layout (host or device destination)
+int from_device(State &s, const void *src, double *out, bool out_is_device) {
+    const int L = s.nl - 1;
+    double *dst = out_is_device ? out : s.stage;
+    hipError_t e;
+    switch (s.prec) {
+    case PREC_F64: e = launch_relayout<double>(nullptr, nullptr, (const double *)src, dst, s.d_perm_n, s.N, L, s.Np, 1, s.stream); break;
+    case PREC_F32: e = launch_relayout<float>(nullptr, nullptr, (const float *)src, dst, s.d_perm_n, s.N, L, s.Np, 1, s.stream); break;
+    case PREC_F32_F16: e = launch_relayout<f16_t>(nullptr, nullptr, (const f16_t *)src, dst, s.d_perm_n, s.N, L, s.Np, 1, s.stream); break;
+    default: e = launch_relayout<bf16_t>(nullptr, nullptr, (const bf16_t *)src, dst, s.d_perm_n, s.N, L, s.Np, 1, s.stream); break;
+    }
+    if (e != hipSuccess) return fail("relayout launch failed: %s", hipGetErrorString(e));
+    if (!out_is_device)
+        HIPCHK(hipMemcpyAsync(out, s.stage, sizeof(double) * (size_t)L * s.N, hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
 int ensure_device(State &s) {
     if (!s.initialized) return fail("tracer not initialized. Call tracer_init_complete first.");
     if (s.on_device) return 0;
-    return build_device(s);
+    if (build_device(s)) return -1;
+    for (int i = 0; i < s.ntr; ++i)
+        if (to_device(s, s.values[i].data(), false, s.dT[i])) return -1;
+    return 0;
 }
 
 int ensure_dropin_buffers(State &s) {
-    const size_t NLn = (size_t)s.N * (s.nl - 1), ELn = (size_t)s.E * (s.nl - 1), NIn = (size_t)s.N * s.nl;
+    const size_t NLn = (size_t)(s.nl - 1) * s.Np, ELn = (size_t)(s.nl - 1) * s.Ep, NIn = (size_t)s.nl * s.Np;
+    const size_t cb = compute_bytes(s.prec);
     if (!s.acc_h) {
-        if (dalloc(s, &s.acc_h, NLn) || dalloc(s, &s.acc_v, NLn) || dalloc(s, &s.dbg_h, ELn) ||
-            dalloc(s, &s.dbg_v, NIn))
+        if (dalloc(s, &s.acc_h, NLn) || dalloc(s, &s.acc_v, NLn) || dalloc_bytes(s, &s.dbg_h, ELn * cb) ||
+            dalloc_bytes(s, &s.dbg_v, NIn * cb))
             return -1;
         HIPCHK(hipMemsetAsync(s.acc_h, 0, NLn * sizeof(double), s.stream));
         HIPCHK(hipMemsetAsync(s.acc_v, 0, NLn * sizeof(double), s.stream));
@@ -486,43 +579,69 @@ int ensure_dropin_buffers(State &s) {
     return 0;
 }
 
-StepArgs step_args(State &s, int i, double dt) {
-    const SchemeSel &sc = s.schemes[i];
-    StepArgs a{};
-    a.hor = sc.hor;
-    a.ver = sc.ver;
-    a.fct = sc.fct;
-    a.ph = sc.ph;
-    a.pv = sc.pv;
-    a.dt = dt;
-    a.T = s.dT[i];
-    a.TAB = s.dT[i];
-    a.flo = s.flo;
-    a.adfh = s.adfh;
-    a.LO = s.LO;
-    a.Rp = s.Rp;
-    a.Rm = s.Rm;
-    a.adfv = s.adfv;
-    return a;
+template <class S_, class C_>
+StepPtrs<S_, C_> make_ptrs(State &s, const int *ids, int kt, bool dropin) {
+    StepPtrs<S_, C_> p{};
+    for (int t = 0; t < kt; ++t) {
+        const int i = ids[t];
+        p.T[t] = static_cast<S_ *>(s.dT[i]);
+        p.TAB[t] = static_cast<const S_ *>(s.dTAB[i] ? s.dTAB[i] : s.dT[i]);
+        p.flo[t] = static_cast<C_ *>(s.flo[t]);
+        p.adfh[t] = static_cast<C_ *>(s.adfh[t]);
+        p.LO[t] = static_cast<C_ *>(s.LO[t]);
+        p.Rp[t] = static_cast<C_ *>(s.Rp[t]);
+        p.Rm[t] = static_cast<C_ *>(s.Rm[t]);
+        p.adfv[t] = static_cast<C_ *>(s.adfv[t]);
+        if (dropin) {
+            p.acc_h[t] = s.acc_h;
+            p.acc_v[t] = s.acc_v;
+            p.dbg_h[t] = static_cast<C_ *>(s.dbg_h);
+            p.dbg_v[t] = static_cast<C_ *>(s.dbg_v);
+        }
+    }
+    return p;
+}
+
+hipError_t launch_group(State &s, const int *ids, int kt, double dt, bool dropin, hipEvent_t *ev) {
+    const SchemeSel &sc = s.schemes[ids[0]];
+    SchemeArgs a{sc.hor, sc.ver, sc.fct, sc.ph, sc.pv, dt};
+    switch (s.prec) {
+    case PREC_F64: return launch_step<double, double, double>(s.m64, make_ptrs<double, double>(s, ids, kt, dropin), kt, a, s.cfg, s.stream, ev);
+    case PREC_F32: return launch_step<float, float, float>(s.m32, make_ptrs<float, float>(s, ids, kt, dropin), kt, a, s.cfg, s.stream, ev);
+    case PREC_F32_F16: return launch_step<f16_t, float, float>(s.m32, make_ptrs<f16_t, float>(s, ids, kt, dropin), kt, a, s.cfg, s.stream, ev);
+    default: return launch_step<bf16_t, float, float>(s.m32, make_ptrs<bf16_t, float>(s, ids, kt, dropin), kt, a, s.cfg, s.stream, ev);
+    }
+}
+
+// Groups of up to kt_max consecutive tracers with identical schemes, one launch sequence each.
+std::vector<std::vector<int>> tracer_groups(const State &s) {
+    std::vector<std::vector<int>> gr;
+    for (int i = 0; i < s.ntr; ++i) {
+        if (!gr.empty() && (int)gr.back().size() < s.kt_max && s.schemes[gr.back()[0]] == s.schemes[i])
+            gr.back().push_back(i);
+        else
+            gr.push_back({i});
+    }
+    return gr;
 }
 
 int advect(State &s, int nsteps, double dt) {
     if (ensure_device(s)) return -1;
     if (s.ntr == 0) return fail("no tracers allocated (tracer_allocate_tracers)");
     for (int i = 0; i < s.ntr; ++i) s.dTAB[i] = nullptr;  // valuesAB := values
+    const auto groups = tracer_groups(s);
     std::vector<hipEvent_t> evs;
     if (s.kernel_timing) {
-        evs.resize((size_t)nsteps * s.ntr * 5);
+        evs.resize((size_t)nsteps * groups.size() * 5);
         for (auto &e : evs) HIPCHK(hipEventCreate(&e));
     }
     HIPCHK(hipEventRecord(s.ev0, s.stream));
     for (int st = 0; st < nsteps; ++st)
-        for (int i = 0; i < s.ntr; ++i) {
-            StepArgs a = step_args(s, i, dt);
-            if (s.kernel_timing) a.ev = &evs[((size_t)st * s.ntr + i) * 5];
-            hipError_t e = launch_fct_step(s.md, a, s.stream);
+        for (size_t gi = 0; gi < groups.size(); ++gi) {
+            hipEvent_t *ev = s.kernel_timing ? &evs[((size_t)st * groups.size() + gi) * 5] : nullptr;
+            hipError_t e = launch_group(s, groups[gi].data(), (int)groups[gi].size(), dt, false, ev);
             if (e != hipSuccess) return fail("kernel launch failed: %s", hipGetErrorString(e));
-            s.last_fct = a.fct;
+            s.last_fct = s.schemes[groups[gi][0]].fct;
         }
     HIPCHK(hipEventRecord(s.ev1, s.stream));
     HIPCHK(hipStreamSynchronize(s.stream));
@@ -542,21 +661,31 @@ int advect(State &s, int nsteps, double dt) {
     return 0;
 }
 
-// Copies a device (rows, ld) array in device numbering to host reference numbering.
-int download_rows(State &s, const double *d, size_t rows, int ld, const std::vector<int> &perm,
-                  double *out) {
-    std::vector<double> h(rows * (size_t)ld);
+// Device work array (level-major, device order, compute precision) -> reference layout host.
+int download_work(State &s, const void *d, int ld, size_t pad, const std::vector<int> &perm, double *out) {
+    const size_t cb = compute_bytes(s.prec);
+    std::vector<char> h((size_t)ld * pad * cb);
     HIPCHK(hipStreamSynchronize(s.stream));
-    HIPCHK(hipMemcpy(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (size_t r = 0; r < rows; ++r)
-        std::memcpy(out + (size_t)perm[r] * ld, &h[r * ld], sizeof(double) * ld);
+    HIPCHK(hipMemcpy(h.data(), d, h.size(), hipMemcpyDeviceToHost));
+    for (size_t r = 0; r < perm.size(); ++r)
+        for (int k = 0; k < ld; ++k) {
+            const size_t i = (size_t)k * pad + r;
+            out[(size_t)perm[r] * ld + k] = cb == 8 ? ((const double *)h.data())[i] : (double)((const float *)h.data())[i];
+        }
     return 0;
 }
 
 int stats(State &s, int id, int all_levels, double *mn, double *mx, double *sm, double *vs) {
     if (ensure_device(s)) return -1;
     if (id < 1 || id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", id, s.ntr);
-    hipError_t e = launch_stats(s.md, s.dT[id - 1], all_levels, s.part, kStatsBlocks, s.stream);
+    const void *T = s.dT[id - 1];
+    hipError_t e;
+    switch (s.prec) {
+    case PREC_F64: e = launch_stats<double, double>(s.m64, (const double *)T, all_levels, s.part, kStatsBlocks, s.stream); break;
+    case PREC_F32: e = launch_stats<float, float>(s.m32, (const float *)T, all_levels, s.part, kStatsBlocks, s.stream); break;
+    case PREC_F32_F16: e = launch_stats<f16_t, float>(s.m32, (const f16_t *)T, all_levels, s.part, kStatsBlocks, s.stream); break;
+    default: e = launch_stats<bf16_t, float>(s.m32, (const bf16_t *)T, all_levels, s.part, kStatsBlocks, s.stream); break;
+    }
     if (e != hipSuccess) return fail("stats launch failed: %s", hipGetErrorString(e));
     std::vector<double4> h(kStatsBlocks);
     HIPCHK(hipStreamSynchronize(s.stream));
@@ -640,11 +769,8 @@ int tracer_get_values_by_id(int tracer_id, double *values, int nz, int nn) {
     if (!s.initialized) return fail("tracer not initialized.");
     if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
     if (nz != s.nl - 1 || nn != s.N) return fail("tracer_get_values: expected (%d, %d), got (%d, %d)", s.nl - 1, s.N, nz, nn);
-    if (!s.on_device) {
-        std::memcpy(values, s.values[tracer_id - 1].data(), sizeof(double) * (size_t)nz * nn);
-        return 0;
-    }
-    return download_rows(s, s.dT[tracer_id - 1], s.N, s.nl - 1, s.perm_n, values);
+    if (ensure_device(s)) return -1;
+    return from_device(s, s.dT[tracer_id - 1], values, false);
 }
 
 int tracer_get_values(double *values, int nz, int nn) { return tracer_get_values_by_id(1, values, nz, nn); }
@@ -679,10 +805,15 @@ int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D) {
     release_device(s);
     const hipStream_t keep = s.stream;
     hipEvent_t k0 = s.ev0, k1 = s.ev1;
+    const int prec = s.prec, ktm = s.kt_max;
+    const LaunchCfg cfg = s.cfg;
     s = State();
+    s.cfg = cfg;
     s.stream = keep;
     s.ev0 = k0;
     s.ev1 = k1;
+    s.prec = prec;
+    s.kt_max = ktm;
     s.nl = nl;
     s.N = nod2D;
     s.E = edge2D;
@@ -750,10 +881,7 @@ int tracer_set_values(int tracer_id, const double *values, int nl, int nod2D) {
     std::vector<double> &v = s.values[tracer_id - 1];
     v.assign(values, values + (size_t)(nl - 1) * nod2D);
     if (s.on_device) {
-        const int L = nl - 1;
-        std::vector<double> h((size_t)L * nod2D);
-        for (int r = 0; r < nod2D; ++r) std::memcpy(&h[(size_t)r * L], &v[(size_t)s.perm_n[r] * L], sizeof(double) * L);
-        HIPCHK(hipMemcpy(s.dT[tracer_id - 1], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        if (to_device(s, v.data(), false, s.dT[tracer_id - 1])) return -1;
         s.dTAB[tracer_id - 1] = nullptr;
     }
     return 0;
@@ -764,12 +892,9 @@ int tracer_set_values_ab(int tracer_id, const double *values, int nl, int nod2D)
     if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
     if (check_dims(s, nl, nod2D, "tracer_set_values_ab")) return -1;
     if (ensure_device(s)) return -1;
-    const int L = nl - 1;
-    std::vector<double> h((size_t)L * nod2D);
-    for (int r = 0; r < nod2D; ++r) std::memcpy(&h[(size_t)r * L], &values[(size_t)s.perm_n[r] * L], sizeof(double) * L);
-    double *p = s.dTAB[tracer_id - 1];
-    if (!p && dalloc(s, &p, h.size())) return -1;
-    HIPCHK(hipMemcpy(p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    void *p = s.dTAB[tracer_id - 1];
+    if (!p && dalloc_bytes(s, &p, (size_t)(nl - 1) * s.Np * storage_bytes(s.prec))) return -1;
+    if (to_device(s, values, false, p)) return -1;
     s.dTAB[tracer_id - 1] = p;
     return 0;
 }
@@ -828,7 +953,7 @@ int tracer_init_complete(void) {
     }
     if (validate(s)) return -1;
     s.initialized = true;
-    return build_device(s);
+    return ensure_device(s);
 }
 
 int tracer_load_mesh_partition(const char *restart_dir) {
@@ -995,10 +1120,52 @@ int tracer_set_scheme(int tracer_id, const char *hor, const char *ver, const cha
     return 0;
 }
 
+int tracer_set_precision(int mode) {
+    State &s = S();
+    if (mode < PREC_F64 || mode > PREC_F32_BF16) return fail("tracer_set_precision: unknown mode %d", mode);
+    if (s.prec == mode) return 0;
+    if (s.on_device) {
+        // keep the current values: read them back at the old precision, rebuild at the new one
+        for (int i = 0; i < s.ntr; ++i)
+            if (from_device(s, s.dT[i], s.values[i].data(), false)) return -1;
+        release_device(s);
+    }
+    s.prec = mode;
+    if (s.initialized) return ensure_device(s);
+    return 0;
+}
+
+int tracer_set_tracers_per_pass(int kt) {
+    State &s = S();
+    if (kt < 1 || kt > kMaxKT) return fail("tracer_set_tracers_per_pass: %d outside [1, %d]", kt, kMaxKT);
+    s.kt_max = kt;
+    return 0;
+}
+
+int tracer_set_launch_config(int lc_edge, int lc_lo, int lc_bounds, int lc_final) {
+    if (lc_edge < 1 || lc_lo < 1 || lc_bounds < 1 || lc_final < 1)
+        return fail("tracer_set_launch_config: levels per thread must be >= 1");
+    State &s = S();
+    s.cfg.lc_edge = lc_edge;
+    s.cfg.lc_lo = lc_lo;
+    s.cfg.lc_bounds = lc_bounds;
+    s.cfg.lc_final = lc_final;
+    return 0;
+}
+
+int tracer_get_launch_config(int *cfg4) {
+    const State &s = S();
+    cfg4[0] = s.cfg.lc_edge;
+    cfg4[1] = s.cfg.lc_lo;
+    cfg4[2] = s.cfg.lc_bounds;
+    cfg4[3] = s.cfg.lc_final;
+    return 0;
+}
+
 int tracer_zero_tendencies(void) {
     State &s = S();
     if (ensure_device(s) || ensure_dropin_buffers(s)) return -1;
-    const size_t NLn = (size_t)s.N * (s.nl - 1);
+    const size_t NLn = (size_t)(s.nl - 1) * s.Np;
     HIPCHK(hipMemsetAsync(s.acc_h, 0, NLn * sizeof(double), s.stream));
     HIPCHK(hipMemsetAsync(s.acc_v, 0, NLn * sizeof(double), s.stream));
     HIPCHK(hipStreamSynchronize(s.stream));
@@ -1010,23 +1177,18 @@ int tracer_do_oce_adv_tra(double dt, int tracer_id) {
     if (!s.initialized) return fail("tracer not initialized.");
     if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
     if (ensure_device(s) || ensure_dropin_buffers(s)) return -1;
-    const int i = tracer_id - 1;
-    const size_t L = s.nl - 1;
-    // the reference zeroes its flux / low-order work arrays at every call
-    HIPCHK(hipMemsetAsync(s.dbg_h, 0, (size_t)s.E * L * sizeof(double), s.stream));
-    HIPCHK(hipMemsetAsync(s.dbg_v, 0, (size_t)s.N * s.nl * sizeof(double), s.stream));
-    HIPCHK(hipMemsetAsync(s.adfh, 0, (size_t)s.E * L * sizeof(double), s.stream));
-    HIPCHK(hipMemsetAsync(s.adfv, 0, (size_t)s.N * s.nl * sizeof(double), s.stream));
-    HIPCHK(hipMemsetAsync(s.LO, 0, (size_t)s.N * L * sizeof(double), s.stream));
-    StepArgs a = step_args(s, i, dt);
-    if (s.dTAB[i]) a.TAB = s.dTAB[i];
-    a.acc_h = s.acc_h;
-    a.acc_v = s.acc_v;
-    a.dbg_h = s.dbg_h;
-    a.dbg_v = s.dbg_v;
-    hipError_t e = launch_fct_step(s.md, a, s.stream);
+    const size_t cb = compute_bytes(s.prec);
+    const size_t NLn = (size_t)(s.nl - 1) * s.Np, ELn = (size_t)(s.nl - 1) * s.Ep, NIn = (size_t)s.nl * s.Np;
+    // the reference zeroes its flux and low-order work arrays at every call
+    HIPCHK(hipMemsetAsync(s.dbg_h, 0, ELn * cb, s.stream));
+    HIPCHK(hipMemsetAsync(s.dbg_v, 0, NIn * cb, s.stream));
+    HIPCHK(hipMemsetAsync(s.adfh[0], 0, ELn * cb, s.stream));
+    HIPCHK(hipMemsetAsync(s.adfv[0], 0, NIn * cb, s.stream));
+    HIPCHK(hipMemsetAsync(s.LO[0], 0, NLn * cb, s.stream));
+    const int id = tracer_id - 1;
+    hipError_t e = launch_group(s, &id, 1, dt, true, nullptr);
     if (e != hipSuccess) return fail("kernel launch failed: %s", hipGetErrorString(e));
-    s.last_fct = a.fct;
+    s.last_fct = s.schemes[id].fct;
     HIPCHK(hipStreamSynchronize(s.stream));
     return 0;
 }
@@ -1035,9 +1197,13 @@ int tracer_get_tendencies(double *dttf_h, double *dttf_v, int nz, int nn) {
     State &s = S();
     if (!s.on_device || !s.acc_h) return fail("tracer_get_tendencies: no tendencies computed yet");
     if (nz != s.nl - 1 || nn != s.N) return fail("tracer_get_tendencies: expected (%d, %d)", s.nl - 1, s.N);
-    if (dttf_h && download_rows(s, s.acc_h, s.N, nz, s.perm_n, dttf_h)) return -1;
-    if (dttf_v && download_rows(s, s.acc_v, s.N, nz, s.perm_n, dttf_v)) return -1;
-    return 0;
+    const int prec = s.prec;
+    s.prec = PREC_F64;  // accumulators are double
+    int rc = 0;
+    if (dttf_h) rc |= download_work(s, s.acc_h, nz, s.Np, s.perm_n, dttf_h);
+    if (dttf_v) rc |= download_work(s, s.acc_v, nz, s.Np, s.perm_n, dttf_v);
+    s.prec = prec;
+    return rc ? -1 : 0;
 }
 
 int tracer_get_work(const char *name, double *out, size_t count) {
@@ -1046,15 +1212,15 @@ int tracer_get_work(const char *name, double *out, size_t count) {
     const size_t L = s.nl - 1;
     if (!strcmp(name, "fct_LO")) {
         if (count != L * s.N) return fail("tracer_get_work(fct_LO): count must be %zu", L * s.N);
-        return download_rows(s, s.LO, s.N, (int)L, s.perm_n, out);
+        return download_work(s, s.LO[0], (int)L, s.Np, s.perm_n, out);
     }
     if (!strcmp(name, "adv_flux_hor")) {
         if (count != L * s.E) return fail("tracer_get_work(adv_flux_hor): count must be %zu", L * s.E);
-        return download_rows(s, s.last_fct ? s.dbg_h : s.adfh, s.E, (int)L, s.perm_e, out);
+        return download_work(s, s.last_fct ? s.dbg_h : s.adfh[0], (int)L, s.Ep, s.perm_e, out);
     }
     if (!strcmp(name, "adv_flux_ver")) {
         if (count != (size_t)s.nl * s.N) return fail("tracer_get_work(adv_flux_ver): count must be %zu", (size_t)s.nl * s.N);
-        return download_rows(s, s.last_fct ? s.dbg_v : s.adfv, s.N, s.nl, s.perm_n, out);
+        return download_work(s, s.last_fct ? s.dbg_v : s.adfv[0], s.nl, s.Np, s.perm_n, out);
     }
     return fail("tracer_get_work: unknown array %s", name);
 }
@@ -1064,10 +1230,8 @@ int tracer_set_values_device(int tracer_id, const void *values, int nl, int nod2
     if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
     if (check_dims(s, nl, nod2D, "tracer_set_values_device")) return -1;
     if (ensure_device(s)) return -1;
-    hipError_t e = launch_gather_rows(static_cast<const double *>(values), s.dT[tracer_id - 1], s.d_perm_n, s.N, nl - 1, s.stream);
-    if (e != hipSuccess) return fail("gather launch failed: %s", hipGetErrorString(e));
+    if (to_device(s, static_cast<const double *>(values), true, s.dT[tracer_id - 1])) return -1;
     s.dTAB[tracer_id - 1] = nullptr;
-    HIPCHK(hipStreamSynchronize(s.stream));
     return 0;
 }
 
@@ -1076,10 +1240,7 @@ int tracer_get_values_device(int tracer_id, void *values, int nz, int nn) {
     if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
     if (nz != s.nl - 1 || nn != s.N) return fail("tracer_get_values_device: expected (%d, %d)", s.nl - 1, s.N);
     if (ensure_device(s)) return -1;
-    hipError_t e = launch_gather_rows(s.dT[tracer_id - 1], static_cast<double *>(values), s.d_inv_n, s.N, nz, s.stream);
-    if (e != hipSuccess) return fail("gather launch failed: %s", hipGetErrorString(e));
-    HIPCHK(hipStreamSynchronize(s.stream));
-    return 0;
+    return from_device(s, s.dT[tracer_id - 1], static_cast<double *>(values), true);
 }
 
 int tracer_set_kernel_timing(int enable) {
@@ -1104,6 +1265,6 @@ int tracer_last_elapsed_ms(double *ms) {
 
 const char *tracer_last_error(void) { return g_err.c_str(); }
 
-const char *tracer_version(void) { return "fesomtra 0.1 (gfx950)"; }
+const char *tracer_version(void) { return "fesomtra 0.2 (gfx950)"; }
 
 }  // extern "C"

@@ -1,0 +1,210 @@
+// Per-column formulas of the tracer advection step (device functions).
+//
+// Every expression keeps the operation order of the reference routine it cites so that the
+// double instantiation reproduces the serial reference bit for bit.  Levels are 1-based (nz),
+// interfaces 1..nlevels_nod2D; ulevels == 1.
+#pragma once
+
+#include "fesomtra_types.hpp"
+
+namespace fesomtra {
+
+__device__ __forceinline__ int xcd_block() {
+    const int per = gridDim.x / kXcds;
+    return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+// Column owned by this thread; consecutive columns of one XCD stay contiguous in the
+// space-filling-curve order (workgroups are dispatched round-robin over the XCDs).
+__device__ __forceinline__ int column_index() { return xcd_block() * blockDim.x + threadIdx.x; }
+
+// First-order upwind edge flux (lib/oce_adv_tra_hor.F90:178-181).
+template <typename C>
+__device__ __forceinline__ C upw1_edge(C t1, C t2, C v) {
+    return -C(0.5) * (t1 * (v + fabs(v)) + t2 * (v - fabs(v)));
+}
+
+// tr_xy(comp, nz, el) = sum over the element's nodes of gradient_sca * T, in node order.
+template <typename C, typename S>
+__device__ __forceinline__ C tr_xy(const C g[3], const S *__restrict__ t, size_t off, int a, int b, int c) {
+    return g[0] * ldc<C>(t, off + a) + g[1] * ldc<C>(t, off + b) + g[2] * ldc<C>(t, off + c);
+}
+
+// Area-weighted mean element gradient around node n at level nz
+// (lib/oce_muscl_adv.F90:394-409), accumulated in nod_in_elem2D order.
+template <typename C, typename S, typename M>
+__device__ __forceinline__ void node_mean_grad(const MeshCol<M> &m, const S *__restrict__ t, int n,
+                                               int nz, C &gx, C &gy) {
+    C tvol = 0, tx = 0, ty = 0;
+    const size_t off = (size_t)(nz - 1) * m.Np;
+    const int cnt = m.nie_num[n];
+    for (int i = 0; i < cnt; ++i) {
+        const int el = m.nie[(size_t)n * m.max_adj + i];
+        if (m.nlev_e[el] - 1 < nz) continue;
+        const C ar = (C)m.elem_area[el];
+        const int *en = m.enodes + (size_t)el * 3;
+        C g0[3], g1[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            g0[c] = (C)m.gsca[(size_t)el * 6 + c];
+            g1[c] = (C)m.gsca[(size_t)el * 6 + 3 + c];
+        }
+        tvol = tvol + ar;
+        tx = tx + tr_xy<C>(g0, t, off, en[0], en[1], en[2]) * ar;
+        ty = ty + tr_xy<C>(g1, t, off, en[0], en[1], en[2]) * ar;
+    }
+    gx = tx / tvol;
+    gy = ty / tvol;
+}
+
+// MUSCL / MFCT high-order flux minus fin (lib/oce_adv_tra_hor.F90:368-382, 748-790);
+// c1, c2 are MUSCL's c_lo switches.
+template <int HOR, typename C>
+__device__ __forceinline__ C ho_edge(C t1, C t2, const C gr[4], C v, C dx, C dy, C a, C num_ord,
+                                     C c1, C c2, C fin) {
+    const C r = (C)kREarth;
+    C Tmean1, Tmean2;
+    if (HOR == HOR_MUSCL) {
+        Tmean2 = t2 - (C(2) * (t2 - t1) + dx * a * gr[1] + dy * r * gr[3]) / C(6) * c2;
+        Tmean1 = t1 + (C(2) * (t2 - t1) + dx * a * gr[0] + dy * r * gr[2]) / C(6) * c1;
+    } else {
+        Tmean2 = t2 - (C(2) * (t2 - t1) + dx * a * gr[1] + dy * r * gr[3]) / C(6);
+        Tmean1 = t1 + (C(2) * (t2 - t1) + dx * a * gr[0] + dy * r * gr[2]) / C(6);
+    }
+    const C cHO = (v + fabs(v)) * Tmean1 + (v - fabs(v)) * Tmean2;
+    return -C(0.5) * (C(1) - num_ord) * cHO - v * num_ord * C(0.5) * (Tmean1 + Tmean2) - fin;
+}
+
+// ------------------------------------------------------------------------------------------
+// Vertical fluxes (lib/oce_adv_tra_ver.F90).  A Col reads one node column straight from memory
+// (used by the less common CDIFF / PPM schemes); QR4C and UPW1 use register windows instead.
+template <typename C, typename S, typename M>
+struct Col {
+    const S *t;
+    const M *w, *ar, *hn, *ho;
+    size_t Np;
+    int n, nzmin, nzmax;
+    __device__ __forceinline__ C T(int nz) const { return ldc<C>(t, (size_t)(nz - 1) * Np + n); }
+    __device__ __forceinline__ C W(int j) const { return (C)w[(size_t)(j - 1) * Np + n]; }
+    __device__ __forceinline__ C A(int j) const { return (C)ar[(size_t)(j - 1) * Np + n]; }
+    __device__ __forceinline__ C HN(int nz) const { return (C)hn[(size_t)(nz - 1) * Np + n]; }
+    __device__ __forceinline__ C HO(int nz) const { return (C)ho[(size_t)(nz - 1) * Np + n]; }
+};
+
+// adv_tra_ver_cdiff (:687-708) at interface j, minus fin; the bottom interface keeps fin.
+template <typename C, typename S, typename M>
+__device__ __forceinline__ C ver_cdiff(const Col<C, S, M> &c, int j, C fin) {
+    if (j == c.nzmax) return fin;
+    if (j == c.nzmin) return -c.W(j) * c.T(j) * c.A(j) - fin;
+    const C tv = C(0.5) * (c.T(j - 1) + c.T(j));
+    return -tv * c.W(j) * c.A(j) - fin;
+}
+
+// PPM interface value tv(j) after all of the reference's assignments (:510-594).
+template <typename C, typename S, typename M>
+__device__ __forceinline__ C ppm_tv(const Col<C, S, M> &c, int j) {
+    const int nzmin = c.nzmin, nzmax = c.nzmax;
+    if (j >= nzmin + 2 && j <= nzmax - 2) {
+        const int nz = j - 1;
+        const C dzjm1 = c.HN(nz - 1), dzj = c.HN(nz), dzjp1 = c.HN(nz + 1), dzjp2 = c.HN(nz + 2);
+        const C tm1 = c.T(nz - 1), t0 = c.T(nz), tp1 = c.T(nz + 1), tp2 = c.T(nz + 2);
+        C deltaj = dzj / (dzjm1 + dzj + dzjp1) *
+                   ((C(2) * dzjm1 + dzj) / (dzjp1 + dzj) * (tp1 - t0) +
+                    (dzj + C(2) * dzjp1) / (dzjm1 + dzj) * (t0 - tm1));
+        C deltajp1 = dzjp1 / (dzj + dzjp1 + dzjp2) *
+                     ((C(2) * dzj + dzjp1) / (dzjp2 + dzjp1) * (tp2 - tp1) +
+                      (dzjp1 + C(2) * dzjp2) / (dzj + dzjp1) * (tp1 - t0));
+        if ((tp1 - t0) * (t0 - tm1) > C(0))
+            deltaj = cmin(cmin(C(fabs(deltaj)), C(2) * fabs(tp1 - t0)), C(2) * fabs(t0 - tm1)) * csign1(deltaj);
+        else
+            deltaj = C(0);
+        if ((tp2 - tp1) * (tp1 - t0) > C(0))
+            deltajp1 = cmin(cmin(C(fabs(deltajp1)), C(2) * fabs(tp2 - tp1)), C(2) * fabs(tp1 - t0)) * csign1(deltajp1);
+        else
+            deltajp1 = C(0);
+        return t0 + dzj / (dzj + dzjp1) * (tp1 - t0) +
+               C(1) / (dzjm1 + dzj + dzjp1 + dzjp2) *
+                   ((C(2) * dzjp1 * dzj) / (dzj + dzjp1) *
+                        ((dzjm1 + dzj) / (C(2) * dzj + dzjp1) - (dzjp2 + dzjp1) / (C(2) * dzjp1 + dzj)) *
+                        (tp1 - t0) -
+                    dzj * (dzjm1 + dzj) / (C(2) * dzj + dzjp1) * deltajp1 +
+                    dzjp1 * (dzjp1 + dzjp2) / (dzj + C(2) * dzjp1) * deltaj);
+    }
+    if (j == nzmax) return c.T(nzmax - 1);
+    if (j == nzmax - 1) {
+        const C s = csign1(c.W(nzmax - 1));
+        return -c.T(nzmax - 2) * cmin(s, C(0)) + c.T(nzmax - 1) * cmax(s, C(0));
+    }
+    if (j == nzmin + 1) return C(0.5) * (c.T(nzmin) + c.T(nzmin + 1));
+    return c.T(nzmin);
+}
+
+// Limited parabola of layer nz (:601-617).
+template <typename C, typename S, typename M>
+__device__ __forceinline__ void ppm_layer(const Col<C, S, M> &c, int nz, C &aL, C &aR, C &aj) {
+    const C t = c.T(nz);
+    aL = ppm_tv(c, nz);
+    aR = ppm_tv(c, nz + 1);
+    if ((aR - t) * (t - aL) <= C(0)) {
+        aL = t;
+        aR = t;
+    }
+    if ((aR - aL) * (t - C(0.5) * (aL + aR)) > (aR - aL) * (aR - aL) / C(6)) aL = C(3) * t - C(2) * aR;
+    if ((aR - aL) * (t - C(0.5) * (aR + aL)) < -(aR - aL) * (aR - aL) / C(6)) aR = C(3) * t - C(2) * aL;
+    aj = C(6) * (t - C(0.5) * (aL + aR));
+}
+
+// adv_tra_vert_ppm (:596-639) at interface j, minus fin.
+template <typename C, typename S, typename M>
+__device__ __forceinline__ C ver_ppm(const Col<C, S, M> &c, int j, C dt, C fin) {
+    if (j == c.nzmin) return -ppm_tv(c, c.nzmin) * c.W(c.nzmin) * c.A(c.nzmin) - fin;
+    if (j == c.nzmax) return C(0) - fin;
+    const C wj = c.W(j);
+    C tvert = C(0), aL, aR, aj;
+    if (wj > C(0)) {  // "W(nz) > 0" branch of layer j
+        ppm_layer(c, j, aL, aR, aj);
+        const C x = cmin(wj * dt / c.HO(j), C(1));
+        tvert = (-aL - C(0.5) * x * (aR - aL + (C(1) - C(2) / C(3) * x) * aj));
+        tvert = tvert * c.A(j) * wj;
+    } else if (wj < C(0)) {  // "W(nz+1) < 0" branch of layer j-1
+        ppm_layer(c, j - 1, aL, aR, aj);
+        const C x = cmin(-wj * dt / c.HO(j - 1), C(1));
+        tvert = (-aR + C(0.5) * x * (aR - aL - (C(1) - C(2) / C(3) * x) * aj));
+        tvert = tvert * c.A(j) * wj;
+    }
+    return tvert - fin;
+}
+
+// QR4C (:394-436) at interface j from the register window tw[i] = T(j-2+i), i = 0..3 and
+// z[i] = Z_3d_n(j-2+i), minus fin.  A 2-layer column updates its middle interface twice, as the
+// reference's statement sequence does.
+template <typename C>
+__device__ __forceinline__ C ver_qr4c_w(int j, int nzmax, const C tw[4], const C z[4], C zb, C w,
+                                        C ar, C num_ord, C fin) {
+    if (j == 1) return -tw[2] * w * ar - fin;
+    if (j == nzmax) return C(0) - fin;
+    const bool second = (j == 2), penult = (j == nzmax - 1);
+    if (second || penult) {
+        const C cv = -C(0.5) * (tw[1] + tw[2]) * w * ar;
+        C f = fin;
+        if (second) f = cv - f;
+        if (penult) f = cv - f;
+        return f;
+    }
+    const C qc = (tw[1] - tw[2]) / (z[1] - z[2]);
+    const C qu = (tw[2] - tw[3]) / (z[2] - z[3]);
+    const C qd = (tw[0] - tw[1]) / (z[0] - z[1]);
+    const C Tmean1 = tw[2] + (C(2) * qc + qu) * (zb - z[2]) / C(3);
+    const C Tmean2 = tw[1] + (C(2) * qc + qd) * (zb - z[1]) / C(3);
+    const C Tmean = (w + fabs(w)) * Tmean1 + (w - fabs(w)) * Tmean2;
+    return (-C(0.5) * (C(1) - num_ord) * Tmean - num_ord * (C(0.5) * (Tmean1 + Tmean2)) * w) * ar - fin;
+}
+
+// adv_tra_ver_upw1 (:300-327) at interface j with T(j-1) = tm, T(j) = tj, minus fin.
+template <typename C>
+__device__ __forceinline__ C ver_upw1_w(int j, int nzmax, C tm, C tj, C w, C ar, C fin) {
+    if (j == 1) return -w * tj * ar - fin;
+    if (j == nzmax) return C(0) - fin;
+    return -C(0.5) * (tj * (w + fabs(w)) + tm * (w - fabs(w))) * ar - fin;
+}
+
+}  // namespace fesomtra

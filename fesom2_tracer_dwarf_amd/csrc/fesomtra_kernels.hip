@@ -1,304 +1,545 @@
-// gfx950 kernels of the FP64 tracer advection step and their launchers.
+// gfx950 kernels of the tracer advection step and their launchers.
 //
-// One tracer step with the FCT limiter is four kernels (do_oce_adv_tra,
-// lib/oce_adv_tra_driver.F90:23-294):
-//   k_edge    per (edge, level): low-order upwind flux and high-order minus low-order flux,
-//             MUSCL/MFCT gradients reconstructed on the fly (no edge_up_dn_grad array)
-//   k_node_lo per (node, level): deterministic CSR gather of the upwind fluxes, vertical upwind
-//             flux, low-order solution fct_LO, vertical antidiffusive flux (QR4C/PPM/CDIFF/UPW1)
-//   k_bounds  per column tile:   2-ring cluster bounds, vertical widening in LDS, P+/P- gathers,
-//             limiter ratios R+/R- (lib/oce_adv_tra_fct.F90:96-374)
-//   k_final   per (node, level): limiting of both flux families with R at both ends,
-//             flux divergence, tendencies and the tracer update (in place)
-// Without FCT: k_edge, k_vert (vertical high-order flux) and k_final.
-//
-// No atomics: every node sums its incident edges in the reference's edge order, so the result
-// is deterministic and equal to the serial reference.
-#include "fesomtra_device.hpp"
+// One step of KT tracers (same scheme) with the FCT limiter is four column kernels
+// (do_oce_adv_tra, lib/oce_adv_tra_driver.F90:23-294):
+//   k_edge    one thread per edge: low-order upwind flux and high-order minus low-order flux
+//             for every level of the edge; MUSCL/MFCT gradients reconstructed on the fly from
+//             the up/down-wind triangles (no edge_up_dn_grad array)
+//   k_lo      one thread per node: deterministic gather of the edge fluxes, vertical upwind
+//             flux, low-order solution fct_LO, vertical antidiffusive flux (QR4C/PPM/...)
+//   k_bounds  one thread per node: cluster bounds over the node's 1-ring, vertical widening,
+//             P+/P- gathers and the limiter ratios R+/R- (lib/oce_adv_tra_fct.F90:96-374)
+//   k_final   one thread per node: limiting with R at both ends, flux divergence, tendencies
+//             and the tracer update in place (lib/oce_adv_tra_fct.F90:389-471,
+//             lib/oce_adv_tra_driver.F90:298-450, src/fesom_analytic.F90:419-437)
+// Without FCT: k_edge, k_vert and k_final.  No atomics: each node adds its incident edges in
+// the reference's edge order, so sums are deterministic and equal the serial reference.
+#include "fesomtra_column.hpp"
 #include "fesomtra_kernels.hpp"
 
 namespace fesomtra {
 
 // ---------------------------------------------------------------------------------------------
-template <int HOR, bool FCT>
-__global__ __launch_bounds__(256) void k_edge(MeshDev m, int cols, const double *__restrict__ T,
-                                              const double *__restrict__ TAB, double num_ord,
-                                              double *__restrict__ flo, double *__restrict__ adfh) {
-    const Tile t = tile_of(m.L, cols, m.E);
-    if (!t.live) return;
-    const int e = t.ent, nz = t.k + 1, L = m.L;
-    const EdgeGeo g = edge_geo(m, e);
-    const int nu12u = g.nu2 > 0 ? min(g.nu1, g.nu2) : g.nu1;
-    const int nl12u = max(g.nl1, g.nl2);
-    if (nz < nu12u || nz > nl12u) return;
-    const double v = edge_vflux(m, g, e, nz);
-    double fin = 0.0;
-    if (FCT) {
-        fin = upw1_edge(T[(size_t)g.n1 * L + t.k], T[(size_t)g.n2 * L + t.k], v);
-        flo[(size_t)e * L + t.k] = fin;
-    }
-    const double a1 = TAB[(size_t)g.n1 * L + t.k], a2 = TAB[(size_t)g.n2 * L + t.k];
-    double f;
-    if (HOR == HOR_UPW1) {
-        f = upw1_edge(a1, a2, v) - fin;
-    } else {
-        double gr[4];
-        edge_grad(m, TAB, g, e, nz, gr);
-        f = ho_edge<HOR>(m, g, e, nz, a1, a2, gr, v, num_ord, fin);
-    }
-    adfh[(size_t)e * L + t.k] = f;
+// Work decomposition: one thread owns `lc` consecutive levels of one column; blockIdx.x selects
+// columns (XCD-aware, see column_index) and blockIdx.y the level chunk.  Vertical stencils that
+// cross a chunk boundary recompute the few values they need from the neighbouring chunk.
+
+// 1-based layer range [nz0, nz1] of this thread's chunk in a column of L wet layers.
+__device__ __forceinline__ bool chunk_layers(int L, int lc, int &nz0, int &nz1) {
+    nz0 = (int)blockIdx.y * lc + 1;
+    if (nz0 > L) return false;
+    nz1 = min(nz0 + lc - 1, L);
+    return true;
 }
 
-// ---------------------------------------------------------------------------------------------
-// Low-order solution and vertical antidiffusive flux (lib/oce_adv_tra_driver.F90:102-216 and the
-// high-order vertical call :266-282 with o_init_zero=.false.).
-template <int VER>
-__global__ __launch_bounds__(256) void k_node_lo(MeshDev m, int cols, const double *__restrict__ T,
-                                                 const double *__restrict__ TAB, double num_ord,
-                                                 double dt, const double *__restrict__ flo,
-                                                 double *__restrict__ LO, double *__restrict__ adfv) {
-    const Tile t = tile_of(m.L, cols, m.N);
-    if (!t.live) return;
-    const int n = t.ent, nz = t.k + 1, L = m.L, NL = m.nl;
-    const int nzmin = m.ulev_n[n], nzmax = m.nlev_n[n];
-    if (nz < nzmin || nz > nzmax - 1) return;
-    double s = 0.0;
-    const int b = m.ne_ptr[n], eend = m.ne_ptr[n + 1];
-    for (int i = b; i < eend; ++i) {
-        const int2 q = m.ne_list[i];
-        if (!in_range(q.y, nz)) continue;
-        const double f = flo[(size_t)(q.x >> 1) * L + t.k];
-        s = (q.x & 1) ? s - f : s + f;
-    }
-    const Column cl = column(m, T, m.we, n);
-    const double fv0 = ver_upw1(cl, nz, 0.0);
-    const double fv1 = ver_upw1(cl, nz + 1, 0.0);
-    const size_t ix = (size_t)n * L + t.k;
-    LO[ix] = (T[ix] * m.hnode[ix] + (s + (fv0 - fv1)) * dt / m.areasvol[(size_t)n * NL + t.k]) /
-             m.hnode_new[ix];
-    const Column ch = column(m, TAB, m.w, n);
-    adfv[(size_t)n * NL + t.k] = ver_ho<VER>(ch, nz, num_ord, dt, fv0);
-    if (nz == nzmax - 1) adfv[(size_t)n * NL + nz] = ver_ho<VER>(ch, nzmax, num_ord, dt, fv1);
-}
-
-// Vertical high-order flux without the limiter (pwvel = we, o_init_zero=.true.).
-template <int VER>
-__global__ __launch_bounds__(256) void k_vert(MeshDev m, int cols, const double *__restrict__ TAB,
-                                              double num_ord, double dt, double *__restrict__ fv) {
-    const Tile t = tile_of(m.L, cols, m.N);
-    if (!t.live) return;
-    const int n = t.ent, nz = t.k + 1, NL = m.nl;
-    const int nzmin = m.ulev_n[n], nzmax = m.nlev_n[n];
-    if (nz < nzmin || nz > nzmax - 1) return;
-    const Column ch = column(m, TAB, m.we, n);
-    fv[(size_t)n * NL + t.k] = ver_ho<VER>(ch, nz, num_ord, dt, 0.0);
-    if (nz == nzmax - 1) fv[(size_t)n * NL + nz] = ver_ho<VER>(ch, nzmax, num_ord, dt, 0.0);
-}
-
-// ---------------------------------------------------------------------------------------------
-// FCT bounds and limiter ratios (lib/oce_adv_tra_fct.F90:96-374).
-__global__ __launch_bounds__(256) void k_bounds(MeshDev m, int cols, const double *__restrict__ T,
-                                                const double *__restrict__ LO,
-                                                const double *__restrict__ adfh,
-                                                const double *__restrict__ adfv, double dt,
-                                                double *__restrict__ Rp, double *__restrict__ Rm) {
-    __shared__ double s_max[256], s_min[256];
-    const Tile t = tile_of(m.L, cols, m.N);
-    const int L = m.L, NL = m.nl;
-    const int n = t.ent, nz = t.k + 1;
-    int nu1 = 0, nl1 = 0;
-    bool wet = false;
-    if (t.live) {
-        nu1 = m.ulev_n[n];
-        nl1 = m.nlev_n[n];
-        wet = nz >= nu1 && nz <= nl1 - 1;
-    }
-    if (wet) {
-        // a1 + a2 + a3: max / min of max(LO, T) / min(LO, T) over the nodes of every element
-        // around n; elements dry at nz contribute -1e3 / +1e3.
-        double vmax = 0.0, vmin = 0.0;
-        const int cnt = m.nie_num[n];
-        for (int i = 0; i < cnt; ++i) {
-            const int el = m.nie[(size_t)n * m.max_adj + i];
-            double emax, emin;
-            if (nz >= m.ulev_e[el] && nz <= m.nlev_e[el] - 1) {
-                const int *en = m.enodes + (size_t)el * 3;
-                const size_t i0 = (size_t)en[0] * L + t.k, i1 = (size_t)en[1] * L + t.k,
-                             i2 = (size_t)en[2] * L + t.k;
-                const double l0 = LO[i0], l1 = LO[i1], l2 = LO[i2];
-                const double t0 = T[i0], t1 = T[i1], t2 = T[i2];
-                emax = dmax(dmax(dmax(l0, t0), dmax(l1, t1)), dmax(l2, t2));
-                emin = dmin(dmin(dmin(l0, t0), dmin(l1, t1)), dmin(l2, t2));
-            } else {
-                emax = -1.0e3;
-                emin = 1.0e3;
+template <int HOR, bool FCT, int KT, typename S, typename M, typename C>
+__global__ __launch_bounds__(256) void k_edge(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, int lc) {
+    const int e = column_index();
+    if (e >= m.E) return;
+    const int4 q = m.edge_ne[e];
+    const int n1 = q.x, n2 = q.y, el1 = q.z, el2 = q.w;
+    const int nl1 = m.nlev_e[el1] - 1;
+    const int nl2 = el2 >= 0 ? m.nlev_e[el2] - 1 : 0;
+    int nz0, nz1;
+    if (!chunk_layers(max(nl1, nl2), lc, nz0, nz1)) return;
+    const size_t Np = m.Np, Tp = m.Tp, Ep = m.Ep;
+    const C cd0 = (C)m.ecd[4 * (size_t)e], cd1 = (C)m.ecd[4 * (size_t)e + 1];
+    const C cd2 = (C)m.ecd[4 * (size_t)e + 2], cd3 = (C)m.ecd[4 * (size_t)e + 3];
+    C dx = 0, dy = 0, a = 0;
+    int shared_end = 0, u[3] = {0, 0, 0}, d[3] = {0, 0, 0}, nb1 = 0, nb2 = 0;
+    C gu[6], gd[6];
+    if (HOR != HOR_UPW1) {
+        dx = (C)m.edxdy[2 * (size_t)e];
+        dy = (C)m.edxdy[2 * (size_t)e + 1];
+        a = (C)kREarth * (C)m.elem_cos[el1];
+        if (el2 >= 0) a = C(0.5) * (a + (C)kREarth * (C)m.elem_cos[el2]);
+        const int2 ud = m.updn[e];
+        if (ud.x >= 0 && ud.y >= 0) {
+            shared_end = min(m.nlev_min[n1], m.nlev_min[n2]) - 1;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                u[c] = m.enodes[(size_t)ud.x * 3 + c];
+                d[c] = m.enodes[(size_t)ud.y * 3 + c];
             }
-            if (i == 0) {
-                vmax = emax;
-                vmin = emin;
-            } else {
-                vmax = dmax(vmax, emax);
-                vmin = dmin(vmin, emin);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                gu[c] = (C)m.gsca[(size_t)ud.x * 6 + c];
+                gd[c] = (C)m.gsca[(size_t)ud.y * 6 + c];
             }
         }
-        s_max[threadIdx.x] = vmax;
-        s_min[threadIdx.x] = vmin;
+        if (HOR == HOR_MUSCL) {
+            nb1 = m.nbl[n1];
+            nb2 = m.nbl[n2];
+        }
     }
-    __syncthreads();
-    if (!wet) return;
-    const size_t ix = (size_t)n * L + t.k;
-    const double lo = LO[ix];
-    double qmax, qmin;
-    const int tid = threadIdx.x;
-    if (nz == nl1 - 1 || nz == nu1) {
-        qmax = s_max[tid] - lo;
-        qmin = s_min[tid] - lo;
-    } else {
-        qmax = dmax(dmax(s_max[tid - 1], s_max[tid]), s_max[tid + 1]) - lo;
-        qmin = dmin(dmin(s_min[tid - 1], s_min[tid]), s_min[tid + 1]) - lo;
+    for (int nz = nz0; nz <= nz1; ++nz) {
+        const int k = nz - 1;
+        // segments C (both elements), D (el1 only), E (el2 only) of lib/oce_adv_tra_hor.F90:166-253
+        const bool use1 = nz <= nl1, use2 = nz <= nl2;
+        C v1 = 0, v2 = 0;
+        if (use1) {
+            const size_t i = (size_t)k * Tp + el1;
+            v1 = (-(C)m.uv[2 * i + 1] * cd0 + (C)m.uv[2 * i] * cd1) * (C)m.helem[i];
+        }
+        if (use2) {
+            const size_t i = (size_t)k * Tp + el2;
+            v2 = ((C)m.uv[2 * i + 1] * cd2 - (C)m.uv[2 * i] * cd3) * (C)m.helem[i];
+        }
+        const C v = (use1 && use2) ? v1 + v2 : (use1 ? v1 : v2);
+        const size_t o = (size_t)k * Np;
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+            const bool same = p.TAB[t] == p.T[t];
+            C fin = 0;
+            C t1 = 0, t2 = 0;
+            if (FCT || same) {
+                t1 = ldc<C>(p.T[t], o + n1);
+                t2 = ldc<C>(p.T[t], o + n2);
+            }
+            if (FCT) {
+                fin = upw1_edge(t1, t2, v);
+                p.flo[t][(size_t)k * Ep + e] = fin;
+            }
+            const C a1 = same ? t1 : ldc<C>(p.TAB[t], o + n1);
+            const C a2 = same ? t2 : ldc<C>(p.TAB[t], o + n2);
+            C f;
+            if (HOR == HOR_UPW1) {
+                f = upw1_edge(a1, a2, v) - fin;
+            } else {
+                C gr[4];
+                if (nz <= shared_end) {
+                    gr[0] = tr_xy<C>(gu, p.TAB[t], o, u[0], u[1], u[2]);
+                    gr[1] = tr_xy<C>(gd, p.TAB[t], o, d[0], d[1], d[2]);
+                    gr[2] = tr_xy<C>(gu + 3, p.TAB[t], o, u[0], u[1], u[2]);
+                    gr[3] = tr_xy<C>(gd + 3, p.TAB[t], o, d[0], d[1], d[2]);
+                } else {
+                    node_mean_grad<C>(m, p.TAB[t], n1, nz, gr[0], gr[2]);
+                    node_mean_grad<C>(m, p.TAB[t], n2, nz, gr[1], gr[3]);
+                }
+                const C c1 = (HOR == HOR_MUSCL) ? ((nb1 - nz >= 0) ? C(1) : C(0)) : C(1);
+                const C c2 = (HOR == HOR_MUSCL) ? ((nb2 - nz >= 0) ? C(1) : C(0)) : C(1);
+                f = ho_edge<HOR, C>(a1, a2, gr, v, dx, dy, a, num_ord, c1, c2, fin);
+            }
+            p.adfh[t][(size_t)k * Ep + e] = f;
+        }
     }
-    // b1: vertical then horizontal contributions, in the reference's order.
-    const double av0 = adfv[(size_t)n * NL + t.k], av1 = adfv[(size_t)n * NL + t.k + 1];
-    double pp = 0.0 + (dmax(0.0, av0) + dmax(0.0, -av1));
-    double pm = 0.0 + (dmin(0.0, av0) + dmin(0.0, -av1));
-    const int b = m.ne_ptr[n], eend = m.ne_ptr[n + 1];
-    for (int i = b; i < eend; ++i) {
-        const int2 q = m.ne_list[i];
-        if (!in_range(q.y, nz)) continue;
-        double a = adfh[(size_t)(q.x >> 1) * L + t.k];
-        if (q.x & 1) a = -a;
-        pp = pp + dmax(0.0, a);
-        pm = pm + dmin(0.0, a);
-    }
-    // b2
-    const double V = m.areasvol[(size_t)n * NL + t.k];
-    const double fp = pp * dt / V + 1e-16;
-    const double fm = pm * dt / V - 1e-16;
-    Rp[ix] = dmin(1.0, qmax / fp);
-    Rm[ix] = dmin(1.0, qmin / fm);
 }
 
 // ---------------------------------------------------------------------------------------------
-// Limited vertical antidiffusive flux at interface j of node n (lib/oce_adv_tra_fct.F90:396-424).
-__device__ __forceinline__ double limit_v(const double *__restrict__ adfv,
-                                          const double *__restrict__ Rp,
-                                          const double *__restrict__ Rm, int L, int NL, int n,
-                                          int j, int nu1, int nl1) {
-    const double a = adfv[(size_t)n * NL + j - 1];
-    if (j > nl1 - 1) return a;  // bottom interface: never limited (always zero)
-    double ae = 1.0;
-    const size_t r = (size_t)n * L + j - 1;
-    if (j == nu1) {
-        ae = a >= 0.0 ? dmin(ae, Rp[r]) : dmin(ae, Rm[r]);
-    } else if (a >= 0.0) {
-        ae = dmin(ae, Rm[r - 1]);
-        ae = dmin(ae, Rp[r]);
-    } else {
-        ae = dmin(ae, Rp[r - 1]);
-        ae = dmin(ae, Rm[r]);
+// Node -> edge entries cached in registers (the first kDegCache, the rest read from memory).
+struct NodeEdges {
+    int b, deg;
+    int e2[kDegCache], top[kDegCache], oth[kDegCache];
+};
+template <typename M>
+__device__ __forceinline__ void load_edges(const MeshCol<M> &m, int n, NodeEdges &ne) {
+    ne.b = m.ne_ptr[n];
+    ne.deg = m.ne_ptr[n + 1] - ne.b;
+#pragma unroll
+    for (int i = 0; i < kDegCache; ++i) {
+        if (i < ne.deg) {
+            ne.e2[i] = m.ne_e2[ne.b + i];
+            ne.top[i] = m.ne_top[ne.b + i];
+            ne.oth[i] = m.ne_other[ne.b + i];
+        } else {
+            ne.e2[i] = 0;
+            ne.top[i] = 0;
+            ne.oth[i] = 0;
+        }
     }
-    return ae * a;
+}
+// Calls f(edge, second, other) for the edges of n active at level nz, in reference order.
+template <typename M, typename F>
+__device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges &ne, int nz, F &&f) {
+#pragma unroll
+    for (int i = 0; i < kDegCache; ++i)
+        if (i < ne.deg && nz <= ne.top[i]) f(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.oth[i]);
+    for (int i = kDegCache; i < ne.deg; ++i) {
+        const int e2 = m.ne_e2[ne.b + i];
+        if (nz <= m.ne_top[ne.b + i]) f(e2 >> 1, (e2 & 1) != 0, m.ne_other[ne.b + i]);
+    }
 }
 
-// FCT limiting (b3) + oce_tra_adv_flux2dtracer (lib/oce_adv_tra_driver.F90:298-450) + caller
-// update.  With acc_h/acc_v the tendencies are accumulated there (do_oce_adv_tra contract);
-// otherwise values += (dttf_h + dttf_v) / hnode_new in place.  dbg_h / dbg_v, when non-null,
-// receive the limited antidiffusive fluxes.
-template <bool FCT>
-__global__ __launch_bounds__(256) void k_final(MeshDev m, int cols, double *__restrict__ T,
-                                               const double *__restrict__ LO,
-                                               const double *__restrict__ fh,
-                                               const double *__restrict__ fv,
-                                               const double *__restrict__ Rp,
-                                               const double *__restrict__ Rm, double dt,
-                                               double *__restrict__ acc_h, double *__restrict__ acc_v,
-                                               double *__restrict__ dbg_h, double *__restrict__ dbg_v) {
-    const Tile t = tile_of(m.L, cols, m.N);
-    if (!t.live) return;
-    const int n = t.ent, nz = t.k + 1, L = m.L, NL = m.nl;
-    const int nu1 = m.ulev_n[n], nl1 = m.nlev_n[n];
-    if (nz < nu1 || nz > nl1 - 1) return;
-    const size_t ix = (size_t)n * L + t.k;
-    const double V = m.areasvol[(size_t)n * NL + t.k];
-    const double tval = T[ix];
-    double dv = acc_v ? acc_v[ix] : 0.0;
-    double top, bot;
-    if (FCT) {
-        dv = dv - tval * m.hnode[ix] + LO[ix] * m.hnode_new[ix];
-        top = limit_v(fv, Rp, Rm, L, NL, n, nz, nu1, nl1);
-        bot = limit_v(fv, Rp, Rm, L, NL, n, nz + 1, nu1, nl1);
-        if (dbg_v) {
-            dbg_v[(size_t)n * NL + t.k] = top;
-            if (nz == nl1 - 1) dbg_v[(size_t)n * NL + nz] = bot;
-        }
-    } else {
-        top = fv[(size_t)n * NL + t.k];
-        bot = fv[(size_t)n * NL + t.k + 1];
+// Column readers of one node (1-based levels; 0 outside the wet range).
+template <typename C, typename S, typename M>
+struct NodeCol {
+    size_t Np;
+    int n, L;
+    __device__ __forceinline__ C lay(const S *a, int nz) const {
+        return (nz >= 1 && nz <= L) ? ldc<C>(a, (size_t)(nz - 1) * Np + n) : C(0);
     }
-    dv = dv + (top - bot) * dt / V;
-    double dh = acc_h ? acc_h[ix] : 0.0;
-    const int b = m.ne_ptr[n], eend = m.ne_ptr[n + 1];
-    for (int i = b; i < eend; ++i) {
-        const int2 q = m.ne_list[i];
-        if (!in_range(q.y, nz)) continue;
-        const int e = q.x >> 1;
-        const bool second = q.x & 1;
-        double f = fh[(size_t)e * L + t.k];
+    __device__ __forceinline__ C mlay(const M *a, int nz) const {
+        return (nz >= 1 && nz <= L) ? (C)a[(size_t)(nz - 1) * Np + n] : C(0);
+    }
+    __device__ __forceinline__ C itf(const M *a, int j) const { return (C)a[(size_t)(j - 1) * Np + n]; }
+};
+
+// High-order vertical flux at interface j minus fin for the window tw = TAB(j-2..j+1),
+// z = Z(j-2..j+1).
+template <int VER, typename C, typename S, typename M>
+__device__ __forceinline__ C ver_ho_at(int j, int nzmax, const C tw[4], const C z[4], C zbj, C wj, C aj,
+                                       C num_ord, C dt, const Col<C, S, M> &col, C fin) {
+    if (VER == VER_QR4C) return ver_qr4c_w<C>(j, nzmax, tw, z, zbj, wj, aj, num_ord, fin);
+    if (VER == VER_UPW1) return ver_upw1_w<C>(j, nzmax, tw[1], tw[2], wj, aj, fin);
+    if (VER == VER_CDIFF) return ver_cdiff(col, j, fin);
+    return ver_ppm(col, j, dt, fin);
+}
+
+template <typename C, typename S, typename M>
+__device__ __forceinline__ Col<C, S, M> make_col(const S *t, const M *w, const MeshCol<M> &m, int n, int nzmax) {
+    Col<C, S, M> c;
+    c.t = t;
+    c.w = w;
+    c.ar = m.area;
+    c.hn = m.hnode_new;
+    c.ho = m.hnode;
+    c.Np = m.Np;
+    c.n = n;
+    c.nzmin = 1;
+    c.nzmax = nzmax;
+    return c;
+}
+
+// Low-order solution and vertical antidiffusive flux (lib/oce_adv_tra_driver.F90:102-216 and
+// the high-order vertical call :266-282 with o_init_zero=.false., pwvel = w).  Interface fluxes
+// are evaluated top-down; layer nz uses the fluxes at its top (nz) and bottom (nz + 1).
+template <int VER, int KT, typename S, typename M, typename C>
+__global__ __launch_bounds__(256) void k_lo(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt, int lc) {
+    const int n = column_index();
+    if (n >= m.N) return;
+    const int nzmax = m.nlev_n[n], L = nzmax - 1;
+    int nz0, nz1;
+    if (!chunk_layers(L, lc, nz0, nz1)) return;
+    const size_t Np = m.Np, Ep = m.Ep;
+    NodeEdges ne;
+    load_edges(m, n, ne);
+    const NodeCol<C, S, M> nc{Np, n, L};
+    C tw[KT][4], z[4], lo_top[KT], adf_top[KT], tcur[KT];
+    Col<C, S, M> col[KT];
+    {
+        const int j = nz0;  // top interface of the chunk
+#pragma unroll
+        for (int i = 0; i < 4; ++i) z[i] = (VER == VER_QR4C) ? nc.mlay(m.Z, j - 2 + i) : C(0);
+        const C wej = nc.itf(m.we, j), wj = nc.itf(m.w, j), aj = nc.itf(m.area, j);
+        const C zbj = (VER == VER_QR4C) ? nc.itf(m.zbar, j) : C(0);
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+            col[t] = make_col<C, S, M>(p.TAB[t], m.w, m, n, nzmax);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) tw[t][i] = nc.lay(p.TAB[t], j - 2 + i);
+            tcur[t] = nc.lay(p.T[t], j);
+            lo_top[t] = ver_upw1_w<C>(j, nzmax, nc.lay(p.T[t], j - 1), tcur[t], wej, aj, C(0));
+            adf_top[t] = ver_ho_at<VER>(j, nzmax, tw[t], z, zbj, wj, aj, num_ord, (C)dt, col[t], lo_top[t]);
+        }
+    }
+    for (int nz = nz0; nz <= nz1; ++nz) {
+        const int j = nz + 1;  // bottom interface of layer nz
+        if (VER == VER_QR4C) {
+            z[0] = z[1];
+            z[1] = z[2];
+            z[2] = z[3];
+            z[3] = nc.mlay(m.Z, j + 1);
+        }
+        const bool bottom = (j == nzmax);
+        const C wej = nc.itf(m.we, j), wj = nc.itf(m.w, j), aj = nc.itf(m.area, j);
+        const C zbj = (VER == VER_QR4C && !bottom) ? nc.itf(m.zbar, j) : C(0);
+        const size_t ix = (size_t)(nz - 1) * Np + n;
+        const double h = (double)m.hnode[ix], hn = (double)m.hnode_new[ix];
+        const double V = (double)m.areasvol[ix];
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+            tw[t][0] = tw[t][1];
+            tw[t][1] = tw[t][2];
+            tw[t][2] = tw[t][3];
+            tw[t][3] = nc.lay(p.TAB[t], j + 1);
+            const C tnext = nc.lay(p.T[t], j);
+            const C lo_bot = ver_upw1_w<C>(j, nzmax, tcur[t], tnext, wej, aj, C(0));
+            const C adf_bot = ver_ho_at<VER>(j, nzmax, tw[t], z, zbj, wj, aj, num_ord, (C)dt, col[t], lo_bot);
+            double s = 0.0;
+            for_edges(m, ne, nz, [&](int e, bool second, int) {
+                const double f = (double)p.flo[t][(size_t)(nz - 1) * Ep + e];
+                s = second ? s - f : s + f;
+            });
+            const double lo = ((double)tcur[t] * h + (s + ((double)lo_top[t] - (double)lo_bot)) * dt / V) / hn;
+            p.LO[t][ix] = (C)lo;
+            p.adfv[t][ix] = adf_top[t];
+            if (bottom) p.adfv[t][(size_t)nz * Np + n] = adf_bot;
+            lo_top[t] = lo_bot;
+            adf_top[t] = adf_bot;
+            tcur[t] = tnext;
+        }
+    }
+}
+
+// Vertical high-order flux without the limiter (pwvel = we, o_init_zero=.true.): interfaces
+// nz0..nz1 of the chunk, plus the bottom interface in the deepest chunk.
+template <int VER, int KT, typename S, typename M, typename C>
+__global__ __launch_bounds__(256) void k_vert(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt, int lc) {
+    const int n = column_index();
+    if (n >= m.N) return;
+    const int nzmax = m.nlev_n[n], L = nzmax - 1;
+    int nz0, nz1;
+    if (!chunk_layers(L, lc, nz0, nz1)) return;
+    const size_t Np = m.Np;
+    const NodeCol<C, S, M> nc{Np, n, L};
+    C tw[KT][4], z[4];
+    Col<C, S, M> col[KT];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) z[i] = (VER == VER_QR4C) ? nc.mlay(m.Z, nz0 - 3 + i) : C(0);
+#pragma unroll
+    for (int t = 0; t < KT; ++t) {
+        col[t] = make_col<C, S, M>(p.TAB[t], m.we, m, n, nzmax);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) tw[t][i] = nc.lay(p.TAB[t], nz0 - 3 + i);
+    }
+    const int jend = nz1 == L ? nzmax : nz1;
+    for (int j = nz0; j <= jend; ++j) {
+        if (VER == VER_QR4C) {
+            z[0] = z[1];
+            z[1] = z[2];
+            z[2] = z[3];
+            z[3] = nc.mlay(m.Z, j + 1);
+        }
+        const bool bottom = (j == nzmax);
+        const C wj = nc.itf(m.we, j), aj = nc.itf(m.area, j);
+        const C zbj = (VER == VER_QR4C && !bottom) ? nc.itf(m.zbar, j) : C(0);
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+            tw[t][0] = tw[t][1];
+            tw[t][1] = tw[t][2];
+            tw[t][2] = tw[t][3];
+            tw[t][3] = nc.lay(p.TAB[t], j + 1);
+            p.adfv[t][(size_t)(j - 1) * Np + n] = ver_ho_at<VER>(j, nzmax, tw[t], z, zbj, wj, aj, num_ord, (C)dt, col[t], C(0));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// FCT bounds and limiter ratios (lib/oce_adv_tra_fct.F90:96-374).  The cluster of node n at
+// level nz is the set of nodes of the elements around n that are wet at nz; with complete
+// nod_in_elem2D that is n plus every neighbour m whose edge (n, m) reaches level nz, and an
+// element dry at nz contributes -1e3 / +1e3 exactly when nz > nlevels_nod2D_min(n) - 1.
+template <int KT, typename S, typename M, typename C>
+__global__ __launch_bounds__(256) void k_bounds(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
+    const int n = column_index();
+    if (n >= m.N) return;
+    const int nzmax = m.nlev_n[n], L = nzmax - 1;
+    int nz0, nz1;
+    if (!chunk_layers(L, lc, nz0, nz1)) return;
+    const size_t Np = m.Np, Ep = m.Ep;
+    const int dry_from = m.nlev_min[n];  // levels >= this have a dry element around n
+    NodeEdges ne;
+    load_edges(m, n, ne);
+    auto cluster = [&](int t, int nz, C &vmax, C &vmin) {
+        const size_t o = (size_t)(nz - 1) * Np;
+        const C l0 = p.LO[t][o + n], t0 = ldc<C>(p.T[t], o + n);
+        vmax = cmax(l0, t0);
+        vmin = cmin(l0, t0);
+        for_edges(m, ne, nz, [&](int, bool, int oth) {
+            const C lo = p.LO[t][o + oth], tt = ldc<C>(p.T[t], o + oth);
+            vmax = cmax(vmax, cmax(lo, tt));
+            vmin = cmin(vmin, cmin(lo, tt));
+        });
+        if (nz >= dry_from) {
+            vmax = cmax(vmax, C(-1.0e3));
+            vmin = cmin(vmin, C(1.0e3));
+        }
+    };
+    C mx[KT][3], mn[KT][3], av0[KT];
+#pragma unroll
+    for (int t = 0; t < KT; ++t) {
+        if (nz0 >= 2 && nz0 < L) cluster(t, nz0 - 1, mx[t][0], mn[t][0]);
+        else mx[t][0] = mn[t][0] = 0;
+        cluster(t, nz0, mx[t][1], mn[t][1]);
+        if (nz0 + 1 <= L && nz0 > 1 && nz0 < L) cluster(t, nz0 + 1, mx[t][2], mn[t][2]);
+        else mx[t][2] = mn[t][2] = 0;
+        av0[t] = p.adfv[t][(size_t)(nz0 - 1) * Np + n];
+    }
+    for (int nz = nz0; nz <= nz1; ++nz) {
+        const size_t ix = (size_t)(nz - 1) * Np + n;
+        const double V = (double)m.areasvol[ix];
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+            const C lo = p.LO[t][ix];
+            C qmax, qmin;
+            if (nz == L || nz == 1) {
+                qmax = mx[t][1] - lo;
+                qmin = mn[t][1] - lo;
+            } else {
+                qmax = cmax(cmax(mx[t][0], mx[t][1]), mx[t][2]) - lo;
+                qmin = cmin(cmin(mn[t][0], mn[t][1]), mn[t][2]) - lo;
+            }
+            const C av1 = p.adfv[t][(size_t)nz * Np + n];
+            double pp = 0.0 + ((double)cmax(C(0), av0[t]) + (double)cmax(C(0), -av1));
+            double pm = 0.0 + ((double)cmin(C(0), av0[t]) + (double)cmin(C(0), -av1));
+            for_edges(m, ne, nz, [&](int e, bool second, int) {
+                C a = p.adfh[t][(size_t)(nz - 1) * Ep + e];
+                if (second) a = -a;
+                pp = pp + (double)cmax(C(0), a);
+                pm = pm + (double)cmin(C(0), a);
+            });
+            const double fp = pp * dt / V + 1e-16;
+            const double fm = pm * dt / V - 1e-16;
+            const double rp = (double)qmax / fp, rm = (double)qmin / fm;
+            p.Rp[t][ix] = (C)(rp < 1.0 ? rp : 1.0);
+            p.Rm[t][ix] = (C)(rm < 1.0 ? rm : 1.0);
+            av0[t] = av1;
+            if (nz < nz1) {  // slide the cluster window to the next layer of this chunk
+                mx[t][0] = mx[t][1];
+                mn[t][0] = mn[t][1];
+                if (nz + 1 == L) {
+                    cluster(t, nz + 1, mx[t][1], mn[t][1]);  // bottom layer uses its own bound
+                } else {
+                    mx[t][1] = mx[t][2];
+                    mn[t][1] = mn[t][2];
+                    if (nz == 1) {  // the surface layer did not need its lower neighbour
+                        cluster(t, nz + 1, mx[t][1], mn[t][1]);
+                    }
+                    cluster(t, nz + 2, mx[t][2], mn[t][2]);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Limiting (b3) + flux divergence (oce_tra_adv_flux2dtracer) + caller update.  With acc_h /
+// acc_v the tendencies are accumulated there (do_oce_adv_tra contract); otherwise values +=
+// (dttf_h + dttf_v) / hnode_new in place.  dbg_h / dbg_v receive the limited fluxes if set.
+template <bool FCT, int KT, typename S, typename M, typename C>
+__global__ __launch_bounds__(256) void k_final(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
+    const int n = column_index();
+    if (n >= m.N) return;
+    const int nzmax = m.nlev_n[n], L = nzmax - 1;
+    int nz0, nz1;
+    if (!chunk_layers(L, lc, nz0, nz1)) return;
+    const size_t Np = m.Np, Ep = m.Ep;
+    NodeEdges ne;
+    load_edges(m, n, ne);
+    // limited flux at interface j given R at layers j-1 (rp0, rm0) and j (rp1, rm1)
+    auto limit = [&](C a, int j, C rp0, C rm0, C rp1, C rm1) -> C {
+        C ae = 1;
+        if (j == 1) ae = a >= C(0) ? cmin(ae, rp1) : cmin(ae, rm1);
+        else if (a >= C(0)) {
+            ae = cmin(ae, rm0);
+            ae = cmin(ae, rp1);
+        } else {
+            ae = cmin(ae, rp0);
+            ae = cmin(ae, rm1);
+        }
+        return ae * a;
+    };
+    C top[KT], rp0[KT], rm0[KT], rp1[KT], rm1[KT];
+    const size_t i0 = (size_t)(nz0 - 1) * Np + n;
+#pragma unroll
+    for (int t = 0; t < KT; ++t) {
+        const C a0 = p.adfv[t][i0];
         if (FCT) {
-            // R of both end nodes (lib/oce_adv_tra_fct.F90:455-468)
-            const int other = second ? m.edge_ne[e].x : m.edge_ne[e].y;
-            const size_t ro = (size_t)other * L + t.k;
-            const int node1 = second ? other : n;
-            const int node2 = second ? n : other;
-            const size_t r1 = node1 == n ? ix : ro, r2 = node2 == n ? ix : ro;
-            double ae = 1.0;
-            if (f >= 0.0) {
-                ae = dmin(ae, Rp[r1]);
-                ae = dmin(ae, Rm[r2]);
-            } else {
-                ae = dmin(ae, Rm[r1]);
-                ae = dmin(ae, Rp[r2]);
-            }
-            f = ae * f;
-            if (dbg_h && !second) dbg_h[(size_t)e * L + t.k] = f;
+            rp1[t] = p.Rp[t][i0];
+            rm1[t] = p.Rm[t][i0];
+            const C rpa = nz0 > 1 ? p.Rp[t][i0 - Np] : C(0);
+            const C rma = nz0 > 1 ? p.Rm[t][i0 - Np] : C(0);
+            rp0[t] = rm0[t] = 0;
+            top[t] = limit(a0, nz0, rpa, rma, rp1[t], rm1[t]);
+            if (p.dbg_v[t]) p.dbg_v[t][i0] = top[t];
+        } else {
+            top[t] = a0;
         }
-        dh = second ? dh - f * dt / V : dh + f * dt / V;
     }
-    if (acc_h) {
-        acc_h[ix] = dh;
-        acc_v[ix] = dv;
-    } else {
-        T[ix] = tval + (dh + dv) / m.hnode_new[ix];
+    for (int nz = nz0; nz <= nz1; ++nz) {
+        const size_t ix = (size_t)(nz - 1) * Np + n;
+        const double V = (double)m.areasvol[ix];
+        const double hn = (double)m.hnode_new[ix];
+        const double h = FCT ? (double)m.hnode[ix] : 0.0;
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+            const double tval = (double)ldc<C>(p.T[t], ix);
+            const C ab = p.adfv[t][ix + Np];
+            C bot;
+            double dv = p.acc_v[t] ? p.acc_v[t][ix] : 0.0;
+            if (FCT) {
+                dv = dv - tval * h + (double)p.LO[t][ix] * hn;
+                rp0[t] = rp1[t];
+                rm0[t] = rm1[t];
+                if (nz + 1 <= L) {
+                    rp1[t] = p.Rp[t][ix + Np];
+                    rm1[t] = p.Rm[t][ix + Np];
+                    bot = limit(ab, nz + 1, rp0[t], rm0[t], rp1[t], rm1[t]);
+                } else {
+                    bot = ab;  // bottom interface: never limited (always zero)
+                }
+                if (p.dbg_v[t]) p.dbg_v[t][ix + Np] = bot;
+            } else {
+                bot = ab;
+            }
+            dv = dv + ((double)top[t] - (double)bot) * dt / V;
+            double dh = p.acc_h[t] ? p.acc_h[t][ix] : 0.0;
+            const C rpn = FCT ? rp0[t] : C(0), rmn = FCT ? rm0[t] : C(0);
+            for_edges(m, ne, nz, [&](int e, bool second, int oth) {
+                C f = p.adfh[t][(size_t)(nz - 1) * Ep + e];
+                if (FCT) {
+                    // R of both end nodes (lib/oce_adv_tra_fct.F90:455-468)
+                    const C rpo = p.Rp[t][(size_t)(nz - 1) * Np + oth];
+                    const C rmo = p.Rm[t][(size_t)(nz - 1) * Np + oth];
+                    const C rp_1 = second ? rpo : rpn, rm_1 = second ? rmo : rmn;  // edges(1)
+                    const C rp_2 = second ? rpn : rpo, rm_2 = second ? rmn : rmo;  // edges(2)
+                    C ae = 1;
+                    if (f >= C(0)) {
+                        ae = cmin(ae, rp_1);
+                        ae = cmin(ae, rm_2);
+                    } else {
+                        ae = cmin(ae, rm_1);
+                        ae = cmin(ae, rp_2);
+                    }
+                    f = ae * f;
+                    if (p.dbg_h[t] && !second) p.dbg_h[t][(size_t)(nz - 1) * Ep + e] = f;
+                }
+                dh = second ? dh - (double)f * dt / V : dh + (double)f * dt / V;
+            });
+            if (p.acc_h[t]) {
+                p.acc_h[t][ix] = dh;
+                p.acc_v[t][ix] = dv;
+            } else {
+                std_(p.T[t], ix, tval + (dh + dv) / hn);
+            }
+            top[t] = bot;
+        }
     }
 }
 
 // ---------------------------------------------------------------------------------------------
-// Renumbering between reference order and device order: out[i*ld + k] = in[map[i]*ld + k].
-__global__ void k_gather_rows(const double *__restrict__ in, double *__restrict__ out,
-                              const int *__restrict__ map, int rows, int ld) {
+// Renumbering / transposition between the reference layout (nl-1, nod2D) in reference order
+// and the device layout [k][Np] in device order.  dir 0: ref -> dev, dir 1: dev -> ref.
+template <typename S>
+__global__ void k_relayout(const double *__restrict__ ref_in, S *__restrict__ dev_out,
+                           const S *__restrict__ dev_in, double *__restrict__ ref_out,
+                           const int *__restrict__ perm, int N, int L, size_t Np, int dir) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)rows * ld) return;
-    const size_t r = i / ld, k = i - r * ld;
-    out[i] = in[(size_t)map[r] * ld + k];
+    if (i >= (size_t)N * L) return;
+    const int k = (int)(i / N), n = (int)(i - (size_t)k * N);
+    const int o = perm[n];  // reference number of device node n
+    if (dir == 0) std_(dev_out, (size_t)k * Np + n, ref_in[(size_t)o * L + k]);
+    else ref_out[(size_t)o * L + k] = ldc<double>(dev_in, (size_t)k * Np + n);
 }
 
-// Stats over wet layers: per-block partial min / max / sum / volume sum.
-__global__ __launch_bounds__(256) void k_stats_partial(MeshDev m, const double *__restrict__ T,
+// Stats over a tracer: per-block partial min / max / sum / volume sum over wet layers (or all).
+template <typename S, typename M>
+__global__ __launch_bounds__(256) void k_stats_partial(const MeshCol<M> m, const S *__restrict__ T,
                                                        int all_levels, double4 *__restrict__ part) {
     __shared__ double smin[256], smax[256], ssum[256], svol[256];
-    double mn = 1.0 / 0.0, mx = -1.0 / 0.0, sm = 0.0, sv = 0.0;
-    const size_t total = (size_t)m.N * m.L;
+    double mn = __builtin_huge_val(), mx = -__builtin_huge_val(), sm = 0.0, sv = 0.0;
+    const size_t total = (size_t)m.L * m.N;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-        const int n = (int)(i / m.L), k = (int)(i - (size_t)n * m.L);
-        const int nz = k + 1;
-        const bool wet = nz >= m.ulev_n[n] && nz <= m.nlev_n[n] - 1;
+        const int k = (int)(i / m.N), n = (int)(i - (size_t)k * m.N);
+        const bool wet = k + 1 <= m.nlev_n[n] - 1;
         if (!all_levels && !wet) continue;
-        const double v = T[i];
-        mn = dmin(mn, v);
-        mx = dmax(mx, v);
+        const size_t ix = (size_t)k * m.Np + n;
+        const double v = ldc<double>(T, ix);
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
         sm += v;
-        if (wet) sv += v * m.hnode[i] * m.areasvol[(size_t)n * m.nl + k];
+        if (wet) sv += v * (double)m.hnode[ix] * (double)m.areasvol[ix];
     }
     smin[threadIdx.x] = mn;
     smax[threadIdx.x] = mx;
@@ -307,8 +548,8 @@ __global__ __launch_bounds__(256) void k_stats_partial(MeshDev m, const double *
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
-            smin[threadIdx.x] = dmin(smin[threadIdx.x], smin[threadIdx.x + s]);
-            smax[threadIdx.x] = dmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+            smin[threadIdx.x] = smin[threadIdx.x] < smin[threadIdx.x + s] ? smin[threadIdx.x] : smin[threadIdx.x + s];
+            smax[threadIdx.x] = smax[threadIdx.x] > smax[threadIdx.x + s] ? smax[threadIdx.x] : smax[threadIdx.x + s];
             ssum[threadIdx.x] += ssum[threadIdx.x + s];
             svol[threadIdx.x] += svol[threadIdx.x + s];
         }
@@ -319,85 +560,119 @@ __global__ __launch_bounds__(256) void k_stats_partial(MeshDev m, const double *
 
 // ---------------------------------------------------------------------------------------------
 // Launchers
-static inline int cols_for(int L, int block) { return block / L; }
+static inline unsigned grid_for(int n_cols, int block) {
+    const unsigned b = (unsigned)((n_cols + block - 1) / block);
+    return (b + kXcds - 1) / kXcds * kXcds;  // multiple of the XCD count (see xcd_block)
+}
 
-#define LAUNCH_CHECK()                                        \
-    do {                                                      \
-        hipError_t _e = hipGetLastError();                    \
-        if (_e != hipSuccess) return _e;                      \
+#define LAUNCH_CHECK()                       \
+    do {                                     \
+        hipError_t _e = hipGetLastError();   \
+        if (_e != hipSuccess) return _e;     \
+    } while (0)
+#define MARK(i)                                         \
+    do {                                                \
+        if (ev) {                                       \
+            hipError_t _r = hipEventRecord(ev[i], s);   \
+            if (_r != hipSuccess) return _r;            \
+        }                                               \
     } while (0)
 
-hipError_t launch_fct_step(const MeshDev &m, const StepArgs &a, hipStream_t s) {
-    const int B = 256;
-    const int cols = cols_for(m.L, B);
-    const dim3 ge((m.E + cols - 1) / cols), gn((m.N + cols - 1) / cols);
-    const bool fct = a.fct;
-#define MARK(i)                                            \
-    do {                                                   \
-        if (a.ev) {                                        \
-            hipError_t _r = hipEventRecord(a.ev[i], s);    \
-            if (_r != hipSuccess) return _r;               \
-        }                                                  \
-    } while (0)
+template <int HOR, bool FCT, int KT, typename S, typename M, typename C>
+static void edge_kernel(dim3 g, hipStream_t s, const MeshCol<M> &m, const StepPtrs<S, C> &p, C ph, int lc) {
+    k_edge<HOR, FCT, KT, S, M, C><<<g, 256, 0, s>>>(m, p, ph, lc);
+}
+
+template <int KT, typename S, typename M, typename C>
+static hipError_t step_kt(const MeshCol<M> &m, const StepPtrs<S, C> &p, const SchemeArgs &sc,
+                          const LaunchCfg &cfg, hipStream_t s, hipEvent_t *ev) {
+    const int L = m.L;
+    auto grid = [&](int n_cols, int lc) { return dim3(grid_for(n_cols, 256), (unsigned)((L + lc - 1) / lc)); };
+    const dim3 ge = grid(m.E, cfg.lc_edge), gl = grid(m.N, cfg.lc_lo), gb = grid(m.N, cfg.lc_bounds),
+               gf = grid(m.N, cfg.lc_final);
+    const C ph = (C)sc.ph, pv = (C)sc.pv;
     MARK(0);
-    switch (a.hor * 2 + (fct ? 1 : 0)) {
-    case HOR_UPW1 * 2 + 0: k_edge<HOR_UPW1, false><<<ge, B, 0, s>>>(m, cols, a.T, a.TAB, a.ph, a.flo, a.adfh); break;
-    case HOR_UPW1 * 2 + 1: k_edge<HOR_UPW1, true><<<ge, B, 0, s>>>(m, cols, a.T, a.TAB, a.ph, a.flo, a.adfh); break;
-    case HOR_MUSCL * 2 + 0: k_edge<HOR_MUSCL, false><<<ge, B, 0, s>>>(m, cols, a.T, a.TAB, a.ph, a.flo, a.adfh); break;
-    case HOR_MUSCL * 2 + 1: k_edge<HOR_MUSCL, true><<<ge, B, 0, s>>>(m, cols, a.T, a.TAB, a.ph, a.flo, a.adfh); break;
-    case HOR_MFCT * 2 + 0: k_edge<HOR_MFCT, false><<<ge, B, 0, s>>>(m, cols, a.T, a.TAB, a.ph, a.flo, a.adfh); break;
-    case HOR_MFCT * 2 + 1: k_edge<HOR_MFCT, true><<<ge, B, 0, s>>>(m, cols, a.T, a.TAB, a.ph, a.flo, a.adfh); break;
+    switch (sc.hor * 2 + (sc.fct ? 1 : 0)) {
+    case HOR_UPW1 * 2 + 0: edge_kernel<HOR_UPW1, false, KT>(ge, s, m, p, ph, cfg.lc_edge); break;
+    case HOR_UPW1 * 2 + 1: edge_kernel<HOR_UPW1, true, KT>(ge, s, m, p, ph, cfg.lc_edge); break;
+    case HOR_MUSCL * 2 + 0: edge_kernel<HOR_MUSCL, false, KT>(ge, s, m, p, ph, cfg.lc_edge); break;
+    case HOR_MUSCL * 2 + 1: edge_kernel<HOR_MUSCL, true, KT>(ge, s, m, p, ph, cfg.lc_edge); break;
+    case HOR_MFCT * 2 + 0: edge_kernel<HOR_MFCT, false, KT>(ge, s, m, p, ph, cfg.lc_edge); break;
+    case HOR_MFCT * 2 + 1: edge_kernel<HOR_MFCT, true, KT>(ge, s, m, p, ph, cfg.lc_edge); break;
     default: return hipErrorInvalidValue;
     }
     LAUNCH_CHECK();
     MARK(1);
-    if (fct) {
-        switch (a.ver) {
-        case VER_UPW1: k_node_lo<VER_UPW1><<<gn, B, 0, s>>>(m, cols, a.T, a.TAB, a.pv, a.dt, a.flo, a.LO, a.adfv); break;
-        case VER_QR4C: k_node_lo<VER_QR4C><<<gn, B, 0, s>>>(m, cols, a.T, a.TAB, a.pv, a.dt, a.flo, a.LO, a.adfv); break;
-        case VER_CDIFF: k_node_lo<VER_CDIFF><<<gn, B, 0, s>>>(m, cols, a.T, a.TAB, a.pv, a.dt, a.flo, a.LO, a.adfv); break;
-        case VER_PPM: k_node_lo<VER_PPM><<<gn, B, 0, s>>>(m, cols, a.T, a.TAB, a.pv, a.dt, a.flo, a.LO, a.adfv); break;
+    if (sc.fct) {
+        switch (sc.ver) {
+        case VER_UPW1: k_lo<VER_UPW1, KT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_QR4C: k_lo<VER_QR4C, KT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_CDIFF: k_lo<VER_CDIFF, KT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_PPM: k_lo<VER_PPM, KT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
         default: return hipErrorInvalidValue;
         }
         LAUNCH_CHECK();
         MARK(2);
-        k_bounds<<<gn, B, 0, s>>>(m, cols, a.T, a.LO, a.adfh, a.adfv, a.dt, a.Rp, a.Rm);
+        k_bounds<KT><<<gb, 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
         LAUNCH_CHECK();
         MARK(3);
-        k_final<true><<<gn, B, 0, s>>>(m, cols, a.T, a.LO, a.adfh, a.adfv, a.Rp, a.Rm, a.dt,
-                                       a.acc_h, a.acc_v, a.dbg_h, a.dbg_v);
+        k_final<true, KT><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
     } else {
-        switch (a.ver) {
-        case VER_UPW1: k_vert<VER_UPW1><<<gn, B, 0, s>>>(m, cols, a.TAB, a.pv, a.dt, a.adfv); break;
-        case VER_QR4C: k_vert<VER_QR4C><<<gn, B, 0, s>>>(m, cols, a.TAB, a.pv, a.dt, a.adfv); break;
-        case VER_CDIFF: k_vert<VER_CDIFF><<<gn, B, 0, s>>>(m, cols, a.TAB, a.pv, a.dt, a.adfv); break;
-        case VER_PPM: k_vert<VER_PPM><<<gn, B, 0, s>>>(m, cols, a.TAB, a.pv, a.dt, a.adfv); break;
+        switch (sc.ver) {
+        case VER_UPW1: k_vert<VER_UPW1, KT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_QR4C: k_vert<VER_QR4C, KT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_CDIFF: k_vert<VER_CDIFF, KT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_PPM: k_vert<VER_PPM, KT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
         default: return hipErrorInvalidValue;
         }
         LAUNCH_CHECK();
         MARK(2);
         MARK(3);
-        k_final<false><<<gn, B, 0, s>>>(m, cols, a.T, nullptr, a.adfh, a.adfv, nullptr, nullptr,
-                                        a.dt, a.acc_h, a.acc_v, nullptr, nullptr);
+        k_final<false, KT><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
     }
     LAUNCH_CHECK();
     MARK(4);
-#undef MARK
     return hipSuccess;
 }
 
-hipError_t launch_gather_rows(const double *in, double *out, const int *map, int rows, int ld,
-                              hipStream_t s) {
-    const size_t total = (size_t)rows * ld;
+template <typename S, typename M, typename C>
+hipError_t launch_step(const MeshCol<M> &m, const StepPtrs<S, C> &p, int kt, const SchemeArgs &sc,
+                       const LaunchCfg &cfg, hipStream_t s, hipEvent_t *ev) {
+    if (cfg.lc_edge < 1 || cfg.lc_lo < 1 || cfg.lc_bounds < 1 || cfg.lc_final < 1) return hipErrorInvalidValue;
+    if (kt == 1) return step_kt<1>(m, p, sc, cfg, s, ev);
+    if (kt == 2) return step_kt<2>(m, p, sc, cfg, s, ev);
+    return hipErrorInvalidValue;
+}
+
+template <typename S>
+hipError_t launch_relayout(const double *ref_in, S *dev_out, const S *dev_in, double *ref_out,
+                           const int *perm, int N, int L, size_t Np, int dir, hipStream_t s) {
+    const size_t total = (size_t)N * L;
     if (total == 0) return hipSuccess;
-    k_gather_rows<<<dim3((unsigned)((total + 255) / 256)), 256, 0, s>>>(in, out, map, rows, ld);
+    k_relayout<S><<<dim3((unsigned)((total + 255) / 256)), 256, 0, s>>>(ref_in, dev_out, dev_in, ref_out,
+                                                                        perm, N, L, Np, dir);
     return hipGetLastError();
 }
 
-hipError_t launch_stats(const MeshDev &m, const double *T, int all_levels, double4 *part,
-                        int nblocks, hipStream_t s) {
-    k_stats_partial<<<nblocks, 256, 0, s>>>(m, T, all_levels, part);
+template <typename S, typename M>
+hipError_t launch_stats(const MeshCol<M> &m, const S *T, int all_levels, double4 *part, int nblocks,
+                        hipStream_t s) {
+    k_stats_partial<S, M><<<nblocks, 256, 0, s>>>(m, T, all_levels, part);
     return hipGetLastError();
 }
+
+// Instantiations: FP64 (bit-exact path) and FP32 arithmetic with FP32 / FP16 / BF16 storage.
+template hipError_t launch_step<double, double, double>(const MeshCol<double> &, const StepPtrs<double, double> &, int, const SchemeArgs &, const LaunchCfg &, hipStream_t, hipEvent_t *);
+template hipError_t launch_step<float, float, float>(const MeshCol<float> &, const StepPtrs<float, float> &, int, const SchemeArgs &, const LaunchCfg &, hipStream_t, hipEvent_t *);
+template hipError_t launch_step<f16_t, float, float>(const MeshCol<float> &, const StepPtrs<f16_t, float> &, int, const SchemeArgs &, const LaunchCfg &, hipStream_t, hipEvent_t *);
+template hipError_t launch_step<bf16_t, float, float>(const MeshCol<float> &, const StepPtrs<bf16_t, float> &, int, const SchemeArgs &, const LaunchCfg &, hipStream_t, hipEvent_t *);
+template hipError_t launch_relayout<double>(const double *, double *, const double *, double *, const int *, int, int, size_t, int, hipStream_t);
+template hipError_t launch_relayout<float>(const double *, float *, const float *, double *, const int *, int, int, size_t, int, hipStream_t);
+template hipError_t launch_relayout<f16_t>(const double *, f16_t *, const f16_t *, double *, const int *, int, int, size_t, int, hipStream_t);
+template hipError_t launch_relayout<bf16_t>(const double *, bf16_t *, const bf16_t *, double *, const int *, int, int, size_t, int, hipStream_t);
+template hipError_t launch_stats<double, double>(const MeshCol<double> &, const double *, int, double4 *, int, hipStream_t);
+template hipError_t launch_stats<float, float>(const MeshCol<float> &, const float *, int, double4 *, int, hipStream_t);
+template hipError_t launch_stats<f16_t, float>(const MeshCol<float> &, const f16_t *, int, double4 *, int, hipStream_t);
+template hipError_t launch_stats<bf16_t, float>(const MeshCol<float> &, const bf16_t *, int, double4 *, int, hipStream_t);
 
 }  // namespace fesomtra

@@ -1,31 +1,35 @@
 // Host-visible launch interface of the tracer advection kernels.
 #pragma once
 
-#include <hip/hip_runtime.h>
+#include "fesomtra_types.hpp"
 
 namespace fesomtra {
 
-struct MeshDev;
-
-// Arguments of one tracer step (device pointers, device numbering).
-struct StepArgs {
-    int hor, ver;        // Hor / Ver codes
-    bool fct;            // limiter "FCT" (else "NONE")
-    double ph, pv, dt;   // tra_adv_ph, tra_adv_pv, time step
-    double *T;           // values (updated in place unless acc_h / acc_v are set)
-    const double *TAB;   // valuesAB (== T for the caller step)
-    double *flo, *adfh;  // (E, L) work
-    double *LO, *Rp, *Rm;  // (N, L) work
-    double *adfv;        // (N, NL) work
-    double *acc_h, *acc_v;  // do_oce_adv_tra accumulators or null
-    double *dbg_h, *dbg_v;  // limited fluxes for inspection or null
-    hipEvent_t *ev;         // optional: 5 events recorded around the 4 stages, or null
+struct SchemeArgs {
+    int hor, ver;       // Hor / Ver codes
+    bool fct;           // limiter "FCT" (else "NONE")
+    double ph, pv, dt;  // tra_adv_ph, tra_adv_pv, time step
 };
 
-hipError_t launch_fct_step(const MeshDev &m, const StepArgs &a, hipStream_t s);
-hipError_t launch_gather_rows(const double *in, double *out, const int *map, int rows, int ld,
-                              hipStream_t s);
-hipError_t launch_stats(const MeshDev &m, const double *T, int all_levels, double4 *part,
-                        int nblocks, hipStream_t s);
+// Levels per thread of each kernel (the column of a node / edge is split into chunks of lc
+// layers, one thread per chunk).
+struct LaunchCfg {
+    int lc_edge = 16, lc_lo = 16, lc_bounds = 16, lc_final = 16;
+};
+
+// One step of kt (1 or 2) tracers sharing one scheme.  ev, when not null, holds 5 events recorded
+// around the 4 stages (edge fluxes, low-order / vertical, bounds, final).
+template <typename S, typename M, typename C>
+hipError_t launch_step(const MeshCol<M> &m, const StepPtrs<S, C> &p, int kt, const SchemeArgs &sc,
+                       const LaunchCfg &cfg, hipStream_t s, hipEvent_t *ev);
+
+// dir 0: reference layout (nl-1, nod2D) float64 -> device layout S; dir 1: back.
+template <typename S>
+hipError_t launch_relayout(const double *ref_in, S *dev_out, const S *dev_in, double *ref_out,
+                           const int *perm, int N, int L, size_t Np, int dir, hipStream_t s);
+
+template <typename S, typename M>
+hipError_t launch_stats(const MeshCol<M> &m, const S *T, int all_levels, double4 *part, int nblocks,
+                        hipStream_t s);
 
 }  // namespace fesomtra

@@ -152,6 +152,21 @@ int tracer_last_elapsed_ms(double *ms);
  * and the number of tracer-steps averaged. */
 int tracer_set_kernel_timing(int enable);
 int tracer_get_kernel_times(double *stage_ms, long *count);
+/* Arithmetic / storage precision (applies to all tracers; values are carried over):
+ * 0 FP64 everywhere (default; bit-identical to the reference dp build),
+ * 1 FP32 flux and reconstruction arithmetic, FP32 mesh and tracer storage,
+ * 2 as 1 with FP16 tracer storage, 3 as 1 with BF16 tracer storage.
+ * In modes 1-3 the nodal flux-divergence sums, the low-order solution, the limiter ratios, the
+ * tracer update and the global sums stay in FP64. */
+int tracer_set_precision(int mode);
+/* Tracers advanced per kernel pass (1 or 2, default 1): consecutive tracers with the same scheme
+ * share one pass, so the mesh and velocities are read once for both. */
+int tracer_set_tracers_per_pass(int kt);
+/* Levels per thread of the four kernels (edge fluxes, low-order / vertical, bounds, final): each
+ * column is split into chunks of that many layers, one thread per chunk.  Defaults come from the
+ * autotuner's table; results do not depend on the choice. */
+int tracer_set_launch_config(int lc_edge, int lc_lo, int lc_bounds, int lc_final);
+int tracer_get_launch_config(int *cfg4);
 /* Last error message (empty if none). */
 const char *tracer_last_error(void);
 /* Library version string. */

@@ -21,9 +21,11 @@ def rel_linf(got, ref):
     return float(np.max(np.abs(got - ref)) / scale)
 
 
-def gpu_run(case, nsteps):
+def gpu_run(case, nsteps, kt=None, lc=None):
     from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
     t = FESOMTracerAdvection()
+    t.lib.tracer_set_tracers_per_pass(kt or 1)
+    t.lib.tracer_set_launch_config(*(lc or (16, 16, 16, 16)))
     t.setup_case(case)
     t.advect_steps(nsteps, case.dt)
     out = t.get_all_values()
@@ -109,3 +111,20 @@ def test_conservation_and_device_tensors():
     st = t.get_statistics()
     assert st["min"] == case.tracers[0].min() and st["max"] == case.tracers[0].max()
     t.cleanup()
+
+
+CHUNK_SCHEMES = [Scheme("MFCT", "QR4C", "FCT", 0.85, 0.75), Scheme("MUSCL", "PPM", "FCT", 0.9, 0.6),
+                 Scheme("UPW1", "CDIFF", "NONE"), Scheme("MFCT", "QR4C", "NONE", 1.0, 1.0)]
+
+
+@pytest.mark.parametrize("lc", [(1, 1, 1, 1), (2, 3, 5, 7), (4, 4, 4, 4), (64, 64, 64, 64)])
+@pytest.mark.parametrize("kt", [1, 2])
+@pytest.mark.parametrize("si", range(len(CHUNK_SCHEMES)))
+def test_level_chunks_and_tracer_pairs(oracle_mod, lc, kt, si):
+    """Results do not depend on how columns are split into level chunks or on pairing tracers."""
+    case = configs.bathymetry_case("bathy12", 12, 13, seed=17, nsteps=2, lev_lo=3,
+                                   scheme=CHUNK_SCHEMES[si], w_amp=3e-4)
+    say(f"case bathy12 scheme {si} lc={lc} kt={kt}")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[2])
+    got = gpu_run(case, 2, kt=kt, lc=lc)
+    assert rel_linf(got, ref[2]) <= RTOL
