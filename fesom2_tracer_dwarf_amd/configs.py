@@ -165,8 +165,12 @@ def bathymetry_case(name: str, n: int, nl: int, seed: int, ntracers: int = 2,
                     permute: bool = True, nsteps: int = 100, umax: float = 0.1,
                     depth_scale: Optional[float] = 500.0, lev_lo: int = 10,
                     lev_hi: Optional[int] = None, L: float = 2.0e6,
-                    scheme: Optional[Scheme] = None, w_amp: float = 0.0) -> Case:
-    """n x n periodic mesh with nl levels stretched 5-250 m and smooth random bathymetry."""
+                    scheme: Optional[Scheme] = None, w_cfl: float = 0.0) -> Case:
+    """n x n periodic mesh with nl levels stretched 5-250 m and smooth random bathymetry.
+
+    ``w_cfl`` > 0 adds an interior vertical velocity whose largest vertical Courant number
+    |w| dt / h_min equals w_cfl (surface and bottom interfaces stay at w = 0).
+    """
     ss = np.random.SeedSequence(seed)
     s_bath, s_perm, s_tr, s_w = ss.spawn(4)
     layer_h = stretched_layers(nl - 1, 5.0, 250.0)
@@ -177,7 +181,9 @@ def bathymetry_case(name: str, n: int, nl: int, seed: int, ntracers: int = 2,
                          seed=int(s_perm.generate_state(1)[0]), permute=permute)
     uv = streamfunction_velocity(mesh, umax, depth_scale)
     w = np.zeros((mesh.nod2D, mesh.nl))
-    if w_amp:
+    dt = cfl_dt(mesh, umax)
+    if w_cfl:
+        w_amp = w_cfl * float(layer_h.min()) / dt
         # Interior-only vertical velocity (surface and bottom interfaces stay 0), smooth in x, y.
         rng = np.random.default_rng(s_w)
         xy = mesh.xy_m
@@ -194,7 +200,7 @@ def bathymetry_case(name: str, n: int, nl: int, seed: int, ntracers: int = 2,
     tracers = np.stack(tr[:ntracers])
     sc = scheme or Scheme()
     return Case(name, mesh, uv, w, w.copy(), w.copy(), tracers,
-                [dataclasses.replace(sc) for _ in range(ntracers)], cfl_dt(mesh, umax), nsteps)
+                [dataclasses.replace(sc) for _ in range(ntracers)], dt, nsteps)
 
 
 def mesh1000(nsteps: int = 100) -> Case:

@@ -45,7 +45,7 @@ def case_hash(case) -> str:
 
 def combo_case(hor, ver, lim):
     return configs.bathymetry_case("combo8", 8, 9, seed=21, nsteps=3, lev_lo=4,
-                                   scheme=Scheme(hor, ver, lim, 0.8, 0.7), w_amp=3e-4)
+                                   scheme=Scheme(hor, ver, lim, 0.8, 0.7), w_cfl=0.3)
 
 
 def analytic_cases():

@@ -54,7 +54,7 @@ SCHEMES = list(itertools.product(["UPW1", "MUSCL", "MFCT"], ["UPW1", "QR4C", "CD
 @pytest.mark.parametrize("hor,ver,lim", SCHEMES)
 def test_schemes_bathymetry_matches_oracle(oracle_mod, hor, ver, lim):
     case = configs.bathymetry_case("bathy10", 10, 12, seed=11, nsteps=3, lev_lo=4,
-                                   scheme=Scheme(hor, ver, lim, 0.85, 0.75), w_amp=3e-4)
+                                   scheme=Scheme(hor, ver, lim, 0.85, 0.75), w_cfl=0.3)
     say(f"case bathy10 {hor}/{ver}/{lim} 3 steps")
     ref, _ = oracle_mod.Oracle(case).run(save_steps=[3])
     got = gpu_run(case, 3)
@@ -67,7 +67,7 @@ def test_dropin_do_oce_adv_tra_matches_oracle(oracle_mod):
     """tracer_do_oce_adv_tra: tendencies, low-order solution and limited fluxes, with a valuesAB
     different from values."""
     from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
-    case = configs.bathymetry_case("bathy16", 16, 14, seed=5, nsteps=1, lev_lo=5, w_amp=2e-4)
+    case = configs.bathymetry_case("bathy16", 16, 14, seed=5, nsteps=1, lev_lo=5, w_cfl=0.3)
     rng = np.random.default_rng(0)
     vals = case.tracers[0]
     vab = vals + 0.01 * rng.standard_normal(vals.shape) * (vals != 0)
@@ -123,8 +123,37 @@ CHUNK_SCHEMES = [Scheme("MFCT", "QR4C", "FCT", 0.85, 0.75), Scheme("MUSCL", "PPM
 def test_level_chunks_and_tracer_pairs(oracle_mod, lc, kt, si):
     """Results do not depend on how columns are split into level chunks or on pairing tracers."""
     case = configs.bathymetry_case("bathy12", 12, 13, seed=17, nsteps=2, lev_lo=3,
-                                   scheme=CHUNK_SCHEMES[si], w_amp=3e-4)
+                                   scheme=CHUNK_SCHEMES[si], w_cfl=0.3)
     say(f"case bathy12 scheme {si} lc={lc} kt={kt}")
     ref, _ = oracle_mod.Oracle(case).run(save_steps=[2])
     got = gpu_run(case, 2, kt=kt, lc=lc)
     assert rel_linf(got, ref[2]) <= RTOL
+
+
+PREC_BOUNDS = {1: 1e-5, 2: 5e-3, 3: 3e-2}  # rel L-inf after 5 steps vs FP64: FP32, FP16, BF16 storage
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3])
+def test_mixed_precision_close_to_fp64(oracle_mod, mode):
+    """FP32 arithmetic with FP32 / FP16 / BF16 storage against the FP64 reference restatement;
+    the volume-weighted sum of the stored field is checked against its initial value."""
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    case = configs.bathymetry_case("bathy20", 20, 16, seed=23, nsteps=5, lev_lo=5, w_cfl=0.3)
+    say(f"case bathy20 precision mode {mode}")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[5])
+    t = FESOMTracerAdvection()
+    t.lib.tracer_set_tracers_per_pass(1)
+    t.lib.tracer_set_launch_config(16, 16, 16, 16)
+    t.lib.tracer_set_precision(mode)
+    t.setup_case(case)
+    v0 = [t.stats(i + 1)["volume_sum"] for i in range(case.ntracers)]
+    t.advect_steps(5, case.dt)
+    got = t.get_all_values()
+    v1 = [t.stats(i + 1)["volume_sum"] for i in range(case.ntracers)]
+    t.lib.tracer_set_precision(0)
+    t.cleanup()
+    err = rel_linf(got, ref[5])
+    drift = max(abs(a - b) / abs(a) for a, b in zip(v0, v1))
+    say(f"  mode {mode}: rel Linf {err:.3e}, volume-sum drift {drift:.3e}")
+    assert err <= PREC_BOUNDS[mode]
+    assert drift <= {1: 1e-6, 2: 1e-3, 3: 1e-2}[mode]

@@ -121,7 +121,7 @@ def test_uniform_tracer_stays_uniform(oracle_mod, hor, ver, lim):
 @pytest.mark.parametrize("hor,ver,lim", ALL_SCHEMES)
 def test_volume_sum_conserved(oracle_mod, hor, ver, lim):
     case = configs.bathymetry_case("cons", 10, 10, seed=8, nsteps=4, lev_lo=4,
-                                   scheme=Scheme(hor, ver, lim, 0.8, 0.9), w_amp=3e-4)
+                                   scheme=Scheme(hor, ver, lim, 0.8, 0.9), w_cfl=0.3)
     m = case.mesh
     vol = m.hnode * m.areasvol[:, :-1]
     wet = configs.wet_mask(m)
@@ -224,7 +224,7 @@ def test_ppm_exact_for_linear_profile(oracle_mod, w_sign):
 
 
 def test_fct_with_upwind_high_order_is_upwind(oracle_mod):
-    base = configs.bathymetry_case("fu", 10, 9, seed=12, nsteps=2, lev_lo=4, w_amp=2e-4)
+    base = configs.bathymetry_case("fu", 10, 9, seed=12, nsteps=2, lev_lo=4, w_cfl=0.3)
     a, _ = oracle_mod.Oracle(with_scheme(base, "UPW1", "UPW1", "FCT")).run(save_steps=[2])
     b, _ = oracle_mod.Oracle(with_scheme(base, "UPW1", "UPW1", "NONE")).run(save_steps=[2])
     assert np.max(np.abs(a[2] - b[2])) <= 1e-13 * np.max(np.abs(b[2]))

@@ -67,7 +67,7 @@ def test_reference_build_reproduces_published_golden():
 @pytest.mark.parametrize("perm", [False, True])
 def test_oracle_equals_live_reference_bathymetry(oracle_mod, tmp_path, perm):
     case = configs.bathymetry_case("live", 14, 16, seed=31, nsteps=4, lev_lo=5, permute=perm,
-                                   scheme=Scheme("MUSCL", "PPM", "FCT", 0.9, 0.6), w_amp=2e-4)
+                                   scheme=Scheme("MUSCL", "PPM", "FCT", 0.9, 0.6), w_cfl=0.3)
     p = bundle.write_bundle(tmp_path / "b", case, save_steps=[4], save_work=True)
     bundle.run_ref(p)
     out, _ = oracle_mod.Oracle(case).run(save_steps=[4])
