@@ -1,16 +1,21 @@
-"""Benchmark: node-level tracer updates/s of the FP64 tracer advection step on one MI355X.
+"""Benchmark: node-level tracer updates/s of the FESOM2 tracer advection step on one MI355X.
 
-Workload (default): 1000x1000 doubly periodic mesh (1M nodes, 2M triangles, 3M edges), 48 levels
-stretched 5-250 m, smooth random bathymetry with 10-48 levels, randomly permuted numbering,
-T and S advected with MFCT / QR4C / FCT in FP64 at CFL 0.5.  One step advances both tracers.
+Headline (default): the plain FP64 step on a 1000x1000 doubly periodic mesh (1M nodes, 2M
+triangles, 3M edges), 48 levels stretched 5-250 m, smooth random bathymetry with 10-48 levels,
+randomly permuted numbering, T and S advected with MFCT / QR4C / FCT at CFL 0.5; one step advances
+both tracers.  Beside it (same JSON line, key "batched") the mixed-precision batched form: 64
+tracers on a 500x500x48 mesh in one interleaved block, FP32 flux arithmetic and storage with FP64
+accumulation, update, limiter ratios and sums.
 
-    python bench.py [--gpus 1] [--steps 100] [--warmup 5]
+    python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--workload mesh1000]
+    python bench.py --workload multimesh32          # 32 uneven meshes, one HIP graph per run
+    python bench.py --precision-study               # 1000-step FP32/FP16/BF16 errors vs FP64
 
 Prints one JSON line.  ``value`` = tracers x wet node-levels x steps / wall time of the timed
-steps (inputs resident in HBM).  ``roofline`` relates the algorithmic bytes of one tracer-step
-(every input read once, the tracer written once, at storage precision) to the mean HIP-event
-duration of the step's kernels; ``traffic`` is the rocprof-measured HBM traffic of the same step
-(profiles/traffic_mesh1000.json).  ``cpu_baseline`` times the CPU restatement of the reference
+steps (inputs resident in HBM).  ``roofline`` relates the algorithmic bytes of one kernel pass
+(every input read once, the tracers written once, at storage precision) to the mean HIP-event
+duration of the pass's kernels; ``traffic`` is the rocprof-measured HBM traffic of the same pass
+(profiles/traffic_<workload>.json).  ``cpu_baseline`` times the CPU restatement of the reference
 on one tracer-step of the same mesh, single-threaded.
 """
 
@@ -18,7 +23,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import os
 import sys
 import time
 from pathlib import Path
@@ -27,10 +31,16 @@ import numpy as np
 
 REPO = Path(__file__).resolve().parent
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+PREC_NAMES = {0: "f64", 1: "f32", 2: "f32/f16-storage", 3: "f32/bf16-storage"}
+PREC_BYTES = {0: (8, 8), 1: (4, 4), 2: (2, 4), 3: (2, 4)}  # (tracer storage, mesh) bytes
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def algorithmic_bytes(mesh, k_tracers: int = 1, s_tr: int = 8, s_mesh: int = 8) -> int:
-    """Bytes one step must move at least: every distinct input once, every output once.
+    """Bytes one pass must move at least: every distinct input once, every output once.
 
     K tracers in and out, per node-layer mesh arrays (hnode, hnode_new, areasvol, Z_3d_n),
     per node-interface arrays (area, w, zbar_3d_n), per element-layer uv(2) and helem, and the
@@ -66,24 +76,175 @@ def cpu_baseline(case, sample_tracer_steps: int = 1):
 def traffic_from_profiles(name: str):
     f = REPO / "profiles" / f"traffic_{name}.json"
     if not f.exists():
-        return None, None
-    d = json.loads(f.read_text())
-    return d.get("hbm_bytes_per_launch"), d
+        return None
+    return json.loads(f.read_text()).get("hbm_bytes_per_launch")
+
+
+def make_case(name: str, nsteps=None):
+    from fesom2_tracer_dwarf_amd import configs
+    if name == "mesh1000":
+        return configs.mesh1000(nsteps or 100)
+    if name == "mesh500x64":
+        return configs.mesh500x64(nsteps or 200)
+    if name == "multimesh32":
+        return configs.multimesh32_union(nsteps or 100)
+    if name == "analytic20":
+        return configs.analytic20(nsteps=nsteps or 10)
+    raise ValueError(name)
+
+
+def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, lc=None,
+        stage_steps: int = 10, compare=None, keep: bool = False, stochastic: bool = True):
+    """Advance `case` on the GPU; returns throughput, device time, per-stage times, drift.
+
+    keep: also return the final values (list per tracer); compare: final values of a reference
+    run of the same steps, against which relative L-inf / L2 over the wet layers are reported."""
+    import torch
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    m = case.mesh
+    t = FESOMTracerAdvection()
+    t.set_precision(prec)
+    t.set_block_width(kb)
+    t.set_graph(graph)
+    t.set_rounding(stochastic)
+    if lc:
+        t.set_launch_config(lc)
+    t.setup_case(case)
+    width = t.block_width()
+    v0 = [t.stats(i + 1)["volume_sum"] for i in range(case.ntracers)]
+    if warmup:
+        t.advect_steps(warmup, case.dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = t.advect_steps(steps, case.dt)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    drift = max(abs(t.stats(i + 1)["volume_sum"] - v0[i]) / abs(v0[i]) for i in range(case.ntracers))
+    kept, err = None, None
+    if keep or compare is not None:
+        from fesom2_tracer_dwarf_amd import configs
+        wet = configs.wet_mask(m)
+        kept = [] if keep else None
+        linf, l2n, l2d, rmax = 0.0, 0.0, 0.0, 0.0
+        for i in range(case.ntracers):
+            v = t.get_tracer_values(i + 1)
+            if keep:
+                kept.append(v)
+            if compare is not None:
+                d = (v - compare[i])[wet]
+                r = compare[i][wet]
+                linf = max(linf, float(np.max(np.abs(d))) / float(np.max(np.abs(r))))
+                l2n += float(np.sum(d * d))
+                l2d += float(np.sum(r * r))
+        if compare is not None:
+            err = {"rel_linf": linf, "rel_l2": float(np.sqrt(l2n / l2d))}
+    # per-stage HIP-event timing: a separate short run outside the timed region (after the
+    # values above were read)
+    t.lib.tracer_set_kernel_timing(1)
+    t.advect_steps(stage_steps, case.dt)
+    stage_ms, passes = t.kernel_times()
+    t.lib.tracer_set_kernel_timing(0)
+    cfg = t.launch_config()
+    t.set_graph(False)
+    t.set_precision(0)
+    t.set_block_width(0)
+    t.cleanup()
+    updates = case.ntracers * m.n_node_levels * steps
+    return {"value": updates / wall, "wall_s": wall, "dev_ms": dev_ms, "stage_ms": stage_ms,
+            "passes_timed": passes, "block_width": width, "drift": drift, "launch_config": cfg,
+            "values": kept, "error": err}
+
+
+def roofline(case, prec: int, r: dict, name: str):
+    s_tr, s_m = PREC_BYTES[prec]
+    k = min(r["block_width"], case.ntracers)
+    B = algorithmic_bytes(case.mesh, k, s_tr, s_m)
+    pass_ms = sum(r["stage_ms"])
+    achieved = B / (pass_ms * 1e-3) / 1e9
+    st = r["stage_ms"]
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(name),
+            "kernel": f"one pass (k_edge + k_lo + k_bounds + k_final) over a block of {k} "
+                      f"tracer(s), {PREC_NAMES[prec]}",
+            "algorithmic_bytes_per_launch": B, "tracers_per_launch": k,
+            "stage_ms_per_launch": {"k_edge": round(st[0], 4), "k_lo": round(st[1], 4),
+                                    "k_bounds": round(st[2], 4), "k_final": round(st[3], 4)}}
+
+
+def describe(case):
+    m = case.mesh
+    sc = case.schemes[0]
+    size = f"{m.nx}x{m.ny}" if m.nx else f"{m.nod2D} nodes"
+    return (f"{case.name}: {size} periodic, nl={m.nl}, {case.ntracers} tracers, "
+            f"{sc.hor}/{sc.ver}/{sc.lim}")
+
+
+def precision_study(args):
+    """FP32 arithmetic with FP32 / FP16 / BF16 storage for many steps against the FP64 GPU path
+    (itself bit-identical to the reference restatement) on the 1000x1000x48 workload."""
+    from fesom2_tracer_dwarf_amd import configs
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    case = configs.mesh1000(args.study_steps)
+    wet = configs.wet_mask(case.mesh)
+    log(f"precision study on {describe(case)}, {args.study_steps} steps")
+    results = {}
+    ref = None
+    for prec, sr in ((0, True), (1, True), (1, False), (2, True), (2, False), (3, True), (3, False)):
+        t = FESOMTracerAdvection()
+        t.set_precision(prec)
+        t.set_rounding(sr)
+        t.setup_case(case)
+        v0 = [t.stats(i + 1)["volume_sum"] for i in range(case.ntracers)]
+        ms = t.advect_steps(args.study_steps, case.dt)
+        vals = t.get_all_values()
+        v1 = [t.stats(i + 1)["volume_sum"] for i in range(case.ntracers)]
+        t.set_precision(0)
+        t.cleanup()
+        if prec == 0:
+            ref = vals
+        per = []
+        for i in range(case.ntracers):
+            d = (vals[i] - ref[i])[wet]
+            rr = ref[i][wet]
+            per.append({"tracer": i + 1,
+                        "rel_l2": float(np.sqrt(np.sum(d * d) / np.sum(rr * rr))),
+                        "rel_linf": float(np.max(np.abs(d)) / np.max(np.abs(rr))),
+                        "volume_sum_rel_drift": abs(v1[i] - v0[i]) / abs(v0[i])})
+        key = PREC_NAMES[prec] + ("" if prec == 0 else (", stochastic rounding" if sr else ", nearest"))
+        results[key] = {"device_ms": ms, "tracers": per}
+        log(f"{key}: {ms:.1f} ms, {per}")
+    print(json.dumps({"study": "precision", "workload": describe(case), "steps": args.study_steps,
+                      "reference": "FP64 GPU path", "results": results}), flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: workload's)")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="mesh1000", choices=["mesh1000", "analytic20"])
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--tracers-per-pass", type=int, default=1,
-                    help="tracers advanced by one kernel pass (1 or 2)")
+    ap.add_argument("--workload", default="mesh1000",
+                    choices=["mesh1000", "mesh500x64", "multimesh32", "analytic20"])
+    ap.add_argument("--precision", type=int, default=None, choices=[0, 1, 2, 3],
+                    help="0 FP64, 1 FP32, 2 FP16 storage, 3 BF16 storage (default: workload's)")
+    ap.add_argument("--block-width", type=int, default=0,
+                    help="tracers interleaved per block and advanced per kernel pass (0: all)")
+    ap.add_argument("--graph", action="store_true", help="replay the timed steps as one HIP graph")
     ap.add_argument("--lc", default="", help="levels per thread: edge,lo,bounds,final")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the mixed-precision batched companion run")
+    ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--no-batched-accuracy", action="store_true",
+                    help="skip the FP64 run of the batched workload used for its error")
+    ap.add_argument("--precision-study", action="store_true")
+    ap.add_argument("--study-steps", type=int, default=1000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (rocprof counter scale)")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU halo design is documented, not built)", file=sys.stderr)
+        sys.exit(2)
     if args.copy_calibration:
         import torch
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
@@ -94,93 +255,80 @@ def main():
         print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
                           "launches": 3}))
         return
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU halo design is documented, not built)", file=sys.stderr)
-        sys.exit(2)
 
     import torch
-    from fesom2_tracer_dwarf_amd import configs
-    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
-
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    case = configs.mesh1000() if args.workload == "mesh1000" else configs.analytic20()
-    m = case.mesh
-    print(f"[bench] workload {case.name}: N={m.nod2D} T={m.elem2D} E={m.edge2D} nl={m.nl} "
-          f"N_L={m.n_node_levels} tracers={case.ntracers}", file=sys.stderr, flush=True)
+    if args.precision_study:
+        precision_study(args)
+        return
 
-    t = FESOMTracerAdvection()
-    t.lib.tracer_set_tracers_per_pass(args.tracers_per_pass)
-    if args.lc:
-        lc = [int(x) for x in args.lc.split(",")]
-        t.lib.tracer_set_launch_config(*lc)
-    t.setup_case(case)
-    v0 = [t.stats(i + 1)["volume_sum"] for i in range(case.ntracers)]
-    t.advect_steps(args.warmup, case.dt)
+    defaults = {"mesh1000": (100, 0, 0, False), "mesh500x64": (200, 1, 64, False),
+                "multimesh32": (100, 1, 4, True), "analytic20": (10, 0, 0, False)}
+    d_steps, d_prec, d_kb, d_graph = defaults[args.workload]
+    steps = args.steps or d_steps
+    prec = d_prec if args.precision is None else args.precision
+    kb = args.block_width or d_kb
+    graph = args.graph or d_graph
+    lc = [int(x) for x in args.lc.split(",")] if args.lc else None
 
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    dev_ms = t.advect_steps(args.steps, case.dt)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    updates = case.ntracers * m.n_node_levels * args.steps
-    value = updates / wall
-    drift = [abs(t.stats(i + 1)["volume_sum"] - v0[i]) / abs(v0[i]) for i in range(case.ntracers)]
-    print(f"[bench] timed {args.steps} steps: wall {wall * 1e3:.2f} ms, device {dev_ms:.2f} ms",
-          file=sys.stderr, flush=True)
+    case = make_case(args.workload)
+    m = case.mesh
+    log(f"workload {describe(case)}: N={m.nod2D} T={m.elem2D} E={m.edge2D} N_L={m.n_node_levels} "
+        f"(generated in {time.perf_counter() - t0:.1f} s)")
+    r = run(case, prec, kb, steps, args.warmup, graph, lc)
+    log(f"timed {steps} steps: wall {r['wall_s'] * 1e3:.2f} ms, device {r['dev_ms']:.2f} ms, "
+        f"{r['value']:.3e} updates/s, stages {r['stage_ms']}")
+    roof = roofline(case, prec, r, args.workload)
 
-    # Per-stage HIP-event timing (separate short run, outside the timed region).
-    import ctypes
-    t.lib.tracer_set_kernel_timing(1)
-    t.advect_steps(min(args.steps, 10), case.dt)
-    stage = (ctypes.c_double * 4)()
-    cnt = ctypes.c_long()
-    t.lib.tracer_get_kernel_times(stage, ctypes.byref(cnt))
-    t.lib.tracer_set_kernel_timing(0)
-    stage_ms = [float(x) for x in stage]
-    step_ms = sum(stage_ms)
-    t.cleanup()
-
-    # one launch sequence advances `kt` tracers: its algorithmic bytes are those of K = kt
-    kt = min(args.tracers_per_pass, case.ntracers)
-    B = algorithmic_bytes(m, kt)
-    achieved = B / (step_ms * 1e-3) / 1e9
-    traffic, prof = traffic_from_profiles(case.name)
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": f"one FP64 step of {kt} tracer(s): k_edge + k_lo + k_bounds + k_final",
-            "algorithmic_bytes_per_launch": B,
-            "tracers_per_launch": kt,
-            "stage_ms_per_launch": {"k_edge": round(stage_ms[0], 4),
-                                         "k_lo": round(stage_ms[1], 4),
-                                         "k_bounds": round(stage_ms[2], 4),
-                                         "k_final": round(stage_ms[3], 4)}}
+    batched = None
+    if args.workload == "mesh1000" and not args.no_batched:
+        bcase = make_case("mesh500x64")
+        log(f"batched companion {describe(bcase)}")
+        ref64 = None
+        if not args.no_batched_accuracy:
+            r64 = run(bcase, 0, 64, args.batched_steps, 0, keep=True)
+            ref64 = r64["values"]
+            log(f"batched FP64 reference: {r64['value']:.3e} updates/s")
+        rb = run(bcase, 1, 64, args.batched_steps, 0, compare=ref64)
+        ref64 = None
+        batched = {"workload": describe(bcase), "precision": "FP32 arithmetic and storage, FP64 "
+                   "accumulation / update / limiter ratios / sums", "block_width": rb["block_width"],
+                   "steps": args.batched_steps, "value": rb["value"], "unit": "node-level updates/s",
+                   "ms_per_step": rb["wall_s"] * 1e3 / args.batched_steps,
+                   "vs_fp64_value": rb["value"] / r["value"],
+                   "volume_sum_rel_drift": rb["drift"],
+                   "error_vs_fp64": rb["error"],
+                   "fp64_same_workload_value": None if args.no_batched_accuracy else r64["value"],
+                   "roofline": roofline(bcase, 1, rb, "mesh500x64")}
+        log(f"batched: {rb['value']:.3e} updates/s ({rb['value'] / r['value']:.2f}x FP64)")
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload in ("mesh1000", "analytic20"):
         cpu = cpu_baseline(case)
 
     out = {
         "metric": "tracer node-level updates/s, FP64 vs mixed-prec batched; % HBM roofline",
-        "value": value,
+        "value": r["value"],
         "unit": "node-level updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": r["wall_s"] * 1e3 / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": PREC_NAMES[prec],
         "data": "synthetic (seeded mesh, bathymetry, tracers and psi velocity)",
-        "config": {"workload": f"{case.name}: {m.nx}x{m.ny} periodic, nl={m.nl}, "
-                               f"{case.ntracers} tracers, MFCT/QR4C/FCT",
-                   "nodes": m.nod2D, "elements": m.elem2D, "edges": m.edge2D,
-                   "node_levels_per_tracer": m.n_node_levels, "tracers": case.ntracers,
+        "config": {"workload": describe(case), "nodes": m.nod2D, "elements": m.elem2D,
+                   "edges": m.edge2D, "node_levels_per_tracer": m.n_node_levels,
+                   "tracers": case.ntracers, "block_width": r["block_width"],
+                   "launch_config": list(r["launch_config"]), "hip_graph": graph,
                    "dt_s": case.dt, "parallelism": "1 GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "volume_sum_rel_drift": max(drift),
+        "volume_sum_rel_drift": r["drift"],
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

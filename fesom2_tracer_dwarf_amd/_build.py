@@ -10,20 +10,24 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 REPO = PKG.parent
 LIB = PKG / "libfesomtra.so"
-SOURCES = ["fesomtra_kernels.hip", "fesomtra_abi.cpp"]
+KERNELS = "fesomtra_kernels.hip"
+ABI = "fesomtra_abi.cpp"
 HEADERS = ["fesomtra_types.hpp", "fesomtra_column.hpp", "fesomtra_kernels.hpp"]
+# The kernel source is compiled once per precision family (FESOMTRA_INST bit), in parallel.
+FAMILIES = {"f64": 1, "f32": 2, "f16": 4, "bf16": 8}
 
 # -ffp-contract=off: the FP64 path keeps every a*b+c unfused, as the reference's CPU build does,
 # so results are bit-identical to it.  Packed FP32 (v_pk_*_f32) instructions are disabled: some of
 # their forms have produced silently wrong results on this platform.
 HIPFLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-x", "hip",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-x", "hip",
     "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
@@ -36,17 +40,39 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> Path:
-    srcs = [str(CSRC / s) for s in SOURCES]
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPFLAGS, f"-I{REPO / 'include'}", f"-I{CSRC}", *srcs, "-o", str(tmp)]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
+def _run(cmds, verbose):
+    procs = []
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                            text=True)))
+    failed = False
+    for cmd, pr in procs:
+        out, _ = pr.communicate()
+        if pr.returncode != 0:
+            sys.stderr.write(out)
+            failed = True
+    if failed:
         raise RuntimeError("hipcc failed building libfesomtra.so")
-    os.replace(tmp, LIB)
+
+
+def build(verbose: bool = False) -> Path:
+    inc = [f"-I{REPO / 'include'}", f"-I{CSRC}"]
+    with tempfile.TemporaryDirectory(prefix="fesomtra_build_") as td:
+        objs, cmds = [], []
+        for fam, bit in FAMILIES.items():
+            obj = str(Path(td) / f"kernels_{fam}.o")
+            cmds.append([hipcc(), *HIPFLAGS, *inc, f"-DFESOMTRA_INST={bit}", "-c",
+                         str(CSRC / KERNELS), "-o", obj])
+            objs.append(obj)
+        obj = str(Path(td) / "abi.o")
+        cmds.append([hipcc(), *HIPFLAGS, *inc, "-c", str(CSRC / ABI), "-o", obj])
+        objs.append(obj)
+        _run(cmds, verbose)
+        tmp = LIB.with_suffix(".so.tmp")
+        _run([[hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp)]], verbose)
+        os.replace(tmp, LIB)
     return LIB
 
 

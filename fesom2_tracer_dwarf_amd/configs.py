@@ -15,6 +15,7 @@ Workloads (sizes fixed here; seeds make every array reproducible bit for bit):
   tracers made of 4 random Gaussian blobs with amplitudes in [0, 1]), 200 steps.
 * ``multimesh32`` 32 independent periodic meshes, nx = ny log-uniform in [64, 512], nl uniform in
   [10, 80], psi amplitude U[0.05, 0.5] m/s, own bathymetry, 4 tracers each, 100 steps.
+  ``union_case`` advances them together as one disconnected mesh with the smallest member dt.
 """
 
 from __future__ import annotations
@@ -25,7 +26,7 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from .mesh import Mesh, periodic_mesh, smooth_bathymetry_levels, stretched_layers
+from .mesh import Mesh, periodic_mesh, smooth_bathymetry_levels, stretched_layers, union_mesh
 
 # Scheme names as the reference spells them (lib/oce_adv_tra_driver.F90:244-282,286).
 HOR_SCHEMES = ("UPW1", "MUSCL", "MFCT")
@@ -116,29 +117,43 @@ def blob_tracer(mesh: Mesh, rng: np.random.Generator, n_blobs: int = 4) -> np.nd
 
 
 def streamfunction_velocity(mesh: Mesh, umax: float,
-                            depth_scale: Optional[float] = None) -> np.ndarray:
+                            depth_scale: Optional[float] = None, taper_layers: float = 4.0) -> np.ndarray:
     """Element velocities (u, v) = (-d psi/dy, d psi/dx) of the P1 interpolant of
-    psi = A sin(2 pi x/Lx) sin(2 pi y/Ly), A chosen so the largest element speed is ``umax``.
+    psi = sin(2 pi x/Lx) sin(2 pi y/Ly), scaled so that the largest element speed is ``umax``.
 
-    The discrete velocity is divergence free on the median-dual cells (the flux through each dual
-    segment telescopes to psi differences), so a uniform tracer stays uniform where all levels are
-    wet.  ``depth_scale`` multiplies layer k by exp(-|Z_k|/depth_scale).
+    Layer k uses psi_k = psi * clip((m(n) - k) / taper_layers, 0, 1), where m(n) is the number of
+    layers in which every element around node n is wet: psi_k vanishes at every node touching a
+    dry element of layer k (the layer's coastline, the no-normal-flow condition) and rises
+    smoothly away from it.  The flux through each median-dual segment telescopes to psi
+    differences, so the transport of every layer is exactly divergence free on the dual cells,
+    also where the bathymetry cuts them: a uniform tracer stays uniform and concentrations stay
+    within their initial range with w = 0.  On a mesh without bathymetry (every column full depth)
+    psi_k = psi.  ``depth_scale`` multiplies layer k by exp(-|Z_k|/depth_scale).
     """
     xy = mesh.xy_m
     psi = np.sin(2 * math.pi * xy[:, 0] / mesh.Lx) * np.sin(2 * math.pi * xy[:, 1] / mesh.Ly)
     en = mesh.elem2D_nodes - 1
     g = mesh.gradient_sca
-    dpsidx = g[:, 0] * psi[en[:, 0]] + g[:, 1] * psi[en[:, 1]] + g[:, 2] * psi[en[:, 2]]
-    dpsidy = g[:, 3] * psi[en[:, 0]] + g[:, 4] * psi[en[:, 1]] + g[:, 5] * psi[en[:, 2]]
-    u = -dpsidy
-    v = dpsidx
-    A = umax / np.max(np.hypot(u, v))
+
+    def curl(p):
+        dpdx = g[:, 0] * p[en[:, 0]] + g[:, 1] * p[en[:, 1]] + g[:, 2] * p[en[:, 2]]
+        dpdy = g[:, 3] * p[en[:, 0]] + g[:, 4] * p[en[:, 1]] + g[:, 5] * p[en[:, 2]]
+        return -dpdy, dpdx
+
     fac = np.ones(mesh.nl - 1)
     if depth_scale is not None:
         fac = np.exp(-np.abs(mesh.Z) / depth_scale)
-    uv = np.empty((mesh.elem2D, mesh.nl - 1, 2), dtype=np.float64)
-    uv[:, :, 0] = (A * u)[:, None] * fac[None, :]
-    uv[:, :, 1] = (A * v)[:, None] * fac[None, :]
+    uv = np.zeros((mesh.elem2D, mesh.nl - 1, 2), dtype=np.float64)
+    full = (mesh.nlevels_nod2D_min - 1).astype(np.float64)
+    flat = bool(np.all(full == mesh.nl - 1))
+    wet_e = mesh.nlevels - 1
+    for k in range(mesh.nl - 1):
+        pk = psi if flat else psi * np.clip((full - k) / taper_layers, 0.0, 1.0)
+        uk, vk = curl(pk)
+        on = wet_e >= k + 1
+        uv[:, k, 0] = np.where(on, uk * fac[k], 0.0)
+        uv[:, k, 1] = np.where(on, vk * fac[k], 0.0)
+    uv *= umax / np.max(np.hypot(uv[:, :, 0], uv[:, :, 1]))
     return uv
 
 
@@ -223,3 +238,27 @@ def multimesh32(nsteps: int = 100, count: int = 32, seed: int = 4,
         cases.append(bathymetry_case(f"multimesh32[{i}]", n, nl, seed=1000 + i, ntracers=4,
                                      nsteps=nsteps, umax=umax, lev_lo=min(10, nl), lev_hi=nl))
     return cases
+
+
+def union_case(cases: Sequence[Case], name: str = "union") -> Case:
+    """Independent cases with the same tracer count and schemes as one case on the disjoint union
+    of their meshes; the time step is the smallest member dt (stable for every member)."""
+    cases = list(cases)
+    ntr = cases[0].ntracers
+    if any(c.ntracers != ntr for c in cases):
+        raise ValueError("union_case: every member needs the same number of tracers")
+    mesh = union_mesh([c.mesh for c in cases])
+    L = mesh.nl - 1
+
+    uv = np.concatenate([np.pad(c.uv, ((0, 0), (0, L - c.uv.shape[1]), (0, 0))) for c in cases])
+    w = np.concatenate([np.pad(c.w, ((0, 0), (0, mesh.nl - c.w.shape[1]))) for c in cases])
+    wi = np.concatenate([np.pad(c.wi, ((0, 0), (0, mesh.nl - c.wi.shape[1]))) for c in cases])
+    we = np.concatenate([np.pad(c.we, ((0, 0), (0, mesh.nl - c.we.shape[1]))) for c in cases])
+    tr = np.concatenate([np.pad(c.tracers, ((0, 0), (0, 0), (0, L - c.tracers.shape[2])))
+                         for c in cases], axis=1)
+    return Case(name, mesh, uv, w, wi, we, tr, [dataclasses.replace(s) for s in cases[0].schemes],
+                min(c.dt for c in cases), cases[0].nsteps)
+
+
+def multimesh32_union(nsteps: int = 100, **kw) -> Case:
+    return union_case(multimesh32(nsteps, **kw), "multimesh32")

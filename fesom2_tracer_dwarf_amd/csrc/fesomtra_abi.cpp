@@ -5,7 +5,9 @@
 // validated, renumbered (nodes along a Hilbert curve, elements and edges after their first
 // node), transposed to the level-major device layout and uploaded to HBM together with the CSR
 // node->edge lists, which keep the reference's edge order so every gathered sum is added in the
-// order of the reference's serial scatter.
+// order of the reference's serial scatter.  Tracers live in blocks of kb interleaved tracers
+// (tracer id i is lane i % kb of block i / kb); one launch sequence advances the lanes of a block
+// that share a scheme.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -51,26 +53,35 @@ struct State {
     std::vector<SchemeSel> schemes;
     bool initialized = false;  // the reference's "tracer_initialized"
     int prec = PREC_F64;
-    int kt_max = 1;
+    int kb_req = 0;  // requested tracer block width (0: automatic)
+    int kb = 1;      // block width of the device storage
     LaunchCfg cfg{};
+    bool use_graph = false;
+    bool stochastic = true;  // stochastic rounding of the update in the single-precision modes
+    uint32_t step_seed = 0;  // advanced once per step
+    uint64_t gen = 0;  // bumped whenever captured launch sequences become stale
 
     // device side
     bool on_device = false;
     hipStream_t stream = nullptr;
     std::vector<int> perm_n, inv_n, perm_t, inv_t, perm_e, inv_e;
     size_t Np = 0, Tp = 0, Ep = 0;
-    MeshCol<double> m64{};
-    MeshCol<float> m32{};
+    MeshCol<double> m64{};  // mesh and velocities stay FP64 in every precision mode
     std::vector<void *> allocs;
-    std::vector<void *> dT, dTAB;  // per tracer, storage type of the precision mode
-    void *flo[kMaxKT] = {}, *adfh[kMaxKT] = {}, *LO[kMaxKT] = {}, *Rp[kMaxKT] = {}, *Rm[kMaxKT] = {};
-    void *adfv[kMaxKT] = {};
+    std::vector<void *> dT, tabbuf;  // per tracer block, storage type of the precision mode
+    std::vector<char> tab_on;        // per block: valuesAB held apart from values
+    void *flo = nullptr, *adfh = nullptr, *LO = nullptr, *Rp = nullptr, *Rm = nullptr, *adfv = nullptr;
     double *acc_h = nullptr, *acc_v = nullptr;
     void *dbg_h = nullptr, *dbg_v = nullptr;
     double *stage = nullptr;  // reference-layout staging buffer (N * L doubles)
     int *d_perm_n = nullptr;
     double4 *part = nullptr;
     bool last_fct = false;
+    int last_lane = 0;
+    hipGraphExec_t gexec = nullptr;  // captured advect_steps sequence
+    int g_nsteps = -1;
+    double g_dt = 0.0;
+    uint64_t g_gen = 0;
     double last_ms = 0.0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool kernel_timing = false;
@@ -132,13 +143,21 @@ int upload(State &s, const T **p, const std::vector<T> &h) {
     return 0;
 }
 
+void drop_graph(State &s) {
+    if (s.gexec) (void)hipGraphExecDestroy(s.gexec);
+    s.gexec = nullptr;
+    s.g_nsteps = -1;
+}
+
 void release_device(State &s) {
-    if (s.stream) hipStreamSynchronize(s.stream);
-    for (void *p : s.allocs) hipFree(p);
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    drop_graph(s);
+    for (void *p : s.allocs) (void)hipFree(p);
     s.allocs.clear();
     s.dT.clear();
-    s.dTAB.clear();
-    for (int i = 0; i < kMaxKT; ++i) s.flo[i] = s.adfh[i] = s.LO[i] = s.Rp[i] = s.Rm[i] = s.adfv[i] = nullptr;
+    s.tabbuf.clear();
+    s.tab_on.clear();
+    s.flo = s.adfh = s.LO = s.Rp = s.Rm = s.adfv = nullptr;
     s.acc_h = s.acc_v = nullptr;
     s.dbg_h = s.dbg_v = nullptr;
     s.stage = nullptr;
@@ -146,7 +165,7 @@ void release_device(State &s) {
     s.part = nullptr;
     s.on_device = false;
     s.m64 = MeshCol<double>{};
-    s.m32 = MeshCol<float>{};
+
 }
 
 int check_dims(const State &s, int nl, int nod2D, const char *who) {
@@ -286,6 +305,15 @@ void derive_nboundary_lay(State &s) {
 
 inline size_t pad64(size_t n) { return (n + 63) / 64 * 64; }
 
+int pow2ceil(int n) {
+    int p = 1;
+    while (p < n) p *= 2;
+    return p;
+}
+// Tracer block width: the requested one, else all tracers in one block (up to kMaxKB).
+int block_width(const State &s) { return s.kb_req > 0 ? s.kb_req : std::min(kMaxKB, pow2ceil(std::max(s.ntr, 1))); }
+int n_blocks(const State &s) { return (s.ntr + s.kb - 1) / s.kb; }
+
 // Device copy of a reference (rows, ld) array, transposed to [ld][rows_padded] in device order.
 template <class M>
 std::vector<M> level_major(const std::vector<double> &src, int ld, const std::vector<int> &perm, size_t pad) {
@@ -352,20 +380,6 @@ int upload_mesh(State &s, MeshCol<M> &m, const std::vector<int> &ints_n1, const 
     if (upload(s, &m.ecd, rows_of<M>(s.ecd, 4, s.perm_e))) return -1;
     if (upload(s, &m.edxdy, rows_of<M>(s.edxdy, 2, s.perm_e))) return -1;
     return 0;
-}
-
-template <class M>
-void copy_int_mesh(MeshCol<M> &dst, const MeshCol<double> &src) {
-    dst.ne_ptr = src.ne_ptr;
-    dst.ne_e2 = src.ne_e2;
-    dst.ne_top = src.ne_top;
-    dst.ne_other = src.ne_other;
-    dst.nie = src.nie;
-    dst.nie_num = src.nie_num;
-    dst.enodes = src.enodes;
-    dst.nlev_e = src.nlev_e;
-    dst.edge_ne = src.edge_ne;
-    dst.updn = src.updn;
 }
 
 int build_device(State &s) {
@@ -486,41 +500,40 @@ int build_device(State &s) {
             upload(s, &m.ne_other, oth))
             return -1;
     }
-    if (s.prec == PREC_F64) {
-        if (upload_mesh<double>(s, s.m64, i_nlev, i_nmin, i_nbl)) return -1;
-    } else {
-        copy_int_mesh(s.m32, s.m64);
-        if (upload_mesh<float>(s, s.m32, i_nlev, i_nmin, i_nbl)) return -1;
-    }
+    if (upload_mesh<double>(s, s.m64, i_nlev, i_nmin, i_nbl)) return -1;
+    if (dalloc(s, &s.m64.q, (size_t)L * s.Ep)) return -1;
     {
         const int *p;
         if (upload(s, &p, s.perm_n)) return -1;
         s.d_perm_n = const_cast<int *>(p);
     }
-    // work arrays
-    const size_t cb = compute_bytes(s.prec), sb = storage_bytes(s.prec);
+    // work arrays, interleaved by the block width
+    s.kb = block_width(s);
+    const size_t cb = compute_bytes(s.prec), sb = storage_bytes(s.prec), kb = s.kb;
     const size_t NLn = (size_t)L * s.Np, ELn = (size_t)L * s.Ep, NIn = (size_t)nl * s.Np;
-    for (int i = 0; i < kMaxKT; ++i) {
-        if (dalloc_bytes(s, &s.flo[i], ELn * cb) || dalloc_bytes(s, &s.adfh[i], ELn * cb) ||
-            dalloc_bytes(s, &s.LO[i], NLn * cb) || dalloc_bytes(s, &s.Rp[i], NLn * cb) ||
-            dalloc_bytes(s, &s.Rm[i], NLn * cb) || dalloc_bytes(s, &s.adfv[i], NIn * cb))
-            return -1;
-    }
+    // the upwind edge fluxes are stored only by the FP64 path (single precision recomputes them)
+    if ((s.prec == PREC_F64 && dalloc_bytes(s, &s.flo, ELn * kb * 8)) || dalloc_bytes(s, &s.adfh, ELn * kb * cb) ||
+        dalloc_bytes(s, &s.LO, NLn * kb * cb) || dalloc_bytes(s, &s.Rp, NLn * kb * cb) ||
+        dalloc_bytes(s, &s.Rm, NLn * kb * cb) || dalloc_bytes(s, &s.adfv, NIn * kb * cb))
+        return -1;
     if (dalloc(s, &s.part, (size_t)kStatsBlocks) || dalloc(s, &s.stage, (size_t)N * L)) return -1;
-    // tracers
-    s.dT.assign(s.ntr, nullptr);
-    s.dTAB.assign(s.ntr, nullptr);
-    for (int i = 0; i < s.ntr; ++i) {
-        if (dalloc_bytes(s, &s.dT[i], NLn * sb)) return -1;
-        HIPCHK(hipMemset(s.dT[i], 0, NLn * sb));
+    // tracer blocks (unused lanes of the last block stay zero and are never advanced)
+    const int nb = n_blocks(s);
+    s.dT.assign(nb, nullptr);
+    s.tabbuf.assign(nb, nullptr);
+    s.tab_on.assign(nb, 0);
+    for (int b = 0; b < nb; ++b) {
+        if (dalloc_bytes(s, &s.dT[b], NLn * kb * sb)) return -1;
+        HIPCHK(hipMemset(s.dT[b], 0, NLn * kb * sb));
     }
     HIPCHK(hipDeviceSynchronize());
     s.on_device = true;
+    s.gen++;
     return 0;
 }
 
-// reference-layout host values -> device tracer storage (through the staging buffer)
-int to_device(State &s, const double *host_or_dev, bool is_device, void *dst) {
+// reference-layout host values -> lane of a device tracer block (through the staging buffer)
+int to_device(State &s, const double *host_or_dev, bool is_device, void *dst, int lane) {
     const int L = s.nl - 1;
     const double *src = host_or_dev;
     if (!is_device) {
@@ -529,26 +542,26 @@ int to_device(State &s, const double *host_or_dev, bool is_device, void *dst) {
     }
     hipError_t e;
     switch (s.prec) {
-    case PREC_F64: e = launch_relayout<double>(src, (double *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, 0, s.stream); break;
-    case PREC_F32: e = launch_relayout<float>(src, (float *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, 0, s.stream); break;
-    case PREC_F32_F16: e = launch_relayout<f16_t>(src, (f16_t *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, 0, s.stream); break;
-    default: e = launch_relayout<bf16_t>(src, (bf16_t *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, 0, s.stream); break;
+    case PREC_F64: e = launch_relayout<double>(src, (double *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 0, s.stream); break;
+    case PREC_F32: e = launch_relayout<float>(src, (float *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 0, s.stream); break;
+    case PREC_F32_F16: e = launch_relayout<f16_t>(src, (f16_t *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 0, s.stream); break;
+    default: e = launch_relayout<bf16_t>(src, (bf16_t *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 0, s.stream); break;
     }
     if (e != hipSuccess) return fail("relayout launch failed: %s", hipGetErrorString(e));
     HIPCHK(hipStreamSynchronize(s.stream));
     return 0;
 }
 
-// device tracer storage -> reference layout (host or device destination)
-int from_device(State &s, const void *src, double *out, bool out_is_device) {
+// lane of a device tracer block -> reference layout (host or device destination)
+int from_device(State &s, const void *src, int lane, double *out, bool out_is_device) {
     const int L = s.nl - 1;
     double *dst = out_is_device ? out : s.stage;
     hipError_t e;
     switch (s.prec) {
-    case PREC_F64: e = launch_relayout<double>(nullptr, nullptr, (const double *)src, dst, s.d_perm_n, s.N, L, s.Np, 1, s.stream); break;
-    case PREC_F32: e = launch_relayout<float>(nullptr, nullptr, (const float *)src, dst, s.d_perm_n, s.N, L, s.Np, 1, s.stream); break;
-    case PREC_F32_F16: e = launch_relayout<f16_t>(nullptr, nullptr, (const f16_t *)src, dst, s.d_perm_n, s.N, L, s.Np, 1, s.stream); break;
-    default: e = launch_relayout<bf16_t>(nullptr, nullptr, (const bf16_t *)src, dst, s.d_perm_n, s.N, L, s.Np, 1, s.stream); break;
+    case PREC_F64: e = launch_relayout<double>(nullptr, nullptr, (const double *)src, dst, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 1, s.stream); break;
+    case PREC_F32: e = launch_relayout<float>(nullptr, nullptr, (const float *)src, dst, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 1, s.stream); break;
+    case PREC_F32_F16: e = launch_relayout<f16_t>(nullptr, nullptr, (const f16_t *)src, dst, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 1, s.stream); break;
+    default: e = launch_relayout<bf16_t>(nullptr, nullptr, (const bf16_t *)src, dst, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 1, s.stream); break;
     }
     if (e != hipSuccess) return fail("relayout launch failed: %s", hipGetErrorString(e));
     if (!out_is_device)
@@ -562,7 +575,15 @@ int ensure_device(State &s) {
     if (s.on_device) return 0;
     if (build_device(s)) return -1;
     for (int i = 0; i < s.ntr; ++i)
-        if (to_device(s, s.values[i].data(), false, s.dT[i])) return -1;
+        if (to_device(s, s.values[i].data(), false, s.dT[i / s.kb], i % s.kb)) return -1;
+    return 0;
+}
+
+// Values of every tracer back to the host copies (before the device storage is rebuilt).
+int pull_values(State &s) {
+    if (!s.on_device) return 0;
+    for (int i = 0; i < s.ntr; ++i)
+        if (from_device(s, s.dT[i / s.kb], i % s.kb, s.values[i].data(), false)) return -1;
     return 0;
 }
 
@@ -580,69 +601,131 @@ int ensure_dropin_buffers(State &s) {
 }
 
 template <class S_, class C_>
-StepPtrs<S_, C_> make_ptrs(State &s, const int *ids, int kt, bool dropin) {
+StepPtrs<S_, C_> make_ptrs(State &s, int blk, unsigned long long mask, bool dropin, uint32_t seed) {
     StepPtrs<S_, C_> p{};
-    for (int t = 0; t < kt; ++t) {
-        const int i = ids[t];
-        p.T[t] = static_cast<S_ *>(s.dT[i]);
-        p.TAB[t] = static_cast<const S_ *>(s.dTAB[i] ? s.dTAB[i] : s.dT[i]);
-        p.flo[t] = static_cast<C_ *>(s.flo[t]);
-        p.adfh[t] = static_cast<C_ *>(s.adfh[t]);
-        p.LO[t] = static_cast<C_ *>(s.LO[t]);
-        p.Rp[t] = static_cast<C_ *>(s.Rp[t]);
-        p.Rm[t] = static_cast<C_ *>(s.Rm[t]);
-        p.adfv[t] = static_cast<C_ *>(s.adfv[t]);
-        if (dropin) {
-            p.acc_h[t] = s.acc_h;
-            p.acc_v[t] = s.acc_v;
-            p.dbg_h[t] = static_cast<C_ *>(s.dbg_h);
-            p.dbg_v[t] = static_cast<C_ *>(s.dbg_v);
-        }
+    p.tid0 = blk * s.kb;
+    p.seed = (s.prec != PREC_F64 && s.stochastic) ? seed : 0u;
+    p.T = static_cast<S_ *>(s.dT[blk]);
+    p.TAB = static_cast<const S_ *>(s.tab_on[blk] ? s.tabbuf[blk] : s.dT[blk]);
+    p.flo = static_cast<double *>(s.flo);
+    p.adfh = static_cast<C_ *>(s.adfh);
+    p.LO = static_cast<C_ *>(s.LO);
+    p.Rp = static_cast<C_ *>(s.Rp);
+    p.Rm = static_cast<C_ *>(s.Rm);
+    p.adfv = static_cast<C_ *>(s.adfv);
+    p.mask = mask;
+    if (dropin) {
+        p.acc_h = s.acc_h;
+        p.acc_v = s.acc_v;
+        p.dbg_h = static_cast<C_ *>(s.dbg_h);
+        p.dbg_v = static_cast<C_ *>(s.dbg_v);
     }
     return p;
 }
 
-hipError_t launch_group(State &s, const int *ids, int kt, double dt, bool dropin, hipEvent_t *ev) {
-    const SchemeSel &sc = s.schemes[ids[0]];
+hipError_t launch_block(State &s, int blk, unsigned long long mask, const SchemeSel &sc, double dt, bool dropin,
+                        hipEvent_t *ev, uint32_t seed = 0) {
     SchemeArgs a{sc.hor, sc.ver, sc.fct, sc.ph, sc.pv, dt};
     switch (s.prec) {
-    case PREC_F64: return launch_step<double, double, double>(s.m64, make_ptrs<double, double>(s, ids, kt, dropin), kt, a, s.cfg, s.stream, ev);
-    case PREC_F32: return launch_step<float, float, float>(s.m32, make_ptrs<float, float>(s, ids, kt, dropin), kt, a, s.cfg, s.stream, ev);
-    case PREC_F32_F16: return launch_step<f16_t, float, float>(s.m32, make_ptrs<f16_t, float>(s, ids, kt, dropin), kt, a, s.cfg, s.stream, ev);
-    default: return launch_step<bf16_t, float, float>(s.m32, make_ptrs<bf16_t, float>(s, ids, kt, dropin), kt, a, s.cfg, s.stream, ev);
+    case PREC_F64: return launch_step<double, double, double>(s.m64, make_ptrs<double, double>(s, blk, mask, dropin, seed), s.kb, a, s.cfg, s.stream, ev);
+    case PREC_F32: return launch_step<float, double, float>(s.m64, make_ptrs<float, float>(s, blk, mask, dropin, seed), s.kb, a, s.cfg, s.stream, ev);
+    case PREC_F32_F16: return launch_step<f16_t, double, float>(s.m64, make_ptrs<f16_t, float>(s, blk, mask, dropin, seed), s.kb, a, s.cfg, s.stream, ev);
+    default: return launch_step<bf16_t, double, float>(s.m64, make_ptrs<bf16_t, float>(s, blk, mask, dropin, seed), s.kb, a, s.cfg, s.stream, ev);
     }
 }
 
-// Groups of up to kt_max consecutive tracers with identical schemes, one launch sequence each.
-std::vector<std::vector<int>> tracer_groups(const State &s) {
-    std::vector<std::vector<int>> gr;
-    for (int i = 0; i < s.ntr; ++i) {
-        if (!gr.empty() && (int)gr.back().size() < s.kt_max && s.schemes[gr.back()[0]] == s.schemes[i])
-            gr.back().push_back(i);
-        else
-            gr.push_back({i});
+uint32_t next_seed(State &s) {
+    if (++s.step_seed == 0u) s.step_seed = 1u;
+    return s.step_seed;
+}
+
+// One launch sequence per (block, scheme): the lanes of a block that share a scheme.
+struct Launch {
+    int blk;
+    unsigned long long mask;
+    SchemeSel sc;
+};
+std::vector<Launch> launch_plan(const State &s) {
+    std::vector<Launch> plan;
+    for (int b = 0; b < n_blocks(s); ++b) {
+        const size_t first = plan.size();
+        for (int l = 0; l < s.kb && b * s.kb + l < s.ntr; ++l) {
+            const SchemeSel &sc = s.schemes[b * s.kb + l];
+            size_t i = first;
+            while (i < plan.size() && !(plan[i].sc == sc)) ++i;
+            if (i == plan.size()) plan.push_back(Launch{b, 0ull, sc});
+            plan[i].mask |= 1ull << l;
+        }
     }
-    return gr;
+    return plan;
+}
+
+// Captures nsteps of the launch plan into one executable graph (kept until the state changes).
+int capture_graph(State &s, const std::vector<Launch> &plan, int nsteps, double dt) {
+    drop_graph(s);
+    HIPCHK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+    hipError_t err = hipSuccess;
+    for (int st = 0; st < nsteps && err == hipSuccess; ++st) {
+        err = launch_qflux(s.m64, s.stream);
+        const uint32_t seed = next_seed(s);
+        for (const Launch &l : plan) {
+            if (err != hipSuccess) break;
+            err = launch_block(s, l.blk, l.mask, l.sc, dt, false, nullptr, seed);
+        }
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s.stream, &graph);
+    if (err != hipSuccess || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail("graph capture failed: %s", hipGetErrorString(err != hipSuccess ? err : ec));
+    }
+    const hipError_t ei = hipGraphInstantiate(&s.gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        s.gexec = nullptr;
+        return fail("hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+    }
+    s.g_nsteps = nsteps;
+    s.g_dt = dt;
+    s.g_gen = s.gen;
+    return 0;
 }
 
 int advect(State &s, int nsteps, double dt) {
     if (ensure_device(s)) return -1;
     if (s.ntr == 0) return fail("no tracers allocated (tracer_allocate_tracers)");
-    for (int i = 0; i < s.ntr; ++i) s.dTAB[i] = nullptr;  // valuesAB := values
-    const auto groups = tracer_groups(s);
+    std::fill(s.tab_on.begin(), s.tab_on.end(), 0);  // valuesAB := values
+    const std::vector<Launch> plan = launch_plan(s);
+    if (!plan.empty()) s.last_fct = plan.back().sc.fct;
+    if (s.use_graph && !s.kernel_timing && nsteps > 0) {
+        if (!s.gexec || s.g_nsteps != nsteps || s.g_dt != dt || s.g_gen != s.gen)
+            if (capture_graph(s, plan, nsteps, dt)) return -1;
+        HIPCHK(hipEventRecord(s.ev0, s.stream));
+        HIPCHK(hipGraphLaunch(s.gexec, s.stream));
+        HIPCHK(hipEventRecord(s.ev1, s.stream));
+        HIPCHK(hipStreamSynchronize(s.stream));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
+        s.last_ms = ms;
+        return 0;
+    }
     std::vector<hipEvent_t> evs;
     if (s.kernel_timing) {
-        evs.resize((size_t)nsteps * groups.size() * 5);
+        evs.resize((size_t)nsteps * plan.size() * 5);
         for (auto &e : evs) HIPCHK(hipEventCreate(&e));
     }
     HIPCHK(hipEventRecord(s.ev0, s.stream));
-    for (int st = 0; st < nsteps; ++st)
-        for (size_t gi = 0; gi < groups.size(); ++gi) {
-            hipEvent_t *ev = s.kernel_timing ? &evs[((size_t)st * groups.size() + gi) * 5] : nullptr;
-            hipError_t e = launch_group(s, groups[gi].data(), (int)groups[gi].size(), dt, false, ev);
+    for (int st = 0; st < nsteps; ++st) {
+        // edge transport of the velocity, shared by every tracer pass of the step
+        hipError_t e = launch_qflux(s.m64, s.stream);
+        if (e != hipSuccess) return fail("kernel launch failed: %s", hipGetErrorString(e));
+        const uint32_t seed = next_seed(s);
+        for (size_t gi = 0; gi < plan.size(); ++gi) {
+            hipEvent_t *ev = s.kernel_timing ? &evs[((size_t)st * plan.size() + gi) * 5] : nullptr;
+            e = launch_block(s, plan[gi].blk, plan[gi].mask, plan[gi].sc, dt, false, ev, seed);
             if (e != hipSuccess) return fail("kernel launch failed: %s", hipGetErrorString(e));
-            s.last_fct = s.schemes[groups[gi][0]].fct;
         }
+    }
     HIPCHK(hipEventRecord(s.ev1, s.stream));
     HIPCHK(hipStreamSynchronize(s.stream));
     float ms = 0.f;
@@ -656,17 +739,19 @@ int advect(State &s, int nsteps, double dt) {
                 s.stage_ms[k] += d;
             }
         s.stage_count += (long)(evs.size() / 5);
-        for (auto &e : evs) hipEventDestroy(e);
+        for (auto &e : evs) (void)hipEventDestroy(e);
     }
     return 0;
 }
 
-// Device work array (level-major, device order, compute precision) -> reference layout host.
-int download_work(State &s, const void *d, int ld, size_t pad, const std::vector<int> &perm, double *out) {
-    const size_t cb = compute_bytes(s.prec);
+// Lane `lane` of a device work array [ld][pad][kb] (device order, element size cb) -> reference
+// layout host.
+int download_work(State &s, const void *d, int ld, size_t pad, const std::vector<int> &perm, double *out,
+                  size_t cb, int kb, int lane) {
     std::vector<char> h((size_t)ld * pad * cb);
     HIPCHK(hipStreamSynchronize(s.stream));
-    HIPCHK(hipMemcpy(h.data(), d, h.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy2D(h.data(), cb, static_cast<const char *>(d) + (size_t)lane * cb, (size_t)kb * cb, cb,
+                       (size_t)ld * pad, hipMemcpyDeviceToHost));
     for (size_t r = 0; r < perm.size(); ++r)
         for (int k = 0; k < ld; ++k) {
             const size_t i = (size_t)k * pad + r;
@@ -678,13 +763,14 @@ int download_work(State &s, const void *d, int ld, size_t pad, const std::vector
 int stats(State &s, int id, int all_levels, double *mn, double *mx, double *sm, double *vs) {
     if (ensure_device(s)) return -1;
     if (id < 1 || id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", id, s.ntr);
-    const void *T = s.dT[id - 1];
+    const void *T = s.dT[(id - 1) / s.kb];
+    const int kb = s.kb, lane = (id - 1) % s.kb;
     hipError_t e;
     switch (s.prec) {
-    case PREC_F64: e = launch_stats<double, double>(s.m64, (const double *)T, all_levels, s.part, kStatsBlocks, s.stream); break;
-    case PREC_F32: e = launch_stats<float, float>(s.m32, (const float *)T, all_levels, s.part, kStatsBlocks, s.stream); break;
-    case PREC_F32_F16: e = launch_stats<f16_t, float>(s.m32, (const f16_t *)T, all_levels, s.part, kStatsBlocks, s.stream); break;
-    default: e = launch_stats<bf16_t, float>(s.m32, (const bf16_t *)T, all_levels, s.part, kStatsBlocks, s.stream); break;
+    case PREC_F64: e = launch_stats<double, double>(s.m64, (const double *)T, kb, lane, all_levels, s.part, kStatsBlocks, s.stream); break;
+    case PREC_F32: e = launch_stats<float, double>(s.m64, (const float *)T, kb, lane, all_levels, s.part, kStatsBlocks, s.stream); break;
+    case PREC_F32_F16: e = launch_stats<f16_t, double>(s.m64, (const f16_t *)T, kb, lane, all_levels, s.part, kStatsBlocks, s.stream); break;
+    default: e = launch_stats<bf16_t, double>(s.m64, (const bf16_t *)T, kb, lane, all_levels, s.part, kStatsBlocks, s.stream); break;
     }
     if (e != hipSuccess) return fail("stats launch failed: %s", hipGetErrorString(e));
     std::vector<double4> h(kStatsBlocks);
@@ -770,7 +856,7 @@ int tracer_get_values_by_id(int tracer_id, double *values, int nz, int nn) {
     if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
     if (nz != s.nl - 1 || nn != s.N) return fail("tracer_get_values: expected (%d, %d), got (%d, %d)", s.nl - 1, s.N, nz, nn);
     if (ensure_device(s)) return -1;
-    return from_device(s, s.dT[tracer_id - 1], values, false);
+    return from_device(s, s.dT[(tracer_id - 1) / s.kb], (tracer_id - 1) % s.kb, values, false);
 }
 
 int tracer_get_values(double *values, int nz, int nn) { return tracer_get_values_by_id(1, values, nz, nn); }
@@ -788,9 +874,9 @@ int tracer_run_workflow(const char *restart_dir, int nsteps, double dt) {
 int tracer_cleanup(void) {
     if (g) {
         release_device(*g);
-        if (g->ev0) hipEventDestroy(g->ev0);
-        if (g->ev1) hipEventDestroy(g->ev1);
-        if (g->stream) hipStreamDestroy(g->stream);
+        if (g->ev0) (void)hipEventDestroy(g->ev0);
+        if (g->ev1) (void)hipEventDestroy(g->ev1);
+        if (g->stream) (void)hipStreamDestroy(g->stream);
         delete g;
         g = nullptr;
     }
@@ -805,7 +891,9 @@ int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D) {
     release_device(s);
     const hipStream_t keep = s.stream;
     hipEvent_t k0 = s.ev0, k1 = s.ev1;
-    const int prec = s.prec, ktm = s.kt_max;
+    const int prec = s.prec, kbr = s.kb_req;
+    const bool graph = s.use_graph, sr = s.stochastic;
+    const uint64_t gen = s.gen;
     const LaunchCfg cfg = s.cfg;
     s = State();
     s.cfg = cfg;
@@ -813,7 +901,10 @@ int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D) {
     s.ev0 = k0;
     s.ev1 = k1;
     s.prec = prec;
-    s.kt_max = ktm;
+    s.kb_req = kbr;
+    s.use_graph = graph;
+    s.stochastic = sr;
+    s.gen = gen + 1;
     s.nl = nl;
     s.N = nod2D;
     s.E = edge2D;
@@ -880,9 +971,10 @@ int tracer_set_values(int tracer_id, const double *values, int nl, int nod2D) {
     if (check_dims(s, nl, nod2D, "tracer_set_values")) return -1;
     std::vector<double> &v = s.values[tracer_id - 1];
     v.assign(values, values + (size_t)(nl - 1) * nod2D);
-    if (s.on_device) {
-        if (to_device(s, v.data(), false, s.dT[tracer_id - 1])) return -1;
-        s.dTAB[tracer_id - 1] = nullptr;
+    if (s.on_device) {  // values, and valuesAB where held apart (lib/tracer_c_interface.F90:613-648)
+        const int b = (tracer_id - 1) / s.kb, lane = (tracer_id - 1) % s.kb;
+        if (to_device(s, v.data(), false, s.dT[b], lane)) return -1;
+        if (s.tab_on[b] && to_device(s, v.data(), false, s.tabbuf[b], lane)) return -1;
     }
     return 0;
 }
@@ -892,11 +984,14 @@ int tracer_set_values_ab(int tracer_id, const double *values, int nl, int nod2D)
     if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
     if (check_dims(s, nl, nod2D, "tracer_set_values_ab")) return -1;
     if (ensure_device(s)) return -1;
-    void *p = s.dTAB[tracer_id - 1];
-    if (!p && dalloc_bytes(s, &p, (size_t)(nl - 1) * s.Np * storage_bytes(s.prec))) return -1;
-    if (to_device(s, values, false, p)) return -1;
-    s.dTAB[tracer_id - 1] = p;
-    return 0;
+    const int b = (tracer_id - 1) / s.kb, lane = (tracer_id - 1) % s.kb;
+    const size_t bytes = (size_t)(nl - 1) * s.Np * s.kb * storage_bytes(s.prec);
+    if (!s.tabbuf[b] && dalloc_bytes(s, &s.tabbuf[b], bytes)) return -1;
+    if (!s.tab_on[b]) {  // the block's other lanes keep valuesAB = values
+        HIPCHK(hipMemcpyAsync(s.tabbuf[b], s.dT[b], bytes, hipMemcpyDeviceToDevice, s.stream));
+        s.tab_on[b] = 1;
+    }
+    return to_device(s, values, false, s.tabbuf[b], lane);
 }
 
 int tracer_set_velocity_zero(int nl, int nod2D, int edge2D) {
@@ -1117,6 +1212,7 @@ int tracer_set_scheme(int tracer_id, const char *hor, const char *ver, const cha
     if ((sc.hor == HOR_MUSCL || sc.hor == HOR_MFCT) && !s.have_muscl)
         return fail("tracer %d: %s needs edge_up_dn_tri (tracer_set_muscl_support)", tracer_id, hor);
     s.schemes[tracer_id - 1] = sc;
+    s.gen++;
     return 0;
 }
 
@@ -1126,19 +1222,50 @@ int tracer_set_precision(int mode) {
     if (s.prec == mode) return 0;
     if (s.on_device) {
         // keep the current values: read them back at the old precision, rebuild at the new one
-        for (int i = 0; i < s.ntr; ++i)
-            if (from_device(s, s.dT[i], s.values[i].data(), false)) return -1;
+        if (pull_values(s)) return -1;
         release_device(s);
     }
     s.prec = mode;
+    s.gen++;
     if (s.initialized) return ensure_device(s);
     return 0;
 }
 
-int tracer_set_tracers_per_pass(int kt) {
+int tracer_set_tracers_per_pass(int kb) {
     State &s = S();
-    if (kt < 1 || kt > kMaxKT) return fail("tracer_set_tracers_per_pass: %d outside [1, %d]", kt, kMaxKT);
-    s.kt_max = kt;
+    if (kb != 0 && (kb < 1 || kb > kMaxKB || (kb & (kb - 1))))
+        return fail("tracer_set_tracers_per_pass: %d is not 0 (automatic) or a power of two in [1, %d]", kb, kMaxKB);
+    const int old = s.kb_req;
+    s.kb_req = kb;
+    if (s.on_device && block_width(s) != s.kb) {  // re-interleave the stored values
+        const int want = s.kb_req;
+        s.kb_req = old;
+        if (pull_values(s)) return -1;
+        s.kb_req = want;
+        release_device(s);
+        if (ensure_device(s)) return -1;
+    }
+    s.gen++;
+    return 0;
+}
+
+int tracer_get_tracers_per_pass(int *kb) {
+    const State &s = S();
+    *kb = s.on_device ? s.kb : block_width(s);
+    return 0;
+}
+
+int tracer_set_rounding(int stochastic) {
+    State &s = S();
+    s.stochastic = stochastic != 0;
+    s.gen++;
+    return 0;
+}
+
+int tracer_set_graph(int enable) {
+    State &s = S();
+    s.use_graph = enable != 0;
+    if (!s.use_graph) drop_graph(s);
     return 0;
 }
 
@@ -1150,6 +1277,7 @@ int tracer_set_launch_config(int lc_edge, int lc_lo, int lc_bounds, int lc_final
     s.cfg.lc_lo = lc_lo;
     s.cfg.lc_bounds = lc_bounds;
     s.cfg.lc_final = lc_final;
+    s.gen++;
     return 0;
 }
 
@@ -1177,18 +1305,20 @@ int tracer_do_oce_adv_tra(double dt, int tracer_id) {
     if (!s.initialized) return fail("tracer not initialized.");
     if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
     if (ensure_device(s) || ensure_dropin_buffers(s)) return -1;
-    const size_t cb = compute_bytes(s.prec);
+    const size_t cb = compute_bytes(s.prec), kb = s.kb;
     const size_t NLn = (size_t)(s.nl - 1) * s.Np, ELn = (size_t)(s.nl - 1) * s.Ep, NIn = (size_t)s.nl * s.Np;
     // the reference zeroes its flux and low-order work arrays at every call
     HIPCHK(hipMemsetAsync(s.dbg_h, 0, ELn * cb, s.stream));
     HIPCHK(hipMemsetAsync(s.dbg_v, 0, NIn * cb, s.stream));
-    HIPCHK(hipMemsetAsync(s.adfh[0], 0, ELn * cb, s.stream));
-    HIPCHK(hipMemsetAsync(s.adfv[0], 0, NIn * cb, s.stream));
-    HIPCHK(hipMemsetAsync(s.LO[0], 0, NLn * cb, s.stream));
-    const int id = tracer_id - 1;
-    hipError_t e = launch_group(s, &id, 1, dt, true, nullptr);
+    HIPCHK(hipMemsetAsync(s.adfh, 0, ELn * kb * cb, s.stream));
+    HIPCHK(hipMemsetAsync(s.adfv, 0, NIn * kb * cb, s.stream));
+    HIPCHK(hipMemsetAsync(s.LO, 0, NLn * kb * cb, s.stream));
+    const int id = tracer_id - 1, lane = id % s.kb;
+    hipError_t e = launch_qflux(s.m64, s.stream);
+    if (e == hipSuccess) e = launch_block(s, id / s.kb, 1ull << lane, s.schemes[id], dt, true, nullptr);
     if (e != hipSuccess) return fail("kernel launch failed: %s", hipGetErrorString(e));
     s.last_fct = s.schemes[id].fct;
+    s.last_lane = lane;
     HIPCHK(hipStreamSynchronize(s.stream));
     return 0;
 }
@@ -1197,30 +1327,30 @@ int tracer_get_tendencies(double *dttf_h, double *dttf_v, int nz, int nn) {
     State &s = S();
     if (!s.on_device || !s.acc_h) return fail("tracer_get_tendencies: no tendencies computed yet");
     if (nz != s.nl - 1 || nn != s.N) return fail("tracer_get_tendencies: expected (%d, %d)", s.nl - 1, s.N);
-    const int prec = s.prec;
-    s.prec = PREC_F64;  // accumulators are double
-    int rc = 0;
-    if (dttf_h) rc |= download_work(s, s.acc_h, nz, s.Np, s.perm_n, dttf_h);
-    if (dttf_v) rc |= download_work(s, s.acc_v, nz, s.Np, s.perm_n, dttf_v);
-    s.prec = prec;
+    int rc = 0;  // accumulators are double, one tracer
+    if (dttf_h) rc |= download_work(s, s.acc_h, nz, s.Np, s.perm_n, dttf_h, 8, 1, 0);
+    if (dttf_v) rc |= download_work(s, s.acc_v, nz, s.Np, s.perm_n, dttf_v, 8, 1, 0);
     return rc ? -1 : 0;
 }
 
 int tracer_get_work(const char *name, double *out, size_t count) {
     State &s = S();
     if (!s.on_device) return fail("tracer_get_work: nothing computed yet");
-    const size_t L = s.nl - 1;
+    const size_t L = s.nl - 1, cb = compute_bytes(s.prec);
+    const int kb = s.kb, lane = s.last_lane;
     if (!strcmp(name, "fct_LO")) {
         if (count != L * s.N) return fail("tracer_get_work(fct_LO): count must be %zu", L * s.N);
-        return download_work(s, s.LO[0], (int)L, s.Np, s.perm_n, out);
+        return download_work(s, s.LO, (int)L, s.Np, s.perm_n, out, cb, kb, lane);
     }
     if (!strcmp(name, "adv_flux_hor")) {
         if (count != L * s.E) return fail("tracer_get_work(adv_flux_hor): count must be %zu", L * s.E);
-        return download_work(s, s.last_fct ? s.dbg_h : s.adfh[0], (int)L, s.Ep, s.perm_e, out);
+        if (s.last_fct && s.dbg_h) return download_work(s, s.dbg_h, (int)L, s.Ep, s.perm_e, out, cb, 1, 0);
+        return download_work(s, s.adfh, (int)L, s.Ep, s.perm_e, out, cb, kb, lane);
     }
     if (!strcmp(name, "adv_flux_ver")) {
         if (count != (size_t)s.nl * s.N) return fail("tracer_get_work(adv_flux_ver): count must be %zu", (size_t)s.nl * s.N);
-        return download_work(s, s.last_fct ? s.dbg_v : s.adfv[0], s.nl, s.Np, s.perm_n, out);
+        if (s.last_fct && s.dbg_v) return download_work(s, s.dbg_v, s.nl, s.Np, s.perm_n, out, cb, 1, 0);
+        return download_work(s, s.adfv, s.nl, s.Np, s.perm_n, out, cb, kb, lane);
     }
     return fail("tracer_get_work: unknown array %s", name);
 }
@@ -1230,8 +1360,9 @@ int tracer_set_values_device(int tracer_id, const void *values, int nl, int nod2
     if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
     if (check_dims(s, nl, nod2D, "tracer_set_values_device")) return -1;
     if (ensure_device(s)) return -1;
-    if (to_device(s, static_cast<const double *>(values), true, s.dT[tracer_id - 1])) return -1;
-    s.dTAB[tracer_id - 1] = nullptr;
+    const int b = (tracer_id - 1) / s.kb, lane = (tracer_id - 1) % s.kb;
+    if (to_device(s, static_cast<const double *>(values), true, s.dT[b], lane)) return -1;
+    if (s.tab_on[b] && to_device(s, static_cast<const double *>(values), true, s.tabbuf[b], lane)) return -1;
     return 0;
 }
 
@@ -1240,7 +1371,7 @@ int tracer_get_values_device(int tracer_id, void *values, int nz, int nn) {
     if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
     if (nz != s.nl - 1 || nn != s.N) return fail("tracer_get_values_device: expected (%d, %d)", s.nl - 1, s.N);
     if (ensure_device(s)) return -1;
-    return from_device(s, s.dT[tracer_id - 1], static_cast<double *>(values), true);
+    return from_device(s, s.dT[(tracer_id - 1) / s.kb], (tracer_id - 1) % s.kb, static_cast<double *>(values), true);
 }
 
 int tracer_set_kernel_timing(int enable) {
@@ -1265,6 +1396,6 @@ int tracer_last_elapsed_ms(double *ms) {
 
 const char *tracer_last_error(void) { return g_err.c_str(); }
 
-const char *tracer_version(void) { return "fesomtra 0.2 (gfx950)"; }
+const char *tracer_version(void) { return "fesomtra 0.3 (gfx950)"; }
 
 }  // extern "C"

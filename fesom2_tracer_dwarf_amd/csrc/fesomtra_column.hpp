@@ -16,6 +16,22 @@ __device__ __forceinline__ int xcd_block() {
 // Column owned by this thread; consecutive columns of one XCD stay contiguous in the
 // space-filling-curve order (workgroups are dispatched round-robin over the XCDs).
 __device__ __forceinline__ int column_index() { return xcd_block() * blockDim.x + threadIdx.x; }
+// (column, lane) of this thread for a tracer block of width KB (a power of two).
+template <int KB>
+__device__ __forceinline__ void column_lane(int &col, int &lane) {
+    const int g = column_index();
+    col = g / KB;
+    lane = g % KB;
+}
+__device__ __forceinline__ bool lane_on(unsigned long long mask, int lane) { return (mask >> lane) & 1ull; }
+
+// Reads lane `lane` of a tracer block: ld(cell) = block[cell * KB + lane] in compute type C.
+template <int KB, typename C, typename S>
+struct LaneLd {
+    const S *a;
+    int lane;
+    __device__ __forceinline__ C operator()(size_t cell) const { return ldc<C>(a, cell * KB + lane); }
+};
 
 // First-order upwind edge flux (lib/oce_adv_tra_hor.F90:178-181).
 template <typename C>
@@ -23,17 +39,24 @@ __device__ __forceinline__ C upw1_edge(C t1, C t2, C v) {
     return -C(0.5) * (t1 * (v + fabs(v)) + t2 * (v - fabs(v)));
 }
 
-// tr_xy(comp, nz, el) = sum over the element's nodes of gradient_sca * T, in node order.
-template <typename C, typename S>
-__device__ __forceinline__ C tr_xy(const C g[3], const S *__restrict__ t, size_t off, int a, int b, int c) {
-    return g[0] * ldc<C>(t, off + a) + g[1] * ldc<C>(t, off + b) + g[2] * ldc<C>(t, off + c);
+// tr_xy(comp, nz, el) = sum over the element's nodes of gradient_sca * T, in node order.  In
+// single precision (C = float) it is evaluated as g1 (T_b - T_a) + g2 (T_c - T_a), equal since
+// the three coefficients of a P1 gradient sum to zero, and exactly zero for a uniform field.
+template <typename C, typename LD>
+__device__ __forceinline__ C tr_xy(const C g[3], const LD &t, size_t off, int a, int b, int c) {
+    if constexpr (std::is_same<C, double>::value) {
+        return g[0] * t(off + a) + g[1] * t(off + b) + g[2] * t(off + c);
+    } else {
+        const C ta = t(off + a);
+        return g[1] * (t(off + b) - ta) + g[2] * (t(off + c) - ta);
+    }
 }
 
 // Area-weighted mean element gradient around node n at level nz
 // (lib/oce_muscl_adv.F90:394-409), accumulated in nod_in_elem2D order.
-template <typename C, typename S, typename M>
-__device__ __forceinline__ void node_mean_grad(const MeshCol<M> &m, const S *__restrict__ t, int n,
-                                               int nz, C &gx, C &gy) {
+template <typename C, typename LD, typename M>
+__device__ __forceinline__ void node_mean_grad(const MeshCol<M> &m, const LD &t, int n, int nz, C &gx,
+                                               C &gy) {
     C tvol = 0, tx = 0, ty = 0;
     const size_t off = (size_t)(nz - 1) * m.Np;
     const int cnt = m.nie_num[n];
@@ -74,16 +97,36 @@ __device__ __forceinline__ C ho_edge(C t1, C t2, const C gr[4], C v, C dx, C dy,
     return -C(0.5) * (C(1) - num_ord) * cHO - v * num_ord * C(0.5) * (Tmean1 + Tmean2) - fin;
 }
 
+// MUSCL / MFCT high-order flux minus the first-order upwind flux of the same values, in the
+// difference form  -(1-th)/2 [(v+|v|) d1 + (v-|v|) d2] - th [v (d1+d2)/2 - |v| (a1-a2)/2]  with
+// d1 = Tmean1 - a1, d2 = Tmean2 - a2: algebraically the reference's ho_edge(...) - upw1_edge(a1,
+// a2, v), evaluated so that only small differences are rounded (used when C is single
+// precision).
+template <int HOR, typename C>
+__device__ __forceinline__ C ho_minus_lo(C a1, C a2, const C gr[4], C v, C dx, C dy, C a, C th,
+                                         C c1, C c2) {
+    const C r = (C)kREarth;
+    C d1 = (C(2) * (a2 - a1) + dx * a * gr[0] + dy * r * gr[2]) / C(6);
+    C d2 = -(C(2) * (a2 - a1) + dx * a * gr[1] + dy * r * gr[3]) / C(6);
+    if (HOR == HOR_MUSCL) {
+        d1 = d1 * c1;
+        d2 = d2 * c2;
+    }
+    const C av = fabs(v);
+    return -C(0.5) * (C(1) - th) * ((v + av) * d1 + (v - av) * d2) -
+           th * (v * C(0.5) * (d1 + d2) - C(0.5) * av * (a1 - a2));
+}
+
 // ------------------------------------------------------------------------------------------
 // Vertical fluxes (lib/oce_adv_tra_ver.F90).  A Col reads one node column straight from memory
 // (used by the less common CDIFF / PPM schemes); QR4C and UPW1 use register windows instead.
-template <typename C, typename S, typename M>
+template <typename C, typename S, typename M, int KB>
 struct Col {
     const S *t;
     const M *w, *ar, *hn, *ho;
     size_t Np;
-    int n, nzmin, nzmax;
-    __device__ __forceinline__ C T(int nz) const { return ldc<C>(t, (size_t)(nz - 1) * Np + n); }
+    int n, lane, nzmin, nzmax;
+    __device__ __forceinline__ C T(int nz) const { return ldc<C>(t, ((size_t)(nz - 1) * Np + n) * KB + lane); }
     __device__ __forceinline__ C W(int j) const { return (C)w[(size_t)(j - 1) * Np + n]; }
     __device__ __forceinline__ C A(int j) const { return (C)ar[(size_t)(j - 1) * Np + n]; }
     __device__ __forceinline__ C HN(int nz) const { return (C)hn[(size_t)(nz - 1) * Np + n]; }
@@ -91,8 +134,8 @@ struct Col {
 };
 
 // adv_tra_ver_cdiff (:687-708) at interface j, minus fin; the bottom interface keeps fin.
-template <typename C, typename S, typename M>
-__device__ __forceinline__ C ver_cdiff(const Col<C, S, M> &c, int j, C fin) {
+template <typename C, typename S, typename M, int KB>
+__device__ __forceinline__ C ver_cdiff(const Col<C, S, M, KB> &c, int j, C fin) {
     if (j == c.nzmax) return fin;
     if (j == c.nzmin) return -c.W(j) * c.T(j) * c.A(j) - fin;
     const C tv = C(0.5) * (c.T(j - 1) + c.T(j));
@@ -100,8 +143,8 @@ __device__ __forceinline__ C ver_cdiff(const Col<C, S, M> &c, int j, C fin) {
 }
 
 // PPM interface value tv(j) after all of the reference's assignments (:510-594).
-template <typename C, typename S, typename M>
-__device__ __forceinline__ C ppm_tv(const Col<C, S, M> &c, int j) {
+template <typename C, typename S, typename M, int KB>
+__device__ __forceinline__ C ppm_tv(const Col<C, S, M, KB> &c, int j) {
     const int nzmin = c.nzmin, nzmax = c.nzmax;
     if (j >= nzmin + 2 && j <= nzmax - 2) {
         const int nz = j - 1;
@@ -139,8 +182,8 @@ __device__ __forceinline__ C ppm_tv(const Col<C, S, M> &c, int j) {
 }
 
 // Limited parabola of layer nz (:601-617).
-template <typename C, typename S, typename M>
-__device__ __forceinline__ void ppm_layer(const Col<C, S, M> &c, int nz, C &aL, C &aR, C &aj) {
+template <typename C, typename S, typename M, int KB>
+__device__ __forceinline__ void ppm_layer(const Col<C, S, M, KB> &c, int nz, C &aL, C &aR, C &aj) {
     const C t = c.T(nz);
     aL = ppm_tv(c, nz);
     aR = ppm_tv(c, nz + 1);
@@ -154,8 +197,8 @@ __device__ __forceinline__ void ppm_layer(const Col<C, S, M> &c, int nz, C &aL, 
 }
 
 // adv_tra_vert_ppm (:596-639) at interface j, minus fin.
-template <typename C, typename S, typename M>
-__device__ __forceinline__ C ver_ppm(const Col<C, S, M> &c, int j, C dt, C fin) {
+template <typename C, typename S, typename M, int KB>
+__device__ __forceinline__ C ver_ppm(const Col<C, S, M, KB> &c, int j, C dt, C fin) {
     if (j == c.nzmin) return -ppm_tv(c, c.nzmin) * c.W(c.nzmin) * c.A(c.nzmin) - fin;
     if (j == c.nzmax) return C(0) - fin;
     const C wj = c.W(j);

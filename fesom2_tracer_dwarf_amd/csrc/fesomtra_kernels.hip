@@ -1,15 +1,15 @@
 // gfx950 kernels of the tracer advection step and their launchers.
 //
-// One step of KT tracers (same scheme) with the FCT limiter is four column kernels
-// (do_oce_adv_tra, lib/oce_adv_tra_driver.F90:23-294):
-//   k_edge    one thread per edge: low-order upwind flux and high-order minus low-order flux
+// One step of a block of KB interleaved tracers (the lanes of `mask` share one scheme) with the
+// FCT limiter is four column kernels (do_oce_adv_tra, lib/oce_adv_tra_driver.F90:23-294):
+//   k_edge    one thread per edge and lane: low-order upwind flux and high-order minus low-order flux
 //             for every level of the edge; MUSCL/MFCT gradients reconstructed on the fly from
 //             the up/down-wind triangles (no edge_up_dn_grad array)
-//   k_lo      one thread per node: deterministic gather of the edge fluxes, vertical upwind
+//   k_lo      one thread per node and lane: deterministic gather of the edge fluxes, vertical upwind
 //             flux, low-order solution fct_LO, vertical antidiffusive flux (QR4C/PPM/...)
-//   k_bounds  one thread per node: cluster bounds over the node's 1-ring, vertical widening,
+//   k_bounds  one thread per node and lane: cluster bounds over the node's 1-ring, vertical widening,
 //             P+/P- gathers and the limiter ratios R+/R- (lib/oce_adv_tra_fct.F90:96-374)
-//   k_final   one thread per node: limiting with R at both ends, flux divergence, tendencies
+//   k_final   one thread per node and lane: limiting with R at both ends, flux divergence, tendencies
 //             and the tracer update in place (lib/oce_adv_tra_fct.F90:389-471,
 //             lib/oce_adv_tra_driver.F90:298-450, src/fesom_analytic.F90:419-437)
 // Without FCT: k_edge, k_vert and k_final.  No atomics: each node adds its incident edges in
@@ -20,8 +20,9 @@
 namespace fesomtra {
 
 // ---------------------------------------------------------------------------------------------
-// Work decomposition: one thread owns `lc` consecutive levels of one column; blockIdx.x selects
-// columns (XCD-aware, see column_index) and blockIdx.y the level chunk.  Vertical stencils that
+// Work decomposition: one thread owns `lc` consecutive levels of one column for one lane of the
+// tracer block; blockIdx.x selects columns (XCD-aware, see column_index) and blockIdx.y the level
+// chunk.  Vertical stencils that
 // cross a chunk boundary recompute the few values they need from the neighbouring chunk.
 
 // 1-based layer range [nz0, nz1] of this thread's chunk in a column of L wet layers.
@@ -32,19 +33,20 @@ __device__ __forceinline__ bool chunk_layers(int L, int lc, int &nz0, int &nz1) 
     return true;
 }
 
-template <int HOR, bool FCT, int KT, typename S, typename M, typename C>
+template <int HOR, bool FCT, int KB, typename S, typename M, typename C>
 __global__ __launch_bounds__(256) void k_edge(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, int lc) {
-    const int e = column_index();
-    if (e >= m.E) return;
+    int e, lane;
+    column_lane<KB>(e, lane);
+    if (e >= m.E || !lane_on(p.mask, lane)) return;
     const int4 q = m.edge_ne[e];
     const int n1 = q.x, n2 = q.y, el1 = q.z, el2 = q.w;
     const int nl1 = m.nlev_e[el1] - 1;
     const int nl2 = el2 >= 0 ? m.nlev_e[el2] - 1 : 0;
     int nz0, nz1;
     if (!chunk_layers(max(nl1, nl2), lc, nz0, nz1)) return;
-    const size_t Np = m.Np, Tp = m.Tp, Ep = m.Ep;
-    const C cd0 = (C)m.ecd[4 * (size_t)e], cd1 = (C)m.ecd[4 * (size_t)e + 1];
-    const C cd2 = (C)m.ecd[4 * (size_t)e + 2], cd3 = (C)m.ecd[4 * (size_t)e + 3];
+    const size_t Np = m.Np, Ep = m.Ep;
+    const LaneLd<KB, C, S> ldT{p.T, lane}, ldA{p.TAB, lane};
+    const bool same = p.TAB == p.T;
     C dx = 0, dy = 0, a = 0;
     int shared_end = 0, u[3] = {0, 0, 0}, d[3] = {0, 0, 0}, nb1 = 0, nb2 = 0;
     C gu[6], gd[6];
@@ -74,55 +76,79 @@ __global__ __launch_bounds__(256) void k_edge(const MeshCol<M> m, const StepPtrs
     }
     for (int nz = nz0; nz <= nz1; ++nz) {
         const int k = nz - 1;
-        // segments C (both elements), D (el1 only), E (el2 only) of lib/oce_adv_tra_hor.F90:166-253
-        const bool use1 = nz <= nl1, use2 = nz <= nl2;
-        C v1 = 0, v2 = 0;
-        if (use1) {
-            const size_t i = (size_t)k * Tp + el1;
-            v1 = (-(C)m.uv[2 * i + 1] * cd0 + (C)m.uv[2 * i] * cd1) * (C)m.helem[i];
-        }
-        if (use2) {
-            const size_t i = (size_t)k * Tp + el2;
-            v2 = ((C)m.uv[2 * i + 1] * cd2 - (C)m.uv[2 * i] * cd3) * (C)m.helem[i];
-        }
-        const C v = (use1 && use2) ? v1 + v2 : (use1 ? v1 : v2);
+        const C v = (C)m.q[(size_t)k * Ep + e];
         const size_t o = (size_t)k * Np;
-#pragma unroll
-        for (int t = 0; t < KT; ++t) {
-            const bool same = p.TAB[t] == p.T[t];
-            C fin = 0;
-            C t1 = 0, t2 = 0;
-            if (FCT || same) {
-                t1 = ldc<C>(p.T[t], o + n1);
-                t2 = ldc<C>(p.T[t], o + n2);
+        const size_t ie = ((size_t)k * Ep + e) * KB + lane;
+        C t1 = 0, t2 = 0;
+        if (FCT || same) {
+            t1 = ldT(o + n1);
+            t2 = ldT(o + n2);
+        }
+        const C a1 = same ? t1 : ldA(o + n1);
+        const C a2 = same ? t2 : ldA(o + n2);
+        C gr[4] = {0, 0, 0, 0}, c1 = 1, c2 = 1;
+        if (HOR != HOR_UPW1) {
+            if (nz <= shared_end) {
+                gr[0] = tr_xy<C>(gu, ldA, o, u[0], u[1], u[2]);
+                gr[1] = tr_xy<C>(gd, ldA, o, d[0], d[1], d[2]);
+                gr[2] = tr_xy<C>(gu + 3, ldA, o, u[0], u[1], u[2]);
+                gr[3] = tr_xy<C>(gd + 3, ldA, o, d[0], d[1], d[2]);
+            } else {
+                node_mean_grad<C>(m, ldA, n1, nz, gr[0], gr[2]);
+                node_mean_grad<C>(m, ldA, n2, nz, gr[1], gr[3]);
             }
+            if (HOR == HOR_MUSCL) {
+                c1 = (nb1 - nz >= 0) ? C(1) : C(0);
+                c2 = (nb2 - nz >= 0) ? C(1) : C(0);
+            }
+        }
+        C f;
+        if constexpr (std::is_same<C, double>::value) {  // the reference's operation order
+            C fin = 0;
             if (FCT) {
                 fin = upw1_edge(t1, t2, v);
-                p.flo[t][(size_t)k * Ep + e] = fin;
+                p.flo[ie] = fin;
             }
-            const C a1 = same ? t1 : ldc<C>(p.TAB[t], o + n1);
-            const C a2 = same ? t2 : ldc<C>(p.TAB[t], o + n2);
-            C f;
-            if (HOR == HOR_UPW1) {
-                f = upw1_edge(a1, a2, v) - fin;
-            } else {
-                C gr[4];
-                if (nz <= shared_end) {
-                    gr[0] = tr_xy<C>(gu, p.TAB[t], o, u[0], u[1], u[2]);
-                    gr[1] = tr_xy<C>(gd, p.TAB[t], o, d[0], d[1], d[2]);
-                    gr[2] = tr_xy<C>(gu + 3, p.TAB[t], o, u[0], u[1], u[2]);
-                    gr[3] = tr_xy<C>(gd + 3, p.TAB[t], o, d[0], d[1], d[2]);
-                } else {
-                    node_mean_grad<C>(m, p.TAB[t], n1, nz, gr[0], gr[2]);
-                    node_mean_grad<C>(m, p.TAB[t], n2, nz, gr[1], gr[3]);
-                }
-                const C c1 = (HOR == HOR_MUSCL) ? ((nb1 - nz >= 0) ? C(1) : C(0)) : C(1);
-                const C c2 = (HOR == HOR_MUSCL) ? ((nb2 - nz >= 0) ? C(1) : C(0)) : C(1);
-                f = ho_edge<HOR, C>(a1, a2, gr, v, dx, dy, a, num_ord, c1, c2, fin);
-            }
-            p.adfh[t][(size_t)k * Ep + e] = f;
+            if (HOR == HOR_UPW1) f = upw1_edge(a1, a2, v) - fin;
+            else f = ho_edge<HOR, C>(a1, a2, gr, v, dx, dy, a, num_ord, c1, c2, fin);
+        } else {
+            // high-order minus upwind of valuesAB in difference form; with the limiter, plus the
+            // upwind difference valuesAB - values.  The upwind fluxes themselves are formed in
+            // double where they are summed (k_lo, k_final).
+            f = (HOR == HOR_UPW1) ? C(0) : ho_minus_lo<HOR, C>(a1, a2, gr, v, dx, dy, a, num_ord, c1, c2);
+            if (FCT && !same) f = f - C(0.5) * ((v + fabs(v)) * (a1 - t1) + (v - fabs(v)) * (a2 - t2));
         }
+        p.adfh[ie] = f;
     }
+}
+
+// Edge transport of the current velocity, q(k, e) = sum over the edge's elements wet at level k
+// of (+-) (u dy - v dx) helem (lib/oce_adv_tra_hor.F90:166-253, segments C, D and E), in double,
+// once per step for all tracers.  Levels below both elements are not written (never read).
+template <typename M>
+__global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.E) return;
+    const int nz = (int)blockIdx.y + 1, k = nz - 1;
+    const int4 q = m.edge_ne[e];
+    const int el1 = q.z, el2 = q.w;
+    const int nl1 = m.nlev_e[el1] - 1;
+    const int nl2 = el2 >= 0 ? m.nlev_e[el2] - 1 : 0;
+    const bool use1 = nz <= nl1, use2 = nz <= nl2;
+    if (!use1 && !use2) return;
+    const size_t Tp = m.Tp;
+    double v1 = 0, v2 = 0;
+    if (use1) {
+        const size_t i = (size_t)k * Tp + el1;
+        v1 = (-(double)m.uv[2 * i + 1] * (double)m.ecd[4 * (size_t)e] +
+              (double)m.uv[2 * i] * (double)m.ecd[4 * (size_t)e + 1]) * (double)m.helem[i];
+    }
+    if (use2) {
+        const size_t i = (size_t)k * Tp + el2;
+        v2 = ((double)m.uv[2 * i + 1] * (double)m.ecd[4 * (size_t)e + 2] -
+              (double)m.uv[2 * i] * (double)m.ecd[4 * (size_t)e + 3]) * (double)m.helem[i];
+    }
+    m.q[(size_t)k * m.Ep + e] = (use1 && use2) ? v1 + v2 : (use1 ? v1 : v2);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -161,12 +187,12 @@ __device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges &
 }
 
 // Column readers of one node (1-based levels; 0 outside the wet range).
-template <typename C, typename S, typename M>
+template <typename C, typename S, typename M, int KB>
 struct NodeCol {
     size_t Np;
-    int n, L;
+    int n, lane, L;
     __device__ __forceinline__ C lay(const S *a, int nz) const {
-        return (nz >= 1 && nz <= L) ? ldc<C>(a, (size_t)(nz - 1) * Np + n) : C(0);
+        return (nz >= 1 && nz <= L) ? ldc<C>(a, ((size_t)(nz - 1) * Np + n) * KB + lane) : C(0);
     }
     __device__ __forceinline__ C mlay(const M *a, int nz) const {
         return (nz >= 1 && nz <= L) ? (C)a[(size_t)(nz - 1) * Np + n] : C(0);
@@ -176,18 +202,19 @@ struct NodeCol {
 
 // High-order vertical flux at interface j minus fin for the window tw = TAB(j-2..j+1),
 // z = Z(j-2..j+1).
-template <int VER, typename C, typename S, typename M>
+template <int VER, typename C, typename S, typename M, int KB>
 __device__ __forceinline__ C ver_ho_at(int j, int nzmax, const C tw[4], const C z[4], C zbj, C wj, C aj,
-                                       C num_ord, C dt, const Col<C, S, M> &col, C fin) {
+                                       C num_ord, C dt, const Col<C, S, M, KB> &col, C fin) {
     if (VER == VER_QR4C) return ver_qr4c_w<C>(j, nzmax, tw, z, zbj, wj, aj, num_ord, fin);
     if (VER == VER_UPW1) return ver_upw1_w<C>(j, nzmax, tw[1], tw[2], wj, aj, fin);
     if (VER == VER_CDIFF) return ver_cdiff(col, j, fin);
     return ver_ppm(col, j, dt, fin);
 }
 
-template <typename C, typename S, typename M>
-__device__ __forceinline__ Col<C, S, M> make_col(const S *t, const M *w, const MeshCol<M> &m, int n, int nzmax) {
-    Col<C, S, M> c;
+template <typename C, typename S, typename M, int KB>
+__device__ __forceinline__ Col<C, S, M, KB> make_col(const S *t, const M *w, const MeshCol<M> &m, int n,
+                                                     int lane, int nzmax) {
+    Col<C, S, M, KB> c;
     c.t = t;
     c.w = w;
     c.ar = m.area;
@@ -195,6 +222,7 @@ __device__ __forceinline__ Col<C, S, M> make_col(const S *t, const M *w, const M
     c.ho = m.hnode;
     c.Np = m.Np;
     c.n = n;
+    c.lane = lane;
     c.nzmin = 1;
     c.nzmax = nzmax;
     return c;
@@ -202,35 +230,36 @@ __device__ __forceinline__ Col<C, S, M> make_col(const S *t, const M *w, const M
 
 // Low-order solution and vertical antidiffusive flux (lib/oce_adv_tra_driver.F90:102-216 and
 // the high-order vertical call :266-282 with o_init_zero=.false., pwvel = w).  Interface fluxes
-// are evaluated top-down; layer nz uses the fluxes at its top (nz) and bottom (nz + 1).
-template <int VER, int KT, typename S, typename M, typename C>
+// are evaluated top-down in double; layer nz uses the fluxes at its top (nz) and bottom (nz + 1).
+// The FP64 path gathers the stored upwind edge fluxes; the single-precision paths form them here
+// in double from q and the two end values.
+template <int VER, int KB, typename S, typename M, typename C>
 __global__ __launch_bounds__(256) void k_lo(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt, int lc) {
-    const int n = column_index();
-    if (n >= m.N) return;
+    using D = double;
+    int n, lane;
+    column_lane<KB>(n, lane);
+    if (n >= m.N || !lane_on(p.mask, lane)) return;
     const int nzmax = m.nlev_n[n], L = nzmax - 1;
     int nz0, nz1;
     if (!chunk_layers(L, lc, nz0, nz1)) return;
     const size_t Np = m.Np, Ep = m.Ep;
     NodeEdges ne;
     load_edges(m, n, ne);
-    const NodeCol<C, S, M> nc{Np, n, L};
-    C tw[KT][4], z[4], lo_top[KT], adf_top[KT], tcur[KT];
-    Col<C, S, M> col[KT];
+    const NodeCol<D, S, M, KB> nc{Np, n, lane, L};
+    const Col<D, S, M, KB> col = make_col<D, S, M, KB>(p.TAB, m.w, m, n, lane, nzmax);
+    const D pv = (D)num_ord;
+    D tw[4], z[4], lo_top, adf_top, tcur;
     {
         const int j = nz0;  // top interface of the chunk
 #pragma unroll
-        for (int i = 0; i < 4; ++i) z[i] = (VER == VER_QR4C) ? nc.mlay(m.Z, j - 2 + i) : C(0);
-        const C wej = nc.itf(m.we, j), wj = nc.itf(m.w, j), aj = nc.itf(m.area, j);
-        const C zbj = (VER == VER_QR4C) ? nc.itf(m.zbar, j) : C(0);
+        for (int i = 0; i < 4; ++i) z[i] = (VER == VER_QR4C) ? nc.mlay(m.Z, j - 2 + i) : D(0);
+        const D wej = nc.itf(m.we, j), wj = nc.itf(m.w, j), aj = nc.itf(m.area, j);
+        const D zbj = (VER == VER_QR4C) ? nc.itf(m.zbar, j) : D(0);
 #pragma unroll
-        for (int t = 0; t < KT; ++t) {
-            col[t] = make_col<C, S, M>(p.TAB[t], m.w, m, n, nzmax);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) tw[t][i] = nc.lay(p.TAB[t], j - 2 + i);
-            tcur[t] = nc.lay(p.T[t], j);
-            lo_top[t] = ver_upw1_w<C>(j, nzmax, nc.lay(p.T[t], j - 1), tcur[t], wej, aj, C(0));
-            adf_top[t] = ver_ho_at<VER>(j, nzmax, tw[t], z, zbj, wj, aj, num_ord, (C)dt, col[t], lo_top[t]);
-        }
+        for (int i = 0; i < 4; ++i) tw[i] = nc.lay(p.TAB, j - 2 + i);
+        tcur = nc.lay(p.T, j);
+        lo_top = ver_upw1_w<D>(j, nzmax, nc.lay(p.T, j - 1), tcur, wej, aj, D(0));
+        adf_top = ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, lo_top);
     }
     for (int nz = nz0; nz <= nz1; ++nz) {
         const int j = nz + 1;  // bottom interface of layer nz
@@ -241,57 +270,62 @@ __global__ __launch_bounds__(256) void k_lo(const MeshCol<M> m, const StepPtrs<S
             z[3] = nc.mlay(m.Z, j + 1);
         }
         const bool bottom = (j == nzmax);
-        const C wej = nc.itf(m.we, j), wj = nc.itf(m.w, j), aj = nc.itf(m.area, j);
-        const C zbj = (VER == VER_QR4C && !bottom) ? nc.itf(m.zbar, j) : C(0);
-        const size_t ix = (size_t)(nz - 1) * Np + n;
-        const double h = (double)m.hnode[ix], hn = (double)m.hnode_new[ix];
-        const double V = (double)m.areasvol[ix];
-#pragma unroll
-        for (int t = 0; t < KT; ++t) {
-            tw[t][0] = tw[t][1];
-            tw[t][1] = tw[t][2];
-            tw[t][2] = tw[t][3];
-            tw[t][3] = nc.lay(p.TAB[t], j + 1);
-            const C tnext = nc.lay(p.T[t], j);
-            const C lo_bot = ver_upw1_w<C>(j, nzmax, tcur[t], tnext, wej, aj, C(0));
-            const C adf_bot = ver_ho_at<VER>(j, nzmax, tw[t], z, zbj, wj, aj, num_ord, (C)dt, col[t], lo_bot);
-            double s = 0.0;
+        const D wej = nc.itf(m.we, j), wj = nc.itf(m.w, j), aj = nc.itf(m.area, j);
+        const D zbj = (VER == VER_QR4C && !bottom) ? nc.itf(m.zbar, j) : D(0);
+        const size_t cell = (size_t)(nz - 1) * Np + n;
+        const double h = (double)m.hnode[cell], hn = (double)m.hnode_new[cell];
+        const double V = (double)m.areasvol[cell];
+        tw[0] = tw[1];
+        tw[1] = tw[2];
+        tw[2] = tw[3];
+        tw[3] = nc.lay(p.TAB, j + 1);
+        const D tnext = nc.lay(p.T, j);
+        const D lo_bot = ver_upw1_w<D>(j, nzmax, tcur, tnext, wej, aj, D(0));
+        const D adf_bot = ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, lo_bot);
+        double s = 0.0;
+        const size_t eo = (size_t)(nz - 1) * Ep, no = (size_t)(nz - 1) * Np;
+        if constexpr (std::is_same<C, double>::value) {
             for_edges(m, ne, nz, [&](int e, bool second, int) {
-                const double f = (double)p.flo[t][(size_t)(nz - 1) * Ep + e];
+                const double f = p.flo[(eo + e) * KB + lane];
                 s = second ? s - f : s + f;
             });
-            const double lo = ((double)tcur[t] * h + (s + ((double)lo_top[t] - (double)lo_bot)) * dt / V) / hn;
-            p.LO[t][ix] = (C)lo;
-            p.adfv[t][ix] = adf_top[t];
-            if (bottom) p.adfv[t][(size_t)nz * Np + n] = adf_bot;
-            lo_top[t] = lo_bot;
-            adf_top[t] = adf_bot;
-            tcur[t] = tnext;
+        } else {
+            for_edges(m, ne, nz, [&](int e, bool second, int oth) {
+                const double to = ldc<double>(p.T, (no + oth) * KB + lane);
+                const double f = upw1_edge<double>(second ? to : tcur, second ? tcur : to, m.q[eo + e]);
+                s = second ? s - f : s + f;
+            });
         }
+        const double lo = ((double)tcur * h + (s + ((double)lo_top - (double)lo_bot)) * dt / V) / hn;
+        p.LO[cell * KB + lane] = (C)lo;
+        p.adfv[cell * KB + lane] = (C)adf_top;
+        if (bottom) p.adfv[(cell + Np) * KB + lane] = (C)adf_bot;
+        lo_top = lo_bot;
+        adf_top = adf_bot;
+        tcur = tnext;
     }
 }
 
-// Vertical high-order flux without the limiter (pwvel = we, o_init_zero=.true.): interfaces
-// nz0..nz1 of the chunk, plus the bottom interface in the deepest chunk.
-template <int VER, int KT, typename S, typename M, typename C>
+// Vertical high-order flux without the limiter (pwvel = we, o_init_zero=.true.), in double:
+// interfaces nz0..nz1 of the chunk, plus the bottom interface in the deepest chunk.
+template <int VER, int KB, typename S, typename M, typename C>
 __global__ __launch_bounds__(256) void k_vert(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt, int lc) {
-    const int n = column_index();
-    if (n >= m.N) return;
+    using D = double;
+    int n, lane;
+    column_lane<KB>(n, lane);
+    if (n >= m.N || !lane_on(p.mask, lane)) return;
     const int nzmax = m.nlev_n[n], L = nzmax - 1;
     int nz0, nz1;
     if (!chunk_layers(L, lc, nz0, nz1)) return;
     const size_t Np = m.Np;
-    const NodeCol<C, S, M> nc{Np, n, L};
-    C tw[KT][4], z[4];
-    Col<C, S, M> col[KT];
+    const NodeCol<D, S, M, KB> nc{Np, n, lane, L};
+    const Col<D, S, M, KB> col = make_col<D, S, M, KB>(p.TAB, m.we, m, n, lane, nzmax);
+    const D pv = (D)num_ord;
+    D tw[4], z[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) z[i] = (VER == VER_QR4C) ? nc.mlay(m.Z, nz0 - 3 + i) : C(0);
+    for (int i = 0; i < 4; ++i) z[i] = (VER == VER_QR4C) ? nc.mlay(m.Z, nz0 - 3 + i) : D(0);
 #pragma unroll
-    for (int t = 0; t < KT; ++t) {
-        col[t] = make_col<C, S, M>(p.TAB[t], m.we, m, n, nzmax);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) tw[t][i] = nc.lay(p.TAB[t], nz0 - 3 + i);
-    }
+    for (int i = 0; i < 4; ++i) tw[i] = nc.lay(p.TAB, nz0 - 3 + i);
     const int jend = nz1 == L ? nzmax : nz1;
     for (int j = nz0; j <= jend; ++j) {
         if (VER == VER_QR4C) {
@@ -301,16 +335,14 @@ __global__ __launch_bounds__(256) void k_vert(const MeshCol<M> m, const StepPtrs
             z[3] = nc.mlay(m.Z, j + 1);
         }
         const bool bottom = (j == nzmax);
-        const C wj = nc.itf(m.we, j), aj = nc.itf(m.area, j);
-        const C zbj = (VER == VER_QR4C && !bottom) ? nc.itf(m.zbar, j) : C(0);
-#pragma unroll
-        for (int t = 0; t < KT; ++t) {
-            tw[t][0] = tw[t][1];
-            tw[t][1] = tw[t][2];
-            tw[t][2] = tw[t][3];
-            tw[t][3] = nc.lay(p.TAB[t], j + 1);
-            p.adfv[t][(size_t)(j - 1) * Np + n] = ver_ho_at<VER>(j, nzmax, tw[t], z, zbj, wj, aj, num_ord, (C)dt, col[t], C(0));
-        }
+        const D wj = nc.itf(m.we, j), aj = nc.itf(m.area, j);
+        const D zbj = (VER == VER_QR4C && !bottom) ? nc.itf(m.zbar, j) : D(0);
+        tw[0] = tw[1];
+        tw[1] = tw[2];
+        tw[2] = tw[3];
+        tw[3] = nc.lay(p.TAB, j + 1);
+        p.adfv[((size_t)(j - 1) * Np + n) * KB + lane] =
+            (C)ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, D(0));
     }
 }
 
@@ -319,10 +351,11 @@ __global__ __launch_bounds__(256) void k_vert(const MeshCol<M> m, const StepPtrs
 // level nz is the set of nodes of the elements around n that are wet at nz; with complete
 // nod_in_elem2D that is n plus every neighbour m whose edge (n, m) reaches level nz, and an
 // element dry at nz contributes -1e3 / +1e3 exactly when nz > nlevels_nod2D_min(n) - 1.
-template <int KT, typename S, typename M, typename C>
+template <int KB, typename S, typename M, typename C>
 __global__ __launch_bounds__(256) void k_bounds(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
-    const int n = column_index();
-    if (n >= m.N) return;
+    int n, lane;
+    column_lane<KB>(n, lane);
+    if (n >= m.N || !lane_on(p.mask, lane)) return;
     const int nzmax = m.nlev_n[n], L = nzmax - 1;
     int nz0, nz1;
     if (!chunk_layers(L, lc, nz0, nz1)) return;
@@ -330,13 +363,14 @@ __global__ __launch_bounds__(256) void k_bounds(const MeshCol<M> m, const StepPt
     const int dry_from = m.nlev_min[n];  // levels >= this have a dry element around n
     NodeEdges ne;
     load_edges(m, n, ne);
-    auto cluster = [&](int t, int nz, C &vmax, C &vmin) {
+    const LaneLd<KB, C, S> ldT{p.T, lane};
+    auto cluster = [&](int nz, C &vmax, C &vmin) {
         const size_t o = (size_t)(nz - 1) * Np;
-        const C l0 = p.LO[t][o + n], t0 = ldc<C>(p.T[t], o + n);
+        const C l0 = p.LO[(o + n) * KB + lane], t0 = ldT(o + n);
         vmax = cmax(l0, t0);
         vmin = cmin(l0, t0);
         for_edges(m, ne, nz, [&](int, bool, int oth) {
-            const C lo = p.LO[t][o + oth], tt = ldc<C>(p.T[t], o + oth);
+            const C lo = p.LO[(o + oth) * KB + lane], tt = ldT(o + oth);
             vmax = cmax(vmax, cmax(lo, tt));
             vmin = cmin(vmin, cmin(lo, tt));
         });
@@ -345,58 +379,53 @@ __global__ __launch_bounds__(256) void k_bounds(const MeshCol<M> m, const StepPt
             vmin = cmin(vmin, C(1.0e3));
         }
     };
-    C mx[KT][3], mn[KT][3], av0[KT];
-#pragma unroll
-    for (int t = 0; t < KT; ++t) {
-        if (nz0 >= 2 && nz0 < L) cluster(t, nz0 - 1, mx[t][0], mn[t][0]);
-        else mx[t][0] = mn[t][0] = 0;
-        cluster(t, nz0, mx[t][1], mn[t][1]);
-        if (nz0 + 1 <= L && nz0 > 1 && nz0 < L) cluster(t, nz0 + 1, mx[t][2], mn[t][2]);
-        else mx[t][2] = mn[t][2] = 0;
-        av0[t] = p.adfv[t][(size_t)(nz0 - 1) * Np + n];
-    }
+    C mx[3], mn[3], av0;
+    if (nz0 >= 2 && nz0 < L) cluster(nz0 - 1, mx[0], mn[0]);
+    else mx[0] = mn[0] = 0;
+    cluster(nz0, mx[1], mn[1]);
+    if (nz0 + 1 <= L && nz0 > 1 && nz0 < L) cluster(nz0 + 1, mx[2], mn[2]);
+    else mx[2] = mn[2] = 0;
+    av0 = p.adfv[((size_t)(nz0 - 1) * Np + n) * KB + lane];
     for (int nz = nz0; nz <= nz1; ++nz) {
-        const size_t ix = (size_t)(nz - 1) * Np + n;
-        const double V = (double)m.areasvol[ix];
-#pragma unroll
-        for (int t = 0; t < KT; ++t) {
-            const C lo = p.LO[t][ix];
-            C qmax, qmin;
-            if (nz == L || nz == 1) {
-                qmax = mx[t][1] - lo;
-                qmin = mn[t][1] - lo;
+        const size_t cell = (size_t)(nz - 1) * Np + n;
+        const double V = (double)m.areasvol[cell];
+        const C lo = p.LO[cell * KB + lane];
+        C qmax, qmin;
+        if (nz == L || nz == 1) {
+            qmax = mx[1] - lo;
+            qmin = mn[1] - lo;
+        } else {
+            qmax = cmax(cmax(mx[0], mx[1]), mx[2]) - lo;
+            qmin = cmin(cmin(mn[0], mn[1]), mn[2]) - lo;
+        }
+        const C av1 = p.adfv[(cell + Np) * KB + lane];
+        double pp = 0.0 + ((double)cmax(C(0), av0) + (double)cmax(C(0), -av1));
+        double pm = 0.0 + ((double)cmin(C(0), av0) + (double)cmin(C(0), -av1));
+        const size_t eo = (size_t)(nz - 1) * Ep;
+        for_edges(m, ne, nz, [&](int e, bool second, int) {
+            C a = p.adfh[(eo + e) * KB + lane];
+            if (second) a = -a;
+            pp = pp + (double)cmax(C(0), a);
+            pm = pm + (double)cmin(C(0), a);
+        });
+        const double fp = pp * dt / V + 1e-16;
+        const double fm = pm * dt / V - 1e-16;
+        const double rp = (double)qmax / fp, rm = (double)qmin / fm;
+        p.Rp[cell * KB + lane] = (C)(rp < 1.0 ? rp : 1.0);
+        p.Rm[cell * KB + lane] = (C)(rm < 1.0 ? rm : 1.0);
+        av0 = av1;
+        if (nz < nz1) {  // slide the cluster window to the next layer of this chunk
+            mx[0] = mx[1];
+            mn[0] = mn[1];
+            if (nz + 1 == L) {
+                cluster(nz + 1, mx[1], mn[1]);  // bottom layer uses its own bound
             } else {
-                qmax = cmax(cmax(mx[t][0], mx[t][1]), mx[t][2]) - lo;
-                qmin = cmin(cmin(mn[t][0], mn[t][1]), mn[t][2]) - lo;
-            }
-            const C av1 = p.adfv[t][(size_t)nz * Np + n];
-            double pp = 0.0 + ((double)cmax(C(0), av0[t]) + (double)cmax(C(0), -av1));
-            double pm = 0.0 + ((double)cmin(C(0), av0[t]) + (double)cmin(C(0), -av1));
-            for_edges(m, ne, nz, [&](int e, bool second, int) {
-                C a = p.adfh[t][(size_t)(nz - 1) * Ep + e];
-                if (second) a = -a;
-                pp = pp + (double)cmax(C(0), a);
-                pm = pm + (double)cmin(C(0), a);
-            });
-            const double fp = pp * dt / V + 1e-16;
-            const double fm = pm * dt / V - 1e-16;
-            const double rp = (double)qmax / fp, rm = (double)qmin / fm;
-            p.Rp[t][ix] = (C)(rp < 1.0 ? rp : 1.0);
-            p.Rm[t][ix] = (C)(rm < 1.0 ? rm : 1.0);
-            av0[t] = av1;
-            if (nz < nz1) {  // slide the cluster window to the next layer of this chunk
-                mx[t][0] = mx[t][1];
-                mn[t][0] = mn[t][1];
-                if (nz + 1 == L) {
-                    cluster(t, nz + 1, mx[t][1], mn[t][1]);  // bottom layer uses its own bound
-                } else {
-                    mx[t][1] = mx[t][2];
-                    mn[t][1] = mn[t][2];
-                    if (nz == 1) {  // the surface layer did not need its lower neighbour
-                        cluster(t, nz + 1, mx[t][1], mn[t][1]);
-                    }
-                    cluster(t, nz + 2, mx[t][2], mn[t][2]);
+                mx[1] = mx[2];
+                mn[1] = mn[2];
+                if (nz == 1) {  // the surface layer did not need its lower neighbour
+                    cluster(nz + 1, mx[1], mn[1]);
                 }
+                cluster(nz + 2, mx[2], mn[2]);
             }
         }
     }
@@ -406,10 +435,12 @@ __global__ __launch_bounds__(256) void k_bounds(const MeshCol<M> m, const StepPt
 // Limiting (b3) + flux divergence (oce_tra_adv_flux2dtracer) + caller update.  With acc_h /
 // acc_v the tendencies are accumulated there (do_oce_adv_tra contract); otherwise values +=
 // (dttf_h + dttf_v) / hnode_new in place.  dbg_h / dbg_v receive the limited fluxes if set.
-template <bool FCT, int KT, typename S, typename M, typename C>
+// acc_* and dbg_* hold one tracer: they are only passed with a single-lane mask.
+template <bool FCT, int KB, typename S, typename M, typename C>
 __global__ __launch_bounds__(256) void k_final(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
-    const int n = column_index();
-    if (n >= m.N) return;
+    int n, lane;
+    column_lane<KB>(n, lane);
+    if (n >= m.N || !lane_on(p.mask, lane)) return;
     const int nzmax = m.nlev_n[n], L = nzmax - 1;
     int nz0, nz1;
     if (!chunk_layers(L, lc, nz0, nz1)) return;
@@ -429,103 +460,113 @@ __global__ __launch_bounds__(256) void k_final(const MeshCol<M> m, const StepPtr
         }
         return ae * a;
     };
-    C top[KT], rp0[KT], rm0[KT], rp1[KT], rm1[KT];
-    const size_t i0 = (size_t)(nz0 - 1) * Np + n;
-#pragma unroll
-    for (int t = 0; t < KT; ++t) {
-        const C a0 = p.adfv[t][i0];
+    auto X = [&](size_t cell) { return cell * KB + lane; };
+    C top, rp0 = 0, rm0 = 0, rp1 = 0, rm1 = 0;
+    const size_t c0 = (size_t)(nz0 - 1) * Np + n;
+    {
+        const C a0 = p.adfv[X(c0)];
         if (FCT) {
-            rp1[t] = p.Rp[t][i0];
-            rm1[t] = p.Rm[t][i0];
-            const C rpa = nz0 > 1 ? p.Rp[t][i0 - Np] : C(0);
-            const C rma = nz0 > 1 ? p.Rm[t][i0 - Np] : C(0);
-            rp0[t] = rm0[t] = 0;
-            top[t] = limit(a0, nz0, rpa, rma, rp1[t], rm1[t]);
-            if (p.dbg_v[t]) p.dbg_v[t][i0] = top[t];
+            rp1 = p.Rp[X(c0)];
+            rm1 = p.Rm[X(c0)];
+            const C rpa = nz0 > 1 ? p.Rp[X(c0 - Np)] : C(0);
+            const C rma = nz0 > 1 ? p.Rm[X(c0 - Np)] : C(0);
+            top = limit(a0, nz0, rpa, rma, rp1, rm1);
+            if (p.dbg_v) p.dbg_v[c0] = top;
         } else {
-            top[t] = a0;
+            top = a0;
         }
     }
     for (int nz = nz0; nz <= nz1; ++nz) {
-        const size_t ix = (size_t)(nz - 1) * Np + n;
-        const double V = (double)m.areasvol[ix];
-        const double hn = (double)m.hnode_new[ix];
-        const double h = FCT ? (double)m.hnode[ix] : 0.0;
-#pragma unroll
-        for (int t = 0; t < KT; ++t) {
-            const double tval = (double)ldc<C>(p.T[t], ix);
-            const C ab = p.adfv[t][ix + Np];
-            C bot;
-            double dv = p.acc_v[t] ? p.acc_v[t][ix] : 0.0;
-            if (FCT) {
-                dv = dv - tval * h + (double)p.LO[t][ix] * hn;
-                rp0[t] = rp1[t];
-                rm0[t] = rm1[t];
-                if (nz + 1 <= L) {
-                    rp1[t] = p.Rp[t][ix + Np];
-                    rm1[t] = p.Rm[t][ix + Np];
-                    bot = limit(ab, nz + 1, rp0[t], rm0[t], rp1[t], rm1[t]);
-                } else {
-                    bot = ab;  // bottom interface: never limited (always zero)
-                }
-                if (p.dbg_v[t]) p.dbg_v[t][ix + Np] = bot;
+        const size_t cell = (size_t)(nz - 1) * Np + n;
+        const double V = (double)m.areasvol[cell];
+        const double hn = (double)m.hnode_new[cell];
+        const double h = FCT ? (double)m.hnode[cell] : 0.0;
+        const double tval = (double)ldc<C>(p.T, X(cell));
+        const C ab = p.adfv[X(cell + Np)];
+        C bot;
+        double dv = p.acc_v ? p.acc_v[cell] : 0.0;
+        if (FCT) {
+            dv = dv - tval * h + (double)p.LO[X(cell)] * hn;
+            rp0 = rp1;
+            rm0 = rm1;
+            if (nz + 1 <= L) {
+                rp1 = p.Rp[X(cell + Np)];
+                rm1 = p.Rm[X(cell + Np)];
+                bot = limit(ab, nz + 1, rp0, rm0, rp1, rm1);
             } else {
-                bot = ab;
+                bot = ab;  // bottom interface: never limited (always zero)
             }
-            dv = dv + ((double)top[t] - (double)bot) * dt / V;
-            double dh = p.acc_h[t] ? p.acc_h[t][ix] : 0.0;
-            const C rpn = FCT ? rp0[t] : C(0), rmn = FCT ? rm0[t] : C(0);
-            for_edges(m, ne, nz, [&](int e, bool second, int oth) {
-                C f = p.adfh[t][(size_t)(nz - 1) * Ep + e];
-                if (FCT) {
-                    // R of both end nodes (lib/oce_adv_tra_fct.F90:455-468)
-                    const C rpo = p.Rp[t][(size_t)(nz - 1) * Np + oth];
-                    const C rmo = p.Rm[t][(size_t)(nz - 1) * Np + oth];
-                    const C rp_1 = second ? rpo : rpn, rm_1 = second ? rmo : rmn;  // edges(1)
-                    const C rp_2 = second ? rpn : rpo, rm_2 = second ? rmn : rmo;  // edges(2)
-                    C ae = 1;
-                    if (f >= C(0)) {
-                        ae = cmin(ae, rp_1);
-                        ae = cmin(ae, rm_2);
-                    } else {
-                        ae = cmin(ae, rm_1);
-                        ae = cmin(ae, rp_2);
-                    }
-                    f = ae * f;
-                    if (p.dbg_h[t] && !second) p.dbg_h[t][(size_t)(nz - 1) * Ep + e] = f;
-                }
-                dh = second ? dh - (double)f * dt / V : dh + (double)f * dt / V;
-            });
-            if (p.acc_h[t]) {
-                p.acc_h[t][ix] = dh;
-                p.acc_v[t][ix] = dv;
-            } else {
-                std_(p.T[t], ix, tval + (dh + dv) / hn);
-            }
-            top[t] = bot;
+            if (p.dbg_v) p.dbg_v[cell + Np] = bot;
+        } else {
+            bot = ab;
         }
+        dv = dv + ((double)top - (double)bot) * dt / V;
+        double dh = p.acc_h ? p.acc_h[cell] : 0.0;
+        const C rpn = FCT ? rp0 : C(0), rmn = FCT ? rm0 : C(0);
+        const size_t eo = (size_t)(nz - 1) * Ep, no = (size_t)(nz - 1) * Np;
+        for_edges(m, ne, nz, [&](int e, bool second, int oth) {
+            C f = p.adfh[X(eo + e)];
+            if (FCT) {
+                // R of both end nodes (lib/oce_adv_tra_fct.F90:455-468)
+                const C rpo = p.Rp[X(no + oth)];
+                const C rmo = p.Rm[X(no + oth)];
+                const C rp_1 = second ? rpo : rpn, rm_1 = second ? rmo : rmn;  // edges(1)
+                const C rp_2 = second ? rpn : rpo, rm_2 = second ? rmn : rmo;  // edges(2)
+                C ae = 1;
+                if (f >= C(0)) {
+                    ae = cmin(ae, rp_1);
+                    ae = cmin(ae, rm_2);
+                } else {
+                    ae = cmin(ae, rm_1);
+                    ae = cmin(ae, rp_2);
+                }
+                f = ae * f;
+                if (p.dbg_h && !second) p.dbg_h[eo + e] = f;
+            }
+            double fd = (double)f;
+            if constexpr (!FCT && !std::is_same<C, double>::value) {
+                // single precision without limiter: the stored flux is high-order minus upwind;
+                // the upwind part of valuesAB is added here in double
+                const double an = ldc<double>(p.TAB, X(cell)), ao = ldc<double>(p.TAB, X(no + oth));
+                fd = fd + upw1_edge<double>(second ? ao : an, second ? an : ao, m.q[eo + e]);
+            }
+            dh = second ? dh - fd * dt / V : dh + fd * dt / V;
+        });
+        if (p.acc_h) {
+            p.acc_h[cell] = dh;
+            p.acc_v[cell] = dv;
+        } else if (p.seed != 0u && !std::is_same<S, double>::value) {
+            st_stochastic(p.T, X(cell), tval + (dh + dv) / hn,
+                          unit_hash((uint32_t)(p.tid0 + lane), (uint32_t)cell, p.seed));
+        } else {
+            std_(p.T, X(cell), tval + (dh + dv) / hn);
+        }
+        top = bot;
     }
 }
 
 // ---------------------------------------------------------------------------------------------
 // Renumbering / transposition between the reference layout (nl-1, nod2D) in reference order
-// and the device layout [k][Np] in device order.  dir 0: ref -> dev, dir 1: dev -> ref.
+// and lane `lane` of a device tracer block [k][Np][kb] in device order.
+// dir 0: ref -> dev, dir 1: dev -> ref.
 template <typename S>
 __global__ void k_relayout(const double *__restrict__ ref_in, S *__restrict__ dev_out,
                            const S *__restrict__ dev_in, double *__restrict__ ref_out,
-                           const int *__restrict__ perm, int N, int L, size_t Np, int dir) {
+                           const int *__restrict__ perm, int N, int L, size_t Np, int kb, int lane, int dir) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)N * L) return;
     const int k = (int)(i / N), n = (int)(i - (size_t)k * N);
     const int o = perm[n];  // reference number of device node n
-    if (dir == 0) std_(dev_out, (size_t)k * Np + n, ref_in[(size_t)o * L + k]);
-    else ref_out[(size_t)o * L + k] = ldc<double>(dev_in, (size_t)k * Np + n);
+    const size_t d = ((size_t)k * Np + n) * kb + lane;
+    if (dir == 0) std_(dev_out, d, ref_in[(size_t)o * L + k]);
+    else ref_out[(size_t)o * L + k] = ldc<double>(dev_in, d);
 }
 
-// Stats over a tracer: per-block partial min / max / sum / volume sum over wet layers (or all).
+// Stats over one tracer (lane of a block): per-block partial min / max / sum / volume sum over
+// wet layers (or all layers).
 template <typename S, typename M>
-__global__ __launch_bounds__(256) void k_stats_partial(const MeshCol<M> m, const S *__restrict__ T,
-                                                       int all_levels, double4 *__restrict__ part) {
+__global__ __launch_bounds__(256) void k_stats_partial(const MeshCol<M> m, const S *__restrict__ T, int kb,
+                                                       int lane, int all_levels, double4 *__restrict__ part) {
     __shared__ double smin[256], smax[256], ssum[256], svol[256];
     double mn = __builtin_huge_val(), mx = -__builtin_huge_val(), sm = 0.0, sv = 0.0;
     const size_t total = (size_t)m.L * m.N;
@@ -535,7 +576,7 @@ __global__ __launch_bounds__(256) void k_stats_partial(const MeshCol<M> m, const
         const bool wet = k + 1 <= m.nlev_n[n] - 1;
         if (!all_levels && !wet) continue;
         const size_t ix = (size_t)k * m.Np + n;
-        const double v = ldc<double>(T, ix);
+        const double v = ldc<double>(T, ix * kb + lane);
         mn = v < mn ? v : mn;
         mx = v > mx ? v : mx;
         sm += v;
@@ -560,9 +601,9 @@ __global__ __launch_bounds__(256) void k_stats_partial(const MeshCol<M> m, const
 
 // ---------------------------------------------------------------------------------------------
 // Launchers
-static inline unsigned grid_for(int n_cols, int block) {
-    const unsigned b = (unsigned)((n_cols + block - 1) / block);
-    return (b + kXcds - 1) / kXcds * kXcds;  // multiple of the XCD count (see xcd_block)
+static inline unsigned grid_for(size_t n_threads, int block) {
+    const size_t b = (n_threads + block - 1) / block;
+    return (unsigned)((b + kXcds - 1) / kXcds * kXcds);  // multiple of the XCD count (see xcd_block)
 }
 
 #define LAUNCH_CHECK()                       \
@@ -578,101 +619,135 @@ static inline unsigned grid_for(int n_cols, int block) {
         }                                               \
     } while (0)
 
-template <int HOR, bool FCT, int KT, typename S, typename M, typename C>
+template <int HOR, bool FCT, int KB, typename S, typename M, typename C>
 static void edge_kernel(dim3 g, hipStream_t s, const MeshCol<M> &m, const StepPtrs<S, C> &p, C ph, int lc) {
-    k_edge<HOR, FCT, KT, S, M, C><<<g, 256, 0, s>>>(m, p, ph, lc);
+    k_edge<HOR, FCT, KB, S, M, C><<<g, 256, 0, s>>>(m, p, ph, lc);
 }
 
-template <int KT, typename S, typename M, typename C>
-static hipError_t step_kt(const MeshCol<M> &m, const StepPtrs<S, C> &p, const SchemeArgs &sc,
+template <int KB, typename S, typename M, typename C>
+static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const SchemeArgs &sc,
                           const LaunchCfg &cfg, hipStream_t s, hipEvent_t *ev) {
     const int L = m.L;
-    auto grid = [&](int n_cols, int lc) { return dim3(grid_for(n_cols, 256), (unsigned)((L + lc - 1) / lc)); };
+    auto grid = [&](int n_cols, int lc) {
+        return dim3(grid_for((size_t)n_cols * KB, 256), (unsigned)((L + lc - 1) / lc));
+    };
     const dim3 ge = grid(m.E, cfg.lc_edge), gl = grid(m.N, cfg.lc_lo), gb = grid(m.N, cfg.lc_bounds),
                gf = grid(m.N, cfg.lc_final);
     const C ph = (C)sc.ph, pv = (C)sc.pv;
     MARK(0);
     switch (sc.hor * 2 + (sc.fct ? 1 : 0)) {
-    case HOR_UPW1 * 2 + 0: edge_kernel<HOR_UPW1, false, KT>(ge, s, m, p, ph, cfg.lc_edge); break;
-    case HOR_UPW1 * 2 + 1: edge_kernel<HOR_UPW1, true, KT>(ge, s, m, p, ph, cfg.lc_edge); break;
-    case HOR_MUSCL * 2 + 0: edge_kernel<HOR_MUSCL, false, KT>(ge, s, m, p, ph, cfg.lc_edge); break;
-    case HOR_MUSCL * 2 + 1: edge_kernel<HOR_MUSCL, true, KT>(ge, s, m, p, ph, cfg.lc_edge); break;
-    case HOR_MFCT * 2 + 0: edge_kernel<HOR_MFCT, false, KT>(ge, s, m, p, ph, cfg.lc_edge); break;
-    case HOR_MFCT * 2 + 1: edge_kernel<HOR_MFCT, true, KT>(ge, s, m, p, ph, cfg.lc_edge); break;
+    case HOR_UPW1 * 2 + 0: edge_kernel<HOR_UPW1, false, KB>(ge, s, m, p, ph, cfg.lc_edge); break;
+    case HOR_UPW1 * 2 + 1: edge_kernel<HOR_UPW1, true, KB>(ge, s, m, p, ph, cfg.lc_edge); break;
+    case HOR_MUSCL * 2 + 0: edge_kernel<HOR_MUSCL, false, KB>(ge, s, m, p, ph, cfg.lc_edge); break;
+    case HOR_MUSCL * 2 + 1: edge_kernel<HOR_MUSCL, true, KB>(ge, s, m, p, ph, cfg.lc_edge); break;
+    case HOR_MFCT * 2 + 0: edge_kernel<HOR_MFCT, false, KB>(ge, s, m, p, ph, cfg.lc_edge); break;
+    case HOR_MFCT * 2 + 1: edge_kernel<HOR_MFCT, true, KB>(ge, s, m, p, ph, cfg.lc_edge); break;
     default: return hipErrorInvalidValue;
     }
     LAUNCH_CHECK();
     MARK(1);
     if (sc.fct) {
         switch (sc.ver) {
-        case VER_UPW1: k_lo<VER_UPW1, KT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-        case VER_QR4C: k_lo<VER_QR4C, KT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-        case VER_CDIFF: k_lo<VER_CDIFF, KT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-        case VER_PPM: k_lo<VER_PPM, KT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_UPW1: k_lo<VER_UPW1, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_QR4C: k_lo<VER_QR4C, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_CDIFF: k_lo<VER_CDIFF, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_PPM: k_lo<VER_PPM, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
         default: return hipErrorInvalidValue;
         }
         LAUNCH_CHECK();
         MARK(2);
-        k_bounds<KT><<<gb, 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
+        k_bounds<KB><<<gb, 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
         LAUNCH_CHECK();
         MARK(3);
-        k_final<true, KT><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+        k_final<true, KB><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
     } else {
         switch (sc.ver) {
-        case VER_UPW1: k_vert<VER_UPW1, KT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-        case VER_QR4C: k_vert<VER_QR4C, KT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-        case VER_CDIFF: k_vert<VER_CDIFF, KT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-        case VER_PPM: k_vert<VER_PPM, KT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_UPW1: k_vert<VER_UPW1, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_QR4C: k_vert<VER_QR4C, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_CDIFF: k_vert<VER_CDIFF, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_PPM: k_vert<VER_PPM, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
         default: return hipErrorInvalidValue;
         }
         LAUNCH_CHECK();
         MARK(2);
         MARK(3);
-        k_final<false, KT><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+        k_final<false, KB><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
     }
     LAUNCH_CHECK();
     MARK(4);
     return hipSuccess;
 }
 
+template <typename M>
+hipError_t launch_qflux(const MeshCol<M> &m, hipStream_t s) {
+    if (m.L < 1 || m.E < 1) return hipSuccess;
+    k_qflux<M><<<dim3((unsigned)((m.E + 255) / 256), (unsigned)m.L), 256, 0, s>>>(m);
+    return hipGetLastError();
+}
+
 template <typename S, typename M, typename C>
-hipError_t launch_step(const MeshCol<M> &m, const StepPtrs<S, C> &p, int kt, const SchemeArgs &sc,
+hipError_t launch_step(const MeshCol<M> &m, const StepPtrs<S, C> &p, int kb, const SchemeArgs &sc,
                        const LaunchCfg &cfg, hipStream_t s, hipEvent_t *ev) {
     if (cfg.lc_edge < 1 || cfg.lc_lo < 1 || cfg.lc_bounds < 1 || cfg.lc_final < 1) return hipErrorInvalidValue;
-    if (kt == 1) return step_kt<1>(m, p, sc, cfg, s, ev);
-    if (kt == 2) return step_kt<2>(m, p, sc, cfg, s, ev);
-    return hipErrorInvalidValue;
+    if (kb < 64 && (p.mask >> kb) != 0ull) return hipErrorInvalidValue;  // lanes beyond the block
+    switch (kb) {
+    case 1: return step_kb<1>(m, p, sc, cfg, s, ev);
+    case 2: return step_kb<2>(m, p, sc, cfg, s, ev);
+    case 4: return step_kb<4>(m, p, sc, cfg, s, ev);
+    case 8: return step_kb<8>(m, p, sc, cfg, s, ev);
+    case 16: return step_kb<16>(m, p, sc, cfg, s, ev);
+    case 32: return step_kb<32>(m, p, sc, cfg, s, ev);
+    case 64: return step_kb<64>(m, p, sc, cfg, s, ev);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <typename S>
 hipError_t launch_relayout(const double *ref_in, S *dev_out, const S *dev_in, double *ref_out,
-                           const int *perm, int N, int L, size_t Np, int dir, hipStream_t s) {
+                           const int *perm, int N, int L, size_t Np, int kb, int lane, int dir, hipStream_t s) {
     const size_t total = (size_t)N * L;
     if (total == 0) return hipSuccess;
+    if (lane < 0 || lane >= kb) return hipErrorInvalidValue;
     k_relayout<S><<<dim3((unsigned)((total + 255) / 256)), 256, 0, s>>>(ref_in, dev_out, dev_in, ref_out,
-                                                                        perm, N, L, Np, dir);
+                                                                        perm, N, L, Np, kb, lane, dir);
     return hipGetLastError();
 }
 
 template <typename S, typename M>
-hipError_t launch_stats(const MeshCol<M> &m, const S *T, int all_levels, double4 *part, int nblocks,
-                        hipStream_t s) {
-    k_stats_partial<S, M><<<nblocks, 256, 0, s>>>(m, T, all_levels, part);
+hipError_t launch_stats(const MeshCol<M> &m, const S *T, int kb, int lane, int all_levels, double4 *part,
+                        int nblocks, hipStream_t s) {
+    if (lane < 0 || lane >= kb) return hipErrorInvalidValue;
+    k_stats_partial<S, M><<<nblocks, 256, 0, s>>>(m, T, kb, lane, all_levels, part);
     return hipGetLastError();
 }
 
-// Instantiations: FP64 (bit-exact path) and FP32 arithmetic with FP32 / FP16 / BF16 storage.
-template hipError_t launch_step<double, double, double>(const MeshCol<double> &, const StepPtrs<double, double> &, int, const SchemeArgs &, const LaunchCfg &, hipStream_t, hipEvent_t *);
-template hipError_t launch_step<float, float, float>(const MeshCol<float> &, const StepPtrs<float, float> &, int, const SchemeArgs &, const LaunchCfg &, hipStream_t, hipEvent_t *);
-template hipError_t launch_step<f16_t, float, float>(const MeshCol<float> &, const StepPtrs<f16_t, float> &, int, const SchemeArgs &, const LaunchCfg &, hipStream_t, hipEvent_t *);
-template hipError_t launch_step<bf16_t, float, float>(const MeshCol<float> &, const StepPtrs<bf16_t, float> &, int, const SchemeArgs &, const LaunchCfg &, hipStream_t, hipEvent_t *);
-template hipError_t launch_relayout<double>(const double *, double *, const double *, double *, const int *, int, int, size_t, int, hipStream_t);
-template hipError_t launch_relayout<float>(const double *, float *, const float *, double *, const int *, int, int, size_t, int, hipStream_t);
-template hipError_t launch_relayout<f16_t>(const double *, f16_t *, const f16_t *, double *, const int *, int, int, size_t, int, hipStream_t);
-template hipError_t launch_relayout<bf16_t>(const double *, bf16_t *, const bf16_t *, double *, const int *, int, int, size_t, int, hipStream_t);
-template hipError_t launch_stats<double, double>(const MeshCol<double> &, const double *, int, double4 *, int, hipStream_t);
-template hipError_t launch_stats<float, float>(const MeshCol<float> &, const float *, int, double4 *, int, hipStream_t);
-template hipError_t launch_stats<f16_t, float>(const MeshCol<float> &, const f16_t *, int, double4 *, int, hipStream_t);
-template hipError_t launch_stats<bf16_t, float>(const MeshCol<float> &, const bf16_t *, int, double4 *, int, hipStream_t);
+// Instantiations: FP64 (bit-exact path) and FP32 reconstruction arithmetic with FP32 / FP16 /
+// BF16 storage (FP64 mesh and velocities in every mode).
+// FESOMTRA_INST selects a subset (bit 0: FP64, 1: FP32, 2: FP16 storage, 3: BF16 storage) so the
+// four families can be compiled as separate objects in parallel.
+#ifndef FESOMTRA_INST
+#define FESOMTRA_INST 15
+#endif
+#define FESOMTRA_INSTANTIATE(S_, M_, C_)                                                                    \
+    template hipError_t launch_step<S_, M_, C_>(const MeshCol<M_> &, const StepPtrs<S_, C_> &, int,         \
+                                                const SchemeArgs &, const LaunchCfg &, hipStream_t,         \
+                                                hipEvent_t *);                                              \
+    template hipError_t launch_relayout<S_>(const double *, S_ *, const S_ *, double *, const int *, int, \
+                                            int, size_t, int, int, int, hipStream_t);                      \
+    template hipError_t launch_stats<S_, M_>(const MeshCol<M_> &, const S_ *, int, int, int, double4 *,    \
+                                             int, hipStream_t);
+#if FESOMTRA_INST & 1
+FESOMTRA_INSTANTIATE(double, double, double)
+template hipError_t launch_qflux<double>(const MeshCol<double> &, hipStream_t);
+#endif
+#if FESOMTRA_INST & 2
+FESOMTRA_INSTANTIATE(float, double, float)
+#endif
+#if FESOMTRA_INST & 4
+FESOMTRA_INSTANTIATE(f16_t, double, float)
+#endif
+#if FESOMTRA_INST & 8
+FESOMTRA_INSTANTIATE(bf16_t, double, float)
+#endif
 
 }  // namespace fesomtra

@@ -17,19 +17,24 @@ struct LaunchCfg {
     int lc_edge = 16, lc_lo = 16, lc_bounds = 16, lc_final = 16;
 };
 
-// One step of kt (1 or 2) tracers sharing one scheme.  ev, when not null, holds 5 events recorded
-// around the 4 stages (edge fluxes, low-order / vertical, bounds, final).
+// One step of the lanes p.mask of a tracer block of width kb (1, 2, 4, ..., 64); the selected
+// lanes share the scheme sc.  ev, when not null, holds 5 events recorded around the 4 stages
+// (edge fluxes, low-order / vertical, bounds, final).
+// Edge transport q = m.q of the current velocity (once per step, before the tracer passes).
+template <typename M>
+hipError_t launch_qflux(const MeshCol<M> &m, hipStream_t s);
+
 template <typename S, typename M, typename C>
-hipError_t launch_step(const MeshCol<M> &m, const StepPtrs<S, C> &p, int kt, const SchemeArgs &sc,
+hipError_t launch_step(const MeshCol<M> &m, const StepPtrs<S, C> &p, int kb, const SchemeArgs &sc,
                        const LaunchCfg &cfg, hipStream_t s, hipEvent_t *ev);
 
-// dir 0: reference layout (nl-1, nod2D) float64 -> device layout S; dir 1: back.
+// dir 0: reference layout (nl-1, nod2D) float64 -> lane of a device tracer block; dir 1: back.
 template <typename S>
 hipError_t launch_relayout(const double *ref_in, S *dev_out, const S *dev_in, double *ref_out,
-                           const int *perm, int N, int L, size_t Np, int dir, hipStream_t s);
+                           const int *perm, int N, int L, size_t Np, int kb, int lane, int dir, hipStream_t s);
 
 template <typename S, typename M>
-hipError_t launch_stats(const MeshCol<M> &m, const S *T, int all_levels, double4 *part, int nblocks,
-                        hipStream_t s);
+hipError_t launch_stats(const MeshCol<M> &m, const S *T, int kb, int lane, int all_levels, double4 *part,
+                        int nblocks, hipStream_t s);
 
 }  // namespace fesomtra

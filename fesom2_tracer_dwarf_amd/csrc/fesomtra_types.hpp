@@ -6,14 +6,20 @@
 //   node-interface  a[j * Np + n]      j = interface-1 in [0, nl)
 //   element-layer   a[k * Tp + el],    velocities uv[(k * Tp + el) * 2 + c]
 //   edge-layer      a[k * Ep + e]
-// One thread owns one column (node, element or edge) and walks its wet levels, so per-entity
-// data is loaded once per column and every per-level load is coalesced over consecutive entities.
+// Tracers and per-tracer work arrays are stored in blocks of KB tracers interleaved innermost:
+//   tracer block    t[(k * Np + n) * KB + lane]     (edge work arrays with Ep in place of Np)
+// One thread owns one (column, lane) pair and walks the column's wet levels; the KB lanes of a
+// column are adjacent threads, so mesh values are fetched once per column for all KB tracers and
+// every tracer access is a contiguous run of KB values over consecutive columns.
 //
-// Precision policy: S = tracer storage, M = mesh / velocity values, C = flux and reconstruction
-// arithmetic and the stored intermediates; the nodal flux-divergence sums, the low-order
-// solution, the limiter ratios and the tracer update always run in double.  With
-// S = M = C = double the kernels keep the reference's operation order and are bit-identical to
-// the serial FP64 reference.
+// Precision policy: S = tracer storage, M = mesh / velocity values (double), C = high-order
+// reconstruction arithmetic and the stored intermediates.  The edge transport q, the low-order
+// fluxes and solution, the vertical fluxes, the nodal flux-divergence sums, the limiter ratios and
+// the tracer update always run in double.  With S = C = double the kernels keep the reference's
+// operation order and are bit-identical to the serial FP64 reference; with C = float the
+// antidiffusive edge flux is evaluated in an algebraically equivalent difference form (exactly
+// zero for a uniform tracer) so that single-precision rounding does not break the discrete
+// cancellation of a divergence-free transport.
 #pragma once
 
 #include <hip/hip_bf16.h>
@@ -27,7 +33,7 @@ namespace fesomtra {
 
 constexpr double kREarth = 6367500.0;  // lib/oce_modules.F90:24
 constexpr int kXcds = 8;               // MI355X: 8 accelerator complex dies, one L2 each
-constexpr int kMaxKT = 2;              // tracers per thread in the column kernels
+constexpr int kMaxKB = 64;             // widest tracer block (one wavefront of lanes)
 constexpr int kDegCache = 8;           // node->edge entries held in registers
 
 enum Hor { HOR_UPW1 = 0, HOR_MUSCL = 1, HOR_MFCT = 2 };
@@ -49,6 +55,45 @@ __device__ __forceinline__ void std_(T *__restrict__ p, size_t i, double v) {
     if constexpr (std::is_same<T, f16_t>::value) p[i] = __float2half_rn((float)v);
     else if constexpr (std::is_same<T, bf16_t>::value) p[i] = __float2bfloat16((float)v);
     else p[i] = (T)v;
+}
+
+// Stochastic rounding of the tracer update into single / half storage: rounds v up with
+// probability (v - down) / (up - down), with `u` uniform in [0, 1), so the stored value is exact
+// in expectation and slowly varying fields neither stagnate nor drift.
+__device__ __forceinline__ uint32_t mix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+__device__ __forceinline__ double unit_hash(uint32_t a, uint32_t b, uint32_t c) {
+    const uint32_t h = mix32(a * 0x9e3779b1u ^ mix32(b * 0x7feb352du ^ mix32(c + 0x68e31da4u)));
+    return (double)(h >> 8) * (1.0 / 16777216.0);
+}
+__device__ __forceinline__ float bf16_bits_to_float(uint32_t b) { return __uint_as_float(b << 16); }
+template <typename T>
+__device__ __forceinline__ void st_stochastic(T *__restrict__ p, size_t i, double v, double u) {
+    if constexpr (std::is_same<T, double>::value) {
+        p[i] = v;
+    } else if constexpr (std::is_same<T, float>::value) {
+        const float lo = __double2float_rd(v), hi = __double2float_ru(v);
+        p[i] = (lo == hi || u >= (v - (double)lo) / ((double)hi - (double)lo)) ? lo : hi;
+    } else if constexpr (std::is_same<T, f16_t>::value) {
+        const __half lo = __float2half_rd(__double2float_rd(v)), hi = __float2half_ru(__double2float_ru(v));
+        const double dl = (double)__half2float(lo), dh = (double)__half2float(hi);
+        p[i] = (dl == dh || u >= (v - dl) / (dh - dl)) ? lo : hi;
+    } else {
+        const float f_lo = __double2float_rd(v), f_hi = __double2float_ru(v);
+        const uint32_t bl = __float_as_uint(f_lo), bh = __float_as_uint(f_hi);
+        // bf16 neighbours: truncate the float bit patterns toward -inf / +inf
+        const uint32_t dn = (bl & 0xffffu) == 0u ? bl >> 16 : (f_lo >= 0.f ? bl >> 16 : (bl >> 16) + 1u);
+        const uint32_t up = (bh & 0xffffu) == 0u ? bh >> 16 : (f_hi >= 0.f ? (bh >> 16) + 1u : bh >> 16);
+        const double dl = (double)bf16_bits_to_float(dn), dh = (double)bf16_bits_to_float(up);
+        const uint32_t pick = (dl == dh || u >= (v - dl) / (dh - dl)) ? dn : up;
+        p[i] = __float2bfloat16(bf16_bits_to_float(pick));
+    }
 }
 
 template <typename C>
@@ -83,20 +128,24 @@ struct MeshCol {
     const int2 *updn;                     // up / down triangle (-1 none)
     const M *ecd;                         // (E, 4) edge_cross_dxdy
     const M *edxdy;                       // (E, 2) edge_dxdy
+    double *q;                            // [L][Ep] edge transport of the current velocity
 };
 
-// Per-slot (tracer within a thread) pointers of one step.
+// Pointers of one step over one tracer block (arrays interleaved by the block width KB).
 template <typename S, typename C>
 struct StepPtrs {
-    S *T[kMaxKT];          // values (updated in place unless accumulating)
-    const S *TAB[kMaxKT];  // valuesAB (may equal T)
-    C *flo[kMaxKT];        // (E) low-order edge flux
-    C *adfh[kMaxKT];       // (E) antidiffusive (or high-order) edge flux
-    C *LO[kMaxKT];         // (N) low-order solution
-    C *Rp[kMaxKT], *Rm[kMaxKT];  // (N) limiter ratios
-    C *adfv[kMaxKT];       // (N interfaces) antidiffusive (or high-order) vertical flux
-    double *acc_h[kMaxKT], *acc_v[kMaxKT];  // do_oce_adv_tra accumulators (or null)
-    C *dbg_h[kMaxKT], *dbg_v[kMaxKT];       // limited fluxes for inspection (or null)
+    S *T;                  // values (updated in place unless accumulating)
+    const S *TAB;          // valuesAB (may equal T)
+    double *flo;           // [L][Ep][KB] low-order edge flux (FP64 path only)
+    C *adfh;               // [L][Ep][KB] antidiffusive (or high-order) edge flux
+    C *LO;                 // [L][Np][KB] low-order solution
+    C *Rp, *Rm;            // [L][Np][KB] limiter ratios
+    C *adfv;               // [nl][Np][KB] antidiffusive (or high-order) vertical flux
+    double *acc_h, *acc_v; // do_oce_adv_tra accumulators [L][Np] of the single selected lane, or null
+    C *dbg_h, *dbg_v;      // limited fluxes [L][Ep], [nl][Np] of the single selected lane, or null
+    unsigned long long mask;  // lanes advanced by this launch (tracers sharing its scheme)
+    int tid0;                 // tracer number of lane 0 (seeds the stochastic rounding)
+    uint32_t seed;            // step counter (seeds the stochastic rounding); 0: round to nearest
 };
 
 }  // namespace fesomtra

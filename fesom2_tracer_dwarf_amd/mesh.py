@@ -393,3 +393,88 @@ def smooth_bathymetry_levels(mesh_nx: int, mesh_ny: int, nl: int, lo: int, hi: i
     f = (f - f.min()) / (f.max() - f.min())
     lev = lo + np.floor(f * (hi - lo + 1)).astype(np.int64)
     return np.clip(lev, lo, hi)
+
+
+def _pad_levels(a: np.ndarray, width: int) -> np.ndarray:
+    """Pads the level axis (last) to ``width`` by repeating the deepest value; the padding lies
+    below every column's bottom and is never read."""
+    if a.shape[-1] == width:
+        return a
+    pad = [(0, 0)] * (a.ndim - 1) + [(0, width - a.shape[-1])]
+    return np.pad(a, pad, mode="edge")
+
+
+def union_mesh(meshes) -> Mesh:
+    """Disjoint union of independent meshes as one mesh (no edge or element joins two of them).
+
+    Node, element and edge numbers of mesh i are offset by the counts of meshes 0..i-1; the level
+    axis is padded to the largest nl; coordinates are shifted along x so that the meshes occupy
+    disjoint strips (a space-filling-curve ordering keeps each mesh contiguous).  A step of the
+    union is a step of every member mesh: the stencils never cross between components.
+    """
+    meshes = list(meshes)
+    nl = max(m.nl for m in meshes)
+    max_adj = max(m.max_adj for m in meshes)
+    no = eo = to = 0
+    x_off = 0.0
+    parts = {k: [] for k in ("coord", "xy", "enodes", "edges", "etri", "nie", "nie_num", "nlev",
+                             "ulev", "nlev_n", "ulev_n", "nlev_min", "ulev_max", "elem_area",
+                             "area", "areasvol", "edxdy", "ecd", "ecos", "gsca", "hnode",
+                             "hnode_new", "helem", "zbar3", "Z3", "updn", "nbl")}
+
+    def shift(a, off):
+        a = np.asarray(a, dtype=np.int64)
+        return np.where(a > 0, a + off, 0).astype(np.int32)
+
+    for m in meshes:
+        c = m.coord_nod2D.copy()
+        c[:, 0] += x_off
+        parts["coord"].append(c)
+        width = float(np.ptp(m.coord_nod2D[:, 0])) if m.nod2D > 1 else 0.0
+        x_off += width * 1.5 + 1e-3
+        if m.xy_m is not None:
+            parts["xy"].append(m.xy_m)
+        parts["enodes"].append(shift(m.elem2D_nodes, no))
+        parts["edges"].append(shift(m.edges, no))
+        parts["etri"].append(shift(m.edge_tri, to))
+        nie = np.zeros((m.nod2D, max_adj), dtype=np.int32)
+        nie[:, :m.max_adj] = shift(m.nod_in_elem2D, to)
+        parts["nie"].append(nie)
+        parts["nie_num"].append(m.nod_in_elem2D_num)
+        parts["nlev"].append(m.nlevels)
+        parts["ulev"].append(m.ulevels)
+        parts["nlev_n"].append(m.nlevels_nod2D)
+        parts["ulev_n"].append(m.ulevels_nod2D)
+        parts["nlev_min"].append(m.nlevels_nod2D_min)
+        parts["ulev_max"].append(m.ulevels_nod2D_max)
+        parts["elem_area"].append(m.elem_area)
+        parts["area"].append(_pad_levels(m.area, nl))
+        parts["areasvol"].append(_pad_levels(m.areasvol, nl))
+        parts["edxdy"].append(m.edge_dxdy)
+        parts["ecd"].append(m.edge_cross_dxdy)
+        parts["ecos"].append(m.elem_cos)
+        parts["gsca"].append(m.gradient_sca)
+        parts["hnode"].append(_pad_levels(m.hnode, nl - 1))
+        parts["hnode_new"].append(_pad_levels(m.hnode_new, nl - 1))
+        parts["helem"].append(_pad_levels(m.helem, nl - 1))
+        parts["zbar3"].append(_pad_levels(m.zbar_3d_n, nl))
+        parts["Z3"].append(_pad_levels(m.Z_3d_n, nl - 1))
+        parts["updn"].append(shift(m.edge_up_dn_tri, to))
+        parts["nbl"].append(m.nboundary_lay)
+        no += m.nod2D
+        eo += m.edge2D
+        to += m.elem2D
+    deepest = max(meshes, key=lambda m: m.nl)
+    cat = {k: np.concatenate(v) for k, v in parts.items() if v}
+    return Mesh(nl=nl, nod2D=no, elem2D=to, edge2D=eo, max_adj=max_adj,
+                coord_nod2D=cat["coord"], elem2D_nodes=cat["enodes"], edges=cat["edges"],
+                edge_tri=cat["etri"], nod_in_elem2D=cat["nie"], nod_in_elem2D_num=cat["nie_num"],
+                nlevels=cat["nlev"], ulevels=cat["ulev"], nlevels_nod2D=cat["nlev_n"],
+                ulevels_nod2D=cat["ulev_n"], nlevels_nod2D_min=cat["nlev_min"],
+                ulevels_nod2D_max=cat["ulev_max"], elem_area=cat["elem_area"], area=cat["area"],
+                areasvol=cat["areasvol"], edge_dxdy=cat["edxdy"], edge_cross_dxdy=cat["ecd"],
+                elem_cos=cat["ecos"], gradient_sca=cat["gsca"], zbar=deepest.zbar.copy(),
+                Z=deepest.Z.copy(), hnode=cat["hnode"], hnode_new=cat["hnode_new"],
+                helem=cat["helem"], zbar_3d_n=cat["zbar3"], Z_3d_n=cat["Z3"],
+                edge_up_dn_tri=cat["updn"], nboundary_lay=cat["nbl"],
+                xy_m=cat.get("xy"))

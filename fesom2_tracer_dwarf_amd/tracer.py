@@ -111,6 +111,42 @@ class FESOMTracerAdvection:
             abi.check(L.tracer_init_complete(), "tracer_init_complete")
         self._keep = []
 
+    # --- execution options ------------------------------------------------------------------------
+    def set_precision(self, mode: int) -> None:
+        """0 FP64, 1 FP32, 2 FP32 arithmetic + FP16 storage, 3 FP32 arithmetic + BF16 storage."""
+        abi.check(self.lib.tracer_set_precision(mode), "tracer_set_precision")
+
+    def set_rounding(self, stochastic: bool) -> None:
+        """Stochastic (default) or round-to-nearest storage of the update in modes 1-3."""
+        abi.check(self.lib.tracer_set_rounding(int(bool(stochastic))), "tracer_set_rounding")
+
+    def set_block_width(self, kb: int) -> None:
+        """Tracers interleaved per block and advanced per kernel pass (power of two, 0 = auto)."""
+        abi.check(self.lib.tracer_set_tracers_per_pass(kb), "tracer_set_tracers_per_pass")
+
+    def block_width(self) -> int:
+        kb = ctypes.c_int()
+        abi.check(self.lib.tracer_get_tracers_per_pass(ctypes.byref(kb)), "tracer_get_tracers_per_pass")
+        return kb.value
+
+    def set_graph(self, enable: bool) -> None:
+        abi.check(self.lib.tracer_set_graph(int(bool(enable))), "tracer_set_graph")
+
+    def set_launch_config(self, lc) -> None:
+        abi.check(self.lib.tracer_set_launch_config(*[int(x) for x in lc]), "tracer_set_launch_config")
+
+    def launch_config(self):
+        cfg = (ctypes.c_int * 4)()
+        abi.check(self.lib.tracer_get_launch_config(cfg), "tracer_get_launch_config")
+        return tuple(cfg)
+
+    def kernel_times(self):
+        """Mean ms per pass of the four stages since tracer_set_kernel_timing(1), and the count."""
+        ms = (ctypes.c_double * 4)()
+        cnt = ctypes.c_long()
+        abi.check(self.lib.tracer_get_kernel_times(ms, ctypes.byref(cnt)), "tracer_get_kernel_times")
+        return list(ms), cnt.value
+
     # --- stepping ---------------------------------------------------------------------------------
     def advect_steps(self, nsteps: int, dt: float) -> float:
         """Advance every tracer nsteps times; returns the device time in ms."""

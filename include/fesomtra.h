@@ -159,12 +159,27 @@ int tracer_get_kernel_times(double *stage_ms, long *count);
  * In modes 1-3 the nodal flux-divergence sums, the low-order solution, the limiter ratios, the
  * tracer update and the global sums stay in FP64. */
 int tracer_set_precision(int mode);
-/* Tracers advanced per kernel pass (1 or 2, default 1): consecutive tracers with the same scheme
- * share one pass, so the mesh and velocities are read once for both. */
-int tracer_set_tracers_per_pass(int kt);
+/* Rounding of the tracer update into single / half / bfloat16 storage (modes 1-3): 1 =
+ * stochastic (default; rounds up with probability equal to the remainder, from a counter-based
+ * hash of tracer number, cell and step, so results are reproducible and independent of the block
+ * width), 0 = round to nearest.  Without it, increments smaller than half a unit in the last place
+ * are lost at every step and slowly varying fields stagnate. */
+int tracer_set_rounding(int stochastic);
+/* Tracer block width kb (a power of two up to 64; 0 = automatic, the default: all tracers, up to
+ * 64, in one block).  Tracer id i is stored as lane (i-1) % kb of block (i-1) / kb, interleaved
+ * innermost in HBM, and one pass of the four kernels advances every lane of a block that shares
+ * a scheme, so the mesh and velocities are read once for all of them.  Stored values are carried
+ * over when the width changes; results do not depend on it. */
+int tracer_set_tracers_per_pass(int kb);
+int tracer_get_tracers_per_pass(int *kb);
+/* enable = 1: tracer_advect_steps(nsteps, dt) captures its whole launch sequence (nsteps x the
+ * passes of every block) into one HIP graph at the first call and replays it for later calls
+ * with the same nsteps and dt until the scheme, precision, width or launch configuration
+ * changes.  Per-stage kernel timing, when enabled, uses the uncaptured path. */
+int tracer_set_graph(int enable);
 /* Levels per thread of the four kernels (edge fluxes, low-order / vertical, bounds, final): each
- * column is split into chunks of that many layers, one thread per chunk.  Defaults come from the
- * autotuner's table; results do not depend on the choice. */
+ * column is split into chunks of that many layers, one thread per chunk and tracer lane.  The
+ * default is 16 for each; results do not depend on the choice. */
 int tracer_set_launch_config(int lc_edge, int lc_lo, int lc_bounds, int lc_final);
 int tracer_get_launch_config(int *cfg4);
 /* Last error message (empty if none). */

@@ -21,14 +21,17 @@ def rel_linf(got, ref):
     return float(np.max(np.abs(got - ref)) / scale)
 
 
-def gpu_run(case, nsteps, kt=None, lc=None):
+def gpu_run(case, nsteps, kb=0, lc=None, prec=0):
     from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
     t = FESOMTracerAdvection()
-    t.lib.tracer_set_tracers_per_pass(kt or 1)
-    t.lib.tracer_set_launch_config(*(lc or (16, 16, 16, 16)))
+    t.set_block_width(kb)
+    t.set_launch_config(lc or (16, 16, 16, 16))
+    t.set_precision(prec)
     t.setup_case(case)
     t.advect_steps(nsteps, case.dt)
     out = t.get_all_values()
+    t.set_precision(0)
+    t.set_block_width(0)
     t.cleanup()
     return out
 
@@ -63,23 +66,25 @@ def test_schemes_bathymetry_matches_oracle(oracle_mod, hor, ver, lim):
     assert err <= RTOL
 
 
-def test_dropin_do_oce_adv_tra_matches_oracle(oracle_mod):
+@pytest.mark.parametrize("tid", [1, 2])
+def test_dropin_do_oce_adv_tra_matches_oracle(oracle_mod, tid):
     """tracer_do_oce_adv_tra: tendencies, low-order solution and limited fluxes, with a valuesAB
-    different from values."""
+    different from values (tracer 2 is lane 1 of a two-tracer block)."""
     from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
     case = configs.bathymetry_case("bathy16", 16, 14, seed=5, nsteps=1, lev_lo=5, w_cfl=0.3)
     rng = np.random.default_rng(0)
-    vals = case.tracers[0]
+    vals = case.tracers[tid - 1]
     vab = vals + 0.01 * rng.standard_normal(vals.shape) * (vals != 0)
-    sc = case.schemes[0]
-    say("case bathy16 drop-in do_oce_adv_tra MFCT/QR4C/FCT")
+    sc = case.schemes[tid - 1]
+    say(f"case bathy16 drop-in do_oce_adv_tra MFCT/QR4C/FCT tracer {tid}")
     o = oracle_mod.Oracle(case)
     ref = oracle_mod.DropIn(o).run(vals, vab, sc, case.dt)
     t = FESOMTracerAdvection()
     t.setup_case(case)
-    t.set_values_ab(1, vab)
+    assert t.block_width() == 2
+    t.set_values_ab(tid, vab)
     t.zero_tendencies()
-    t.do_oce_adv_tra(case.dt, 1)
+    t.do_oce_adv_tra(case.dt, tid)
     h, v = t.get_tendencies()
     got = {"dttf_h": h, "dttf_v": v, "fct_LO": t.get_work("fct_LO"),
            "adv_flux_hor": t.get_work("adv_flux_hor"), "adv_flux_ver": t.get_work("adv_flux_ver")}
@@ -118,19 +123,114 @@ CHUNK_SCHEMES = [Scheme("MFCT", "QR4C", "FCT", 0.85, 0.75), Scheme("MUSCL", "PPM
 
 
 @pytest.mark.parametrize("lc", [(1, 1, 1, 1), (2, 3, 5, 7), (4, 4, 4, 4), (64, 64, 64, 64)])
-@pytest.mark.parametrize("kt", [1, 2])
+@pytest.mark.parametrize("kb", [1, 2, 4])
 @pytest.mark.parametrize("si", range(len(CHUNK_SCHEMES)))
-def test_level_chunks_and_tracer_pairs(oracle_mod, lc, kt, si):
-    """Results do not depend on how columns are split into level chunks or on pairing tracers."""
-    case = configs.bathymetry_case("bathy12", 12, 13, seed=17, nsteps=2, lev_lo=3,
+def test_level_chunks_and_block_widths(oracle_mod, lc, kb, si):
+    """Results do not depend on how columns are split into level chunks or on how the three
+    tracers are interleaved into blocks (kb=2: a full and a half block, kb=4: one unused lane)."""
+    case = configs.bathymetry_case("bathy12", 12, 13, seed=17, ntracers=3, nsteps=2, lev_lo=3,
                                    scheme=CHUNK_SCHEMES[si], w_cfl=0.3)
-    say(f"case bathy12 scheme {si} lc={lc} kt={kt}")
+    say(f"case bathy12 scheme {si} lc={lc} kb={kb}")
     ref, _ = oracle_mod.Oracle(case).run(save_steps=[2])
-    got = gpu_run(case, 2, kt=kt, lc=lc)
+    got = gpu_run(case, 2, kb=kb, lc=lc)
     assert rel_linf(got, ref[2]) <= RTOL
 
 
-PREC_BOUNDS = {1: 1e-5, 2: 5e-3, 3: 3e-2}  # rel L-inf after 5 steps vs FP64: FP32, FP16, BF16 storage
+def test_mixed_schemes_in_one_block(oracle_mod):
+    """Four tracers with three different schemes in one block of width 4: one masked pass per
+    scheme."""
+    case = configs.bathymetry_case("bathy12m", 12, 13, seed=19, ntracers=4, nsteps=3, lev_lo=3,
+                                   w_cfl=0.3)
+    case.schemes = [Scheme("MFCT", "QR4C", "FCT", 0.85, 0.75), Scheme("UPW1", "PPM", "NONE"),
+                    Scheme("MFCT", "QR4C", "FCT", 0.85, 0.75), Scheme("MUSCL", "CDIFF", "FCT", 0.9, 0.6)]
+    say("case bathy12m mixed schemes kb=4")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[3])
+    got = gpu_run(case, 3, kb=4)
+    assert rel_linf(got, ref[3]) <= RTOL
+
+
+def test_graph_replays_match_oracle(oracle_mod):
+    """advect_steps captured into one HIP graph and replayed three times, with eager work
+    (statistics, a value read-back) between the replays; every replay is checked."""
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    case = configs.bathymetry_case("bathy14g", 14, 12, seed=29, ntracers=3, nsteps=6, lev_lo=4,
+                                   w_cfl=0.3)
+    say("case bathy14g graph replays")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[2, 4, 6])
+    t = FESOMTracerAdvection()
+    t.set_graph(True)
+    t.setup_case(case)
+    for step in (2, 4, 6):
+        t.advect_steps(2, case.dt)
+        t.stats(1)
+        got = t.get_all_values()
+        say(f"  after {step} steps: rel Linf {rel_linf(got, ref[step]):.3e}")
+        assert rel_linf(got, ref[step]) <= RTOL, step
+    t.set_graph(False)
+    t.cleanup()
+
+
+def test_block_width_change_keeps_values(oracle_mod):
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    case = configs.bathymetry_case("bathy10w", 10, 11, seed=31, ntracers=3, nsteps=2, lev_lo=3,
+                                   w_cfl=0.3)
+    say("case bathy10w width 1 -> 4 -> 2")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[1, 2])
+    t = FESOMTracerAdvection()
+    t.set_block_width(1)
+    t.setup_case(case)
+    t.advect_steps(1, case.dt)
+    t.set_block_width(4)
+    assert rel_linf(t.get_all_values(), ref[1]) <= RTOL
+    t.advect_steps(1, case.dt)
+    t.set_block_width(2)
+    got = t.get_all_values()
+    t.set_block_width(0)
+    t.cleanup()
+    assert rel_linf(got, ref[2]) <= RTOL
+
+
+def test_fp32_batch_of_64_equals_single_lanes():
+    """FP32 arithmetic: 64 tracers in one interleaved block give bit-identical results to the
+    same tracers advanced one per block (each lane runs the same operations)."""
+    case = configs.bathymetry_case("bathy8x64", 8, 10, seed=37, ntracers=64, nsteps=2, lev_lo=3,
+                                   w_cfl=0.3)
+    say("case bathy8x64 FP32 kb=64 vs kb=1")
+    a = gpu_run(case, 2, kb=64, prec=1)
+    b = gpu_run(case, 2, kb=1, prec=1)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_uniform_field_preserved_every_precision(mode):
+    """A uniform tracer on a full-depth mesh under a divergence-free flow stays uniform: to FP64
+    round-off in FP64, and exactly in the single-precision modes (low-order fluxes in double,
+    antidiffusive fluxes and gradients in difference form)."""
+    case = configs.bathymetry_case("flat12", 12, 10, seed=41, ntracers=2, nsteps=20, lev_lo=10,
+                                   lev_hi=10)
+    wet = configs.wet_mask(case.mesh)
+    case.tracers[:] = np.where(wet, 35.0, 0.0)[None]
+    say(f"case flat12 uniform 35.0, precision mode {mode}")
+    got = gpu_run(case, 20, prec=mode)
+    err = float(np.max(np.abs(got[:, wet] - 35.0)))
+    say(f"  max |T - 35| = {err:.3e}")
+    assert err <= (1e-12 * 35.0 if mode == 0 else 0.0)
+
+
+def test_fp32_200_steps_within_1e5_of_fp64(oracle_mod):
+    """FP32 arithmetic and storage (FP64 accumulation, update, limiter ratios) over 200 steps
+    against the FP64 reference restatement: relative L-infinity <= 1e-5."""
+    case = configs.bathymetry_case("bathy20", 20, 16, seed=23, ntracers=2, nsteps=200, lev_lo=5,
+                                   w_cfl=0.3)
+    say("case bathy20 FP32 200 steps")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[200])
+    got = gpu_run(case, 200, prec=1)
+    err = rel_linf(got, ref[200])
+    say(f"  rel Linf {err:.3e}")
+    assert err <= 1e-5
+
+
+PREC_BOUNDS = {1: 1e-6, 2: 5e-3, 3: 3e-2}  # rel L-inf after 5 steps vs FP64: FP32, FP16, BF16 storage
 
 
 @pytest.mark.parametrize("mode", [1, 2, 3])
@@ -142,9 +242,8 @@ def test_mixed_precision_close_to_fp64(oracle_mod, mode):
     say(f"case bathy20 precision mode {mode}")
     ref, _ = oracle_mod.Oracle(case).run(save_steps=[5])
     t = FESOMTracerAdvection()
-    t.lib.tracer_set_tracers_per_pass(1)
-    t.lib.tracer_set_launch_config(16, 16, 16, 16)
-    t.lib.tracer_set_precision(mode)
+    t.set_launch_config((16, 16, 16, 16))
+    t.set_precision(mode)
     t.setup_case(case)
     v0 = [t.stats(i + 1)["volume_sum"] for i in range(case.ntracers)]
     t.advect_steps(5, case.dt)
@@ -157,3 +256,16 @@ def test_mixed_precision_close_to_fp64(oracle_mod, mode):
     say(f"  mode {mode}: rel Linf {err:.3e}, volume-sum drift {drift:.3e}")
     assert err <= PREC_BOUNDS[mode]
     assert drift <= {1: 1e-6, 2: 1e-3, 3: 1e-2}[mode]
+
+
+def test_union_of_meshes_matches_oracle(oracle_mod):
+    """Three independent meshes of different size and depth advanced as one disjoint union
+    (the multi-mesh batch), FP64 against the oracle on the same union."""
+    cs = [configs.bathymetry_case(f"m{i}", n, nl, seed=60 + i, ntracers=4, nsteps=3, lev_lo=3,
+                                  lev_hi=nl, umax=u, w_cfl=0.3)
+          for i, (n, nl, u) in enumerate([(9, 11, 0.1), (14, 17, 0.3), (7, 8, 0.05)])]
+    u = configs.union_case(cs)
+    say("case union of 3 meshes, 4 tracers, FP64")
+    ref, _ = oracle_mod.Oracle(u).run(save_steps=[3])
+    got = gpu_run(u, 3, kb=4)
+    assert rel_linf(got, ref[3]) <= RTOL

@@ -125,3 +125,44 @@ def test_streamfunction_flow_is_divergence_free():
     np.add.at(div, m.edges[:, 1] - 1, -v)
     assert np.max(np.abs(div)) <= 1e-12 * np.max(np.abs(v))
     assert np.isclose(np.max(np.hypot(case.uv[:, 0, 0], case.uv[:, 0, 1])), 0.1, rtol=1e-12)
+
+
+def test_union_case_equals_members_in_oracle(oracle_mod):
+    """A step of the disjoint union is a step of every member (same dt), bit for bit."""
+    import dataclasses
+    cs = [configs.bathymetry_case(f"m{i}", n, nl, seed=50 + i, ntracers=2, nsteps=2, lev_lo=3,
+                                  lev_hi=nl, umax=u, w_cfl=0.3)
+          for i, (n, nl, u) in enumerate([(6, 9, 0.1), (8, 13, 0.3), (5, 7, 0.05)])]
+    u = configs.union_case(cs)
+    assert u.mesh.nod2D == sum(c.mesh.nod2D for c in cs) and u.mesh.nl == 13
+    assert u.dt == min(c.dt for c in cs)
+    ref, _ = oracle_mod.Oracle(u).run(save_steps=[2])
+    off = 0
+    for c in cs:
+        r, _ = oracle_mod.Oracle(dataclasses.replace(c, dt=u.dt)).run(save_steps=[2])
+        part = ref[2][:, off:off + c.mesh.nod2D]
+        assert np.array_equal(part[:, :, :c.mesh.nl - 1], r[2])
+        assert not np.any(part[:, :, c.mesh.nl - 1:])
+        off += c.mesh.nod2D
+
+
+def test_bathymetry_velocity_keeps_tracers_bounded(oracle_mod):
+    """The layer-wise tapered streamfunction is divergence free on the cut dual cells: a uniform
+    tracer stays uniform to round-off and T / S stay inside their initial range (FCT with a
+    divergence-free transport obeys a discrete maximum principle)."""
+    case = configs.bathymetry_case("u", 20, 16, seed=3, ntracers=2, nsteps=60, lev_lo=4)
+    wet = configs.wet_mask(case.mesh)
+    lo = [case.tracers[i][wet].min() for i in range(2)]
+    hi = [case.tracers[i][wet].max() for i in range(2)]
+    out, _ = oracle_mod.Oracle(case).run(save_steps=[60])
+    for i in range(2):
+        v = out[60][i][wet]
+        assert v.min() >= lo[i] - 1e-12 * abs(hi[i]) and v.max() <= hi[i] * (1 + 1e-12), i
+    case.tracers[:] = np.where(wet, 35.0, 0.0)[None]
+    out, _ = oracle_mod.Oracle(case).run(save_steps=[60])
+    assert np.max(np.abs(out[60][:, wet] - 35.0)) <= 1e-12 * 35.0
+    # the largest element speed equals umax; layers below an element's bottom carry no velocity
+    sp = np.hypot(case.uv[:, :, 0], case.uv[:, :, 1])
+    assert abs(sp.max() - 0.1) <= 1e-15
+    k = np.arange(case.mesh.nl - 1)[None, :]
+    assert not np.any(sp[k >= (case.mesh.nlevels - 1)[:, None]])
