@@ -32,7 +32,7 @@ import numpy as np
 REPO = Path(__file__).resolve().parent
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 PREC_NAMES = {0: "f64", 1: "f32", 2: "f32/f16-storage", 3: "f32/bf16-storage"}
-PREC_BYTES = {0: (8, 8), 1: (4, 4), 2: (2, 4), 3: (2, 4)}  # (tracer storage, mesh) bytes
+PREC_BYTES = {0: (8, 8), 1: (4, 8), 2: (2, 8), 3: (2, 8)}  # (tracer storage, mesh) bytes; mesh is FP64 always
 
 
 def log(msg: str) -> None:
@@ -94,7 +94,8 @@ def make_case(name: str, nsteps=None):
 
 
 def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, lc=None,
-        stage_steps: int = 10, compare=None, keep: bool = False, stochastic: bool = True):
+        stage_steps: int = 10, compare=None, keep: bool = False, stochastic: bool = True,
+        tune: bool = True):
     """Advance `case` on the GPU; returns throughput, device time, per-stage times, drift.
 
     keep: also return the final values (list per tracer); compare: final values of a reference
@@ -111,6 +112,11 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
         t.set_launch_config(lc)
     t.setup_case(case)
     width = t.block_width()
+    tuned = None
+    if tune and not lc:
+        from fesom2_tracer_dwarf_amd import autotune
+        tuned = autotune.tune(t, case, prec, use_cache=False)
+        log(f"autotuned levels per thread {tuned['launch_config']} (stage ms {[round(x, 3) for x in tuned['stage_ms']]})")
     v0 = [t.stats(i + 1)["volume_sum"] for i in range(case.ntracers)]
     if warmup:
         t.advect_steps(warmup, case.dt)
@@ -152,7 +158,7 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
     updates = case.ntracers * m.n_node_levels * steps
     return {"value": updates / wall, "wall_s": wall, "dev_ms": dev_ms, "stage_ms": stage_ms,
             "passes_timed": passes, "block_width": width, "drift": drift, "launch_config": cfg,
-            "values": kept, "error": err}
+            "values": kept, "error": err, "autotune": tuned}
 
 
 def roofline(case, prec: int, r: dict, name: str):
@@ -230,6 +236,8 @@ def main():
                     help="tracers interleaved per block and advanced per kernel pass (0: all)")
     ap.add_argument("--graph", action="store_true", help="replay the timed steps as one HIP graph")
     ap.add_argument("--lc", default="", help="levels per thread: edge,lo,bounds,final")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch configuration instead of autotuning it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the mixed-precision batched companion run")
@@ -276,7 +284,7 @@ def main():
     m = case.mesh
     log(f"workload {describe(case)}: N={m.nod2D} T={m.elem2D} E={m.edge2D} N_L={m.n_node_levels} "
         f"(generated in {time.perf_counter() - t0:.1f} s)")
-    r = run(case, prec, kb, steps, args.warmup, graph, lc)
+    r = run(case, prec, kb, steps, args.warmup, graph, lc, tune=not args.no_autotune)
     log(f"timed {steps} steps: wall {r['wall_s'] * 1e3:.2f} ms, device {r['dev_ms']:.2f} ms, "
         f"{r['value']:.3e} updates/s, stages {r['stage_ms']}")
     roof = roofline(case, prec, r, args.workload)
@@ -287,10 +295,10 @@ def main():
         log(f"batched companion {describe(bcase)}")
         ref64 = None
         if not args.no_batched_accuracy:
-            r64 = run(bcase, 0, 64, args.batched_steps, 0, keep=True)
+            r64 = run(bcase, 0, 64, args.batched_steps, 0, keep=True, tune=not args.no_autotune)
             ref64 = r64["values"]
             log(f"batched FP64 reference: {r64['value']:.3e} updates/s")
-        rb = run(bcase, 1, 64, args.batched_steps, 0, compare=ref64)
+        rb = run(bcase, 1, 64, args.batched_steps, 0, compare=ref64, tune=not args.no_autotune)
         ref64 = None
         batched = {"workload": describe(bcase), "precision": "FP32 arithmetic and storage, FP64 "
                    "accumulation / update / limiter ratios / sums", "block_width": rb["block_width"],

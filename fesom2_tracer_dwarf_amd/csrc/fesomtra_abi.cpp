@@ -70,6 +70,7 @@ struct State {
     std::vector<void *> allocs;
     std::vector<void *> dT, tabbuf;  // per tracer block, storage type of the precision mode
     std::vector<char> tab_on;        // per block: valuesAB held apart from values
+    std::vector<void *> ckpt;        // per block: device copy kept by tracer_checkpoint
     void *flo = nullptr, *adfh = nullptr, *LO = nullptr, *Rp = nullptr, *Rm = nullptr, *adfv = nullptr;
     double *acc_h = nullptr, *acc_v = nullptr;
     void *dbg_h = nullptr, *dbg_v = nullptr;
@@ -157,6 +158,7 @@ void release_device(State &s) {
     s.dT.clear();
     s.tabbuf.clear();
     s.tab_on.clear();
+    s.ckpt.clear();
     s.flo = s.adfh = s.LO = s.Rp = s.Rm = s.adfv = nullptr;
     s.acc_h = s.acc_v = nullptr;
     s.dbg_h = s.dbg_v = nullptr;
@@ -1252,6 +1254,26 @@ int tracer_set_tracers_per_pass(int kb) {
 int tracer_get_tracers_per_pass(int *kb) {
     const State &s = S();
     *kb = s.on_device ? s.kb : block_width(s);
+    return 0;
+}
+
+int tracer_checkpoint(int restore) {
+    State &s = S();
+    if (ensure_device(s)) return -1;
+    const size_t bytes = (size_t)(s.nl - 1) * s.Np * s.kb * storage_bytes(s.prec);
+    if (!restore) {
+        if (s.ckpt.size() != s.dT.size()) s.ckpt.assign(s.dT.size(), nullptr);
+        for (size_t b = 0; b < s.dT.size(); ++b) {
+            if (!s.ckpt[b] && dalloc_bytes(s, &s.ckpt[b], bytes)) return -1;
+            HIPCHK(hipMemcpyAsync(s.ckpt[b], s.dT[b], bytes, hipMemcpyDeviceToDevice, s.stream));
+        }
+    } else {
+        if (s.ckpt.size() != s.dT.size()) return fail("tracer_checkpoint: no checkpoint of the current storage");
+        for (size_t b = 0; b < s.dT.size(); ++b)
+            HIPCHK(hipMemcpyAsync(s.dT[b], s.ckpt[b], bytes, hipMemcpyDeviceToDevice, s.stream));
+        std::fill(s.tab_on.begin(), s.tab_on.end(), 0);
+    }
+    HIPCHK(hipStreamSynchronize(s.stream));
     return 0;
 }
 

@@ -16,12 +16,19 @@ __device__ __forceinline__ int xcd_block() {
 // Column owned by this thread; consecutive columns of one XCD stay contiguous in the
 // space-filling-curve order (workgroups are dispatched round-robin over the XCDs).
 __device__ __forceinline__ int column_index() { return xcd_block() * blockDim.x + threadIdx.x; }
-// (column, lane) of this thread for a tracer block of width KB (a power of two).
+// (column, lane) of this thread for a tracer block of width KB (a power of two).  With KB = 64 a
+// wavefront is exactly one column: the column index is made wavefront-uniform, so every
+// per-column quantity lives in scalar registers and its loads are scalar loads.
 template <int KB>
 __device__ __forceinline__ void column_lane(int &col, int &lane) {
     const int g = column_index();
-    col = g / KB;
-    lane = g % KB;
+    if constexpr (KB == 64) {
+        col = __builtin_amdgcn_readfirstlane(g / KB);
+        lane = (int)(threadIdx.x & 63u);
+    } else {
+        col = g / KB;
+        lane = g % KB;
+    }
 }
 __device__ __forceinline__ bool lane_on(unsigned long long mask, int lane) { return (mask >> lane) & 1ull; }
 
@@ -59,17 +66,18 @@ __device__ __forceinline__ void node_mean_grad(const MeshCol<M> &m, const LD &t,
                                                C &gy) {
     C tvol = 0, tx = 0, ty = 0;
     const size_t off = (size_t)(nz - 1) * m.Np;
-    const int cnt = m.nie_num[n];
+    const int cnt = cld(m.nie_num, n);
     for (int i = 0; i < cnt; ++i) {
-        const int el = m.nie[(size_t)n * m.max_adj + i];
-        if (m.nlev_e[el] - 1 < nz) continue;
-        const C ar = (C)m.elem_area[el];
-        const int *en = m.enodes + (size_t)el * 3;
+        const int el = cld(m.nie, (size_t)n * m.max_adj + i);
+        if (cld(m.nlev_e, el) - 1 < nz) continue;
+        const C ar = (C)cld(m.elem_area, el);
+        const int en[3] = {cld(m.enodes, (size_t)el * 3), cld(m.enodes, (size_t)el * 3 + 1),
+                           cld(m.enodes, (size_t)el * 3 + 2)};
         C g0[3], g1[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            g0[c] = (C)m.gsca[(size_t)el * 6 + c];
-            g1[c] = (C)m.gsca[(size_t)el * 6 + 3 + c];
+            g0[c] = (C)cld(m.gsca, (size_t)el * 6 + c);
+            g1[c] = (C)cld(m.gsca, (size_t)el * 6 + 3 + c);
         }
         tvol = tvol + ar;
         tx = tx + tr_xy<C>(g0, t, off, en[0], en[1], en[2]) * ar;
@@ -127,10 +135,10 @@ struct Col {
     size_t Np;
     int n, lane, nzmin, nzmax;
     __device__ __forceinline__ C T(int nz) const { return ldc<C>(t, ((size_t)(nz - 1) * Np + n) * KB + lane); }
-    __device__ __forceinline__ C W(int j) const { return (C)w[(size_t)(j - 1) * Np + n]; }
-    __device__ __forceinline__ C A(int j) const { return (C)ar[(size_t)(j - 1) * Np + n]; }
-    __device__ __forceinline__ C HN(int nz) const { return (C)hn[(size_t)(nz - 1) * Np + n]; }
-    __device__ __forceinline__ C HO(int nz) const { return (C)ho[(size_t)(nz - 1) * Np + n]; }
+    __device__ __forceinline__ C W(int j) const { return (C)cld(w, (size_t)(j - 1) * Np + n); }
+    __device__ __forceinline__ C A(int j) const { return (C)cld(ar, (size_t)(j - 1) * Np + n); }
+    __device__ __forceinline__ C HN(int nz) const { return (C)cld(hn, (size_t)(nz - 1) * Np + n); }
+    __device__ __forceinline__ C HO(int nz) const { return (C)cld(ho, (size_t)(nz - 1) * Np + n); }
 };
 
 // adv_tra_ver_cdiff (:687-708) at interface j, minus fin; the bottom interface keeps fin.

@@ -38,10 +38,10 @@ __global__ __launch_bounds__(256) void k_edge(const MeshCol<M> m, const StepPtrs
     int e, lane;
     column_lane<KB>(e, lane);
     if (e >= m.E || !lane_on(p.mask, lane)) return;
-    const int4 q = m.edge_ne[e];
+    const int4 q = cld(m.edge_ne, e);
     const int n1 = q.x, n2 = q.y, el1 = q.z, el2 = q.w;
-    const int nl1 = m.nlev_e[el1] - 1;
-    const int nl2 = el2 >= 0 ? m.nlev_e[el2] - 1 : 0;
+    const int nl1 = cld(m.nlev_e, el1) - 1;
+    const int nl2 = el2 >= 0 ? cld(m.nlev_e, el2) - 1 : 0;
     int nz0, nz1;
     if (!chunk_layers(max(nl1, nl2), lc, nz0, nz1)) return;
     const size_t Np = m.Np, Ep = m.Ep;
@@ -51,32 +51,32 @@ __global__ __launch_bounds__(256) void k_edge(const MeshCol<M> m, const StepPtrs
     int shared_end = 0, u[3] = {0, 0, 0}, d[3] = {0, 0, 0}, nb1 = 0, nb2 = 0;
     C gu[6], gd[6];
     if (HOR != HOR_UPW1) {
-        dx = (C)m.edxdy[2 * (size_t)e];
-        dy = (C)m.edxdy[2 * (size_t)e + 1];
-        a = (C)kREarth * (C)m.elem_cos[el1];
-        if (el2 >= 0) a = C(0.5) * (a + (C)kREarth * (C)m.elem_cos[el2]);
-        const int2 ud = m.updn[e];
+        dx = (C)cld(m.edxdy, 2 * (size_t)e);
+        dy = (C)cld(m.edxdy, 2 * (size_t)e + 1);
+        a = (C)kREarth * (C)cld(m.elem_cos, el1);
+        if (el2 >= 0) a = C(0.5) * (a + (C)kREarth * (C)cld(m.elem_cos, el2));
+        const int2 ud = cld(m.updn, e);
         if (ud.x >= 0 && ud.y >= 0) {
-            shared_end = min(m.nlev_min[n1], m.nlev_min[n2]) - 1;
+            shared_end = min(cld(m.nlev_min, n1), cld(m.nlev_min, n2)) - 1;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                u[c] = m.enodes[(size_t)ud.x * 3 + c];
-                d[c] = m.enodes[(size_t)ud.y * 3 + c];
+                u[c] = cld(m.enodes, (size_t)ud.x * 3 + c);
+                d[c] = cld(m.enodes, (size_t)ud.y * 3 + c);
             }
 #pragma unroll
             for (int c = 0; c < 6; ++c) {
-                gu[c] = (C)m.gsca[(size_t)ud.x * 6 + c];
-                gd[c] = (C)m.gsca[(size_t)ud.y * 6 + c];
+                gu[c] = (C)cld(m.gsca, (size_t)ud.x * 6 + c);
+                gd[c] = (C)cld(m.gsca, (size_t)ud.y * 6 + c);
             }
         }
         if (HOR == HOR_MUSCL) {
-            nb1 = m.nbl[n1];
-            nb2 = m.nbl[n2];
+            nb1 = cld(m.nbl, n1);
+            nb2 = cld(m.nbl, n2);
         }
     }
     for (int nz = nz0; nz <= nz1; ++nz) {
         const int k = nz - 1;
-        const C v = (C)m.q[(size_t)k * Ep + e];
+        const C v = (C)cld(m.q, (size_t)k * Ep + e);
         const size_t o = (size_t)k * Np;
         const size_t ie = ((size_t)k * Ep + e) * KB + lane;
         C t1 = 0, t2 = 0;
@@ -124,31 +124,34 @@ __global__ __launch_bounds__(256) void k_edge(const MeshCol<M> m, const StepPtrs
 
 // Edge transport of the current velocity, q(k, e) = sum over the edge's elements wet at level k
 // of (+-) (u dy - v dx) helem (lib/oce_adv_tra_hor.F90:166-253, segments C, D and E), in double,
-// once per step for all tracers.  Levels below both elements are not written (never read).
+// once per step for all tracers.  One thread per edge walks its levels; levels below both
+// elements are not written (never read).
 template <typename M>
 __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = column_index();
     if (e >= m.E) return;
-    const int nz = (int)blockIdx.y + 1, k = nz - 1;
-    const int4 q = m.edge_ne[e];
+    const int4 q = cld(m.edge_ne, e);
     const int el1 = q.z, el2 = q.w;
-    const int nl1 = m.nlev_e[el1] - 1;
-    const int nl2 = el2 >= 0 ? m.nlev_e[el2] - 1 : 0;
-    const bool use1 = nz <= nl1, use2 = nz <= nl2;
-    if (!use1 && !use2) return;
-    const size_t Tp = m.Tp;
-    double v1 = 0, v2 = 0;
-    if (use1) {
-        const size_t i = (size_t)k * Tp + el1;
-        v1 = (-(double)m.uv[2 * i + 1] * (double)m.ecd[4 * (size_t)e] +
-              (double)m.uv[2 * i] * (double)m.ecd[4 * (size_t)e + 1]) * (double)m.helem[i];
+    const int nl1 = cld(m.nlev_e, el1) - 1;
+    const int nl2 = el2 >= 0 ? cld(m.nlev_e, el2) - 1 : 0;
+    const double cd0 = cld(m.ecd, 4 * (size_t)e), cd1 = cld(m.ecd, 4 * (size_t)e + 1);
+    const double cd2 = cld(m.ecd, 4 * (size_t)e + 2), cd3 = cld(m.ecd, 4 * (size_t)e + 3);
+    const size_t Tp = m.Tp, Ep = m.Ep;
+    const int top = max(nl1, nl2);
+    for (int nz = 1; nz <= top; ++nz) {
+        const int k = nz - 1;
+        const bool use1 = nz <= nl1, use2 = nz <= nl2;
+        double v1 = 0, v2 = 0;
+        if (use1) {
+            const size_t i = (size_t)k * Tp + el1;
+            v1 = (-(double)cld(m.uv, 2 * i + 1) * cd0 + (double)cld(m.uv, 2 * i) * cd1) * (double)cld(m.helem, i);
+        }
+        if (use2) {
+            const size_t i = (size_t)k * Tp + el2;
+            v2 = ((double)cld(m.uv, 2 * i + 1) * cd2 - (double)cld(m.uv, 2 * i) * cd3) * (double)cld(m.helem, i);
+        }
+        m.q[(size_t)k * Ep + e] = (use1 && use2) ? v1 + v2 : (use1 ? v1 : v2);
     }
-    if (use2) {
-        const size_t i = (size_t)k * Tp + el2;
-        v2 = ((double)m.uv[2 * i + 1] * (double)m.ecd[4 * (size_t)e + 2] -
-              (double)m.uv[2 * i] * (double)m.ecd[4 * (size_t)e + 3]) * (double)m.helem[i];
-    }
-    m.q[(size_t)k * m.Ep + e] = (use1 && use2) ? v1 + v2 : (use1 ? v1 : v2);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -159,14 +162,14 @@ struct NodeEdges {
 };
 template <typename M>
 __device__ __forceinline__ void load_edges(const MeshCol<M> &m, int n, NodeEdges &ne) {
-    ne.b = m.ne_ptr[n];
-    ne.deg = m.ne_ptr[n + 1] - ne.b;
+    ne.b = cld(m.ne_ptr, n);
+    ne.deg = cld(m.ne_ptr, n + 1) - ne.b;
 #pragma unroll
     for (int i = 0; i < kDegCache; ++i) {
         if (i < ne.deg) {
-            ne.e2[i] = m.ne_e2[ne.b + i];
-            ne.top[i] = m.ne_top[ne.b + i];
-            ne.oth[i] = m.ne_other[ne.b + i];
+            ne.e2[i] = cld(m.ne_e2, ne.b + i);
+            ne.top[i] = cld(m.ne_top, ne.b + i);
+            ne.oth[i] = cld(m.ne_other, ne.b + i);
         } else {
             ne.e2[i] = 0;
             ne.top[i] = 0;
@@ -181,8 +184,8 @@ __device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges &
     for (int i = 0; i < kDegCache; ++i)
         if (i < ne.deg && nz <= ne.top[i]) f(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.oth[i]);
     for (int i = kDegCache; i < ne.deg; ++i) {
-        const int e2 = m.ne_e2[ne.b + i];
-        if (nz <= m.ne_top[ne.b + i]) f(e2 >> 1, (e2 & 1) != 0, m.ne_other[ne.b + i]);
+        const int e2 = cld(m.ne_e2, ne.b + i);
+        if (nz <= cld(m.ne_top, ne.b + i)) f(e2 >> 1, (e2 & 1) != 0, cld(m.ne_other, ne.b + i));
     }
 }
 
@@ -195,9 +198,9 @@ struct NodeCol {
         return (nz >= 1 && nz <= L) ? ldc<C>(a, ((size_t)(nz - 1) * Np + n) * KB + lane) : C(0);
     }
     __device__ __forceinline__ C mlay(const M *a, int nz) const {
-        return (nz >= 1 && nz <= L) ? (C)a[(size_t)(nz - 1) * Np + n] : C(0);
+        return (nz >= 1 && nz <= L) ? (C)cld(a, (size_t)(nz - 1) * Np + n) : C(0);
     }
-    __device__ __forceinline__ C itf(const M *a, int j) const { return (C)a[(size_t)(j - 1) * Np + n]; }
+    __device__ __forceinline__ C itf(const M *a, int j) const { return (C)cld(a, (size_t)(j - 1) * Np + n); }
 };
 
 // High-order vertical flux at interface j minus fin for the window tw = TAB(j-2..j+1),
@@ -239,7 +242,7 @@ __global__ __launch_bounds__(256) void k_lo(const MeshCol<M> m, const StepPtrs<S
     int n, lane;
     column_lane<KB>(n, lane);
     if (n >= m.N || !lane_on(p.mask, lane)) return;
-    const int nzmax = m.nlev_n[n], L = nzmax - 1;
+    const int nzmax = cld(m.nlev_n, n), L = nzmax - 1;
     int nz0, nz1;
     if (!chunk_layers(L, lc, nz0, nz1)) return;
     const size_t Np = m.Np, Ep = m.Ep;
@@ -273,8 +276,8 @@ __global__ __launch_bounds__(256) void k_lo(const MeshCol<M> m, const StepPtrs<S
         const D wej = nc.itf(m.we, j), wj = nc.itf(m.w, j), aj = nc.itf(m.area, j);
         const D zbj = (VER == VER_QR4C && !bottom) ? nc.itf(m.zbar, j) : D(0);
         const size_t cell = (size_t)(nz - 1) * Np + n;
-        const double h = (double)m.hnode[cell], hn = (double)m.hnode_new[cell];
-        const double V = (double)m.areasvol[cell];
+        const double h = (double)cld(m.hnode, cell), hn = (double)cld(m.hnode_new, cell);
+        const double V = (double)cld(m.areasvol, cell);
         tw[0] = tw[1];
         tw[1] = tw[2];
         tw[2] = tw[3];
@@ -292,7 +295,7 @@ __global__ __launch_bounds__(256) void k_lo(const MeshCol<M> m, const StepPtrs<S
         } else {
             for_edges(m, ne, nz, [&](int e, bool second, int oth) {
                 const double to = ldc<double>(p.T, (no + oth) * KB + lane);
-                const double f = upw1_edge<double>(second ? to : tcur, second ? tcur : to, m.q[eo + e]);
+                const double f = upw1_edge<double>(second ? to : tcur, second ? tcur : to, cld(m.q, eo + e));
                 s = second ? s - f : s + f;
             });
         }
@@ -314,7 +317,7 @@ __global__ __launch_bounds__(256) void k_vert(const MeshCol<M> m, const StepPtrs
     int n, lane;
     column_lane<KB>(n, lane);
     if (n >= m.N || !lane_on(p.mask, lane)) return;
-    const int nzmax = m.nlev_n[n], L = nzmax - 1;
+    const int nzmax = cld(m.nlev_n, n), L = nzmax - 1;
     int nz0, nz1;
     if (!chunk_layers(L, lc, nz0, nz1)) return;
     const size_t Np = m.Np;
@@ -356,11 +359,11 @@ __global__ __launch_bounds__(256) void k_bounds(const MeshCol<M> m, const StepPt
     int n, lane;
     column_lane<KB>(n, lane);
     if (n >= m.N || !lane_on(p.mask, lane)) return;
-    const int nzmax = m.nlev_n[n], L = nzmax - 1;
+    const int nzmax = cld(m.nlev_n, n), L = nzmax - 1;
     int nz0, nz1;
     if (!chunk_layers(L, lc, nz0, nz1)) return;
     const size_t Np = m.Np, Ep = m.Ep;
-    const int dry_from = m.nlev_min[n];  // levels >= this have a dry element around n
+    const int dry_from = cld(m.nlev_min, n);  // levels >= this have a dry element around n
     NodeEdges ne;
     load_edges(m, n, ne);
     const LaneLd<KB, C, S> ldT{p.T, lane};
@@ -388,7 +391,7 @@ __global__ __launch_bounds__(256) void k_bounds(const MeshCol<M> m, const StepPt
     av0 = p.adfv[((size_t)(nz0 - 1) * Np + n) * KB + lane];
     for (int nz = nz0; nz <= nz1; ++nz) {
         const size_t cell = (size_t)(nz - 1) * Np + n;
-        const double V = (double)m.areasvol[cell];
+        const double V = (double)cld(m.areasvol, cell);
         const C lo = p.LO[cell * KB + lane];
         C qmax, qmin;
         if (nz == L || nz == 1) {
@@ -441,7 +444,7 @@ __global__ __launch_bounds__(256) void k_final(const MeshCol<M> m, const StepPtr
     int n, lane;
     column_lane<KB>(n, lane);
     if (n >= m.N || !lane_on(p.mask, lane)) return;
-    const int nzmax = m.nlev_n[n], L = nzmax - 1;
+    const int nzmax = cld(m.nlev_n, n), L = nzmax - 1;
     int nz0, nz1;
     if (!chunk_layers(L, lc, nz0, nz1)) return;
     const size_t Np = m.Np, Ep = m.Ep;
@@ -478,9 +481,9 @@ __global__ __launch_bounds__(256) void k_final(const MeshCol<M> m, const StepPtr
     }
     for (int nz = nz0; nz <= nz1; ++nz) {
         const size_t cell = (size_t)(nz - 1) * Np + n;
-        const double V = (double)m.areasvol[cell];
-        const double hn = (double)m.hnode_new[cell];
-        const double h = FCT ? (double)m.hnode[cell] : 0.0;
+        const double V = (double)cld(m.areasvol, cell);
+        const double hn = (double)cld(m.hnode_new, cell);
+        const double h = FCT ? (double)cld(m.hnode, cell) : 0.0;
         const double tval = (double)ldc<C>(p.T, X(cell));
         const C ab = p.adfv[X(cell + Np)];
         C bot;
@@ -528,7 +531,7 @@ __global__ __launch_bounds__(256) void k_final(const MeshCol<M> m, const StepPtr
                 // single precision without limiter: the stored flux is high-order minus upwind;
                 // the upwind part of valuesAB is added here in double
                 const double an = ldc<double>(p.TAB, X(cell)), ao = ldc<double>(p.TAB, X(no + oth));
-                fd = fd + upw1_edge<double>(second ? ao : an, second ? an : ao, m.q[eo + e]);
+                fd = fd + upw1_edge<double>(second ? ao : an, second ? an : ao, cld(m.q, eo + e));
             }
             dh = second ? dh - fd * dt / V : dh + fd * dt / V;
         });
@@ -573,14 +576,14 @@ __global__ __launch_bounds__(256) void k_stats_partial(const MeshCol<M> m, const
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
         const int k = (int)(i / m.N), n = (int)(i - (size_t)k * m.N);
-        const bool wet = k + 1 <= m.nlev_n[n] - 1;
+        const bool wet = k + 1 <= cld(m.nlev_n, n) - 1;
         if (!all_levels && !wet) continue;
         const size_t ix = (size_t)k * m.Np + n;
         const double v = ldc<double>(T, ix * kb + lane);
         mn = v < mn ? v : mn;
         mx = v > mx ? v : mx;
         sm += v;
-        if (wet) sv += v * (double)m.hnode[ix] * (double)m.areasvol[ix];
+        if (wet) sv += v * (double)cld(m.hnode, ix) * (double)cld(m.areasvol, ix);
     }
     smin[threadIdx.x] = mn;
     smax[threadIdx.x] = mx;
@@ -681,7 +684,7 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
 template <typename M>
 hipError_t launch_qflux(const MeshCol<M> &m, hipStream_t s) {
     if (m.L < 1 || m.E < 1) return hipSuccess;
-    k_qflux<M><<<dim3((unsigned)((m.E + 255) / 256), (unsigned)m.L), 256, 0, s>>>(m);
+    k_qflux<M><<<dim3(grid_for((size_t)m.E, 256)), 256, 0, s>>>(m);
     return hipGetLastError();
 }
 

@@ -43,6 +43,22 @@ enum Prec { PREC_F64 = 0, PREC_F32 = 1, PREC_F32_F16 = 2, PREC_F32_BF16 = 3 };
 typedef __half f16_t;
 typedef __hip_bfloat16 bf16_t;
 
+// Read-only mesh data: loads through the constant address space, so that loads at a
+// wavefront-uniform index become scalar loads (one per wavefront into scalar registers); the
+// arrays are never written while a kernel that reads them runs.
+template <class T>
+__device__ __forceinline__ T cld(const T *p, size_t i) {
+    return ((const __attribute__((address_space(4))) T *)p)[i];
+}
+__device__ __forceinline__ int4 cld(const int4 *p, size_t i) {
+    const int *q = reinterpret_cast<const int *>(p) + 4 * i;
+    return make_int4(cld(q, 0), cld(q, 1), cld(q, 2), cld(q, 3));
+}
+__device__ __forceinline__ int2 cld(const int2 *p, size_t i) {
+    const int *q = reinterpret_cast<const int *>(p) + 2 * i;
+    return make_int2(cld(q, 0), cld(q, 1));
+}
+
 // Loads converting to the compute type C, stores rounding from double.
 template <typename C, typename T>
 __device__ __forceinline__ C ldc(const T *__restrict__ p, size_t i) {

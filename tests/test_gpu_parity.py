@@ -269,3 +269,28 @@ def test_union_of_meshes_matches_oracle(oracle_mod):
     ref, _ = oracle_mod.Oracle(u).run(save_steps=[3])
     got = gpu_run(u, 3, kb=4)
     assert rel_linf(got, ref[3]) <= RTOL
+
+
+def test_autotune_and_checkpoint_keep_state(oracle_mod, tmp_path, monkeypatch):
+    """The autotuner runs trial steps between a device checkpoint and its restore: afterwards the
+    state is untouched and the tuned configuration gives the reference result."""
+    from fesom2_tracer_dwarf_amd import autotune
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    monkeypatch.setenv("FESOMTRA_TUNE_CACHE", str(tmp_path / "tune.json"))
+    case = configs.bathymetry_case("bathy13t", 13, 14, seed=43, ntracers=3, nsteps=3, lev_lo=4,
+                                   w_cfl=0.3)
+    say("case bathy13t autotune + checkpoint")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[3])
+    t = FESOMTracerAdvection()
+    t.setup_case(case)
+    res = autotune.tune(t, case, 0, candidates=(2, 5, 13), steps=2)
+    assert all(c in (2, 5, 13) for c in res["launch_config"]) and not res["cached"]
+    assert tuple(t.launch_config()) == tuple(res["launch_config"])
+    assert np.array_equal(t.get_all_values(), case.tracers)
+    t.advect_steps(3, case.dt)
+    got = t.get_all_values()
+    again = autotune.tune(t, case, 0, candidates=(2, 5, 13), steps=2)
+    assert again["cached"] and list(again["launch_config"]) == list(res["launch_config"])
+    t.set_launch_config((16, 16, 16, 16))
+    t.cleanup()
+    assert rel_linf(got, ref[3]) <= RTOL
