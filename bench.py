@@ -95,7 +95,7 @@ def make_case(name: str, nsteps=None):
 
 def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, lc=None,
         stage_steps: int = 10, compare=None, keep: bool = False, stochastic: bool = True,
-        tune: bool = True):
+        tune: bool = True, fused: bool = False):
     """Advance `case` on the GPU; returns throughput, device time, per-stage times, drift.
 
     keep: also return the final values (list per tracer); compare: final values of a reference
@@ -112,8 +112,10 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
         t.set_launch_config(lc)
     t.setup_case(case)
     width = t.block_width()
+    if fused:
+        t.set_fused(True)
     tuned = None
-    if tune and not lc:
+    if tune and not lc and not fused:
         from fesom2_tracer_dwarf_amd import autotune
         tuned = autotune.tune(t, case, prec, use_cache=False)
         log(f"autotuned levels per thread {tuned['launch_config']} (stage ms {[round(x, 3) for x in tuned['stage_ms']]})")
@@ -152,6 +154,7 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
     t.lib.tracer_set_kernel_timing(0)
     cfg = t.launch_config()
     t.set_graph(False)
+    t.set_fused(False)
     t.set_precision(0)
     t.set_block_width(0)
     t.cleanup()
@@ -170,11 +173,13 @@ def roofline(case, prec: int, r: dict, name: str):
     st = r["stage_ms"]
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(name),
-            "kernel": f"one pass (k_edge + k_lo + k_bounds + k_final) over a block of {k} "
-                      f"tracer(s), {PREC_NAMES[prec]}",
+            "kernel": (f"one pass (k_edge + k_lo + k_bounds + k_final) over a block of {k} "
+                       f"tracer(s), {PREC_NAMES[prec]}") if st[0] or st[1] or st[2] else
+                      f"one fused pass (k_tile) over {k} tracer, {PREC_NAMES[prec]}",
             "algorithmic_bytes_per_launch": B, "tracers_per_launch": k,
-            "stage_ms_per_launch": {"k_edge": round(st[0], 4), "k_lo": round(st[1], 4),
-                                    "k_bounds": round(st[2], 4), "k_final": round(st[3], 4)}}
+            "stage_ms_per_launch": ({"k_edge": round(st[0], 4), "k_lo": round(st[1], 4),
+                                     "k_bounds": round(st[2], 4), "k_final": round(st[3], 4)}
+                                    if st[0] or st[1] or st[2] else {"k_tile": round(st[3], 4)})}
 
 
 def describe(case):
@@ -235,6 +240,8 @@ def main():
     ap.add_argument("--block-width", type=int, default=0,
                     help="tracers interleaved per block and advanced per kernel pass (0: all)")
     ap.add_argument("--graph", action="store_true", help="replay the timed steps as one HIP graph")
+    ap.add_argument("--fused", type=int, default=0, choices=[0, 1],
+                    help="fused single-pass kernel for FP64 FCT tracers, one tracer per block")
     ap.add_argument("--lc", default="", help="levels per thread: edge,lo,bounds,final")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch configuration instead of autotuning it")
@@ -275,7 +282,8 @@ def main():
     d_steps, d_prec, d_kb, d_graph = defaults[args.workload]
     steps = args.steps or d_steps
     prec = d_prec if args.precision is None else args.precision
-    kb = args.block_width or d_kb
+    fused = bool(args.fused) and prec == 0
+    kb = 1 if fused else (args.block_width or d_kb)
     graph = args.graph or d_graph
     lc = [int(x) for x in args.lc.split(",")] if args.lc else None
 
@@ -284,7 +292,7 @@ def main():
     m = case.mesh
     log(f"workload {describe(case)}: N={m.nod2D} T={m.elem2D} E={m.edge2D} N_L={m.n_node_levels} "
         f"(generated in {time.perf_counter() - t0:.1f} s)")
-    r = run(case, prec, kb, steps, args.warmup, graph, lc, tune=not args.no_autotune)
+    r = run(case, prec, kb, steps, args.warmup, graph, lc, tune=not args.no_autotune, fused=fused)
     log(f"timed {steps} steps: wall {r['wall_s'] * 1e3:.2f} ms, device {r['dev_ms']:.2f} ms, "
         f"{r['value']:.3e} updates/s, stages {r['stage_ms']}")
     roof = roofline(case, prec, r, args.workload)
@@ -331,7 +339,7 @@ def main():
         "config": {"workload": describe(case), "nodes": m.nod2D, "elements": m.elem2D,
                    "edges": m.edge2D, "node_levels_per_tracer": m.n_node_levels,
                    "tracers": case.ntracers, "block_width": r["block_width"],
-                   "launch_config": list(r["launch_config"]), "hip_graph": graph,
+                   "launch_config": list(r["launch_config"]), "hip_graph": graph, "fused": fused,
                    "dt_s": case.dt, "parallelism": "1 GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -18,8 +18,9 @@ CSRC = PKG / "csrc"
 REPO = PKG.parent
 LIB = PKG / "libfesomtra.so"
 KERNELS = "fesomtra_kernels.hip"
+TILE = "fesomtra_tile.hip"
 ABI = "fesomtra_abi.cpp"
-HEADERS = ["fesomtra_types.hpp", "fesomtra_column.hpp", "fesomtra_kernels.hpp"]
+HEADERS = ["fesomtra_types.hpp", "fesomtra_column.hpp", "fesomtra_kernels.hpp", "fesomtra_tile.hpp"]
 # The kernel source is compiled once per precision family (FESOMTRA_INST bit), in parallel.
 FAMILIES = {"f64": 1, "f32": 2, "f16": 4, "bf16": 8}
 
@@ -66,6 +67,9 @@ def build(verbose: bool = False) -> Path:
             cmds.append([hipcc(), *HIPFLAGS, *inc, f"-DFESOMTRA_INST={bit}", "-c",
                          str(CSRC / KERNELS), "-o", obj])
             objs.append(obj)
+        obj = str(Path(td) / "tile.o")
+        cmds.append([hipcc(), *HIPFLAGS, *inc, "-c", str(CSRC / TILE), "-o", obj])
+        objs.append(obj)
         obj = str(Path(td) / "abi.o")
         cmds.append([hipcc(), *HIPFLAGS, *inc, "-c", str(CSRC / ABI), "-o", obj])
         objs.append(obj)

@@ -21,6 +21,7 @@
 
 #include "../../include/fesomtra.h"
 #include "fesomtra_kernels.hpp"
+#include "fesomtra_tile.hpp"
 
 using namespace fesomtra;
 
@@ -67,6 +68,15 @@ struct State {
     std::vector<int> perm_n, inv_n, perm_t, inv_t, perm_e, inv_e;
     size_t Np = 0, Tp = 0, Ep = 0;
     MeshCol<double> m64{};  // mesh and velocities stay FP64 in every precision mode
+    // device-order host tables kept for the tile builder
+    std::vector<int> h_ptr, h_e2, h_top, h_oth, h_nie, h_nie_cnt, h_en, h_nlev;
+    std::vector<int2> h_ud;
+    // fused single-pass path (FP64, FCT, one tracer per block)
+    bool fused = false;
+    bool tiles_built = false;
+    TileMesh tm{};
+    std::vector<void *> dT2;    // second tracer buffer per block (output of a fused step)
+    std::vector<char> t2_sync;  // dT2 holds the same below-bottom values as dT
     std::vector<void *> allocs;
     std::vector<void *> dT, tabbuf;  // per tracer block, storage type of the precision mode
     std::vector<char> tab_on;        // per block: valuesAB held apart from values
@@ -159,6 +169,10 @@ void release_device(State &s) {
     s.tabbuf.clear();
     s.tab_on.clear();
     s.ckpt.clear();
+    s.dT2.clear();
+    s.t2_sync.clear();
+    s.tiles_built = false;
+    s.tm = TileMesh{};
     s.flo = s.adfh = s.LO = s.Rp = s.Rm = s.adfv = nullptr;
     s.acc_h = s.acc_v = nullptr;
     s.dbg_h = s.dbg_v = nullptr;
@@ -448,6 +462,8 @@ int build_device(State &s) {
             for (int k = 0; k < cnt[n]; ++k) h[(size_t)n * A + k] = s.inv_t[s.nie[(size_t)o * A + k]];
         }
         if (upload(s, &m.nie, h) || upload(s, &m.nie_num, cnt)) return -1;
+        s.h_nie = h;
+        s.h_nie_cnt = cnt;
     }
     {
         std::vector<int> en((size_t)3 * T), nle(T);
@@ -457,6 +473,7 @@ int build_device(State &s) {
             nle[t] = s.nlev_e[o];
         }
         if (upload(s, &m.enodes, en) || upload(s, &m.nlev_e, nle)) return -1;
+        s.h_en = en;
     }
     std::vector<int> top_of(E);
     {
@@ -476,6 +493,7 @@ int build_device(State &s) {
             top_of[o] = std::max(s.nlev_e[t1] - 1, t2 >= 0 ? s.nlev_e[t2] - 1 : 0);
         }
         if (upload(s, &m.edge_ne, ne) || upload(s, &m.updn, ud)) return -1;
+        s.h_ud = ud;
     }
     // CSR node -> incident edges in ascending reference edge number (the serial scatter order)
     {
@@ -501,7 +519,12 @@ int build_device(State &s) {
         if (upload(s, &m.ne_ptr, ptr) || upload(s, &m.ne_e2, e2) || upload(s, &m.ne_top, top) ||
             upload(s, &m.ne_other, oth))
             return -1;
+        s.h_ptr = ptr;
+        s.h_e2 = e2;
+        s.h_top = top;
+        s.h_oth = oth;
     }
+    s.h_nlev = i_nlev;
     if (upload_mesh<double>(s, s.m64, i_nlev, i_nmin, i_nbl)) return -1;
     if (dalloc(s, &s.m64.q, (size_t)L * s.Ep)) return -1;
     {
@@ -636,6 +659,176 @@ hipError_t launch_block(State &s, int blk, unsigned long long mask, const Scheme
     }
 }
 
+// Tiles of the fused path: consecutive device nodes (a stretch of the Hilbert curve) as owned
+// nodes, plus three rings of neighbours; shrunk until the region, its edge entries and its
+// elements fit the kernel's thread and LDS budget.  Tables are packed as the kernel reads them
+// (fesomtra_tile.hpp).  Returns -1 (with a message) if the mesh exceeds the packing limits.
+int build_tiles(State &s) {
+    const int N = s.N, A = s.max_adj;
+    if (s.nl > 128) return fail("fused step: nl=%d above 128", s.nl);
+    if ((int64_t)s.E * 2 >= (1 << 24)) return fail("fused step: %d edges above the packing limit", s.E);
+    std::vector<int> stamp(N, -1), loc(N, 0), estamp(s.T, -1), eloc(s.T, 0);
+    std::vector<int> t_slot0, t_nslot, t_sp0, t_ent0, t_nent, t_sel0, t_nsel, t_el0, t_nel, t_lmax;
+    std::vector<int> s_node, s_ring, s_eptr, s_elptr, sel, el_id, el_loc;
+    std::vector<int2> ent;
+    int start = 0, stamp_id = 0;
+    std::vector<int> region, ring_of, tel;
+    while (start < N) {
+        int P = std::min(kTileOwned, N - start);
+        for (;;) {
+            ++stamp_id;
+            region.clear();
+            ring_of.clear();
+            for (int n = start; n < start + P; ++n) {
+                stamp[n] = stamp_id;
+                loc[n] = (int)region.size();
+                region.push_back(n);
+                ring_of.push_back(0);
+            }
+            size_t f0 = 0;
+            for (int r = 1; r <= 3; ++r) {
+                const size_t f1 = region.size();
+                for (size_t i = f0; i < f1; ++i) {
+                    const int u = region[i];
+                    for (int k = s.h_ptr[u]; k < s.h_ptr[u + 1]; ++k) {
+                        const int v = s.h_oth[k];
+                        if (stamp[v] != stamp_id) {
+                            stamp[v] = stamp_id;
+                            loc[v] = (int)region.size();
+                            region.push_back(v);
+                            ring_of.push_back(r);
+                        }
+                    }
+                }
+                f0 = f1;
+            }
+            bool ok = (int)region.size() <= kTileThreads;
+            tel.clear();
+            int nent = 0, nsel = 0;
+            if (ok) {
+                auto add_el = [&](int el) {
+                    if (estamp[el] != stamp_id) {
+                        estamp[el] = stamp_id;
+                        eloc[el] = (int)tel.size();
+                        tel.push_back(el);
+                    }
+                };
+                for (size_t i = 0; i < region.size(); ++i) {
+                    if (ring_of[i] > 2) continue;
+                    const int u = region[i];
+                    nent += s.h_ptr[u + 1] - s.h_ptr[u];
+                    nsel += s.h_nie_cnt[u];
+                    for (int k = 0; k < s.h_nie_cnt[u]; ++k) add_el(s.h_nie[(size_t)u * A + k]);
+                    if (ring_of[i] <= 1) {
+                        if (s.h_ptr[u + 1] - s.h_ptr[u] > kTileDeg)
+                            return fail("fused step: node with %d edges (more than %d)", s.h_ptr[u + 1] - s.h_ptr[u], kTileDeg);
+                        for (int k = s.h_ptr[u]; k < s.h_ptr[u + 1]; ++k) {
+                            const int2 ud = s.h_ud[s.h_e2[k] >> 1];
+                            if (ud.x >= 0 && ud.y >= 0) {
+                                add_el(ud.x);
+                                add_el(ud.y);
+                            }
+                        }
+                    }
+                }
+                ok = (int)tel.size() <= kTileElems && nent <= kTileEntries && nsel <= kTileSel;
+                for (size_t i = 0; i < tel.size() && ok; ++i)
+                    for (int c = 0; c < 3; ++c)
+                        if (stamp[s.h_en[3 * (size_t)tel[i] + c]] != stamp_id) ok = false;  // node outside the region
+            }
+            if (ok) break;
+            if (P == 1) return fail("fused step: a one-node tile does not fit the region limits");
+            P = std::max(1, P * 3 / 4);
+        }
+        t_slot0.push_back((int)s_node.size());
+        t_nslot.push_back((int)region.size());
+        t_sp0.push_back((int)s_eptr.size());
+        t_ent0.push_back((int)ent.size());
+        t_sel0.push_back((int)sel.size());
+        t_el0.push_back((int)el_id.size());
+        t_nel.push_back((int)tel.size());
+        const int ent_base = (int)ent.size(), sel_base = (int)sel.size();
+        int lmax = 0;
+        for (size_t i = 0; i < region.size(); ++i) {
+            const int u = region[i];
+            s_node.push_back(u);
+            s_ring.push_back(ring_of[i]);
+            s_eptr.push_back((int)ent.size() - ent_base);
+            s_elptr.push_back((int)sel.size() - sel_base);
+            lmax = std::max(lmax, s.h_nlev[u] - 1);
+            if (ring_of[i] > 2) continue;
+            for (int k = s.h_ptr[u]; k < s.h_ptr[u + 1]; ++k) {
+                const int2 ud = s.h_ud[s.h_e2[k] >> 1];
+                const bool has_ud = ring_of[i] <= 1 && ud.x >= 0 && ud.y >= 0;
+                const int up = has_ud ? eloc[ud.x] : 2047, dn = has_ud ? eloc[ud.y] : 2047;
+                ent.push_back(make_int2((s.h_e2[k] << 7) | s.h_top[k], loc[s.h_oth[k]] | (up << 10) | (dn << 21)));
+            }
+            for (int k = 0; k < s.h_nie_cnt[u]; ++k) sel.push_back(eloc[s.h_nie[(size_t)u * A + k]]);
+        }
+        s_eptr.push_back((int)ent.size() - ent_base);
+        s_elptr.push_back((int)sel.size() - sel_base);
+        t_nent.push_back((int)ent.size() - ent_base);
+        t_nsel.push_back((int)sel.size() - sel_base);
+        for (const int el : tel) {
+            el_id.push_back(el);
+            el_loc.push_back(loc[s.h_en[3 * (size_t)el]] | (loc[s.h_en[3 * (size_t)el + 1]] << 10) |
+                             (loc[s.h_en[3 * (size_t)el + 2]] << 20));
+        }
+        t_lmax.push_back(lmax);
+        start += P;
+    }
+    while (t_slot0.size() % kXcds) {  // empty tiles pad the grid to the XCD count
+        t_slot0.push_back((int)s_node.size());
+        t_nslot.push_back(0);
+        t_sp0.push_back((int)s_eptr.size());
+        s_eptr.push_back(0);
+        s_elptr.push_back(0);
+        t_ent0.push_back((int)ent.size());
+        t_nent.push_back(0);
+        t_sel0.push_back((int)sel.size());
+        t_nsel.push_back(0);
+        t_el0.push_back((int)el_id.size());
+        t_nel.push_back(0);
+        t_lmax.push_back(0);
+    }
+    TileMesh &t = s.tm;
+    t.ntiles = (int)t_slot0.size();
+    if (upload(s, &t.t_slot0, t_slot0) || upload(s, &t.t_nslot, t_nslot) || upload(s, &t.t_sp0, t_sp0) ||
+        upload(s, &t.t_ent0, t_ent0) || upload(s, &t.t_nent, t_nent) || upload(s, &t.t_sel0, t_sel0) ||
+        upload(s, &t.t_nsel, t_nsel) || upload(s, &t.t_el0, t_el0) || upload(s, &t.t_nel, t_nel) ||
+        upload(s, &t.t_lmax, t_lmax) || upload(s, &t.s_node, s_node) || upload(s, &t.s_ring, s_ring) ||
+        upload(s, &t.s_eptr, s_eptr) || upload(s, &t.s_elptr, s_elptr) || upload(s, &t.ent, ent) ||
+        upload(s, &t.sel, sel) || upload(s, &t.el_id, el_id) || upload(s, &t.el_loc, el_loc))
+        return -1;
+    s.tiles_built = true;
+    return 0;
+}
+
+// The fused path applies to FP64 blocks of one tracer with the FCT limiter.
+bool fused_applies(const State &s, const SchemeSel &sc) {
+    return s.fused && s.prec == PREC_F64 && s.kb == 1 && sc.fct;
+}
+
+int ensure_fused(State &s) {
+    if (!s.tiles_built && build_tiles(s)) return -1;
+    const size_t bytes = (size_t)(s.nl - 1) * s.Np * s.kb * storage_bytes(s.prec);
+    if (s.dT2.size() != s.dT.size()) {
+        s.dT2.assign(s.dT.size(), nullptr);
+        s.t2_sync.assign(s.dT.size(), 0);
+    }
+    for (size_t b = 0; b < s.dT.size(); ++b) {
+        if (!s.dT2[b] && dalloc_bytes(s, &s.dT2[b], bytes)) return -1;
+        if (!s.t2_sync[b]) {  // values below the bottom are never written by a step: copy them once
+            HIPCHK(hipMemcpyAsync(s.dT2[b], s.dT[b], bytes, hipMemcpyDeviceToDevice, s.stream));
+            s.t2_sync[b] = 1;
+        }
+    }
+    return 0;
+}
+
+// Marks the second buffers stale after the primary storage was written from outside a step.
+void touch_values(State &s) { std::fill(s.t2_sync.begin(), s.t2_sync.end(), 0); }
+
 uint32_t next_seed(State &s) {
     if (++s.step_seed == 0u) s.step_seed = 1u;
     return s.step_seed;
@@ -662,19 +855,47 @@ std::vector<Launch> launch_plan(const State &s) {
     return plan;
 }
 
+// nsteps of the launch plan on the library's stream.  Fused blocks alternate between their two
+// buffers and end in the primary one (one device copy after an odd number of steps), so the
+// stored values are always in dT whatever the path.  evs (optional) receives 5 events per pass.
+hipError_t run_steps(State &s, const std::vector<Launch> &plan, int nsteps, double dt, hipEvent_t *evs) {
+    std::vector<char> where(s.dT.size(), 0);
+    for (int st = 0; st < nsteps; ++st) {
+        // edge transport of the velocity, shared by every tracer pass of the step
+        hipError_t e = launch_qflux(s.m64, s.stream);
+        if (e != hipSuccess) return e;
+        const uint32_t seed = next_seed(s);
+        for (size_t gi = 0; gi < plan.size(); ++gi) {
+            const Launch &l = plan[gi];
+            hipEvent_t *ev = evs ? &evs[((size_t)st * plan.size() + gi) * 5] : nullptr;
+            if (fused_applies(s, l.sc)) {
+                double *in = static_cast<double *>(where[l.blk] ? s.dT2[l.blk] : s.dT[l.blk]);
+                double *out = static_cast<double *>(where[l.blk] ? s.dT[l.blk] : s.dT2[l.blk]);
+                for (int k = 0; k < 4 && ev; ++k)
+                    if ((e = hipEventRecord(ev[k], s.stream)) != hipSuccess) return e;
+                e = launch_tile_step(s.m64, s.tm, in, out, l.sc.hor, l.sc.ver, l.sc.ph, l.sc.pv, dt, s.stream);
+                if (e == hipSuccess && ev) e = hipEventRecord(ev[4], s.stream);
+                where[l.blk] ^= 1;
+            } else {
+                e = launch_block(s, l.blk, l.mask, l.sc, dt, false, ev, seed);
+            }
+            if (e != hipSuccess) return e;
+        }
+    }
+    const size_t bytes = (size_t)(s.nl - 1) * s.Np * s.kb * storage_bytes(s.prec);
+    for (size_t b = 0; b < where.size(); ++b)
+        if (where[b]) {
+            hipError_t e = hipMemcpyAsync(s.dT[b], s.dT2[b], bytes, hipMemcpyDeviceToDevice, s.stream);
+            if (e != hipSuccess) return e;
+        }
+    return hipSuccess;
+}
+
 // Captures nsteps of the launch plan into one executable graph (kept until the state changes).
 int capture_graph(State &s, const std::vector<Launch> &plan, int nsteps, double dt) {
     drop_graph(s);
     HIPCHK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
-    hipError_t err = hipSuccess;
-    for (int st = 0; st < nsteps && err == hipSuccess; ++st) {
-        err = launch_qflux(s.m64, s.stream);
-        const uint32_t seed = next_seed(s);
-        for (const Launch &l : plan) {
-            if (err != hipSuccess) break;
-            err = launch_block(s, l.blk, l.mask, l.sc, dt, false, nullptr, seed);
-        }
-    }
+    const hipError_t err = run_steps(s, plan, nsteps, dt, nullptr);
     hipGraph_t graph = nullptr;
     const hipError_t ec = hipStreamEndCapture(s.stream, &graph);
     if (err != hipSuccess || ec != hipSuccess) {
@@ -698,6 +919,7 @@ int advect(State &s, int nsteps, double dt) {
     if (s.ntr == 0) return fail("no tracers allocated (tracer_allocate_tracers)");
     std::fill(s.tab_on.begin(), s.tab_on.end(), 0);  // valuesAB := values
     const std::vector<Launch> plan = launch_plan(s);
+    if (s.fused && s.prec == PREC_F64 && s.kb == 1 && ensure_fused(s)) return -1;
     if (!plan.empty()) s.last_fct = plan.back().sc.fct;
     if (s.use_graph && !s.kernel_timing && nsteps > 0) {
         if (!s.gexec || s.g_nsteps != nsteps || s.g_dt != dt || s.g_gen != s.gen)
@@ -717,16 +939,9 @@ int advect(State &s, int nsteps, double dt) {
         for (auto &e : evs) HIPCHK(hipEventCreate(&e));
     }
     HIPCHK(hipEventRecord(s.ev0, s.stream));
-    for (int st = 0; st < nsteps; ++st) {
-        // edge transport of the velocity, shared by every tracer pass of the step
-        hipError_t e = launch_qflux(s.m64, s.stream);
+    {
+        const hipError_t e = run_steps(s, plan, nsteps, dt, s.kernel_timing ? evs.data() : nullptr);
         if (e != hipSuccess) return fail("kernel launch failed: %s", hipGetErrorString(e));
-        const uint32_t seed = next_seed(s);
-        for (size_t gi = 0; gi < plan.size(); ++gi) {
-            hipEvent_t *ev = s.kernel_timing ? &evs[((size_t)st * plan.size() + gi) * 5] : nullptr;
-            e = launch_block(s, plan[gi].blk, plan[gi].mask, plan[gi].sc, dt, false, ev, seed);
-            if (e != hipSuccess) return fail("kernel launch failed: %s", hipGetErrorString(e));
-        }
     }
     HIPCHK(hipEventRecord(s.ev1, s.stream));
     HIPCHK(hipStreamSynchronize(s.stream));
@@ -894,7 +1109,7 @@ int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D) {
     const hipStream_t keep = s.stream;
     hipEvent_t k0 = s.ev0, k1 = s.ev1;
     const int prec = s.prec, kbr = s.kb_req;
-    const bool graph = s.use_graph, sr = s.stochastic;
+    const bool graph = s.use_graph, sr = s.stochastic, fused = s.fused;
     const uint64_t gen = s.gen;
     const LaunchCfg cfg = s.cfg;
     s = State();
@@ -906,6 +1121,7 @@ int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D) {
     s.kb_req = kbr;
     s.use_graph = graph;
     s.stochastic = sr;
+    s.fused = fused;
     s.gen = gen + 1;
     s.nl = nl;
     s.N = nod2D;
@@ -977,6 +1193,7 @@ int tracer_set_values(int tracer_id, const double *values, int nl, int nod2D) {
         const int b = (tracer_id - 1) / s.kb, lane = (tracer_id - 1) % s.kb;
         if (to_device(s, v.data(), false, s.dT[b], lane)) return -1;
         if (s.tab_on[b] && to_device(s, v.data(), false, s.tabbuf[b], lane)) return -1;
+        touch_values(s);
     }
     return 0;
 }
@@ -1272,8 +1489,23 @@ int tracer_checkpoint(int restore) {
         for (size_t b = 0; b < s.dT.size(); ++b)
             HIPCHK(hipMemcpyAsync(s.dT[b], s.ckpt[b], bytes, hipMemcpyDeviceToDevice, s.stream));
         std::fill(s.tab_on.begin(), s.tab_on.end(), 0);
+        touch_values(s);
     }
     HIPCHK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+int tracer_set_fused(int enable) {
+    State &s = S();
+    s.fused = enable != 0;
+    s.gen++;
+    if (s.fused && s.initialized) {
+        if (ensure_device(s)) return -1;
+        if (!s.tiles_built && build_tiles(s)) {
+            s.fused = false;
+            return -1;
+        }
+    }
     return 0;
 }
 
@@ -1385,6 +1617,7 @@ int tracer_set_values_device(int tracer_id, const void *values, int nl, int nod2
     const int b = (tracer_id - 1) / s.kb, lane = (tracer_id - 1) % s.kb;
     if (to_device(s, static_cast<const double *>(values), true, s.dT[b], lane)) return -1;
     if (s.tab_on[b] && to_device(s, static_cast<const double *>(values), true, s.tabbuf[b], lane)) return -1;
+    touch_values(s);
     return 0;
 }
 

@@ -189,48 +189,6 @@ __device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges &
     }
 }
 
-// Column readers of one node (1-based levels; 0 outside the wet range).
-template <typename C, typename S, typename M, int KB>
-struct NodeCol {
-    size_t Np;
-    int n, lane, L;
-    __device__ __forceinline__ C lay(const S *a, int nz) const {
-        return (nz >= 1 && nz <= L) ? ldc<C>(a, ((size_t)(nz - 1) * Np + n) * KB + lane) : C(0);
-    }
-    __device__ __forceinline__ C mlay(const M *a, int nz) const {
-        return (nz >= 1 && nz <= L) ? (C)cld(a, (size_t)(nz - 1) * Np + n) : C(0);
-    }
-    __device__ __forceinline__ C itf(const M *a, int j) const { return (C)cld(a, (size_t)(j - 1) * Np + n); }
-};
-
-// High-order vertical flux at interface j minus fin for the window tw = TAB(j-2..j+1),
-// z = Z(j-2..j+1).
-template <int VER, typename C, typename S, typename M, int KB>
-__device__ __forceinline__ C ver_ho_at(int j, int nzmax, const C tw[4], const C z[4], C zbj, C wj, C aj,
-                                       C num_ord, C dt, const Col<C, S, M, KB> &col, C fin) {
-    if (VER == VER_QR4C) return ver_qr4c_w<C>(j, nzmax, tw, z, zbj, wj, aj, num_ord, fin);
-    if (VER == VER_UPW1) return ver_upw1_w<C>(j, nzmax, tw[1], tw[2], wj, aj, fin);
-    if (VER == VER_CDIFF) return ver_cdiff(col, j, fin);
-    return ver_ppm(col, j, dt, fin);
-}
-
-template <typename C, typename S, typename M, int KB>
-__device__ __forceinline__ Col<C, S, M, KB> make_col(const S *t, const M *w, const MeshCol<M> &m, int n,
-                                                     int lane, int nzmax) {
-    Col<C, S, M, KB> c;
-    c.t = t;
-    c.w = w;
-    c.ar = m.area;
-    c.hn = m.hnode_new;
-    c.ho = m.hnode;
-    c.Np = m.Np;
-    c.n = n;
-    c.lane = lane;
-    c.nzmin = 1;
-    c.nzmax = nzmax;
-    return c;
-}
-
 // Low-order solution and vertical antidiffusive flux (lib/oce_adv_tra_driver.F90:102-216 and
 // the high-order vertical call :266-282 with o_init_zero=.false., pwvel = w).  Interface fluxes
 // are evaluated top-down in double; layer nz uses the fluxes at its top (nz) and bottom (nz + 1).

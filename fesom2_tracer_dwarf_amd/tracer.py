@@ -129,6 +129,10 @@ class FESOMTracerAdvection:
         abi.check(self.lib.tracer_get_tracers_per_pass(ctypes.byref(kb)), "tracer_get_tracers_per_pass")
         return kb.value
 
+    def set_fused(self, enable: bool) -> None:
+        """Single fused kernel per step for FP64 FCT tracers in blocks of one."""
+        abi.check(self.lib.tracer_set_fused(int(bool(enable))), "tracer_set_fused")
+
     def set_graph(self, enable: bool) -> None:
         abi.check(self.lib.tracer_set_graph(int(bool(enable))), "tracer_set_graph")
 

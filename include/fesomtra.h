@@ -180,6 +180,12 @@ int tracer_get_tracers_per_pass(int *kb);
  * with the same nsteps and dt until the scheme, precision, width or launch configuration
  * changes.  Per-stage kernel timing, when enabled, uses the uncaptured path. */
 int tracer_set_graph(int enable);
+/* enable = 1: FP64 tracers with the FCT limiter in blocks of width 1 are advanced by one fused
+ * kernel per step (tiles of nodes with a three-ring halo, exchanging values level by level in
+ * LDS), which stores no intermediate arrays.  Results are bit-identical to the default path.
+ * Returns -1 if the mesh cannot be tiled (a node with more than 8 edges); other tracers keep the
+ * multi-kernel path. */
+int tracer_set_fused(int enable);
 /* Levels per thread of the four kernels (edge fluxes, low-order / vertical, bounds, final): each
  * column is split into chunks of that many layers, one thread per chunk and tracer lane.  The
  * default is 16 for each; results do not depend on the choice. */
