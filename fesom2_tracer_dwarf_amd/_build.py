@@ -58,8 +58,9 @@ def _run(cmds, verbose):
         raise RuntimeError("hipcc failed building libfesomtra.so")
 
 
-def build(verbose: bool = False) -> Path:
-    inc = [f"-I{REPO / 'include'}", f"-I{CSRC}"]
+def build(verbose: bool = False, defines=(), out: Path = LIB) -> Path:
+    """Compiles the library; ``defines`` are extra -D options, ``out`` the shared object."""
+    inc = [f"-I{REPO / 'include'}", f"-I{CSRC}", *[f"-D{d}" for d in defines]]
     with tempfile.TemporaryDirectory(prefix="fesomtra_build_") as td:
         objs, cmds = [], []
         for fam, bit in FAMILIES.items():
@@ -74,10 +75,12 @@ def build(verbose: bool = False) -> Path:
         cmds.append([hipcc(), *HIPFLAGS, *inc, "-c", str(CSRC / ABI), "-o", obj])
         objs.append(obj)
         _run(cmds, verbose)
-        tmp = LIB.with_suffix(".so.tmp")
+        out = Path(out)
+        out.parent.mkdir(parents=True, exist_ok=True)
+        tmp = out.with_suffix(".so.tmp")
         _run([[hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp)]], verbose)
-        os.replace(tmp, LIB)
-    return LIB
+        os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -8,11 +8,13 @@ fails loudly if the library has not been built: there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
+import os
 from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "libfesomtra.so"
+# FESOMTRA_LIB selects another build of the library (e.g. one compiled with different options).
+LIB_PATH = Path(os.environ.get("FESOMTRA_LIB", Path(__file__).resolve().parent / "libfesomtra.so"))
 
 _c_int = ctypes.c_int
 _c_double = ctypes.c_double

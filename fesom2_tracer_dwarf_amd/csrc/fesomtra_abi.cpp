@@ -81,7 +81,7 @@ struct State {
     std::vector<void *> dT, tabbuf;  // per tracer block, storage type of the precision mode
     std::vector<char> tab_on;        // per block: valuesAB held apart from values
     std::vector<void *> ckpt;        // per block: device copy kept by tracer_checkpoint
-    void *flo = nullptr, *adfh = nullptr, *LO = nullptr, *Rp = nullptr, *Rm = nullptr, *adfv = nullptr;
+    void *adfh = nullptr, *LO = nullptr, *Rp = nullptr, *Rm = nullptr, *adfv = nullptr;
     double *acc_h = nullptr, *acc_v = nullptr;
     void *dbg_h = nullptr, *dbg_v = nullptr;
     double *stage = nullptr;  // reference-layout staging buffer (N * L doubles)
@@ -173,7 +173,7 @@ void release_device(State &s) {
     s.t2_sync.clear();
     s.tiles_built = false;
     s.tm = TileMesh{};
-    s.flo = s.adfh = s.LO = s.Rp = s.Rm = s.adfv = nullptr;
+    s.adfh = s.LO = s.Rp = s.Rm = s.adfv = nullptr;
     s.acc_h = s.acc_v = nullptr;
     s.dbg_h = s.dbg_v = nullptr;
     s.stage = nullptr;
@@ -536,8 +536,8 @@ int build_device(State &s) {
     s.kb = block_width(s);
     const size_t cb = compute_bytes(s.prec), sb = storage_bytes(s.prec), kb = s.kb;
     const size_t NLn = (size_t)L * s.Np, ELn = (size_t)L * s.Ep, NIn = (size_t)nl * s.Np;
-    // the upwind edge fluxes are stored only by the FP64 path (single precision recomputes them)
-    if ((s.prec == PREC_F64 && dalloc_bytes(s, &s.flo, ELn * kb * 8)) || dalloc_bytes(s, &s.adfh, ELn * kb * cb) ||
+    // the upwind edge fluxes are recomputed where they are summed, never stored
+    if (dalloc_bytes(s, &s.adfh, ELn * kb * cb) ||
         dalloc_bytes(s, &s.LO, NLn * kb * cb) || dalloc_bytes(s, &s.Rp, NLn * kb * cb) ||
         dalloc_bytes(s, &s.Rm, NLn * kb * cb) || dalloc_bytes(s, &s.adfv, NIn * kb * cb))
         return -1;
@@ -632,7 +632,6 @@ StepPtrs<S_, C_> make_ptrs(State &s, int blk, unsigned long long mask, bool drop
     p.seed = (s.prec != PREC_F64 && s.stochastic) ? seed : 0u;
     p.T = static_cast<S_ *>(s.dT[blk]);
     p.TAB = static_cast<const S_ *>(s.tab_on[blk] ? s.tabbuf[blk] : s.dT[blk]);
-    p.flo = static_cast<double *>(s.flo);
     p.adfh = static_cast<C_ *>(s.adfh);
     p.LO = static_cast<C_ *>(s.LO);
     p.Rp = static_cast<C_ *>(s.Rp);

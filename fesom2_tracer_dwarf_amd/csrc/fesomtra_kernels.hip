@@ -19,6 +19,14 @@
 
 namespace fesomtra {
 
+// Occupancy target (waves per SIMD) handed to the register allocator: columns of one lane per
+// thread are latency-bound gathers and gain from more resident waves.
+#ifndef FESOMTRA_MIN_WAVES
+#define FESOMTRA_MIN_WAVES 4
+#endif
+template <int KB>
+constexpr int kMinWaves = KB == 64 ? 1 : FESOMTRA_MIN_WAVES;
+
 // ---------------------------------------------------------------------------------------------
 // Work decomposition: one thread owns `lc` consecutive levels of one column for one lane of the
 // tracer block; blockIdx.x selects columns (XCD-aware, see column_index) and blockIdx.y the level
@@ -34,7 +42,7 @@ __device__ __forceinline__ bool chunk_layers(int L, int lc, int &nz0, int &nz1) 
 }
 
 template <int HOR, bool FCT, int KB, typename S, typename M, typename C>
-__global__ __launch_bounds__(256) void k_edge(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, int lc) {
+__global__ __launch_bounds__(256, kMinWaves<KB>) void k_edge(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, int lc) {
     int e, lane;
     column_lane<KB>(e, lane);
     if (e >= m.E || !lane_on(p.mask, lane)) return;
@@ -104,11 +112,7 @@ __global__ __launch_bounds__(256) void k_edge(const MeshCol<M> m, const StepPtrs
         }
         C f;
         if constexpr (std::is_same<C, double>::value) {  // the reference's operation order
-            C fin = 0;
-            if (FCT) {
-                fin = upw1_edge(t1, t2, v);
-                p.flo[ie] = fin;
-            }
+            const C fin = FCT ? upw1_edge(t1, t2, v) : C(0);
             if (HOR == HOR_UPW1) f = upw1_edge(a1, a2, v) - fin;
             else f = ho_edge<HOR, C>(a1, a2, gr, v, dx, dy, a, num_ord, c1, c2, fin);
         } else {
@@ -192,10 +196,9 @@ __device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges &
 // Low-order solution and vertical antidiffusive flux (lib/oce_adv_tra_driver.F90:102-216 and
 // the high-order vertical call :266-282 with o_init_zero=.false., pwvel = w).  Interface fluxes
 // are evaluated top-down in double; layer nz uses the fluxes at its top (nz) and bottom (nz + 1).
-// The FP64 path gathers the stored upwind edge fluxes; the single-precision paths form them here
-// in double from q and the two end values.
+// The upwind edge fluxes are formed here in double from q and the two end values (never stored).
 template <int VER, int KB, typename S, typename M, typename C>
-__global__ __launch_bounds__(256) void k_lo(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt, int lc) {
+__global__ __launch_bounds__(256, kMinWaves<KB>) void k_lo(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt, int lc) {
     using D = double;
     int n, lane;
     column_lane<KB>(n, lane);
@@ -243,20 +246,15 @@ __global__ __launch_bounds__(256) void k_lo(const MeshCol<M> m, const StepPtrs<S
         const D tnext = nc.lay(p.T, j);
         const D lo_bot = ver_upw1_w<D>(j, nzmax, tcur, tnext, wej, aj, D(0));
         const D adf_bot = ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, lo_bot);
+        // upwind edge fluxes (lib/oce_adv_tra_hor.F90:178-181) from q and the two end values,
+        // summed in reference edge order
         double s = 0.0;
         const size_t eo = (size_t)(nz - 1) * Ep, no = (size_t)(nz - 1) * Np;
-        if constexpr (std::is_same<C, double>::value) {
-            for_edges(m, ne, nz, [&](int e, bool second, int) {
-                const double f = p.flo[(eo + e) * KB + lane];
-                s = second ? s - f : s + f;
-            });
-        } else {
-            for_edges(m, ne, nz, [&](int e, bool second, int oth) {
-                const double to = ldc<double>(p.T, (no + oth) * KB + lane);
-                const double f = upw1_edge<double>(second ? to : tcur, second ? tcur : to, cld(m.q, eo + e));
-                s = second ? s - f : s + f;
-            });
-        }
+        for_edges(m, ne, nz, [&](int e, bool second, int oth) {
+            const double to = ldc<double>(p.T, (no + oth) * KB + lane);
+            const double f = upw1_edge<double>(second ? to : tcur, second ? tcur : to, cld(m.q, eo + e));
+            s = second ? s - f : s + f;
+        });
         const double lo = ((double)tcur * h + (s + ((double)lo_top - (double)lo_bot)) * dt / V) / hn;
         p.LO[cell * KB + lane] = (C)lo;
         p.adfv[cell * KB + lane] = (C)adf_top;
@@ -270,7 +268,7 @@ __global__ __launch_bounds__(256) void k_lo(const MeshCol<M> m, const StepPtrs<S
 // Vertical high-order flux without the limiter (pwvel = we, o_init_zero=.true.), in double:
 // interfaces nz0..nz1 of the chunk, plus the bottom interface in the deepest chunk.
 template <int VER, int KB, typename S, typename M, typename C>
-__global__ __launch_bounds__(256) void k_vert(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt, int lc) {
+__global__ __launch_bounds__(256, kMinWaves<KB>) void k_vert(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt, int lc) {
     using D = double;
     int n, lane;
     column_lane<KB>(n, lane);
@@ -313,7 +311,7 @@ __global__ __launch_bounds__(256) void k_vert(const MeshCol<M> m, const StepPtrs
 // nod_in_elem2D that is n plus every neighbour m whose edge (n, m) reaches level nz, and an
 // element dry at nz contributes -1e3 / +1e3 exactly when nz > nlevels_nod2D_min(n) - 1.
 template <int KB, typename S, typename M, typename C>
-__global__ __launch_bounds__(256) void k_bounds(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
+__global__ __launch_bounds__(256, kMinWaves<KB>) void k_bounds(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
     int n, lane;
     column_lane<KB>(n, lane);
     if (n >= m.N || !lane_on(p.mask, lane)) return;
@@ -398,7 +396,7 @@ __global__ __launch_bounds__(256) void k_bounds(const MeshCol<M> m, const StepPt
 // (dttf_h + dttf_v) / hnode_new in place.  dbg_h / dbg_v receive the limited fluxes if set.
 // acc_* and dbg_* hold one tracer: they are only passed with a single-lane mask.
 template <bool FCT, int KB, typename S, typename M, typename C>
-__global__ __launch_bounds__(256) void k_final(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
+__global__ __launch_bounds__(256, kMinWaves<KB>) void k_final(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
     int n, lane;
     column_lane<KB>(n, lane);
     if (n >= m.N || !lane_on(p.mask, lane)) return;

@@ -152,7 +152,6 @@ template <typename S, typename C>
 struct StepPtrs {
     S *T;                  // values (updated in place unless accumulating)
     const S *TAB;          // valuesAB (may equal T)
-    double *flo;           // [L][Ep][KB] low-order edge flux (FP64 path only)
     C *adfh;               // [L][Ep][KB] antidiffusive (or high-order) edge flux
     C *LO;                 // [L][Np][KB] low-order solution
     C *Rp, *Rm;            // [L][Np][KB] limiter ratios
