@@ -65,3 +65,51 @@ def test_error_paths_without_gpu():
     w = np.zeros((4, 10))
     assert lib.tracer_set_velocity(abi.dptr(uv), abi.dptr(w), 10, 4, 12) == -1
     lib.tracer_cleanup()
+
+
+def test_setup_validation_without_gpu():
+    """Mesh checks made at tracer_init_complete (before any device work): cavities, node levels
+    below element levels, missing edge_tri(1); option setters reject bad values."""
+    from fesom2_tracer_dwarf_amd import configs
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    lib = abi.load()
+    case = configs.bathymetry_case("v", 6, 8, seed=3, nsteps=1, lev_lo=3)
+    m = case.mesh
+    t = FESOMTracerAdvection()
+    # cavity: ulevels_nod2D > 1
+    t.setup_case(case, upload=False)
+    ul = m.ulevels_nod2D.astype(np.int32).copy()
+    ul[3] = 2
+    assert lib.tracer_set_levels(abi.iptr(m.nlevels_nod2D.astype(np.int32)), abi.iptr(ul), m.nod2D) == 0
+    assert lib.tracer_init_complete() == -1
+    assert b"cavities" in lib.tracer_last_error()
+    # a node shallower than one of its elements
+    t.setup_case(case, upload=False)
+    nl = m.nlevels_nod2D.astype(np.int32).copy()
+    el = m.elem2D_nodes[0] - 1
+    nl[el[0]] = m.nlevels[0] - 1
+    assert lib.tracer_set_levels(abi.iptr(nl), abi.iptr(m.ulevels_nod2D.astype(np.int32)), m.nod2D) == 0
+    assert lib.tracer_init_complete() == -1
+    assert b"fewer levels" in lib.tracer_last_error()
+    # edge_tri(1) missing
+    t.setup_case(case, upload=False)
+    et = m.edge_tri.astype(np.int32).copy()
+    et[5, 0] = 0
+    assert lib.tracer_set_connectivity(abi.iptr(m.elem2D_nodes.astype(np.int32)), abi.iptr(m.edges.astype(np.int32)),
+                                       abi.iptr(et), abi.iptr(m.nod_in_elem2D.astype(np.int32)),
+                                       abi.iptr(m.nod_in_elem2D_num.astype(np.int32)), m.max_adj, m.nod2D,
+                                       m.elem2D, m.edge2D) == 0
+    assert lib.tracer_init_complete() == -1
+    assert b"edge_tri(1)" in lib.tracer_last_error()
+    # option setters
+    assert lib.tracer_set_tracers_per_pass(3) == -1
+    assert lib.tracer_set_tracers_per_pass(128) == -1
+    assert lib.tracer_set_tracers_per_pass(64) == 0
+    assert lib.tracer_set_tracers_per_pass(0) == 0
+    assert lib.tracer_set_precision(7) == -1
+    assert lib.tracer_set_launch_config(0, 16, 16, 16) == -1
+    assert lib.tracer_set_scheme(1, b"UPW1", b"UPW1", b"FCX", 1.0, 1.0) == -1
+    assert lib.tracer_set_scheme(1, b"UPW1", b"QR4X", b"FCT", 1.0, 1.0) == -1
+    assert b"Unknown vertical" in lib.tracer_last_error()
+    assert lib.tracer_set_scheme(9, b"UPW1", b"UPW1", b"FCT", 1.0, 1.0) == -1
+    lib.tracer_cleanup()
