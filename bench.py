@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import os
 import sys
 import time
 from pathlib import Path
@@ -57,7 +58,8 @@ def algorithmic_bytes(mesh, k_tracers: int = 1, s_tr: int = 8, s_mesh: int = 8) 
                + N * (20 + 4 * adj))
 
 
-def cpu_baseline(case, sample_tracer_steps: int = 1):
+def port_baseline(case, sample_tracer_steps: int = 1):
+    """The CPU restatement (oracle/, C, one thread) on one tracer-step of the full mesh."""
     sys.path.insert(0, str(REPO / "oracle"))
     import oracle_py
     o = oracle_py.Oracle(case)
@@ -71,6 +73,49 @@ def cpu_baseline(case, sample_tracer_steps: int = 1):
             "sample": f"{sample_tracer_steps} tracer-step(s) of tracer T on the full "
                       f"{case.name} mesh, CPU restatement of the reference (oracle/), "
                       f"1 thread, {dt:.2f} s"}
+
+
+def reference_baseline(case, exe: Path, threads: int):
+    """The reference Fortran (lib/*.F90 built from its sources with flang -O2 -fopenmp, its own
+    OpenMP regions and lock-based scatter) on one tracer-step of the full mesh: gradients +
+    fill_up_dn_grad, do_oce_adv_tra and the caller's update, timed with MPI_Wtime."""
+    import dataclasses
+    import resource
+    import subprocess
+    import tempfile
+    sys.path.insert(0, str(REPO / "oracle" / "ref"))
+    import bundle
+    one = dataclasses.replace(case, tracers=case.tracers[:1], schemes=case.schemes[:1], nsteps=1)
+
+    def unlimited_stack():
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+    with tempfile.TemporaryDirectory(prefix="fesomtra_ref_") as td:
+        p = bundle.write_bundle(Path(td) / "b", one, timing=True)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_STACKSIZE="512M")
+        subprocess.run([str(exe), str(p)], capture_output=True, text=True, env=env, timeout=900,
+                       check=True, preexec_fn=unlimited_stack)
+        t = bundle.read_timing(p)
+    total = t["seconds_gradients"] + t["seconds_advection"] + t["seconds_update"]
+    nl = t["node_levels_per_tracer"] * t["tracer_steps"]
+    return {"value": nl / total, "unit": "node-level updates/s", "cores": threads, "kind": "reference",
+            "sample": f"1 tracer-step of tracer T on the full {case.name} mesh ({int(nl)} node-levels), "
+                      f"reference Fortran lib/*.F90 (flang -O2 -fdefault-real-8 -fopenmp, "
+                      f"{threads} OpenMP threads): gradients+fill_up_dn_grad "
+                      f"{t['seconds_gradients']:.2f} s, do_oce_adv_tra {t['seconds_advection']:.2f} s, "
+                      f"update {t['seconds_update']:.2f} s; advection-only rate "
+                      f"{nl / t['seconds_advection']:.3e}/s"}
+
+
+def cpu_baseline(case):
+    exe = REPO / "oracle" / "_ref" / "ref_driver_omp"
+    if exe.exists():
+        threads = max(1, min(16, os.cpu_count() or 1))
+        try:
+            return reference_baseline(case, exe, threads)
+        except Exception as err:  # the port below is the documented fallback
+            log(f"reference CPU baseline failed ({err}); timing the CPU restatement instead")
+    return port_baseline(case)
 
 
 def traffic_from_profiles(name: str):

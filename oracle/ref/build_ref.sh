@@ -75,4 +75,24 @@ LIBS=(-L"$MPI_ROOT/lib" -Wl,-rpath,"$MPI_ROOT/lib" -lmpifort -lmpi)
 "$FC" -c "${FFLAGS[@]}" "$HERE/ref_driver.F90" -o "$OUT/obj/ref_driver.o"
 "$FC" "$OUT/obj/ref_driver.o" "${OBJS[@]}" "${LIBS[@]}" -o "$OUT/ref_driver"
 
-echo "build_ref: built $OUT/fesom_tracer_analytic and $OUT/ref_driver"
+# OpenMP variant of the same driver for the CPU timing baseline (the reference's own OpenMP
+# regions; its lock-based scatter is not bitwise reproducible, so it is never used as a checker).
+OMPFLAGS=(-cpp -O2 -fdefault-real-8 -ffp-contract=off -fopenmp -I"$OUT/inc" -module-dir "$OUT/mod_omp"
+          -I"$OUT/mod_omp")
+mkdir -p "$OUT/obj_omp" "$OUT/mod_omp"
+"$FC" -c "${OMPFLAGS[@]}" "$OUT/inc/mpi_module.F90" -o "$OUT/obj_omp/mpi_module.o"
+OMP_OBJS=("$OUT/obj_omp/mpi_module.o")
+for f in $ORDER; do
+    src="$REF/lib/$f.F90"
+    obj="$OUT/obj_omp/$f.o"
+    if [ ! -f "$obj" ] || [ "$src" -nt "$obj" ]; then
+        "$FC" -c "${OMPFLAGS[@]}" "$src" -o "$obj"
+    fi
+    OMP_OBJS+=("$obj")
+done
+"$FC" -c "${OMPFLAGS[@]}" "$HERE/ref_driver.F90" -o "$OUT/obj_omp/ref_driver.o"
+OMP_LIB_DIR="${OMP_LIB_DIR:-/opt/rocm/llvm/lib}"
+"$FC" -fopenmp "$OUT/obj_omp/ref_driver.o" "${OMP_OBJS[@]}" "${LIBS[@]}" -Wl,-rpath,"$OMP_LIB_DIR" \
+    -o "$OUT/ref_driver_omp"
+
+echo "build_ref: built $OUT/fesom_tracer_analytic, $OUT/ref_driver and $OUT/ref_driver_omp"

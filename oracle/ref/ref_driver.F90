@@ -29,6 +29,9 @@ program ref_driver
   use oce_adv_tra_driver_module, only: do_oce_adv_tra
   use oce_adv_tra_fct_module, only: oce_adv_tra_fct_init
   use oce_muscl_adv_module, only: fill_up_dn_grad
+#if defined(_OPENMP)
+  use omp_lib
+#endif
   implicit none
 
   type(t_mesh),   target :: mesh
@@ -79,6 +82,13 @@ program ref_driver
   partit%myDim_nod2D = nod2D;   partit%eDim_nod2D = 0
   partit%myDim_elem2D = elem2D; partit%eDim_elem2D = 0; partit%eXDim_elem2D = 0
   partit%myDim_edge2D = edge2D; partit%eDim_edge2D = 0
+#if defined(_OPENMP)
+  ! the node locks of the reference's OpenMP scatter (as lib/gen_modules_partitioning.F90:578-585)
+  allocate(partit%plock(max(nod2D, elem2D)))
+  do i = 1, size(partit%plock)
+     call omp_init_lock(partit%plock(i))
+  end do
+#endif
 
   allocate(mesh%elem2D_nodes(3, elem2D), mesh%edges(2, edge2D), mesh%edge_tri(2, edge2D))
   allocate(mesh%nod_in_elem2D(max_adj, nod2D), mesh%nod_in_elem2D_num(nod2D))
