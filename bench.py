@@ -294,6 +294,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the mixed-precision batched companion run")
     ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--rounding", default="stochastic", choices=["stochastic", "nearest"],
+                    help="rounding of the tracer update in the single-precision modes")
     ap.add_argument("--no-batched-accuracy", action="store_true",
                     help="skip the FP64 run of the batched workload used for its error")
     ap.add_argument("--precision-study", action="store_true")
@@ -337,7 +339,8 @@ def main():
     m = case.mesh
     log(f"workload {describe(case)}: N={m.nod2D} T={m.elem2D} E={m.edge2D} N_L={m.n_node_levels} "
         f"(generated in {time.perf_counter() - t0:.1f} s)")
-    r = run(case, prec, kb, steps, args.warmup, graph, lc, tune=not args.no_autotune, fused=fused)
+    r = run(case, prec, kb, steps, args.warmup, graph, lc, tune=not args.no_autotune, fused=fused,
+            stochastic=args.rounding == "stochastic")
     log(f"timed {steps} steps: wall {r['wall_s'] * 1e3:.2f} ms, device {r['dev_ms']:.2f} ms, "
         f"{r['value']:.3e} updates/s, stages {r['stage_ms']}")
     roof = roofline(case, prec, r, args.workload)
@@ -351,10 +354,12 @@ def main():
             r64 = run(bcase, 0, 64, args.batched_steps, 0, keep=True, tune=not args.no_autotune)
             ref64 = r64["values"]
             log(f"batched FP64 reference: {r64['value']:.3e} updates/s")
-        rb = run(bcase, 1, 64, args.batched_steps, 0, compare=ref64, tune=not args.no_autotune)
+        rb = run(bcase, 1, 64, args.batched_steps, 0, compare=ref64, tune=not args.no_autotune,
+                 stochastic=args.rounding == "stochastic")
         ref64 = None
         batched = {"workload": describe(bcase), "precision": "FP32 arithmetic and storage, FP64 "
-                   "accumulation / update / limiter ratios / sums", "block_width": rb["block_width"],
+                   "accumulation / update / limiter ratios / sums", "rounding": args.rounding,
+                   "block_width": rb["block_width"],
                    "steps": args.batched_steps, "value": rb["value"], "unit": "node-level updates/s",
                    "ms_per_step": rb["wall_s"] * 1e3 / args.batched_steps,
                    "vs_fp64_value": rb["value"] / r["value"],

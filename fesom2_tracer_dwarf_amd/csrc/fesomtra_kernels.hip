@@ -256,7 +256,9 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_lo(const MeshCol<M> m, c
             s = second ? s - f : s + f;
         });
         const double lo = ((double)tcur * h + (s + ((double)lo_top - (double)lo_bot)) * dt / V) / hn;
-        p.LO[cell * KB + lane] = (C)lo;
+        // FP64 stores LO; single precision stores the increment LO - T, so that low-order changes
+        // far below the tracer's own magnitude survive the rounding to C
+        p.LO[cell * KB + lane] = std::is_same<C, double>::value ? (C)lo : (C)(lo - (double)tcur);
         p.adfv[cell * KB + lane] = (C)adf_top;
         if (bottom) p.adfv[(cell + Np) * KB + lane] = (C)adf_bot;
         lo_top = lo_bot;
@@ -322,23 +324,30 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_bounds(const MeshCol<M> 
     const int dry_from = cld(m.nlev_min, n);  // levels >= this have a dry element around n
     NodeEdges ne;
     load_edges(m, n, ne);
-    const LaneLd<KB, C, S> ldT{p.T, lane};
-    auto cluster = [&](int nz, C &vmax, C &vmin) {
+    using D = double;
+    const LaneLd<KB, D, S> ldT{p.T, lane};
+    // LO of a cell with tracer value t (stored as LO in FP64, as LO - T in single precision)
+    auto lo_at = [&](size_t cell, D t) -> D {
+        if constexpr (std::is_same<C, double>::value) return p.LO[cell * KB + lane];
+        else return t + (D)p.LO[cell * KB + lane];
+    };
+    auto cluster = [&](int nz, D &vmax, D &vmin) {
         const size_t o = (size_t)(nz - 1) * Np;
-        const C l0 = p.LO[(o + n) * KB + lane], t0 = ldT(o + n);
+        const D t0 = ldT(o + n), l0 = lo_at(o + n, t0);
         vmax = cmax(l0, t0);
         vmin = cmin(l0, t0);
         for_edges(m, ne, nz, [&](int, bool, int oth) {
-            const C lo = p.LO[(o + oth) * KB + lane], tt = ldT(o + oth);
+            const D tt = ldT(o + oth), lo = lo_at(o + oth, tt);
             vmax = cmax(vmax, cmax(lo, tt));
             vmin = cmin(vmin, cmin(lo, tt));
         });
         if (nz >= dry_from) {
-            vmax = cmax(vmax, C(-1.0e3));
-            vmin = cmin(vmin, C(1.0e3));
+            vmax = cmax(vmax, D(-1.0e3));
+            vmin = cmin(vmin, D(1.0e3));
         }
     };
-    C mx[3], mn[3], av0;
+    D mx[3], mn[3];
+    C av0;
     if (nz0 >= 2 && nz0 < L) cluster(nz0 - 1, mx[0], mn[0]);
     else mx[0] = mn[0] = 0;
     cluster(nz0, mx[1], mn[1]);
@@ -348,8 +357,8 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_bounds(const MeshCol<M> 
     for (int nz = nz0; nz <= nz1; ++nz) {
         const size_t cell = (size_t)(nz - 1) * Np + n;
         const double V = (double)cld(m.areasvol, cell);
-        const C lo = p.LO[cell * KB + lane];
-        C qmax, qmin;
+        const D lo = lo_at(cell, ldT(cell));
+        D qmax, qmin;
         if (nz == L || nz == 1) {
             qmax = mx[1] - lo;
             qmin = mn[1] - lo;
@@ -445,7 +454,8 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_final(const MeshCol<M> m
         C bot;
         double dv = p.acc_v ? p.acc_v[cell] : 0.0;
         if (FCT) {
-            dv = dv - tval * h + (double)p.LO[X(cell)] * hn;
+            const double lo = std::is_same<C, double>::value ? (double)p.LO[X(cell)] : tval + (double)p.LO[X(cell)];
+            dv = dv - tval * h + lo * hn;
             rp0 = rp1;
             rm0 = rm1;
             if (nz + 1 <= L) {

@@ -138,7 +138,8 @@ int tracer_get_stats_by_id(int tracer_id, double *tmin, double *tmax, double *ts
                            double *vsum);
 /* Work arrays of the last tracer_do_oce_adv_tra / advect call for inspection: name is
  * "fct_LO" (nl-1, nod2D), "adv_flux_hor" (nl-1, edge2D) or "adv_flux_ver" (nl, nod2D), the
- * latter two after limiting as the reference leaves them.  count = number of doubles. */
+ * latter two after limiting as the reference leaves them.  count = number of doubles.  In the
+ * single-precision modes "fct_LO" holds the increment fct_LO - values. */
 int tracer_get_work(const char *name, double *out, size_t count);
 /* Device-pointer variants of tracer_set_values / tracer_get_values for callers whose tracers
  * already live in HBM (e.g. PyTorch tensors): values(nl-1, nod2D) float64 in reference order. */
