@@ -60,6 +60,7 @@ struct State {
     bool use_graph = false;
     bool stochastic = true;  // stochastic rounding of the update in the single-precision modes
     uint32_t step_seed = 0;  // advanced once per step
+    uint32_t ckpt_seed = 0;  // step_seed at the last tracer_checkpoint(0)
     uint64_t gen = 0;  // bumped whenever captured launch sequences become stale
 
     // device side
@@ -1483,12 +1484,14 @@ int tracer_checkpoint(int restore) {
             if (!s.ckpt[b] && dalloc_bytes(s, &s.ckpt[b], bytes)) return -1;
             HIPCHK(hipMemcpyAsync(s.ckpt[b], s.dT[b], bytes, hipMemcpyDeviceToDevice, s.stream));
         }
+        s.ckpt_seed = s.step_seed;
     } else {
         if (s.ckpt.size() != s.dT.size()) return fail("tracer_checkpoint: no checkpoint of the current storage");
         for (size_t b = 0; b < s.dT.size(); ++b)
             HIPCHK(hipMemcpyAsync(s.dT[b], s.ckpt[b], bytes, hipMemcpyDeviceToDevice, s.stream));
         std::fill(s.tab_on.begin(), s.tab_on.end(), 0);
         touch_values(s);
+        s.step_seed = s.ckpt_seed;  // the restored state continues the same rounding sequence
     }
     HIPCHK(hipStreamSynchronize(s.stream));
     return 0;

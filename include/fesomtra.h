@@ -166,8 +166,9 @@ int tracer_set_precision(int mode);
  * width), 0 = round to nearest.  Without it, increments smaller than half a unit in the last place
  * are lost at every step and slowly varying fields stagnate. */
 int tracer_set_rounding(int stochastic);
-/* restore = 0: keeps a device copy of every tracer (in HBM, storage precision); restore = 1:
- * puts it back (valuesAB := values).  Lets a caller time or tune steps without losing state. */
+/* restore = 0: keeps a device copy of every tracer (in HBM, storage precision) and the step
+ * counter of the stochastic rounding; restore = 1: puts both back (valuesAB := values).  Lets a
+ * caller time or tune steps without changing later results. */
 int tracer_checkpoint(int restore);
 /* Tracer block width kb (a power of two up to 64; 0 = automatic, the default: all tracers, up to
  * 64, in one block).  Tracer id i is stored as lane (i-1) % kb of block (i-1) / kb, interleaved

@@ -349,3 +349,25 @@ def test_fused_graph_and_mixed_limiters(oracle_mod):
     t.set_graph(False)
     t.set_block_width(0)
     t.cleanup()
+
+
+def test_autotune_does_not_change_single_precision_results():
+    """The checkpoint restores the stochastic-rounding step counter with the tracers, so a tuned
+    run gives bit-identical FP32 results to an untuned one."""
+    from fesom2_tracer_dwarf_amd import autotune
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    case = configs.bathymetry_case("bathy12r", 12, 11, seed=59, ntracers=4, nsteps=4, lev_lo=3,
+                                   w_cfl=0.3)
+    say("case bathy12r FP32 with and without autotune")
+    outs = []
+    for tune in (False, True):
+        t = FESOMTracerAdvection()
+        t.set_precision(1)
+        t.setup_case(case)
+        if tune:
+            autotune.tune(t, case, 1, candidates=(3, 7), steps=2, use_cache=False)
+        t.advect_steps(4, case.dt)
+        outs.append(t.get_all_values())
+        t.set_precision(0)
+        t.cleanup()
+    assert np.array_equal(outs[0], outs[1])
