@@ -75,6 +75,8 @@ _SIGNATURES = {
     "tracer_get_tracers_per_pass": (_c_int, [_pi]),
     "tracer_set_graph": (_c_int, [_c_int]),
     "tracer_set_fused": (_c_int, [_c_int]),
+    "tracer_set_halo_tiles": (_c_int, [_c_int]),
+    "tracer_get_halo_tiles": (_c_int, [_pi]),
     "tracer_set_rounding": (_c_int, [_c_int]),
     "tracer_checkpoint": (_c_int, [_c_int]),
     "tracer_set_launch_config": (_c_int, [_c_int, _c_int, _c_int, _c_int]),

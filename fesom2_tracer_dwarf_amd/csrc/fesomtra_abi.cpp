@@ -75,6 +75,9 @@ struct State {
     // fused single-pass path (FP64, FCT, one tracer per block)
     bool fused = false;
     bool tiles_built = false;
+    // halo tiles of the node kernels (tracer blocks of width <= 4)
+    bool halo_tiles = true;
+    int ht_built = 0;  // number of tiles built for the current block width (0: none)
     TileMesh tm{};
     std::vector<void *> dT2;    // second tracer buffer per block (output of a fused step)
     std::vector<char> t2_sync;  // dT2 holds the same below-bottom values as dT
@@ -174,6 +177,7 @@ void release_device(State &s) {
     s.t2_sync.clear();
     s.tiles_built = false;
     s.tm = TileMesh{};
+    s.ht_built = 0;
     s.adfh = s.LO = s.Rp = s.Rm = s.adfv = nullptr;
     s.acc_h = s.acc_v = nullptr;
     s.dbg_h = s.dbg_v = nullptr;
@@ -399,6 +403,47 @@ int upload_mesh(State &s, MeshCol<M> &m, const std::vector<int> &ints_n1, const 
     return 0;
 }
 
+// Halo tiles of the node kernels for a tracer block of width kb <= 4: tile t owns the
+// tc = 256 / kb consecutive device nodes from t * tc (a compact patch of the space-filling
+// curve); its halo is the ascending list of far ends of its nodes' edges outside the tile.  A
+// tile's workgroup stages two slots per thread (its column and one halo node), so the halo may
+// hold at most tc nodes; meshes with a larger halo anywhere run the untiled kernels.
+int build_halo_tiles(State &s) {
+    MeshCol<double> &m = s.m64;
+    m.ht_n = m.ht_kb = 0;
+    s.ht_built = 0;
+    if (s.kb > 4 || s.N < 1) return 0;
+    const int tc = 256 / s.kb, N = s.N, nt = (N + tc - 1) / tc;
+    std::vector<int> hptr(nt + 1, 0), halo, loc(s.h_oth.size()), lmax(nt, 0), hl;
+    for (int t = 0; t < nt; ++t) {
+        const int a0 = t * tc, a1 = std::min(N, a0 + tc);
+        hl.clear();
+        for (int n = a0; n < a1; ++n) {
+            lmax[t] = std::max(lmax[t], s.h_nlev[n] - 1);
+            for (int i = s.h_ptr[n]; i < s.h_ptr[n + 1]; ++i)
+                if (s.h_oth[i] < a0 || s.h_oth[i] >= a1) hl.push_back(s.h_oth[i]);
+        }
+        std::sort(hl.begin(), hl.end());
+        hl.erase(std::unique(hl.begin(), hl.end()), hl.end());
+        if ((int)hl.size() > tc) return 0;
+        for (int n = a0; n < a1; ++n)
+            for (int i = s.h_ptr[n]; i < s.h_ptr[n + 1]; ++i) {
+                const int o = s.h_oth[i];
+                loc[i] = (o >= a0 && o < a1) ? o - a0 : tc + (int)(std::lower_bound(hl.begin(), hl.end(), o) - hl.begin());
+            }
+        halo.insert(halo.end(), hl.begin(), hl.end());
+        hptr[t + 1] = (int)halo.size();
+    }
+    if (halo.empty()) halo.push_back(0);  // keep a valid allocation
+    if (upload(s, &m.ht_loc, loc) || upload(s, &m.ht_hptr, hptr) || upload(s, &m.ht_halo, halo) ||
+        upload(s, &m.ht_lmax, lmax))
+        return -1;
+    s.ht_built = nt;
+    m.ht_kb = s.kb;
+    m.ht_n = s.halo_tiles ? nt : 0;
+    return 0;
+}
+
 int build_device(State &s) {
     if (validate(s)) return -1;
     for (const auto &sc : s.schemes)
@@ -552,6 +597,7 @@ int build_device(State &s) {
         if (dalloc_bytes(s, &s.dT[b], NLn * kb * sb)) return -1;
         HIPCHK(hipMemset(s.dT[b], 0, NLn * kb * sb));
     }
+    if (build_halo_tiles(s)) return -1;
     HIPCHK(hipDeviceSynchronize());
     s.on_device = true;
     s.gen++;
@@ -1109,7 +1155,7 @@ int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D) {
     const hipStream_t keep = s.stream;
     hipEvent_t k0 = s.ev0, k1 = s.ev1;
     const int prec = s.prec, kbr = s.kb_req;
-    const bool graph = s.use_graph, sr = s.stochastic, fused = s.fused;
+    const bool graph = s.use_graph, sr = s.stochastic, fused = s.fused, ht = s.halo_tiles;
     const uint64_t gen = s.gen;
     const LaunchCfg cfg = s.cfg;
     s = State();
@@ -1122,6 +1168,7 @@ int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D) {
     s.use_graph = graph;
     s.stochastic = sr;
     s.fused = fused;
+    s.halo_tiles = ht;
     s.gen = gen + 1;
     s.nl = nl;
     s.N = nod2D;
@@ -1508,6 +1555,20 @@ int tracer_set_fused(int enable) {
             return -1;
         }
     }
+    return 0;
+}
+
+int tracer_set_halo_tiles(int enable) {
+    State &s = S();
+    s.halo_tiles = enable != 0;
+    s.m64.ht_n = s.halo_tiles ? s.ht_built : 0;
+    s.gen++;
+    return 0;
+}
+
+int tracer_get_halo_tiles(int *ntiles) {
+    if (!ntiles) return fail("tracer_get_halo_tiles: null pointer");
+    *ntiles = S().m64.ht_n;
     return 0;
 }
 

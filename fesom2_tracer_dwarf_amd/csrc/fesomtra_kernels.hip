@@ -14,6 +14,8 @@
 //             lib/oce_adv_tra_driver.F90:298-450, src/fesom_analytic.F90:419-437)
 // Without FCT: k_edge, k_vert and k_final.  No atomics: each node adds its incident edges in
 // the reference's edge order, so sums are deterministic and equal the serial reference.
+// For tracer blocks of width <= 4 the three FCT node kernels run on halo tiles (see HaloTile):
+// neighbour values are staged in LDS per level instead of being gathered once per edge.
 #include "fesomtra_column.hpp"
 #include "fesomtra_kernels.hpp"
 
@@ -26,6 +28,12 @@ namespace fesomtra {
 #endif
 template <int KB>
 constexpr int kMinWaves = KB == 64 ? 1 : FESOMTRA_MIN_WAVES;
+// Halo-tiled node kernels hold the staging registers as well; their workgroups also share LDS.
+#ifndef FESOMTRA_MIN_WAVES_TILED
+#define FESOMTRA_MIN_WAVES_TILED 4
+#endif
+template <bool TILED, int KB>
+constexpr int kWaves = TILED ? FESOMTRA_MIN_WAVES_TILED : kMinWaves<KB>;
 
 // ---------------------------------------------------------------------------------------------
 // Work decomposition: one thread owns `lc` consecutive levels of one column for one lane of the
@@ -160,60 +168,139 @@ __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
 
 // ---------------------------------------------------------------------------------------------
 // Node -> edge entries cached in registers (the first kDegCache, the rest read from memory).
+// The far end is a node number (m.ne_other) or, with PACK (halo tiles), the far node's tile
+// slot (m.ht_loc, < 2^15) kept with the entry's deepest level in one register.
+template <bool PACK>
 struct NodeEdges {
     int b, deg;
-    int e2[kDegCache], top[kDegCache], oth[kDegCache];
+    int e2[kDegCache], top[kDegCache], oth[PACK ? 1 : kDegCache];
+    __device__ __forceinline__ int top_of(int i) const { return PACK ? (top[i] & 0xffff) : top[i]; }
+    __device__ __forceinline__ int far_of(int i) const { return PACK ? (top[i] >> 16) : oth[i]; }
 };
-template <typename M>
-__device__ __forceinline__ void load_edges(const MeshCol<M> &m, int n, NodeEdges &ne) {
+template <bool PACK, typename M>
+__device__ __forceinline__ void load_edges(const MeshCol<M> &m, int n, NodeEdges<PACK> &ne) {
     ne.b = cld(m.ne_ptr, n);
     ne.deg = cld(m.ne_ptr, n + 1) - ne.b;
 #pragma unroll
     for (int i = 0; i < kDegCache; ++i) {
-        if (i < ne.deg) {
-            ne.e2[i] = cld(m.ne_e2, ne.b + i);
-            ne.top[i] = cld(m.ne_top, ne.b + i);
-            ne.oth[i] = cld(m.ne_other, ne.b + i);
+        const bool on = i < ne.deg;
+        ne.e2[i] = on ? cld(m.ne_e2, ne.b + i) : 0;
+        if constexpr (PACK) {
+            ne.top[i] = on ? (cld(m.ht_loc, ne.b + i) << 16) | cld(m.ne_top, ne.b + i) : 0;
         } else {
-            ne.e2[i] = 0;
-            ne.top[i] = 0;
-            ne.oth[i] = 0;
+            ne.top[i] = on ? cld(m.ne_top, ne.b + i) : 0;
+            ne.oth[i] = on ? cld(m.ne_other, ne.b + i) : 0;
         }
     }
 }
-// Calls f(edge, second, other) for the edges of n active at level nz, in reference order.
-template <typename M, typename F>
-__device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges &ne, int nz, F &&f) {
+template <bool PACK>
+__device__ __forceinline__ void no_edges(NodeEdges<PACK> &ne) {
+    ne.b = ne.deg = 0;
+#pragma unroll
+    for (int i = 0; i < kDegCache; ++i) ne.e2[i] = ne.top[i] = 0;
+#pragma unroll
+    for (int i = 0; i < (PACK ? 1 : kDegCache); ++i) ne.oth[i] = 0;
+}
+// Calls f(edge, second, far) for the edges of n active at level nz, in reference order.
+template <bool PACK, typename M, typename F>
+__device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges<PACK> &ne, int nz, F &&f) {
 #pragma unroll
     for (int i = 0; i < kDegCache; ++i)
-        if (i < ne.deg && nz <= ne.top[i]) f(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.oth[i]);
+        if (i < ne.deg && nz <= ne.top_of(i)) f(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
     for (int i = kDegCache; i < ne.deg; ++i) {
         const int e2 = cld(m.ne_e2, ne.b + i);
-        if (nz <= cld(m.ne_top, ne.b + i)) f(e2 >> 1, (e2 & 1) != 0, cld(m.ne_other, ne.b + i));
+        if (nz <= cld(m.ne_top, ne.b + i))
+            f(e2 >> 1, (e2 & 1) != 0, cld(PACK ? m.ht_loc : m.ne_other, ne.b + i));
     }
 }
+
+// ---------------------------------------------------------------------------------------------
+// Halo tiles (node kernels of tracer blocks of width KB <= 4).  A workgroup of 256 threads owns
+// TC = 256 / KB consecutive columns (a compact patch of the space-filling curve) and walks its
+// level chunk top-down.  At every level it stages the neighbour quantity of the tile and its
+// one-ring halo in LDS (double-buffered; the next level is fetched into registers while the
+// current one is computed), so a neighbour value is read from memory once per tile instead of
+// once per incident edge.  Thread (col, lane) stages lane `lane` of tile column col (slot 0)
+// and of halo node col (slot 1), so a halo holds at most TC nodes; LDS slot of (local node,
+// lane) is local * KB + lane.  Arithmetic and summation order are those of the untiled kernels.
+constexpr int kHtSlots = 512;
+
+template <int KB, typename M>
+struct HaloTile {
+    static constexpr int TC = 256 / KB;
+    int n0, nz_b, nz_e;  // first column; level range of this chunk (1-based, tile's wet layers)
+    int lmax;
+    int snode[2];
+    bool son[2];
+    // false for the whole workgroup when it has nothing to do
+    __device__ __forceinline__ bool init(const MeshCol<M> &m, int lc) {
+        const int t = xcd_block();
+        if (t >= m.ht_n) return false;
+        n0 = t * TC;
+        lmax = cld(m.ht_lmax, t);
+        nz_b = (int)blockIdx.y * lc + 1;
+        if (nz_b > lmax) return false;
+        nz_e = min(nz_b + lc - 1, lmax);
+        const int hb = cld(m.ht_hptr, t), H = cld(m.ht_hptr, t + 1) - hb;
+        const int col = (int)threadIdx.x / KB;
+        snode[0] = min(n0 + col, m.N - 1);  // columns past the last node: in bounds, never read
+        son[0] = true;
+        son[1] = col < H;
+        snode[1] = son[1] ? cld(m.ht_halo, hb + col) : 0;
+        return true;
+    }
+    // f(i, node) for the staging slots of this thread
+    template <typename F>
+    __device__ __forceinline__ void each(F &&f) const {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            if (son[i]) f(i, snode[i]);
+    }
+};
 
 // Low-order solution and vertical antidiffusive flux (lib/oce_adv_tra_driver.F90:102-216 and
 // the high-order vertical call :266-282 with o_init_zero=.false., pwvel = w).  Interface fluxes
 // are evaluated top-down in double; layer nz uses the fluxes at its top (nz) and bottom (nz + 1).
-// The upwind edge fluxes are formed here in double from q and the two end values (never stored).
-template <int VER, int KB, typename S, typename M, typename C>
-__global__ __launch_bounds__(256, kMinWaves<KB>) void k_lo(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt, int lc) {
+// The upwind edge fluxes are formed here in double from q and the two end values (never stored);
+// TILED stages the neighbours' values per level in LDS.
+template <bool TILED, int VER, int KB, typename S, typename M, typename C>
+__global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt, int lc) {
     using D = double;
-    int n, lane;
-    column_lane<KB>(n, lane);
-    if (n >= m.N || !lane_on(p.mask, lane)) return;
-    const int nzmax = cld(m.nlev_n, n), L = nzmax - 1;
-    int nz0, nz1;
-    if (!chunk_layers(L, lc, nz0, nz1)) return;
+    __shared__ D sT[TILED ? 2 : 1][TILED ? kHtSlots : 1];
+    HaloTile<KB, M> ht;
+    int n, lane, nzmax, L, nz0, nz1, lv0, lv1;
+    bool act = true;
+    if constexpr (TILED) {
+        if (!ht.init(m, lc)) return;
+        n = ht.n0 + (int)threadIdx.x / KB;
+        lane = (int)threadIdx.x % KB;
+        act = n < m.N && lane_on(p.mask, lane);
+        nzmax = act ? cld(m.nlev_n, n) : 1;
+        L = nzmax - 1;
+        nz0 = ht.nz_b;
+        nz1 = min(ht.nz_e, L);
+        act = act && nz0 <= nz1;
+        lv0 = ht.nz_b;
+        lv1 = ht.nz_e;
+    } else {
+        column_lane<KB>(n, lane);
+        if (n >= m.N || !lane_on(p.mask, lane)) return;
+        nzmax = cld(m.nlev_n, n);
+        L = nzmax - 1;
+        if (!chunk_layers(L, lc, nz0, nz1)) return;
+        lv0 = nz0;
+        lv1 = nz1;
+    }
     const size_t Np = m.Np, Ep = m.Ep;
-    NodeEdges ne;
-    load_edges(m, n, ne);
-    const NodeCol<D, S, M, KB> nc{Np, n, lane, L};
-    const Col<D, S, M, KB> col = make_col<D, S, M, KB>(p.TAB, m.w, m, n, lane, nzmax);
+    NodeEdges<TILED> ne;
+    if (act) load_edges(m, n, ne);
+    else no_edges(ne);
+    const int nc_n = act ? n : 0;
+    const NodeCol<D, S, M, KB> nc{Np, nc_n, lane, L};
+    const Col<D, S, M, KB> col = make_col<D, S, M, KB>(p.TAB, m.w, m, nc_n, lane, nzmax);
     const D pv = (D)num_ord;
-    D tw[4], z[4], lo_top, adf_top, tcur;
-    {
+    D tw[4], z[4], lo_top = 0, adf_top = 0, tcur = 0;
+    if (act) {
         const int j = nz0;  // top interface of the chunk
 #pragma unroll
         for (int i = 0; i < 4; ++i) z[i] = (VER == VER_QR4C) ? nc.mlay(m.Z, j - 2 + i) : D(0);
@@ -225,7 +312,18 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_lo(const MeshCol<M> m, c
         lo_top = ver_upw1_w<D>(j, nzmax, nc.lay(p.T, j - 1), tcur, wej, aj, D(0));
         adf_top = ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, lo_top);
     }
-    for (int nz = nz0; nz <= nz1; ++nz) {
+    D st[2];
+    if constexpr (TILED)
+        ht.each([&](int i, int node) { st[i] = ldc<D>(p.T, ((size_t)(lv0 - 1) * Np + node) * KB + lane); });
+    for (int nz = lv0; nz <= lv1; ++nz) {
+        const int buf = nz & 1;
+        if constexpr (TILED) {
+            ht.each([&](int i, int) { sT[buf][threadIdx.x + 256 * i] = st[i]; });
+            __syncthreads();
+            if (nz < lv1)
+                ht.each([&](int i, int node) { st[i] = ldc<D>(p.T, ((size_t)nz * Np + node) * KB + lane); });
+            if (!act || nz > nz1) continue;
+        }
         const int j = nz + 1;  // bottom interface of layer nz
         if (VER == VER_QR4C) {
             z[0] = z[1];
@@ -251,7 +349,9 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_lo(const MeshCol<M> m, c
         double s = 0.0;
         const size_t eo = (size_t)(nz - 1) * Ep, no = (size_t)(nz - 1) * Np;
         for_edges(m, ne, nz, [&](int e, bool second, int oth) {
-            const double to = ldc<double>(p.T, (no + oth) * KB + lane);
+            double to;
+            if constexpr (TILED) to = sT[buf][oth * KB + lane];
+            else to = ldc<double>(p.T, (no + oth) * KB + lane);
             const double f = upw1_edge<double>(second ? to : tcur, second ? tcur : to, cld(m.q, eo + e));
             s = second ? s - f : s + f;
         });
@@ -312,61 +412,80 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_vert(const MeshCol<M> m,
 // level nz is the set of nodes of the elements around n that are wet at nz; with complete
 // nod_in_elem2D that is n plus every neighbour m whose edge (n, m) reaches level nz, and an
 // element dry at nz contributes -1e3 / +1e3 exactly when nz > nlevels_nod2D_min(n) - 1.
-template <int KB, typename S, typename M, typename C>
-__global__ __launch_bounds__(256, kMinWaves<KB>) void k_bounds(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
-    int n, lane;
-    column_lane<KB>(n, lane);
-    if (n >= m.N || !lane_on(p.mask, lane)) return;
-    const int nzmax = cld(m.nlev_n, n), L = nzmax - 1;
-    int nz0, nz1;
-    if (!chunk_layers(L, lc, nz0, nz1)) return;
-    const size_t Np = m.Np, Ep = m.Ep;
-    const int dry_from = cld(m.nlev_min, n);  // levels >= this have a dry element around n
-    NodeEdges ne;
-    load_edges(m, n, ne);
+// The column is swept over cluster levels j: after the cluster of level j is formed, layer j - 1
+// (which needs the clusters j - 2, j - 1, j) is finished; the surface and bottom layers use
+// their own cluster only (the bottom layer L is finished at j = L + 1, where no cluster is
+// formed).  TILED stages max(LO, T) and min(LO, T) of the tile and its halo.
+template <bool TILED, int KB, typename S, typename M, typename C>
+__global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
     using D = double;
+    __shared__ D shi[TILED ? 2 : 1][TILED ? kHtSlots : 1], slo[TILED ? 2 : 1][TILED ? kHtSlots : 1];
+    HaloTile<KB, M> ht;
+    int n, lane, L, nz0, nz1, jb, je, js;  // sweep j = jb..je; levels up to js are staged
+    bool act = true;
+    if constexpr (TILED) {
+        if (!ht.init(m, lc)) return;
+        n = ht.n0 + (int)threadIdx.x / KB;
+        lane = (int)threadIdx.x % KB;
+        act = n < m.N && lane_on(p.mask, lane);
+        L = act ? cld(m.nlev_n, n) - 1 : 0;
+        nz0 = ht.nz_b;
+        nz1 = min(ht.nz_e, L);
+        act = act && nz0 <= nz1;
+        jb = max(1, ht.nz_b - 1);
+        je = ht.nz_e + 1;
+        js = min(ht.nz_e + 1, ht.lmax);
+    } else {
+        column_lane<KB>(n, lane);
+        if (n >= m.N || !lane_on(p.mask, lane)) return;
+        L = cld(m.nlev_n, n) - 1;
+        if (!chunk_layers(L, lc, nz0, nz1)) return;
+        jb = max(1, nz0 - 1);
+        je = nz1 + 1;
+        js = 0;
+    }
+    const size_t Np = m.Np, Ep = m.Ep;
+    const int dry_from = act ? cld(m.nlev_min, n) : 0;  // levels >= this have a dry element around n
+    NodeEdges<TILED> ne;
+    if (act) load_edges(m, n, ne);
+    else no_edges(ne);
     const LaneLd<KB, D, S> ldT{p.T, lane};
     // LO of a cell with tracer value t (stored as LO in FP64, as LO - T in single precision)
     auto lo_at = [&](size_t cell, D t) -> D {
         if constexpr (std::is_same<C, double>::value) return p.LO[cell * KB + lane];
         else return t + (D)p.LO[cell * KB + lane];
     };
-    auto cluster = [&](int nz, D &vmax, D &vmin) {
-        const size_t o = (size_t)(nz - 1) * Np;
-        const D t0 = ldT(o + n), l0 = lo_at(o + n, t0);
-        vmax = cmax(l0, t0);
-        vmin = cmin(l0, t0);
-        for_edges(m, ne, nz, [&](int, bool, int oth) {
-            const D tt = ldT(o + oth), lo = lo_at(o + oth, tt);
-            vmax = cmax(vmax, cmax(lo, tt));
-            vmin = cmin(vmin, cmin(lo, tt));
-        });
+    auto cluster = [&](int nz, int buf, D &vmax, D &vmin) {
+        if constexpr (TILED) {
+            vmax = shi[buf][threadIdx.x];
+            vmin = slo[buf][threadIdx.x];
+            for_edges(m, ne, nz, [&](int, bool, int oth) {
+                vmax = cmax(vmax, shi[buf][oth * KB + lane]);
+                vmin = cmin(vmin, slo[buf][oth * KB + lane]);
+            });
+        } else {
+            const size_t o = (size_t)(nz - 1) * Np;
+            const D t0 = ldT(o + n), l0 = lo_at(o + n, t0);
+            vmax = cmax(l0, t0);
+            vmin = cmin(l0, t0);
+            for_edges(m, ne, nz, [&](int, bool, int oth) {
+                const D tt = ldT(o + oth), lo = lo_at(o + oth, tt);
+                vmax = cmax(vmax, cmax(lo, tt));
+                vmin = cmin(vmin, cmin(lo, tt));
+            });
+        }
         if (nz >= dry_from) {
             vmax = cmax(vmax, D(-1.0e3));
             vmin = cmin(vmin, D(1.0e3));
         }
     };
-    D mx[3], mn[3];
-    C av0;
-    if (nz0 >= 2 && nz0 < L) cluster(nz0 - 1, mx[0], mn[0]);
-    else mx[0] = mn[0] = 0;
-    cluster(nz0, mx[1], mn[1]);
-    if (nz0 + 1 <= L && nz0 > 1 && nz0 < L) cluster(nz0 + 1, mx[2], mn[2]);
-    else mx[2] = mn[2] = 0;
-    av0 = p.adfv[((size_t)(nz0 - 1) * Np + n) * KB + lane];
-    for (int nz = nz0; nz <= nz1; ++nz) {
+    // P+/P-, Q+/Q- and the ratios of layer nz with cluster bounds (bmax, bmin)
+    auto finish = [&](int nz, D bmax, D bmin) {
         const size_t cell = (size_t)(nz - 1) * Np + n;
         const double V = (double)cld(m.areasvol, cell);
         const D lo = lo_at(cell, ldT(cell));
-        D qmax, qmin;
-        if (nz == L || nz == 1) {
-            qmax = mx[1] - lo;
-            qmin = mn[1] - lo;
-        } else {
-            qmax = cmax(cmax(mx[0], mx[1]), mx[2]) - lo;
-            qmin = cmin(cmin(mn[0], mn[1]), mn[2]) - lo;
-        }
-        const C av1 = p.adfv[(cell + Np) * KB + lane];
+        const D qmax = bmax - lo, qmin = bmin - lo;
+        const C av0 = p.adfv[cell * KB + lane], av1 = p.adfv[(cell + Np) * KB + lane];
         double pp = 0.0 + ((double)cmax(C(0), av0) + (double)cmax(C(0), -av1));
         double pm = 0.0 + ((double)cmin(C(0), av0) + (double)cmin(C(0), -av1));
         const size_t eo = (size_t)(nz - 1) * Ep;
@@ -381,20 +500,40 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_bounds(const MeshCol<M> 
         const double rp = (double)qmax / fp, rm = (double)qmin / fm;
         p.Rp[cell * KB + lane] = (C)(rp < 1.0 ? rp : 1.0);
         p.Rm[cell * KB + lane] = (C)(rm < 1.0 ? rm : 1.0);
-        av0 = av1;
-        if (nz < nz1) {  // slide the cluster window to the next layer of this chunk
-            mx[0] = mx[1];
-            mn[0] = mn[1];
-            if (nz + 1 == L) {
-                cluster(nz + 1, mx[1], mn[1]);  // bottom layer uses its own bound
-            } else {
-                mx[1] = mx[2];
-                mn[1] = mn[2];
-                if (nz == 1) {  // the surface layer did not need its lower neighbour
-                    cluster(nz + 1, mx[1], mn[1]);
-                }
-                cluster(nz + 2, mx[2], mn[2]);
+    };
+    D hi_r[2], lo_r[2];
+    auto fetch = [&](int j) {
+        ht.each([&](int i, int node) {
+            const size_t c = (size_t)(j - 1) * Np + node;
+            const D t = ldT(c), l = lo_at(c, t);
+            hi_r[i] = cmax(l, t);
+            lo_r[i] = cmin(l, t);
+        });
+    };
+    if constexpr (TILED) fetch(jb);
+    D mx2 = 0, mn2 = 0, mx1 = 0, mn1 = 0, mx0 = 0, mn0 = 0;  // clusters of levels j-2, j-1, j
+    for (int j = jb; j <= je; ++j) {
+        const int buf = j & 1;
+        if constexpr (TILED) {
+            if (j <= js) {
+                ht.each([&](int i, int) {
+                    shi[buf][threadIdx.x + 256 * i] = hi_r[i];
+                    slo[buf][threadIdx.x + 256 * i] = lo_r[i];
+                });
+                __syncthreads();
+                if (j < js) fetch(j + 1);
             }
+            if (!act) continue;
+        }
+        mx2 = mx1;
+        mn2 = mn1;
+        mx1 = mx0;
+        mn1 = mn0;
+        if (j <= L) cluster(j, buf, mx0, mn0);
+        const int nz = j - 1;
+        if (nz >= nz0 && nz <= nz1) {
+            const bool own = nz == 1 || nz == L;
+            finish(nz, own ? mx1 : cmax(cmax(mx2, mx1), mx0), own ? mn1 : cmin(cmin(mn2, mn1), mn0));
         }
     }
 }
@@ -404,17 +543,36 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_bounds(const MeshCol<M> 
 // acc_v the tendencies are accumulated there (do_oce_adv_tra contract); otherwise values +=
 // (dttf_h + dttf_v) / hnode_new in place.  dbg_h / dbg_v receive the limited fluxes if set.
 // acc_* and dbg_* hold one tracer: they are only passed with a single-lane mask.
-template <bool FCT, int KB, typename S, typename M, typename C>
-__global__ __launch_bounds__(256, kMinWaves<KB>) void k_final(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
-    int n, lane;
-    column_lane<KB>(n, lane);
-    if (n >= m.N || !lane_on(p.mask, lane)) return;
-    const int nzmax = cld(m.nlev_n, n), L = nzmax - 1;
-    int nz0, nz1;
-    if (!chunk_layers(L, lc, nz0, nz1)) return;
+template <bool TILED, bool FCT, int KB, typename S, typename M, typename C>
+__global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
+    static_assert(!TILED || FCT, "tiles stage the limiter ratios");
+    __shared__ C sRp[TILED ? 2 : 1][TILED ? kHtSlots : 1], sRm[TILED ? 2 : 1][TILED ? kHtSlots : 1];
+    HaloTile<KB, M> ht;
+    int n, lane, L, nz0, nz1, lv0, lv1;
+    bool act = true;
+    if constexpr (TILED) {
+        if (!ht.init(m, lc)) return;
+        n = ht.n0 + (int)threadIdx.x / KB;
+        lane = (int)threadIdx.x % KB;
+        act = n < m.N && lane_on(p.mask, lane);
+        L = act ? cld(m.nlev_n, n) - 1 : 0;
+        nz0 = ht.nz_b;
+        nz1 = min(ht.nz_e, L);
+        act = act && nz0 <= nz1;
+        lv0 = ht.nz_b;
+        lv1 = ht.nz_e;
+    } else {
+        column_lane<KB>(n, lane);
+        if (n >= m.N || !lane_on(p.mask, lane)) return;
+        L = cld(m.nlev_n, n) - 1;
+        if (!chunk_layers(L, lc, nz0, nz1)) return;
+        lv0 = nz0;
+        lv1 = nz1;
+    }
     const size_t Np = m.Np, Ep = m.Ep;
-    NodeEdges ne;
-    load_edges(m, n, ne);
+    NodeEdges<TILED> ne;
+    if (act) load_edges(m, n, ne);
+    else no_edges(ne);
     // limited flux at interface j given R at layers j-1 (rp0, rm0) and j (rp1, rm1)
     auto limit = [&](C a, int j, C rp0, C rm0, C rp1, C rm1) -> C {
         C ae = 1;
@@ -429,9 +587,9 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_final(const MeshCol<M> m
         return ae * a;
     };
     auto X = [&](size_t cell) { return cell * KB + lane; };
-    C top, rp0 = 0, rm0 = 0, rp1 = 0, rm1 = 0;
+    C top = 0, rp0 = 0, rm0 = 0, rp1 = 0, rm1 = 0;
     const size_t c0 = (size_t)(nz0 - 1) * Np + n;
-    {
+    if (act) {
         const C a0 = p.adfv[X(c0)];
         if (FCT) {
             rp1 = p.Rp[X(c0)];
@@ -444,7 +602,31 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_final(const MeshCol<M> m
             top = a0;
         }
     }
-    for (int nz = nz0; nz <= nz1; ++nz) {
+    C sp[2], sm[2];
+    if constexpr (TILED) {
+        ht.each([&](int i, int node) {
+            const size_t c = (size_t)(lv0 - 1) * Np + node;
+            sp[i] = p.Rp[X(c)];
+            sm[i] = p.Rm[X(c)];
+        });
+    }
+    for (int nz = lv0; nz <= lv1; ++nz) {
+        const int buf = nz & 1;
+        if constexpr (TILED) {
+            ht.each([&](int i, int) {
+                sRp[buf][threadIdx.x + 256 * i] = sp[i];
+                sRm[buf][threadIdx.x + 256 * i] = sm[i];
+            });
+            __syncthreads();
+            if (nz < lv1) {
+                ht.each([&](int i, int node) {
+                    const size_t c = (size_t)nz * Np + node;
+                    sp[i] = p.Rp[X(c)];
+                    sm[i] = p.Rm[X(c)];
+                });
+            }
+            if (!act || nz > nz1) continue;
+        }
         const size_t cell = (size_t)(nz - 1) * Np + n;
         const double V = (double)cld(m.areasvol, cell);
         const double hn = (double)cld(m.hnode_new, cell);
@@ -477,8 +659,14 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_final(const MeshCol<M> m
             C f = p.adfh[X(eo + e)];
             if (FCT) {
                 // R of both end nodes (lib/oce_adv_tra_fct.F90:455-468)
-                const C rpo = p.Rp[X(no + oth)];
-                const C rmo = p.Rm[X(no + oth)];
+                C rpo, rmo;
+                if constexpr (TILED) {
+                    rpo = sRp[buf][oth * KB + lane];
+                    rmo = sRm[buf][oth * KB + lane];
+                } else {
+                    rpo = p.Rp[X(no + oth)];
+                    rmo = p.Rm[X(no + oth)];
+                }
                 const C rp_1 = second ? rpo : rpn, rm_1 = second ? rmo : rmn;  // edges(1)
                 const C rp_2 = second ? rpn : rpo, rm_2 = second ? rmn : rmo;  // edges(2)
                 C ae = 1;
@@ -616,19 +804,43 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
     LAUNCH_CHECK();
     MARK(1);
     if (sc.fct) {
-        switch (sc.ver) {
-        case VER_UPW1: k_lo<VER_UPW1, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-        case VER_QR4C: k_lo<VER_QR4C, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-        case VER_CDIFF: k_lo<VER_CDIFF, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-        case VER_PPM: k_lo<VER_PPM, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-        default: return hipErrorInvalidValue;
+        // halo-tiled node kernels when the tiles were built for this block width
+        bool tiled = false;
+        if constexpr (KB <= 4) tiled = m.ht_n > 0 && m.ht_kb == KB;
+        const unsigned gt = grid_for((size_t)m.ht_n, 1);
+        auto tgrid = [&](int lc) { return dim3(gt, (unsigned)((L + lc - 1) / lc)); };
+        if (tiled) {
+            if constexpr (KB <= 4) {
+                const dim3 tl = tgrid(cfg.lc_lo);
+                switch (sc.ver) {
+                case VER_UPW1: k_lo<true, VER_UPW1, KB><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+                case VER_QR4C: k_lo<true, VER_QR4C, KB><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+                case VER_CDIFF: k_lo<true, VER_CDIFF, KB><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+                case VER_PPM: k_lo<true, VER_PPM, KB><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+                default: return hipErrorInvalidValue;
+                }
+                LAUNCH_CHECK();
+                MARK(2);
+                k_bounds<true, KB><<<tgrid(cfg.lc_bounds), 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
+                LAUNCH_CHECK();
+                MARK(3);
+                k_final<true, true, KB><<<tgrid(cfg.lc_final), 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+            }
+        } else {
+            switch (sc.ver) {
+            case VER_UPW1: k_lo<false, VER_UPW1, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+            case VER_QR4C: k_lo<false, VER_QR4C, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+            case VER_CDIFF: k_lo<false, VER_CDIFF, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+            case VER_PPM: k_lo<false, VER_PPM, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+            default: return hipErrorInvalidValue;
+            }
+            LAUNCH_CHECK();
+            MARK(2);
+            k_bounds<false, KB><<<gb, 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
+            LAUNCH_CHECK();
+            MARK(3);
+            k_final<false, true, KB><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
         }
-        LAUNCH_CHECK();
-        MARK(2);
-        k_bounds<KB><<<gb, 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
-        LAUNCH_CHECK();
-        MARK(3);
-        k_final<true, KB><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
     } else {
         switch (sc.ver) {
         case VER_UPW1: k_vert<VER_UPW1, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
@@ -640,7 +852,7 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
         LAUNCH_CHECK();
         MARK(2);
         MARK(3);
-        k_final<false, KB><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+        k_final<false, false, KB><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
     }
     LAUNCH_CHECK();
     MARK(4);

@@ -124,6 +124,12 @@ __device__ __forceinline__ C csign1(C x) { return signbit(x) ? C(-1) : C(1); }
 template <typename M>
 struct MeshCol {
     int N, T, E, nl, L, Np, Tp, Ep, max_adj;
+    // Halo tiles of the node kernels for tracer blocks of width ht_kb (<= 4; ht_n = 0: none):
+    // tile t owns the 256 / ht_kb consecutive nodes from t * 256 / ht_kb and lists its one-ring
+    // halo; ht_loc gives, per node -> edge entry, the far node's slot in its tile (tile column,
+    // or 256 / ht_kb + halo position).
+    int ht_n, ht_kb;
+    const int *ht_loc, *ht_hptr, *ht_halo, *ht_lmax;  // (entries), (ht_n + 1), (halo), (ht_n) wet layers
     // nodes
     const int *nlev_n, *nlev_min, *nbl;   // nlevels_nod2D, nlevels_nod2D_min, nboundary_lay
     const int *ne_ptr;                    // CSR node -> edges, reference edge order

@@ -133,6 +133,16 @@ class FESOMTracerAdvection:
         """Single fused kernel per step for FP64 FCT tracers in blocks of one."""
         abi.check(self.lib.tracer_set_fused(int(bool(enable))), "tracer_set_fused")
 
+    def set_halo_tiles(self, enable: bool) -> None:
+        """LDS halo tiles for the FCT node kernels of blocks of width <= 4 (default on)."""
+        abi.check(self.lib.tracer_set_halo_tiles(int(bool(enable))), "tracer_set_halo_tiles")
+
+    def halo_tiles(self) -> int:
+        """Number of halo tiles in use (0: untiled kernels)."""
+        n = ctypes.c_int()
+        abi.check(self.lib.tracer_get_halo_tiles(ctypes.byref(n)), "tracer_get_halo_tiles")
+        return n.value
+
     def set_graph(self, enable: bool) -> None:
         abi.check(self.lib.tracer_set_graph(int(bool(enable))), "tracer_set_graph")
 

@@ -371,3 +371,59 @@ def test_autotune_does_not_change_single_precision_results():
         t.set_precision(0)
         t.cleanup()
     assert np.array_equal(outs[0], outs[1])
+
+
+TILE_SCHEMES = [Scheme("MFCT", "QR4C", "FCT", 0.85, 0.75), Scheme("MUSCL", "PPM", "FCT", 0.9, 0.6),
+                Scheme("UPW1", "CDIFF", "FCT"), Scheme("MFCT", "UPW1", "FCT", 1.0, 1.0)]
+
+
+def _run_tiles(case, nsteps, kb, tiles, prec=0, lc=(16, 5, 7, 3)):
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    t = FESOMTracerAdvection()
+    t.set_block_width(kb)
+    t.set_halo_tiles(tiles)
+    t.set_launch_config(lc)
+    t.set_precision(prec)
+    t.setup_case(case)
+    ntiles = t.halo_tiles()
+    t.advect_steps(nsteps, case.dt)
+    out = t.get_all_values()
+    t.set_precision(0)
+    t.set_block_width(0)
+    t.set_halo_tiles(True)
+    t.cleanup()
+    return out, ntiles
+
+
+@pytest.mark.parametrize("kb", [1, 2, 4])
+@pytest.mark.parametrize("si", range(len(TILE_SCHEMES)))
+def test_halo_tiles_match_untiled_and_oracle(oracle_mod, kb, si):
+    """The halo-tiled FCT node kernels (many tiles with halos, level chunks) give the same bits
+    as the untiled kernels and as the reference restatement."""
+    case = configs.bathymetry_case("bathy48", 48, 45, seed=71, ntracers=3, nsteps=3, lev_lo=3,
+                                   scheme=TILE_SCHEMES[si], w_cfl=0.3)
+    say(f"case bathy48 scheme {si} kb={kb} tiled vs untiled")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[3])
+    tiled, nt = _run_tiles(case, 3, kb, True)
+    plain, n0 = _run_tiles(case, 3, kb, False)
+    assert nt == -(-case.mesh.nod2D // (256 // kb)) and n0 == 0
+    assert np.array_equal(tiled, plain)
+    assert rel_linf(tiled, ref[3]) <= RTOL
+
+
+@pytest.mark.parametrize("prec", [1, 3])
+def test_halo_tiles_single_precision_bitwise(prec):
+    case = configs.bathymetry_case("bathy40p", 40, 37, seed=73, ntracers=2, nsteps=4, lev_lo=3,
+                                   w_cfl=0.3)
+    say(f"case bathy40p precision {prec} tiled vs untiled")
+    tiled, nt = _run_tiles(case, 4, 2, True, prec=prec)
+    plain, _ = _run_tiles(case, 4, 2, False, prec=prec)
+    assert nt > 0
+    assert np.array_equal(tiled, plain)
+
+
+def test_no_halo_tiles_for_wide_blocks():
+    case = configs.bathymetry_case("bathy20w", 20, 18, seed=75, ntracers=8, nsteps=1, lev_lo=3,
+                                   w_cfl=0.3)
+    _, nt = _run_tiles(case, 1, 8, True)
+    assert nt == 0
