@@ -70,14 +70,16 @@ struct State {
     size_t Np = 0, Tp = 0, Ep = 0;
     MeshCol<double> m64{};  // mesh and velocities stay FP64 in every precision mode
     // device-order host tables kept for the tile builder
-    std::vector<int> h_ptr, h_e2, h_top, h_oth, h_nie, h_nie_cnt, h_en, h_nlev;
+    std::vector<int> h_ptr, h_e2, h_top, h_oth, h_nie, h_nie_cnt, h_en, h_nlev, h_nle;
     std::vector<int2> h_ud;
+    std::vector<int4> h_ne;
     // fused single-pass path (FP64, FCT, one tracer per block)
     bool fused = false;
     bool tiles_built = false;
     // halo tiles of the node kernels (tracer blocks of width <= 4)
     bool halo_tiles = true;
-    int ht_built = 0;  // number of tiles built for the current block width (0: none)
+    int ht_built = 0;  // number of node tiles built for the current block width (0: none)
+    int et_built = 0;  // number of edge tiles built for the current block width (0: none)
     TileMesh tm{};
     std::vector<void *> dT2;    // second tracer buffer per block (output of a fused step)
     std::vector<char> t2_sync;  // dT2 holds the same below-bottom values as dT
@@ -178,6 +180,7 @@ void release_device(State &s) {
     s.tiles_built = false;
     s.tm = TileMesh{};
     s.ht_built = 0;
+    s.et_built = 0;
     s.adfh = s.LO = s.Rp = s.Rm = s.adfv = nullptr;
     s.acc_h = s.acc_v = nullptr;
     s.dbg_h = s.dbg_v = nullptr;
@@ -444,6 +447,61 @@ int build_halo_tiles(State &s) {
     return 0;
 }
 
+// Edge tiles of the edge-flux kernel for a tracer block of width kb <= 4: tile t owns the
+// te = 256 / kb consecutive device edges from t * te (edges are ordered by their smaller node,
+// so a tile is a compact patch); its node list holds every node the tile's edges read at the
+// levels where the up- and down-wind triangles are both wet (the two ends and the triangles'
+// corners), at most 2 * te nodes (two staging slots per thread), else the edge kernel runs
+// untiled.  Per edge the eight slots are packed two to a word.
+int build_edge_tiles(State &s) {
+    MeshCol<double> &m = s.m64;
+    m.et_n = m.et_kb = 0;
+    s.et_built = 0;
+    if (s.kb > 4 || s.E < 1) return 0;
+    const int te = 256 / s.kb, E = s.E, nt = (E + te - 1) / te;
+    std::vector<int> nptr(nt + 1, 0), nodes, loc((size_t)4 * E, 0), lmax(nt, 0), nl;
+    for (int t = 0; t < nt; ++t) {
+        const int a0 = t * te, a1 = std::min(E, a0 + te);
+        nl.clear();
+        for (int e = a0; e < a1; ++e) {
+            const int4 q = s.h_ne[e];
+            lmax[t] = std::max(lmax[t], std::max(s.h_nle[q.z] - 1, q.w >= 0 ? s.h_nle[q.w] - 1 : 0));
+            nl.push_back(q.x);
+            nl.push_back(q.y);
+            const int2 ud = s.h_ud[e];
+            if (ud.x >= 0 && ud.y >= 0)
+                for (int c = 0; c < 3; ++c) {
+                    nl.push_back(s.h_en[3 * (size_t)ud.x + c]);
+                    nl.push_back(s.h_en[3 * (size_t)ud.y + c]);
+                }
+        }
+        std::sort(nl.begin(), nl.end());
+        nl.erase(std::unique(nl.begin(), nl.end()), nl.end());
+        if ((int)nl.size() > 2 * te) return 0;
+        auto slot = [&](int node) { return (int)(std::lower_bound(nl.begin(), nl.end(), node) - nl.begin()); };
+        for (int e = a0; e < a1; ++e) {
+            const int4 q = s.h_ne[e];
+            const int2 ud = s.h_ud[e];
+            int sl[8] = {slot(q.x), slot(q.y), 0, 0, 0, 0, 0, 0};
+            if (ud.x >= 0 && ud.y >= 0)
+                for (int c = 0; c < 3; ++c) {
+                    sl[2 + c] = slot(s.h_en[3 * (size_t)ud.x + c]);
+                    sl[5 + c] = slot(s.h_en[3 * (size_t)ud.y + c]);
+                }
+            for (int i = 0; i < 4; ++i) loc[4 * (size_t)e + i] = sl[2 * i] | (sl[2 * i + 1] << 16);
+        }
+        nodes.insert(nodes.end(), nl.begin(), nl.end());
+        nptr[t + 1] = (int)nodes.size();
+    }
+    if (upload(s, &m.et_loc, loc) || upload(s, &m.et_nptr, nptr) || upload(s, &m.et_nodes, nodes) ||
+        upload(s, &m.et_lmax, lmax))
+        return -1;
+    s.et_built = nt;
+    m.et_kb = s.kb;
+    m.et_n = s.halo_tiles ? nt : 0;
+    return 0;
+}
+
 int build_device(State &s) {
     if (validate(s)) return -1;
     for (const auto &sc : s.schemes)
@@ -520,6 +578,7 @@ int build_device(State &s) {
         }
         if (upload(s, &m.enodes, en) || upload(s, &m.nlev_e, nle)) return -1;
         s.h_en = en;
+        s.h_nle = nle;
     }
     std::vector<int> top_of(E);
     {
@@ -540,6 +599,7 @@ int build_device(State &s) {
         }
         if (upload(s, &m.edge_ne, ne) || upload(s, &m.updn, ud)) return -1;
         s.h_ud = ud;
+        s.h_ne = ne;
     }
     // CSR node -> incident edges in ascending reference edge number (the serial scatter order)
     {
@@ -597,7 +657,7 @@ int build_device(State &s) {
         if (dalloc_bytes(s, &s.dT[b], NLn * kb * sb)) return -1;
         HIPCHK(hipMemset(s.dT[b], 0, NLn * kb * sb));
     }
-    if (build_halo_tiles(s)) return -1;
+    if (build_halo_tiles(s) || build_edge_tiles(s)) return -1;
     HIPCHK(hipDeviceSynchronize());
     s.on_device = true;
     s.gen++;
@@ -1562,6 +1622,7 @@ int tracer_set_halo_tiles(int enable) {
     State &s = S();
     s.halo_tiles = enable != 0;
     s.m64.ht_n = s.halo_tiles ? s.ht_built : 0;
+    s.m64.et_n = s.halo_tiles ? s.et_built : 0;
     s.gen++;
     return 0;
 }

@@ -49,17 +49,124 @@ __device__ __forceinline__ bool chunk_layers(int L, int lc, int &nz0, int &nz1) 
     return true;
 }
 
-template <int HOR, bool FCT, int KB, typename S, typename M, typename C>
-__global__ __launch_bounds__(256, kMinWaves<KB>) void k_edge(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, int lc) {
-    int e, lane;
-    column_lane<KB>(e, lane);
-    if (e >= m.E || !lane_on(p.mask, lane)) return;
+// ---------------------------------------------------------------------------------------------
+// Halo tiles (node kernels of tracer blocks of width KB <= 4).  A workgroup of 256 threads owns
+// TC = 256 / KB consecutive columns (a compact patch of the space-filling curve) and walks its
+// level chunk top-down.  At every level it stages the neighbour quantity of the tile and its
+// one-ring halo in LDS (double-buffered; the next level is fetched into registers while the
+// current one is computed), so a neighbour value is read from memory once per tile instead of
+// once per incident edge.  Thread (col, lane) stages lane `lane` of tile column col (slot 0)
+// and of halo node col (slot 1), so a halo holds at most TC nodes; LDS slot of (local node,
+// lane) is local * KB + lane.  Arithmetic and summation order are those of the untiled kernels.
+constexpr int kHtSlots = 512;
+
+template <int KB, typename M>
+struct HaloTile {
+    static constexpr int TC = 256 / KB;
+    int n0, nz_b, nz_e;  // first column; level range of this chunk (1-based, tile's wet layers)
+    int lmax;
+    int snode[2];
+    bool son[2];
+    // false for the whole workgroup when it has nothing to do
+    __device__ __forceinline__ bool init(const MeshCol<M> &m, int lc) {
+        const int t = xcd_block();
+        if (t >= m.ht_n) return false;
+        n0 = t * TC;
+        lmax = cld(m.ht_lmax, t);
+        nz_b = (int)blockIdx.y * lc + 1;
+        if (nz_b > lmax) return false;
+        nz_e = min(nz_b + lc - 1, lmax);
+        const int hb = cld(m.ht_hptr, t), H = cld(m.ht_hptr, t + 1) - hb;
+        const int col = (int)threadIdx.x / KB;
+        snode[0] = min(n0 + col, m.N - 1);  // columns past the last node: in bounds, never read
+        son[0] = true;
+        son[1] = col < H;
+        snode[1] = son[1] ? cld(m.ht_halo, hb + col) : 0;
+        return true;
+    }
+    // f(i, node) for the staging slots of this thread
+    template <typename F>
+    __device__ __forceinline__ void each(F &&f) const {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            if (son[i]) f(i, snode[i]);
+    }
+};
+
+// Edge tiles (edge-flux kernel of tracer blocks of width KB <= 4): a workgroup owns TE = 256 / KB
+// consecutive edges and, per level, stages the tracer values of every node its edges read (both
+// ends, and the corners of the up- and down-wind triangles) in LDS; thread (edge, lane) stages
+// lane `lane` of tile nodes edge and TE + edge.  The node-mean gradients of the few levels below
+// the shared wet depth of an edge's triangles still read memory.
+template <int KB, typename M>
+struct EdgeTile {
+    static constexpr int TE = 256 / KB;
+    int e0, nz_b, nz_e;
+    int snode[2];
+    bool son[2];
+    __device__ __forceinline__ bool init(const MeshCol<M> &m, int lc) {
+        const int t = xcd_block();
+        if (t >= m.et_n) return false;
+        e0 = t * TE;
+        const int lmax = cld(m.et_lmax, t);
+        nz_b = (int)blockIdx.y * lc + 1;
+        if (nz_b > lmax) return false;
+        nz_e = min(nz_b + lc - 1, lmax);
+        const int nb = cld(m.et_nptr, t), nn = cld(m.et_nptr, t + 1) - nb;
+        const int col = (int)threadIdx.x / KB;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            son[i] = col + i * TE < nn;
+            snode[i] = son[i] ? cld(m.et_nodes, nb + col + i * TE) : 0;
+        }
+        return true;
+    }
+    template <typename F>
+    __device__ __forceinline__ void each(F &&f) const {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            if (son[i]) f(i, snode[i]);
+    }
+};
+// Reads lane `lane` of staged tile slot `slot`: ld(slot) = buf[slot * KB + lane].
+template <int KB, typename C>
+struct SlotLd {
+    const C *buf;
+    int lane;
+    __device__ __forceinline__ C operator()(size_t slot) const { return buf[slot * KB + lane]; }
+};
+
+template <bool TILED, int HOR, bool FCT, int KB, typename S, typename M, typename C>
+__global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, int lc) {
+    __shared__ C sT[TILED ? 2 : 1][TILED ? kHtSlots : 1], sA[TILED ? 2 : 1][TILED ? kHtSlots : 1];
+    EdgeTile<KB, M> et;
+    int e, lane, nz0, nz1, lv0, lv1;
+    bool act = true;
+    if constexpr (TILED) {
+        if (!et.init(m, lc)) return;
+        e = et.e0 + (int)threadIdx.x / KB;
+        lane = (int)threadIdx.x % KB;
+        act = e < m.E && lane_on(p.mask, lane);
+        if (!act) e = 0;  // in-bounds setup loads; nothing is computed or stored
+    } else {
+        column_lane<KB>(e, lane);
+        if (e >= m.E || !lane_on(p.mask, lane)) return;
+    }
     const int4 q = cld(m.edge_ne, e);
     const int n1 = q.x, n2 = q.y, el1 = q.z, el2 = q.w;
     const int nl1 = cld(m.nlev_e, el1) - 1;
     const int nl2 = el2 >= 0 ? cld(m.nlev_e, el2) - 1 : 0;
-    int nz0, nz1;
-    if (!chunk_layers(max(nl1, nl2), lc, nz0, nz1)) return;
+    if constexpr (TILED) {
+        nz0 = et.nz_b;
+        nz1 = min(et.nz_e, max(nl1, nl2));
+        act = act && nz0 <= nz1;
+        lv0 = et.nz_b;
+        lv1 = et.nz_e;
+    } else {
+        if (!chunk_layers(max(nl1, nl2), lc, nz0, nz1)) return;
+        lv0 = nz0;
+        lv1 = nz1;
+    }
     const size_t Np = m.Np, Ep = m.Ep;
     const LaneLd<KB, C, S> ldT{p.T, lane}, ldA{p.TAB, lane};
     const bool same = p.TAB == p.T;
@@ -90,25 +197,59 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_edge(const MeshCol<M> m,
             nb2 = cld(m.nbl, n2);
         }
     }
-    for (int nz = nz0; nz <= nz1; ++nz) {
+    // tile slots of n1, n2, u0..u2, d0..d2 (two 16-bit slots per word)
+    int sl[4] = {0, 0, 0, 0};
+    if constexpr (TILED) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sl[i] = cld(m.et_loc, 4 * (size_t)e + i);
+    }
+    auto slot = [&](int k) { return (sl[k >> 1] >> (16 * (k & 1))) & 0xffff; };
+    C stT[2], stA[2];
+    auto fetch = [&](int nz) {
+        const size_t o = (size_t)(nz - 1) * Np;
+        et.each([&](int i, int node) {
+            stT[i] = ldT(o + node);
+            if (!same) stA[i] = ldA(o + node);
+        });
+    };
+    if constexpr (TILED) fetch(lv0);
+    for (int nz = lv0; nz <= lv1; ++nz) {
+        const int buf = nz & 1;
+        if constexpr (TILED) {
+            et.each([&](int i, int) {
+                sT[buf][threadIdx.x + 256 * i] = stT[i];
+                if (!same) sA[buf][threadIdx.x + 256 * i] = stA[i];
+            });
+            __syncthreads();
+            if (nz < lv1) fetch(nz + 1);
+            if (!act || nz > nz1) continue;
+        }
         const int k = nz - 1;
         const C v = (C)cld(m.q, (size_t)k * Ep + e);
         const size_t o = (size_t)k * Np;
         const size_t ie = ((size_t)k * Ep + e) * KB + lane;
+        const SlotLd<KB, C> lsT{sT[TILED ? buf : 0], lane}, lsA{same ? sT[TILED ? buf : 0] : sA[TILED ? buf : 0], lane};
         C t1 = 0, t2 = 0;
         if (FCT || same) {
-            t1 = ldT(o + n1);
-            t2 = ldT(o + n2);
+            t1 = TILED ? lsT(slot(0)) : ldT(o + n1);
+            t2 = TILED ? lsT(slot(1)) : ldT(o + n2);
         }
-        const C a1 = same ? t1 : ldA(o + n1);
-        const C a2 = same ? t2 : ldA(o + n2);
+        const C a1 = same ? t1 : (TILED ? lsA(slot(0)) : ldA(o + n1));
+        const C a2 = same ? t2 : (TILED ? lsA(slot(1)) : ldA(o + n2));
         C gr[4] = {0, 0, 0, 0}, c1 = 1, c2 = 1;
         if (HOR != HOR_UPW1) {
             if (nz <= shared_end) {
-                gr[0] = tr_xy<C>(gu, ldA, o, u[0], u[1], u[2]);
-                gr[1] = tr_xy<C>(gd, ldA, o, d[0], d[1], d[2]);
-                gr[2] = tr_xy<C>(gu + 3, ldA, o, u[0], u[1], u[2]);
-                gr[3] = tr_xy<C>(gd + 3, ldA, o, d[0], d[1], d[2]);
+                if constexpr (TILED) {
+                    gr[0] = tr_xy<C>(gu, lsA, 0, slot(2), slot(3), slot(4));
+                    gr[1] = tr_xy<C>(gd, lsA, 0, slot(5), slot(6), slot(7));
+                    gr[2] = tr_xy<C>(gu + 3, lsA, 0, slot(2), slot(3), slot(4));
+                    gr[3] = tr_xy<C>(gd + 3, lsA, 0, slot(5), slot(6), slot(7));
+                } else {
+                    gr[0] = tr_xy<C>(gu, ldA, o, u[0], u[1], u[2]);
+                    gr[1] = tr_xy<C>(gd, ldA, o, d[0], d[1], d[2]);
+                    gr[2] = tr_xy<C>(gu + 3, ldA, o, u[0], u[1], u[2]);
+                    gr[3] = tr_xy<C>(gd + 3, ldA, o, d[0], d[1], d[2]);
+                }
             } else {
                 node_mean_grad<C>(m, ldA, n1, nz, gr[0], gr[2]);
                 node_mean_grad<C>(m, ldA, n2, nz, gr[1], gr[3]);
@@ -213,50 +354,6 @@ __device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges<P
             f(e2 >> 1, (e2 & 1) != 0, cld(PACK ? m.ht_loc : m.ne_other, ne.b + i));
     }
 }
-
-// ---------------------------------------------------------------------------------------------
-// Halo tiles (node kernels of tracer blocks of width KB <= 4).  A workgroup of 256 threads owns
-// TC = 256 / KB consecutive columns (a compact patch of the space-filling curve) and walks its
-// level chunk top-down.  At every level it stages the neighbour quantity of the tile and its
-// one-ring halo in LDS (double-buffered; the next level is fetched into registers while the
-// current one is computed), so a neighbour value is read from memory once per tile instead of
-// once per incident edge.  Thread (col, lane) stages lane `lane` of tile column col (slot 0)
-// and of halo node col (slot 1), so a halo holds at most TC nodes; LDS slot of (local node,
-// lane) is local * KB + lane.  Arithmetic and summation order are those of the untiled kernels.
-constexpr int kHtSlots = 512;
-
-template <int KB, typename M>
-struct HaloTile {
-    static constexpr int TC = 256 / KB;
-    int n0, nz_b, nz_e;  // first column; level range of this chunk (1-based, tile's wet layers)
-    int lmax;
-    int snode[2];
-    bool son[2];
-    // false for the whole workgroup when it has nothing to do
-    __device__ __forceinline__ bool init(const MeshCol<M> &m, int lc) {
-        const int t = xcd_block();
-        if (t >= m.ht_n) return false;
-        n0 = t * TC;
-        lmax = cld(m.ht_lmax, t);
-        nz_b = (int)blockIdx.y * lc + 1;
-        if (nz_b > lmax) return false;
-        nz_e = min(nz_b + lc - 1, lmax);
-        const int hb = cld(m.ht_hptr, t), H = cld(m.ht_hptr, t + 1) - hb;
-        const int col = (int)threadIdx.x / KB;
-        snode[0] = min(n0 + col, m.N - 1);  // columns past the last node: in bounds, never read
-        son[0] = true;
-        son[1] = col < H;
-        snode[1] = son[1] ? cld(m.ht_halo, hb + col) : 0;
-        return true;
-    }
-    // f(i, node) for the staging slots of this thread
-    template <typename F>
-    __device__ __forceinline__ void each(F &&f) const {
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-            if (son[i]) f(i, snode[i]);
-    }
-};
 
 // Low-order solution and vertical antidiffusive flux (lib/oce_adv_tra_driver.F90:102-216 and
 // the high-order vertical call :266-282 with o_init_zero=.false., pwvel = w).  Interface fluxes
@@ -778,7 +875,14 @@ static inline unsigned grid_for(size_t n_threads, int block) {
 
 template <int HOR, bool FCT, int KB, typename S, typename M, typename C>
 static void edge_kernel(dim3 g, hipStream_t s, const MeshCol<M> &m, const StepPtrs<S, C> &p, C ph, int lc) {
-    k_edge<HOR, FCT, KB, S, M, C><<<g, 256, 0, s>>>(m, p, ph, lc);
+    if constexpr (KB <= 4) {
+        if (m.et_n > 0 && m.et_kb == KB) {  // edge tiles built for this block width
+            const dim3 gt(grid_for((size_t)m.et_n, 1), g.y);
+            k_edge<true, HOR, FCT, KB, S, M, C><<<gt, 256, 0, s>>>(m, p, ph, lc);
+            return;
+        }
+    }
+    k_edge<false, HOR, FCT, KB, S, M, C><<<g, 256, 0, s>>>(m, p, ph, lc);
 }
 
 template <int KB, typename S, typename M, typename C>

@@ -130,6 +130,11 @@ struct MeshCol {
     // or 256 / ht_kb + halo position).
     int ht_n, ht_kb;
     const int *ht_loc, *ht_hptr, *ht_halo, *ht_lmax;  // (entries), (ht_n + 1), (halo), (ht_n) wet layers
+    // Edge tiles of the edge-flux kernel (same widths): tile t owns the 256 / et_kb consecutive
+    // edges from t * 256 / et_kb and lists the nodes they read (ends and up/down-wind triangle
+    // corners); et_loc holds per edge the slots of n1, n2, u0, u1, u2, d0, d1, d2 (16 bits each).
+    int et_n, et_kb;
+    const int *et_loc, *et_nptr, *et_nodes, *et_lmax;  // (E, 4), (et_n + 1), (nodes), (et_n) layers
     // nodes
     const int *nlev_n, *nlev_min, *nbl;   // nlevels_nod2D, nlevels_nod2D_min, nboundary_lay
     const int *ne_ptr;                    // CSR node -> edges, reference edge order

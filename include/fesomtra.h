@@ -188,12 +188,13 @@ int tracer_set_graph(int enable);
  * Returns -1 if the mesh cannot be tiled (a node with more than 8 edges); other tracers keep the
  * multi-kernel path. */
 int tracer_set_fused(int enable);
-/* enable = 1 (default): with the FCT limiter and tracer blocks of width 1, 2 or 4, the low-order,
- * bounds and limiting kernels run on halo tiles: a workgroup owns 256 / width consecutive nodes
- * (a compact patch of the renumbered mesh) and stages the neighbour values of the tile and its
- * one-ring halo in LDS, level by level, instead of gathering them once per edge.  Results are
- * bit-identical either way; meshes whose tiles have halos larger than the tile use the untiled
- * kernels.  tracer_get_halo_tiles returns the number of tiles in use (0: untiled). */
+/* enable = 1 (default): for tracer blocks of width 1, 2 or 4 the kernels run on tiles that stage
+ * neighbour values in LDS, level by level, instead of gathering them once per edge: the edge-flux
+ * kernel on tiles of 256 / width consecutive edges (with the nodes they read), and with the FCT
+ * limiter the low-order, bounds and limiting kernels on tiles of 256 / width consecutive nodes
+ * (a compact patch of the renumbered mesh) with their one-ring halo.  Results are bit-identical
+ * either way; meshes whose tiles need more than the staging space use the untiled kernels.
+ * tracer_get_halo_tiles returns the number of node tiles in use (0: untiled). */
 int tracer_set_halo_tiles(int enable);
 int tracer_get_halo_tiles(int *ntiles);
 /* Levels per thread of the four kernels (edge fluxes, low-order / vertical, bounds, final): each

@@ -374,7 +374,8 @@ def test_autotune_does_not_change_single_precision_results():
 
 
 TILE_SCHEMES = [Scheme("MFCT", "QR4C", "FCT", 0.85, 0.75), Scheme("MUSCL", "PPM", "FCT", 0.9, 0.6),
-                Scheme("UPW1", "CDIFF", "FCT"), Scheme("MFCT", "UPW1", "FCT", 1.0, 1.0)]
+                Scheme("UPW1", "CDIFF", "FCT"), Scheme("MFCT", "UPW1", "FCT", 1.0, 1.0),
+                Scheme("MUSCL", "QR4C", "NONE", 0.8, 0.7), Scheme("MFCT", "PPM", "NONE", 1.0, 1.0)]
 
 
 def _run_tiles(case, nsteps, kb, tiles, prec=0, lc=(16, 5, 7, 3)):
@@ -398,8 +399,8 @@ def _run_tiles(case, nsteps, kb, tiles, prec=0, lc=(16, 5, 7, 3)):
 @pytest.mark.parametrize("kb", [1, 2, 4])
 @pytest.mark.parametrize("si", range(len(TILE_SCHEMES)))
 def test_halo_tiles_match_untiled_and_oracle(oracle_mod, kb, si):
-    """The halo-tiled FCT node kernels (many tiles with halos, level chunks) give the same bits
-    as the untiled kernels and as the reference restatement."""
+    """The tiled kernels (edge tiles, and with FCT node tiles with halos; level chunks) give the
+    same bits as the untiled kernels and as the reference restatement."""
     case = configs.bathymetry_case("bathy48", 48, 45, seed=71, ntracers=3, nsteps=3, lev_lo=3,
                                    scheme=TILE_SCHEMES[si], w_cfl=0.3)
     say(f"case bathy48 scheme {si} kb={kb} tiled vs untiled")
