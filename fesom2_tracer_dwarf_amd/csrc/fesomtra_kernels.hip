@@ -552,8 +552,8 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         if constexpr (std::is_same<C, double>::value) return p.LO[cell * KB + lane];
         else return t + (D)p.LO[cell * KB + lane];
     };
-    auto cluster = [&](int nz, int buf, D &vmax, D &vmin) {
-        if constexpr (TILED) {
+    auto cluster = [&](int nz, int buf, D &vmax, D &vmin, D &own_lo) {
+        if constexpr (TILED) {  // own_lo: taken from the staging fetch by the caller
             vmax = shi[buf][threadIdx.x];
             vmin = slo[buf][threadIdx.x];
             for_edges(m, ne, nz, [&](int, bool, int oth) {
@@ -563,6 +563,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         } else {
             const size_t o = (size_t)(nz - 1) * Np;
             const D t0 = ldT(o + n), l0 = lo_at(o + n, t0);
+            own_lo = l0;
             vmax = cmax(l0, t0);
             vmin = cmin(l0, t0);
             for_edges(m, ne, nz, [&](int, bool, int oth) {
@@ -576,13 +577,15 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
             vmin = cmin(vmin, D(1.0e3));
         }
     };
-    // P+/P-, Q+/Q- and the ratios of layer nz with cluster bounds (bmax, bmin)
-    auto finish = [&](int nz, D bmax, D bmin) {
+    // P+/P-, Q+/Q- and the ratios of layer nz with cluster bounds (bmax, bmin) and low-order
+    // value lo; the vertical flux at the layer's bottom is kept for the next layer (av_bot)
+    C av_bot = 0;
+    auto finish = [&](int nz, D bmax, D bmin, D lo) {
         const size_t cell = (size_t)(nz - 1) * Np + n;
         const double V = (double)cld(m.areasvol, cell);
-        const D lo = lo_at(cell, ldT(cell));
         const D qmax = bmax - lo, qmin = bmin - lo;
-        const C av0 = p.adfv[cell * KB + lane], av1 = p.adfv[(cell + Np) * KB + lane];
+        const C av0 = nz == nz0 ? p.adfv[cell * KB + lane] : av_bot, av1 = p.adfv[(cell + Np) * KB + lane];
+        av_bot = av1;
         double pp = 0.0 + ((double)cmax(C(0), av0) + (double)cmax(C(0), -av1));
         double pm = 0.0 + ((double)cmin(C(0), av0) + (double)cmin(C(0), -av1));
         const size_t eo = (size_t)(nz - 1) * Ep;
@@ -598,25 +601,29 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         p.Rp[cell * KB + lane] = (C)(rp < 1.0 ? rp : 1.0);
         p.Rm[cell * KB + lane] = (C)(rm < 1.0 ? rm : 1.0);
     };
-    D hi_r[2], lo_r[2];
+    D hi_r[2], lo_r[2], lo_next = 0;  // staging registers; LO of this column's fetched level
     auto fetch = [&](int j) {
         ht.each([&](int i, int node) {
             const size_t c = (size_t)(j - 1) * Np + node;
             const D t = ldT(c), l = lo_at(c, t);
             hi_r[i] = cmax(l, t);
             lo_r[i] = cmin(l, t);
+            if (i == 0) lo_next = l;
         });
     };
     if constexpr (TILED) fetch(jb);
     D mx2 = 0, mn2 = 0, mx1 = 0, mn1 = 0, mx0 = 0, mn0 = 0;  // clusters of levels j-2, j-1, j
+    D lo1 = 0, lo0 = 0;                                      // this column's LO at j-1, j
     for (int j = jb; j <= je; ++j) {
         const int buf = j & 1;
+        D lo_j = 0;
         if constexpr (TILED) {
             if (j <= js) {
                 ht.each([&](int i, int) {
                     shi[buf][threadIdx.x + 256 * i] = hi_r[i];
                     slo[buf][threadIdx.x + 256 * i] = lo_r[i];
                 });
+                lo_j = lo_next;
                 __syncthreads();
                 if (j < js) fetch(j + 1);
             }
@@ -626,11 +633,15 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         mn2 = mn1;
         mx1 = mx0;
         mn1 = mn0;
-        if (j <= L) cluster(j, buf, mx0, mn0);
+        lo1 = lo0;
+        if (j <= L) {
+            cluster(j, buf, mx0, mn0, lo0);
+            if constexpr (TILED) lo0 = lo_j;
+        }
         const int nz = j - 1;
         if (nz >= nz0 && nz <= nz1) {
             const bool own = nz == 1 || nz == L;
-            finish(nz, own ? mx1 : cmax(cmax(mx2, mx1), mx0), own ? mn1 : cmin(cmin(mn2, mn1), mn0));
+            finish(nz, own ? mx1 : cmax(cmax(mx2, mx1), mx0), own ? mn1 : cmin(cmin(mn2, mn1), mn0), lo1);
         }
     }
 }
@@ -738,8 +749,13 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
             rp0 = rp1;
             rm0 = rm1;
             if (nz + 1 <= L) {
-                rp1 = p.Rp[X(cell + Np)];
-                rm1 = p.Rm[X(cell + Np)];
+                if (TILED && nz < lv1) {  // staging slot 0 holds this column's next level
+                    rp1 = sp[0];
+                    rm1 = sm[0];
+                } else {
+                    rp1 = p.Rp[X(cell + Np)];
+                    rm1 = p.Rm[X(cell + Np)];
+                }
                 bot = limit(ab, nz + 1, rp0, rm0, rp1, rm1);
             } else {
                 bot = ab;  // bottom interface: never limited (always zero)
