@@ -355,6 +355,39 @@ __device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges<P
     }
 }
 
+// Two-phase form of for_edges: load(edge, second, far) for every edge active at level nz is
+// issued first (the loads of all cached entries are in flight together), then use(value, edge,
+// second, far) consumes them in reference order, so a node's sum waits for one memory latency
+// per level instead of one per edge.  It holds all loaded values at once: where that would spill
+// (the low-order kernel of narrow blocks, whose columns hold more per-thread state) the
+// one-phase form is kept (TWO = false).
+template <int KB>
+constexpr bool kTwoPhase = KB > 4;
+
+template <bool TWO, bool PACK, typename M, typename LOAD, typename USE>
+__device__ __forceinline__ void for_edges_ld(const MeshCol<M> &m, const NodeEdges<PACK> &ne, int nz, LOAD &&load,
+                                             USE &&use) {
+    if constexpr (!TWO) {
+        for_edges(m, ne, nz, [&](int e, bool second, int far) { use(load(e, second, far), e, second, far); });
+        return;
+    }
+    using V = decltype(load(0, false, 0));
+    V v[kDegCache];
+#pragma unroll
+    for (int i = 0; i < kDegCache; ++i)
+        if (i < ne.deg && nz <= ne.top_of(i)) v[i] = load(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
+#pragma unroll
+    for (int i = 0; i < kDegCache; ++i)
+        if (i < ne.deg && nz <= ne.top_of(i)) use(v[i], ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
+    for (int i = kDegCache; i < ne.deg; ++i) {
+        const int e2 = cld(m.ne_e2, ne.b + i);
+        if (nz <= cld(m.ne_top, ne.b + i)) {
+            const int far = cld(PACK ? m.ht_loc : m.ne_other, ne.b + i);
+            use(load(e2 >> 1, (e2 & 1) != 0, far), e2 >> 1, (e2 & 1) != 0, far);
+        }
+    }
+}
+
 // Low-order solution and vertical antidiffusive flux (lib/oce_adv_tra_driver.F90:102-216 and
 // the high-order vertical call :266-282 with o_init_zero=.false., pwvel = w).  Interface fluxes
 // are evaluated top-down in double; layer nz uses the fluxes at its top (nz) and bottom (nz + 1).
@@ -445,13 +478,18 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         // summed in reference edge order
         double s = 0.0;
         const size_t eo = (size_t)(nz - 1) * Ep, no = (size_t)(nz - 1) * Np;
-        for_edges(m, ne, nz, [&](int e, bool second, int oth) {
-            double to;
-            if constexpr (TILED) to = sT[buf][oth * KB + lane];
-            else to = ldc<double>(p.T, (no + oth) * KB + lane);
-            const double f = upw1_edge<double>(second ? to : tcur, second ? tcur : to, cld(m.q, eo + e));
-            s = second ? s - f : s + f;
-        });
+        for_edges_ld<kTwoPhase<KB>>(
+            m, ne, nz,
+            [&](int e, bool, int oth) {
+                double to;
+                if constexpr (TILED) to = sT[buf][oth * KB + lane];
+                else to = ldc<double>(p.T, (no + oth) * KB + lane);
+                return make_double2(to, (double)cld(m.q, eo + e));
+            },
+            [&](double2 v, int, bool second, int) {
+                const double f = upw1_edge<double>(second ? v.x : tcur, second ? tcur : v.x, v.y);
+                s = second ? s - f : s + f;
+            });
         const double lo = ((double)tcur * h + (s + ((double)lo_top - (double)lo_bot)) * dt / V) / hn;
         // FP64 stores LO; single precision stores the increment LO - T, so that low-order changes
         // far below the tracer's own magnitude survive the rounding to C
@@ -566,11 +604,16 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
             own_lo = l0;
             vmax = cmax(l0, t0);
             vmin = cmin(l0, t0);
-            for_edges(m, ne, nz, [&](int, bool, int oth) {
-                const D tt = ldT(o + oth), lo = lo_at(o + oth, tt);
-                vmax = cmax(vmax, cmax(lo, tt));
-                vmin = cmin(vmin, cmin(lo, tt));
-            });
+            for_edges_ld<true>(
+                m, ne, nz,
+                [&](int, bool, int oth) {
+                    const D tt = ldT(o + oth);
+                    return make_double2(tt, lo_at(o + oth, tt));
+                },
+                [&](double2 v, int, bool, int) {
+                    vmax = cmax(vmax, cmax(v.y, v.x));
+                    vmin = cmin(vmin, cmin(v.y, v.x));
+                });
         }
         if (nz >= dry_from) {
             vmax = cmax(vmax, D(-1.0e3));
@@ -589,12 +632,13 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         double pp = 0.0 + ((double)cmax(C(0), av0) + (double)cmax(C(0), -av1));
         double pm = 0.0 + ((double)cmin(C(0), av0) + (double)cmin(C(0), -av1));
         const size_t eo = (size_t)(nz - 1) * Ep;
-        for_edges(m, ne, nz, [&](int e, bool second, int) {
-            C a = p.adfh[(eo + e) * KB + lane];
-            if (second) a = -a;
-            pp = pp + (double)cmax(C(0), a);
-            pm = pm + (double)cmin(C(0), a);
-        });
+        for_edges_ld<true>(
+            m, ne, nz, [&](int e, bool, int) { return p.adfh[(eo + e) * KB + lane]; },
+            [&](C a, int, bool second, int) {
+                if (second) a = -a;
+                pp = pp + (double)cmax(C(0), a);
+                pm = pm + (double)cmin(C(0), a);
+            });
         const double fp = pp * dt / V + 1e-16;
         const double fm = pm * dt / V - 1e-16;
         const double rp = (double)qmax / fp, rm = (double)qmin / fm;
@@ -651,6 +695,13 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
 // acc_v the tendencies are accumulated there (do_oce_adv_tra contract); otherwise values +=
 // (dttf_h + dttf_v) / hnode_new in place.  dbg_h / dbg_v receive the limited fluxes if set.
 // acc_* and dbg_* hold one tracer: they are only passed with a single-lane mask.
+// Per-edge inputs of the flux divergence: antidiffusive flux, the far node's limiter ratios,
+// and (single precision without limiter) the far node's valuesAB and the transport.
+template <typename C>
+struct EdgeVals {
+    C f, rp, rm;
+    double ao, q;
+};
 template <bool TILED, bool FCT, int KB, typename S, typename M, typename C>
 __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
     static_assert(!TILED || FCT, "tiles stage the limiter ratios");
@@ -768,20 +819,16 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
         double dh = p.acc_h ? p.acc_h[cell] : 0.0;
         const C rpn = FCT ? rp0 : C(0), rmn = FCT ? rm0 : C(0);
         const size_t eo = (size_t)(nz - 1) * Ep, no = (size_t)(nz - 1) * Np;
-        for_edges(m, ne, nz, [&](int e, bool second, int oth) {
-            C f = p.adfh[X(eo + e)];
+        // single precision without limiter: the stored flux is high-order minus upwind; the
+        // upwind part of valuesAB is added here in double
+        constexpr bool kAddUpw = !FCT && !std::is_same<C, double>::value;
+        const double an = kAddUpw ? ldc<double>(p.TAB, X(cell)) : 0.0;
+        auto add_edge = [&](const EdgeVals<C> &v, int e, bool second) {
+            C f = v.f;
             if (FCT) {
                 // R of both end nodes (lib/oce_adv_tra_fct.F90:455-468)
-                C rpo, rmo;
-                if constexpr (TILED) {
-                    rpo = sRp[buf][oth * KB + lane];
-                    rmo = sRm[buf][oth * KB + lane];
-                } else {
-                    rpo = p.Rp[X(no + oth)];
-                    rmo = p.Rm[X(no + oth)];
-                }
-                const C rp_1 = second ? rpo : rpn, rm_1 = second ? rmo : rmn;  // edges(1)
-                const C rp_2 = second ? rpn : rpo, rm_2 = second ? rmn : rmo;  // edges(2)
+                const C rp_1 = second ? v.rp : rpn, rm_1 = second ? v.rm : rmn;  // edges(1)
+                const C rp_2 = second ? rpn : v.rp, rm_2 = second ? rmn : v.rm;  // edges(2)
                 C ae = 1;
                 if (f >= C(0)) {
                     ae = cmin(ae, rp_1);
@@ -794,14 +841,33 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
                 if (p.dbg_h && !second) p.dbg_h[eo + e] = f;
             }
             double fd = (double)f;
-            if constexpr (!FCT && !std::is_same<C, double>::value) {
-                // single precision without limiter: the stored flux is high-order minus upwind;
-                // the upwind part of valuesAB is added here in double
-                const double an = ldc<double>(p.TAB, X(cell)), ao = ldc<double>(p.TAB, X(no + oth));
-                fd = fd + upw1_edge<double>(second ? ao : an, second ? an : ao, cld(m.q, eo + e));
-            }
+            if (kAddUpw) fd = fd + upw1_edge<double>(second ? v.ao : an, second ? an : v.ao, v.q);
             dh = second ? dh - fd * dt / V : dh + fd * dt / V;
-        });
+        };
+        if constexpr (TILED) {
+            // antidiffusive fluxes from memory, issued together; limiter ratios from the tile
+            for_edges_ld<true>(
+                m, ne, nz, [&](int e, bool, int) { return p.adfh[X(eo + e)]; },
+                [&](C f, int e, bool second, int oth) {
+                    add_edge(EdgeVals<C>{f, sRp[buf][oth * KB + lane], sRm[buf][oth * KB + lane], 0.0, 0.0}, e, second);
+                });
+        } else {
+            for_edges_ld<kTwoPhase<KB>>(
+                m, ne, nz,
+                [&](int e, bool, int oth) {
+                    EdgeVals<C> v{p.adfh[X(eo + e)], C(0), C(0), 0.0, 0.0};
+                    if (FCT) {
+                        v.rp = p.Rp[X(no + oth)];
+                        v.rm = p.Rm[X(no + oth)];
+                    }
+                    if (kAddUpw) {
+                        v.ao = ldc<double>(p.TAB, X(no + oth));
+                        v.q = cld(m.q, eo + e);
+                    }
+                    return v;
+                },
+                [&](const EdgeVals<C> &v, int e, bool second, int) { add_edge(v, e, second); });
+        }
         if (p.acc_h) {
             p.acc_h[cell] = dh;
             p.acc_v[cell] = dv;
