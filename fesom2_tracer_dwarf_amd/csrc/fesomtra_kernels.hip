@@ -204,15 +204,17 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
         for (int i = 0; i < 4; ++i) sl[i] = cld(m.et_loc, 4 * (size_t)e + i);
     }
     auto slot = [&](int k) { return (sl[k >> 1] >> (16 * (k & 1))) & 0xffff; };
-    C stT[2], stA[2];
+    C stT[2], stA[2], qn = 0;  // staging registers and this edge's transport, a level ahead
     auto fetch = [&](int nz) {
         const size_t o = (size_t)(nz - 1) * Np;
         et.each([&](int i, int node) {
             stT[i] = ldT(o + node);
             if (!same) stA[i] = ldA(o + node);
         });
+        if (act && nz <= nz1) qn = (C)cld(m.q, (size_t)(nz - 1) * Ep + e);
     };
     if constexpr (TILED) fetch(lv0);
+    C qcur = 0;
     for (int nz = lv0; nz <= lv1; ++nz) {
         const int buf = nz & 1;
         if constexpr (TILED) {
@@ -220,12 +222,13 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
                 sT[buf][threadIdx.x + 256 * i] = stT[i];
                 if (!same) sA[buf][threadIdx.x + 256 * i] = stA[i];
             });
+            qcur = qn;
             __syncthreads();
             if (nz < lv1) fetch(nz + 1);
             if (!act || nz > nz1) continue;
         }
         const int k = nz - 1;
-        const C v = (C)cld(m.q, (size_t)k * Ep + e);
+        const C v = TILED ? qcur : (C)cld(m.q, (size_t)k * Ep + e);
         const size_t o = (size_t)k * Np;
         const size_t ie = ((size_t)k * Ep + e) * KB + lane;
         const SlotLd<KB, C> lsT{sT[TILED ? buf : 0], lane}, lsA{same ? sT[TILED ? buf : 0] : sA[TILED ? buf : 0], lane};
