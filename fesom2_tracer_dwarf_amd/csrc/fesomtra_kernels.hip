@@ -362,7 +362,7 @@ __device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges<P
 // issued first (the loads of all cached entries are in flight together), then use(value, edge,
 // second, far) consumes them in reference order, so a node's sum waits for one memory latency
 // per level instead of one per edge.  It holds all loaded values at once: where that would spill
-// (the low-order kernel of narrow blocks, whose columns hold more per-thread state) the
+// (the untiled low-order kernel of narrow blocks, whose columns hold more per-thread state) the
 // one-phase form is kept (TWO = false).
 template <int KB>
 constexpr bool kTwoPhase = KB > 4;
@@ -481,18 +481,23 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         // summed in reference edge order
         double s = 0.0;
         const size_t eo = (size_t)(nz - 1) * Ep, no = (size_t)(nz - 1) * Np;
-        for_edges_ld<kTwoPhase<KB>>(
-            m, ne, nz,
-            [&](int e, bool, int oth) {
-                double to;
-                if constexpr (TILED) to = sT[buf][oth * KB + lane];
-                else to = ldc<double>(p.T, (no + oth) * KB + lane);
-                return make_double2(to, (double)cld(m.q, eo + e));
-            },
-            [&](double2 v, int, bool second, int) {
-                const double f = upw1_edge<double>(second ? v.x : tcur, second ? tcur : v.x, v.y);
-                s = second ? s - f : s + f;
-            });
+        auto add_edge = [&](double to, double qe, bool second) {
+            const double f = upw1_edge<double>(second ? to : tcur, second ? tcur : to, qe);
+            s = second ? s - f : s + f;
+        };
+        if constexpr (TILED) {
+            // transports from memory, issued together; neighbour values from the tile
+            for_edges_ld<true>(
+                m, ne, nz, [&](int e, bool, int) { return (double)cld(m.q, eo + e); },
+                [&](double qe, int, bool second, int oth) { add_edge(sT[buf][oth * KB + lane], qe, second); });
+        } else {
+            for_edges_ld<kTwoPhase<KB>>(
+                m, ne, nz,
+                [&](int e, bool, int oth) {
+                    return make_double2(ldc<double>(p.T, (no + oth) * KB + lane), (double)cld(m.q, eo + e));
+                },
+                [&](double2 v, int, bool second, int) { add_edge(v.x, v.y, second); });
+        }
         const double lo = ((double)tcur * h + (s + ((double)lo_top - (double)lo_bot)) * dt / V) / hn;
         // FP64 stores LO; single precision stores the increment LO - T, so that low-order changes
         // far below the tracer's own magnitude survive the rounding to C
