@@ -314,19 +314,23 @@ __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
 // Node -> edge entries cached in registers (the first kDegCache, the rest read from memory).
 // The far end is a node number (m.ne_other) or, with PACK (halo tiles), the far node's tile
 // slot (m.ht_loc, < 2^15) kept with the entry's deepest level in one register.
-template <bool PACK>
+#ifndef FESOMTRA_DEG_CACHE_WIDE
+#define FESOMTRA_DEG_CACHE_WIDE 6
+#endif
+template <bool PACK, int NC = kDegCache>
 struct NodeEdges {
+    static constexpr int kN = NC;
     int b, deg;
-    int e2[kDegCache], top[kDegCache], oth[PACK ? 1 : kDegCache];
+    int e2[NC], top[NC], oth[PACK ? 1 : NC];
     __device__ __forceinline__ int top_of(int i) const { return PACK ? (top[i] & 0xffff) : top[i]; }
     __device__ __forceinline__ int far_of(int i) const { return PACK ? (top[i] >> 16) : oth[i]; }
 };
-template <bool PACK, typename M>
-__device__ __forceinline__ void load_edges(const MeshCol<M> &m, int n, NodeEdges<PACK> &ne) {
+template <bool PACK, int NC, typename M>
+__device__ __forceinline__ void load_edges(const MeshCol<M> &m, int n, NodeEdges<PACK, NC> &ne) {
     ne.b = cld(m.ne_ptr, n);
     ne.deg = cld(m.ne_ptr, n + 1) - ne.b;
 #pragma unroll
-    for (int i = 0; i < kDegCache; ++i) {
+    for (int i = 0; i < NC; ++i) {
         const bool on = i < ne.deg;
         ne.e2[i] = on ? cld(m.ne_e2, ne.b + i) : 0;
         if constexpr (PACK) {
@@ -337,21 +341,21 @@ __device__ __forceinline__ void load_edges(const MeshCol<M> &m, int n, NodeEdges
         }
     }
 }
-template <bool PACK>
-__device__ __forceinline__ void no_edges(NodeEdges<PACK> &ne) {
+template <bool PACK, int NC>
+__device__ __forceinline__ void no_edges(NodeEdges<PACK, NC> &ne) {
     ne.b = ne.deg = 0;
 #pragma unroll
-    for (int i = 0; i < kDegCache; ++i) ne.e2[i] = ne.top[i] = 0;
+    for (int i = 0; i < NC; ++i) ne.e2[i] = ne.top[i] = 0;
 #pragma unroll
-    for (int i = 0; i < (PACK ? 1 : kDegCache); ++i) ne.oth[i] = 0;
+    for (int i = 0; i < (PACK ? 1 : NC); ++i) ne.oth[i] = 0;
 }
 // Calls f(edge, second, far) for the edges of n active at level nz, in reference order.
-template <bool PACK, typename M, typename F>
-__device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges<PACK> &ne, int nz, F &&f) {
+template <bool PACK, int NC, typename M, typename F>
+__device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges<PACK, NC> &ne, int nz, F &&f) {
 #pragma unroll
-    for (int i = 0; i < kDegCache; ++i)
+    for (int i = 0; i < NC; ++i)
         if (i < ne.deg && nz <= ne.top_of(i)) f(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
-    for (int i = kDegCache; i < ne.deg; ++i) {
+    for (int i = NC; i < ne.deg; ++i) {
         const int e2 = cld(m.ne_e2, ne.b + i);
         if (nz <= cld(m.ne_top, ne.b + i))
             f(e2 >> 1, (e2 & 1) != 0, cld(PACK ? m.ht_loc : m.ne_other, ne.b + i));
@@ -366,23 +370,27 @@ __device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges<P
 // one-phase form is kept (TWO = false).
 template <int KB>
 constexpr bool kTwoPhase = KB > 4;
+// Node -> edge entries held in registers: with one column per wavefront (KB = 64) they are
+// scalar registers, which the wide kernels are short of.
+template <int KB>
+constexpr int kDegCacheFor = KB == 64 ? FESOMTRA_DEG_CACHE_WIDE : kDegCache;
 
-template <bool TWO, bool PACK, typename M, typename LOAD, typename USE>
-__device__ __forceinline__ void for_edges_ld(const MeshCol<M> &m, const NodeEdges<PACK> &ne, int nz, LOAD &&load,
+template <bool TWO, bool PACK, int NC, typename M, typename LOAD, typename USE>
+__device__ __forceinline__ void for_edges_ld(const MeshCol<M> &m, const NodeEdges<PACK, NC> &ne, int nz, LOAD &&load,
                                              USE &&use) {
     if constexpr (!TWO) {
         for_edges(m, ne, nz, [&](int e, bool second, int far) { use(load(e, second, far), e, second, far); });
         return;
     }
     using V = decltype(load(0, false, 0));
-    V v[kDegCache];
+    V v[NC];
 #pragma unroll
-    for (int i = 0; i < kDegCache; ++i)
+    for (int i = 0; i < NC; ++i)
         if (i < ne.deg && nz <= ne.top_of(i)) v[i] = load(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
 #pragma unroll
-    for (int i = 0; i < kDegCache; ++i)
+    for (int i = 0; i < NC; ++i)
         if (i < ne.deg && nz <= ne.top_of(i)) use(v[i], ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
-    for (int i = kDegCache; i < ne.deg; ++i) {
+    for (int i = NC; i < ne.deg; ++i) {
         const int e2 = cld(m.ne_e2, ne.b + i);
         if (nz <= cld(m.ne_top, ne.b + i)) {
             const int far = cld(PACK ? m.ht_loc : m.ne_other, ne.b + i);
@@ -425,7 +433,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         lv1 = nz1;
     }
     const size_t Np = m.Np, Ep = m.Ep;
-    NodeEdges<TILED> ne;
+    NodeEdges<TILED, kDegCacheFor<KB>> ne;
     if (act) load_edges(m, n, ne);
     else no_edges(ne);
     const int nc_n = act ? n : 0;
@@ -589,7 +597,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
     }
     const size_t Np = m.Np, Ep = m.Ep;
     const int dry_from = act ? cld(m.nlev_min, n) : 0;  // levels >= this have a dry element around n
-    NodeEdges<TILED> ne;
+    NodeEdges<TILED, kDegCacheFor<KB>> ne;
     if (act) load_edges(m, n, ne);
     else no_edges(ne);
     const LaneLd<KB, D, S> ldT{p.T, lane};
@@ -737,7 +745,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
         lv1 = nz1;
     }
     const size_t Np = m.Np, Ep = m.Ep;
-    NodeEdges<TILED> ne;
+    NodeEdges<TILED, kDegCacheFor<KB>> ne;
     if (act) load_edges(m, n, ne);
     else no_edges(ne);
     // limited flux at interface j given R at layers j-1 (rp0, rm0) and j (rp1, rm1)
