@@ -230,7 +230,7 @@ __device__ __forceinline__ C ver_ppm(const Col<C, S, M, KB> &c, int j, C dt, C f
 // reference's statement sequence does.
 template <typename C>
 __device__ __forceinline__ C ver_qr4c_w(int j, int nzmax, const C tw[4], const C z[4], C zb, C w,
-                                        C ar, C num_ord, C fin) {
+                                        C ar, C num_ord, C fin, const C *qw = nullptr) {
     if (j == 1) return -tw[2] * w * ar - fin;
     if (j == nzmax) return C(0) - fin;
     const bool second = (j == 2), penult = (j == nzmax - 1);
@@ -241,9 +241,11 @@ __device__ __forceinline__ C ver_qr4c_w(int j, int nzmax, const C tw[4], const C
         if (penult) f = cv - f;
         return f;
     }
-    const C qc = (tw[1] - tw[2]) / (z[1] - z[2]);
-    const C qu = (tw[2] - tw[3]) / (z[2] - z[3]);
-    const C qd = (tw[0] - tw[1]) / (z[0] - z[1]);
+    // qd, qc, qu: the quotient (T(i) - T(i+1)) / (Z(i) - Z(i+1)) for i = j-2, j-1, j; a caller
+    // sweeping the column passes them in (qw), each computed once (same operands, same value)
+    const C qc = qw ? qw[1] : (tw[1] - tw[2]) / (z[1] - z[2]);
+    const C qu = qw ? qw[2] : (tw[2] - tw[3]) / (z[2] - z[3]);
+    const C qd = qw ? qw[0] : (tw[0] - tw[1]) / (z[0] - z[1]);
     const C Tmean1 = tw[2] + (C(2) * qc + qu) * (zb - z[2]) / C(3);
     const C Tmean2 = tw[1] + (C(2) * qc + qd) * (zb - z[1]) / C(3);
     const C Tmean = (w + fabs(w)) * Tmean1 + (w - fabs(w)) * Tmean2;
@@ -257,6 +259,25 @@ __device__ __forceinline__ C ver_upw1_w(int j, int nzmax, C tm, C tj, C w, C ar,
     if (j == nzmax) return C(0) - fin;
     return -C(0.5) * (tj * (w + fabs(w)) + tm * (w - fabs(w))) * ar - fin;
 }
+
+// QR4C's three quotients at interface j, (T(i) - T(i+1)) / (Z(i) - Z(i+1)) for i = j-2, j-1, j,
+// kept while a column is swept downward: one new quotient per interface.
+template <typename C>
+struct QuotWin {
+    C q[3];
+    __device__ __forceinline__ static C quot(C ta, C tb, C za, C zb) { return (ta - tb) / (za - zb); }
+    // all three for the window tw = TAB(j-2..j+1), z = Z(j-2..j+1)
+    __device__ __forceinline__ void init(const C tw[4], const C z[4]) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) q[i] = quot(tw[i], tw[i + 1], z[i], z[i + 1]);
+    }
+    // after the window moved down by one interface
+    __device__ __forceinline__ void slide(const C tw[4], const C z[4]) {
+        q[0] = q[1];
+        q[1] = q[2];
+        q[2] = quot(tw[2], tw[3], z[2], z[3]);
+    }
+};
 
 // Column readers of one node (1-based levels; 0 outside the wet range).
 template <typename C, typename S, typename M, int KB>
@@ -273,11 +294,12 @@ struct NodeCol {
 };
 
 // High-order vertical flux at interface j minus fin for the window tw = TAB(j-2..j+1),
-// z = Z(j-2..j+1).
+// z = Z(j-2..j+1) (QR4C: with the window's quotients qw when given, see QuotWin).
 template <int VER, typename C, typename S, typename M, int KB>
 __device__ __forceinline__ C ver_ho_at(int j, int nzmax, const C tw[4], const C z[4], C zbj, C wj, C aj,
-                                       C num_ord, C dt, const Col<C, S, M, KB> &col, C fin) {
-    if (VER == VER_QR4C) return ver_qr4c_w<C>(j, nzmax, tw, z, zbj, wj, aj, num_ord, fin);
+                                       C num_ord, C dt, const Col<C, S, M, KB> &col, C fin,
+                                       const C *qw = nullptr) {
+    if (VER == VER_QR4C) return ver_qr4c_w<C>(j, nzmax, tw, z, zbj, wj, aj, num_ord, fin, qw);
     if (VER == VER_UPW1) return ver_upw1_w<C>(j, nzmax, tw[1], tw[2], wj, aj, fin);
     if (VER == VER_CDIFF) return ver_cdiff(col, j, fin);
     return ver_ppm(col, j, dt, fin);

@@ -441,6 +441,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
     const Col<D, S, M, KB> col = make_col<D, S, M, KB>(p.TAB, m.w, m, nc_n, lane, nzmax);
     const D pv = (D)num_ord;
     D tw[4], z[4], lo_top = 0, adf_top = 0, tcur = 0;
+    QuotWin<D> qw;
     if (act) {
         const int j = nz0;  // top interface of the chunk
 #pragma unroll
@@ -451,7 +452,8 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         for (int i = 0; i < 4; ++i) tw[i] = nc.lay(p.TAB, j - 2 + i);
         tcur = nc.lay(p.T, j);
         lo_top = ver_upw1_w<D>(j, nzmax, nc.lay(p.T, j - 1), tcur, wej, aj, D(0));
-        adf_top = ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, lo_top);
+        if (VER == VER_QR4C) qw.init(tw, z);
+        adf_top = ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, lo_top, qw.q);
     }
     D st[2];
     if constexpr (TILED)
@@ -482,9 +484,10 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         tw[1] = tw[2];
         tw[2] = tw[3];
         tw[3] = nc.lay(p.TAB, j + 1);
+        if (VER == VER_QR4C) qw.slide(tw, z);
         const D tnext = nc.lay(p.T, j);
         const D lo_bot = ver_upw1_w<D>(j, nzmax, tcur, tnext, wej, aj, D(0));
-        const D adf_bot = ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, lo_bot);
+        const D adf_bot = ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, lo_bot, qw.q);
         // upwind edge fluxes (lib/oce_adv_tra_hor.F90:178-181) from q and the two end values,
         // summed in reference edge order
         double s = 0.0;
@@ -538,6 +541,11 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_vert(const MeshCol<M> m,
     for (int i = 0; i < 4; ++i) z[i] = (VER == VER_QR4C) ? nc.mlay(m.Z, nz0 - 3 + i) : D(0);
 #pragma unroll
     for (int i = 0; i < 4; ++i) tw[i] = nc.lay(p.TAB, nz0 - 3 + i);
+    QuotWin<D> qw;  // primed so that the first slide below gives the window of interface nz0
+    if (VER == VER_QR4C) {
+        qw.q[1] = QuotWin<D>::quot(tw[1], tw[2], z[1], z[2]);
+        qw.q[2] = QuotWin<D>::quot(tw[2], tw[3], z[2], z[3]);
+    }
     const int jend = nz1 == L ? nzmax : nz1;
     for (int j = nz0; j <= jend; ++j) {
         if (VER == VER_QR4C) {
@@ -553,8 +561,9 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_vert(const MeshCol<M> m,
         tw[1] = tw[2];
         tw[2] = tw[3];
         tw[3] = nc.lay(p.TAB, j + 1);
+        if (VER == VER_QR4C) qw.slide(tw, z);
         p.adfv[((size_t)(j - 1) * Np + n) * KB + lane] =
-            (C)ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, D(0));
+            (C)ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, D(0), qw.q);
     }
 }
 
