@@ -370,6 +370,11 @@ __device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges<P
 // one-phase form is kept (TWO = false).
 template <int KB>
 constexpr bool kTwoPhase = KB > 4;
+// The flux divergence divides every edge flux of a cell by the cell's volume; with one column
+// per wavefront (KB = 64, bound by vector instructions) it does so through one reciprocal per
+// cell (div_reuse, bit-identical to dividing).
+template <int KB>
+constexpr bool kDivReuse = KB == 64;
 // Node -> edge entries held in registers: with one column per wavefront (KB = 64) they are
 // scalar registers, which the wide kernels are short of.
 template <int KB>
@@ -840,7 +845,8 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
         } else {
             bot = ab;
         }
-        dv = dv + ((double)top - (double)bot) * dt / V;
+        const double yV = kDivReuse<KB> ? 1.0 / V : 0.0;
+        dv = dv + (kDivReuse<KB> ? div_reuse(((double)top - (double)bot) * dt, V, yV) : ((double)top - (double)bot) * dt / V);
         double dh = p.acc_h ? p.acc_h[cell] : 0.0;
         const C rpn = FCT ? rp0 : C(0), rmn = FCT ? rm0 : C(0);
         const size_t eo = (size_t)(nz - 1) * Ep, no = (size_t)(nz - 1) * Np;
@@ -867,7 +873,8 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
             }
             double fd = (double)f;
             if (kAddUpw) fd = fd + upw1_edge<double>(second ? v.ao : an, second ? an : v.ao, v.q);
-            dh = second ? dh - fd * dt / V : dh + fd * dt / V;
+            const double x = kDivReuse<KB> ? div_reuse(fd * dt, V, yV) : fd * dt / V;
+            dh = second ? dh - x : dh + x;
         };
         if constexpr (TILED) {
             // antidiffusive fluxes from memory, issued together; limiter ratios from the tile

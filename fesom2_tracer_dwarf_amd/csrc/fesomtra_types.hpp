@@ -112,6 +112,20 @@ __device__ __forceinline__ void st_stochastic(T *__restrict__ p, size_t i, doubl
     }
 }
 
+// a / b for a divisor used many times, given y = 1 / b (correctly rounded): q0 = a y is within
+// one unit in the last place of a / b, the residual a - b q0 is exact with a fused multiply-add,
+// and q0 + (a - b q0) y is the correctly rounded quotient (P. W. Markstein, IBM J. Res. Dev. 34,
+// 1990).  A zero numerator returns a y (the signed zero of a / b); quotients near the underflow
+// or overflow thresholds, and non-finite ones, take the division itself.  The result is
+// bit-identical to a / b.
+__device__ __forceinline__ double div_reuse(double a, double b, double y) {
+    const double q0 = a * y;
+    if (a == 0.0) return q0;
+    const double aq = fabs(q0);
+    if (aq > 0x1p-960 && aq < 0x1p960) return fma(fma(-q0, b, a), y, q0);
+    return a / b;
+}
+
 template <typename C>
 __device__ __forceinline__ C cmin(C a, C b) { return a < b ? a : b; }
 template <typename C>
