@@ -113,9 +113,9 @@ __device__ __forceinline__ C ho_edge(C t1, C t2, const C gr[4], C v, C dx, C dy,
 template <int HOR, typename C>
 __device__ __forceinline__ C ho_minus_lo(C a1, C a2, const C gr[4], C v, C dx, C dy, C a, C th,
                                          C c1, C c2) {
-    const C r = (C)kREarth;
-    C d1 = (C(2) * (a2 - a1) + dx * a * gr[0] + dy * r * gr[2]) / C(6);
-    C d2 = -(C(2) * (a2 - a1) + dx * a * gr[1] + dy * r * gr[3]) / C(6);
+    const C r = (C)kREarth, sixth = C(1) / C(6);  // single precision: multiply by 1/6
+    C d1 = (C(2) * (a2 - a1) + dx * a * gr[0] + dy * r * gr[2]) * sixth;
+    C d2 = -(C(2) * (a2 - a1) + dx * a * gr[1] + dy * r * gr[3]) * sixth;
     if (HOR == HOR_MUSCL) {
         d1 = d1 * c1;
         d2 = d2 * c2;

@@ -620,20 +620,24 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         if constexpr (std::is_same<C, double>::value) return p.LO[cell * KB + lane];
         else return t + (D)p.LO[cell * KB + lane];
     };
+    // max / min of the cluster: the reference's (a > b ? a : b) in FP64; single precision uses the
+    // hardware max / min (one instruction; equal but for the choice between signed zeros)
+    auto bmax = [](D a, D b) { return std::is_same<C, double>::value ? cmax(a, b) : fmax(a, b); };
+    auto bmin = [](D a, D b) { return std::is_same<C, double>::value ? cmin(a, b) : fmin(a, b); };
     auto cluster = [&](int nz, int buf, D &vmax, D &vmin, D &own_lo) {
         if constexpr (TILED) {  // own_lo: taken from the staging fetch by the caller
             vmax = shi[buf][threadIdx.x];
             vmin = slo[buf][threadIdx.x];
             for_edges(m, ne, nz, [&](int, bool, int oth) {
-                vmax = cmax(vmax, shi[buf][oth * KB + lane]);
-                vmin = cmin(vmin, slo[buf][oth * KB + lane]);
+                vmax = bmax(vmax, shi[buf][oth * KB + lane]);
+                vmin = bmin(vmin, slo[buf][oth * KB + lane]);
             });
         } else {
             const size_t o = (size_t)(nz - 1) * Np;
             const D t0 = ldT(o + n), l0 = lo_at(o + n, t0);
             own_lo = l0;
-            vmax = cmax(l0, t0);
-            vmin = cmin(l0, t0);
+            vmax = bmax(l0, t0);
+            vmin = bmin(l0, t0);
             for_edges_ld<true>(
                 m, ne, nz,
                 [&](int, bool, int oth) {
@@ -641,22 +645,22 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
                     return make_double2(tt, lo_at(o + oth, tt));
                 },
                 [&](double2 v, int, bool, int) {
-                    vmax = cmax(vmax, cmax(v.y, v.x));
-                    vmin = cmin(vmin, cmin(v.y, v.x));
+                    vmax = bmax(vmax, bmax(v.y, v.x));
+                    vmin = bmin(vmin, bmin(v.y, v.x));
                 });
         }
         if (nz >= dry_from) {
-            vmax = cmax(vmax, D(-1.0e3));
-            vmin = cmin(vmin, D(1.0e3));
+            vmax = bmax(vmax, D(-1.0e3));
+            vmin = bmin(vmin, D(1.0e3));
         }
     };
-    // P+/P-, Q+/Q- and the ratios of layer nz with cluster bounds (bmax, bmin) and low-order
+    // P+/P-, Q+/Q- and the ratios of layer nz with cluster bounds (hi, lw) and low-order
     // value lo; the vertical flux at the layer's bottom is kept for the next layer (av_bot)
     C av_bot = 0;
-    auto finish = [&](int nz, D bmax, D bmin, D lo) {
+    auto finish = [&](int nz, D hi, D lw, D lo) {
         const size_t cell = (size_t)(nz - 1) * Np + n;
         const double V = (double)cld(m.areasvol, cell);
-        const D qmax = bmax - lo, qmin = bmin - lo;
+        const D qmax = hi - lo, qmin = lw - lo;
         const C av0 = nz == nz0 ? p.adfv[cell * KB + lane] : av_bot, av1 = p.adfv[(cell + Np) * KB + lane];
         av_bot = av1;
         double pp = 0.0 + ((double)cmax(C(0), av0) + (double)cmax(C(0), -av1));
@@ -680,8 +684,8 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         ht.each([&](int i, int node) {
             const size_t c = (size_t)(j - 1) * Np + node;
             const D t = ldT(c), l = lo_at(c, t);
-            hi_r[i] = cmax(l, t);
-            lo_r[i] = cmin(l, t);
+            hi_r[i] = bmax(l, t);
+            lo_r[i] = bmin(l, t);
             if (i == 0) lo_next = l;
         });
     };
@@ -715,7 +719,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         const int nz = j - 1;
         if (nz >= nz0 && nz <= nz1) {
             const bool own = nz == 1 || nz == L;
-            finish(nz, own ? mx1 : cmax(cmax(mx2, mx1), mx0), own ? mn1 : cmin(cmin(mn2, mn1), mn0), lo1);
+            finish(nz, own ? mx1 : bmax(bmax(mx2, mx1), mx0), own ? mn1 : bmin(bmin(mn2, mn1), mn0), lo1);
         }
     }
 }
