@@ -2,9 +2,9 @@
 //
 // One step of a block of KB interleaved tracers (the lanes of `mask` share one scheme) with the
 // FCT limiter is four column kernels (do_oce_adv_tra, lib/oce_adv_tra_driver.F90:23-294):
-//   k_edge    one thread per edge and lane: low-order upwind flux and high-order minus low-order flux
-//             for every level of the edge; MUSCL/MFCT gradients reconstructed on the fly from
-//             the up/down-wind triangles (no edge_up_dn_grad array)
+//   k_edge    one thread per edge and lane: high-order minus low-order (upwind) flux for every
+//             level of the edge; MUSCL/MFCT gradients reconstructed on the fly from the
+//             up/down-wind triangles (no edge_up_dn_grad array)
 //   k_lo      one thread per node and lane: deterministic gather of the edge fluxes, vertical upwind
 //             flux, low-order solution fct_LO, vertical antidiffusive flux (QR4C/PPM/...)
 //   k_bounds  one thread per node and lane: cluster bounds over the node's 1-ring, vertical widening,
@@ -14,8 +14,10 @@
 //             lib/oce_adv_tra_driver.F90:298-450, src/fesom_analytic.F90:419-437)
 // Without FCT: k_edge, k_vert and k_final.  No atomics: each node adds its incident edges in
 // the reference's edge order, so sums are deterministic and equal the serial reference.
-// For tracer blocks of width <= 4 the three FCT node kernels run on halo tiles (see HaloTile):
-// neighbour values are staged in LDS per level instead of being gathered once per edge.
+// For tracer blocks of width <= 4 the kernels run on tiles (EdgeTile for k_edge, HaloTile for the
+// three FCT node kernels): neighbour values are staged in LDS per level instead of being
+// gathered once per edge.  A node's sum over its edges issues the loads of all its edges before
+// consuming them (for_edges_ld), so a level waits for one memory latency rather than one per edge.
 #include "fesomtra_column.hpp"
 #include "fesomtra_kernels.hpp"
 

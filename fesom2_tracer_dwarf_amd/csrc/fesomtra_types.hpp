@@ -10,7 +10,9 @@
 //   tracer block    t[(k * Np + n) * KB + lane]     (edge work arrays with Ep in place of Np)
 // One thread owns one (column, lane) pair and walks the column's wet levels; the KB lanes of a
 // column are adjacent threads, so mesh values are fetched once per column for all KB tracers and
-// every tracer access is a contiguous run of KB values over consecutive columns.
+// every tracer access is a contiguous run of KB values over consecutive columns.  Narrow blocks
+// (KB <= 4) work on tiles of 256 / KB consecutive columns or edges whose neighbour values are
+// staged in LDS level by level (MeshCol::ht_*, MeshCol::et_*).
 //
 // Precision policy: S = tracer storage, M = mesh / velocity values (double), C = high-order
 // reconstruction arithmetic and the stored intermediates.  The edge transport q, the low-order
