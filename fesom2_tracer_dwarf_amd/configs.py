@@ -117,9 +117,11 @@ def blob_tracer(mesh: Mesh, rng: np.random.Generator, n_blobs: int = 4) -> np.nd
 
 
 def streamfunction_velocity(mesh: Mesh, umax: float,
-                            depth_scale: Optional[float] = None, taper_layers: float = 4.0) -> np.ndarray:
-    """Element velocities (u, v) = (-d psi/dy, d psi/dx) of the P1 interpolant of
-    psi = sin(2 pi x/Lx) sin(2 pi y/Ly), scaled so that the largest element speed is ``umax``.
+                            depth_scale: Optional[float] = None, taper_layers: float = 4.0,
+                            psi: Optional[np.ndarray] = None) -> np.ndarray:
+    """Element velocities (u, v) = (-d psi/dy, d psi/dx) of the P1 interpolant of a nodal
+    streamfunction, by default psi = sin(2 pi x/Lx) sin(2 pi y/Ly) on a periodic mesh, scaled so
+    that the largest element speed is ``umax``.
 
     Layer k uses psi_k = psi * clip((m(n) - k) / taper_layers, 0, 1), where m(n) is the number of
     layers in which every element around node n is wet: psi_k vanishes at every node touching a
@@ -130,8 +132,9 @@ def streamfunction_velocity(mesh: Mesh, umax: float,
     within their initial range with w = 0.  On a mesh without bathymetry (every column full depth)
     psi_k = psi.  ``depth_scale`` multiplies layer k by exp(-|Z_k|/depth_scale).
     """
-    xy = mesh.xy_m
-    psi = np.sin(2 * math.pi * xy[:, 0] / mesh.Lx) * np.sin(2 * math.pi * xy[:, 1] / mesh.Ly)
+    if psi is None:
+        xy = mesh.xy_m
+        psi = np.sin(2 * math.pi * xy[:, 0] / mesh.Lx) * np.sin(2 * math.pi * xy[:, 1] / mesh.Ly)
     en = mesh.elem2D_nodes - 1
     g = mesh.gradient_sca
 
@@ -215,6 +218,52 @@ def bathymetry_case(name: str, n: int, nl: int, seed: int, ntracers: int = 2,
     tracers = np.stack(tr[:ntracers])
     sc = scheme or Scheme()
     return Case(name, mesh, uv, w, w.copy(), w.copy(), tracers,
+                [dataclasses.replace(sc) for _ in range(ntracers)], dt, nsteps)
+
+
+def sphere_case(name: str, mesh: Mesh, seed: int = 0, ntracers: int = 2, nsteps: int = 10,
+                umax: float = 0.1, scheme: Optional[Scheme] = None, w_cfl: float = 0.0,
+                depth_scale: Optional[float] = 1000.0) -> Case:
+    """A case on a mesh of the sphere (a FESOM2 file mesh, see mesh_io.sphere_mesh).
+
+    Flow: the streamfunction psi = sin(2 lon) cos(lat)^2 (model coordinates), set to zero on the
+    coastline (the nodes of boundary edges) so that no transport crosses it, layered as in
+    :func:`streamfunction_velocity`.  T: a Gaussian blob of width 0.3 rad around a seeded ocean
+    node on 10; S: 34 + 2 sin(lat); further tracers: seeded blobs.  dt: half the smallest edge
+    length over ``umax``.  ``w_cfl`` adds an interior vertical velocity as in
+    :func:`bathymetry_case`.
+    """
+    from .mesh_io import edge_lengths
+    rng = np.random.default_rng(seed)
+    lon, lat = mesh.coord_nod2D[:, 0], mesh.coord_nod2D[:, 1]
+    coast = np.zeros(mesh.nod2D, dtype=bool)
+    coast[(mesh.edges[mesh.edge_tri[:, 1] == 0] - 1).ravel()] = True
+    psi = np.where(coast, 0.0, np.sin(2.0 * lon) * np.cos(lat) ** 2)
+    uv = streamfunction_velocity(mesh, umax, depth_scale, psi=psi)
+    dt = 0.5 * float(edge_lengths(mesh).min()) / umax
+    w = np.zeros((mesh.nod2D, mesh.nl))
+    if w_cfl:
+        h_min = float(np.min(mesh.zbar[:-1] - mesh.zbar[1:]))
+        ph = rng.uniform(0, 2 * math.pi, 2)
+        horiz = np.sin(lon + ph[0]) * np.cos(2.0 * lat + ph[1])
+        k = np.arange(1, mesh.nl + 1)[None, :]
+        interior = (k > mesh.ulevels_nod2D[:, None]) & (k < mesh.nlevels_nod2D[:, None])
+        prof = np.sin(math.pi * (k - 1) / (mesh.nl - 1))
+        w = np.where(interior, w_cfl * h_min / dt * horiz[:, None] * prof, 0.0)
+    wet = wet_mask(mesh)
+
+    def blob(n0, sigma, amp, base):
+        c = math.cos(lat[n0]) * np.cos(lat) * np.cos(lon - lon[n0]) + math.sin(lat[n0]) * np.sin(lat)
+        d = np.arccos(np.clip(c, -1.0, 1.0))
+        g = base + amp * np.exp(-d * d / (2.0 * sigma * sigma))
+        return np.where(wet, np.broadcast_to(g[:, None], wet.shape), 0.0)
+
+    tr = [blob(int(rng.integers(mesh.nod2D)), 0.3, 10.0, 10.0),
+          np.where(wet, np.broadcast_to((34.0 + 2.0 * np.sin(lat))[:, None], wet.shape), 0.0)]
+    while len(tr) < ntracers:
+        tr.append(blob(int(rng.integers(mesh.nod2D)), rng.uniform(0.1, 0.5), rng.uniform(0.0, 1.0), 0.0))
+    sc = scheme or Scheme()
+    return Case(name, mesh, uv, w, w.copy(), w.copy(), np.stack(tr[:ntracers]),
                 [dataclasses.replace(sc) for _ in range(ntracers)], dt, nsteps)
 
 

@@ -94,7 +94,10 @@ def stretched_layers(n_layers: int, h_top: float, h_bottom: float) -> np.ndarray
 
 
 def _min_image(d: np.ndarray, period: float) -> np.ndarray:
-    """Minimum-image difference on a periodic axis (exact for lattice coordinates)."""
+    """Minimum-image difference on a periodic axis (exact for lattice coordinates); an infinite
+    period leaves ``d`` unchanged."""
+    if not np.isfinite(period):
+        return d
     d = np.where(d > 0.5 * period, d - period, d)
     return np.where(d < -0.5 * period, d + period, d)
 
@@ -313,14 +316,15 @@ def _finish_mesh(nl, xy, enodes, ednodes, edtri, elem_nlev, layer_h, Lx, Ly, nx,
     )
 
 
-def _up_down_triangles(xy, enodes, ednodes, nie, nie_num, Lx, Ly) -> np.ndarray:
+def _up_down_triangles(xy, enodes, ednodes, nie, nie_num, Lx, Ly, require_all: bool = True) -> np.ndarray:
     """Up- and down-wind triangle of every edge (restates lib/oce_muscl_adv.F90:245-338).
 
     For edge n1->n2 with x = X(n2) - X(n1), the upwind triangle is the element around n1 whose
     corner at n1 contains -x, the downwind one the element around n2 containing +x.  Every
     element satisfying one of the three tests is assigned in ``nod_in_elem2D`` order (the
     reference's ``cycle`` continues the search), so the last match wins; 0-based result, -1 if
-    none matches.
+    none matches (allowed with ``require_all=False``: edges at a coast, as in the reference).
+    Coordinates wrap with periods ``Lx``, ``Ly`` (``inf``: no wrap).
     """
     E = ednodes.shape[0]
     out = np.full((E, 2), -1, dtype=np.int64)
@@ -353,7 +357,7 @@ def _up_down_triangles(xy, enodes, ednodes, nie, nie_num, Lx, Ly) -> np.ndarray:
                   (ab == ax) | (ax == 0)
             hit &= ok
             out[hit, side] = elem[hit]
-    if np.any(out < 0):
+    if require_all and np.any(out < 0):
         raise ValueError("an edge has no up- or down-wind triangle")
     return out
 

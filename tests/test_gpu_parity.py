@@ -428,3 +428,49 @@ def test_no_halo_tiles_for_wide_blocks():
                                    w_cfl=0.3)
     _, nt = _run_tiles(case, 1, 8, True)
     assert nt == 0
+
+
+# --- FESOM2 'pi' mesh of the reference's test data: sphere, coasts (455 boundary edges, edges
+# without up/down-wind triangles), node degrees up to 8, 47 layers of 5-250 m.
+PI_SCHEMES = [Scheme("MFCT", "QR4C", "FCT", 0.85, 0.75), Scheme("MUSCL", "PPM", "FCT", 0.9, 0.6),
+              Scheme("UPW1", "CDIFF", "NONE"), Scheme("MFCT", "QR4C", "NONE", 1.0, 1.0)]
+
+
+@pytest.fixture(scope="module")
+def pi_mesh():
+    from pathlib import Path
+    from fesom2_tracer_dwarf_amd import mesh_io
+    with np.load(Path(__file__).resolve().parent / "golden" / "pi_mesh.npz") as d:
+        return mesh_io.sphere_mesh({k: d[k] for k in d.files})
+
+
+@pytest.mark.parametrize("kb,tiles", [(0, True), (1, True), (0, False)])
+@pytest.mark.parametrize("si", range(len(PI_SCHEMES)))
+def test_pi_mesh_matches_oracle(oracle_mod, pi_mesh, si, kb, tiles):
+    case = configs.sphere_case("pi", pi_mesh, seed=7, ntracers=2, nsteps=3, scheme=PI_SCHEMES[si], w_cfl=0.3)
+    say(f"case pi mesh scheme {si} kb={kb} tiles={tiles}")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[3])
+    got, _ = _run_tiles(case, 3, kb, tiles, lc=(16, 16, 16, 16))
+    err = rel_linf(got, ref[3])
+    say(f"pi mesh rel Linf {err:.3e}")
+    assert err <= RTOL
+
+
+def test_pi_mesh_wide_block_matches_oracle(oracle_mod, pi_mesh):
+    """Four tracers in a 64-wide block (one column per wavefront): nodes of degree 7 and 8 go
+    beyond the six node->edge entries held in registers."""
+    case = configs.sphere_case("pi4", pi_mesh, seed=9, ntracers=4, nsteps=3, w_cfl=0.3)
+    say("case pi mesh 4 tracers kb=64")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[3])
+    got = gpu_run(case, 3, kb=64)
+    assert rel_linf(got, ref[3]) <= RTOL
+
+
+def test_pi_mesh_single_precision(oracle_mod, pi_mesh):
+    case = configs.sphere_case("pi32", pi_mesh, seed=11, ntracers=2, nsteps=5)
+    say("case pi mesh FP32 5 steps")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[5])
+    got = gpu_run(case, 5, prec=1)
+    err = rel_linf(got, ref[5])
+    say(f"pi mesh FP32 rel Linf {err:.3e}")
+    assert err <= 1e-5
