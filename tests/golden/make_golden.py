@@ -5,6 +5,7 @@ stores the reference's outputs plus a SHA-256 of the inputs it was made from, so
 generator is detected instead of silently comparing different cases.
 
     python tests/golden/make_golden.py        (needs oracle/_ref built by oracle/ref/build_ref.sh)
+    python tests/golden/make_golden.py --only pi   (only the FESOM2 pi-mesh fixture)
 """
 
 from __future__ import annotations
@@ -53,10 +54,48 @@ def analytic_cases():
             "analytic20_upw1_upw1_none": configs.analytic20(Scheme("UPW1", "UPW1", "NONE"))}
 
 
+PI_SCHEMES = [Scheme("MFCT", "QR4C", "FCT", 0.85, 0.75), Scheme("MUSCL", "PPM", "FCT", 0.9, 0.6),
+              Scheme("UPW1", "CDIFF", "NONE")]
+
+
+def pi_cases():
+    """The reference's FESOM2 'pi' test mesh (tests/golden/pi_mesh.npz) on the sphere, 3 steps."""
+    from fesom2_tracer_dwarf_amd import mesh_io
+    with np.load(HERE / "pi_mesh.npz") as d:
+        mesh = mesh_io.sphere_mesh({k: d[k] for k in d.files})
+    return {f"{s.hor}_{s.ver}_{s.lim}": configs.sphere_case("pi", mesh, seed=13, ntracers=2, nsteps=3,
+                                                             scheme=s, w_cfl=0.3) for s in PI_SCHEMES}
+
+
+def values_hash(v: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(v, dtype=np.float64).tobytes()).hexdigest()
+
+
+def write_pi(td):
+    """Per scheme: the SHA-256 of the reference's values after 3 steps (the full fields would be
+    megabytes) and every 97th node's column for diagnostics."""
+    out, hashes = {}, []
+    for name, case in pi_cases().items():
+        p = bundle.write_bundle(Path(td) / f"pi_{name}", case, save_steps=[3])
+        bundle.run_ref(p)
+        v = bundle.read_values(p, case, 3)
+        out[f"{name}_sha256"] = values_hash(v)
+        out[f"{name}_sample"] = v[:, ::97, :]
+        hashes.append(case_hash(case))
+    np.savez_compressed(HERE / "pi_mesh_schemes.npz",
+                        input_sha256=hashlib.sha256("".join(hashes).encode()).hexdigest(), **out)
+    print("wrote pi_mesh_schemes")
+
+
 def main():
     if not bundle.ref_available():
         raise SystemExit("oracle/_ref/ref_driver not built: run oracle/ref/build_ref.sh")
+    if sys.argv[1:3] == ["--only", "pi"]:
+        with tempfile.TemporaryDirectory() as td:
+            write_pi(td)
+        return
     with tempfile.TemporaryDirectory() as td:
+        write_pi(td)
         for name, case in analytic_cases().items():
             p = bundle.write_bundle(Path(td) / name, case, save_steps=[1, 2, 10], save_work=True)
             bundle.run_ref(p)

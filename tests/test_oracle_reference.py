@@ -72,3 +72,26 @@ def test_oracle_equals_live_reference_bathymetry(oracle_mod, tmp_path, perm):
     bundle.run_ref(p)
     out, _ = oracle_mod.Oracle(case).run(save_steps=[4])
     assert np.array_equal(out[4], bundle.read_values(p, case, 4))
+
+
+def test_oracle_equals_reference_fixture_pi_mesh(oracle_mod):
+    """The reference's FESOM2 pi test mesh (sphere, coasts, degrees up to 8): the reference's
+    values after 3 steps (as SHA-256, with sampled columns) for three scheme combinations."""
+    import hashlib
+    fx = np.load(GOLDEN / "pi_mesh_schemes.npz")
+    hashes = []
+    for name, case in make_golden.pi_cases().items():
+        hashes.append(make_golden.case_hash(case))
+        out, _ = oracle_mod.Oracle(case).run(save_steps=[3])
+        assert np.array_equal(out[3][:, ::97, :], fx[f"{name}_sample"]), name
+        assert make_golden.values_hash(out[3]) == str(fx[f"{name}_sha256"]), name
+    assert str(fx["input_sha256"]) == hashlib.sha256("".join(hashes).encode()).hexdigest()
+
+
+@needs_ref
+def test_oracle_equals_live_reference_pi_mesh(oracle_mod, tmp_path):
+    case = make_golden.pi_cases()["MUSCL_PPM_FCT"]
+    p = bundle.write_bundle(tmp_path / "pi", case, save_steps=[3])
+    bundle.run_ref(p)
+    out, _ = oracle_mod.Oracle(case).run(save_steps=[3])
+    assert np.array_equal(out[3], bundle.read_values(p, case, 3))
