@@ -20,6 +20,7 @@ LIB = PKG / "libfesomtra.so"
 KERNELS = "fesomtra_kernels.hip"
 TILE = "fesomtra_tile.hip"
 ABI = "fesomtra_abi.cpp"
+MESHFILE = "fesomtra_meshfile.cpp"
 HEADERS = ["fesomtra_types.hpp", "fesomtra_column.hpp", "fesomtra_kernels.hpp", "fesomtra_tile.hpp"]
 # The kernel source is compiled once per precision family (FESOMTRA_INST bit), in parallel.
 FAMILIES = {"f64": 1, "f32": 2, "f16": 4, "bf16": 8}
@@ -73,6 +74,9 @@ def build(verbose: bool = False, defines=(), out: Path = LIB) -> Path:
         objs.append(obj)
         obj = str(Path(td) / "abi.o")
         cmds.append([hipcc(), *HIPFLAGS, *inc, "-c", str(CSRC / ABI), "-o", obj])
+        objs.append(obj)
+        obj = str(Path(td) / "meshfile.o")
+        cmds.append([hipcc(), *HIPFLAGS, *inc, "-c", str(CSRC / MESHFILE), "-o", obj])
         objs.append(obj)
         _run(cmds, verbose)
         out = Path(out)

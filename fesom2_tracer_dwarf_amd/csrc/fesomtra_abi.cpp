@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/fesomtra.h"
+#include "fesomtra_meshfile.hpp"
 #include "fesomtra_kernels.hpp"
 #include "fesomtra_tile.hpp"
 
@@ -1382,7 +1383,24 @@ int tracer_load_mesh_partition(const char *restart_dir) {
 }
 
 int tracer_load_mesh_from_files(const char *mesh_path) {
-    return fail("tracer_load_mesh_from_files(%s): mesh files are not supported", mesh_path ? mesh_path : "");
+    if (!mesh_path) return fail("tracer_load_mesh_from_files: null path");
+    FileMesh fm;
+    std::string err;
+    if (read_mesh_dir(mesh_path, fm, err)) return fail("tracer_load_mesh_from_files(%s): %s", mesh_path, err.c_str());
+    // hand the arrays to the setters (their checks apply); one partition holds the whole mesh
+    if (tracer_set_mesh_dims(fm.nl, fm.N, fm.E, fm.T) || tracer_set_partition(fm.N, 0) ||
+        tracer_set_coordinates(fm.coord.data(), fm.N) ||
+        tracer_set_connectivity(fm.enodes.data(), fm.edges.data(), fm.edge_tri.data(), fm.nie.data(),
+                                fm.nie_num.data(), fm.max_adj, fm.N, fm.T, fm.E) ||
+        tracer_set_levels(fm.nlev_n.data(), fm.ulev_n.data(), fm.N) ||
+        tracer_set_elem_levels(fm.nlev_e.data(), fm.ulev_e.data(), fm.T) ||
+        tracer_set_geometry(fm.elem_area.data(), fm.area.data(), fm.areasvol.data(), fm.edxdy.data(), fm.ecd.data(),
+                            fm.elem_cos.data(), fm.gsca.data(), fm.nl, fm.N, fm.T, fm.E) ||
+        tracer_set_vertical_grid(fm.hnode.data(), fm.hnode.data(), fm.helem.data(), fm.Z.data(), fm.zbar3.data(), fm.nl,
+                                 fm.N, fm.T) ||
+        tracer_set_zbar(fm.zbar.data(), fm.nl) || tracer_set_muscl_support(fm.updn.data(), fm.nbl.data(), fm.N, fm.E))
+        return -1;
+    return 0;
 }
 
 int tracer_set_zbar(const double *zbar, int nl) {

@@ -62,6 +62,26 @@ def read_mesh_files(path) -> Dict[str, np.ndarray]:
     }
 
 
+def write_mesh_files(raw: Dict[str, np.ndarray], path) -> None:
+    """Writes arrays in the layout of :func:`read_mesh_files` as a FESOM2 mesh directory (full
+    double precision, so that reading it back gives the same numbers)."""
+    p = Path(path)
+    p.mkdir(parents=True, exist_ok=True)
+    N = len(raw["lon"])
+    lines = [str(N)] + [f"{int(i)} {float(x)!r} {float(y)!r} {int(c)}" for i, x, y, c in
+                        zip(raw["nod_index"], raw["lon"], raw["lat"], raw["coast"])]
+    p.joinpath("nod2d.out").write_text("\n".join(lines) + "\n")
+    el = np.asarray(raw["elem2D_nodes"])
+    p.joinpath("elem2d.out").write_text("\n".join([str(len(el))] + [" ".join(map(str, r)) for r in el.tolist()]) + "\n")
+    z = np.asarray(raw["zbar"])
+    p.joinpath("aux3d.out").write_text("\n".join([str(len(z))] + [repr(float(v)) for v in z]) + "\n")
+    p.joinpath("nlvls.out").write_text("\n".join(map(str, np.asarray(raw["nlevels_nod2D"]).tolist())) + "\n")
+    p.joinpath("elvls.out").write_text("\n".join(map(str, np.asarray(raw["nlevels"]).tolist())) + "\n")
+    p.joinpath("edgenum.out").write_text("\n".join(map(str, np.asarray(raw["edge_num"]).tolist())) + "\n")
+    for name, key in (("edges.out", "edges"), ("edge_tri.out", "edge_tri")):
+        p.joinpath(name).write_text("\n".join(" ".join(map(str, r)) for r in np.asarray(raw[key]).tolist()) + "\n")
+
+
 def euler_matrix(alpha: float = 50.0, beta: float = 15.0, gamma: float = -90.0) -> np.ndarray:
     """Rotation from geographic to model coordinates for Euler angles in degrees: first alpha about
     z, then beta about the new x, then gamma about the new z (set_mesh_transform_matrix,

@@ -64,15 +64,16 @@ class FESOMTracerAdvection:
         return self.lib.tracer_finalize_mpi()
 
     # --- array-based initialisation ---------------------------------------------------------------
-    def setup_case(self, case, upload: bool = True) -> None:
-        """Hand a generated case (mesh, velocities, tracers, schemes) to the library."""
+    def setup_case(self, case, upload: bool = True, mesh_files: Optional[str] = None) -> None:
+        """Hand a generated case (mesh, velocities, tracers, schemes) to the library.  With
+        ``mesh_files`` the library reads the mesh itself from that FESOM2 mesh directory
+        (tracer_load_mesh_from_files) instead of taking ``case.mesh``'s arrays; ``case.mesh`` must
+        be the same mesh (its sizes and element order index the velocities and tracers)."""
         L = self.lib
         m = case.mesh
         self.nl, self.nod2D, self.elem2D, self.edge2D = m.nl, m.nod2D, m.elem2D, m.edge2D
         self.num_tracers = case.ntracers
         abi.check(L.tracer_init_mpi(0), "tracer_init_mpi")
-        abi.check(L.tracer_set_mesh_dims(m.nl, m.nod2D, m.edge2D, m.elem2D), "tracer_set_mesh_dims")
-        abi.check(L.tracer_set_partition(m.nod2D, 0), "tracer_set_partition")
         self._keep = []
 
         def I(a):
@@ -85,20 +86,25 @@ class FESOMTracerAdvection:
             self._keep.append(a)
             return abi.dptr(a)
 
-        abi.check(L.tracer_set_coordinates(D(m.coord_nod2D), m.nod2D), "tracer_set_coordinates")
-        abi.check(L.tracer_set_connectivity(I(m.elem2D_nodes), I(m.edges), I(m.edge_tri),
-                                            I(m.nod_in_elem2D), I(m.nod_in_elem2D_num), m.max_adj,
-                                            m.nod2D, m.elem2D, m.edge2D), "tracer_set_connectivity")
-        abi.check(L.tracer_set_levels(I(m.nlevels_nod2D), I(m.ulevels_nod2D), m.nod2D), "tracer_set_levels")
-        abi.check(L.tracer_set_elem_levels(I(m.nlevels), I(m.ulevels), m.elem2D), "tracer_set_elem_levels")
-        abi.check(L.tracer_set_geometry(D(m.elem_area), D(m.area), D(m.areasvol), D(m.edge_dxdy),
-                                        D(m.edge_cross_dxdy), D(m.elem_cos), D(m.gradient_sca),
-                                        m.nl, m.nod2D, m.elem2D, m.edge2D), "tracer_set_geometry")
-        abi.check(L.tracer_set_vertical_grid(D(m.hnode), D(m.hnode_new), D(m.helem), D(m.Z_3d_n),
-                                             D(m.zbar_3d_n), m.nl, m.nod2D, m.elem2D),
-                  "tracer_set_vertical_grid")
-        abi.check(L.tracer_set_muscl_support(I(m.edge_up_dn_tri), I(m.nboundary_lay), m.nod2D,
-                                             m.edge2D), "tracer_set_muscl_support")
+        if mesh_files is not None:
+            abi.check(L.tracer_load_mesh_from_files(str(mesh_files).encode()), "tracer_load_mesh_from_files")
+        else:
+            abi.check(L.tracer_set_mesh_dims(m.nl, m.nod2D, m.edge2D, m.elem2D), "tracer_set_mesh_dims")
+            abi.check(L.tracer_set_partition(m.nod2D, 0), "tracer_set_partition")
+            abi.check(L.tracer_set_coordinates(D(m.coord_nod2D), m.nod2D), "tracer_set_coordinates")
+            abi.check(L.tracer_set_connectivity(I(m.elem2D_nodes), I(m.edges), I(m.edge_tri),
+                                                I(m.nod_in_elem2D), I(m.nod_in_elem2D_num), m.max_adj,
+                                                m.nod2D, m.elem2D, m.edge2D), "tracer_set_connectivity")
+            abi.check(L.tracer_set_levels(I(m.nlevels_nod2D), I(m.ulevels_nod2D), m.nod2D), "tracer_set_levels")
+            abi.check(L.tracer_set_elem_levels(I(m.nlevels), I(m.ulevels), m.elem2D), "tracer_set_elem_levels")
+            abi.check(L.tracer_set_geometry(D(m.elem_area), D(m.area), D(m.areasvol), D(m.edge_dxdy),
+                                            D(m.edge_cross_dxdy), D(m.elem_cos), D(m.gradient_sca),
+                                            m.nl, m.nod2D, m.elem2D, m.edge2D), "tracer_set_geometry")
+            abi.check(L.tracer_set_vertical_grid(D(m.hnode), D(m.hnode_new), D(m.helem), D(m.Z_3d_n),
+                                                 D(m.zbar_3d_n), m.nl, m.nod2D, m.elem2D),
+                      "tracer_set_vertical_grid")
+            abi.check(L.tracer_set_muscl_support(I(m.edge_up_dn_tri), I(m.nboundary_lay), m.nod2D,
+                                                 m.edge2D), "tracer_set_muscl_support")
         abi.check(L.tracer_set_velocity_elem(D(case.uv), D(case.w), D(case.wi), D(case.we), m.nl,
                                              m.nod2D, m.elem2D), "tracer_set_velocity_elem")
         abi.check(L.tracer_allocate_tracers(case.ntracers, m.nl, m.nod2D, 2), "tracer_allocate_tracers")

@@ -71,8 +71,14 @@ int tracer_set_velocity_zero(int nl, int nod2D, int edge2D);
 int tracer_set_velocity(const double *uv, const double *w, int nl, int nod2D, int edge2D);
 /* :749 Validates the mesh, reorders it along a space-filling curve and uploads it to HBM. */
 int tracer_init_complete(void);
-/* :769, :830 File-mesh readers: unsupported, return -1. */
+/* :769 Restart-format mesh partitions: unsupported, returns -1. */
 int tracer_load_mesh_partition(const char *restart_dir);
+/* :830 Reads a FESOM2 mesh directory (nod2d.out, elem2d.out, aux3d.out, nlvls.out, elvls.out,
+ *      edgenum.out, edges.out, edge_tri.out; one partition, no cavities), rotates the nodes to
+ *      the model grid (default Euler angles 50, 15, -90 degrees), orients the triangles and
+ *      derives the spherical geometry and the MUSCL support arrays as the reference's mesh set-up
+ *      does (longitude differences wrapped into [-pi, pi]); replaces tracer_set_mesh_dims and the
+ *      mesh setters.  Tracers, schemes and element velocities follow as with an array mesh. */
 int tracer_load_mesh_from_files(const char *mesh_path);
 /* :875 hnode = hnode_new = helem = zbar(k) - zbar(k+1) from tracer_set_zbar. */
 int tracer_init_ale_arrays(void);
