@@ -75,6 +75,10 @@ _SIGNATURES = {
     "tracer_get_tracers_per_pass": (_c_int, [_pi]),
     "tracer_set_graph": (_c_int, [_c_int]),
     "tracer_set_fused": (_c_int, [_c_int]),
+    "tracer_set_wsplit": (_c_int, [_c_int]),
+    "tracer_get_wsplit": (_c_int, [_pi]),
+    "tracer_set_ab_mode": (_c_int, [_c_int, ctypes.c_double]),
+    "tracer_get_ab_mode": (_c_int, [_pi, _pd]),
     "tracer_set_halo_tiles": (_c_int, [_c_int]),
     "tracer_get_halo_tiles": (_c_int, [_pi]),
     "tracer_set_rounding": (_c_int, [_c_int]),

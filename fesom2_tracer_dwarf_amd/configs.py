@@ -63,6 +63,7 @@ class Case:
     schemes: List[Scheme]
     dt: float
     nsteps: int
+    use_wsplit: bool = False  # dynamics%use_wsplit: w = we + wi, wi advected implicitly
 
     @property
     def ntracers(self) -> int:
@@ -158,6 +159,17 @@ def streamfunction_velocity(mesh: Mesh, umax: float,
         uv[:, k, 1] = np.where(on, vk * fac[k], 0.0)
     uv *= umax / np.max(np.hypot(uv[:, :, 0], uv[:, :, 1]))
     return uv
+
+
+def split_vertical_velocity(case: Case, cap: float) -> Case:
+    """The case with its vertical velocity split as FESOM2's w_split does: the explicit part
+    ``we = sign(w) * min(|w|, cap * max|w|)`` and the implicit rest ``wi = w - we``, with
+    dynamics%use_wsplit on (0 < cap < 1 leaves the strongest interfaces partly implicit)."""
+    w = np.asarray(case.w, dtype=np.float64)
+    lim = cap * float(np.abs(w).max())
+    we = np.sign(w) * np.minimum(np.abs(w), lim)
+    wi = w - we
+    return dataclasses.replace(case, we=we, wi=wi, use_wsplit=True)
 
 
 def cfl_dt(mesh: Mesh, umax: float, cfl: float = 0.5) -> float:

@@ -76,6 +76,13 @@ struct State {
     std::vector<int4> h_ne;
     // fused single-pass path (FP64, FCT, one tracer per block)
     bool fused = false;
+    bool wsplit = false;  // dynamics%use_wsplit
+    int ab_mode = 0;      // valuesAB per step: 0 := values, 1 frozen, 2 AB2 interpolation
+    double ab_eps = 0.1;  // AB2 offset epsilon
+    std::vector<void *> toldbuf;                    // per block: valuesold (ab_mode 2)
+    std::vector<std::vector<double>> ab_host, old_host;  // valuesAB / valuesold across rebuilds
+    std::vector<void *> ckpt_ab, ckpt_old;          // per block: copies kept by tracer_checkpoint
+    int ckpt_ab_mode = 0;
     bool tiles_built = false;
     // halo tiles of the node kernels (tracer blocks of width <= 4)
     bool halo_tiles = true;
@@ -91,6 +98,7 @@ struct State {
     void *adfh = nullptr, *LO = nullptr, *Rp = nullptr, *Rm = nullptr, *adfv = nullptr;
     double *acc_h = nullptr, *acc_v = nullptr;
     void *dbg_h = nullptr, *dbg_v = nullptr;
+    void *tri = nullptr;      // (c', t') of the implicit vertical solve (allocated with wsplit)
     double *stage = nullptr;  // reference-layout staging buffer (N * L doubles)
     int *d_perm_n = nullptr;
     double4 *part = nullptr;
@@ -176,6 +184,9 @@ void release_device(State &s) {
     s.tabbuf.clear();
     s.tab_on.clear();
     s.ckpt.clear();
+    s.toldbuf.clear();
+    s.ckpt_ab.clear();
+    s.ckpt_old.clear();
     s.dT2.clear();
     s.t2_sync.clear();
     s.tiles_built = false;
@@ -185,6 +196,7 @@ void release_device(State &s) {
     s.adfh = s.LO = s.Rp = s.Rm = s.adfv = nullptr;
     s.acc_h = s.acc_v = nullptr;
     s.dbg_h = s.dbg_v = nullptr;
+    s.tri = nullptr;
     s.stage = nullptr;
     s.d_perm_n = nullptr;
     s.part = nullptr;
@@ -381,6 +393,8 @@ int upload_mesh(State &s, MeshCol<M> &m, const std::vector<int> &ints_n1, const 
     if (upload(s, &m.w, level_major<M>(s.w, nl, s.perm_n, s.Np))) return -1;
     m.we = m.w;
     if (s.we != s.w && upload(s, &m.we, level_major<M>(s.we, nl, s.perm_n, s.Np))) return -1;
+    m.wi = m.w;
+    if (s.wi != s.w && upload(s, &m.wi, level_major<M>(s.wi, nl, s.perm_n, s.Np))) return -1;
     // elements
     std::vector<double> ea(s.T), ec(s.T);
     for (int t = 0; t < s.T; ++t) {
@@ -500,6 +514,55 @@ int build_edge_tiles(State &s) {
     s.et_built = nt;
     m.et_kb = s.kb;
     m.et_n = s.halo_tiles ? nt : 0;
+    return 0;
+}
+
+int pull_values(State &s);
+
+// A setter changed data the device copy is built from: the next use rebuilds it, with the
+// tracer values as last advanced on the device.
+int invalidate(State &s) {
+    if (s.on_device && pull_values(s)) return -1;
+    s.on_device = false;
+    return 0;
+}
+
+// New velocities while the device copy stands: uv, w, w_e and w_i are written into the device
+// arrays in place (w_e / w_i get arrays of their own when they first differ from w), so a caller
+// that changes the velocity every step keeps its tracers, tiles and captured graph.
+int refresh_velocity(State &s) {
+    if (!s.on_device) return 0;
+    MeshCol<double> &m = s.m64;
+    const int L = s.nl - 1, nl = s.nl;
+    std::vector<double> h((size_t)L * s.Tp * 2, 0.0);
+    for (int t = 0; t < s.T; ++t) {
+        const double *row = &s.uv[(size_t)s.perm_t[t] * L * 2];
+        for (int k = 0; k < L; ++k) {
+            h[((size_t)k * s.Tp + t) * 2] = row[2 * k];
+            h[((size_t)k * s.Tp + t) * 2 + 1] = row[2 * k + 1];
+        }
+    }
+    HIPCHK(hipStreamSynchronize(s.stream));
+    HIPCHK(hipMemcpy(const_cast<double *>(m.uv), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    const std::vector<double> w = level_major<double>(s.w, nl, s.perm_n, s.Np);
+    HIPCHK(hipMemcpy(const_cast<double *>(m.w), w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+    auto part = [&](const std::vector<double> &src, const double *&dst) -> int {
+        if (src == s.w) {
+            if (dst != m.w) s.gen++;  // kernels and graphs take the pointer
+            dst = m.w;
+            return 0;
+        }
+        const std::vector<double> d = level_major<double>(src, nl, s.perm_n, s.Np);
+        if (dst == m.w) {
+            double *q;
+            if (dalloc(s, &q, d.size())) return -1;
+            dst = q;
+            s.gen++;
+        }
+        HIPCHK(hipMemcpy(const_cast<double *>(dst), d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
+        return 0;
+    };
+    if (part(s.we, m.we) || part(s.wi, m.wi)) return -1;
     return 0;
 }
 
@@ -703,20 +766,65 @@ int from_device(State &s, const void *src, int lane, double *out, bool out_is_de
     return 0;
 }
 
+// valuesAB (and valuesold) storage of every block for ab_mode 1 / 2, set to the values, or to
+// the host copies kept across a rebuild where there are any.
+int ab_prepare(State &s) {
+    if (s.ab_mode == 0) return 0;
+    const size_t bytes = (size_t)(s.nl - 1) * s.Np * s.kb * storage_bytes(s.prec);
+    if (s.toldbuf.size() != s.dT.size()) s.toldbuf.assign(s.dT.size(), nullptr);
+    for (size_t b = 0; b < s.dT.size(); ++b) {
+        if (!s.tabbuf[b] && dalloc_bytes(s, &s.tabbuf[b], bytes)) return -1;
+        HIPCHK(hipMemcpyAsync(s.tabbuf[b], s.dT[b], bytes, hipMemcpyDeviceToDevice, s.stream));
+        if (s.ab_mode == 2) {
+            if (!s.toldbuf[b] && dalloc_bytes(s, &s.toldbuf[b], bytes)) return -1;
+            HIPCHK(hipMemcpyAsync(s.toldbuf[b], s.dT[b], bytes, hipMemcpyDeviceToDevice, s.stream));
+        }
+        s.tab_on[b] = 1;
+    }
+    for (int i = 0; i < s.ntr; ++i) {
+        const int b = i / s.kb, lane = i % s.kb;
+        if (i < (int)s.ab_host.size() && !s.ab_host[i].empty() &&
+            to_device(s, s.ab_host[i].data(), false, s.tabbuf[b], lane))
+            return -1;
+        if (s.ab_mode == 2 && i < (int)s.old_host.size() && !s.old_host[i].empty() &&
+            to_device(s, s.old_host[i].data(), false, s.toldbuf[b], lane))
+            return -1;
+    }
+    s.ab_host.clear();
+    s.old_host.clear();
+    HIPCHK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
 int ensure_device(State &s) {
     if (!s.initialized) return fail("tracer not initialized. Call tracer_init_complete first.");
     if (s.on_device) return 0;
     if (build_device(s)) return -1;
     for (int i = 0; i < s.ntr; ++i)
         if (to_device(s, s.values[i].data(), false, s.dT[i / s.kb], i % s.kb)) return -1;
-    return 0;
+    return ab_prepare(s);
 }
 
-// Values of every tracer back to the host copies (before the device storage is rebuilt).
+// Values of every tracer back to the host copies (before the device storage is rebuilt), with
+// valuesAB / valuesold when the time-level mode keeps them.
 int pull_values(State &s) {
     if (!s.on_device) return 0;
     for (int i = 0; i < s.ntr; ++i)
         if (from_device(s, s.dT[i / s.kb], i % s.kb, s.values[i].data(), false)) return -1;
+    s.ab_host.clear();
+    s.old_host.clear();
+    if (s.ab_mode != 0) {
+        const size_t n = (size_t)(s.nl - 1) * s.N;
+        s.ab_host.assign(s.ntr, std::vector<double>(n));
+        if (s.ab_mode == 2) s.old_host.assign(s.ntr, std::vector<double>(n));
+        for (int i = 0; i < s.ntr; ++i) {
+            const int b = i / s.kb, lane = i % s.kb;
+            if (s.tab_on[b] && from_device(s, s.tabbuf[b], lane, s.ab_host[i].data(), false)) return -1;
+            if (!s.tab_on[b]) s.ab_host[i] = s.values[i];
+            if (s.ab_mode == 2 && s.toldbuf[b] && from_device(s, s.toldbuf[b], lane, s.old_host[i].data(), false))
+                return -1;
+        }
+    }
     return 0;
 }
 
@@ -746,6 +854,7 @@ StepPtrs<S_, C_> make_ptrs(State &s, int blk, unsigned long long mask, bool drop
     p.Rm = static_cast<C_ *>(s.Rm);
     p.adfv = static_cast<C_ *>(s.adfv);
     p.mask = mask;
+    p.tri = static_cast<double2 *>(s.tri);
     if (dropin) {
         p.acc_h = s.acc_h;
         p.acc_v = s.acc_v;
@@ -757,7 +866,7 @@ StepPtrs<S_, C_> make_ptrs(State &s, int blk, unsigned long long mask, bool drop
 
 hipError_t launch_block(State &s, int blk, unsigned long long mask, const SchemeSel &sc, double dt, bool dropin,
                         hipEvent_t *ev, uint32_t seed = 0) {
-    SchemeArgs a{sc.hor, sc.ver, sc.fct, sc.ph, sc.pv, dt};
+    SchemeArgs a{sc.hor, sc.ver, sc.fct, sc.ph, sc.pv, dt, s.wsplit && sc.fct};
     switch (s.prec) {
     case PREC_F64: return launch_step<double, double, double>(s.m64, make_ptrs<double, double>(s, blk, mask, dropin, seed), s.kb, a, s.cfg, s.stream, ev);
     case PREC_F32: return launch_step<float, double, float>(s.m64, make_ptrs<float, float>(s, blk, mask, dropin, seed), s.kb, a, s.cfg, s.stream, ev);
@@ -911,9 +1020,16 @@ int build_tiles(State &s) {
     return 0;
 }
 
-// The fused path applies to FP64 blocks of one tracer with the FCT limiter.
+// The fused path applies to FP64 blocks of one tracer with the FCT limiter (without the
+// implicit vertical split, with valuesAB := values).
 bool fused_applies(const State &s, const SchemeSel &sc) {
-    return s.fused && s.prec == PREC_F64 && s.kb == 1 && sc.fct;
+    return s.fused && !s.wsplit && s.ab_mode == 0 && s.prec == PREC_F64 && s.kb == 1 && sc.fct;
+}
+
+// Scratch of the implicit vertical solve: one (c', t') pair per layer and lane.
+int ensure_wsplit(State &s) {
+    if (!s.wsplit || s.tri) return 0;
+    return dalloc_bytes(s, &s.tri, (size_t)(s.nl - 1) * s.Np * s.kb * sizeof(double2));
 }
 
 int ensure_fused(State &s) {
@@ -939,6 +1055,17 @@ void touch_values(State &s) { std::fill(s.t2_sync.begin(), s.t2_sync.end(), 0); 
 uint32_t next_seed(State &s) {
     if (++s.step_seed == 0u) s.step_seed = 1u;
     return s.step_seed;
+}
+
+// valuesAB / valuesold of a block from its values (FESOM2's AB2 interpolation, before a step).
+hipError_t launch_ab2_block(State &s, int b) {
+    const size_t n = (size_t)(s.nl - 1) * s.Np * s.kb;
+    switch (s.prec) {
+    case PREC_F64: return launch_ab2<double>((const double *)s.dT[b], (double *)s.tabbuf[b], (double *)s.toldbuf[b], n, s.ab_eps, s.stream);
+    case PREC_F32: return launch_ab2<float>((const float *)s.dT[b], (float *)s.tabbuf[b], (float *)s.toldbuf[b], n, s.ab_eps, s.stream);
+    case PREC_F32_F16: return launch_ab2<f16_t>((const f16_t *)s.dT[b], (f16_t *)s.tabbuf[b], (f16_t *)s.toldbuf[b], n, s.ab_eps, s.stream);
+    default: return launch_ab2<bf16_t>((const bf16_t *)s.dT[b], (bf16_t *)s.tabbuf[b], (bf16_t *)s.toldbuf[b], n, s.ab_eps, s.stream);
+    }
 }
 
 // One launch sequence per (block, scheme): the lanes of a block that share a scheme.
@@ -970,6 +1097,9 @@ hipError_t run_steps(State &s, const std::vector<Launch> &plan, int nsteps, doub
     for (int st = 0; st < nsteps; ++st) {
         // edge transport of the velocity, shared by every tracer pass of the step
         hipError_t e = launch_qflux(s.m64, s.stream);
+        if (e != hipSuccess) return e;
+        if (s.ab_mode == 2)
+            for (size_t b = 0; b < s.dT.size() && e == hipSuccess; ++b) e = launch_ab2_block(s, (int)b);
         if (e != hipSuccess) return e;
         const uint32_t seed = next_seed(s);
         for (size_t gi = 0; gi < plan.size(); ++gi) {
@@ -1024,7 +1154,8 @@ int capture_graph(State &s, const std::vector<Launch> &plan, int nsteps, double 
 int advect(State &s, int nsteps, double dt) {
     if (ensure_device(s)) return -1;
     if (s.ntr == 0) return fail("no tracers allocated (tracer_allocate_tracers)");
-    std::fill(s.tab_on.begin(), s.tab_on.end(), 0);  // valuesAB := values
+    if (ensure_wsplit(s)) return -1;
+    if (s.ab_mode == 0) std::fill(s.tab_on.begin(), s.tab_on.end(), 0);  // valuesAB := values
     const std::vector<Launch> plan = launch_plan(s);
     if (s.fused && s.prec == PREC_F64 && s.kb == 1 && ensure_fused(s)) return -1;
     if (!plan.empty()) s.last_fct = plan.back().sc.fct;
@@ -1246,7 +1377,7 @@ int tracer_set_levels(const int *nlevels, const int *ulevels, int nod2D) {
     s.nlev_n.assign(nlevels, nlevels + nod2D);
     s.ulev_n.assign(ulevels, ulevels + nod2D);
     s.have_nlev = true;
-    s.on_device = false;
+    if (invalidate(s)) return -1;
     return 0;
 }
 
@@ -1257,7 +1388,7 @@ int tracer_set_elem_levels(const int *nlevels, const int *ulevels, int elem2D) {
     s.nlev_e.assign(nlevels, nlevels + elem2D);
     s.ulev_e.assign(ulevels, ulevels + elem2D);
     s.have_elev = true;
-    s.on_device = false;
+    if (invalidate(s)) return -1;
     return 0;
 }
 
@@ -1267,7 +1398,7 @@ int tracer_set_thickness(const double *hnode, int nl, int nod2D) {
     s.hnode.assign(hnode, hnode + (size_t)(nl - 1) * nod2D);
     s.hnode_new = s.hnode;
     s.have_hnode = true;
-    s.on_device = false;
+    if (invalidate(s)) return -1;
     return 0;
 }
 
@@ -1297,11 +1428,16 @@ int tracer_set_values(int tracer_id, const double *values, int nl, int nod2D) {
     if (check_dims(s, nl, nod2D, "tracer_set_values")) return -1;
     std::vector<double> &v = s.values[tracer_id - 1];
     v.assign(values, values + (size_t)(nl - 1) * nod2D);
-    if (s.on_device) {  // values, and valuesAB where held apart (lib/tracer_c_interface.F90:613-648)
+    // values, and valuesAB / valuesold where held apart (lib/tracer_c_interface.F90:613-648)
+    if (s.on_device) {
         const int b = (tracer_id - 1) / s.kb, lane = (tracer_id - 1) % s.kb;
         if (to_device(s, v.data(), false, s.dT[b], lane)) return -1;
         if (s.tab_on[b] && to_device(s, v.data(), false, s.tabbuf[b], lane)) return -1;
+        if (s.ab_mode == 2 && s.toldbuf[b] && to_device(s, v.data(), false, s.toldbuf[b], lane)) return -1;
         touch_values(s);
+    } else {
+        if (tracer_id - 1 < (int)s.ab_host.size()) s.ab_host[tracer_id - 1].clear();
+        if (tracer_id - 1 < (int)s.old_host.size()) s.old_host[tracer_id - 1].clear();
     }
     return 0;
 }
@@ -1330,8 +1466,7 @@ int tracer_set_velocity_zero(int nl, int nod2D, int edge2D) {
     s.wi = s.w;
     s.we = s.w;
     s.have_vel = true;
-    s.on_device = false;
-    return 0;
+    return refresh_velocity(s);
 }
 
 int tracer_set_velocity(const double *uv, const double *w, int nl, int nod2D, int edge2D) {
@@ -1346,8 +1481,7 @@ int tracer_set_velocity(const double *uv, const double *w, int nl, int nod2D, in
     s.wi = s.w;
     s.we = s.w;
     s.have_vel = true;
-    s.on_device = false;
-    return 0;
+    return refresh_velocity(s);
 }
 
 int tracer_set_velocity_elem(const double *uv, const double *w, const double *w_i, const double *w_e,
@@ -1360,8 +1494,7 @@ int tracer_set_velocity_elem(const double *uv, const double *w, const double *w_
     s.wi = w_i ? std::vector<double>(w_i, w_i + (size_t)nl * nod2D) : s.w;
     s.we = w_e ? std::vector<double>(w_e, w_e + (size_t)nl * nod2D) : s.w;
     s.have_vel = true;
-    s.on_device = false;
-    return 0;
+    return refresh_velocity(s);
 }
 
 int tracer_init_complete(void) {
@@ -1432,7 +1565,7 @@ int tracer_init_ale_arrays(void) {
     s.hnode_new = s.hnode;
     s.have_hnode = true;
     s.have_vert = true;
-    s.on_device = false;
+    if (invalidate(s)) return -1;
     return 0;
 }
 
@@ -1447,7 +1580,7 @@ int tracer_set_coordinates(const double *coord_nod2D, int nod2D) {
     if (!s.dims || nod2D != s.N) return fail("tracer_set_coordinates: nod2D=%d does not match", nod2D);
     s.coord.assign(coord_nod2D, coord_nod2D + 2 * (size_t)nod2D);
     s.have_coord = true;
-    s.on_device = false;
+    if (invalidate(s)) return -1;
     return 0;
 }
 
@@ -1487,7 +1620,7 @@ int tracer_set_connectivity(const int *elem2D_nodes, const int *edges, const int
         }
     }
     s.have_conn = true;
-    s.on_device = false;
+    if (invalidate(s)) return -1;
     return 0;
 }
 
@@ -1506,7 +1639,7 @@ int tracer_set_geometry(const double *elem_area, const double *area, const doubl
     s.elem_cos.assign(elem_cos, elem_cos + elem2D);
     s.gsca.assign(gradient_sca, gradient_sca + (size_t)6 * elem2D);
     s.have_geom = true;
-    s.on_device = false;
+    if (invalidate(s)) return -1;
     return 0;
 }
 
@@ -1524,7 +1657,7 @@ int tracer_set_vertical_grid(const double *hnode, const double *hnode_new, const
     s.zbar3.assign(zbar_3d_n, zbar_3d_n + (size_t)nl * nod2D);
     s.have_hnode = true;
     s.have_vert = true;
-    s.on_device = false;
+    if (invalidate(s)) return -1;
     return 0;
 }
 
@@ -1539,7 +1672,7 @@ int tracer_set_muscl_support(const int *edge_up_dn_tri, const int *nboundary_lay
     }
     s.nbl.assign(nboundary_lay, nboundary_lay + nod2D);
     s.have_muscl = true;
-    s.on_device = false;
+    if (invalidate(s)) return -1;
     return 0;
 }
 
@@ -1605,20 +1738,75 @@ int tracer_checkpoint(int restore) {
     const size_t bytes = (size_t)(s.nl - 1) * s.Np * s.kb * storage_bytes(s.prec);
     if (!restore) {
         if (s.ckpt.size() != s.dT.size()) s.ckpt.assign(s.dT.size(), nullptr);
+        if (s.ckpt_ab.size() != s.dT.size()) s.ckpt_ab.assign(s.dT.size(), nullptr);
+        if (s.ckpt_old.size() != s.dT.size()) s.ckpt_old.assign(s.dT.size(), nullptr);
         for (size_t b = 0; b < s.dT.size(); ++b) {
             if (!s.ckpt[b] && dalloc_bytes(s, &s.ckpt[b], bytes)) return -1;
             HIPCHK(hipMemcpyAsync(s.ckpt[b], s.dT[b], bytes, hipMemcpyDeviceToDevice, s.stream));
+            if (s.ab_mode != 0) {  // the time levels the next steps read
+                if (!s.ckpt_ab[b] && dalloc_bytes(s, &s.ckpt_ab[b], bytes)) return -1;
+                HIPCHK(hipMemcpyAsync(s.ckpt_ab[b], s.tabbuf[b], bytes, hipMemcpyDeviceToDevice, s.stream));
+                if (s.ab_mode == 2) {
+                    if (!s.ckpt_old[b] && dalloc_bytes(s, &s.ckpt_old[b], bytes)) return -1;
+                    HIPCHK(hipMemcpyAsync(s.ckpt_old[b], s.toldbuf[b], bytes, hipMemcpyDeviceToDevice, s.stream));
+                }
+            }
         }
         s.ckpt_seed = s.step_seed;
+        s.ckpt_ab_mode = s.ab_mode;
     } else {
         if (s.ckpt.size() != s.dT.size()) return fail("tracer_checkpoint: no checkpoint of the current storage");
-        for (size_t b = 0; b < s.dT.size(); ++b)
+        if (s.ckpt_ab_mode != s.ab_mode) return fail("tracer_checkpoint: the valuesAB mode changed since the checkpoint");
+        for (size_t b = 0; b < s.dT.size(); ++b) {
             HIPCHK(hipMemcpyAsync(s.dT[b], s.ckpt[b], bytes, hipMemcpyDeviceToDevice, s.stream));
-        std::fill(s.tab_on.begin(), s.tab_on.end(), 0);
+            if (s.ab_mode != 0)
+                HIPCHK(hipMemcpyAsync(s.tabbuf[b], s.ckpt_ab[b], bytes, hipMemcpyDeviceToDevice, s.stream));
+            if (s.ab_mode == 2)
+                HIPCHK(hipMemcpyAsync(s.toldbuf[b], s.ckpt_old[b], bytes, hipMemcpyDeviceToDevice, s.stream));
+        }
+        if (s.ab_mode == 0) std::fill(s.tab_on.begin(), s.tab_on.end(), 0);
         touch_values(s);
         s.step_seed = s.ckpt_seed;  // the restored state continues the same rounding sequence
     }
     HIPCHK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+int tracer_set_ab_mode(int mode, double epsilon) {
+    State &s = S();
+    if (mode < 0 || mode > 2) return fail("tracer_set_ab_mode: unknown mode %d", mode);
+    if (!(epsilon >= 0.0 && epsilon <= 1.0)) return fail("tracer_set_ab_mode: epsilon=%g outside [0, 1]", epsilon);
+    if (mode == s.ab_mode && epsilon == s.ab_eps) return 0;
+    s.ab_mode = mode;
+    s.ab_eps = epsilon;
+    s.ab_host.clear();
+    s.old_host.clear();
+    s.gen++;
+    if (!s.on_device) return 0;
+    if (mode == 0) {
+        std::fill(s.tab_on.begin(), s.tab_on.end(), 0);
+        return 0;
+    }
+    return ab_prepare(s);  // valuesAB (and valuesold) := values
+}
+
+int tracer_get_ab_mode(int *mode, double *epsilon) {
+    const State &s = S();
+    if (mode) *mode = s.ab_mode;
+    if (epsilon) *epsilon = s.ab_eps;
+    return 0;
+}
+
+int tracer_set_wsplit(int enable) {
+    State &s = S();
+    s.wsplit = enable != 0;
+    s.gen++;
+    return 0;
+}
+
+int tracer_get_wsplit(int *enabled) {
+    if (!enabled) return fail("tracer_get_wsplit: null pointer");
+    *enabled = S().wsplit ? 1 : 0;
     return 0;
 }
 
@@ -1700,7 +1888,7 @@ int tracer_do_oce_adv_tra(double dt, int tracer_id) {
     State &s = S();
     if (!s.initialized) return fail("tracer not initialized.");
     if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
-    if (ensure_device(s) || ensure_dropin_buffers(s)) return -1;
+    if (ensure_device(s) || ensure_dropin_buffers(s) || ensure_wsplit(s)) return -1;
     const size_t cb = compute_bytes(s.prec), kb = s.kb;
     const size_t NLn = (size_t)(s.nl - 1) * s.Np, ELn = (size_t)(s.nl - 1) * s.Ep, NIn = (size_t)s.nl * s.Np;
     // the reference zeroes its flux and low-order work arrays at every call
@@ -1759,6 +1947,8 @@ int tracer_set_values_device(int tracer_id, const void *values, int nl, int nod2
     const int b = (tracer_id - 1) / s.kb, lane = (tracer_id - 1) % s.kb;
     if (to_device(s, static_cast<const double *>(values), true, s.dT[b], lane)) return -1;
     if (s.tab_on[b] && to_device(s, static_cast<const double *>(values), true, s.tabbuf[b], lane)) return -1;
+    if (s.ab_mode == 2 && s.toldbuf[b] && to_device(s, static_cast<const double *>(values), true, s.toldbuf[b], lane))
+        return -1;
     touch_values(s);
     return 0;
 }

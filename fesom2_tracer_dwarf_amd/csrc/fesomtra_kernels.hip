@@ -410,9 +410,12 @@ __device__ __forceinline__ void for_edges_ld(const MeshCol<M> &m, const NodeEdge
 // the high-order vertical call :266-282 with o_init_zero=.false., pwvel = w).  Interface fluxes
 // are evaluated top-down in double; layer nz uses the fluxes at its top (nz) and bottom (nz + 1).
 // The upwind edge fluxes are formed here in double from q and the two end values (never stored);
-// TILED stages the neighbours' values per level in LDS.
+// TILED stages the neighbours' values per level in LDS.  The low-order vertical flux uses w_e;
+// with wsplit (:221-231) the flux subtracted from the high-order one is the upwind flux of the
+// full w instead, and k_vimpl then adds the implicit part to the stored LO.
 template <bool TILED, int VER, int KB, typename S, typename M, typename C>
-__global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt, int lc) {
+__global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt, int lc,
+                                                                 bool wsplit) {
     using D = double;
     __shared__ D sT[TILED ? 2 : 1][TILED ? kHtSlots : 1];
     HaloTile<KB, M> ht;
@@ -458,9 +461,11 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
 #pragma unroll
         for (int i = 0; i < 4; ++i) tw[i] = nc.lay(p.TAB, j - 2 + i);
         tcur = nc.lay(p.T, j);
-        lo_top = ver_upw1_w<D>(j, nzmax, nc.lay(p.T, j - 1), tcur, wej, aj, D(0));
+        const D tup = nc.lay(p.T, j - 1);
+        lo_top = ver_upw1_w<D>(j, nzmax, tup, tcur, wej, aj, D(0));
+        const D fin_top = wsplit ? ver_upw1_w<D>(j, nzmax, tup, tcur, wj, aj, D(0)) : lo_top;
         if (VER == VER_QR4C) qw.init(tw, z);
-        adf_top = ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, lo_top, qw.q);
+        adf_top = ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, fin_top, qw.q);
     }
     D st[2];
     if constexpr (TILED)
@@ -494,7 +499,8 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         if (VER == VER_QR4C) qw.slide(tw, z);
         const D tnext = nc.lay(p.T, j);
         const D lo_bot = ver_upw1_w<D>(j, nzmax, tcur, tnext, wej, aj, D(0));
-        const D adf_bot = ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, lo_bot, qw.q);
+        const D fin_bot = wsplit ? ver_upw1_w<D>(j, nzmax, tcur, tnext, wj, aj, D(0)) : lo_bot;
+        const D adf_bot = ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, fin_bot, qw.q);
         // upwind edge fluxes (lib/oce_adv_tra_hor.F90:178-181) from q and the two end values,
         // summed in reference edge order
         double s = 0.0;
@@ -525,6 +531,85 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         lo_top = lo_bot;
         adf_top = adf_bot;
         tcur = tnext;
+    }
+}
+
+// Implicit first-order upwind vertical advection of the low-order solution with the implicit
+// part w_i of the split vertical velocity (lib/oce_adv_tra_ver.F90:90-244, called from
+// lib/oce_adv_tra_driver.F90:221-226 when dynamics%use_wsplit): per column and lane, the
+// tridiagonal system on hnode_new is built and eliminated top-down in double, storing (c', t')
+// in p.tri, then back-substituted bottom-up and added to LO.  Operation order as the reference
+// (FP64 results are bit-identical).  Columns have at least two wet layers (checked at setup).
+template <int KB, typename S, typename M, typename C>
+__global__ __launch_bounds__(256) void k_vimpl(const MeshCol<M> m, const StepPtrs<S, C> p, double dt) {
+    int n, lane;
+    column_lane<KB>(n, lane);
+    if (n >= m.N || !lane_on(p.mask, lane)) return;
+    const int L = cld(m.nlev_n, n) - 1;
+    if (L < 2) return;
+    const size_t Np = m.Np;
+    auto at = [&](int nz) { return ((size_t)(nz - 1) * Np + n) * KB + lane; };
+    // LO in double (single precision stores the increment LO - T)
+    auto lo_at = [&](int nz) -> double {
+        if constexpr (std::is_same<C, double>::value) return p.LO[at(nz)];
+        else return ldc<double>(p.T, at(nz)) + (double)p.LO[at(nz)];
+    };
+    auto lo_put = [&](int nz, double v) {
+        if constexpr (std::is_same<C, double>::value) p.LO[at(nz)] = v;
+        else p.LO[at(nz)] = (C)(v - ldc<double>(p.T, at(nz)));
+    };
+    const double zinv = 1.0 * dt;
+    double t0 = 0.0, t1 = lo_at(1), t2 = lo_at(2), cpp = 0.0, tpp = 0.0;
+    double w1 = (double)cld(m.wi, (size_t)n), a1 = (double)cld(m.area, (size_t)n);
+    for (int nz = 1; nz <= L; ++nz) {
+        const size_t cell = (size_t)(nz - 1) * Np + n;
+        const double hn = (double)cld(m.hnode_new, cell), V = (double)cld(m.areasvol, cell);
+        double a, b, c, v = zinv * a1 / V;
+        if (nz == 1) {
+            a = 0.0;
+            b = hn + w1 * v;
+        } else {
+            a = cmin(0.0, w1) * v;
+            b = hn + cmax(0.0, w1) * v;
+        }
+        double w2 = 0.0, a2 = 0.0;
+        if (nz < L) {
+            w2 = (double)cld(m.wi, cell + Np);
+            a2 = (double)cld(m.area, cell + Np);
+            v = zinv * a2 / V;
+            b = b - cmin(0.0, w2) * v;
+            c = -cmax(0.0, w2) * v;
+        } else {
+            c = 0.0;
+        }
+        double tr;
+        if (nz == 1) tr = -(b - hn) * t1 - c * t2;
+        else if (nz < L) tr = -a * t0 - (b - hn) * t1 - c * t2;
+        else tr = -a * t0 - (b - hn) * t1;
+        double cp, tp;
+        if (nz == 1) {
+            cp = c / b;
+            tp = tr / b;
+        } else {
+            const double mm = b - cpp * a;
+            cp = c / mm;
+            tp = (tr - tpp * a) / mm;
+        }
+        if (nz < L) p.tri[at(nz)] = make_double2(cp, tp);
+        cpp = cp;
+        tpp = tp;
+        w1 = w2;
+        a1 = a2;
+        t0 = t1;
+        t1 = t2;
+        t2 = nz + 2 <= L ? lo_at(nz + 2) : 0.0;
+    }
+    double x = tpp;
+    lo_put(L, t0 + x);
+    for (int nz = L - 1; nz >= 1; --nz) {
+        const double2 ct = p.tri[at(nz)];
+        x = ct.y - ct.x * x;
+        lo_put(nz, lo_at(nz) + x);
     }
 }
 
@@ -936,6 +1021,21 @@ __global__ void k_relayout(const double *__restrict__ ref_in, S *__restrict__ de
     else ref_out[(size_t)o * L + k] = ldc<double>(dev_in, d);
 }
 
+// FESOM2's second-order Adams-Bashforth interpolation of a tracer block before a step:
+// valuesAB = -(0.5+eps)*valuesold + (1.5+eps)*values, then valuesold := values (eps of
+// lib/oce_modules.F90:75, valuesold of lib/MOD_TRACER.F90:13; c0 = 0.5+eps, c1 = 1.5+eps).  In
+// double, stored at the storage precision (FP64: the Fortran expression's rounding exactly).
+template <typename S>
+__global__ __launch_bounds__(256) void k_ab2(const S *__restrict__ T, S *__restrict__ AB, S *__restrict__ old, size_t n,
+                                             double c0, double c1) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double t = ldc<double>(T, i), o = ldc<double>(old, i);
+        std_(AB, i, -c0 * o + c1 * t);
+        old[i] = T[i];
+    }
+}
+
 // Stats over one tracer (lane of a block): per-block partial min / max / sum / volume sum over
 // wet layers (or all layers).
 template <typename S, typename M>
@@ -1037,13 +1137,18 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
             if constexpr (KB <= 4) {
                 const dim3 tl = tgrid(cfg.lc_lo);
                 switch (sc.ver) {
-                case VER_UPW1: k_lo<true, VER_UPW1, KB><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-                case VER_QR4C: k_lo<true, VER_QR4C, KB><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-                case VER_CDIFF: k_lo<true, VER_CDIFF, KB><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-                case VER_PPM: k_lo<true, VER_PPM, KB><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+                case VER_UPW1: k_lo<true, VER_UPW1, KB><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+                case VER_QR4C: k_lo<true, VER_QR4C, KB><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+                case VER_CDIFF: k_lo<true, VER_CDIFF, KB><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+                case VER_PPM: k_lo<true, VER_PPM, KB><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
                 default: return hipErrorInvalidValue;
                 }
                 LAUNCH_CHECK();
+                if (sc.wsplit) {
+                    if (!p.tri) return hipErrorInvalidValue;
+                    k_vimpl<KB><<<dim3(grid_for((size_t)m.N * KB, 256)), 256, 0, s>>>(m, p, sc.dt);
+                    LAUNCH_CHECK();
+                }
                 MARK(2);
                 k_bounds<true, KB><<<tgrid(cfg.lc_bounds), 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
                 LAUNCH_CHECK();
@@ -1052,13 +1157,18 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
             }
         } else {
             switch (sc.ver) {
-            case VER_UPW1: k_lo<false, VER_UPW1, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-            case VER_QR4C: k_lo<false, VER_QR4C, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-            case VER_CDIFF: k_lo<false, VER_CDIFF, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-            case VER_PPM: k_lo<false, VER_PPM, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+            case VER_UPW1: k_lo<false, VER_UPW1, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+            case VER_QR4C: k_lo<false, VER_QR4C, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+            case VER_CDIFF: k_lo<false, VER_CDIFF, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+            case VER_PPM: k_lo<false, VER_PPM, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
             default: return hipErrorInvalidValue;
             }
             LAUNCH_CHECK();
+            if (sc.wsplit) {
+                if (!p.tri) return hipErrorInvalidValue;
+                k_vimpl<KB><<<dim3(grid_for((size_t)m.N * KB, 256)), 256, 0, s>>>(m, p, sc.dt);
+                LAUNCH_CHECK();
+            }
             MARK(2);
             k_bounds<false, KB><<<gb, 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
             LAUNCH_CHECK();
@@ -1118,6 +1228,14 @@ hipError_t launch_relayout(const double *ref_in, S *dev_out, const S *dev_in, do
     return hipGetLastError();
 }
 
+template <typename S>
+hipError_t launch_ab2(const S *T, S *AB, S *old, size_t n, double eps, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, (size_t)kXcds * 2048);
+    k_ab2<S><<<dim3((unsigned)blocks), 256, 0, s>>>(T, AB, old, n, 0.5 + eps, 1.5 + eps);
+    return hipGetLastError();
+}
+
 template <typename S, typename M>
 hipError_t launch_stats(const MeshCol<M> &m, const S *T, int kb, int lane, int all_levels, double4 *part,
                         int nblocks, hipStream_t s) {
@@ -1140,7 +1258,8 @@ hipError_t launch_stats(const MeshCol<M> &m, const S *T, int kb, int lane, int a
     template hipError_t launch_relayout<S_>(const double *, S_ *, const S_ *, double *, const int *, int, \
                                             int, size_t, int, int, int, hipStream_t);                      \
     template hipError_t launch_stats<S_, M_>(const MeshCol<M_> &, const S_ *, int, int, int, double4 *,    \
-                                             int, hipStream_t);
+                                             int, hipStream_t);                                              \
+    template hipError_t launch_ab2<S_>(const S_ *, S_ *, S_ *, size_t, double, hipStream_t);
 #if FESOMTRA_INST & 1
 FESOMTRA_INSTANTIATE(double, double, double)
 template hipError_t launch_qflux<double>(const MeshCol<double> &, hipStream_t);

@@ -6,9 +6,10 @@
 namespace fesomtra {
 
 struct SchemeArgs {
-    int hor, ver;       // Hor / Ver codes
-    bool fct;           // limiter "FCT" (else "NONE")
-    double ph, pv, dt;  // tra_adv_ph, tra_adv_pv, time step
+    int hor, ver;         // Hor / Ver codes
+    bool fct;             // limiter "FCT" (else "NONE")
+    double ph, pv, dt;    // tra_adv_ph, tra_adv_pv, time step
+    bool wsplit = false;  // dynamics%use_wsplit (FCT only): implicit low-order part with w_i
 };
 
 // Levels per thread of each kernel (the column of a node / edge is split into chunks of lc
@@ -32,6 +33,11 @@ hipError_t launch_step(const MeshCol<M> &m, const StepPtrs<S, C> &p, int kb, con
 template <typename S>
 hipError_t launch_relayout(const double *ref_in, S *dev_out, const S *dev_in, double *ref_out,
                            const int *perm, int N, int L, size_t Np, int kb, int lane, int dir, hipStream_t s);
+
+// valuesAB = -(0.5+eps)*valuesold + (1.5+eps)*values and valuesold := values over n entries of a
+// tracer block (FESOM2's AB2 interpolation).
+template <typename S>
+hipError_t launch_ab2(const S *T, S *AB, S *old, size_t n, double eps, hipStream_t s);
 
 template <typename S, typename M>
 hipError_t launch_stats(const MeshCol<M> &m, const S *T, int kb, int lane, int all_levels, double4 *part,

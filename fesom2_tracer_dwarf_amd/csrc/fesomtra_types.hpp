@@ -160,7 +160,7 @@ struct MeshCol {
     const int *nie, *nie_num;             // nod_in_elem2D (N, max_adj), reference list order
     const M *hnode, *hnode_new, *Z;       // [L][Np]
     const M *area, *areasvol, *zbar;      // [nl][Np]
-    const M *w, *we;                      // [nl][Np]
+    const M *w, *we, *wi;                 // [nl][Np] w, its explicit and implicit parts
     // elements
     const int *enodes, *nlev_e;           // (T, 3), (T)
     const M *elem_area, *elem_cos, *gsca; // (T), (T), (T, 6)
@@ -185,6 +185,7 @@ struct StepPtrs {
     C *adfv;               // [nl][Np][KB] antidiffusive (or high-order) vertical flux
     double *acc_h, *acc_v; // do_oce_adv_tra accumulators [L][Np] of the single selected lane, or null
     C *dbg_h, *dbg_v;      // limited fluxes [L][Ep], [nl][Np] of the single selected lane, or null
+    double2 *tri;          // [L][Np][KB] (c', t') of the implicit vertical solve, or null
     unsigned long long mask;  // lanes advanced by this launch (tracers sharing its scheme)
     int tid0;                 // tracer number of lane 0 (seeds the stochastic rounding)
     uint32_t seed;            // step counter (seeds the stochastic rounding); 0: round to nearest

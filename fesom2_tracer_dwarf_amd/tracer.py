@@ -107,6 +107,7 @@ class FESOMTracerAdvection:
                                                  m.edge2D), "tracer_set_muscl_support")
         abi.check(L.tracer_set_velocity_elem(D(case.uv), D(case.w), D(case.wi), D(case.we), m.nl,
                                              m.nod2D, m.elem2D), "tracer_set_velocity_elem")
+        abi.check(L.tracer_set_wsplit(int(bool(getattr(case, "use_wsplit", False)))), "tracer_set_wsplit")
         abi.check(L.tracer_allocate_tracers(case.ntracers, m.nl, m.nod2D, 2), "tracer_allocate_tracers")
         for i in range(case.ntracers):
             abi.check(L.tracer_set_values(i + 1, D(case.tracers[i]), m.nl, m.nod2D), "tracer_set_values")
@@ -138,6 +139,24 @@ class FESOMTracerAdvection:
     def set_fused(self, enable: bool) -> None:
         """Single fused kernel per step for FP64 FCT tracers in blocks of one."""
         abi.check(self.lib.tracer_set_fused(int(bool(enable))), "tracer_set_fused")
+
+    def set_wsplit(self, enable: bool) -> None:
+        """dynamics%use_wsplit: implicit low-order vertical part with w_i (FCT tracers)."""
+        abi.check(self.lib.tracer_set_wsplit(int(bool(enable))), "tracer_set_wsplit")
+
+    def wsplit(self) -> bool:
+        v = ctypes.c_int()
+        abi.check(self.lib.tracer_get_wsplit(ctypes.byref(v)), "tracer_get_wsplit")
+        return bool(v.value)
+
+    def set_ab_mode(self, mode: int, epsilon: float = 0.1) -> None:
+        """valuesAB per step: 0 := values, 1 frozen (the reference's drivers), 2 FESOM2's AB2."""
+        abi.check(self.lib.tracer_set_ab_mode(int(mode), float(epsilon)), "tracer_set_ab_mode")
+
+    def ab_mode(self):
+        mode, eps = ctypes.c_int(), ctypes.c_double()
+        abi.check(self.lib.tracer_get_ab_mode(ctypes.byref(mode), ctypes.byref(eps)), "tracer_get_ab_mode")
+        return mode.value, eps.value
 
     def set_halo_tiles(self, enable: bool) -> None:
         """LDS halo tiles for the FCT node kernels of blocks of width <= 4 (default on)."""
@@ -177,6 +196,19 @@ class FESOMTracerAdvection:
 
     def do_oce_adv_tra(self, dt: float, tracer_id: int) -> None:
         abi.check(self.lib.tracer_do_oce_adv_tra(dt, tracer_id), "tracer_do_oce_adv_tra")
+
+    def set_velocity(self, uv: np.ndarray, w: np.ndarray, wi: Optional[np.ndarray] = None,
+                     we: Optional[np.ndarray] = None) -> None:
+        """Element velocities uv (elem2D, nl-1, 2) and w, w_i, w_e (nod2D, nl); the device copy is
+        updated in place (tracers and captured graphs are kept)."""
+        arrs = [_f64(uv), _f64(w)] + [None if a is None else _f64(a) for a in (wi, we)]
+        ptr = [None if a is None else abi.dptr(a) for a in arrs]
+        abi.check(self.lib.tracer_set_velocity_elem(ptr[0], ptr[1], ptr[2], ptr[3], self.nl, self.nod2D,
+                                                    self.elem2D), "tracer_set_velocity_elem")
+
+    def checkpoint(self, restore: bool = False) -> None:
+        """Keep (restore=False) or put back (restore=True) the tracers and their time levels."""
+        abi.check(self.lib.tracer_checkpoint(int(bool(restore))), "tracer_checkpoint")
 
     def zero_tendencies(self) -> None:
         abi.check(self.lib.tracer_zero_tendencies(), "tracer_zero_tendencies")

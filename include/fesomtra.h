@@ -194,6 +194,27 @@ int tracer_set_graph(int enable);
  * Returns -1 if the mesh cannot be tiled (a node with more than 8 edges); other tracers keep the
  * multi-kernel path. */
 int tracer_set_fused(int enable);
+/* dynamics%use_wsplit of do_oce_adv_tra (lib/oce_adv_tra_driver.F90:221-231): enable = 1 splits
+ * the vertical velocity for tracers with the FCT limiter: the low-order solution takes the
+ * explicit upwind flux of w_e, then the implicit first-order upwind advection with w_i
+ * (adv_tra_vert_impl, lib/oce_adv_tra_ver.F90:90-244, a tridiagonal solve per column), and the
+ * low-order part of the antidiffusive vertical flux is the upwind flux of the full w.  w, w_i and
+ * w_e come from tracer_set_velocity_elem.  Default 0, as the reference. */
+int tracer_set_wsplit(int enable);
+int tracer_get_wsplit(int *enabled);
+/* valuesAB, the time level the high-order fluxes and gradients read, in tracer_advect_step(s):
+ * mode 0 (default): valuesAB := values at every step.
+ * mode 1: frozen, as the reference's drivers leave it: set to values by tracer_set_values (and
+ *         when the mode is selected) and never changed by a step (src/fesom_analytic.F90:335,
+ *         lib/tracer_c_interface.F90:613-648, lib/tracer_c_interface.F90:163-225).
+ * mode 2: FESOM2's second-order Adams-Bashforth interpolation before every step,
+ *         valuesAB = -(0.5+epsilon)*valuesold + (1.5+epsilon)*values, then valuesold := values
+ *         (epsilon = 0.1 in lib/oce_modules.F90:75, valuesold lib/MOD_TRACER.F90:13); selecting
+ *         the mode and tracer_set_values set valuesold := values.
+ * epsilon in [0, 1] (used by mode 2).  valuesAB / valuesold are held at the storage precision and
+ * kept across changes of velocity, precision or block width; tracer_checkpoint keeps them too. */
+int tracer_set_ab_mode(int mode, double epsilon);
+int tracer_get_ab_mode(int *mode, double *epsilon);
 /* enable = 1 (default): for tracer blocks of width 1, 2 or 4 the kernels run on tiles that stage
  * neighbour values in LDS, level by level, instead of gathering them once per edge: the edge-flux
  * kernel on tiles of 256 / width consecutive edges (with the nodes they read), and with the FCT

@@ -262,6 +262,80 @@ void fo_adv_tra_hor_mfct(const fo_mesh *m, const double *vel, const double *ttf,
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* lib/oce_adv_tra_ver.F90:90-244: implicit first-order upwind vertical advection of ttf (the
+ * low-order solution) with the implicit part w of the split vertical velocity, one tridiagonal
+ * solve per column on hnode_new (the reference also builds zbar_n / Z_n there, which the solve
+ * never reads).  The reference's single-layer branch reads ttf outside the column, so columns
+ * with fewer than two wet layers are rejected (-1). */
+int fo_adv_tra_vert_impl(const fo_mesh *m, double dt, const double *W, double *ttf)
+{
+    const int L = m->nl - 1, NL = m->nl;
+    double *a = (double *)calloc((size_t)NL + 1, sizeof(double));
+    double *b = (double *)calloc((size_t)NL + 1, sizeof(double));
+    double *c = (double *)calloc((size_t)NL + 1, sizeof(double));
+    double *tr = (double *)calloc((size_t)NL + 1, sizeof(double));
+    double *cp = (double *)calloc((size_t)NL + 1, sizeof(double));
+    double *tp = (double *)calloc((size_t)NL + 1, sizeof(double));
+    int rc = (a && b && c && tr && cp && tp) ? 0 : -1;
+    for (int n = 1; n <= m->nod2D && rc == 0; ++n) {
+        const int nzmax = m->nlevels_nod2D[n - 1], nzmin = m->ulevels_nod2D[n - 1];
+        if (nzmax - 1 - nzmin < 1) {
+            rc = -1;
+            break;
+        }
+        const double *w = &W[IX2(NL, 1, n)] - 1, *ar = &m->area[IX2(NL, 1, n)] - 1;
+        const double *av = &m->areasvol[IX2(NL, 1, n)] - 1, *hn = &m->hnode_new[IX2(L, 1, n)] - 1;
+        double *t = &ttf[IX2(L, 1, n)] - 1;
+        for (int k = 0; k <= NL; ++k) a[k] = b[k] = c[k] = tr[k] = cp[k] = tp[k] = 0.0;
+        const double zinv = 1.0 * dt;
+        double v_adv, dz, mm;
+        int nz = nzmin;
+        a[nz] = 0.0;
+        v_adv = zinv * ar[nz] / av[nz];
+        b[nz] = hn[nz] + w[nz] * v_adv;
+        v_adv = zinv * ar[nz + 1] / av[nz];
+        b[nz] = b[nz] - dmin(0.0, w[nz + 1]) * v_adv;
+        c[nz] = -dmax(0.0, w[nz + 1]) * v_adv;
+        for (nz = nzmin + 1; nz <= nzmax - 2; ++nz) {
+            v_adv = zinv * ar[nz] / av[nz];
+            a[nz] = dmin(0.0, w[nz]) * v_adv;
+            b[nz] = hn[nz] + dmax(0.0, w[nz]) * v_adv;
+            v_adv = zinv * ar[nz + 1] / av[nz];
+            b[nz] = b[nz] - dmin(0.0, w[nz + 1]) * v_adv;
+            c[nz] = -dmax(0.0, w[nz + 1]) * v_adv;
+        }
+        nz = nzmax - 1;
+        v_adv = zinv * ar[nz] / av[nz];
+        a[nz] = dmin(0.0, w[nz]) * v_adv;
+        b[nz] = hn[nz] + dmax(0.0, w[nz]) * v_adv;
+        c[nz] = 0.0;
+        nz = nzmin;
+        dz = hn[nz];
+        tr[nz] = -(b[nz] - dz) * t[nz] - c[nz] * t[nz + 1];
+        for (nz = nzmin + 1; nz <= nzmax - 2; ++nz) {
+            dz = hn[nz];
+            tr[nz] = -a[nz] * t[nz - 1] - (b[nz] - dz) * t[nz] - c[nz] * t[nz + 1];
+        }
+        nz = nzmax - 1;
+        dz = hn[nz];
+        tr[nz] = -a[nz] * t[nz - 1] - (b[nz] - dz) * t[nz];
+        nz = nzmin;
+        cp[nz] = c[nz] / b[nz];
+        tp[nz] = tr[nz] / b[nz];
+        for (nz = nzmin + 1; nz <= nzmax - 1; ++nz) {
+            mm = b[nz] - cp[nz - 1] * a[nz];
+            cp[nz] = c[nz] / mm;
+            tp[nz] = (tr[nz] - tp[nz - 1] * a[nz]) / mm;
+        }
+        tr[nzmax - 1] = tp[nzmax - 1];
+        for (nz = nzmax - 2; nz >= nzmin; --nz) tr[nz] = tp[nz] - cp[nz] * tr[nz + 1];
+        for (nz = nzmin; nz <= nzmax - 1; ++nz) t[nz] = t[nz] + tr[nz];
+    }
+    free(a); free(b); free(c); free(tr); free(cp); free(tp);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* lib/oce_adv_tra_ver.F90:248-335 */
 void fo_adv_tra_ver_upw1(const fo_mesh *m, const double *W, const double *ttf, double *flux,
                          int init_zero)
@@ -588,7 +662,6 @@ int fo_do_oce_adv_tra(const fo_mesh *m, const fo_scheme *sc, double dt, const do
                       const double *w, const double *wi, const double *we, const double *ttf,
                       const double *ttfAB, fo_work *wk)
 {
-    (void)wi; /* only read by the implicit vertical split (use_wsplit), which is off */
     const int L = m->nl - 1, NL = m->nl;
     const int fct = sc->lim == FO_LIM_FCT;
     if (fct) {
@@ -611,6 +684,11 @@ int fo_do_oce_adv_tra(const fo_mesh *m, const fo_scheme *sc, double dt, const do
                                      (lo[IX2(L, nz, n)] + (wk->adv_flux_ver[IX2(NL, nz, n)] - wk->adv_flux_ver[IX2(NL, nz + 1, n)])) *
                                          dt / m->areasvol[IX2(NL, nz, n)]) /
                                     m->hnode_new[IX2(L, nz, n)];
+        if (sc->use_wsplit) {
+            /* :221-231 implicit part with wi, then the low-order vertical flux of the full w */
+            if (fo_adv_tra_vert_impl(m, dt, wi, lo)) return -1;
+            fo_adv_tra_ver_upw1(m, w, ttf, wk->adv_flux_ver, 1);
+        }
     }
     const int zero = !fct;
     switch (sc->hor) {
@@ -642,6 +720,27 @@ int fo_do_oce_adv_tra(const fo_mesh *m, const fo_scheme *sc, double dt, const do
         fo_flux2dtracer(m, dt, wk->dttf_h, wk->dttf_v, wk->adv_flux_hor, wk->adv_flux_ver, 0, NULL,
                         NULL);
     }
+    return 0;
+}
+
+/* Caller step (src/fesom_analytic.F90:404-437) with a given valuesAB. */
+int fo_tracer_step_ab(const fo_mesh *m, const fo_scheme *sc, double dt, const double *vel,
+                      const double *w, const double *wi, const double *we, double *values,
+                      const double *values_ab, fo_work *wk)
+{
+    const int L = m->nl - 1;
+    memset(wk->dttf_h, 0, sizeof(double) * (size_t)L * (size_t)m->nod2D);
+    memset(wk->dttf_v, 0, sizeof(double) * (size_t)L * (size_t)m->nod2D);
+    if (sc->hor == FO_HOR_MUSCL || sc->hor == FO_HOR_MFCT) {
+        fo_tracer_gradients(m, values_ab, wk->tr_xy);
+        fo_fill_up_dn_grad(m, wk->tr_xy, wk->edge_up_dn_grad);
+    }
+    int rc = fo_do_oce_adv_tra(m, sc, dt, vel, w, wi, we, values, values_ab, wk);
+    if (rc) return rc;
+    for (int k = 1; k <= m->nod2D; ++k)
+        for (int nz = m->ulevels_nod2D[k - 1]; nz <= m->nlevels_nod2D[k - 1] - 1; ++nz)
+            values[IX2(L, nz, k)] = values[IX2(L, nz, k)] +
+                                    (wk->dttf_h[IX2(L, nz, k)] + wk->dttf_v[IX2(L, nz, k)]) / m->hnode_new[IX2(L, nz, k)];
     return 0;
 }
 

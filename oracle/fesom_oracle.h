@@ -47,6 +47,7 @@ enum { FO_LIM_NONE = 0, FO_LIM_FCT = 1 };
 typedef struct fo_scheme {
     int hor, ver, lim;
     double ph, pv;
+    int use_wsplit; /* dynamics%use_wsplit: implicit low-order vertical part with wi */
 } fo_scheme;
 
 /* Work arrays of one tracer step (the reference's T_TRACER_WORK plus tr_xy). */
@@ -80,6 +81,7 @@ void fo_adv_tra_ver_cdiff(const fo_mesh *m, const double *w, const double *ttf, 
                           int init_zero);
 void fo_oce_tra_adv_fct(const fo_mesh *m, double dt, const double *ttf, const double *lo,
                         double *adf_h, double *adf_v, fo_work *wk);
+int fo_adv_tra_vert_impl(const fo_mesh *m, double dt, const double *w, double *ttf);
 void fo_flux2dtracer(const fo_mesh *m, double dt, double *dttf_h, double *dttf_v,
                      const double *flux_h, const double *flux_v, int use_lo, const double *ttf,
                      const double *lo);
@@ -95,6 +97,11 @@ int fo_do_oce_adv_tra(const fo_mesh *m, const fo_scheme *sc, double dt, const do
 int fo_tracer_step(const fo_mesh *m, const fo_scheme *sc, double dt, const double *vel,
                    const double *w, const double *wi, const double *we, double *values,
                    fo_work *wk);
+
+/* The same with a separate valuesAB (gradients and high-order fluxes read values_ab). */
+int fo_tracer_step_ab(const fo_mesh *m, const fo_scheme *sc, double dt, const double *vel,
+                      const double *w, const double *wi, const double *we, double *values,
+                      const double *values_ab, fo_work *wk);
 
 /* Allocation helpers for fo_work (returns 0 on success). */
 int fo_work_alloc(const fo_mesh *m, fo_work *wk);

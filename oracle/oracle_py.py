@@ -40,7 +40,7 @@ class FoMesh(ctypes.Structure):
 
 class FoScheme(ctypes.Structure):
     _fields_ = [("hor", ctypes.c_int), ("ver", ctypes.c_int), ("lim", ctypes.c_int),
-                ("ph", ctypes.c_double), ("pv", ctypes.c_double)]
+                ("ph", ctypes.c_double), ("pv", ctypes.c_double), ("use_wsplit", ctypes.c_int)]
 
 
 class FoWork(ctypes.Structure):
@@ -71,6 +71,12 @@ def lib() -> ctypes.CDLL:
                                         ctypes.c_double, _dp, _dp, _dp, _dp, _dp, _dp,
                                         ctypes.POINTER(FoWork)]
         L.fo_do_oce_adv_tra.restype = ctypes.c_int
+        L.fo_tracer_step_ab.argtypes = [ctypes.POINTER(FoMesh), ctypes.POINTER(FoScheme),
+                                        ctypes.c_double, _dp, _dp, _dp, _dp, _dp, _dp,
+                                        ctypes.POINTER(FoWork)]
+        L.fo_tracer_step_ab.restype = ctypes.c_int
+        L.fo_adv_tra_vert_impl.argtypes = [ctypes.POINTER(FoMesh), ctypes.c_double, _dp, _dp]
+        L.fo_adv_tra_vert_impl.restype = ctypes.c_int
         L.fo_work_alloc.argtypes = [ctypes.POINTER(FoMesh), ctypes.POINTER(FoWork)]
         L.fo_work_alloc.restype = ctypes.c_int
         L.fo_work_free.argtypes = [ctypes.POINTER(FoWork)]
@@ -115,6 +121,7 @@ class Oracle:
         self.w = np.ascontiguousarray(case.w, dtype=np.float64)
         self.wi = np.ascontiguousarray(case.wi, dtype=np.float64)
         self.we = np.ascontiguousarray(case.we, dtype=np.float64)
+        self.use_wsplit = int(bool(getattr(case, "use_wsplit", False)))
         self.work = FoWork()
         if lib().fo_work_alloc(ctypes.byref(self.mesh), ctypes.byref(self.work)) != 0:
             raise MemoryError("oracle work allocation failed")
@@ -125,9 +132,9 @@ class Oracle:
         except Exception:
             pass
 
-    @staticmethod
-    def scheme(sc) -> FoScheme:
-        return FoScheme(HOR[sc.hor], VER[sc.ver], LIM[sc.lim], float(sc.ph), float(sc.pv))
+    def scheme(self, sc) -> FoScheme:
+        return FoScheme(HOR[sc.hor], VER[sc.ver], LIM[sc.lim], float(sc.ph), float(sc.pv),
+                        self.use_wsplit)
 
     def step(self, values: np.ndarray, sc, dt: float) -> None:
         """Advance one tracer (N, nl-1) float64 C-contiguous array in place by one step."""
@@ -137,6 +144,18 @@ class Oracle:
                                   _p(self.w, ctypes.c_double), _p(self.wi, ctypes.c_double),
                                   _p(self.we, ctypes.c_double), _p(values, ctypes.c_double),
                                   ctypes.byref(self.work))
+        if rc != 0:
+            raise RuntimeError(f"oracle step failed ({rc})")
+
+    def step_ab(self, values: np.ndarray, values_ab: np.ndarray, sc, dt: float) -> None:
+        """One step of one tracer with a separate valuesAB (both (N, nl-1) float64)."""
+        assert values.dtype == np.float64 and values.flags.c_contiguous
+        tab = np.ascontiguousarray(values_ab, dtype=np.float64)
+        s = self.scheme(sc)
+        rc = lib().fo_tracer_step_ab(ctypes.byref(self.mesh), ctypes.byref(s), dt, _p(self.uv, ctypes.c_double),
+                                     _p(self.w, ctypes.c_double), _p(self.wi, ctypes.c_double),
+                                     _p(self.we, ctypes.c_double), _p(values, ctypes.c_double),
+                                     _p(tab, ctypes.c_double), ctypes.byref(self.work))
         if rc != 0:
             raise RuntimeError(f"oracle step failed ({rc})")
 
@@ -154,18 +173,30 @@ class Oracle:
         return out
 
     def run(self, nsteps: Optional[int] = None, save_steps: Iterable[int] = (),
-            work_step1: bool = False):
-        """Advance all tracers; returns ({step: (ntr, N, nl-1)}, work arrays of tracer 0 step 1)."""
+            work_step1: bool = False, ab_mode: int = 0, ab_epsilon: float = 0.1):
+        """Advance all tracers; returns ({step: (ntr, N, nl-1)}, work arrays of tracer 0 step 1).
+
+        ab_mode 0: valuesAB := values every step; 1: valuesAB frozen at the initial values (the
+        reference's drivers, src/fesom_analytic.F90:335 and lib/tracer_c_interface.F90:613);
+        2: FESOM2's AB2 interpolation valuesAB = -(0.5+eps)*valuesold + (1.5+eps)*values, then
+        valuesold := values (eps as lib/oce_modules.F90:75), valuesold starting at the values."""
         case = self.case
         ns = case.nsteps if nsteps is None else nsteps
         vals = [np.ascontiguousarray(case.tracers[i], dtype=np.float64).copy()
                 for i in range(case.ntracers)]
+        ab = [v.copy() for v in vals]
+        old = [v.copy() for v in vals]
         save = set(save_steps)
         out = {}
         work = None
         for s in range(1, ns + 1):
             for i in range(case.ntracers):
-                if work_step1 and s == 1 and i == 0:
+                if ab_mode != 0:
+                    if ab_mode == 2:
+                        ab[i] = -(0.5 + ab_epsilon) * old[i] + (1.5 + ab_epsilon) * vals[i]
+                        old[i] = vals[i].copy()
+                    self.step_ab(vals[i], ab[i], case.schemes[i], case.dt)
+                elif work_step1 and s == 1 and i == 0:
                     # gradients are clobbered by the limiter: capture them before advection
                     self.step_with_capture(vals[i], case.schemes[i], case.dt)
                     work = self._captured

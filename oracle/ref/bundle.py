@@ -31,14 +31,16 @@ def ref_available() -> bool:
 
 
 def write_bundle(path, case, save_steps: Sequence[int] = (), save_work: bool = False,
-                 timing: bool = False, nsteps: Optional[int] = None) -> Path:
+                 timing: bool = False, nsteps: Optional[int] = None, ab_mode: int = 0) -> Path:
+    """ab_mode: 0 valuesAB := values every step, 1 frozen at the initial values, 2 FESOM2's AB2
+    interpolation; dynamics%use_wsplit is taken from case.use_wsplit (default off)."""
     path = Path(path)
     path.mkdir(parents=True, exist_ok=True)
     m = case.mesh
     ns = case.nsteps if nsteps is None else nsteps
     lines = [f"{m.nl} {m.nod2D} {m.elem2D} {m.edge2D} {m.max_adj}",
              f"{case.ntracers} {ns} {case.dt!r}",
-             f"{int(save_work)} {int(timing)}",
+             f"{int(save_work)} {int(timing)} {int(bool(getattr(case, 'use_wsplit', False)))} {int(ab_mode)}",
              f"{len(save_steps)}"]
     if save_steps:
         lines.append(" ".join(str(int(s)) for s in save_steps))

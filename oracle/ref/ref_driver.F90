@@ -6,10 +6,16 @@
 !
 !   per tracer and step (caller semantics of src/fesom_analytic.F90:404-437):
 !     del_ttf_advhoriz = del_ttf_advvert = 0
-!     valuesAB := values                          (the shipped drivers freeze valuesAB instead)
+!     valuesAB by ab_mode: 0  valuesAB := values
+!                          1  valuesAB frozen at the initial values (the shipped drivers,
+!                             src/fesom_analytic.F90:335)
+!                          2  FESOM2's AB2 interpolation valuesAB = -(0.5+epsilon)*valuesold
+!                             + (1.5+epsilon)*values, valuesold := values (epsilon of
+!                             lib/oce_modules.F90:75; the dwarf never computes it)
 !     tr_xy    := P1 gradient of valuesAB with gradient_sca (FESOM2's tracer_gradient_elements)
 !     call fill_up_dn_grad                        (lib/oce_muscl_adv.F90:361-530)
-!     call do_oce_adv_tra(dt, uv, w, wi, we, ...) (lib/oce_adv_tra_driver.F90:23-294)
+!     call do_oce_adv_tra(dt, uv, w, wi, we, ...) (lib/oce_adv_tra_driver.F90:23-294), with
+!                                                 dynamics%use_wsplit from the bundle
 !     values += (del_ttf_advhoriz + del_ttf_advvert) / hnode_new
 !
 ! Outputs (in the bundle directory):
@@ -40,7 +46,8 @@ program ref_driver
   type(t_dyn),    target :: dyn
   character(len=4096)    :: dir
   integer :: ierr, nl, nod2D, elem2D, edge2D, max_adj, ntr, nsteps, nsave, i, n, k, e, istep, nz
-  integer :: save_work, timing
+  integer :: save_work, timing, use_wsplit, ab_mode
+  real(kind=WP), allocatable :: vold(:,:,:)
   integer, allocatable :: save_steps(:)
   real(kind=WP) :: dt
   character(len=20) :: hor, ver, lim
@@ -61,7 +68,7 @@ program ref_driver
   open(newunit=u, file=trim(dir)//'/dims.txt', status='old', action='read')
   read(u,*) nl, nod2D, elem2D, edge2D, max_adj
   read(u,*) ntr, nsteps, dt
-  read(u,*) save_work, timing
+  read(u,*) save_work, timing, use_wsplit, ab_mode
   read(u,*) nsave
   allocate(save_steps(nsave))
   if (nsave > 0) read(u,*) save_steps
@@ -123,13 +130,18 @@ program ref_driver
   call rd_r2(mesh%helem, 'helem')
   call rd_r2(mesh%zbar_3d_n, 'zbar_3d_n')
   call rd_r2(mesh%Z_3d_n, 'Z_3d_n')
+  ! bottom depth per node: adv_tra_vert_impl reads it (into arrays its solve never uses)
+  allocate(mesh%zbar_n_bot(nod2D))
+  do k = 1, nod2D
+     mesh%zbar_n_bot(k) = mesh%zbar_3d_n(mesh%nlevels_nod2D(k), k)
+  end do
 
   allocate(dyn%uv(2, nl-1, elem2D), dyn%w(nl, nod2D), dyn%w_i(nl, nod2D), dyn%w_e(nl, nod2D))
   call rd_r3(dyn%uv, 'uv')
   call rd_r2(dyn%w, 'w')
   call rd_r2(dyn%w_i, 'wi')
   call rd_r2(dyn%w_e, 'we')
-  dyn%use_wsplit = .false.
+  dyn%use_wsplit = (use_wsplit /= 0)
 
   allocate(tbuf(nl-1, nod2D, ntr))
   call rd_r3(tbuf, 'values')
@@ -138,6 +150,8 @@ program ref_driver
      tracers%data(n)%values = tbuf(:, :, n)
      tracers%data(n)%valuesAB = tbuf(:, :, n)
   end do
+  allocate(vold(nl-1, nod2D, ntr))
+  vold = tbuf
 
   allocate(tracers%work%del_ttf_advhoriz(nl-1, nod2D), tracers%work%del_ttf_advvert(nl-1, nod2D))
   call oce_adv_tra_fct_init(tracers%work, partit, mesh)
@@ -155,7 +169,12 @@ program ref_driver
         tracers%work%del_ttf_advhoriz = 0.0_MP
         tracers%work%del_ttf_advvert  = 0.0_MP
         t0 = MPI_Wtime()
-        tracers%data(n)%valuesAB = tracers%data(n)%values
+        if (ab_mode == 0) then
+           tracers%data(n)%valuesAB = tracers%data(n)%values
+        else if (ab_mode == 2) then
+           tracers%data(n)%valuesAB = -(0.5_WP+epsilon)*vold(:, :, n) + (1.5_WP+epsilon)*tracers%data(n)%values
+           vold(:, :, n) = tracers%data(n)%values
+        end if
         call gradients(tracers%data(n)%valuesAB)
         call fill_up_dn_grad(tracers%work, partit, mesh)
         if (save_work /= 0 .and. istep == 1 .and. n == 1) &

@@ -6,6 +6,7 @@ generator is detected instead of silently comparing different cases.
 
     python tests/golden/make_golden.py        (needs oracle/_ref built by oracle/ref/build_ref.sh)
     python tests/golden/make_golden.py --only pi   (only the FESOM2 pi-mesh fixture)
+    python tests/golden/make_golden.py --only wsplit_ab   (only the split-w / valuesAB fixture)
 """
 
 from __future__ import annotations
@@ -41,6 +42,10 @@ def case_hash(case) -> str:
         h.update(np.ascontiguousarray(a).tobytes())
     h.update(repr([(s.hor, s.ver, s.lim, s.ph, s.pv) for s in case.schemes]).encode())
     h.update(repr(case.dt).encode())
+    if getattr(case, "use_wsplit", False):
+        for a in (case.wi, case.we):
+            h.update(np.ascontiguousarray(a).tobytes())
+        h.update(b"use_wsplit")
     return h.hexdigest()
 
 
@@ -87,6 +92,33 @@ def write_pi(td):
     print("wrote pi_mesh_schemes")
 
 
+def wsplit_ab_cases():
+    """name -> (case, ab_mode): the implicit vertical split (dynamics%use_wsplit) with each
+    vertical scheme, and the frozen / AB2 valuesAB modes of the caller."""
+    out = {}
+    for ver in ("UPW1", "QR4C", "CDIFF", "PPM"):
+        c = configs.bathymetry_case("wsplit", 10, 12, seed=41, nsteps=3, lev_lo=4,
+                                    scheme=Scheme("MFCT", ver, "FCT", 0.8, 0.7), w_cfl=1.5)
+        out[f"wsplit_{ver}"] = (configs.split_vertical_velocity(c, 0.4), 0)
+    c = configs.bathymetry_case("ab", 10, 12, seed=42, nsteps=3, lev_lo=4,
+                                scheme=Scheme("MUSCL", "QR4C", "FCT", 0.8, 0.7), w_cfl=0.3)
+    out["ab_frozen"] = (c, 1)
+    out["ab_ab2"] = (c, 2)
+    return out
+
+
+def write_wsplit_ab(td):
+    out, hashes = {}, []
+    for name, (case, ab) in wsplit_ab_cases().items():
+        p = bundle.write_bundle(Path(td) / name, case, save_steps=[3], ab_mode=ab)
+        bundle.run_ref(p)
+        out[name] = bundle.read_values(p, case, 3)
+        hashes.append(case_hash(case) + str(ab))
+    np.savez_compressed(HERE / "wsplit_ab.npz",
+                        input_sha256=hashlib.sha256("".join(hashes).encode()).hexdigest(), **out)
+    print("wrote wsplit_ab")
+
+
 def main():
     if not bundle.ref_available():
         raise SystemExit("oracle/_ref/ref_driver not built: run oracle/ref/build_ref.sh")
@@ -94,8 +126,13 @@ def main():
         with tempfile.TemporaryDirectory() as td:
             write_pi(td)
         return
+    if sys.argv[1:3] == ["--only", "wsplit_ab"]:
+        with tempfile.TemporaryDirectory() as td:
+            write_wsplit_ab(td)
+        return
     with tempfile.TemporaryDirectory() as td:
         write_pi(td)
+        write_wsplit_ab(td)
         for name, case in analytic_cases().items():
             p = bundle.write_bundle(Path(td) / name, case, save_steps=[1, 2, 10], save_work=True)
             bundle.run_ref(p)

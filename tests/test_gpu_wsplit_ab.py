@@ -1,0 +1,196 @@
+"""GPU parity of the implicit vertical split (dynamics%use_wsplit) and of the valuesAB time levels
+(frozen as the reference's drivers, FESOM2's AB2 interpolation) against the CPU restatement, and
+of the state the library keeps across velocity updates, rebuilds and checkpoints.
+
+FP64 bar: relative L-infinity <= 1e-12 (observed: bit-identical); FP32 arithmetic: <= 1e-5.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from fesom2_tracer_dwarf_amd import configs
+from fesom2_tracer_dwarf_amd.configs import Scheme
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-12
+
+
+def rel_linf(got, ref):
+    return float(np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-300))
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def split_case(ver, seed=51, nsteps=3, hor="MFCT"):
+    c = configs.bathymetry_case("wsplit", 10, 12, seed=seed, nsteps=nsteps, lev_lo=4,
+                                scheme=Scheme(hor, ver, "FCT", 0.8, 0.7), w_cfl=1.5)
+    return configs.split_vertical_velocity(c, 0.4)
+
+
+def ab_case(seed=52, nsteps=4):
+    return configs.bathymetry_case("ab", 10, 12, seed=seed, nsteps=nsteps, lev_lo=4,
+                                   scheme=Scheme("MUSCL", "QR4C", "FCT", 0.8, 0.7), w_cfl=0.3)
+
+
+def make(case, kb=0, prec=0, tiles=True, ab=0, graph=False, fused=False):
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    t = FESOMTracerAdvection()
+    t.set_block_width(kb)
+    t.set_precision(prec)
+    t.set_halo_tiles(tiles)
+    t.setup_case(case)
+    t.set_ab_mode(ab)
+    t.set_graph(graph)
+    t.set_fused(fused)
+    return t
+
+
+def done(t):
+    t.cleanup()  # releases the library state, options included
+
+
+@pytest.mark.parametrize("ver", ["UPW1", "QR4C", "CDIFF", "PPM"])
+@pytest.mark.parametrize("kb,tiles", [(1, True), (1, False), (2, True), (64, True)])
+def test_wsplit_matches_oracle(oracle_mod, ver, kb, tiles):
+    case = split_case(ver)
+    say(f"case wsplit MFCT/{ver}/FCT kb={kb} tiles={tiles} 3 steps")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[3])
+    t = make(case, kb=kb, tiles=tiles)
+    assert t.wsplit()
+    t.advect_steps(3, case.dt)
+    got = t.get_all_values()
+    done(t)
+    err = rel_linf(got, ref[3])
+    say(f"  rel Linf {err:.3e} bitwise {np.array_equal(got, ref[3])}")
+    assert err <= RTOL
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-5), (3, 2e-2)])
+def test_wsplit_single_precision(oracle_mod, prec, tol):
+    case = split_case("QR4C", seed=53)
+    say(f"case wsplit single precision mode {prec}")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[3])
+    t = make(case, prec=prec)
+    t.advect_steps(3, case.dt)
+    got = t.get_all_values()
+    done(t)
+    err = rel_linf(got, ref[3])
+    say(f"  rel Linf {err:.3e}")
+    assert np.all(np.isfinite(got)) and err <= tol
+
+
+def test_wsplit_dropin_work_arrays(oracle_mod):
+    """tracer_do_oce_adv_tra with the split: low-order solution after the implicit solve, the
+    limited vertical flux and the tendencies."""
+    case = split_case("PPM", seed=54, nsteps=1, hor="MUSCL")
+    say("case wsplit drop-in do_oce_adv_tra MUSCL/PPM/FCT")
+    o = oracle_mod.Oracle(case)
+    work = oracle_mod.DropIn(o).run(case.tracers[0], case.tracers[0], case.schemes[0], case.dt)
+    t = make(case, kb=1)
+    t.zero_tendencies()
+    t.do_oce_adv_tra(case.dt, 1)
+    h, v = t.get_tendencies()
+    lo, fv = t.get_work("fct_LO"), t.get_work("adv_flux_ver")
+    done(t)
+    for name, got, ref in (("fct_LO", lo, work["fct_LO"]), ("adv_flux_ver", fv, work["adv_flux_ver"]),
+                           ("dttf_h", h, work["dttf_h"]), ("dttf_v", v, work["dttf_v"])):
+        err = rel_linf(got, ref)
+        say(f"  {name} rel Linf {err:.3e}")
+        assert err <= RTOL, name
+
+
+def test_wsplit_graph_and_fused_request(oracle_mod):
+    """Graph replay with the split; the fused single-kernel path does not apply to it and the
+    request falls back to the multi-kernel path."""
+    case = split_case("QR4C", seed=55)
+    say("case wsplit graph + fused request kb=1")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[3])
+    t = make(case, kb=1, graph=True, fused=True)
+    t.advect_steps(3, case.dt)
+    got = t.get_all_values()
+    done(t)
+    err = rel_linf(got, ref[3])
+    say(f"  rel Linf {err:.3e}")
+    assert err <= RTOL
+
+
+@pytest.mark.parametrize("ab", [1, 2])
+@pytest.mark.parametrize("kb", [1, 2, 64])
+def test_ab_modes_match_oracle(oracle_mod, ab, kb):
+    case = ab_case()
+    say(f"case ab mode {ab} kb={kb} 4 steps")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[4], ab_mode=ab)
+    t = make(case, kb=kb, ab=ab)
+    assert t.ab_mode() == (ab, 0.1)
+    t.advect_steps(4, case.dt)
+    got = t.get_all_values()
+    done(t)
+    err = rel_linf(got, ref[4])
+    say(f"  rel Linf {err:.3e} bitwise {np.array_equal(got, ref[4])}")
+    assert err <= RTOL
+
+
+def test_ab2_single_precision(oracle_mod):
+    case = ab_case(seed=56)
+    say("case ab2 FP32")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[4], ab_mode=2)
+    t = make(case, prec=1, ab=2)
+    t.advect_steps(4, case.dt)
+    got = t.get_all_values()
+    done(t)
+    err = rel_linf(got, ref[4])
+    say(f"  rel Linf {err:.3e}")
+    assert err <= 1e-5
+
+
+def test_ab2_graph_checkpoint_and_rebuild(oracle_mod):
+    """AB2 in a captured graph; tracer_checkpoint keeps valuesAB / valuesold; a change of block
+    width mid-run (device storage rebuilt) and a velocity update carry them over."""
+    case = ab_case(seed=57, nsteps=6)
+    say("case ab2 graph / checkpoint / rebuild / velocity update")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[2, 6], ab_mode=2)
+    t = make(case, kb=2, ab=2, graph=True)
+    t.checkpoint(False)
+    t.advect_steps(2, case.dt)
+    first = t.get_all_values()
+    t.checkpoint(True)
+    t.advect_steps(2, case.dt)
+    again = t.get_all_values()
+    t.set_block_width(1)  # re-interleaves the storage
+    t.set_velocity(case.uv, case.w, case.wi, case.we)
+    t.advect_steps(2, case.dt)
+    t.set_block_width(2)
+    t.advect_steps(2, case.dt)
+    got = t.get_all_values()
+    done(t)
+    say(f"  step 2 {rel_linf(first, ref[2]):.3e} repeat {np.array_equal(first, again)} "
+        f"step 6 {rel_linf(got, ref[6]):.3e}")
+    assert np.array_equal(first, again)
+    assert rel_linf(first, ref[2]) <= RTOL
+    assert rel_linf(got, ref[6]) <= RTOL
+
+
+def test_velocity_update_keeps_tracers(oracle_mod):
+    """A new velocity between steps (as a time-stepping caller sets it) is written into the device
+    copy in place: the tracers advanced so far are kept and the next steps use the new field."""
+    case = configs.bathymetry_case("vel", 10, 12, seed=58, nsteps=2, lev_lo=4, w_cfl=0.3)
+    other = configs.bathymetry_case("vel2", 10, 12, seed=58, nsteps=2, lev_lo=4, w_cfl=0.3,
+                                    umax=0.05)
+    assert np.array_equal(other.mesh.edges, case.mesh.edges)
+    say("case velocity update between steps")
+    mid, _ = oracle_mod.Oracle(case).run(save_steps=[2])
+    case2 = dataclasses.replace(case, uv=other.uv, w=other.w, wi=other.wi, we=other.we,
+                                tracers=mid[2], dt=case.dt)
+    ref, _ = oracle_mod.Oracle(case2).run(save_steps=[2])
+    t = make(case, kb=2, graph=True)
+    t.advect_steps(2, case.dt)
+    t.set_velocity(other.uv, other.w, other.wi, other.we)
+    t.advect_steps(2, case.dt)
+    got = t.get_all_values()
+    done(t)
+    err = rel_linf(got, ref[2])
+    say(f"  rel Linf {err:.3e}")
+    assert err <= RTOL
