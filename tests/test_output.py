@@ -1,0 +1,73 @@
+"""The dwarf's output formats (lib/mesh_output.F90:19-162): files written by the compiled
+reference driver (tests/golden/mesh_output_ref, from tests/golden/make_mesh_output.py) are read
+and written back byte for byte, and a mesh of this package is written in the same layout."""
+import filecmp
+import re
+from pathlib import Path
+
+import numpy as np
+
+from fesom2_tracer_dwarf_amd import configs, output
+
+REF = Path(__file__).resolve().parent / "golden" / "mesh_output_ref"
+FILES = ["mesh_info.txt", "coord_nod2D.bin", "elem2D_nodes.bin", "zbar.bin", "depth.bin",
+         "scalar_info.txt", "temperature.bin", "salinity.bin"]
+
+
+def test_reference_files_round_trip_bytewise(tmp_path):
+    m = output.read_mesh_for_python(REF)
+    info, frames = output.read_scalar_output(REF)
+    output.write_mesh_arrays(tmp_path, m["coord_m"], m["elem_nodes0"], m["zbar"], m["depth"])
+    nod2D, nz = int(info["nod2D"]), int(info["nz"])
+    with output.ScalarOutput(tmp_path, nod2D, nz, int(info["nsteps"]), int(info["num_tracers"]),
+                             wp=int(info["wp"])) as so:
+        for k in range(frames["temperature"].shape[0]):
+            so.write_step([frames["temperature"][k], frames["salinity"][k]])
+    for f in FILES:
+        assert filecmp.cmp(REF / f, tmp_path / f, shallow=False), f
+
+
+def test_reference_files_are_what_the_formats_say():
+    """Layout facts checked against the run itself: 6 x 5 periodic mesh (30 nodes on a 20 km
+    grid), 4 levels of a 1000 m ocean, an initial frame plus one per step, frame k of
+    temperature summing to the plain sum the driver printed for step k."""
+    m = output.read_mesh_for_python(REF)
+    info, frames = output.read_scalar_output(REF)
+    nod2D = int(info["nod2D"])
+    assert m["coord_m"].shape == (nod2D, 2) and m["elem_nodes0"].shape == (2 * nod2D, 3)
+    assert m["elem_nodes0"].min() == 0 and m["elem_nodes0"].max() == nod2D - 1
+    steps = np.unique(np.round(np.diff(np.unique(m["coord_m"][:, 0])), 6))
+    assert steps.size == 1  # a regular grid in metres
+    assert np.array_equal(m["zbar"], -np.arange(4) * 1000.0 / 3) and np.all(m["depth"] == -1000.0)
+    assert info["tracer_1"] == "temperature" and info["tracer_2"] == "salinity"
+    assert frames["temperature"].shape == (int(info["nsteps"]) + 1, nod2D, int(info["nz"]))
+    printed = [float(re.search(r"=\s+\S+\s+\S+\s+(\S+)", l).group(1))
+               for l in (REF / "console_steps.txt").read_text().splitlines()]
+    for k, s in enumerate(printed, start=1):
+        assert abs(frames["temperature"][k].sum() - s) <= 1e-9 * abs(s), k
+
+
+def test_write_mesh_for_python_of_a_package_mesh(tmp_path):
+    case = configs.bathymetry_case("out", 8, 10, seed=9, nsteps=1, lev_lo=4)
+    mesh = case.mesh
+    output.write_mesh_for_python(mesh, tmp_path)
+    m = output.read_mesh_for_python(tmp_path)
+    assert (tmp_path / "mesh_info.txt").read_text() == \
+        f"nod2D={mesh.nod2D}\nelem2D={mesh.elem2D}\nnl={mesh.nl}\n"
+    assert np.array_equal(m["coord_m"], mesh.coord_nod2D * output.R_EARTH)
+    assert np.array_equal(m["elem_nodes0"], mesh.elem2D_nodes)
+    assert np.array_equal(m["zbar"], mesh.zbar)
+    bottom = np.array([mesh.zbar_3d_n[n, mesh.nlevels_nod2D[n] - 1] for n in range(mesh.nod2D)])
+    assert np.array_equal(m["depth"], bottom) and np.all(bottom < 0)
+    # raw bytes: (2, nod2D) column-major float64 = x0 y0 x1 y1 ...
+    raw = np.fromfile(tmp_path / "coord_nod2D.bin", dtype="<f8")
+    assert raw[0] == mesh.coord_nod2D[0, 0] * output.R_EARTH
+    assert raw[1] == mesh.coord_nod2D[0, 1] * output.R_EARTH
+
+
+def test_scalar_output_names_beyond_two_tracers(tmp_path):
+    with output.ScalarOutput(tmp_path, 3, 2, 1, 3) as so:
+        so.write_step([np.full((3, 2), float(i)) for i in range(3)])
+    info, fr = output.read_scalar_output(tmp_path)
+    assert [info[f"tracer_{i}"] for i in (1, 2, 3)] == ["temperature", "salinity", "tracer3"]
+    assert fr["tracer3"].shape == (1, 3, 2) and np.all(fr["tracer3"] == 2.0)
