@@ -62,10 +62,11 @@ def node_depth(mesh) -> np.ndarray:
 
 
 def write_mesh_for_python(mesh, output_dir, depth: Optional[np.ndarray] = None) -> Path:
-    """write_mesh_for_python (lib/mesh_output.F90:19-80) for a Mesh of this package."""
+    """write_mesh_for_python (lib/mesh_output.F90:19-80) for a Mesh of this package (whose
+    connectivity is 1-based, as the reference's; the file is 0-based)."""
     coord_m = np.asarray(mesh.coord_nod2D, dtype=np.float64) * R_EARTH
     d = node_depth(mesh) if depth is None else depth
-    return write_mesh_arrays(output_dir, coord_m, mesh.elem2D_nodes, mesh.zbar, d)
+    return write_mesh_arrays(output_dir, coord_m, np.asarray(mesh.elem2D_nodes) - 1, mesh.zbar, d)
 
 
 def _read_info(path: Path) -> Dict[str, str]:

@@ -55,7 +55,8 @@ def test_write_mesh_for_python_of_a_package_mesh(tmp_path):
     assert (tmp_path / "mesh_info.txt").read_text() == \
         f"nod2D={mesh.nod2D}\nelem2D={mesh.elem2D}\nnl={mesh.nl}\n"
     assert np.array_equal(m["coord_m"], mesh.coord_nod2D * output.R_EARTH)
-    assert np.array_equal(m["elem_nodes0"], mesh.elem2D_nodes)
+    assert np.array_equal(m["elem_nodes0"], mesh.elem2D_nodes - 1)  # 1-based mesh, 0-based file
+    assert m["elem_nodes0"].min() == 0 and m["elem_nodes0"].max() == mesh.nod2D - 1
     assert np.array_equal(m["zbar"], mesh.zbar)
     bottom = np.array([mesh.zbar_3d_n[n, mesh.nlevels_nod2D[n] - 1] for n in range(mesh.nod2D)])
     assert np.array_equal(m["depth"], bottom) and np.all(bottom < 0)
