@@ -138,9 +138,16 @@ struct SlotLd {
     __device__ __forceinline__ C operator()(size_t slot) const { return buf[slot * KB + lane]; }
 };
 
-template <bool TILED, int HOR, bool FCT, int KB, typename S, typename M, typename C>
+#ifndef FESOMTRA_EDGE_PIPE
+#define FESOMTRA_EDGE_PIPE 2
+#endif
+constexpr int kEdgePipe = FESOMTRA_EDGE_PIPE;  // levels of loads in flight in the tiled edge kernel
+static_assert(kEdgePipe == 1 || kEdgePipe == 2, "edge pipeline depth 1 or 2");
+
+// SAME (tiled form only): valuesAB is values, so only one field is staged.
+template <bool TILED, int HOR, bool FCT, int KB, typename S, typename M, typename C, bool SAME = false>
 __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, int lc) {
-    __shared__ C sT[TILED ? 2 : 1][TILED ? kHtSlots : 1], sA[TILED ? 2 : 1][TILED ? kHtSlots : 1];
+    __shared__ C sT[TILED ? 2 : 1][TILED ? kHtSlots : 1], sA[TILED && !SAME ? 2 : 1][TILED && !SAME ? kHtSlots : 1];
     EdgeTile<KB, M> et;
     int e, lane, nz0, nz1, lv0, lv1;
     bool act = true;
@@ -171,7 +178,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
     }
     const size_t Np = m.Np, Ep = m.Ep;
     const LaneLd<KB, C, S> ldT{p.T, lane}, ldA{p.TAB, lane};
-    const bool same = p.TAB == p.T;
+    const bool same = TILED ? SAME : p.TAB == p.T;
     C dx = 0, dy = 0, a = 0;
     int shared_end = 0, u[3] = {0, 0, 0}, d[3] = {0, 0, 0}, nb1 = 0, nb2 = 0;
     C gu[6], gd[6];
@@ -206,34 +213,46 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
         for (int i = 0; i < 4; ++i) sl[i] = cld(m.et_loc, 4 * (size_t)e + i);
     }
     auto slot = [&](int k) { return (sl[k >> 1] >> (16 * (k & 1))) & 0xffff; };
-    C stT[2], stA[2], qn = 0;  // staging registers and this edge's transport, a level ahead
-    auto fetch = [&](int nz) {
+    // staging registers and this edge's transport of the next kEdgePipe levels (loads of two
+    // levels in flight while a level is computed)
+    // two levels in flight only where the registers allow it (one staged field, no MUSCL switch)
+    constexpr int P = (SAME && HOR != HOR_MUSCL) ? kEdgePipe : 1;
+    C stT[P][2], stA[SAME ? 1 : P][2], qn[P];
+    auto fetch = [&](int nz, auto set) {
+        constexpr int r = decltype(set)::value;
         const size_t o = (size_t)(nz - 1) * Np;
         et.each([&](int i, int node) {
-            stT[i] = ldT(o + node);
-            if (!same) stA[i] = ldA(o + node);
+            stT[r][i] = ldT(o + node);
+            if constexpr (!SAME)
+                if (!same) stA[r][i] = ldA(o + node);
         });
-        if (act && nz <= nz1) qn = (C)cld(m.q, (size_t)(nz - 1) * Ep + e);
+        qn[r] = (act && nz <= nz1) ? (C)cld(m.q, (size_t)(nz - 1) * Ep + e) : C(0);
     };
-    if constexpr (TILED) fetch(lv0);
-    C qcur = 0;
-    for (int nz = lv0; nz <= lv1; ++nz) {
+    if constexpr (TILED) {
+        fetch(lv0, std::integral_constant<int, 0>{});
+        if (P > 1 && lv0 < lv1) fetch(lv0 + 1, std::integral_constant<int, P - 1>{});
+    }
+    auto level = [&](int nz, auto set) {
+        constexpr int r = decltype(set)::value;
         const int buf = nz & 1;
+        C qcur = 0;
         if constexpr (TILED) {
             et.each([&](int i, int) {
-                sT[buf][threadIdx.x + 256 * i] = stT[i];
-                if (!same) sA[buf][threadIdx.x + 256 * i] = stA[i];
+                sT[buf][threadIdx.x + 256 * i] = stT[r][i];
+                if constexpr (!SAME)
+                    if (!same) sA[buf][threadIdx.x + 256 * i] = stA[r][i];
             });
-            qcur = qn;
+            qcur = qn[r];
             __syncthreads();
-            if (nz < lv1) fetch(nz + 1);
-            if (!act || nz > nz1) continue;
+            if (nz + P <= lv1) fetch(nz + P, set);
+            if (!act || nz > nz1) return;
         }
         const int k = nz - 1;
         const C v = TILED ? qcur : (C)cld(m.q, (size_t)k * Ep + e);
         const size_t o = (size_t)k * Np;
         const size_t ie = ((size_t)k * Ep + e) * KB + lane;
-        const SlotLd<KB, C> lsT{sT[TILED ? buf : 0], lane}, lsA{same ? sT[TILED ? buf : 0] : sA[TILED ? buf : 0], lane};
+        const SlotLd<KB, C> lsT{sT[TILED ? buf : 0], lane},
+            lsA{same ? sT[TILED ? buf : 0] : sA[TILED && !SAME ? buf : 0], lane};
         C t1 = 0, t2 = 0;
         if (FCT || same) {
             t1 = TILED ? lsT(slot(0)) : ldT(o + n1);
@@ -277,6 +296,12 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
             if (FCT && !same) f = f - C(0.5) * ((v + fabs(v)) * (a1 - t1) + (v - fabs(v)) * (a2 - t2));
         }
         p.adfh[ie] = f;
+    };
+    // unrolled by the pipeline depth, so every register set is addressed statically
+    for (int nz = lv0; nz <= lv1; nz += P) {
+        level(nz, std::integral_constant<int, 0>{});
+        if constexpr (P > 1)
+            if (nz + 1 <= lv1) level(nz + 1, std::integral_constant<int, P - 1>{});
     }
 }
 
@@ -1098,7 +1123,8 @@ static void edge_kernel(dim3 g, hipStream_t s, const MeshCol<M> &m, const StepPt
     if constexpr (KB <= 4) {
         if (m.et_n > 0 && m.et_kb == KB) {  // edge tiles built for this block width
             const dim3 gt(grid_for((size_t)m.et_n, 1), g.y);
-            k_edge<true, HOR, FCT, KB, S, M, C><<<gt, 256, 0, s>>>(m, p, ph, lc);
+            if (p.TAB == p.T) k_edge<true, HOR, FCT, KB, S, M, C, true><<<gt, 256, 0, s>>>(m, p, ph, lc);
+            else k_edge<true, HOR, FCT, KB, S, M, C, false><<<gt, 256, 0, s>>>(m, p, ph, lc);
             return;
         }
     }
