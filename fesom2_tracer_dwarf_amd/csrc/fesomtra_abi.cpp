@@ -384,11 +384,18 @@ int upload_mesh(State &s, MeshCol<M> &m, const std::vector<int> &ints_n1, const 
     m.Ep = (int)s.Ep;
     m.max_adj = s.max_adj;
     if (upload(s, &m.nlev_n, ints_n1) || upload(s, &m.nlev_min, ints_n2) || upload(s, &m.nbl, ints_n3)) return -1;
+    // inputs identical bit for bit share one device array (hnode_new = hnode without ALE motion,
+    // areasvol = area without cavities), so kernels reading both fetch it once
+    auto same_bits = [](const std::vector<double> &x, const std::vector<double> &y) {
+        return x.size() == y.size() && (x.empty() || std::memcmp(x.data(), y.data(), x.size() * sizeof(double)) == 0);
+    };
     if (upload(s, &m.hnode, level_major<M>(s.hnode, L, s.perm_n, s.Np))) return -1;
-    if (upload(s, &m.hnode_new, level_major<M>(s.hnode_new, L, s.perm_n, s.Np))) return -1;
+    if (same_bits(s.hnode_new, s.hnode)) m.hnode_new = m.hnode;
+    else if (upload(s, &m.hnode_new, level_major<M>(s.hnode_new, L, s.perm_n, s.Np))) return -1;
     if (upload(s, &m.Z, level_major<M>(s.Z, L, s.perm_n, s.Np))) return -1;
     if (upload(s, &m.area, level_major<M>(s.area, nl, s.perm_n, s.Np))) return -1;
-    if (upload(s, &m.areasvol, level_major<M>(s.areasvol, nl, s.perm_n, s.Np))) return -1;
+    if (same_bits(s.areasvol, s.area)) m.areasvol = m.area;
+    else if (upload(s, &m.areasvol, level_major<M>(s.areasvol, nl, s.perm_n, s.Np))) return -1;
     if (upload(s, &m.zbar, level_major<M>(s.zbar3, nl, s.perm_n, s.Np))) return -1;
     if (upload(s, &m.w, level_major<M>(s.w, nl, s.perm_n, s.Np))) return -1;
     m.we = m.w;

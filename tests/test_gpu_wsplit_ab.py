@@ -194,3 +194,26 @@ def test_velocity_update_keeps_tracers(oracle_mod):
     err = rel_linf(got, ref[2])
     say(f"  rel Linf {err:.3e}")
     assert err <= RTOL
+
+
+def test_distinct_thickness_and_volume_areas_match_oracle(oracle_mod):
+    """hnode_new != hnode (ALE layer motion) and areasvol != area: the kernels read the separate
+    arrays (identical inputs share one device array)."""
+    case = configs.bathymetry_case("ale", 10, 12, seed=59, nsteps=3, lev_lo=4, w_cfl=0.3)
+    m = case.mesh
+    rng = np.random.default_rng(7)
+    wet = np.arange(m.nl - 1)[None, :] < (m.nlevels_nod2D[:, None] - 1)
+    hn = np.where(wet, m.hnode * (1.0 + 0.02 * rng.uniform(-1, 1, m.hnode.shape)), m.hnode)
+    av = m.areasvol * np.where(rng.uniform(size=m.areasvol.shape) < 0.3, 0.97, 1.0)
+    ale = dataclasses.replace(case, mesh=dataclasses.replace(m, hnode_new=hn, areasvol=av))
+    assert not np.array_equal(ale.mesh.hnode_new, ale.mesh.hnode)
+    say("case distinct hnode_new / areasvol 3 steps")
+    ref, _ = oracle_mod.Oracle(ale).run(save_steps=[3])
+    for kb in (1, 2):
+        t = make(ale, kb=kb)
+        t.advect_steps(3, ale.dt)
+        got = t.get_all_values()
+        done(t)
+        err = rel_linf(got, ref[3])
+        say(f"  kb={kb} rel Linf {err:.3e}")
+        assert err <= RTOL
