@@ -321,19 +321,41 @@ __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
     const double cd2 = cld(m.ecd, 4 * (size_t)e + 2), cd3 = cld(m.ecd, 4 * (size_t)e + 3);
     const size_t Tp = m.Tp, Ep = m.Ep;
     const int top = max(nl1, nl2);
-    for (int nz = 1; nz <= top; ++nz) {
-        const int k = nz - 1;
+    // Levels are independent: the loads are unconditional (a dry element's level is clamped to
+    // a wet one, a missing element replaced by the first, and the value not used) and the loop
+    // is unrolled, so the loads of several levels are in flight together.
+    const int e2 = el2 >= 0 ? el2 : el1, w1 = max(nl1, 1), w2 = max(nl2, 1);
+    auto level = [&](int nz, double u1, double y1, double h1, double u2, double y2, double h2) {
         const bool use1 = nz <= nl1, use2 = nz <= nl2;
-        double v1 = 0, v2 = 0;
-        if (use1) {
-            const size_t i = (size_t)k * Tp + el1;
-            v1 = (-(double)cld(m.uv, 2 * i + 1) * cd0 + (double)cld(m.uv, 2 * i) * cd1) * (double)cld(m.helem, i);
+        const double v1 = (-y1 * cd0 + u1 * cd1) * h1;
+        const double v2 = (y2 * cd2 - u2 * cd3) * h2;
+        m.q[(size_t)(nz - 1) * Ep + e] = (use1 && use2) ? v1 + v2 : (use1 ? v1 : v2);
+    };
+    auto idx = [&](int nz, int w, int el) { return (size_t)(min(nz, w) - 1) * Tp + el; };
+    constexpr int B = 4;  // levels per batch: all their loads are issued before the first store
+    if (top >= B) {
+        for (int nb = 1; nb <= top; nb += B) {
+            const int n0 = min(nb, top - B + 1);  // the last batch moves back and rewrites equal values
+            double u1[B], y1[B], h1[B], u2[B], y2[B], h2[B];
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                const size_t i1 = idx(n0 + j, w1, el1), i2 = idx(n0 + j, w2, e2);
+                u1[j] = (double)cld(m.uv, 2 * i1);
+                y1[j] = (double)cld(m.uv, 2 * i1 + 1);
+                h1[j] = (double)cld(m.helem, i1);
+                u2[j] = (double)cld(m.uv, 2 * i2);
+                y2[j] = (double)cld(m.uv, 2 * i2 + 1);
+                h2[j] = (double)cld(m.helem, i2);
+            }
+#pragma unroll
+            for (int j = 0; j < B; ++j) level(n0 + j, u1[j], y1[j], h1[j], u2[j], y2[j], h2[j]);
         }
-        if (use2) {
-            const size_t i = (size_t)k * Tp + el2;
-            v2 = ((double)cld(m.uv, 2 * i + 1) * cd2 - (double)cld(m.uv, 2 * i) * cd3) * (double)cld(m.helem, i);
+    } else {
+        for (int nz = 1; nz <= top; ++nz) {
+            const size_t i1 = idx(nz, w1, el1), i2 = idx(nz, w2, e2);
+            level(nz, (double)cld(m.uv, 2 * i1), (double)cld(m.uv, 2 * i1 + 1), (double)cld(m.helem, i1),
+                  (double)cld(m.uv, 2 * i2), (double)cld(m.uv, 2 * i2 + 1), (double)cld(m.helem, i2));
         }
-        m.q[(size_t)k * Ep + e] = (use1 && use2) ? v1 + v2 : (use1 ? v1 : v2);
     }
 }
 
