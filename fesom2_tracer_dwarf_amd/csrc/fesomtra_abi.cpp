@@ -575,6 +575,13 @@ int refresh_velocity(State &s) {
 
 int build_device(State &s) {
     if (validate(s)) return -1;
+    if (!s.have_vel) {  // no velocity given yet: at rest, as the reference's zero-initialised arrays
+        s.uv.assign((size_t)2 * (s.nl - 1) * s.T, 0.0);
+        s.w.assign((size_t)s.nl * s.N, 0.0);
+        s.wi = s.w;
+        s.we = s.w;
+        s.have_vel = true;
+    }
     for (const auto &sc : s.schemes)
         if ((sc.hor == HOR_MUSCL || sc.hor == HOR_MFCT) && !s.have_muscl)
             return fail("MUSCL/MFCT need edge_up_dn_tri (tracer_set_muscl_support)");
@@ -1467,7 +1474,8 @@ int tracer_set_values_ab(int tracer_id, const double *values, int nl, int nod2D)
 int tracer_set_velocity_zero(int nl, int nod2D, int edge2D) {
     State &s = S();
     if (check_dims(s, nl, nod2D, "tracer_set_velocity_zero")) return -1;
-    if (edge2D != s.E) return fail("tracer_set_velocity_zero: edge2D=%d != %d", edge2D, s.E);
+    // edge2D only sizes the reference's own (unused) edge array; every velocity here is zeroed
+    if (edge2D < 0) return fail("tracer_set_velocity_zero: edge2D=%d", edge2D);
     s.uv.assign((size_t)2 * (nl - 1) * s.T, 0.0);
     s.w.assign((size_t)nl * nod2D, 0.0);
     s.wi = s.w;
@@ -1540,6 +1548,10 @@ int tracer_load_mesh_from_files(const char *mesh_path) {
                                  fm.N, fm.T) ||
         tracer_set_zbar(fm.zbar.data(), fm.nl) || tracer_set_muscl_support(fm.updn.data(), fm.nbl.data(), fm.N, fm.E))
         return -1;
+    // as the reference (:830-870), a loaded mesh counts as initialised: tracer_get_size, the ALE
+    // and advection set-up calls and stepping are available before tracer_init_complete
+    if (validate(S())) return -1;
+    S().initialized = true;
     return 0;
 }
 

@@ -63,7 +63,7 @@ int tracer_set_partition(int myDim_nod2D, int eDim_nod2D);
 int tracer_allocate_tracers(int num_tracers, int nl, int nod2D, int AB_order);
 /* :613 values(nl-1, nod2D), tracer_id 1-based. */
 int tracer_set_values(int tracer_id, const double *values, int nl, int nod2D);
-/* :652 */
+/* :652 uv, w, w_i, w_e := 0.  edge2D only sized the reference's own edge array and is not checked. */
 int tracer_set_velocity_zero(int nl, int nod2D, int edge2D);
 /* :694 The reference stores uv as (nl-1, edge2D, 2) although the kernels read element velocities
  *      vel(2, nl-1, elem2D); this entry point accepts only an all-zero uv (sets w) and otherwise
@@ -78,7 +78,10 @@ int tracer_load_mesh_partition(const char *restart_dir);
  *      the model grid (default Euler angles 50, 15, -90 degrees), orients the triangles and
  *      derives the spherical geometry and the MUSCL support arrays as the reference's mesh set-up
  *      does (longitude differences wrapped into [-pi, pi]); replaces tracer_set_mesh_dims and the
- *      mesh setters.  Tracers, schemes and element velocities follow as with an array mesh. */
+ *      mesh setters.  Tracers, schemes and element velocities follow as with an array mesh.  As
+ *      in the reference, the loaded mesh counts as initialised: tracer_get_size,
+ *      tracer_init_ale_arrays, tracer_init_for_advection and stepping work before
+ *      tracer_init_complete (velocities default to zero). */
 int tracer_load_mesh_from_files(const char *mesh_path);
 /* :875 hnode = hnode_new = helem = zbar(k) - zbar(k+1) from tracer_set_zbar. */
 int tracer_init_ale_arrays(void);
