@@ -21,7 +21,9 @@ KERNELS = "fesomtra_kernels.hip"
 TILE = "fesomtra_tile.hip"
 ABI = "fesomtra_abi.cpp"
 MESHFILE = "fesomtra_meshfile.cpp"
-HEADERS = ["fesomtra_types.hpp", "fesomtra_column.hpp", "fesomtra_kernels.hpp", "fesomtra_tile.hpp"]
+RESTART = "fesomtra_restart.cpp"
+HEADERS = ["fesomtra_types.hpp", "fesomtra_column.hpp", "fesomtra_kernels.hpp", "fesomtra_tile.hpp",
+           "fesomtra_meshfile.hpp", "fesomtra_restart.hpp"]
 # The kernel source is compiled once per precision family (FESOMTRA_INST bit), in parallel.
 FAMILIES = {"f64": 1, "f32": 2, "f16": 4, "bf16": 8}
 
@@ -77,6 +79,9 @@ def build(verbose: bool = False, defines=(), out: Path = LIB) -> Path:
         objs.append(obj)
         obj = str(Path(td) / "meshfile.o")
         cmds.append([hipcc(), *HIPFLAGS, *inc, "-c", str(CSRC / MESHFILE), "-o", obj])
+        objs.append(obj)
+        obj = str(Path(td) / "restart.o")
+        cmds.append([hipcc(), *HIPFLAGS, *inc, "-c", str(CSRC / RESTART), "-o", obj])
         objs.append(obj)
         _run(cmds, verbose)
         out = Path(out)

@@ -19,7 +19,10 @@
 #include <string>
 #include <vector>
 
+#include <sys/stat.h>
+
 #include "../../include/fesomtra.h"
+#include "fesomtra_restart.hpp"
 #include "fesomtra_meshfile.hpp"
 #include "fesomtra_kernels.hpp"
 #include "fesomtra_tile.hpp"
@@ -77,6 +80,8 @@ struct State {
     // fused single-pass path (FP64, FCT, one tracer per block)
     bool fused = false;
     bool wsplit = false;  // dynamics%use_wsplit
+    RestartFiles rs;            // restart files the state was read from (tracer_finalize rewrites them)
+    bool from_restart = false;
     int ab_mode = 0;      // valuesAB per step: 0 := values, 1 frozen, 2 AB2 interpolation
     double ab_eps = 0.1;  // AB2 offset epsilon
     std::vector<void *> toldbuf;                    // per block: valuesold (ab_mode 2)
@@ -1259,6 +1264,183 @@ int stats(State &s, int id, int all_levels, double *mn, double *mx, double *sm, 
     return 0;
 }
 
+
+// Library state -> restart items (lib/io_restart_derivedtype.F90).  A state read from restart
+// files keeps every item of those files; the tracers, schemes, MUSCL support arrays and
+// velocities are written from the current state, the other items of a state built from arrays
+// are what the mesh setters hold (connectivity and levels 1-based as in the reference).
+void restart_of_state(const State &s, RestartFiles &o) {
+    if (s.from_restart) o = s.rs;
+    const int N = s.N, T = s.T, E = s.E, nl = s.nl, L = nl - 1;
+    auto plus1 = [](const std::vector<int> &v) {
+        std::vector<int> r(v.size());
+        for (size_t i = 0; i < v.size(); ++i) r[i] = v[i] + 1;  // -1 (none) becomes 0
+        return r;
+    };
+    if (!s.from_restart) {
+        o.set_scalar_int("mesh.nod2D", N);
+        o.set_scalar_int("mesh.edge2D", E);
+        o.set_scalar_int("mesh.edge2D_in", E);
+        o.set_scalar_int("mesh.elem2D", T);
+        o.set_scalar_int("mesh.nl", nl);
+        o.set_ints("mesh.elem2D_nodes", plus1(s.enodes), {3, T});
+        o.set_ints("mesh.edges", plus1(s.edges), {2, E});
+        o.set_ints("mesh.edge_tri", plus1(s.etri), {2, E});
+        o.set_reals("mesh.elem_area", s.elem_area, {T});
+        o.set_reals("mesh.edge_dxdy", s.edxdy, {2, E});
+        o.set_reals("mesh.edge_cross_dxdy", s.ecd, {4, E});
+        o.set_reals("mesh.elem_cos", s.elem_cos, {T});
+        std::vector<int> nie((size_t)s.max_adj * N, 0);
+        for (int n = 0; n < N; ++n)
+            for (int k = 0; k < s.nie_num[n]; ++k) nie[(size_t)n * s.max_adj + k] = s.nie[(size_t)n * s.max_adj + k] + 1;
+        o.set_ints("mesh.nod_in_elem2D", nie, {s.max_adj, N});
+        o.set_ints("mesh.nod_in_elem2D_num", s.nie_num, {N});
+        o.set_reals("mesh.gradient_sca", s.gsca, {6, T});
+        if (s.have_zbar) o.set_reals("mesh.zbar", s.zbar, {nl});
+        o.set_ints("mesh.ulevels", s.ulev_e, {T});
+        o.set_ints("mesh.ulevels_nod2D", s.ulev_n, {N});
+        o.set_ints("mesh.nlevels", s.nlev_e, {T});
+        o.set_ints("mesh.nlevels_nod2D", s.nlev_n, {N});
+        // nlevels_nod2D_min: the fewest levels of the node's elements (lib/oce_mesh.F90:1522-1560)
+        std::vector<int> nmin(N, nl);
+        for (int t = 0; t < T; ++t)
+            for (int c = 0; c < 3; ++c) {
+                const int n = s.enodes[3 * (size_t)t + c];
+                nmin[n] = std::min(nmin[n], s.nlev_e[t]);
+            }
+        o.set_ints("mesh.nlevels_nod2D_min", nmin, {N});
+        std::vector<int> umax(N, 1);  // ulevels_nod2D_max: the deepest top level of its elements
+        for (int t = 0; t < T; ++t)
+            for (int c = 0; c < 3; ++c) {
+                const int n = s.enodes[3 * (size_t)t + c];
+                umax[n] = std::max(umax[n], s.ulev_e[t]);
+            }
+        o.set_ints("mesh.ulevels_nod2D_max", umax, {N});
+        o.set_reals("mesh.area", s.area, {nl, N});
+        o.set_reals("mesh.areasvol", s.areasvol, {nl, N});
+        o.set_reals("mesh.hnode", s.hnode, {L, N});
+        o.set_reals("mesh.hnode_new", s.hnode_new, {L, N});
+        o.set_reals("mesh.zbar_3d_n", s.zbar3, {nl, N});
+        o.set_reals("mesh.Z_3d_n", s.Z, {L, N});
+        o.set_reals("mesh.helem", s.helem, {L, T});
+        o.set_scalar_int("npes", 1);
+        o.set_scalar_int("mype", 0);
+        o.set_scalar_int("maxPEnum", 100);
+        o.set_scalar_int("myDim_nod2D", N);
+        o.set_scalar_int("myDim_elem2D", T);
+        o.set_scalar_int("myDim_edge2D", E);
+        o.set_scalar_int("dyn.opt_visc", 5);
+    }
+    o.set_scalar_int("num_tracers", s.ntr);
+    auto hor_name = [](int h) { return h == HOR_MUSCL ? "MUSCL" : h == HOR_MFCT ? "MFCT" : "UPW1"; };
+    auto ver_name = [](int v) { return v == VER_QR4C ? "QR4C" : v == VER_CDIFF ? "CDIFF" : v == VER_PPM ? "PPM" : "UPW1"; };
+    for (int i = 0; i < s.ntr; ++i) {
+        const std::string p = "tracer." + std::to_string(i + 1) + ".";
+        o.set_reals(p + "values", s.values[i], {L, N});
+        // valuesAB: the time level the next step reads (values unless the library keeps it apart)
+        const bool ab = s.ab_mode != 0 && i < (int)s.ab_host.size() && !s.ab_host[i].empty();
+        o.set_reals(p + "valuesAB", ab ? s.ab_host[i] : s.values[i], {L, N});
+        const SchemeSel &sc = s.schemes[i];
+        o.set_chars(p + "tra_adv_hor", hor_name(sc.hor));
+        o.set_chars(p + "tra_adv_ver", ver_name(sc.ver));
+        o.set_chars(p + "tra_adv_lim", sc.fct ? "FCT" : "NONE");
+        o.set_scalar_real(p + "tra_adv_ph", sc.ph);
+        o.set_scalar_real(p + "tra_adv_pv", sc.pv);
+        if (!o.get(p + "ID")) o.set_scalar_int(p + "ID", i + 1);
+    }
+    if (s.have_muscl) {
+        o.set_ints("work.nboundary_lay", s.nbl, {N});
+        o.set_ints("work.edge_up_dn_tri", plus1(s.updn), {2, E});
+    }
+    o.set_logical("dyn.use_wsplit", s.wsplit);
+    if (s.have_vel) {
+        o.set_reals("dyn.uv", s.uv, {2, L, T});
+        o.set_reals("dyn.w", s.w, {nl, N});
+        o.set_reals("dyn.w_e", s.we, {nl, N});
+        o.set_reals("dyn.w_i", s.wi, {nl, N});
+    }
+}
+
+// tracer_init / tracer_load_mesh_partition: <dir>/np1 restart files -> the setters.
+int load_restart_dir(const char *dir, bool tracers, const char *who) {
+    const std::string np = std::string(dir) + "/np1";
+    struct stat st;
+    if (::stat(np.c_str(), &st) != 0) return fail("could not find restart directory: %s", np.c_str());
+    RestartFiles r;
+    std::string err;
+    if (read_restart(np, tracers, r, err)) return fail("%s: %s", who, err.c_str());
+    const int N = r.scalar_int("mesh.nod2D"), E = r.scalar_int("mesh.edge2D"), T = r.scalar_int("mesh.elem2D");
+    const int nl = r.scalar_int("mesh.nl"), L = nl - 1;
+    auto need = [&](const std::string &name, size_t n) -> bool {
+        const RsField *f = r.get(name);
+        const size_t have = f ? f->data.size() / (f->kind == 'r' ? 8 : 4) : 0;
+        if (have == n) return true;
+        fail("%s: %s/%s has %zu entries, expected %zu (one partition, eDim = 0)", who, np.c_str(), name.c_str(), have, n);
+        return false;
+    };
+    const RsField *nief = r.get("mesh.nod_in_elem2D");
+    const int max_adj = nief && nief->dims.size() == 2 ? nief->dims[0] : 0;
+    if (!need("mesh.elem2D_nodes", 3 * (size_t)T) || !need("mesh.edges", 2 * (size_t)E) ||
+        !need("mesh.edge_tri", 2 * (size_t)E) || !need("mesh.nod_in_elem2D", (size_t)max_adj * N) ||
+        !need("mesh.nod_in_elem2D_num", N) || !need("mesh.nlevels_nod2D", N) || !need("mesh.ulevels_nod2D", N) ||
+        !need("mesh.nlevels", T) || !need("mesh.ulevels", T) || !need("mesh.elem_area", T) ||
+        !need("mesh.area", (size_t)nl * N) || !need("mesh.areasvol", (size_t)nl * N) ||
+        !need("mesh.edge_dxdy", 2 * (size_t)E) || !need("mesh.edge_cross_dxdy", 4 * (size_t)E) ||
+        !need("mesh.elem_cos", T) || !need("mesh.gradient_sca", 6 * (size_t)T) || !need("mesh.hnode", (size_t)L * N) ||
+        !need("mesh.hnode_new", (size_t)L * N) || !need("mesh.helem", (size_t)L * T) ||
+        !need("mesh.Z_3d_n", (size_t)L * N) || !need("mesh.zbar_3d_n", (size_t)nl * N))
+        return -1;
+    const auto enodes = r.ints("mesh.elem2D_nodes"), edges = r.ints("mesh.edges"), etri = r.ints("mesh.edge_tri");
+    const auto nie = r.ints("mesh.nod_in_elem2D"), nien = r.ints("mesh.nod_in_elem2D_num");
+    const auto nlev_n = r.ints("mesh.nlevels_nod2D"), ulev_n = r.ints("mesh.ulevels_nod2D");
+    const auto nlev_e = r.ints("mesh.nlevels"), ulev_e = r.ints("mesh.ulevels");
+    const auto ea = r.reals("mesh.elem_area"), ar = r.reals("mesh.area"), av = r.reals("mesh.areasvol");
+    const auto dxdy = r.reals("mesh.edge_dxdy"), ecd = r.reals("mesh.edge_cross_dxdy"), ec = r.reals("mesh.elem_cos");
+    const auto gs = r.reals("mesh.gradient_sca"), hn = r.reals("mesh.hnode"), hnn = r.reals("mesh.hnode_new");
+    const auto he = r.reals("mesh.helem"), Z = r.reals("mesh.Z_3d_n"), zb3 = r.reals("mesh.zbar_3d_n"), zb = r.reals("mesh.zbar");
+    if (tracer_set_mesh_dims(nl, N, E, T) ||
+        tracer_set_partition(r.scalar_int("myDim_nod2D"), r.scalar_int("eDim_nod2D")) ||
+        tracer_set_connectivity(enodes.data(), edges.data(), etri.data(), nie.data(), nien.data(), max_adj, N, T, E) ||
+        tracer_set_levels(nlev_n.data(), ulev_n.data(), N) || tracer_set_elem_levels(nlev_e.data(), ulev_e.data(), T) ||
+        tracer_set_geometry(ea.data(), ar.data(), av.data(), dxdy.data(), ecd.data(), ec.data(), gs.data(), nl, N, T, E) ||
+        tracer_set_vertical_grid(hn.data(), hnn.data(), he.data(), Z.data(), zb3.data(), nl, N, T) ||
+        (zb.size() == (size_t)nl && tracer_set_zbar(zb.data(), nl)))  // zbar: only tracer_init_ale_arrays reads it
+        return -1;
+    const auto updn = r.ints("work.edge_up_dn_tri"), nbl = r.ints("work.nboundary_lay");
+    if (updn.size() == 2 * (size_t)E && nbl.size() == (size_t)N &&
+        tracer_set_muscl_support(updn.data(), nbl.data(), N, E))
+        return -1;
+    // velocities (not allocated in the file: at rest; w_e / w_i of another shape: w)
+    std::vector<double> uv = r.reals("dyn.uv"), w = r.reals("dyn.w"), we = r.reals("dyn.w_e"), wi = r.reals("dyn.w_i");
+    if (uv.size() != 2 * (size_t)L * T) uv.assign(2 * (size_t)L * T, 0.0);
+    if (w.size() != (size_t)nl * N) w.assign((size_t)nl * N, 0.0);
+    if (tracer_set_velocity_elem(uv.data(), w.data(), wi.size() == w.size() ? wi.data() : nullptr,
+                                 we.size() == w.size() ? we.data() : nullptr, nl, N, T) ||
+        tracer_set_wsplit(r.logical("dyn.use_wsplit") ? 1 : 0))
+        return -1;
+    if (tracers) {
+        const int ntr = r.scalar_int("num_tracers");
+        if (ntr < 1) return fail("%s: %s/t_tracer.0 holds no tracers", who, np.c_str());
+        if (tracer_allocate_tracers(ntr, nl, N, 2)) return -1;
+        for (int i = 1; i <= ntr; ++i) {
+            const std::string p = "tracer." + std::to_string(i) + ".";
+            if (!need(p + "values", (size_t)L * N)) return -1;
+            const auto v = r.reals(p + "values");
+            if (tracer_set_values(i, v.data(), nl, N) ||
+                tracer_set_scheme(i, r.chars(p + "tra_adv_hor").c_str(), r.chars(p + "tra_adv_ver").c_str(),
+                                  r.chars(p + "tra_adv_lim").c_str(), r.scalar_real(p + "tra_adv_ph", 1.0),
+                                  r.scalar_real(p + "tra_adv_pv", 1.0)))
+                return -1;
+        }
+    }
+    State &s = S();
+    if (validate(s)) return -1;
+    s.rs = std::move(r);
+    s.from_restart = true;
+    s.initialized = true;  // the reference's tracer_initialized
+    return 0;
+}
+
 }  // namespace
 
 // =============================================================================================
@@ -1273,8 +1455,8 @@ int tracer_init_mpi(int mpi_comm) {
 int tracer_finalize_mpi(void) { return 0; }
 
 int tracer_init(const char *restart_dir) {
-    return fail("tracer_init(%s): derived-type restart files are not supported; initialise from "
-                "arrays (tracer_set_* then tracer_init_complete)", restart_dir ? restart_dir : "");
+    if (!restart_dir) return fail("tracer_init: null path");
+    return load_restart_dir(restart_dir, true, "tracer_init");
 }
 
 int tracer_advect_step(double dt) {
@@ -1331,13 +1513,30 @@ int tracer_get_values_by_id(int tracer_id, double *values, int nz, int nn) {
 int tracer_get_values(double *values, int nz, int nn) { return tracer_get_values_by_id(1, values, nz, nn); }
 
 int tracer_finalize(const char *output_dir) {
-    return fail("tracer_finalize(%s): restart output is not supported", output_dir ? output_dir : "");
+    State &s = S();
+    if (!s.initialized) return fail("tracer not initialized.");
+    if (!output_dir) return fail("tracer_finalize: null path");
+    if (s.on_device && pull_values(s)) return -1;
+    RestartFiles out;
+    restart_of_state(s, out);
+    std::string err;
+    const std::string dir(output_dir);
+    if (write_restart(dir + "/np1", dir + "/meta.time", out, s.ntr, err)) return fail("tracer_finalize: %s", err.c_str());
+    std::printf("Restart files written to: %s/np1\n", output_dir);
+    std::fflush(stdout);
+    s.initialized = false;  // as the reference: initialise again before stepping
+    return 0;
 }
 
 int tracer_run_workflow(const char *restart_dir, int nsteps, double dt) {
-    (void)nsteps;
-    (void)dt;
-    return tracer_init(restart_dir);
+    if (tracer_init(restart_dir)) return -1;
+    for (int step = 1; step <= nsteps; ++step) {
+        double tmin, tmax, tsum;
+        if (tracer_advect_step(dt) || tracer_get_stats(&tmin, &tmax, &tsum)) return -1;
+        std::printf("Step %3d: min/max/sum = %15.6E%15.6E%15.6E\n", step, tmin, tmax, tsum);
+    }
+    std::fflush(stdout);
+    return tracer_finalize(restart_dir);
 }
 
 int tracer_cleanup(void) {
@@ -1361,7 +1560,9 @@ int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D) {
     const hipStream_t keep = s.stream;
     hipEvent_t k0 = s.ev0, k1 = s.ev1;
     const int prec = s.prec, kbr = s.kb_req;
-    const bool graph = s.use_graph, sr = s.stochastic, fused = s.fused, ht = s.halo_tiles;
+    const bool graph = s.use_graph, sr = s.stochastic, fused = s.fused, ht = s.halo_tiles, ws = s.wsplit;
+    const int abm = s.ab_mode;
+    const double abe = s.ab_eps;
     const uint64_t gen = s.gen;
     const LaunchCfg cfg = s.cfg;
     s = State();
@@ -1375,6 +1576,9 @@ int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D) {
     s.stochastic = sr;
     s.fused = fused;
     s.halo_tiles = ht;
+    s.wsplit = ws;
+    s.ab_mode = abm;
+    s.ab_eps = abe;
     s.gen = gen + 1;
     s.nl = nl;
     s.N = nod2D;
@@ -1527,7 +1731,8 @@ int tracer_init_complete(void) {
 }
 
 int tracer_load_mesh_partition(const char *restart_dir) {
-    return fail("tracer_load_mesh_partition(%s): partition files are not supported", restart_dir ? restart_dir : "");
+    if (!restart_dir) return fail("tracer_load_mesh_partition: null path");
+    return load_restart_dir(restart_dir, false, "tracer_load_mesh_partition");
 }
 
 int tracer_load_mesh_from_files(const char *mesh_path) {

@@ -32,8 +32,11 @@ extern "C" {
 int tracer_init_mpi(int mpi_comm);
 /* :89 */
 int tracer_finalize_mpi(void);
-/* :113 Restart-file initialisation: the derived-type restart format is not supported;
- *      returns -1 with a message. */
+/* :113 Reads the derived-type restart files <restart_dir>/np1/t_mesh.0, t_partit.0, t_tracer.0
+ *      and t_dynamics.0 (lib/io_restart_derivedtype.F90, as the reference's FP64 build writes
+ *      them: one unformatted record per type) and fills mesh, velocities (w, w_i, w_e, use_wsplit),
+ *      tracers and their schemes through the setters; one partition (eDim = 0), no cavities.
+ *      -1 "could not find restart directory" when <restart_dir>/np1 is missing. */
 int tracer_init(const char *restart_dir);
 /* :163 One time step of every tracer: zero tendencies, do_oce_adv_tra, values += (dttf_h +
  *      dttf_v)/hnode_new, with valuesAB := values.  (The reference advects tracer 1 only and
@@ -45,9 +48,14 @@ int tracer_get_stats(double *tmin, double *tmax, double *tsum);
 int tracer_get_size(int *nz, int *nn);
 /* :280 Copies tracer 1 out, values(nz=nl-1, nn=nod2D). */
 int tracer_get_values(double *values, int nz, int nn);
-/* :308 Writes restart files in the reference: unsupported, returns -1. */
+/* :308 Writes <output_dir>/np1/t_*.0 and <output_dir>/meta.time in the reference's restart
+ *      format with the current tracers (and their valuesAB), schemes and velocities; a state read
+ *      by tracer_init keeps every other item of its files, a state built from arrays writes the
+ *      items the advection reads (the rest as not allocated).  Ends the initialised state, as the
+ *      reference does. */
 int tracer_finalize(const char *output_dir);
-/* :356 Restart-based workflow: unsupported, returns -1. */
+/* :356 tracer_init(restart_dir), nsteps x (tracer_advect_step, tracer_get_stats, a line
+ *      "Step n: min/max/sum = ..."), tracer_finalize(restart_dir). */
 int tracer_run_workflow(const char *restart_dir, int nsteps, double dt);
 /* :391 Releases all device and host state. */
 int tracer_cleanup(void);
@@ -71,7 +79,8 @@ int tracer_set_velocity_zero(int nl, int nod2D, int edge2D);
 int tracer_set_velocity(const double *uv, const double *w, int nl, int nod2D, int edge2D);
 /* :749 Validates the mesh, reorders it along a space-filling curve and uploads it to HBM. */
 int tracer_init_complete(void);
-/* :769 Restart-format mesh partitions: unsupported, returns -1. */
+/* :769 As tracer_init without the tracers: mesh and velocities from <restart_dir>/np1/t_mesh.0,
+ *      t_partit.0 and t_dynamics.0; tracers follow with tracer_allocate_tracers. */
 int tracer_load_mesh_partition(const char *restart_dir);
 /* :830 Reads a FESOM2 mesh directory (nod2d.out, elem2d.out, aux3d.out, nlvls.out, elvls.out,
  *      edgenum.out, edges.out, edge_tri.out; one partition, no cavities), rotates the nodes to

@@ -18,6 +18,7 @@
 !                                                 dynamics%use_wsplit from the bundle
 !     values += (del_ttf_advhoriz + del_ttf_advvert) / hnode_new
 !
+! Usage: ref_driver <bundle-dir> [<restart-dir>]
 ! Outputs (in the bundle directory):
 !   out_values_sNNNN.bin   all tracers (nl-1, nod2D, ntr) after each requested step
 !   out_w_*.bin            work arrays of tracer 1 at step 1 (edge_up_dn_grad before advection,
@@ -35,6 +36,8 @@ program ref_driver
   use oce_adv_tra_driver_module, only: do_oce_adv_tra
   use oce_adv_tra_fct_module, only: oce_adv_tra_fct_init
   use oce_muscl_adv_module, only: fill_up_dn_grad
+  use restart_derivedtype_module, only: write_all_bin_restarts, read_all_bin_restarts
+  use fortran_utils, only: int_to_txt, mkdir
 #if defined(_OPENMP)
   use omp_lib
 #endif
@@ -44,7 +47,9 @@ program ref_driver
   type(t_partit), target :: partit
   type(t_tracer), target :: tracers
   type(t_dyn),    target :: dyn
-  character(len=4096)    :: dir
+  character(len=4096)    :: dir, rdir, arg
+  logical                :: rmode
+  integer, dimension(3)  :: ctime = (/0, 0, 0/)
   integer :: ierr, nl, nod2D, elem2D, edge2D, max_adj, ntr, nsteps, nsave, i, n, k, e, istep, nz
   integer :: save_work, timing, use_wsplit, ab_mode
   real(kind=WP), allocatable :: vold(:,:,:)
@@ -61,10 +66,42 @@ program ref_driver
   call par_init(partit)
   call get_command_argument(1, dir)
   if (len_trim(dir) == 0) then
-     write(*,*) 'usage: ref_driver <bundle-dir>'
+     write(*,*) 'usage: ref_driver <bundle-dir> [<restart-dir>] | --restart <restart-dir> <out-dir> <nsteps> <dt>'
      call MPI_Abort(MPI_COMM_WORLD, 2, ierr)
   end if
-
+  rmode = trim(dir) == '--restart'
+  if (rmode) then
+     ! state from derived-type restart files; values after the last step into <out-dir>
+     call get_command_argument(2, rdir)
+     call get_command_argument(3, dir)
+     call get_command_argument(4, arg)
+     read(arg, *) nsteps
+     call get_command_argument(5, arg)
+     read(arg, *) dt
+     call read_all_bin_restarts(trim(rdir)//'/np'//int_to_txt(partit%npes), partit=partit, mesh=mesh, &
+          dynamics=dyn, tracers=tracers)
+     nl = mesh%nl; nod2D = mesh%nod2D; elem2D = mesh%elem2D; edge2D = mesh%edge2D
+     ntr = tracers%num_tracers
+     save_work = 0; timing = 0; ab_mode = 0
+     nsave = 1
+     allocate(save_steps(1))
+     save_steps(1) = nsteps
+     rdir = ''
+     if (.not. allocated(mesh%zbar_n_bot)) then
+        allocate(mesh%zbar_n_bot(nod2D))
+        do k = 1, nod2D
+           mesh%zbar_n_bot(k) = mesh%zbar_3d_n(mesh%nlevels_nod2D(k), k)
+        end do
+     end if
+     allocate(tbuf(nl-1, nod2D, ntr), vold(nl-1, nod2D, ntr))
+     do n = 1, ntr
+        vold(:, :, n) = tracers%data(n)%values
+        if (.not. allocated(tracers%data(n)%valuesAB)) allocate(tracers%data(n)%valuesAB(nl-1, nod2D))
+     end do
+     if (allocated(tracers%work%fct_LO)) deallocate(tracers%work%fct_LO, tracers%work%adv_flux_hor, &
+          tracers%work%adv_flux_ver, tracers%work%fct_ttf_max, tracers%work%fct_ttf_min, &
+          tracers%work%fct_plus, tracers%work%fct_minus)
+  else
   open(newunit=u, file=trim(dir)//'/dims.txt', status='old', action='read')
   read(u,*) nl, nod2D, elem2D, edge2D, max_adj
   read(u,*) ntr, nsteps, dt
@@ -89,13 +126,6 @@ program ref_driver
   partit%myDim_nod2D = nod2D;   partit%eDim_nod2D = 0
   partit%myDim_elem2D = elem2D; partit%eDim_elem2D = 0; partit%eXDim_elem2D = 0
   partit%myDim_edge2D = edge2D; partit%eDim_edge2D = 0
-#if defined(_OPENMP)
-  ! the node locks of the reference's OpenMP scatter (as lib/gen_modules_partitioning.F90:578-585)
-  allocate(partit%plock(max(nod2D, elem2D)))
-  do i = 1, size(partit%plock)
-     call omp_init_lock(partit%plock(i))
-  end do
-#endif
 
   allocate(mesh%elem2D_nodes(3, elem2D), mesh%edges(2, edge2D), mesh%edge_tri(2, edge2D))
   allocate(mesh%nod_in_elem2D(max_adj, nod2D), mesh%nod_in_elem2D_num(nod2D))
@@ -152,14 +182,23 @@ program ref_driver
   end do
   allocate(vold(nl-1, nod2D, ntr))
   vold = tbuf
-
-  allocate(tracers%work%del_ttf_advhoriz(nl-1, nod2D), tracers%work%del_ttf_advvert(nl-1, nod2D))
-  call oce_adv_tra_fct_init(tracers%work, partit, mesh)
   allocate(tracers%work%edge_up_dn_tri(2, edge2D), tracers%work%nboundary_lay(nod2D))
-  allocate(tracers%work%edge_up_dn_grad(4, nl-1, edge2D))
-  tracers%work%edge_up_dn_grad = 0.0_MP
   call rd_i2(tracers%work%edge_up_dn_tri, 'edge_up_dn_tri')
   call rd_i1(tracers%work%nboundary_lay, 'nboundary_lay')
+  end if
+#if defined(_OPENMP)
+  ! the node locks of the reference's OpenMP scatter (as lib/gen_modules_partitioning.F90:578-585)
+  allocate(partit%plock(max(nod2D, elem2D)))
+  do i = 1, size(partit%plock)
+     call omp_init_lock(partit%plock(i))
+  end do
+#endif
+
+  if (.not. allocated(tracers%work%del_ttf_advhoriz)) &
+       allocate(tracers%work%del_ttf_advhoriz(nl-1, nod2D), tracers%work%del_ttf_advvert(nl-1, nod2D))
+  call oce_adv_tra_fct_init(tracers%work, partit, mesh)
+  if (.not. allocated(tracers%work%edge_up_dn_grad)) allocate(tracers%work%edge_up_dn_grad(4, nl-1, edge2D))
+  tracers%work%edge_up_dn_grad = 0.0_MP
   allocate(tr_xy(2, nl-1, elem2D))
   tr_xy = 0.0_WP
 
@@ -222,6 +261,15 @@ program ref_driver
      write(u, '(A,ES16.8)') 'seconds_advection ', t_adv
      write(u, '(A,ES16.8)') 'seconds_update ', t_upd
      close(u)
+  end if
+  ! optional second argument: write the state as the reference's derived-type restart files
+  ! (lib/io_restart_derivedtype.F90) into <dir>/np<npes>/ and <dir>/meta.time
+  if (.not. rmode) call get_command_argument(2, rdir)
+  if (len_trim(rdir) > 0) then
+     call mkdir(trim(rdir))
+     call mkdir(trim(rdir)//'/np'//int_to_txt(partit%npes))
+     call write_all_bin_restarts(ctime, trim(rdir)//'/np'//int_to_txt(partit%npes), trim(rdir)//'/meta.time', &
+          partit=partit, mesh=mesh, dynamics=dyn, tracers=tracers)
   end if
   call MPI_Finalize(ierr)
 
