@@ -91,3 +91,29 @@ def test_mesh_from_files_sequence_runs(tmp_path):
         assert abs(tsum.value - temperature.sum()) <= 1e-12 * abs(temperature.sum())
     assert lib.tracer_finalize_mpi() == 0
     lib.tracer_cleanup()
+
+
+@pytest.mark.gpu
+def test_demo_tracer_restart_workflow(tmp_path):
+    """python/demo_tracer.py:210-260 through the Python wrapper's reference-named methods: restart
+    files in, ten steps with statistics, restart files out (a reference-written restart)."""
+    import shutil
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    print("case demo_tracer restart workflow (10 steps)", flush=True)
+    path = tmp_path / "fesom_bin_restart"
+    shutil.copytree(GOLDEN / "restart_ref" / "rst2", path)
+    before = (path / "np1" / "t_tracer.0").read_bytes()
+    t = FESOMTracerAdvection()
+    t.lib.tracer_cleanup()
+    assert t.initialize_mpi(0) == 0
+    assert t.initialize(str(path)) == 0
+    sums = []
+    for _ in range(10):
+        assert t.advect_step(1.0e-3) == 0
+        st = t.get_statistics()
+        assert np.isfinite(st["min"]) and st["min"] <= st["max"]
+        sums.append(st["sum"])
+    assert t.finalize(str(path)) == 0
+    assert t.finalize_mpi() == 0
+    assert (path / "np1" / "t_tracer.0").read_bytes() != before  # the advanced tracers were written
+    t.cleanup()
