@@ -95,3 +95,19 @@ def test_oracle_equals_live_reference_pi_mesh(oracle_mod, tmp_path):
     bundle.run_ref(p)
     out, _ = oracle_mod.Oracle(case).run(save_steps=[3])
     assert np.array_equal(out[3], bundle.read_values(p, case, 3))
+
+
+@needs_ref
+@pytest.mark.parametrize("nl", [3, 4])
+@pytest.mark.parametrize("hor,ver,lim", [("MFCT", "QR4C", "FCT"), ("MUSCL", "PPM", "FCT"), ("UPW1", "CDIFF", "NONE"),
+                                         ("MFCT", "QR4C", "NONE")])
+def test_oracle_equals_live_reference_thin_columns(oracle_mod, tmp_path, nl, hor, ver, lim):
+    """Two- and three-layer columns everywhere: the surface, second and bottom rows of QR4C / PPM
+    and the limiter's vertical widening meet (including QR4C's double update of a two-layer column,
+    lib/oce_adv_tra_ver.F90:405-411)."""
+    case = configs.bathymetry_case("thin", 8, nl, seed=5, nsteps=3, lev_lo=nl,
+                                   scheme=Scheme(hor, ver, lim, 0.8, 0.7), w_cfl=0.3)
+    p = bundle.write_bundle(tmp_path / "b", case, save_steps=[3])
+    bundle.run_ref(p)
+    out, _ = oracle_mod.Oracle(case).run(save_steps=[3])
+    assert np.array_equal(out[3], bundle.read_values(p, case, 3))
