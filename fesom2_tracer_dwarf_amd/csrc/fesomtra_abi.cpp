@@ -85,6 +85,8 @@ struct State {
     int ab_mode = 0;      // valuesAB per step: 0 := values, 1 frozen, 2 AB2 interpolation
     double ab_eps = 0.1;  // AB2 offset epsilon
     std::vector<void *> toldbuf;                    // per block: valuesold (ab_mode 2)
+    std::vector<void *> tabbuf2;                    // per block: valuesAB of the next step (ab_mode 2)
+    bool ab_fresh = true;  // ab_mode 2: valuesAB to be formed from values and valuesold first
     std::vector<std::vector<double>> ab_host, old_host;  // valuesAB / valuesold across rebuilds
     std::vector<void *> ckpt_ab, ckpt_old;          // per block: copies kept by tracer_checkpoint
     int ckpt_ab_mode = 0;
@@ -190,6 +192,7 @@ void release_device(State &s) {
     s.tab_on.clear();
     s.ckpt.clear();
     s.toldbuf.clear();
+    s.tabbuf2.clear();
     s.ckpt_ab.clear();
     s.ckpt_old.clear();
     s.dT2.clear();
@@ -795,7 +798,9 @@ int ab_prepare(State &s) {
         if (!s.tabbuf[b] && dalloc_bytes(s, &s.tabbuf[b], bytes)) return -1;
         HIPCHK(hipMemcpyAsync(s.tabbuf[b], s.dT[b], bytes, hipMemcpyDeviceToDevice, s.stream));
         if (s.ab_mode == 2) {
+            if (s.tabbuf2.size() != s.dT.size()) s.tabbuf2.assign(s.dT.size(), nullptr);
             if (!s.toldbuf[b] && dalloc_bytes(s, &s.toldbuf[b], bytes)) return -1;
+            if (!s.tabbuf2[b] && dalloc_bytes(s, &s.tabbuf2[b], bytes)) return -1;
             HIPCHK(hipMemcpyAsync(s.toldbuf[b], s.dT[b], bytes, hipMemcpyDeviceToDevice, s.stream));
         }
         s.tab_on[b] = 1;
@@ -811,6 +816,7 @@ int ab_prepare(State &s) {
     }
     s.ab_host.clear();
     s.old_host.clear();
+    s.ab_fresh = true;
     HIPCHK(hipStreamSynchronize(s.stream));
     return 0;
 }
@@ -860,13 +866,22 @@ int ensure_dropin_buffers(State &s) {
     return 0;
 }
 
+// ab_parity (AB2 steps): 0 reads valuesAB from the primary buffer and writes the next step's to the
+// second one, 1 the other way round.
 template <class S_, class C_>
-StepPtrs<S_, C_> make_ptrs(State &s, int blk, unsigned long long mask, bool dropin, uint32_t seed) {
+StepPtrs<S_, C_> make_ptrs(State &s, int blk, unsigned long long mask, bool dropin, uint32_t seed, int ab_parity) {
     StepPtrs<S_, C_> p{};
     p.tid0 = blk * s.kb;
     p.seed = (s.prec != PREC_F64 && s.stochastic) ? seed : 0u;
     p.T = static_cast<S_ *>(s.dT[blk]);
     p.TAB = static_cast<const S_ *>(s.tab_on[blk] ? s.tabbuf[blk] : s.dT[blk]);
+    if (s.ab_mode == 2 && !dropin && blk < (int)s.tabbuf2.size() && s.tabbuf2[blk]) {
+        p.TAB = static_cast<const S_ *>(ab_parity ? s.tabbuf2[blk] : s.tabbuf[blk]);
+        p.ab_next = static_cast<S_ *>(ab_parity ? s.tabbuf[blk] : s.tabbuf2[blk]);
+        p.ab_old = static_cast<S_ *>(s.toldbuf[blk]);
+        p.ab_c0 = 0.5 + s.ab_eps;
+        p.ab_c1 = 1.5 + s.ab_eps;
+    }
     p.adfh = static_cast<C_ *>(s.adfh);
     p.LO = static_cast<C_ *>(s.LO);
     p.Rp = static_cast<C_ *>(s.Rp);
@@ -884,13 +899,13 @@ StepPtrs<S_, C_> make_ptrs(State &s, int blk, unsigned long long mask, bool drop
 }
 
 hipError_t launch_block(State &s, int blk, unsigned long long mask, const SchemeSel &sc, double dt, bool dropin,
-                        hipEvent_t *ev, uint32_t seed = 0) {
+                        hipEvent_t *ev, uint32_t seed = 0, int ab_parity = 0) {
     SchemeArgs a{sc.hor, sc.ver, sc.fct, sc.ph, sc.pv, dt, s.wsplit && sc.fct};
     switch (s.prec) {
-    case PREC_F64: return launch_step<double, double, double>(s.m64, make_ptrs<double, double>(s, blk, mask, dropin, seed), s.kb, a, s.cfg, s.stream, ev);
-    case PREC_F32: return launch_step<float, double, float>(s.m64, make_ptrs<float, float>(s, blk, mask, dropin, seed), s.kb, a, s.cfg, s.stream, ev);
-    case PREC_F32_F16: return launch_step<f16_t, double, float>(s.m64, make_ptrs<f16_t, float>(s, blk, mask, dropin, seed), s.kb, a, s.cfg, s.stream, ev);
-    default: return launch_step<bf16_t, double, float>(s.m64, make_ptrs<bf16_t, float>(s, blk, mask, dropin, seed), s.kb, a, s.cfg, s.stream, ev);
+    case PREC_F64: return launch_step<double, double, double>(s.m64, make_ptrs<double, double>(s, blk, mask, dropin, seed, ab_parity), s.kb, a, s.cfg, s.stream, ev);
+    case PREC_F32: return launch_step<float, double, float>(s.m64, make_ptrs<float, float>(s, blk, mask, dropin, seed, ab_parity), s.kb, a, s.cfg, s.stream, ev);
+    case PREC_F32_F16: return launch_step<f16_t, double, float>(s.m64, make_ptrs<f16_t, float>(s, blk, mask, dropin, seed, ab_parity), s.kb, a, s.cfg, s.stream, ev);
+    default: return launch_step<bf16_t, double, float>(s.m64, make_ptrs<bf16_t, float>(s, blk, mask, dropin, seed, ab_parity), s.kb, a, s.cfg, s.stream, ev);
     }
 }
 
@@ -1111,13 +1126,20 @@ std::vector<Launch> launch_plan(const State &s) {
 // nsteps of the launch plan on the library's stream.  Fused blocks alternate between their two
 // buffers and end in the primary one (one device copy after an odd number of steps), so the
 // stored values are always in dT whatever the path.  evs (optional) receives 5 events per pass.
-hipError_t run_steps(State &s, const std::vector<Launch> &plan, int nsteps, double dt, hipEvent_t *evs) {
+// AB2 (ab_mode 2): the update of every step also writes the next step's valuesAB and valuesold
+// (k_final); valuesAB alternates between two buffers and ends in the primary one.  The
+// interpolation kernel runs before the first step when valuesAB is stale (values or the mode
+// were just set) and always at the start of a captured sequence (it is idempotent: valuesAB
+// then already equals the interpolation of values and valuesold).
+hipError_t run_steps(State &s, const std::vector<Launch> &plan, int nsteps, double dt, hipEvent_t *evs,
+                     bool capture = false) {
     std::vector<char> where(s.dT.size(), 0);
+    const bool ab2 = s.ab_mode == 2;
     for (int st = 0; st < nsteps; ++st) {
         // edge transport of the velocity, shared by every tracer pass of the step
         hipError_t e = launch_qflux(s.m64, s.stream);
         if (e != hipSuccess) return e;
-        if (s.ab_mode == 2)
+        if (ab2 && st == 0 && (s.ab_fresh || capture))
             for (size_t b = 0; b < s.dT.size() && e == hipSuccess; ++b) e = launch_ab2_block(s, (int)b);
         if (e != hipSuccess) return e;
         const uint32_t seed = next_seed(s);
@@ -1133,7 +1155,7 @@ hipError_t run_steps(State &s, const std::vector<Launch> &plan, int nsteps, doub
                 if (e == hipSuccess && ev) e = hipEventRecord(ev[4], s.stream);
                 where[l.blk] ^= 1;
             } else {
-                e = launch_block(s, l.blk, l.mask, l.sc, dt, false, ev, seed);
+                e = launch_block(s, l.blk, l.mask, l.sc, dt, false, ev, seed, ab2 ? (st & 1) : 0);
             }
             if (e != hipSuccess) return e;
         }
@@ -1144,6 +1166,12 @@ hipError_t run_steps(State &s, const std::vector<Launch> &plan, int nsteps, doub
             hipError_t e = hipMemcpyAsync(s.dT[b], s.dT2[b], bytes, hipMemcpyDeviceToDevice, s.stream);
             if (e != hipSuccess) return e;
         }
+    if (ab2 && (nsteps & 1))  // the next step's valuesAB was written to the second buffer
+        for (size_t b = 0; b < s.dT.size(); ++b) {
+            hipError_t e = hipMemcpyAsync(s.tabbuf[b], s.tabbuf2[b], bytes, hipMemcpyDeviceToDevice, s.stream);
+            if (e != hipSuccess) return e;
+        }
+    if (!capture && nsteps > 0) s.ab_fresh = false;
     return hipSuccess;
 }
 
@@ -1151,7 +1179,7 @@ hipError_t run_steps(State &s, const std::vector<Launch> &plan, int nsteps, doub
 int capture_graph(State &s, const std::vector<Launch> &plan, int nsteps, double dt) {
     drop_graph(s);
     HIPCHK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
-    const hipError_t err = run_steps(s, plan, nsteps, dt, nullptr);
+    const hipError_t err = run_steps(s, plan, nsteps, dt, nullptr, true);
     hipGraph_t graph = nullptr;
     const hipError_t ec = hipStreamEndCapture(s.stream, &graph);
     if (err != hipSuccess || ec != hipSuccess) {
@@ -1183,6 +1211,7 @@ int advect(State &s, int nsteps, double dt) {
             if (capture_graph(s, plan, nsteps, dt)) return -1;
         HIPCHK(hipEventRecord(s.ev0, s.stream));
         HIPCHK(hipGraphLaunch(s.gexec, s.stream));
+        if (nsteps > 0) s.ab_fresh = false;
         HIPCHK(hipEventRecord(s.ev1, s.stream));
         HIPCHK(hipStreamSynchronize(s.stream));
         float ms = 0.f;
@@ -1652,6 +1681,7 @@ int tracer_set_values(int tracer_id, const double *values, int nl, int nod2D) {
         if (to_device(s, v.data(), false, s.dT[b], lane)) return -1;
         if (s.tab_on[b] && to_device(s, v.data(), false, s.tabbuf[b], lane)) return -1;
         if (s.ab_mode == 2 && s.toldbuf[b] && to_device(s, v.data(), false, s.toldbuf[b], lane)) return -1;
+        s.ab_fresh = true;
         touch_values(s);
     } else {
         if (tracer_id - 1 < (int)s.ab_host.size()) s.ab_host[tracer_id - 1].clear();
@@ -1988,6 +2018,7 @@ int tracer_checkpoint(int restore) {
             if (s.ab_mode == 2)
                 HIPCHK(hipMemcpyAsync(s.toldbuf[b], s.ckpt_old[b], bytes, hipMemcpyDeviceToDevice, s.stream));
         }
+        s.ab_fresh = true;  // recomputing valuesAB from the restored levels is exact
         if (s.ab_mode == 0) std::fill(s.tab_on.begin(), s.tab_on.end(), 0);
         touch_values(s);
         s.step_seed = s.ckpt_seed;  // the restored state continues the same rounding sequence
@@ -2173,6 +2204,7 @@ int tracer_set_values_device(int tracer_id, const void *values, int nl, int nod2
     if (s.tab_on[b] && to_device(s, static_cast<const double *>(values), true, s.tabbuf[b], lane)) return -1;
     if (s.ab_mode == 2 && s.toldbuf[b] && to_device(s, static_cast<const double *>(values), true, s.toldbuf[b], lane))
         return -1;
+    s.ab_fresh = true;
     touch_values(s);
     return 0;
 }

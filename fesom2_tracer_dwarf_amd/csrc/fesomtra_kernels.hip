@@ -870,7 +870,9 @@ struct EdgeVals {
     C f, rp, rm;
     double ao, q;
 };
-template <bool TILED, bool FCT, int KB, typename S, typename M, typename C>
+// AB2: also writes the next step's valuesAB and valuesold (a separate instantiation, so that the
+// other runs keep their register budget).
+template <bool TILED, bool FCT, int KB, typename S, typename M, typename C, bool AB2 = false>
 __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
     static_assert(!TILED || FCT, "tiles stage the limiter ratios");
     __shared__ C sRp[TILED ? 2 : 1][TILED ? kHtSlots : 1], sRm[TILED ? 2 : 1][TILED ? kHtSlots : 1];
@@ -1047,6 +1049,15 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
         } else {
             std_(p.T, X(cell), tval + (dh + dv) / hn);
         }
+        if constexpr (AB2) {
+            if (!p.acc_h) {
+                // FESOM2's AB2 interpolation for the next step from the value just stored and this
+                // step's (lib/oce_modules.F90:75): valuesAB = -(0.5+eps) valuesold + (1.5+eps)
+                // values with valuesold := the values this step started from
+                std_(p.ab_next, X(cell), -p.ab_c0 * tval + p.ab_c1 * ldc<double>(p.T, X(cell)));
+                std_(p.ab_old, X(cell), tval);
+            }
+        }
         top = bot;
     }
 }
@@ -1201,7 +1212,8 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
                 k_bounds<true, KB><<<tgrid(cfg.lc_bounds), 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
                 LAUNCH_CHECK();
                 MARK(3);
-                k_final<true, true, KB><<<tgrid(cfg.lc_final), 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+                if (p.ab_next) k_final<true, true, KB, S, M, C, true><<<tgrid(cfg.lc_final), 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+                else k_final<true, true, KB><<<tgrid(cfg.lc_final), 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
             }
         } else {
             switch (sc.ver) {
@@ -1221,7 +1233,8 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
             k_bounds<false, KB><<<gb, 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
             LAUNCH_CHECK();
             MARK(3);
-            k_final<false, true, KB><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+            if (p.ab_next) k_final<false, true, KB, S, M, C, true><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+            else k_final<false, true, KB><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
         }
     } else {
         switch (sc.ver) {
@@ -1234,7 +1247,8 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
         LAUNCH_CHECK();
         MARK(2);
         MARK(3);
-        k_final<false, false, KB><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+        if (p.ab_next) k_final<false, false, KB, S, M, C, true><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+        else k_final<false, false, KB><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
     }
     LAUNCH_CHECK();
     MARK(4);

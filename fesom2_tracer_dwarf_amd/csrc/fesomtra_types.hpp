@@ -186,6 +186,8 @@ struct StepPtrs {
     double *acc_h, *acc_v; // do_oce_adv_tra accumulators [L][Np] of the single selected lane, or null
     C *dbg_h, *dbg_v;      // limited fluxes [L][Ep], [nl][Np] of the single selected lane, or null
     double2 *tri;          // [L][Np][KB] (c', t') of the implicit vertical solve, or null
+    S *ab_next, *ab_old;   // AB2: valuesAB of the next step and valuesold, written by the update, or null
+    double ab_c0, ab_c1;   // 0.5 + epsilon, 1.5 + epsilon
     unsigned long long mask;  // lanes advanced by this launch (tracers sharing its scheme)
     int tid0;                 // tracer number of lane 0 (seeds the stochastic rounding)
     uint32_t seed;            // step counter (seeds the stochastic rounding); 0: round to nearest

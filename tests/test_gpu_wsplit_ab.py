@@ -217,3 +217,35 @@ def test_distinct_thickness_and_volume_areas_match_oracle(oracle_mod):
         err = rel_linf(got, ref[3])
         say(f"  kb={kb} rel Linf {err:.3e}")
         assert err <= RTOL
+
+
+def test_ab2_set_values_between_steps(oracle_mod):
+    """tracer_set_values during an AB2 run resets that tracer's valuesold (and so its next
+    valuesAB) to the new values, as the reference's setter does (lib/tracer_c_interface.F90:613-648),
+    while the other tracer keeps its time levels."""
+    case = ab_case(seed=61, nsteps=4)
+    e = 0.1
+    o = oracle_mod.Oracle(case)
+    v = [np.ascontiguousarray(case.tracers[i]).copy() for i in range(case.ntracers)]
+    old = [x.copy() for x in v]
+    fresh = 0.5 * case.tracers[0] + 1.0
+    for s in range(4):
+        if s == 2:
+            v[0] = fresh.copy()
+            old[0] = fresh.copy()
+        for i in range(case.ntracers):
+            ab = -(0.5 + e) * old[i] + (1.5 + e) * v[i]
+            old[i] = v[i].copy()
+            o.step_ab(v[i], ab, case.schemes[i], case.dt)
+    ref = np.stack(v)
+    say("case ab2 with tracer_set_values between steps")
+    for graph in (False, True):
+        t = make(case, kb=2, ab=2, graph=graph)
+        t.advect_steps(2, case.dt)
+        t.set_values(1, fresh)
+        t.advect_steps(2, case.dt)
+        got = t.get_all_values()
+        done(t)
+        err = rel_linf(got, ref)
+        say(f"  graph={graph} rel Linf {err:.3e}")
+        assert err <= RTOL
