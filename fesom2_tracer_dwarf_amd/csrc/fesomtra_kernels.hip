@@ -143,11 +143,20 @@ struct SlotLd {
 #endif
 constexpr int kEdgePipe = FESOMTRA_EDGE_PIPE;  // levels of loads in flight in the tiled edge kernel
 static_assert(kEdgePipe == 1 || kEdgePipe == 2, "edge pipeline depth 1 or 2");
+#ifndef FESOMTRA_EDGE_PAIR
+#define FESOMTRA_EDGE_PAIR 1
+#endif
+// With two levels in flight, stage both in LDS behind one barrier (four LDS buffers).
+constexpr bool kEdgePair = FESOMTRA_EDGE_PAIR != 0;
 
 // SAME (tiled form only): valuesAB is values, so only one field is staged.
 template <bool TILED, int HOR, bool FCT, int KB, typename S, typename M, typename C, bool SAME = false>
 __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, int lc) {
-    __shared__ C sT[TILED ? 2 : 1][TILED ? kHtSlots : 1], sA[TILED && !SAME ? 2 : 1][TILED && !SAME ? kHtSlots : 1];
+    // two levels in flight only where the registers allow it (one staged field, no MUSCL switch)
+    constexpr int P = (SAME && HOR != HOR_MUSCL) ? kEdgePipe : 1;
+    constexpr bool PAIR = TILED && P == 2 && kEdgePair;
+    constexpr int NB = PAIR ? 4 : 2;  // LDS buffers of the staged field, level nz in nz % NB
+    __shared__ C sT[TILED ? NB : 1][TILED ? kHtSlots : 1], sA[TILED && !SAME ? 2 : 1][TILED && !SAME ? kHtSlots : 1];
     EdgeTile<KB, M> et;
     int e, lane, nz0, nz1, lv0, lv1;
     bool act = true;
@@ -213,10 +222,8 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
         for (int i = 0; i < 4; ++i) sl[i] = cld(m.et_loc, 4 * (size_t)e + i);
     }
     auto slot = [&](int k) { return (sl[k >> 1] >> (16 * (k & 1))) & 0xffff; };
-    // staging registers and this edge's transport of the next kEdgePipe levels (loads of two
-    // levels in flight while a level is computed)
-    // two levels in flight only where the registers allow it (one staged field, no MUSCL switch)
-    constexpr int P = (SAME && HOR != HOR_MUSCL) ? kEdgePipe : 1;
+    // staging registers and this edge's transport of the next P levels (loads of two levels in
+    // flight while a level is computed)
     C stT[P][2], stA[SAME ? 1 : P][2], qn[P];
     auto fetch = [&](int nz, auto set) {
         constexpr int r = decltype(set)::value;
@@ -232,27 +239,24 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
         fetch(lv0, std::integral_constant<int, 0>{});
         if (P > 1 && lv0 < lv1) fetch(lv0 + 1, std::integral_constant<int, P - 1>{});
     }
-    auto level = [&](int nz, auto set) {
+    auto stage = [&](int nz, auto set) {
         constexpr int r = decltype(set)::value;
-        const int buf = nz & 1;
-        C qcur = 0;
-        if constexpr (TILED) {
-            et.each([&](int i, int) {
-                sT[buf][threadIdx.x + 256 * i] = stT[r][i];
-                if constexpr (!SAME)
-                    if (!same) sA[buf][threadIdx.x + 256 * i] = stA[r][i];
-            });
-            qcur = qn[r];
-            __syncthreads();
-            if (nz + P <= lv1) fetch(nz + P, set);
-            if (!act || nz > nz1) return;
-        }
+        et.each([&](int i, int) {
+            sT[nz % NB][threadIdx.x + 256 * i] = stT[r][i];
+            if constexpr (!SAME)
+                if (!same) sA[nz & 1][threadIdx.x + 256 * i] = stA[r][i];
+        });
+    };
+    // the edge flux of level nz (staged in LDS when TILED; qcur is its transport)
+    auto compute = [&](int nz, C qcur) {
+        if (TILED && (!act || nz > nz1)) return;
+        const int buf = nz % NB;
         const int k = nz - 1;
         const C v = TILED ? qcur : (C)cld(m.q, (size_t)k * Ep + e);
         const size_t o = (size_t)k * Np;
         const size_t ie = ((size_t)k * Ep + e) * KB + lane;
         const SlotLd<KB, C> lsT{sT[TILED ? buf : 0], lane},
-            lsA{same ? sT[TILED ? buf : 0] : sA[TILED && !SAME ? buf : 0], lane};
+            lsA{same ? sT[TILED ? buf : 0] : sA[TILED && !SAME ? (nz & 1) : 0], lane};
         C t1 = 0, t2 = 0;
         if (FCT || same) {
             t1 = TILED ? lsT(slot(0)) : ldT(o + n1);
@@ -297,11 +301,39 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
         }
         p.adfh[ie] = f;
     };
-    // unrolled by the pipeline depth, so every register set is addressed statically
-    for (int nz = lv0; nz <= lv1; nz += P) {
-        level(nz, std::integral_constant<int, 0>{});
-        if constexpr (P > 1)
-            if (nz + 1 <= lv1) level(nz + 1, std::integral_constant<int, P - 1>{});
+    // one level: stage it, barrier, issue the loads of level nz + P, compute
+    auto level = [&](int nz, auto set) {
+        constexpr int r = decltype(set)::value;
+        C qcur = 0;
+        if constexpr (TILED) {
+            stage(nz, set);
+            qcur = qn[r];
+            __syncthreads();
+            if (nz + P <= lv1) fetch(nz + P, set);
+        }
+        compute(nz, qcur);
+    };
+    if constexpr (PAIR) {
+        // levels nz and nz + 1 staged together behind one barrier; the buffers of the pair
+        // before are free once every thread has passed it
+        for (int nz = lv0; nz <= lv1; nz += 2) {
+            const bool two = nz + 1 <= lv1;
+            stage(nz, std::integral_constant<int, 0>{});
+            if (two) stage(nz + 1, std::integral_constant<int, 1>{});
+            const C q0 = qn[0], q1 = qn[1];
+            __syncthreads();
+            if (nz + 2 <= lv1) fetch(nz + 2, std::integral_constant<int, 0>{});
+            if (nz + 3 <= lv1) fetch(nz + 3, std::integral_constant<int, 1>{});
+            compute(nz, q0);
+            if (two) compute(nz + 1, q1);
+        }
+    } else {
+        // unrolled by the pipeline depth, so every register set is addressed statically
+        for (int nz = lv0; nz <= lv1; nz += P) {
+            level(nz, std::integral_constant<int, 0>{});
+            if constexpr (P > 1)
+                if (nz + 1 <= lv1) level(nz + 1, std::integral_constant<int, P - 1>{});
+        }
     }
 }
 
@@ -363,6 +395,12 @@ __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
 // Node -> edge entries cached in registers (the first kDegCache, the rest read from memory).
 // The far end is a node number (m.ne_other) or, with PACK (halo tiles), the far node's tile
 // slot (m.ht_loc, < 2^15) kept with the entry's deepest level in one register.
+// Six entries cover an interior node of a regular triangulation and most nodes of an ocean mesh;
+// eight spilled registers in the tiled FP64 node kernels (1.25 -> 1.12 ms for the low-order
+// kernel on the 1000x1000x48 mesh with six).
+#ifndef FESOMTRA_DEG_CACHE_NODE
+#define FESOMTRA_DEG_CACHE_NODE 6
+#endif
 #ifndef FESOMTRA_DEG_CACHE_WIDE
 #define FESOMTRA_DEG_CACHE_WIDE 6
 #endif
@@ -427,7 +465,7 @@ constexpr bool kDivReuse = KB == 64;
 // Node -> edge entries held in registers: with one column per wavefront (KB = 64) they are
 // scalar registers, which the wide kernels are short of.
 template <int KB>
-constexpr int kDegCacheFor = KB == 64 ? FESOMTRA_DEG_CACHE_WIDE : kDegCache;
+constexpr int kDegCacheFor = KB == 64 ? FESOMTRA_DEG_CACHE_WIDE : FESOMTRA_DEG_CACHE_NODE;
 
 template <bool TWO, bool PACK, int NC, typename M, typename LOAD, typename USE>
 __device__ __forceinline__ void for_edges_ld(const MeshCol<M> &m, const NodeEdges<PACK, NC> &ne, int nz, LOAD &&load,
@@ -514,18 +552,17 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         if (VER == VER_QR4C) qw.init(tw, z);
         adf_top = ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, fin_top, qw.q);
     }
+    // staging registers of the next level (tiled form; the neighbour values of one level in
+    // flight while a level is computed: staging two levels behind one barrier measured slower)
     D st[2];
-    if constexpr (TILED)
-        ht.each([&](int i, int node) { st[i] = ldc<D>(p.T, ((size_t)(lv0 - 1) * Np + node) * KB + lane); });
-    for (int nz = lv0; nz <= lv1; ++nz) {
+    auto fetch = [&](int nz) {
+        ht.each([&](int i, int node) { st[i] = ldc<D>(p.T, ((size_t)(nz - 1) * Np + node) * KB + lane); });
+    };
+    auto stage = [&](int nz) { ht.each([&](int i, int) { sT[nz & 1][threadIdx.x + 256 * i] = st[i]; }); };
+    // layer nz of this column (its neighbours' values staged in sT[nz & 1] when TILED)
+    auto body = [&](int nz) {
+        if (TILED && (!act || nz > nz1)) return;
         const int buf = nz & 1;
-        if constexpr (TILED) {
-            ht.each([&](int i, int) { sT[buf][threadIdx.x + 256 * i] = st[i]; });
-            __syncthreads();
-            if (nz < lv1)
-                ht.each([&](int i, int node) { st[i] = ldc<D>(p.T, ((size_t)nz * Np + node) * KB + lane); });
-            if (!act || nz > nz1) continue;
-        }
         const int j = nz + 1;  // bottom interface of layer nz
         if (VER == VER_QR4C) {
             z[0] = z[1];
@@ -578,6 +615,17 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         lo_top = lo_bot;
         adf_top = adf_bot;
         tcur = tnext;
+    };
+    if constexpr (!TILED) {
+        for (int nz = lv0; nz <= lv1; ++nz) body(nz);
+    } else {
+        fetch(lv0);
+        for (int nz = lv0; nz <= lv1; ++nz) {
+            stage(nz);
+            __syncthreads();
+            if (nz < lv1) fetch(nz + 1);
+            body(nz);
+        }
     }
 }
 
