@@ -467,18 +467,18 @@ constexpr bool kDivReuse = KB == 64;
 template <int KB>
 constexpr int kDegCacheFor = KB == 64 ? FESOMTRA_DEG_CACHE_WIDE : FESOMTRA_DEG_CACHE_NODE;
 
-template <bool TWO, bool PACK, int NC, typename M, typename LOAD, typename USE>
-__device__ __forceinline__ void for_edges_ld(const MeshCol<M> &m, const NodeEdges<PACK, NC> &ne, int nz, LOAD &&load,
-                                             USE &&use) {
-    if constexpr (!TWO) {
-        for_edges(m, ne, nz, [&](int e, bool second, int far) { use(load(e, second, far), e, second, far); });
-        return;
-    }
-    using V = decltype(load(0, false, 0));
-    V v[NC];
+// The load phase of the cached entries active at level nz, into v; edges_use consumes them
+// (and loads and consumes the entries past the cache).  Split so that other loads can be issued
+// between the two.
+template <bool PACK, int NC, typename LOAD, typename V>
+__device__ __forceinline__ void edges_pre(const NodeEdges<PACK, NC> &ne, int nz, LOAD &&load, V (&v)[NC]) {
 #pragma unroll
     for (int i = 0; i < NC; ++i)
         if (i < ne.deg && nz <= ne.top_of(i)) v[i] = load(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
+}
+template <bool PACK, int NC, typename M, typename V, typename LOAD, typename USE>
+__device__ __forceinline__ void edges_use(const MeshCol<M> &m, const NodeEdges<PACK, NC> &ne, int nz, const V (&v)[NC],
+                                          LOAD &&load, USE &&use) {
 #pragma unroll
     for (int i = 0; i < NC; ++i)
         if (i < ne.deg && nz <= ne.top_of(i)) use(v[i], ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
@@ -489,6 +489,18 @@ __device__ __forceinline__ void for_edges_ld(const MeshCol<M> &m, const NodeEdge
             use(load(e2 >> 1, (e2 & 1) != 0, far), e2 >> 1, (e2 & 1) != 0, far);
         }
     }
+}
+template <bool TWO, bool PACK, int NC, typename M, typename LOAD, typename USE>
+__device__ __forceinline__ void for_edges_ld(const MeshCol<M> &m, const NodeEdges<PACK, NC> &ne, int nz, LOAD &&load,
+                                             USE &&use) {
+    if constexpr (!TWO) {
+        for_edges(m, ne, nz, [&](int e, bool second, int far) { use(load(e, second, far), e, second, far); });
+        return;
+    }
+    using V = decltype(load(0, false, 0));
+    V v[NC];
+    edges_pre(ne, nz, load, v);
+    edges_use(m, ne, nz, v, load, use);
 }
 
 // Low-order solution and vertical antidiffusive flux (lib/oce_adv_tra_driver.F90:102-216 and
@@ -839,17 +851,33 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
     // P+/P-, Q+/Q- and the ratios of layer nz with cluster bounds (hi, lw) and low-order
     // value lo; the vertical flux at the layer's bottom is kept for the next layer (av_bot)
     C av_bot = 0;
-    auto finish = [&](int nz, D hi, D lw, D lo) {
+    // the memory inputs of finish (the cached edges' antidiffusive fluxes, the vertical fluxes at
+    // the layer's top and bottom, the volume), loaded before the cluster is formed so that both
+    // sets of loads are in flight together
+    struct Pre {
+        double V;
+        C av0, av1;
+        C f[decltype(ne)::kN];
+    };
+    auto preload = [&](int nz, Pre &pre) {
         const size_t cell = (size_t)(nz - 1) * Np + n;
-        const double V = (double)cld(m.areasvol, cell);
+        pre.V = (double)cld(m.areasvol, cell);
+        pre.av0 = nz == nz0 ? p.adfv[cell * KB + lane] : av_bot;
+        pre.av1 = p.adfv[(cell + Np) * KB + lane];
+        const size_t eo = (size_t)(nz - 1) * Ep;
+        edges_pre(ne, nz, [&](int e, bool, int) { return p.adfh[(eo + e) * KB + lane]; }, pre.f);
+    };
+    auto finish = [&](int nz, D hi, D lw, D lo, const Pre &pre) {
+        const size_t cell = (size_t)(nz - 1) * Np + n;
+        const double V = pre.V;
         const D qmax = hi - lo, qmin = lw - lo;
-        const C av0 = nz == nz0 ? p.adfv[cell * KB + lane] : av_bot, av1 = p.adfv[(cell + Np) * KB + lane];
+        const C av0 = pre.av0, av1 = pre.av1;
         av_bot = av1;
         double pp = 0.0 + ((double)cmax(C(0), av0) + (double)cmax(C(0), -av1));
         double pm = 0.0 + ((double)cmin(C(0), av0) + (double)cmin(C(0), -av1));
         const size_t eo = (size_t)(nz - 1) * Ep;
-        for_edges_ld<true>(
-            m, ne, nz, [&](int e, bool, int) { return p.adfh[(eo + e) * KB + lane]; },
+        edges_use(
+            m, ne, nz, pre.f, [&](int e, bool, int) { return p.adfh[(eo + e) * KB + lane]; },
             [&](C a, int, bool second, int) {
                 if (second) a = -a;
                 pp = pp + (double)cmax(C(0), a);
@@ -894,14 +922,17 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         mx1 = mx0;
         mn1 = mn0;
         lo1 = lo0;
+        const int nz = j - 1;
+        const bool fin = nz >= nz0 && nz <= nz1;
+        Pre pre;
+        if (fin) preload(nz, pre);
         if (j <= L) {
             cluster(j, buf, mx0, mn0, lo0);
             if constexpr (TILED) lo0 = lo_j;
         }
-        const int nz = j - 1;
-        if (nz >= nz0 && nz <= nz1) {
+        if (fin) {
             const bool own = nz == 1 || nz == L;
-            finish(nz, own ? mx1 : bmax(bmax(mx2, mx1), mx0), own ? mn1 : bmin(bmin(mn2, mn1), mn0), lo1);
+            finish(nz, own ? mx1 : bmax(bmax(mx2, mx1), mx0), own ? mn1 : bmin(bmin(mn2, mn1), mn0), lo1, pre);
         }
     }
 }
@@ -918,11 +949,21 @@ struct EdgeVals {
     C f, rp, rm;
     double ao, q;
 };
+#ifndef FESOMTRA_FINAL_PRE
+#define FESOMTRA_FINAL_PRE 1
+#endif
+// Untiled update kernel: the edge inputs of a layer are loaded before the column's own inputs
+// are used (one memory round per layer instead of two), where the registers allow it (one column
+// per wavefront, or single-precision values; narrow FP64 columns would spill).
+constexpr bool kFinalPre = FESOMTRA_FINAL_PRE != 0;
+template <bool TILED, int KB, typename C>
+constexpr bool kFinalPreFor = kFinalPre && !TILED && (KB == 64 || !std::is_same<C, double>::value);
 // AB2: also writes the next step's valuesAB and valuesold (a separate instantiation, so that the
 // other runs keep their register budget).
 template <bool TILED, bool FCT, int KB, typename S, typename M, typename C, bool AB2 = false>
 __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
     static_assert(!TILED || FCT, "tiles stage the limiter ratios");
+    constexpr bool PRE = kFinalPreFor<TILED, KB, C>;
     __shared__ C sRp[TILED ? 2 : 1][TILED ? kHtSlots : 1], sRm[TILED ? 2 : 1][TILED ? kHtSlots : 1];
     HaloTile<KB, M> ht;
     int n, lane, L, nz0, nz1, lv0, lv1;
@@ -1005,6 +1046,25 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
             if (!act || nz > nz1) continue;
         }
         const size_t cell = (size_t)(nz - 1) * Np + n;
+        const size_t eo = (size_t)(nz - 1) * Ep, no = (size_t)(nz - 1) * Np;
+        // single precision without limiter: the stored flux is high-order minus upwind; the
+        // upwind part of valuesAB is added here in double
+        constexpr bool kAddUpw = !FCT && !std::is_same<C, double>::value;
+        // untiled: the cached edges' inputs are loaded first, in flight with the column's own
+        auto edge_load = [&](int e, bool, int oth) {
+            EdgeVals<C> v{p.adfh[X(eo + e)], C(0), C(0), 0.0, 0.0};
+            if (FCT) {
+                v.rp = p.Rp[X(no + oth)];
+                v.rm = p.Rm[X(no + oth)];
+            }
+            if (kAddUpw) {
+                v.ao = ldc<double>(p.TAB, X(no + oth));
+                v.q = cld(m.q, eo + e);
+            }
+            return v;
+        };
+        EdgeVals<C> pre[PRE ? decltype(ne)::kN : 1];
+        if constexpr (PRE) edges_pre(ne, nz, edge_load, pre);
         const double V = (double)cld(m.areasvol, cell);
         const double hn = (double)cld(m.hnode_new, cell);
         const double h = FCT ? (double)cld(m.hnode, cell) : 0.0;
@@ -1037,10 +1097,6 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
         dv = dv + (kDivReuse<KB> ? div_reuse(((double)top - (double)bot) * dt, V, yV) : ((double)top - (double)bot) * dt / V);
         double dh = p.acc_h ? p.acc_h[cell] : 0.0;
         const C rpn = FCT ? rp0 : C(0), rmn = FCT ? rm0 : C(0);
-        const size_t eo = (size_t)(nz - 1) * Ep, no = (size_t)(nz - 1) * Np;
-        // single precision without limiter: the stored flux is high-order minus upwind; the
-        // upwind part of valuesAB is added here in double
-        constexpr bool kAddUpw = !FCT && !std::is_same<C, double>::value;
         const double an = kAddUpw ? ldc<double>(p.TAB, X(cell)) : 0.0;
         auto add_edge = [&](const EdgeVals<C> &v, int e, bool second) {
             C f = v.f;
@@ -1071,22 +1127,11 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
                 [&](C f, int e, bool second, int oth) {
                     add_edge(EdgeVals<C>{f, sRp[buf][oth * KB + lane], sRm[buf][oth * KB + lane], 0.0, 0.0}, e, second);
                 });
+        } else if constexpr (PRE) {
+            edges_use(m, ne, nz, pre, edge_load, [&](const EdgeVals<C> &v, int e, bool second, int) { add_edge(v, e, second); });
         } else {
-            for_edges_ld<kTwoPhase<KB>>(
-                m, ne, nz,
-                [&](int e, bool, int oth) {
-                    EdgeVals<C> v{p.adfh[X(eo + e)], C(0), C(0), 0.0, 0.0};
-                    if (FCT) {
-                        v.rp = p.Rp[X(no + oth)];
-                        v.rm = p.Rm[X(no + oth)];
-                    }
-                    if (kAddUpw) {
-                        v.ao = ldc<double>(p.TAB, X(no + oth));
-                        v.q = cld(m.q, eo + e);
-                    }
-                    return v;
-                },
-                [&](const EdgeVals<C> &v, int e, bool second, int) { add_edge(v, e, second); });
+            for_edges_ld<kTwoPhase<KB>>(m, ne, nz, edge_load,
+                                        [&](const EdgeVals<C> &v, int e, bool second, int) { add_edge(v, e, second); });
         }
         if (p.acc_h) {
             p.acc_h[cell] = dh;
