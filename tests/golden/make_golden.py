@@ -6,11 +6,12 @@ generator is detected instead of silently comparing different cases.
 
     python tests/golden/make_golden.py        (needs oracle/_ref built by oracle/ref/build_ref.sh)
     python tests/golden/make_golden.py --only pi   (only the FESOM2 pi-mesh fixture)
-    python tests/golden/make_golden.py --only wsplit_ab   (only the split-w / valuesAB fixture)
+    python tests/golden/make_golden.py --only wsplit_ab   (only the split-w / valuesAB / ALE fixture)
 """
 
 from __future__ import annotations
 
+import dataclasses
 import hashlib
 import sys
 import tempfile
@@ -104,7 +105,21 @@ def wsplit_ab_cases():
                                 scheme=Scheme("MUSCL", "QR4C", "FCT", 0.8, 0.7), w_cfl=0.3)
     out["ab_frozen"] = (c, 1)
     out["ab_ab2"] = (c, 2)
+    out["ale_MFCT_QR4C"] = (ale_case(), 0)
     return out
+
+
+def ale_case(seed=59, ver="QR4C"):
+    """Layer thickness that changes over the step (hnode_new != hnode, as under ALE) and volume
+    areas that differ from the interface areas (areasvol != area)."""
+    c = configs.bathymetry_case("ale", 10, 12, seed=seed, nsteps=3, lev_lo=4,
+                                scheme=Scheme("MFCT", ver, "FCT", 0.8, 0.7), w_cfl=0.3)
+    m = c.mesh
+    rng = np.random.default_rng(seed + 1)
+    wet = np.arange(m.nl - 1)[None, :] < (m.nlevels_nod2D[:, None] - 1)
+    hn = np.where(wet, m.hnode * (1.0 + 0.02 * rng.uniform(-1, 1, m.hnode.shape)), m.hnode)
+    av = m.areasvol * np.where(rng.uniform(size=m.areasvol.shape) < 0.3, 0.97, 1.0)
+    return dataclasses.replace(c, mesh=dataclasses.replace(m, hnode_new=hn, areasvol=av))
 
 
 def write_wsplit_ab(td):

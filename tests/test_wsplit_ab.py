@@ -65,6 +65,20 @@ def test_oracle_equals_live_reference_ab_modes(oracle_mod, tmp_path, ab):
     assert np.array_equal(out[4], bundle.read_values(p, case, 4))
 
 
+@needs_ref
+@pytest.mark.parametrize("ver", ["QR4C", "PPM"])
+def test_oracle_equals_live_reference_ale_thickness(oracle_mod, tmp_path, ver):
+    """hnode_new != hnode and areasvol != area: the LO solution (lib/oce_adv_tra_driver.F90:208),
+    the flux divergence (:339, :360, :411, :424), the limiter's P+/P- (lib/oce_adv_tra_fct.F90:368-370)
+    and the caller's update (src/fesom_analytic.F90:435) use the new thickness and the volume areas."""
+    case = make_golden.ale_case(seed=61, ver=ver)
+    assert not np.array_equal(case.mesh.hnode_new, case.mesh.hnode)
+    p = bundle.write_bundle(tmp_path / "ale", case, save_steps=[3], save_work=True)
+    bundle.run_ref(p)
+    out, _ = oracle_mod.Oracle(case).run(save_steps=[3])
+    assert np.array_equal(out[3], bundle.read_values(p, case, 3))
+
+
 def _vert_impl(oracle_mod, case, lo):
     """fo_adv_tra_vert_impl on a (N, nl-1) field with the case's w_i."""
     o = oracle_mod.Oracle(case)
