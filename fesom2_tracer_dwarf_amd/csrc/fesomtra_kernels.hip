@@ -392,7 +392,7 @@ __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Node -> edge entries cached in registers (the first kDegCache, the rest read from memory).
+// Node -> edge entries cached in registers (the first NC, the rest read from memory).
 // The far end is a node number (m.ne_other) or, with PACK (halo tiles), the far node's tile
 // slot (m.ht_loc, < 2^15) kept with the entry's deepest level in one register.
 // Six entries cover an interior node of a regular triangulation and most nodes of an ocean mesh;
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
 #ifndef FESOMTRA_DEG_CACHE_WIDE
 #define FESOMTRA_DEG_CACHE_WIDE 6
 #endif
-template <bool PACK, int NC = kDegCache>
+template <bool PACK, int NC>
 struct NodeEdges {
     static constexpr int kN = NC;
     int b, deg;

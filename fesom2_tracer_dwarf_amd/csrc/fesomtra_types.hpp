@@ -36,7 +36,6 @@ namespace fesomtra {
 constexpr double kREarth = 6367500.0;  // lib/oce_modules.F90:24
 constexpr int kXcds = 8;               // MI355X: 8 accelerator complex dies, one L2 each
 constexpr int kMaxKB = 64;             // widest tracer block (one wavefront of lanes)
-constexpr int kDegCache = 8;           // node->edge entries held in registers
 
 enum Hor { HOR_UPW1 = 0, HOR_MUSCL = 1, HOR_MFCT = 2 };
 enum Ver { VER_UPW1 = 0, VER_QR4C = 1, VER_CDIFF = 2, VER_PPM = 3 };
