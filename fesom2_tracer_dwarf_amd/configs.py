@@ -26,7 +26,7 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from .mesh import Mesh, periodic_mesh, smooth_bathymetry_levels, stretched_layers, union_mesh
+from .mesh import Mesh, cavity_ulevels, periodic_mesh, smooth_bathymetry_levels, stretched_layers, union_mesh
 
 # Scheme names as the reference spells them (lib/oce_adv_tra_driver.F90:244-282,286).
 HOR_SCHEMES = ("UPW1", "MUSCL", "MFCT")
@@ -195,11 +195,13 @@ def bathymetry_case(name: str, n: int, nl: int, seed: int, ntracers: int = 2,
                     permute: bool = True, nsteps: int = 100, umax: float = 0.1,
                     depth_scale: Optional[float] = 500.0, lev_lo: int = 10,
                     lev_hi: Optional[int] = None, L: float = 2.0e6,
-                    scheme: Optional[Scheme] = None, w_cfl: float = 0.0) -> Case:
+                    scheme: Optional[Scheme] = None, w_cfl: float = 0.0, cavity: int = 0) -> Case:
     """n x n periodic mesh with nl levels stretched 5-250 m and smooth random bathymetry.
 
     ``w_cfl`` > 0 adds an interior vertical velocity whose largest vertical Courant number
-    |w| dt / h_min equals w_cfl (surface and bottom interfaces stay at w = 0).
+    |w| dt / h_min equals w_cfl (surface and bottom interfaces stay at w = 0).  ``cavity`` > 0
+    puts an ice-shelf cavity up to that many layers deep over the domain's centre (ulevels > 1);
+    the velocity above an element's top is zero.
     """
     ss = np.random.SeedSequence(seed)
     s_bath, s_perm, s_tr, s_w = ss.spawn(4)
@@ -207,9 +209,13 @@ def bathymetry_case(name: str, n: int, nl: int, seed: int, ntracers: int = 2,
     hi = nl if lev_hi is None else lev_hi
     lev = smooth_bathymetry_levels(n, n, nl, min(lev_lo, hi), hi,
                                    int(s_bath.generate_state(1)[0]))
-    mesh = periodic_mesh(n, n, L, L, layer_h, elem_nlevels=lev,
+    ulev = cavity_ulevels(n, n, lev, cavity) if cavity else None
+    mesh = periodic_mesh(n, n, L, L, layer_h, elem_nlevels=lev, elem_ulevels=ulev,
                          seed=int(s_perm.generate_state(1)[0]), permute=permute)
     uv = streamfunction_velocity(mesh, umax, depth_scale)
+    if cavity:
+        dry = np.arange(1, mesh.nl)[None, :] < mesh.ulevels[:, None]
+        uv = np.where(dry[..., None] if uv.ndim == 3 and uv.shape[-1] == 2 else dry, 0.0, uv)
     w = np.zeros((mesh.nod2D, mesh.nl))
     dt = cfl_dt(mesh, umax)
     if w_cfl:

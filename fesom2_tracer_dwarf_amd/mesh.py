@@ -118,12 +118,14 @@ def _ragged_lists(owner: np.ndarray, member: np.ndarray, n_owner: int):
 
 
 def periodic_mesh(nx: int, ny: int, Lx: float, Ly: float, layer_h: np.ndarray, *,
-                  elem_nlevels: Optional[np.ndarray] = None,
+                  elem_nlevels: Optional[np.ndarray] = None, elem_ulevels: Optional[np.ndarray] = None,
                   seed: Optional[int] = None, permute: bool = False) -> Mesh:
     """Build a doubly periodic ``nx`` x ``ny`` cell mesh, two triangles per cell.
 
     ``layer_h`` gives the nl-1 nominal layer thicknesses (surface first).  ``elem_nlevels`` gives
     per-element level counts (in canonical element order) for bathymetry; default: all full depth.
+    ``elem_ulevels`` gives each element's first wet layer (an ice-shelf cavity where > 1; at least
+    two wet layers must remain); default: 1 everywhere.
     With ``permute`` the node, element and edge numbering, the edge directions and the starting
     vertex of every triangle are shuffled from ``seed`` (the geometry is unchanged).
     """
@@ -177,6 +179,9 @@ def periodic_mesh(nx: int, ny: int, Lx: float, Ly: float, layer_h: np.ndarray, *
             raise ValueError("elem_nlevels must have one entry per element")
         if elem_nlev.min() < 3 or elem_nlev.max() > nl:
             raise ValueError("element level counts must lie in [3, nl]")
+    elem_ulev = np.ones(T, dtype=np.int64) if elem_ulevels is None else np.asarray(elem_ulevels, dtype=np.int64).copy()
+    if elem_ulev.shape != (T,) or elem_ulev.min() < 1 or np.any(elem_nlev - elem_ulev < 2):
+        raise ValueError("elem_ulevels: one entry >= 1 per element, with nlevels - ulevels >= 2")
 
     # Lattice coordinates: metres are integers when dx, dy are, which keeps the up/down-wind
     # predicates exact.
@@ -201,6 +206,7 @@ def periodic_mesh(nx: int, ny: int, Lx: float, Ly: float, layer_h: np.ndarray, *
         idx = (np.arange(3)[None, :] + rot[:, None]) % 3
         enodes = np.take_along_axis(enodes, idx, axis=1)
         elem_nlev = elem_nlev[inv_t]
+        elem_ulev = elem_ulev[inv_t]
         ednodes = pn[ednodes]
         edtri = pt[edtri]
         ednodes[flip] = ednodes[flip][:, ::-1]
@@ -208,10 +214,10 @@ def periodic_mesh(nx: int, ny: int, Lx: float, Ly: float, layer_h: np.ndarray, *
         ednodes = ednodes[inv_e]
         edtri = edtri[inv_e]
 
-    return _finish_mesh(nl, xy, enodes, ednodes, edtri, elem_nlev, layer_h, Lx, Ly, nx, ny)
+    return _finish_mesh(nl, xy, enodes, ednodes, edtri, elem_nlev, layer_h, Lx, Ly, nx, ny, elem_ulev)
 
 
-def _finish_mesh(nl, xy, enodes, ednodes, edtri, elem_nlev, layer_h, Lx, Ly, nx, ny) -> Mesh:
+def _finish_mesh(nl, xy, enodes, ednodes, edtri, elem_nlev, layer_h, Lx, Ly, nx, ny, elem_ulev=None) -> Mesh:
     """Derive adjacency, levels, geometry and MUSCL support arrays (0-based inputs)."""
     N = xy.shape[0]
     T = enodes.shape[0]
@@ -224,15 +230,16 @@ def _finish_mesh(nl, xy, enodes, ednodes, edtri, elem_nlev, layer_h, Lx, Ly, nx,
     max_adj = nie.shape[1]
     nie_valid = np.arange(max_adj)[None, :] < nie_num[:, None]
 
-    # Levels (no cavities: ulevels == 1).
+    # Levels: a node is wet from the shallowest top to the deepest bottom of its elements
+    # (ulevels == 1 everywhere unless a cavity is given).
     nlevels = elem_nlev.astype(np.int32)
-    ulevels = np.ones(T, dtype=np.int32)
+    ulevels = (np.ones(T) if elem_ulev is None else elem_ulev).astype(np.int32)
     lev_tab = np.where(nie_valid, nlevels[nie], 0)
     nlevels_nod2D = lev_tab.max(axis=1).astype(np.int32)       # node = max over its elements
     nlevels_nod2D_min = np.where(nie_valid, nlevels[nie], np.iinfo(np.int32).max).min(axis=1)
     nlevels_nod2D_min = nlevels_nod2D_min.astype(np.int32)     # lib/oce_mesh.F90:1522-1560
-    ulevels_nod2D = np.ones(N, dtype=np.int32)
-    ulevels_nod2D_max = np.ones(N, dtype=np.int32)
+    ulevels_nod2D = np.where(nie_valid, ulevels[nie], np.iinfo(np.int32).max).min(axis=1).astype(np.int32)
+    ulevels_nod2D_max = np.where(nie_valid, ulevels[nie], 0).max(axis=1).astype(np.int32)
 
     # ---------------------------------------------------------------- geometry (metres)
     p1 = xy[enodes[:, 0]]
@@ -287,9 +294,11 @@ def _finish_mesh(nl, xy, enodes, ednodes, edtri, elem_nlev, layer_h, Lx, Ly, nx,
     # Vertical grid (no ALE motion: hnode_new == hnode).
     zbar = np.concatenate([[0.0], -np.cumsum(layer_h)])
     Z = 0.5 * (zbar[:-1] + zbar[1:])
-    hnode = np.broadcast_to(layer_h, (N, nl - 1)).copy()
+    # layers above a cavity's top have no thickness
+    lay = np.arange(1, nl)[None, :]
+    hnode = np.where(lay >= ulevels_nod2D[:, None], layer_h[None, :], 0.0)
     hnode_new = hnode.copy()
-    helem = np.broadcast_to(layer_h, (T, nl - 1)).copy()
+    helem = np.where(lay >= ulevels[:, None], layer_h[None, :], 0.0)
     zbar_3d_n = np.broadcast_to(zbar, (N, nl)).copy()
     Z_3d_n = np.broadcast_to(Z, (N, nl - 1)).copy()
 
@@ -371,6 +380,24 @@ def _nboundary_lay(nl, N, ednodes, edtri, nlevels) -> np.ndarray:
     for col in range(2):
         np.minimum.at(nbl, ednodes[:, col], lev)
     return nbl.astype(np.int32)
+
+
+def cavity_ulevels(mesh_nx: int, mesh_ny: int, elem_nlev: np.ndarray, depth: int, radius: float = 0.3) -> np.ndarray:
+    """First wet layer of every element (canonical order) under an ice shelf over a disc of the
+    given radius (a fraction of the domain) at the domain's centre: 1 + ceil(depth (1 - r/R))
+    inside, 1 outside, and at most nlevels - 2 so that two wet layers remain."""
+    ii, jj = np.meshgrid(np.arange(mesh_nx), np.arange(mesh_ny), indexing="xy")
+    ii = ii.ravel().astype(np.float64)
+    jj = jj.ravel().astype(np.float64)
+    cx = np.empty(2 * ii.size)
+    cy = np.empty(2 * ii.size)
+    cx[0::2] = (ii + 2.0 / 3.0) / mesh_nx
+    cy[0::2] = (jj + 1.0 / 3.0) / mesh_ny
+    cx[1::2] = (ii + 1.0 / 3.0) / mesh_nx
+    cy[1::2] = (jj + 2.0 / 3.0) / mesh_ny
+    r = np.hypot(cx - 0.5, cy - 0.5) / radius
+    ul = 1 + np.where(r < 1.0, np.ceil(depth * (1.0 - r)), 0.0).astype(np.int64)
+    return np.minimum(ul, np.asarray(elem_nlev, dtype=np.int64) - 2)
 
 
 def smooth_bathymetry_levels(mesh_nx: int, mesh_ny: int, nl: int, lo: int, hi: int,

@@ -7,6 +7,7 @@ generator is detected instead of silently comparing different cases.
     python tests/golden/make_golden.py        (needs oracle/_ref built by oracle/ref/build_ref.sh)
     python tests/golden/make_golden.py --only pi   (only the FESOM2 pi-mesh fixture)
     python tests/golden/make_golden.py --only wsplit_ab   (only the split-w / valuesAB / ALE fixture)
+    python tests/golden/make_golden.py --only cavity   (only the ice-shelf cavity fixture)
 """
 
 from __future__ import annotations
@@ -122,6 +123,31 @@ def ale_case(seed=59, ver="QR4C"):
     return dataclasses.replace(c, mesh=dataclasses.replace(m, hnode_new=hn, areasvol=av))
 
 
+def cavity_cases():
+    """name -> case: ice-shelf cavities (ulevels > 1 under a disc, up to three layers deep) with
+    each horizontal / vertical scheme family, and with the implicit vertical split."""
+    out = {}
+    for hor, ver, lim in (("MFCT", "QR4C", "FCT"), ("MUSCL", "PPM", "FCT"), ("UPW1", "CDIFF", "NONE"),
+                          ("MFCT", "UPW1", "FCT")):
+        out[f"cavity_{hor}_{ver}_{lim}"] = configs.bathymetry_case(
+            "cav", 12, 12, seed=5, nsteps=3, lev_lo=6, w_cfl=0.3, cavity=3, scheme=Scheme(hor, ver, lim, 0.8, 0.7))
+    c = configs.bathymetry_case("cavws", 12, 12, seed=7, nsteps=3, lev_lo=6, w_cfl=1.5, cavity=3,
+                                scheme=Scheme("MFCT", "PPM", "FCT", 0.8, 0.7))
+    out["cavity_wsplit_PPM"] = configs.split_vertical_velocity(c, 0.4)
+    return out
+
+
+def write_cavity(td):
+    out, hashes = {}, []
+    for name, case in cavity_cases().items():
+        p = bundle.write_bundle(Path(td) / name, case, save_steps=[3])
+        bundle.run_ref(p)
+        out[name] = bundle.read_values(p, case, 3)
+        hashes.append(case_hash(case))
+    np.savez_compressed(HERE / "cavity.npz", input_sha256=hashlib.sha256("".join(hashes).encode()).hexdigest(), **out)
+    print("wrote cavity")
+
+
 def write_wsplit_ab(td):
     out, hashes = {}, []
     for name, (case, ab) in wsplit_ab_cases().items():
@@ -145,9 +171,14 @@ def main():
         with tempfile.TemporaryDirectory() as td:
             write_wsplit_ab(td)
         return
+    if sys.argv[1:3] == ["--only", "cavity"]:
+        with tempfile.TemporaryDirectory() as td:
+            write_cavity(td)
+        return
     with tempfile.TemporaryDirectory() as td:
         write_pi(td)
         write_wsplit_ab(td)
+        write_cavity(td)
         for name, case in analytic_cases().items():
             p = bundle.write_bundle(Path(td) / name, case, save_steps=[1, 2, 10], save_work=True)
             bundle.run_ref(p)
