@@ -149,6 +149,25 @@ static_assert(kEdgePipe == 1 || kEdgePipe == 2, "edge pipeline depth 1 or 2");
 // With two levels in flight, stage both in LDS behind one barrier (four LDS buffers).
 constexpr bool kEdgePair = FESOMTRA_EDGE_PAIR != 0;
 
+#ifndef FESOMTRA_EDGE_UNTILED_PF
+#define FESOMTRA_EDGE_UNTILED_PF 1
+#endif
+// Untiled form with one column per wavefront: the node values and the transport of the next
+// level are loaded while a level is computed (two register sets).
+template <bool TILED, int KB>
+constexpr bool kEdgeUntiledPf = FESOMTRA_EDGE_UNTILED_PF != 0 && !TILED && KB == 64;
+// Node values of one level for the untiled prefetch: valuesAB at n1, n2, the up-wind triangle's
+// and the down-wind triangle's corners; values at n1, n2; the transport.
+template <typename C>
+struct EdgePf {
+    C a[8], t[2], q;
+};
+template <typename C>
+struct ArrLd {
+    const C *v;
+    __device__ __forceinline__ C operator()(size_t i) const { return v[i]; }
+};
+
 // SAME (tiled form only): valuesAB is values, so only one field is staged.
 template <bool TILED, int HOR, bool FCT, int KB, typename S, typename M, typename C, bool SAME = false>
 __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, int lc) {
@@ -247,23 +266,44 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
                 if (!same) sA[nz & 1][threadIdx.x + 256 * i] = stA[r][i];
         });
     };
-    // the edge flux of level nz (staged in LDS when TILED; qcur is its transport)
-    auto compute = [&](int nz, C qcur) {
+    constexpr bool UPF = kEdgeUntiledPf<TILED, KB>;
+    auto pf_load = [&](int nz, EdgePf<C> &f) {
+        const size_t o = (size_t)(nz - 1) * Np;
+        f.q = (C)cld(m.q, (size_t)(nz - 1) * Ep + e);
+        f.a[0] = ldA(o + n1);
+        f.a[1] = ldA(o + n2);
+        f.t[0] = f.a[0];
+        f.t[1] = f.a[1];
+        if (FCT && !same) {
+            f.t[0] = ldT(o + n1);
+            f.t[1] = ldT(o + n2);
+        }
+        if (HOR != HOR_UPW1 && nz <= shared_end) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                f.a[2 + c] = ldA(o + u[c]);
+                f.a[5 + c] = ldA(o + d[c]);
+            }
+        }
+    };
+    // the edge flux of level nz (staged in LDS when TILED; qcur is its transport; pf: the level's
+    // prefetched values in the untiled prefetch form)
+    auto compute = [&](int nz, C qcur, const EdgePf<C> *pf) {
         if (TILED && (!act || nz > nz1)) return;
         const int buf = nz % NB;
         const int k = nz - 1;
-        const C v = TILED ? qcur : (C)cld(m.q, (size_t)k * Ep + e);
+        const C v = TILED ? qcur : (UPF ? pf->q : (C)cld(m.q, (size_t)k * Ep + e));
         const size_t o = (size_t)k * Np;
         const size_t ie = ((size_t)k * Ep + e) * KB + lane;
         const SlotLd<KB, C> lsT{sT[TILED ? buf : 0], lane},
             lsA{same ? sT[TILED ? buf : 0] : sA[TILED && !SAME ? (nz & 1) : 0], lane};
         C t1 = 0, t2 = 0;
         if (FCT || same) {
-            t1 = TILED ? lsT(slot(0)) : ldT(o + n1);
-            t2 = TILED ? lsT(slot(1)) : ldT(o + n2);
+            t1 = TILED ? lsT(slot(0)) : (UPF ? pf->t[0] : ldT(o + n1));
+            t2 = TILED ? lsT(slot(1)) : (UPF ? pf->t[1] : ldT(o + n2));
         }
-        const C a1 = same ? t1 : (TILED ? lsA(slot(0)) : ldA(o + n1));
-        const C a2 = same ? t2 : (TILED ? lsA(slot(1)) : ldA(o + n2));
+        const C a1 = same ? t1 : (TILED ? lsA(slot(0)) : (UPF ? pf->a[0] : ldA(o + n1)));
+        const C a2 = same ? t2 : (TILED ? lsA(slot(1)) : (UPF ? pf->a[1] : ldA(o + n2)));
         C gr[4] = {0, 0, 0, 0}, c1 = 1, c2 = 1;
         if (HOR != HOR_UPW1) {
             if (nz <= shared_end) {
@@ -272,6 +312,12 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
                     gr[1] = tr_xy<C>(gd, lsA, 0, slot(5), slot(6), slot(7));
                     gr[2] = tr_xy<C>(gu + 3, lsA, 0, slot(2), slot(3), slot(4));
                     gr[3] = tr_xy<C>(gd + 3, lsA, 0, slot(5), slot(6), slot(7));
+                } else if constexpr (UPF) {
+                    const ArrLd<C> la{pf->a};
+                    gr[0] = tr_xy<C>(gu, la, 0, 2, 3, 4);
+                    gr[1] = tr_xy<C>(gd, la, 0, 5, 6, 7);
+                    gr[2] = tr_xy<C>(gu + 3, la, 0, 2, 3, 4);
+                    gr[3] = tr_xy<C>(gd + 3, la, 0, 5, 6, 7);
                 } else {
                     gr[0] = tr_xy<C>(gu, ldA, o, u[0], u[1], u[2]);
                     gr[1] = tr_xy<C>(gd, ldA, o, d[0], d[1], d[2]);
@@ -311,7 +357,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
             __syncthreads();
             if (nz + P <= lv1) fetch(nz + P, set);
         }
-        compute(nz, qcur);
+        compute(nz, qcur, nullptr);
     };
     if constexpr (PAIR) {
         // levels nz and nz + 1 staged together behind one barrier; the buffers of the pair
@@ -324,8 +370,21 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
             __syncthreads();
             if (nz + 2 <= lv1) fetch(nz + 2, std::integral_constant<int, 0>{});
             if (nz + 3 <= lv1) fetch(nz + 3, std::integral_constant<int, 1>{});
-            compute(nz, q0);
-            if (two) compute(nz + 1, q1);
+            compute(nz, q0, nullptr);
+            if (two) compute(nz + 1, q1, nullptr);
+        }
+    } else if constexpr (UPF) {
+        // two register sets, alternating: level nz + 1 loads while level nz computes
+        EdgePf<C> f0, f1;
+        pf_load(lv0, f0);
+        for (int nz = lv0; nz <= lv1; nz += 2) {
+            const bool two = nz + 1 <= lv1;
+            if (two) pf_load(nz + 1, f1);
+            compute(nz, C(0), &f0);
+            if (two) {
+                if (nz + 2 <= lv1) pf_load(nz + 2, f0);
+                compute(nz + 1, C(0), &f1);
+            }
         }
     } else {
         // unrolled by the pipeline depth, so every register set is addressed statically
