@@ -171,8 +171,8 @@ struct ArrLd {
 // SAME (tiled form only): valuesAB is values, so only one field is staged.
 template <bool TILED, int HOR, bool FCT, int KB, typename S, typename M, typename C, bool SAME = false>
 __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, int lc) {
-    // two levels in flight only where the registers allow it (one staged field, no MUSCL switch)
-    constexpr int P = (SAME && HOR != HOR_MUSCL) ? kEdgePipe : 1;
+    // two levels in flight only where the registers allow it (one staged field)
+    constexpr int P = SAME ? kEdgePipe : 1;
     constexpr bool PAIR = TILED && P == 2 && kEdgePair;
     constexpr int NB = PAIR ? 4 : 2;  // LDS buffers of the staged field, level nz in nz % NB
     __shared__ C sT[TILED ? NB : 1][TILED ? kHtSlots : 1], sA[TILED && !SAME ? 2 : 1][TILED && !SAME ? kHtSlots : 1];
