@@ -834,6 +834,11 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_vert(const MeshCol<M> m,
 // (which needs the clusters j - 2, j - 1, j) is finished; the surface and bottom layers use
 // their own cluster only (the bottom layer L is finished at j = L + 1, where no cluster is
 // formed).  TILED stages max(LO, T) and min(LO, T) of the tile and its halo.
+#ifndef FESOMTRA_BOUNDS_PF
+#define FESOMTRA_BOUNDS_PF 1
+#endif
+template <bool TILED, int KB>
+constexpr bool kBoundsPfFor = FESOMTRA_BOUNDS_PF != 0 && !TILED && KB == 64;
 template <bool TILED, int KB, typename S, typename M, typename C>
 __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
     using D = double;
@@ -907,6 +912,47 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
             vmin = bmin(vmin, D(1.0e3));
         }
     };
+    // 64-wide untiled form: the next level's cluster values (raw storage values of the node and
+    // of its cached neighbours) are loaded while a level is finished, in two register sets
+    constexpr bool CPF = kBoundsPfFor<TILED, KB>;
+    struct TL {
+        S t;
+        C l;
+    };
+    struct CPre {
+        TL own;
+        TL v[decltype(ne)::kN];
+    };
+    auto tl_load = [&](size_t o, int node) {
+        const size_t q = (o + node) * KB + lane;
+        return TL{p.T[q], p.LO[q]};
+    };
+    auto cl_load = [&](int nz, CPre &c) {
+        const size_t o = (size_t)(nz - 1) * Np;
+        c.own = tl_load(o, n);
+        edges_pre(ne, nz, [&](int, bool, int oth) { return tl_load(o, oth); }, c.v);
+    };
+    auto cluster_pf = [&](int nz, const CPre &c, D &vmax, D &vmin, D &own_lo) {
+        const size_t o = (size_t)(nz - 1) * Np;
+        auto lo_of = [](D t, C l) -> D {
+            if constexpr (std::is_same<C, double>::value) return (D)l;
+            else return t + (D)l;
+        };
+        const D t0 = ldc<D>(&c.own.t, 0), l0 = lo_of(t0, c.own.l);
+        own_lo = l0;
+        vmax = bmax(l0, t0);
+        vmin = bmin(l0, t0);
+        edges_use(m, ne, nz, c.v, [&](int, bool, int oth) { return tl_load(o, oth); },
+                  [&](const TL &v, int, bool, int) {
+                      const D tt = ldc<D>(&v.t, 0), ll = lo_of(tt, v.l);
+                      vmax = bmax(vmax, bmax(ll, tt));
+                      vmin = bmin(vmin, bmin(ll, tt));
+                  });
+        if (nz >= dry_from) {
+            vmax = bmax(vmax, D(-1.0e3));
+            vmin = bmin(vmin, D(1.0e3));
+        }
+    };
     // P+/P-, Q+/Q- and the ratios of layer nz with cluster bounds (hi, lw) and low-order
     // value lo; the vertical flux at the layer's bottom is kept for the next layer (av_bot)
     C av_bot = 0;
@@ -961,6 +1007,42 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
     if constexpr (TILED) fetch(jb);
     D mx2 = 0, mn2 = 0, mx1 = 0, mn1 = 0, mx0 = 0, mn0 = 0;  // clusters of levels j-2, j-1, j
     D lo1 = 0, lo0 = 0;                                      // this column's LO at j-1, j
+    // level j of the sweep (tiled: staged in buffer buf, lo_j this column's LO there; CPF: the
+    // prefetched cluster values cp)
+    auto step = [&](int j, int buf, D lo_j, const CPre *cp) {
+        mx2 = mx1;
+        mn2 = mn1;
+        mx1 = mx0;
+        mn1 = mn0;
+        lo1 = lo0;
+        const int nz = j - 1;
+        const bool fin = nz >= nz0 && nz <= nz1;
+        Pre pre;
+        if (fin) preload(nz, pre);
+        if (j <= L) {
+            if constexpr (CPF) cluster_pf(j, *cp, mx0, mn0, lo0);
+            else cluster(j, buf, mx0, mn0, lo0);
+            if constexpr (TILED) lo0 = lo_j;
+        }
+        if (fin) {
+            const bool own = nz == 1 || nz == L;
+            finish(nz, own ? mx1 : bmax(bmax(mx2, mx1), mx0), own ? mn1 : bmin(bmin(mn2, mn1), mn0), lo1, pre);
+        }
+    };
+    if constexpr (CPF) {
+        // two register sets, alternating: the cluster values of level j + 1 load during level j
+        CPre c0, c1;
+        if (jb <= L) cl_load(jb, c0);
+        for (int j = jb; j <= je; j += 2) {
+            if (j + 1 <= L) cl_load(j + 1, c1);
+            step(j, 0, D(0), &c0);
+            if (j + 1 <= je) {
+                if (j + 2 <= L) cl_load(j + 2, c0);
+                step(j + 1, 0, D(0), &c1);
+            }
+        }
+        return;
+    }
     for (int j = jb; j <= je; ++j) {
         const int buf = j & 1;
         D lo_j = 0;
@@ -976,23 +1058,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
             }
             if (!act) continue;
         }
-        mx2 = mx1;
-        mn2 = mn1;
-        mx1 = mx0;
-        mn1 = mn0;
-        lo1 = lo0;
-        const int nz = j - 1;
-        const bool fin = nz >= nz0 && nz <= nz1;
-        Pre pre;
-        if (fin) preload(nz, pre);
-        if (j <= L) {
-            cluster(j, buf, mx0, mn0, lo0);
-            if constexpr (TILED) lo0 = lo_j;
-        }
-        if (fin) {
-            const bool own = nz == 1 || nz == L;
-            finish(nz, own ? mx1 : bmax(bmax(mx2, mx1), mx0), own ? mn1 : bmin(bmin(mn2, mn1), mn0), lo1, pre);
-        }
+        step(j, buf, lo_j, nullptr);
     }
 }
 
