@@ -1007,9 +1007,9 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
     if constexpr (TILED) fetch(jb);
     D mx2 = 0, mn2 = 0, mx1 = 0, mn1 = 0, mx0 = 0, mn0 = 0;  // clusters of levels j-2, j-1, j
     D lo1 = 0, lo0 = 0;                                      // this column's LO at j-1, j
-    // level j of the sweep (tiled: staged in buffer buf, lo_j this column's LO there; CPF: the
-    // prefetched cluster values cp)
-    auto step = [&](int j, int buf, D lo_j, const CPre *cp) {
+    // level j of the sweep with the prefetched cluster values cp (CPF; the other forms run the
+    // same sequence inline below)
+    auto step = [&](int j, const CPre *cp) {
         mx2 = mx1;
         mn2 = mn1;
         mx1 = mx0;
@@ -1019,11 +1019,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         const bool fin = nz >= nz0 && nz <= nz1;
         Pre pre;
         if (fin) preload(nz, pre);
-        if (j <= L) {
-            if constexpr (CPF) cluster_pf(j, *cp, mx0, mn0, lo0);
-            else cluster(j, buf, mx0, mn0, lo0);
-            if constexpr (TILED) lo0 = lo_j;
-        }
+        if (j <= L) cluster_pf(j, *cp, mx0, mn0, lo0);
         if (fin) {
             const bool own = nz == 1 || nz == L;
             finish(nz, own ? mx1 : bmax(bmax(mx2, mx1), mx0), own ? mn1 : bmin(bmin(mn2, mn1), mn0), lo1, pre);
@@ -1035,10 +1031,10 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         if (jb <= L) cl_load(jb, c0);
         for (int j = jb; j <= je; j += 2) {
             if (j + 1 <= L) cl_load(j + 1, c1);
-            step(j, 0, D(0), &c0);
+            step(j, &c0);
             if (j + 1 <= je) {
                 if (j + 2 <= L) cl_load(j + 2, c0);
-                step(j + 1, 0, D(0), &c1);
+                step(j + 1, &c1);
             }
         }
         return;
@@ -1058,7 +1054,23 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
             }
             if (!act) continue;
         }
-        step(j, buf, lo_j, nullptr);
+        mx2 = mx1;
+        mn2 = mn1;
+        mx1 = mx0;
+        mn1 = mn0;
+        lo1 = lo0;
+        const int nz = j - 1;
+        const bool fin = nz >= nz0 && nz <= nz1;
+        Pre pre;
+        if (fin) preload(nz, pre);
+        if (j <= L) {
+            cluster(j, buf, mx0, mn0, lo0);
+            if constexpr (TILED) lo0 = lo_j;
+        }
+        if (fin) {
+            const bool own = nz == 1 || nz == L;
+            finish(nz, own ? mx1 : bmax(bmax(mx2, mx1), mx0), own ? mn1 : bmin(bmin(mn2, mn1), mn0), lo1, pre);
+        }
     }
 }
 
