@@ -190,14 +190,16 @@ def test_block_width_change_keeps_values(oracle_mod):
     assert rel_linf(got, ref[2]) <= RTOL
 
 
-def test_fp32_batch_of_64_equals_single_lanes():
-    """FP32 arithmetic: 64 tracers in one interleaved block give bit-identical results to the
-    same tracers advanced one per block (each lane runs the same operations)."""
+@pytest.mark.parametrize("prec", [1, 2, 3])
+def test_fp32_batch_of_64_equals_single_lanes(prec):
+    """FP32 arithmetic (FP32, FP16 or BF16 storage): 64 tracers in one interleaved block give
+    bit-identical results to the same tracers advanced one per block (each lane runs the same
+    operations; the stochastic rounding is keyed by tracer number, cell and step)."""
     case = configs.bathymetry_case("bathy8x64", 8, 10, seed=37, ntracers=64, nsteps=2, lev_lo=3,
                                    w_cfl=0.3)
-    say("case bathy8x64 FP32 kb=64 vs kb=1")
-    a = gpu_run(case, 2, kb=64, prec=1)
-    b = gpu_run(case, 2, kb=1, prec=1)
+    say(f"case bathy8x64 precision {prec} kb=64 vs kb=1")
+    a = gpu_run(case, 2, kb=64, prec=prec)
+    b = gpu_run(case, 2, kb=1, prec=prec)
     assert np.array_equal(a, b)
 
 
