@@ -1381,6 +1381,12 @@ static inline unsigned grid_for(size_t n_threads, int block) {
         }                                               \
     } while (0)
 
+template <typename S>
+__global__ __launch_bounds__(256) void k_copy(S *__restrict__ dst, const S *__restrict__ src, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 template <int HOR, bool FCT, int KB, typename S, typename M, typename C>
 static void edge_kernel(dim3 g, hipStream_t s, const MeshCol<M> &m, const StepPtrs<S, C> &p, C ph, int lc) {
     if constexpr (KB <= 4) {
@@ -1477,8 +1483,21 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
         LAUNCH_CHECK();
         MARK(2);
         MARK(3);
-        if (p.ab_next) k_final<false, false, KB, S, M, C, true><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
-        else k_final<false, false, KB><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+        // single precision without limiter: the update adds the upwind part of valuesAB at the
+        // far nodes, so when valuesAB is values and the update is in place it must read a snapshot
+        // taken before the first store (the LO buffer, unused without the limiter, holds it)
+        StepPtrs<S, C> pf = p;
+        if constexpr (!std::is_same<C, double>::value) {
+            if (p.TAB == p.T && !p.acc_h) {
+                const size_t n = (size_t)L * m.Np * KB;
+                static_assert(sizeof(S) <= sizeof(C), "the snapshot fits the LO buffer");
+                k_copy<S><<<dim3(grid_for(n, 256)), 256, 0, s>>>(reinterpret_cast<S *>(p.LO), p.T, n);
+                LAUNCH_CHECK();
+                pf.TAB = reinterpret_cast<const S *>(p.LO);
+            }
+        }
+        if (pf.ab_next) k_final<false, false, KB, S, M, C, true><<<gf, 256, 0, s>>>(m, pf, sc.dt, cfg.lc_final);
+        else k_final<false, false, KB><<<gf, 256, 0, s>>>(m, pf, sc.dt, cfg.lc_final);
     }
     LAUNCH_CHECK();
     MARK(4);

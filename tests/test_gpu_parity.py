@@ -203,6 +203,20 @@ def test_fp32_batch_of_64_equals_single_lanes(prec):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("hor,ver,lim", [("UPW1", "QR4C", "FCT"), ("MUSCL", "PPM", "FCT"), ("MFCT", "CDIFF", "NONE")])
+def test_batch_of_64_equals_single_lanes_schemes(prec, hor, ver, lim):
+    """The 64-wide kernels (one column per wavefront, next level's values loaded during the
+    current one) against one tracer per block, bit for bit, for the other horizontal schemes and
+    without the limiter, in FP64 and in FP32."""
+    case = configs.bathymetry_case("bathy9x64", 9, 11, seed=39, ntracers=64, nsteps=2, lev_lo=3,
+                                   scheme=Scheme(hor, ver, lim, 0.8, 0.7), w_cfl=0.3)
+    say(f"case bathy9x64 {hor}/{ver}/{lim} precision {prec} kb=64 vs kb=1")
+    a = gpu_run(case, 2, kb=64, prec=prec)
+    b = gpu_run(case, 2, kb=1, prec=prec)
+    assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_uniform_field_preserved_every_precision(mode):
     """A uniform tracer on a full-depth mesh under a divergence-free flow stays uniform: to FP64
