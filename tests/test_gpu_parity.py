@@ -249,6 +249,20 @@ def test_fp32_200_steps_within_1e5_of_fp64(oracle_mod):
 PREC_BOUNDS = {1: 1e-6, 2: 5e-3, 3: 3e-2}  # rel L-inf after 5 steps vs FP64: FP32, FP16, BF16 storage
 
 
+@pytest.mark.parametrize("hor,ver", [("MFCT", "QR4C"), ("UPW1", "PPM"), ("MUSCL", "CDIFF")])
+def test_fp32_without_limiter_close_to_fp64(oracle_mod, hor, ver):
+    """FP32 without the limiter (valuesAB is values, update in place: the upwind part added in
+    double reads the values as they were before the update) against the FP64 restatement."""
+    case = configs.bathymetry_case("bathy20n", 20, 16, seed=29, nsteps=5, lev_lo=5, w_cfl=0.3,
+                                   scheme=Scheme(hor, ver, "NONE", 0.8, 0.7))
+    say(f"case bathy20n {hor}/{ver}/NONE FP32 5 steps")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[5])
+    got = gpu_run(case, 5, prec=1)
+    err = rel_linf(got, ref[5])
+    say(f"  rel Linf {err:.3e}")
+    assert err <= PREC_BOUNDS[1]
+
+
 @pytest.mark.parametrize("mode", [1, 2, 3])
 def test_mixed_precision_close_to_fp64(oracle_mod, mode):
     """FP32 arithmetic with FP32 / FP16 / BF16 storage against the FP64 reference restatement;
