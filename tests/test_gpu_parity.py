@@ -442,11 +442,12 @@ def test_halo_tiles_match_untiled_and_oracle(oracle_mod, kb, si):
     assert rel_linf(tiled, ref[3]) <= RTOL
 
 
+@pytest.mark.parametrize("lim", ["FCT", "NONE"])
 @pytest.mark.parametrize("prec", [1, 3])
-def test_halo_tiles_single_precision_bitwise(prec):
+def test_halo_tiles_single_precision_bitwise(prec, lim):
     case = configs.bathymetry_case("bathy40p", 40, 37, seed=73, ntracers=2, nsteps=4, lev_lo=3,
-                                   w_cfl=0.3)
-    say(f"case bathy40p precision {prec} tiled vs untiled")
+                                   w_cfl=0.3, scheme=Scheme("MFCT", "QR4C", lim, 0.85, 0.75))
+    say(f"case bathy40p precision {prec} limiter {lim} tiled vs untiled")
     tiled, nt = _run_tiles(case, 4, 2, True, prec=prec)
     plain, _ = _run_tiles(case, 4, 2, False, prec=prec)
     assert nt > 0
