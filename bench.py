@@ -12,16 +12,23 @@ accumulation, update, limiter ratios and sums.
     python bench.py --precision-study               # 1000-step FP32/FP16/BF16 errors vs FP64
 
 Prints one JSON line.  ``value`` = tracers x wet node-levels x steps / wall time of the timed
-steps (inputs resident in HBM).  ``roofline`` relates the algorithmic bytes of one kernel pass
-(every input read once, the tracers written once, at storage precision) to the mean HIP-event
-duration of the pass's kernels; ``traffic`` is the rocprof-measured HBM traffic of the same pass
-(profiles/traffic_<workload>.json).  ``cpu_baseline`` times the CPU restatement of the reference
-on one tracer-step of the same mesh, single-threaded.
+steps (inputs resident in HBM).  ``roofline`` relates the algorithmic bytes of one step (every
+input read once, the tracers written once, at storage precision) to the mean HIP-event duration
+of the whole step on the library's stream (the edge-transport kernel k_qflux and every pass's
+four kernels); ``traffic`` is the rocprof-measured HBM traffic of one step
+(profiles/traffic_<workload>.json, used only when it was measured on the current kernel sources:
+the file carries their hash).  ``cpu_baseline`` times the reference Fortran (built from its
+sources into oracle/_ref, OpenMP) on one tracer-step of the same mesh, or, when that build is
+absent, the C restatement on one thread, saying so.
+
+    python bench.py --rocprof-summary DIR --workload mesh1000   # rocprof CSVs -> profiles/
 """
 
 from __future__ import annotations
 
 import argparse
+import csv
+import hashlib
 import json
 import os
 import sys
@@ -107,22 +114,65 @@ def reference_baseline(case, exe: Path, threads: int):
                       f"{nl / t['seconds_advection']:.3e}/s"}
 
 
+def host_cpus():
+    """(threads to use, description): the CPUs this process may run on, capped by OMP_NUM_THREADS
+    where the environment sets it (the CPU share a one-GPU job is given on a shared node)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(avail, int(env)) if env.isdigit() and int(env) > 0 else avail
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    desc = (f"{threads} threads on a host with {os.cpu_count()} logical CPUs ({avail} available to "
+            f"this process{', OMP_NUM_THREADS=' + env if env else ''}){'; ' + model if model else ''}")
+    return threads, desc
+
+
 def cpu_baseline(case):
     exe = REPO / "oracle" / "_ref" / "ref_driver_omp"
+    threads, desc = host_cpus()
     if exe.exists():
-        threads = max(1, min(16, os.cpu_count() or 1))
-        try:
-            return reference_baseline(case, exe, threads)
-        except Exception as err:  # the port below is the documented fallback
-            log(f"reference CPU baseline failed ({err}); timing the CPU restatement instead")
-    return port_baseline(case)
+        out = reference_baseline(case, exe, threads)
+        out["host"] = desc
+        return out
+    why = (f"the reference build {exe.relative_to(REPO)} is absent (oracle/ref/build_ref.sh builds it "
+           "from the reference sources); timing the C restatement instead")
+    log("CPU BASELINE: " + why)
+    out = port_baseline(case)
+    out["host"] = desc
+    out["note"] = why
+    return out
+
+
+def kernel_source_hash() -> str:
+    """Hash of the kernel sources and build recipe: a stored traffic figure applies to one build."""
+    h = hashlib.sha256()
+    csrc = REPO / "fesom2_tracer_dwarf_amd" / "csrc"
+    for f in sorted(list(csrc.glob("*.hip")) + list(csrc.glob("*.hpp")) + list(csrc.glob("*.cpp")) + list(csrc.glob("*.inc"))
+                    + [REPO / "fesom2_tracer_dwarf_amd" / "_build.py"]):
+        h.update(f.name.encode())
+        h.update(f.read_bytes())
+    return h.hexdigest()[:16]
 
 
 def traffic_from_profiles(name: str):
+    """HBM bytes of one step from profiles/traffic_<name>.json, if measured on these sources."""
     f = REPO / "profiles" / f"traffic_{name}.json"
     if not f.exists():
-        return None
-    return json.loads(f.read_text()).get("hbm_bytes_per_launch")
+        return None, f"no {f.relative_to(REPO)}"
+    d = json.loads(f.read_text())
+    if d.get("source_hash") != kernel_source_hash():
+        return None, (f"{f.relative_to(REPO)} was measured on other kernel sources "
+                      f"({d.get('source_hash')} != {kernel_source_hash()})")
+    return d.get("hbm_bytes_per_step"), f"{f.relative_to(REPO)} (source hash {d['source_hash']})"
 
 
 def make_case(name: str, nsteps=None):
@@ -196,6 +246,7 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
     t.lib.tracer_set_kernel_timing(1)
     t.advect_steps(stage_steps, case.dt)
     stage_ms, passes = t.kernel_times()
+    qflux_ms, step_ms, nsteps_timed = t.step_times()
     t.lib.tracer_set_kernel_timing(0)
     cfg = t.launch_config()
     t.set_graph(False)
@@ -206,25 +257,129 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
     updates = case.ntracers * m.n_node_levels * steps
     return {"value": updates / wall, "wall_s": wall, "dev_ms": dev_ms, "stage_ms": stage_ms,
             "passes_timed": passes, "block_width": width, "drift": drift, "launch_config": cfg,
-            "values": kept, "error": err, "autotune": tuned}
+            "values": kept, "error": err, "autotune": tuned, "qflux_ms": qflux_ms,
+            "step_ms": step_ms, "steps_timed": nsteps_timed}
 
 
 def roofline(case, prec: int, r: dict, name: str):
+    """The step as the unit: algorithmic bytes of one step of every tracer over the mean HIP-event
+    time of the whole step (k_qflux and every pass of the step)."""
     s_tr, s_m = PREC_BYTES[prec]
-    k = min(r["block_width"], case.ntracers)
+    k = case.ntracers
     B = algorithmic_bytes(case.mesh, k, s_tr, s_m)
-    pass_ms = sum(r["stage_ms"])
-    achieved = B / (pass_ms * 1e-3) / 1e9
+    step_ms = r["step_ms"]
+    achieved = B / (step_ms * 1e-3) / 1e9
     st = r["stage_ms"]
+    passes = max(1, round(r["passes_timed"] / max(r["steps_timed"], 1)))
+    traffic, source = traffic_from_profiles(name)
+    fused = not (st[0] or st[1] or st[2])
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(name),
-            "kernel": (f"one pass (k_edge + k_lo + k_bounds + k_final) over a block of {k} "
-                       f"tracer(s), {PREC_NAMES[prec]}") if st[0] or st[1] or st[2] else
-                      f"one fused pass (k_tile) over {k} tracer, {PREC_NAMES[prec]}",
-            "algorithmic_bytes_per_launch": B, "tracers_per_launch": k,
-            "stage_ms_per_launch": ({"k_edge": round(st[0], 4), "k_lo": round(st[1], 4),
-                                     "k_bounds": round(st[2], 4), "k_final": round(st[3], 4)}
-                                    if st[0] or st[1] or st[2] else {"k_tile": round(st[3], 4)})}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": source,
+            "kernel": ("one step: k_qflux + " + (f"{passes} x " if passes > 1 else "") +
+                       ("fused pass (k_tile)" if fused else "pass (k_edge + k_lo + k_bounds + k_final)") +
+                       f" over {k} tracer(s) in blocks of {r['block_width']}, {PREC_NAMES[prec]}"),
+            "algorithmic_bytes_per_step": B, "tracers_per_step": k, "step_ms": round(step_ms, 4),
+            "stage_ms": ({"k_qflux": round(r["qflux_ms"], 4), "k_tile": round(st[3], 4)} if fused else
+                         {"k_qflux": round(r["qflux_ms"], 4), "k_edge": round(st[0], 4), "k_lo": round(st[1], 4),
+                          "k_bounds": round(st[2], 4), "k_final": round(st[3], 4)}),
+            "passes_per_step": passes}
+
+
+PROFILE_ARGS = "--steps 5 --warmup 1 --no-cpu-baseline --no-batched --no-autotune"
+
+
+def _rocprof_rows(d: Path, suffix: str):
+    files = sorted(d.rglob(f"*{suffix}"))
+    if not files:
+        raise SystemExit(f"no *{suffix} under {d}")
+    rows = []
+    for f in files:
+        with open(f, newline="") as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def _short(name: str) -> str:
+    """fesomtra::k_edge<true, 2, ...>(...) -> fesomtra::k_edge"""
+    return name.split("<")[0].split("(")[0].strip()
+
+
+def _counter_means(d: Path, counter: str):
+    """Mean value (KiB) per dispatch of `counter` for each kernel name."""
+    acc = {}
+    for r in _rocprof_rows(d, "counter_collection.csv"):
+        if r.get("Counter_Name") != counter:
+            continue
+        k = _short(r["Kernel_Name"])
+        acc.setdefault(k, []).append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def rocprof_summary(args):
+    """profiles/kernels_<workload>.txt and profiles/traffic_<workload>.json from rocprofv3 output
+    directories under DIR: k (kernel trace, --stats), f (--pmc FETCH_SIZE), w (--pmc WRITE_SIZE) of
+    `bench.py --workload W PROFILE_ARGS`, and cf / cw (the same counters on --copy-calibration,
+    which copies 1 GiB, to scale the counters to bytes)."""
+    import subprocess
+    d = Path(args.rocprof_summary)
+    w = args.workload
+    stats = {}
+    for r in _rocprof_rows(d / "k", "kernel_stats.csv"):
+        k = _short(r["Name"])
+        calls = int(r["Calls"])
+        tot = float(r["TotalDurationNs"])
+        c0 = stats.get(k, (0, 0.0, float("inf"), 0.0))
+        stats[k] = (c0[0] + calls, c0[1] + tot, min(c0[2], float(r["MinNs"])), max(c0[3], float(r["MaxNs"])))
+    fetch, write = _counter_means(d / "f", "FETCH_SIZE"), _counter_means(d / "w", "WRITE_SIZE")
+    copy_bytes = float(1 << 30)
+    cf = _counter_means(d / "cf", "FETCH_SIZE")
+    cw = _counter_means(d / "cw", "WRITE_SIZE")
+    ck = max(cf, key=cf.get)  # the copy kernel dominates the calibration run
+    fscale = copy_bytes / (cf[ck] * 1024.0)
+    wscale = copy_bytes / (cw[max(cw, key=cw.get)] * 1024.0)
+    step = ["fesomtra::k_qflux", "fesomtra::k_edge", "fesomtra::k_lo", "fesomtra::k_bounds", "fesomtra::k_final"]
+    lines = [f"# {w}: rocprofv3 kernel trace and HBM counters of bench.py ({PROFILE_ARGS})", "",
+             "Commands (each a run of its own; the copy calibration moves 1 GiB):",
+             f"    rocprofv3 --kernel-trace --stats -f csv -d <dir>/k -o k -- python -u bench.py --workload {w} {PROFILE_ARGS}",
+             f"    rocprofv3 --pmc FETCH_SIZE -f csv -d <dir>/f -o f -- python -u bench.py --workload {w} {PROFILE_ARGS}",
+             f"    rocprofv3 --pmc WRITE_SIZE -f csv -d <dir>/w -o w -- python -u bench.py --workload {w} {PROFILE_ARGS}",
+             "    rocprofv3 --pmc FETCH_SIZE -f csv -d <dir>/cf -o cf -- python -u bench.py --copy-calibration",
+             "    rocprofv3 --pmc WRITE_SIZE -f csv -d <dir>/cw -o cw -- python -u bench.py --copy-calibration",
+             f"    python bench.py --rocprof-summary <dir> --workload {w}",
+             f"Counter scale from the copy: FETCH_SIZE x{fscale:.3f}, WRITE_SIZE x{wscale:.3f} (bytes per KiB counted / 1024)",
+             f"Kernel source hash {kernel_source_hash()}", "",
+             f"{'kernel':<28}{'calls':>7}{'avg_us':>10}{'min_us':>10}{'max_us':>10}{'HBM_read_MB':>13}{'HBM_write_MB':>14}"]
+    rows = sorted(stats.items(), key=lambda kv: -kv[1][1])
+    for k, (calls, tot, mn, mx) in rows:
+        rd = fetch.get(k, 0.0) * 1024 * fscale / 1e6
+        wr = write.get(k, 0.0) * 1024 * wscale / 1e6
+        lines.append(f"{k:<28}{calls:>7}{tot / calls / 1e3:>10.1f}{mn / 1e3:>10.1f}{mx / 1e3:>10.1f}{rd:>13.1f}{wr:>14.1f}")
+    present = [k for k in step if k in stats]
+    # per step: k_qflux once, the pass kernels as often per step as they ran per k_qflux call
+    nq = stats["fesomtra::k_qflux"][0]
+    us, rd, wr = 0.0, 0.0, 0.0
+    for k in present:
+        calls, tot = stats[k][0], stats[k][1]
+        per = calls / nq
+        us += tot / nq / 1e3
+        rd += fetch.get(k, 0.0) * 1024 * fscale * per
+        wr += write.get(k, 0.0) * 1024 * wscale * per
+    lines += ["", f"one step ({' + '.join(k.split('::')[1] for k in present)}): {us:.1f} us, "
+              f"HBM read {rd / 1e6:.1f} MB + write {wr / 1e6:.1f} MB = {(rd + wr) / 1e6:.1f} MB"]
+    try:
+        commit = subprocess.run(["git", "-C", str(REPO), "rev-parse", "--short", "HEAD"], capture_output=True,
+                                text=True).stdout.strip()
+    except OSError:
+        commit = ""
+    out = REPO / "profiles"
+    (out / f"kernels_{w}.txt").write_text("\n".join(lines) + "\n")
+    (out / f"traffic_{w}.json").write_text(json.dumps({
+        "workload": w, "hbm_bytes_per_step": int(rd + wr), "hbm_read_bytes_per_step": int(rd),
+        "hbm_write_bytes_per_step": int(wr), "step_kernels": present, "step_us": us,
+        "fetch_scale": fscale, "write_scale": wscale, "source_hash": kernel_source_hash(),
+        "commit_before_measurement": commit, "source": f"profiles/kernels_{w}.txt"}, indent=1) + "\n")
+    print("\n".join(lines))
 
 
 def describe(case):
@@ -302,7 +457,12 @@ def main():
     ap.add_argument("--study-steps", type=int, default=1000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (rocprof counter scale)")
+    ap.add_argument("--rocprof-summary", default="",
+                    help="summarise rocprofv3 CSV directories DIR/{k,f,w,cf,cw} into profiles/ (no GPU)")
     args = ap.parse_args()
+    if args.rocprof_summary:
+        rocprof_summary(args)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU halo design is documented, not built)", file=sys.stderr)

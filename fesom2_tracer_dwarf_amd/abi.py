@@ -56,6 +56,7 @@ _SIGNATURES = {
     "tracer_set_zbar": (_c_int, [_pd, _c_int]),
     "tracer_set_muscl_support": (_c_int, [_pi, _pi, _c_int, _c_int]),
     "tracer_set_velocity_elem": (_c_int, [_pd, _pd, _pd, _pd, _c_int, _c_int, _c_int]),
+    "tracer_set_node_dt": (_c_int, [_pd, _c_int]),
     "tracer_set_scheme": (_c_int, [_c_int, _c_char_p, _c_char_p, _c_char_p, _c_double, _c_double]),
     "tracer_advect_steps": (_c_int, [_c_int, _c_double]),
     "tracer_do_oce_adv_tra": (_c_int, [_c_double, _c_int]),
@@ -70,6 +71,9 @@ _SIGNATURES = {
     "tracer_last_elapsed_ms": (_c_int, [_pd]),
     "tracer_set_kernel_timing": (_c_int, [_c_int]),
     "tracer_get_kernel_times": (_c_int, [_pd, ctypes.POINTER(ctypes.c_long)]),
+    "tracer_get_step_times": (_c_int, [_pd, _pd, ctypes.POINTER(ctypes.c_long)]),
+    "tracer_set_values_ab_device": (_c_int, [_c_int, _vp, _c_int, _c_int]),
+    "tracer_get_tendencies_device": (_c_int, [_vp, _vp, _c_int, _c_int]),
     "tracer_set_precision": (_c_int, [_c_int]),
     "tracer_set_tracers_per_pass": (_c_int, [_c_int]),
     "tracer_get_tracers_per_pass": (_c_int, [_pi]),

@@ -15,7 +15,8 @@ Workloads (sizes fixed here; seeds make every array reproducible bit for bit):
   tracers made of 4 random Gaussian blobs with amplitudes in [0, 1]), 200 steps.
 * ``multimesh32`` 32 independent periodic meshes, nx = ny log-uniform in [64, 512], nl uniform in
   [10, 80], psi amplitude U[0.05, 0.5] m/s, own bathymetry, 4 tracers each, 100 steps.
-  ``union_case`` advances them together as one disconnected mesh with the smallest member dt.
+  ``union_case`` advances them together as one disconnected mesh, each member with its own dt
+  (a time-step factor per node, stepped with dt = 1).
 """
 
 from __future__ import annotations
@@ -64,6 +65,7 @@ class Case:
     dt: float
     nsteps: int
     use_wsplit: bool = False  # dynamics%use_wsplit: w = we + wi, wi advected implicitly
+    dt_node: Optional[np.ndarray] = None  # (N) time-step factor per node (None: 1); node n steps dt * dt_node[n]
 
     @property
     def ntracers(self) -> int:
@@ -307,13 +309,21 @@ def multimesh32(nsteps: int = 100, count: int = 32, seed: int = 4,
     return cases
 
 
-def union_case(cases: Sequence[Case], name: str = "union") -> Case:
+def union_case(cases: Sequence[Case], name: str = "union", own_dt: bool = True) -> Case:
     """Independent cases with the same tracer count and schemes as one case on the disjoint union
-    of their meshes; the time step is the smallest member dt (stable for every member)."""
+    of their meshes.  own_dt: every member keeps its own time step (``dt_node`` holds the member's
+    dt at each of its nodes and the union steps with dt = 1, so node n advances with exactly its
+    member's dt); otherwise the union steps with the smallest member dt (stable for every member)."""
     cases = list(cases)
     ntr = cases[0].ntracers
     if any(c.ntracers != ntr for c in cases):
         raise ValueError("union_case: every member needs the same number of tracers")
+    for c in cases[1:]:
+        if [dataclasses.astuple(a) for a in c.schemes] != [dataclasses.astuple(a) for a in cases[0].schemes]:
+            raise ValueError(f"union_case: member {c.name} has other schemes than {cases[0].name}; "
+                             "the tracers of a union share their schemes")
+        if c.use_wsplit != cases[0].use_wsplit:
+            raise ValueError("union_case: every member needs the same use_wsplit")
     mesh = union_mesh([c.mesh for c in cases])
     L = mesh.nl - 1
 
@@ -323,8 +333,13 @@ def union_case(cases: Sequence[Case], name: str = "union") -> Case:
     we = np.concatenate([np.pad(c.we, ((0, 0), (0, mesh.nl - c.we.shape[1]))) for c in cases])
     tr = np.concatenate([np.pad(c.tracers, ((0, 0), (0, 0), (0, L - c.tracers.shape[2])))
                          for c in cases], axis=1)
-    return Case(name, mesh, uv, w, wi, we, tr, [dataclasses.replace(s) for s in cases[0].schemes],
-                min(c.dt for c in cases), cases[0].nsteps)
+    schemes = [dataclasses.replace(s) for s in cases[0].schemes]
+    if own_dt:
+        dt_node = np.concatenate([np.full(c.mesh.nod2D, float(c.dt)) for c in cases])
+        return Case(name, mesh, uv, w, wi, we, tr, schemes, 1.0, cases[0].nsteps,
+                    use_wsplit=cases[0].use_wsplit, dt_node=dt_node)
+    return Case(name, mesh, uv, w, wi, we, tr, schemes, min(c.dt for c in cases), cases[0].nsteps,
+                use_wsplit=cases[0].use_wsplit)
 
 
 def multimesh32_union(nsteps: int = 100, **kw) -> Case:

@@ -49,6 +49,7 @@ struct State {
     std::vector<double> elem_area, area, areasvol, edxdy, ecd, elem_cos, gsca;
     std::vector<double> hnode, hnode_new, helem, Z, zbar3, zbar;
     std::vector<double> uv, w, wi, we;
+    std::vector<double> node_dt;  // time-step factor per node (empty: 1 everywhere)
     bool have_coord = false, have_conn = false, have_nlev = false, have_elev = false;
     bool have_geom = false, have_vert = false, have_hnode = false, have_muscl = false;
     bool have_vel = false, have_zbar = false;
@@ -111,6 +112,8 @@ struct State {
     double4 *part = nullptr;
     bool last_fct = false;
     int last_lane = 0;
+    uint64_t q_vel = ~0ull;  // velocity version the edge transport m.q was computed from
+    uint64_t vel_gen = 0;    // bumped whenever the device velocity changes
     hipGraphExec_t gexec = nullptr;  // captured advect_steps sequence
     int g_nsteps = -1;
     double g_dt = 0.0;
@@ -120,6 +123,11 @@ struct State {
     bool kernel_timing = false;
     double stage_ms[4] = {0, 0, 0, 0};
     long stage_count = 0;
+    double qflux_ms = 0.0, step_ms = 0.0;  // summed k_qflux and whole-step times (kernel timing)
+    long step_count = 0;
+    uint32_t *d_seed = nullptr;  // device copy of step_seed, advanced by a kernel per step
+    bool dropin_valid = false;   // the work arrays hold the last tracer_do_oce_adv_tra call's
+    bool lo_dirty = false;       // the LO buffer held a values snapshot (single precision, no limiter)
 };
 
 State *g = nullptr;
@@ -197,6 +205,9 @@ void release_device(State &s) {
     s.ckpt_old.clear();
     s.dT2.clear();
     s.t2_sync.clear();
+    s.d_seed = nullptr;
+    s.dropin_valid = false;
+    s.q_vel = ~0ull;
     s.tiles_built = false;
     s.tm = TileMesh{};
     s.ht_built = 0;
@@ -546,6 +557,7 @@ int invalidate(State &s) {
 // arrays in place (w_e / w_i get arrays of their own when they first differ from w), so a caller
 // that changes the velocity every step keeps its tracers, tiles and captured graph.
 int refresh_velocity(State &s) {
+    s.vel_gen++;
     if (!s.on_device) return 0;
     MeshCol<double> &m = s.m64;
     const int L = s.nl - 1, nl = s.nl;
@@ -579,6 +591,15 @@ int refresh_velocity(State &s) {
     };
     if (part(s.we, m.we) || part(s.wi, m.wi)) return -1;
     return 0;
+}
+
+// The per-node time-step factors in device order (m.dtn = null without them).
+int upload_node_dt(State &s) {
+    s.m64.dtn = nullptr;
+    if (s.node_dt.empty()) return 0;
+    std::vector<double> h(s.Np, 1.0);
+    for (int n = 0; n < s.N; ++n) h[n] = s.node_dt[s.perm_n[n]];
+    return upload(s, &s.m64.dtn, h);
 }
 
 int build_device(State &s) {
@@ -718,6 +739,7 @@ int build_device(State &s) {
     }
     s.h_nlev = i_nlev;
     if (upload_mesh<double>(s, s.m64, i_nlev, i_nmin, i_nbl)) return -1;
+    if (upload_node_dt(s)) return -1;
     if (dalloc(s, &s.m64.q, (size_t)L * s.Ep)) return -1;
     {
         const int *p;
@@ -734,6 +756,12 @@ int build_device(State &s) {
         dalloc_bytes(s, &s.Rm, NLn * kb * cb) || dalloc_bytes(s, &s.adfv, NIn * kb * cb))
         return -1;
     if (dalloc(s, &s.part, (size_t)kStatsBlocks) || dalloc(s, &s.stage, (size_t)N * L)) return -1;
+    if (dalloc(s, &s.d_seed, 64)) return -1;
+    // work arrays start at zero: the kernels write only the levels a mesh entity has, so the
+    // levels below stay zero, as the reference's zeroed work arrays read there
+    HIPCHK(hipMemset(s.adfh, 0, ELn * kb * cb));
+    HIPCHK(hipMemset(s.LO, 0, NLn * kb * cb));
+    HIPCHK(hipMemset(s.adfv, 0, NIn * kb * cb));
     // tracer blocks (unused lanes of the last block stay zero and are never advanced)
     const int nb = n_blocks(s);
     s.dT.assign(nb, nullptr);
@@ -862,6 +890,9 @@ int ensure_dropin_buffers(State &s) {
             return -1;
         HIPCHK(hipMemsetAsync(s.acc_h, 0, NLn * sizeof(double), s.stream));
         HIPCHK(hipMemsetAsync(s.acc_v, 0, NLn * sizeof(double), s.stream));
+        // the limited fluxes are written at the levels an edge / node has; the rest stays zero
+        HIPCHK(hipMemsetAsync(s.dbg_h, 0, ELn * cb, s.stream));
+        HIPCHK(hipMemsetAsync(s.dbg_v, 0, NIn * cb, s.stream));
     }
     return 0;
 }
@@ -873,6 +904,7 @@ StepPtrs<S_, C_> make_ptrs(State &s, int blk, unsigned long long mask, bool drop
     StepPtrs<S_, C_> p{};
     p.tid0 = blk * s.kb;
     p.seed = (s.prec != PREC_F64 && s.stochastic) ? seed : 0u;
+    p.seed_dev = p.seed ? s.d_seed : nullptr;
     p.T = static_cast<S_ *>(s.dT[blk]);
     p.TAB = static_cast<const S_ *>(s.tab_on[blk] ? s.tabbuf[blk] : s.dT[blk]);
     if (s.ab_mode == 2 && !dropin && blk < (int)s.tabbuf2.size() && s.tabbuf2[blk]) {
@@ -1057,6 +1089,7 @@ int build_tiles(State &s) {
 // The fused path applies to FP64 blocks of one tracer with the FCT limiter (without the
 // implicit vertical split, with valuesAB := values).
 bool fused_applies(const State &s, const SchemeSel &sc) {
+    if (!s.node_dt.empty()) return false;  // the fused kernel takes one time step for all nodes
     return s.fused && !s.wsplit && s.ab_mode == 0 && s.prec == PREC_F64 && s.kb == 1 && sc.fct;
 }
 
@@ -1090,6 +1123,8 @@ uint32_t next_seed(State &s) {
     if (++s.step_seed == 0u) s.step_seed = 1u;
     return s.step_seed;
 }
+// The stochastic rounding's step counter is used (single precision with stochastic rounding).
+bool seeded(const State &s) { return s.prec != PREC_F64 && s.stochastic; }
 
 // valuesAB / valuesold of a block from its values (FESOM2's AB2 interpolation, before a step).
 hipError_t launch_ab2_block(State &s, int b) {
@@ -1123,9 +1158,15 @@ std::vector<Launch> launch_plan(const State &s) {
     return plan;
 }
 
+// Events of the kernel timing per step: k_qflux start and end, then 5 per pass (around its 4
+// stages); the last pass's end closes the step.
+size_t step_events(size_t passes) { return 2 + 5 * passes; }
+
 // nsteps of the launch plan on the library's stream.  Fused blocks alternate between their two
 // buffers and end in the primary one (one device copy after an odd number of steps), so the
-// stored values are always in dT whatever the path.  evs (optional) receives 5 events per pass.
+// stored values are always in dT whatever the path.  evs (optional) receives step_events(passes)
+// events per step.  With stochastic rounding every step first advances the device step counter
+// (d_seed), which the caller set to step_seed before the sequence (also before a graph replay).
 // AB2 (ab_mode 2): the update of every step also writes the next step's valuesAB and valuesold
 // (k_final); valuesAB alternates between two buffers and ends in the primary one.  The
 // interpolation kernel runs before the first step when valuesAB is stale (values or the mode
@@ -1135,17 +1176,23 @@ hipError_t run_steps(State &s, const std::vector<Launch> &plan, int nsteps, doub
                      bool capture = false) {
     std::vector<char> where(s.dT.size(), 0);
     const bool ab2 = s.ab_mode == 2;
+    const size_t nev = step_events(plan.size());
     for (int st = 0; st < nsteps; ++st) {
+        hipEvent_t *sev = evs ? &evs[(size_t)st * nev] : nullptr;
+        hipError_t e = sev ? hipEventRecord(sev[0], s.stream) : hipSuccess;
         // edge transport of the velocity, shared by every tracer pass of the step
-        hipError_t e = launch_qflux(s.m64, s.stream);
+        if (e == hipSuccess) e = launch_qflux(s.m64, s.stream);
+        if (e == hipSuccess && sev) e = hipEventRecord(sev[1], s.stream);
         if (e != hipSuccess) return e;
+        s.q_vel = s.vel_gen;
         if (ab2 && st == 0 && (s.ab_fresh || capture))
             for (size_t b = 0; b < s.dT.size() && e == hipSuccess; ++b) e = launch_ab2_block(s, (int)b);
+        if (e == hipSuccess && seeded(s)) e = launch_seed(s.d_seed, 0u, 1, s.stream);
         if (e != hipSuccess) return e;
-        const uint32_t seed = next_seed(s);
+        const uint32_t seed = 1u;  // stochastic rounding on; the step's value is read from d_seed
         for (size_t gi = 0; gi < plan.size(); ++gi) {
             const Launch &l = plan[gi];
-            hipEvent_t *ev = evs ? &evs[((size_t)st * plan.size() + gi) * 5] : nullptr;
+            hipEvent_t *ev = evs ? &sev[2 + 5 * gi] : nullptr;
             if (fused_applies(s, l.sc)) {
                 double *in = static_cast<double *>(where[l.blk] ? s.dT2[l.blk] : s.dT[l.blk]);
                 double *out = static_cast<double *>(where[l.blk] ? s.dT[l.blk] : s.dT2[l.blk]);
@@ -1206,6 +1253,15 @@ int advect(State &s, int nsteps, double dt) {
     const std::vector<Launch> plan = launch_plan(s);
     if (s.fused && s.prec == PREC_F64 && s.kb == 1 && ensure_fused(s)) return -1;
     if (!plan.empty()) s.last_fct = plan.back().sc.fct;
+    if (nsteps > 0) s.dropin_valid = false;  // the work arrays now hold this call's intermediates
+    for (const Launch &l : plan)
+        if (nsteps > 0 && !l.sc.fct && s.prec != PREC_F64) s.lo_dirty = true;
+    // the device step counter starts from the host's; the host mirrors the steps' advances
+    if (seeded(s) && nsteps > 0) {
+        const hipError_t e = launch_seed(s.d_seed, s.step_seed, 0, s.stream);
+        if (e != hipSuccess) return fail("kernel launch failed: %s", hipGetErrorString(e));
+        for (int i = 0; i < nsteps; ++i) next_seed(s);
+    }
     if (s.use_graph && !s.kernel_timing && nsteps > 0) {
         if (!s.gexec || s.g_nsteps != nsteps || s.g_dt != dt || s.g_gen != s.gen)
             if (capture_graph(s, plan, nsteps, dt)) return -1;
@@ -1220,8 +1276,9 @@ int advect(State &s, int nsteps, double dt) {
         return 0;
     }
     std::vector<hipEvent_t> evs;
+    const size_t nev = step_events(plan.size());
     if (s.kernel_timing) {
-        evs.resize((size_t)nsteps * plan.size() * 5);
+        evs.resize((size_t)nsteps * nev);
         for (auto &e : evs) HIPCHK(hipEventCreate(&e));
     }
     HIPCHK(hipEventRecord(s.ev0, s.stream));
@@ -1234,16 +1291,24 @@ int advect(State &s, int nsteps, double dt) {
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
     s.last_ms = ms;
-    if (s.kernel_timing) {
-        for (size_t b = 0; b + 4 < evs.size(); b += 5)
-            for (int k = 0; k < 4; ++k) {
-                float d = 0.f;
-                HIPCHK(hipEventElapsedTime(&d, evs[b + k], evs[b + k + 1]));
-                s.stage_ms[k] += d;
-            }
-        s.stage_count += (long)(evs.size() / 5);
-        for (auto &e : evs) (void)hipEventDestroy(e);
+    if (s.kernel_timing && !plan.empty()) {
+        for (int st = 0; st < nsteps; ++st) {
+            hipEvent_t *sev = &evs[(size_t)st * nev];
+            float d = 0.f;
+            HIPCHK(hipEventElapsedTime(&d, sev[0], sev[1]));
+            s.qflux_ms += d;
+            HIPCHK(hipEventElapsedTime(&d, sev[0], sev[nev - 1]));
+            s.step_ms += d;
+            for (size_t gi = 0; gi < plan.size(); ++gi)
+                for (int k = 0; k < 4; ++k) {
+                    HIPCHK(hipEventElapsedTime(&d, sev[2 + 5 * gi + k], sev[2 + 5 * gi + k + 1]));
+                    s.stage_ms[k] += d;
+                }
+        }
+        s.stage_count += (long)nsteps * (long)plan.size();
+        s.step_count += nsteps;
     }
+    for (auto &e : evs) (void)hipEventDestroy(e);
     return 0;
 }
 
@@ -2133,32 +2198,41 @@ int tracer_zero_tendencies(void) {
     State &s = S();
     if (ensure_device(s) || ensure_dropin_buffers(s)) return -1;
     const size_t NLn = (size_t)(s.nl - 1) * s.Np;
+    // queued on the library's stream ahead of the next drop-in call; no host wait
     HIPCHK(hipMemsetAsync(s.acc_h, 0, NLn * sizeof(double), s.stream));
     HIPCHK(hipMemsetAsync(s.acc_v, 0, NLn * sizeof(double), s.stream));
-    HIPCHK(hipStreamSynchronize(s.stream));
     return 0;
 }
 
+// The reference zeroes its flux and low-order work arrays at every call; here they are zeroed
+// once when allocated and the kernels write exactly the levels each edge / node has (the same set
+// at every call), so the levels below stay zero.  The edge transport depends on the velocity
+// only: it is computed once per velocity (tracer_set_velocity_elem bumps the version), not once
+// per tracer.  The kernels run on the library's stream without a host synchronisation: every
+// call that reads results (tracer_get_tendencies, tracer_get_work, ...) waits for them.
 int tracer_do_oce_adv_tra(double dt, int tracer_id) {
     State &s = S();
     if (!s.initialized) return fail("tracer not initialized.");
     if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
     if (ensure_device(s) || ensure_dropin_buffers(s) || ensure_wsplit(s)) return -1;
-    const size_t cb = compute_bytes(s.prec), kb = s.kb;
-    const size_t NLn = (size_t)(s.nl - 1) * s.Np, ELn = (size_t)(s.nl - 1) * s.Ep, NIn = (size_t)s.nl * s.Np;
-    // the reference zeroes its flux and low-order work arrays at every call
-    HIPCHK(hipMemsetAsync(s.dbg_h, 0, ELn * cb, s.stream));
-    HIPCHK(hipMemsetAsync(s.dbg_v, 0, NIn * cb, s.stream));
-    HIPCHK(hipMemsetAsync(s.adfh, 0, ELn * kb * cb, s.stream));
-    HIPCHK(hipMemsetAsync(s.adfv, 0, NIn * kb * cb, s.stream));
-    HIPCHK(hipMemsetAsync(s.LO, 0, NLn * kb * cb, s.stream));
+    if (s.lo_dirty) {  // a snapshot of values was kept in the LO buffer, also below the bottom
+        HIPCHK(hipMemsetAsync(s.LO, 0, (size_t)(s.nl - 1) * s.Np * s.kb * compute_bytes(s.prec), s.stream));
+        s.lo_dirty = false;
+    }
     const int id = tracer_id - 1, lane = id % s.kb;
-    hipError_t e = launch_qflux(s.m64, s.stream);
+    hipError_t e = hipSuccess;
+    if (s.q_vel != s.vel_gen) {
+        e = launch_qflux(s.m64, s.stream);
+        s.q_vel = s.vel_gen;
+    }
     if (e == hipSuccess) e = launch_block(s, id / s.kb, 1ull << lane, s.schemes[id], dt, true, nullptr);
-    if (e != hipSuccess) return fail("kernel launch failed: %s", hipGetErrorString(e));
+    if (e != hipSuccess) {
+        s.q_vel = ~0ull;
+        return fail("kernel launch failed: %s", hipGetErrorString(e));
+    }
     s.last_fct = s.schemes[id].fct;
     s.last_lane = lane;
-    HIPCHK(hipStreamSynchronize(s.stream));
+    s.dropin_valid = true;
     return 0;
 }
 
@@ -2174,7 +2248,9 @@ int tracer_get_tendencies(double *dttf_h, double *dttf_v, int nz, int nn) {
 
 int tracer_get_work(const char *name, double *out, size_t count) {
     State &s = S();
-    if (!s.on_device) return fail("tracer_get_work: nothing computed yet");
+    if (!s.on_device || !s.dropin_valid)
+        return fail("tracer_get_work: no tracer_do_oce_adv_tra call since the device state was built or last "
+                    "advanced (tracer_advect_step(s) reuse the work arrays for every tracer)");
     const size_t L = s.nl - 1, cb = compute_bytes(s.prec);
     const int kb = s.kb, lane = s.last_lane;
     if (!strcmp(name, "fct_LO")) {
@@ -2184,6 +2260,9 @@ int tracer_get_work(const char *name, double *out, size_t count) {
     if (!strcmp(name, "adv_flux_hor")) {
         if (count != L * s.E) return fail("tracer_get_work(adv_flux_hor): count must be %zu", L * s.E);
         if (s.last_fct && s.dbg_h) return download_work(s, s.dbg_h, (int)L, s.Ep, s.perm_e, out, cb, 1, 0);
+        if (s.prec != PREC_F64)  // the single-precision kernels store the high-order flux minus the upwind flux
+            return fail("tracer_get_work(adv_flux_hor): without the limiter the single-precision modes do not "
+                        "store the high-order flux itself");
         return download_work(s, s.adfh, (int)L, s.Ep, s.perm_e, out, cb, kb, lane);
     }
     if (!strcmp(name, "adv_flux_ver")) {
@@ -2217,11 +2296,74 @@ int tracer_get_values_device(int tracer_id, void *values, int nz, int nn) {
     return from_device(s, s.dT[(tracer_id - 1) / s.kb], (tracer_id - 1) % s.kb, static_cast<double *>(values), true);
 }
 
+int tracer_set_values_ab_device(int tracer_id, const void *values, int nl, int nod2D) {
+    State &s = S();
+    if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
+    if (check_dims(s, nl, nod2D, "tracer_set_values_ab_device")) return -1;
+    if (!values) return fail("tracer_set_values_ab_device: null pointer");
+    if (ensure_device(s)) return -1;
+    const int b = (tracer_id - 1) / s.kb, lane = (tracer_id - 1) % s.kb;
+    const size_t bytes = (size_t)(nl - 1) * s.Np * s.kb * storage_bytes(s.prec);
+    if (!s.tabbuf[b] && dalloc_bytes(s, &s.tabbuf[b], bytes)) return -1;
+    if (!s.tab_on[b]) {  // the block's other lanes keep valuesAB = values
+        HIPCHK(hipMemcpyAsync(s.tabbuf[b], s.dT[b], bytes, hipMemcpyDeviceToDevice, s.stream));
+        s.tab_on[b] = 1;
+    }
+    return to_device(s, static_cast<const double *>(values), true, s.tabbuf[b], lane);
+}
+
+int tracer_get_tendencies_device(void *dttf_h, void *dttf_v, int nz, int nn) {
+    State &s = S();
+    if (!s.on_device || !s.acc_h) return fail("tracer_get_tendencies_device: no tendencies computed yet");
+    if (nz != s.nl - 1 || nn != s.N) return fail("tracer_get_tendencies_device: expected (%d, %d)", s.nl - 1, s.N);
+    const double *src[2] = {s.acc_h, s.acc_v};
+    void *dst[2] = {dttf_h, dttf_v};
+    for (int i = 0; i < 2; ++i) {
+        if (!dst[i]) continue;
+        const hipError_t e = launch_relayout<double>(nullptr, nullptr, src[i], static_cast<double *>(dst[i]),
+                                                     s.d_perm_n, s.N, nz, s.Np, 1, 0, 1, s.stream);
+        if (e != hipSuccess) return fail("relayout launch failed: %s", hipGetErrorString(e));
+    }
+    HIPCHK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+int tracer_set_node_dt(const double *factor, int nod2D) {
+    State &s = S();
+    if (!s.dims) return fail("tracer_set_node_dt: call tracer_set_mesh_dims first");
+    if (factor && nod2D != s.N) return fail("tracer_set_node_dt: nod2D=%d != %d", nod2D, s.N);
+    if (factor) {
+        for (int n = 0; n < nod2D; ++n)
+            if (!(factor[n] > 0.0) || !std::isfinite(factor[n]))
+                return fail("tracer_set_node_dt: factor of node %d is %g (must be finite and > 0)", n + 1, factor[n]);
+        s.node_dt.assign(factor, factor + nod2D);
+    } else {
+        s.node_dt.clear();
+    }
+    if (s.on_device) {
+        HIPCHK(hipStreamSynchronize(s.stream));
+        if (upload_node_dt(s)) return -1;
+        s.gen++;  // kernels and captured graphs take the pointer
+    }
+    return 0;
+}
+
 int tracer_set_kernel_timing(int enable) {
     State &s = S();
     s.kernel_timing = enable != 0;
     for (double &v : s.stage_ms) v = 0.0;
     s.stage_count = 0;
+    s.qflux_ms = s.step_ms = 0.0;
+    s.step_count = 0;
+    return 0;
+}
+
+int tracer_get_step_times(double *qflux_ms, double *step_ms, long *count) {
+    if (!qflux_ms || !step_ms || !count) return fail("tracer_get_step_times: null pointer");
+    const State &s = S();
+    *qflux_ms = s.step_count ? s.qflux_ms / s.step_count : 0.0;
+    *step_ms = s.step_count ? s.step_ms / s.step_count : 0.0;
+    *count = s.step_count;
     return 0;
 }
 

@@ -569,8 +569,8 @@ __device__ __forceinline__ void for_edges_ld(const MeshCol<M> &m, const NodeEdge
 // TILED stages the neighbours' values per level in LDS.  The low-order vertical flux uses w_e;
 // with wsplit (:221-231) the flux subtracted from the high-order one is the upwind flux of the
 // full w instead, and k_vimpl then adds the implicit part to the stored LO.
-template <bool TILED, int VER, int KB, typename S, typename M, typename C>
-__global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt, int lc,
+template <bool TILED, int VER, int KB, typename S, typename M, typename C, bool NDT = false>
+__global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt_step, int lc,
                                                                  bool wsplit) {
     using D = double;
     __shared__ D sT[TILED ? 2 : 1][TILED ? kHtSlots : 1];
@@ -603,6 +603,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
     if (act) load_edges(m, n, ne);
     else no_edges(ne);
     const int nc_n = act ? n : 0;
+    const double dt = node_dt<NDT>(m, nc_n, dt_step);
     const NodeCol<D, S, M, KB> nc{Np, nc_n, lane, L};
     const Col<D, S, M, KB> col = make_col<D, S, M, KB>(p.TAB, m.w, m, nc_n, lane, nzmax);
     const D pv = (D)num_ord;
@@ -706,11 +707,12 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
 // tridiagonal system on hnode_new is built and eliminated top-down in double, storing (c', t')
 // in p.tri, then back-substituted bottom-up and added to LO.  Operation order as the reference
 // (FP64 results are bit-identical).  Columns have at least two wet layers (checked at setup).
-template <int KB, typename S, typename M, typename C>
-__global__ __launch_bounds__(256) void k_vimpl(const MeshCol<M> m, const StepPtrs<S, C> p, double dt) {
+template <int KB, typename S, typename M, typename C, bool NDT = false>
+__global__ __launch_bounds__(256) void k_vimpl(const MeshCol<M> m, const StepPtrs<S, C> p, double dt_step) {
     int n, lane;
     column_lane<KB>(n, lane);
     if (n >= m.N || !lane_on(p.mask, lane)) return;
+    const double dt = node_dt<NDT>(m, n, dt_step);
     const int L = cld(m.nlev_n, n) - 1;
     if (L < 2) return;
     const size_t Np = m.Np;
@@ -781,12 +783,13 @@ __global__ __launch_bounds__(256) void k_vimpl(const MeshCol<M> m, const StepPtr
 
 // Vertical high-order flux without the limiter (pwvel = we, o_init_zero=.true.), in double:
 // interfaces nz0..nz1 of the chunk, plus the bottom interface in the deepest chunk.
-template <int VER, int KB, typename S, typename M, typename C>
-__global__ __launch_bounds__(256, kMinWaves<KB>) void k_vert(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt, int lc) {
+template <int VER, int KB, typename S, typename M, typename C, bool NDT = false>
+__global__ __launch_bounds__(256, kMinWaves<KB>) void k_vert(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt_step, int lc) {
     using D = double;
     int n, lane;
     column_lane<KB>(n, lane);
     if (n >= m.N || !lane_on(p.mask, lane)) return;
+    const double dt = node_dt<NDT>(m, n, dt_step);
     const int nzmax = cld(m.nlev_n, n), L = nzmax - 1;
     int nz0, nz1;
     if (!chunk_layers(L, lc, nz0, nz1)) return;
@@ -839,8 +842,8 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_vert(const MeshCol<M> m,
 #endif
 template <bool TILED, int KB>
 constexpr bool kBoundsPfFor = FESOMTRA_BOUNDS_PF != 0 && !TILED && KB == 64;
-template <bool TILED, int KB, typename S, typename M, typename C>
-__global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
+template <bool TILED, int KB, typename S, typename M, typename C, bool NDT = false>
+__global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshCol<M> m, const StepPtrs<S, C> p, double dt_step, int lc) {
     using D = double;
     __shared__ D shi[TILED ? 2 : 1][TILED ? kHtSlots : 1], slo[TILED ? 2 : 1][TILED ? kHtSlots : 1];
     HaloTile<KB, M> ht;
@@ -869,6 +872,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
     }
     const size_t Np = m.Np, Ep = m.Ep;
     const int dry_from = act ? cld(m.nlev_min, n) : 0;  // levels >= this have a dry element around n
+    const double dt = node_dt<NDT>(m, act ? n : 0, dt_step);
     NodeEdges<TILED, kDegCacheFor<KB>> ne;
     if (act) load_edges(m, n, ne);
     else no_edges(ne);
@@ -1097,8 +1101,8 @@ template <bool TILED, int KB, typename C>
 constexpr bool kFinalPreFor = kFinalPre && !TILED && (KB == 64 || !std::is_same<C, double>::value);
 // AB2: also writes the next step's valuesAB and valuesold (a separate instantiation, so that the
 // other runs keep their register budget).
-template <bool TILED, bool FCT, int KB, typename S, typename M, typename C, bool AB2 = false>
-__global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCol<M> m, const StepPtrs<S, C> p, double dt, int lc) {
+template <bool TILED, bool FCT, int KB, typename S, typename M, typename C, bool AB2 = false, bool NDT = false>
+__global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCol<M> m, const StepPtrs<S, C> p, double dt_step, int lc) {
     static_assert(!TILED || FCT, "tiles stage the limiter ratios");
     constexpr bool PRE = kFinalPreFor<TILED, KB, C>;
     __shared__ C sRp[TILED ? 2 : 1][TILED ? kHtSlots : 1], sRm[TILED ? 2 : 1][TILED ? kHtSlots : 1];
@@ -1124,6 +1128,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
         lv0 = nz0;
         lv1 = nz1;
     }
+    const double dt = node_dt<NDT>(m, act ? n : 0, dt_step);
     const size_t Np = m.Np, Ep = m.Ep;
     NodeEdges<TILED, kDegCacheFor<KB>> ne;
     if (act) load_edges(m, n, ne);
@@ -1274,8 +1279,9 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
             p.acc_h[cell] = dh;
             p.acc_v[cell] = dv;
         } else if (p.seed != 0u && !std::is_same<S, double>::value) {
+            const uint32_t step = p.seed_dev ? *p.seed_dev : p.seed;
             st_stochastic(p.T, X(cell), tval + (dh + dv) / hn,
-                          unit_hash((uint32_t)(p.tid0 + lane), (uint32_t)cell, p.seed));
+                          unit_hash((uint32_t)(p.tid0 + lane), (uint32_t)cell, step));
         } else {
             std_(p.T, X(cell), tval + (dh + dv) / hn);
         }
@@ -1381,6 +1387,22 @@ static inline unsigned grid_for(size_t n_threads, int block) {
         }                                               \
     } while (0)
 
+#if !defined(FESOMTRA_INST) || (FESOMTRA_INST & 1)
+// Step counter of the stochastic rounding: set to `v`, or advanced by one (skipping 0, which
+// means "round to nearest"), by a single thread, so a captured graph advances it at every replay.
+__global__ void k_seed(uint32_t *c, uint32_t v, int advance) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        uint32_t x = advance ? c[0] + 1u : v;
+        if (advance && x == 0u) x = 1u;
+        c[0] = x;
+    }
+}
+hipError_t launch_seed(uint32_t *c, uint32_t v, int advance, hipStream_t s) {
+    k_seed<<<1, 64, 0, s>>>(c, v, advance);
+    return hipGetLastError();
+}
+#endif
+
 template <typename S>
 __global__ __launch_bounds__(256) void k_copy(S *__restrict__ dst, const S *__restrict__ src, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1400,7 +1422,7 @@ static void edge_kernel(dim3 g, hipStream_t s, const MeshCol<M> &m, const StepPt
     k_edge<false, HOR, FCT, KB, S, M, C><<<g, 256, 0, s>>>(m, p, ph, lc);
 }
 
-template <int KB, typename S, typename M, typename C>
+template <int KB, bool NDT, typename S, typename M, typename C>
 static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const SchemeArgs &sc,
                           const LaunchCfg &cfg, hipStream_t s, hipEvent_t *ev) {
     const int L = m.L;
@@ -1432,52 +1454,52 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
             if constexpr (KB <= 4) {
                 const dim3 tl = tgrid(cfg.lc_lo);
                 switch (sc.ver) {
-                case VER_UPW1: k_lo<true, VER_UPW1, KB><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-                case VER_QR4C: k_lo<true, VER_QR4C, KB><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-                case VER_CDIFF: k_lo<true, VER_CDIFF, KB><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-                case VER_PPM: k_lo<true, VER_PPM, KB><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+                case VER_UPW1: k_lo<true, VER_UPW1, KB, S, M, C, NDT><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+                case VER_QR4C: k_lo<true, VER_QR4C, KB, S, M, C, NDT><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+                case VER_CDIFF: k_lo<true, VER_CDIFF, KB, S, M, C, NDT><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+                case VER_PPM: k_lo<true, VER_PPM, KB, S, M, C, NDT><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
                 default: return hipErrorInvalidValue;
                 }
                 LAUNCH_CHECK();
                 if (sc.wsplit) {
                     if (!p.tri) return hipErrorInvalidValue;
-                    k_vimpl<KB><<<dim3(grid_for((size_t)m.N * KB, 256)), 256, 0, s>>>(m, p, sc.dt);
+                    k_vimpl<KB, S, M, C, NDT><<<dim3(grid_for((size_t)m.N * KB, 256)), 256, 0, s>>>(m, p, sc.dt);
                     LAUNCH_CHECK();
                 }
                 MARK(2);
-                k_bounds<true, KB><<<tgrid(cfg.lc_bounds), 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
+                k_bounds<true, KB, S, M, C, NDT><<<tgrid(cfg.lc_bounds), 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
                 LAUNCH_CHECK();
                 MARK(3);
-                if (p.ab_next) k_final<true, true, KB, S, M, C, true><<<tgrid(cfg.lc_final), 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
-                else k_final<true, true, KB><<<tgrid(cfg.lc_final), 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+                if (p.ab_next) k_final<true, true, KB, S, M, C, true, NDT><<<tgrid(cfg.lc_final), 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+                else k_final<true, true, KB, S, M, C, false, NDT><<<tgrid(cfg.lc_final), 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
             }
         } else {
             switch (sc.ver) {
-            case VER_UPW1: k_lo<false, VER_UPW1, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-            case VER_QR4C: k_lo<false, VER_QR4C, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-            case VER_CDIFF: k_lo<false, VER_CDIFF, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-            case VER_PPM: k_lo<false, VER_PPM, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+            case VER_UPW1: k_lo<false, VER_UPW1, KB, S, M, C, NDT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+            case VER_QR4C: k_lo<false, VER_QR4C, KB, S, M, C, NDT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+            case VER_CDIFF: k_lo<false, VER_CDIFF, KB, S, M, C, NDT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+            case VER_PPM: k_lo<false, VER_PPM, KB, S, M, C, NDT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
             default: return hipErrorInvalidValue;
             }
             LAUNCH_CHECK();
             if (sc.wsplit) {
                 if (!p.tri) return hipErrorInvalidValue;
-                k_vimpl<KB><<<dim3(grid_for((size_t)m.N * KB, 256)), 256, 0, s>>>(m, p, sc.dt);
+                k_vimpl<KB, S, M, C, NDT><<<dim3(grid_for((size_t)m.N * KB, 256)), 256, 0, s>>>(m, p, sc.dt);
                 LAUNCH_CHECK();
             }
             MARK(2);
-            k_bounds<false, KB><<<gb, 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
+            k_bounds<false, KB, S, M, C, NDT><<<gb, 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
             LAUNCH_CHECK();
             MARK(3);
-            if (p.ab_next) k_final<false, true, KB, S, M, C, true><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
-            else k_final<false, true, KB><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+            if (p.ab_next) k_final<false, true, KB, S, M, C, true, NDT><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+            else k_final<false, true, KB, S, M, C, false, NDT><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
         }
     } else {
         switch (sc.ver) {
-        case VER_UPW1: k_vert<VER_UPW1, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-        case VER_QR4C: k_vert<VER_QR4C, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-        case VER_CDIFF: k_vert<VER_CDIFF, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-        case VER_PPM: k_vert<VER_PPM, KB><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_UPW1: k_vert<VER_UPW1, KB, S, M, C, NDT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_QR4C: k_vert<VER_QR4C, KB, S, M, C, NDT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_CDIFF: k_vert<VER_CDIFF, KB, S, M, C, NDT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_PPM: k_vert<VER_PPM, KB, S, M, C, NDT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
         default: return hipErrorInvalidValue;
         }
         LAUNCH_CHECK();
@@ -1496,8 +1518,8 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
                 pf.TAB = reinterpret_cast<const S *>(p.LO);
             }
         }
-        if (pf.ab_next) k_final<false, false, KB, S, M, C, true><<<gf, 256, 0, s>>>(m, pf, sc.dt, cfg.lc_final);
-        else k_final<false, false, KB><<<gf, 256, 0, s>>>(m, pf, sc.dt, cfg.lc_final);
+        if (pf.ab_next) k_final<false, false, KB, S, M, C, true, NDT><<<gf, 256, 0, s>>>(m, pf, sc.dt, cfg.lc_final);
+        else k_final<false, false, KB, S, M, C, false, NDT><<<gf, 256, 0, s>>>(m, pf, sc.dt, cfg.lc_final);
     }
     LAUNCH_CHECK();
     MARK(4);
@@ -1516,14 +1538,16 @@ hipError_t launch_step(const MeshCol<M> &m, const StepPtrs<S, C> &p, int kb, con
                        const LaunchCfg &cfg, hipStream_t s, hipEvent_t *ev) {
     if (cfg.lc_edge < 1 || cfg.lc_lo < 1 || cfg.lc_bounds < 1 || cfg.lc_final < 1) return hipErrorInvalidValue;
     if (kb < 64 && (p.mask >> kb) != 0ull) return hipErrorInvalidValue;  // lanes beyond the block
+    // a per-node time step (m.dtn) selects kernels that load it; the others keep dt scalar
+    const bool ndt = m.dtn != nullptr;
     switch (kb) {
-    case 1: return step_kb<1>(m, p, sc, cfg, s, ev);
-    case 2: return step_kb<2>(m, p, sc, cfg, s, ev);
-    case 4: return step_kb<4>(m, p, sc, cfg, s, ev);
-    case 8: return step_kb<8>(m, p, sc, cfg, s, ev);
-    case 16: return step_kb<16>(m, p, sc, cfg, s, ev);
-    case 32: return step_kb<32>(m, p, sc, cfg, s, ev);
-    case 64: return step_kb<64>(m, p, sc, cfg, s, ev);
+    case 1: return ndt ? step_kb<1, true>(m, p, sc, cfg, s, ev) : step_kb<1, false>(m, p, sc, cfg, s, ev);
+    case 2: return ndt ? step_kb<2, true>(m, p, sc, cfg, s, ev) : step_kb<2, false>(m, p, sc, cfg, s, ev);
+    case 4: return ndt ? step_kb<4, true>(m, p, sc, cfg, s, ev) : step_kb<4, false>(m, p, sc, cfg, s, ev);
+    case 8: return ndt ? step_kb<8, true>(m, p, sc, cfg, s, ev) : step_kb<8, false>(m, p, sc, cfg, s, ev);
+    case 16: return ndt ? step_kb<16, true>(m, p, sc, cfg, s, ev) : step_kb<16, false>(m, p, sc, cfg, s, ev);
+    case 32: return ndt ? step_kb<32, true>(m, p, sc, cfg, s, ev) : step_kb<32, false>(m, p, sc, cfg, s, ev);
+    case 64: return ndt ? step_kb<64, true>(m, p, sc, cfg, s, ev) : step_kb<64, false>(m, p, sc, cfg, s, ev);
     default: return hipErrorInvalidValue;
     }
 }

@@ -25,6 +25,10 @@ struct LaunchCfg {
 template <typename M>
 hipError_t launch_qflux(const MeshCol<M> &m, hipStream_t s);
 
+// Step counter of the stochastic rounding in device memory: advance = 0 sets it to v, 1 adds one
+// (skipping 0).  Built with every precision family's object only once (FP64 family).
+hipError_t launch_seed(uint32_t *c, uint32_t v, int advance, hipStream_t s);
+
 template <typename S, typename M, typename C>
 hipError_t launch_step(const MeshCol<M> &m, const StepPtrs<S, C> &p, int kb, const SchemeArgs &sc,
                        const LaunchCfg &cfg, hipStream_t s, hipEvent_t *ev);

@@ -171,7 +171,17 @@ struct MeshCol {
     const M *ecd;                         // (E, 4) edge_cross_dxdy
     const M *edxdy;                       // (E, 2) edge_dxdy
     double *q;                            // [L][Ep] edge transport of the current velocity
+    const double *dtn;                    // (N) time-step factor per node, or null (all 1)
 };
+
+// Time step of node n's cells: dt, or dt * dtn[n] where the mesh carries a factor per node (a
+// union of independent meshes, each advanced with its own time step).  NDT: kernels compiled for
+// such meshes (the others keep dt a scalar kernel argument).
+template <bool NDT, typename M>
+__device__ __forceinline__ double node_dt(const MeshCol<M> &m, int n, double dt) {
+    if constexpr (NDT) return dt * (double)cld(m.dtn, n);
+    else return dt;
+}
 
 // Pointers of one step over one tracer block (arrays interleaved by the block width KB).
 template <typename S, typename C>
@@ -189,7 +199,10 @@ struct StepPtrs {
     double ab_c0, ab_c1;   // 0.5 + epsilon, 1.5 + epsilon
     unsigned long long mask;  // lanes advanced by this launch (tracers sharing its scheme)
     int tid0;                 // tracer number of lane 0 (seeds the stochastic rounding)
-    uint32_t seed;            // step counter (seeds the stochastic rounding); 0: round to nearest
+    uint32_t seed;            // nonzero: stochastic rounding of the update; 0: round to nearest
+    const uint32_t *seed_dev; // the step counter that seeds the stochastic rounding, in device
+                              // memory (advanced by a kernel per step, so captured graphs replay
+                              // with new seeds); null: `seed` itself is the counter
 };
 
 }  // namespace fesomtra

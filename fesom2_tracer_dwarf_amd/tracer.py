@@ -108,6 +108,8 @@ class FESOMTracerAdvection:
         abi.check(L.tracer_set_velocity_elem(D(case.uv), D(case.w), D(case.wi), D(case.we), m.nl,
                                              m.nod2D, m.elem2D), "tracer_set_velocity_elem")
         abi.check(L.tracer_set_wsplit(int(bool(getattr(case, "use_wsplit", False)))), "tracer_set_wsplit")
+        dtn = getattr(case, "dt_node", None)
+        abi.check(L.tracer_set_node_dt(None if dtn is None else D(dtn), m.nod2D), "tracer_set_node_dt")
         abi.check(L.tracer_allocate_tracers(case.ntracers, m.nl, m.nod2D, 2), "tracer_allocate_tracers")
         for i in range(case.ntracers):
             abi.check(L.tracer_set_values(i + 1, D(case.tracers[i]), m.nl, m.nod2D), "tracer_set_values")
@@ -186,6 +188,14 @@ class FESOMTracerAdvection:
         abi.check(self.lib.tracer_get_kernel_times(ms, ctypes.byref(cnt)), "tracer_get_kernel_times")
         return list(ms), cnt.value
 
+    def step_times(self):
+        """Mean ms per step of the edge-transport kernel and of the whole step (k_qflux through
+        the last pass) since tracer_set_kernel_timing(1), and the number of steps."""
+        q, st, cnt = ctypes.c_double(), ctypes.c_double(), ctypes.c_long()
+        abi.check(self.lib.tracer_get_step_times(ctypes.byref(q), ctypes.byref(st), ctypes.byref(cnt)),
+                  "tracer_get_step_times")
+        return q.value, st.value, cnt.value
+
     # --- stepping ---------------------------------------------------------------------------------
     def advect_steps(self, nsteps: int, dt: float) -> float:
         """Advance every tracer nsteps times; returns the device time in ms."""
@@ -255,6 +265,31 @@ class FESOMTracerAdvection:
         torch.cuda.current_stream().synchronize()
         abi.check(self.lib.tracer_set_values_device(tracer_id, ctypes.c_void_p(t.data_ptr()),
                                                     self.nl, self.nod2D), "tracer_set_values_device")
+
+    def set_values_ab_tensor(self, tracer_id: int, t) -> None:
+        """valuesAB from a float64 device tensor of shape (nod2D, nl-1) (reference numbering)."""
+        import torch
+        assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+        assert tuple(t.shape) == (self.nod2D, self.nl - 1)
+        torch.cuda.current_stream().synchronize()
+        abi.check(self.lib.tracer_set_values_ab_device(tracer_id, ctypes.c_void_p(t.data_ptr()),
+                                                       self.nl, self.nod2D), "tracer_set_values_ab_device")
+
+    def get_tendencies_tensor(self, dttf_h=None, dttf_v=None):
+        """del_ttf_advhoriz / advvert into float64 device tensors (nod2D, nl-1)."""
+        import torch
+        shape = (self.nod2D, self.nl - 1)
+        if dttf_h is None:
+            dttf_h = torch.empty(shape, dtype=torch.float64, device="cuda")
+        if dttf_v is None:
+            dttf_v = torch.empty(shape, dtype=torch.float64, device="cuda")
+        for t in (dttf_h, dttf_v):
+            assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and tuple(t.shape) == shape
+        torch.cuda.current_stream().synchronize()
+        abi.check(self.lib.tracer_get_tendencies_device(ctypes.c_void_p(dttf_h.data_ptr()),
+                                                        ctypes.c_void_p(dttf_v.data_ptr()),
+                                                        self.nl - 1, self.nod2D), "tracer_get_tendencies_device")
+        return dttf_h, dttf_v
 
     def get_values_tensor(self, tracer_id: int, out=None):
         import torch

@@ -130,6 +130,13 @@ int tracer_set_muscl_support(const int *edge_up_dn_tri, const int *nboundary_lay
  * do_oce_adv_tra reads (lib/oce_adv_tra_driver.F90:42-45).  w_i / w_e may be NULL (= w). */
 int tracer_set_velocity_elem(const double *uv, const double *w, const double *w_i,
                              const double *w_e, int nl, int nod2D, int elem2D);
+/* [new] Time-step factor per node, factor(nod2D) > 0 (NULL: 1 everywhere, the default): the
+ * cells of node n advance with dt * factor(n), where dt is the argument of the stepping calls.
+ * For a mesh that is a disjoint union of independent meshes (the library advances them in one
+ * launch sequence) each member keeps its own time step: pass its dt as the factor of its nodes
+ * and step with dt = 1.  A factor that differs between the two ends of an edge is not
+ * conservative; the library does not check it. */
+int tracer_set_node_dt(const double *factor, int nod2D);
 /* Scheme of one tracer with the reference's names: hor "UPW1" | "MUSCL" | "MFCT",
  * ver "UPW1" | "QR4C" | "CDIFF" | "PPM", lim "NONE" | "FCT"; ph, pv as tra_adv_ph / tra_adv_pv
  * (lib/MOD_TRACER.F90:18-22).  Unknown names return -1 (the reference aborts). */
@@ -142,7 +149,10 @@ int tracer_set_scheme(int tracer_id, const char *hor, const char *ver, const cha
 int tracer_advect_steps(int nsteps, double dt);
 /* The do_oce_adv_tra contract for one tracer (lib/oce_adv_tra_driver.F90:23-294): accumulates
  * into the device tendencies del_ttf_advhoriz / del_ttf_advvert (zeroed by
- * tracer_zero_tendencies, as the caller zeroes them) from values and valuesAB of tracer_id. */
+ * tracer_zero_tendencies, as the caller zeroes them) from values and valuesAB of tracer_id.
+ * The edge transport of the velocity is computed once per velocity (tracer_set_velocity_elem
+ * starts a new one) and shared by the tracers; the kernels are queued on the library's stream
+ * and the call returns without waiting for them (every call that reads results waits). */
 int tracer_do_oce_adv_tra(double dt, int tracer_id);
 int tracer_zero_tendencies(void);
 /* Copies del_ttf_advhoriz / del_ttf_advvert (nz=nl-1, nn=nod2D) out. */
@@ -150,14 +160,22 @@ int tracer_get_tendencies(double *dttf_h, double *dttf_v, int nz, int nn);
 /* valuesAB of one tracer (nl-1, nod2D), for tracer_do_oce_adv_tra; tracer_set_values and
  * tracer_advect_step reset it to values. */
 int tracer_set_values_ab(int tracer_id, const double *values, int nl, int nod2D);
+/* Device-pointer variants of tracer_set_values_ab and tracer_get_tendencies for callers whose
+ * arrays live in HBM: valuesAB(nl-1, nod2D) and dttf_h / dttf_v (nz=nl-1, nn=nod2D), float64 in
+ * reference order (either tendency pointer may be NULL). */
+int tracer_set_values_ab_device(int tracer_id, const void *values, int nl, int nod2D);
+int tracer_get_tendencies_device(void *dttf_h, void *dttf_v, int nz, int nn);
 int tracer_get_values_by_id(int tracer_id, double *values, int nz, int nn);
 /* Over the wet layers of tracer_id: min, max, plain sum and sum of T*hnode*areasvol (FP64). */
 int tracer_get_stats_by_id(int tracer_id, double *tmin, double *tmax, double *tsum,
                            double *vsum);
-/* Work arrays of the last tracer_do_oce_adv_tra / advect call for inspection: name is
- * "fct_LO" (nl-1, nod2D), "adv_flux_hor" (nl-1, edge2D) or "adv_flux_ver" (nl, nod2D), the
- * latter two after limiting as the reference leaves them.  count = number of doubles.  In the
- * single-precision modes "fct_LO" holds the increment fct_LO - values. */
+/* Work arrays of the last tracer_do_oce_adv_tra call for inspection: name is "fct_LO"
+ * (nl-1, nod2D), "adv_flux_hor" (nl-1, edge2D) or "adv_flux_ver" (nl, nod2D), the latter two
+ * after limiting as the reference leaves them.  count = number of doubles.  Returns -1 when a
+ * tracer_advect_step(s) call came after it (the steps reuse the arrays for every tracer).  In the
+ * single-precision modes "fct_LO" holds the increment fct_LO - values, and "adv_flux_hor" of a
+ * tracer without the limiter is not available (-1: the kernels store the high-order minus the
+ * upwind flux). */
 int tracer_get_work(const char *name, double *out, size_t count);
 /* Device-pointer variants of tracer_set_values / tracer_get_values for callers whose tracers
  * already live in HBM (e.g. PyTorch tensors): values(nl-1, nod2D) float64 in reference order. */
@@ -171,6 +189,10 @@ int tracer_last_elapsed_ms(double *ms);
  * and the number of tracer-steps averaged. */
 int tracer_set_kernel_timing(int enable);
 int tracer_get_kernel_times(double *stage_ms, long *count);
+/* With the kernel timing on: mean ms per step of the edge-transport kernel (k_qflux, once per
+ * step) and of the whole step (from before k_qflux to after the last pass), and the number of
+ * steps averaged. */
+int tracer_get_step_times(double *qflux_ms, double *step_ms, long *count);
 /* Arithmetic / storage precision (applies to all tracers; values are carried over):
  * 0 FP64 everywhere (default; bit-identical to the reference dp build),
  * 1 FP32 flux and reconstruction arithmetic, FP32 mesh and tracer storage,

@@ -90,10 +90,20 @@ def _p(a: np.ndarray, ctype):
     return a.ctypes.data_as(ctypes.POINTER(ctype))
 
 
+def ab2_interpolate(values_old: np.ndarray, values: np.ndarray, epsilon: float) -> np.ndarray:
+    """FESOM2's second-order Adams-Bashforth time level of the high-order fluxes,
+    valuesAB = -(0.5+epsilon)*valuesold + (1.5+epsilon)*values (epsilon of lib/oce_modules.F90:75,
+    valuesold of lib/MOD_TRACER.F90:13), evaluated in that operation order."""
+    return -(0.5 + epsilon) * values_old + (1.5 + epsilon) * values
+
+
 class Oracle:
     """CPU restatement bound to one mesh and velocity field."""
 
     def __init__(self, case):
+        if getattr(case, "dt_node", None) is not None:
+            raise ValueError("the restatement advances every node with one time step; "
+                             "run the members of a union with their own dt separately")
         m = case.mesh
         self.case = case
         self._keep = []
@@ -193,7 +203,7 @@ class Oracle:
             for i in range(case.ntracers):
                 if ab_mode != 0:
                     if ab_mode == 2:
-                        ab[i] = -(0.5 + ab_epsilon) * old[i] + (1.5 + ab_epsilon) * vals[i]
+                        ab[i] = ab2_interpolate(old[i], vals[i], ab_epsilon)
                         old[i] = vals[i].copy()
                     self.step_ab(vals[i], ab[i], case.schemes[i], case.dt)
                 elif work_step1 and s == 1 and i == 0:

@@ -1,0 +1,214 @@
+"""The benchmark workloads at their full size on the GPU, through the C ABI.
+
+* C1 mesh1000 (1000 x 1000 periodic, 48 levels, bathymetry, permuted numbering, T and S,
+  MFCT/QR4C/FCT): one FP64 step against the oracle over the whole field (rel Linf <= 1e-12: the
+  kernels keep the reference's operation and summation order); over its 100 steps the properties
+  that do not depend on size: the volume-weighted sum of each tracer conserved to FP64 round-off,
+  T and S inside their initial range (FCT with a divergence-free transport), a uniform tracer
+  staying uniform, and two runs agreeing bit for bit.
+* C2 (C1 with FP32 arithmetic and FP32 / FP16 / BF16 storage, 1000 steps): rel L2 / Linf and the
+  sum drift against the FP64 GPU path (itself equal to the oracle, above) within stated bounds.
+* C3 mesh500x64 (500 x 500 x 48, 64 tracers in one block): one FP64 step against the oracle on
+  four lanes; 200 FP32 steps within rel Linf 1e-5 of 200 FP64 steps.
+* C4 multimesh32 (32 uneven meshes in one union, each member with its own time step): one FP64
+  step of the union against the oracle on three members run alone with their own dt; 100 FP32
+  and BF16 steps against 100 FP64 steps.
+"""
+import numpy as np
+import pytest
+
+from fesom2_tracer_dwarf_amd import configs
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-12
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def rel_linf(got, ref):
+    return float(np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-300))
+
+
+def run_gpu(case, nsteps, prec=0, values=None, stats=False, kb=0):
+    """nsteps on the GPU; returns the final values (ntr, N, nl-1) and, with stats, the
+    volume-weighted sums before and after."""
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    t = FESOMTracerAdvection()
+    t.set_precision(prec)
+    t.set_block_width(kb)
+    t.setup_case(case)
+    if values is not None:
+        for i in range(case.ntracers):
+            t.set_values(i + 1, values[i])
+    v0 = [t.stats(i + 1)["volume_sum"] for i in range(case.ntracers)] if stats else None
+    t.advect_steps(nsteps, case.dt)
+    out = t.get_all_values()
+    v1 = [t.stats(i + 1)["volume_sum"] for i in range(case.ntracers)] if stats else None
+    t.set_precision(0)
+    t.set_block_width(0)
+    t.cleanup()
+    return (out, v0, v1) if stats else out
+
+
+def errors(got, ref, wet):
+    d = (got - ref)[:, wet]
+    r = ref[:, wet]
+    return (float(np.sqrt(np.sum(d * d) / np.sum(r * r))), float(np.max(np.abs(d)) / np.max(np.abs(r))))
+
+
+# ---- C1 ------------------------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def c1():
+    say("generating C1 mesh1000 (1000x1000x48, 2 tracers)")
+    return configs.mesh1000(100)
+
+
+@pytest.mark.timeout(900)
+def test_c1_one_step_fp64_matches_oracle(c1, oracle_mod):
+    say("C1 FP64 1 step on the GPU and in the oracle (both tracers, whole field)")
+    got = run_gpu(c1, 1)
+    ref, _ = oracle_mod.Oracle(c1).run(nsteps=1, save_steps=[1])
+    for i in range(c1.ntracers):
+        err = rel_linf(got[i], ref[1][i])
+        say(f"  tracer {i + 1}: rel Linf {err:.3e} bitwise {np.array_equal(got[i], ref[1][i])}")
+        assert err <= RTOL
+
+
+@pytest.mark.timeout(600)
+def test_c1_100_steps_conserve_bound_and_repeat(c1):
+    say("C1 FP64 100 steps: conservation, FCT range, determinism")
+    wet = configs.wet_mask(c1.mesh)
+    out, v0, v1 = run_gpu(c1, 100, stats=True)
+    for i in range(c1.ntracers):
+        drift = abs(v1[i] - v0[i]) / abs(v0[i])
+        lo, hi = c1.tracers[i][wet].min(), c1.tracers[i][wet].max()
+        vmin, vmax = out[i][wet].min(), out[i][wet].max()
+        say(f"  tracer {i + 1}: volume-sum drift {drift:.3e}, range [{vmin:.6g}, {vmax:.6g}] of [{lo:.6g}, {hi:.6g}]")
+        assert drift <= 1e-13
+        assert vmin >= lo - 1e-12 * abs(hi) and vmax <= hi + 1e-12 * abs(hi)
+        assert not np.any(out[i][~wet])  # below the bottom nothing is written
+    again = run_gpu(c1, 100)
+    assert np.array_equal(again, out)
+
+
+@pytest.mark.timeout(600)
+def test_c1_uniform_tracer_stays_uniform(c1):
+    say("C1 FP64 100 steps of a uniform tracer")
+    wet = configs.wet_mask(c1.mesh)
+    vals = np.stack([np.where(wet, 35.0, 0.0), np.where(wet, 4.0, 0.0)])
+    out = run_gpu(c1, 100, values=vals)
+    for i, c in enumerate((35.0, 4.0)):
+        dev = float(np.max(np.abs(out[i][wet] - c)))
+        say(f"  tracer {i + 1}: max |T - {c}| = {dev:.3e}")
+        assert dev <= 1e-12 * c
+
+
+# ---- C2 ------------------------------------------------------------------------------------------
+
+# (rel L2, rel Linf, volume-sum drift) after 1000 steps with stochastic rounding, per storage
+C2_BOUNDS = {1: (1e-5, 3e-3, 1e-8), 2: (2e-2, 0.15, 1e-5), 3: (5e-2, 0.3, 1e-4)}
+
+
+@pytest.fixture(scope="module")
+def c2_fp64(c1):
+    say("C2 reference: C1 FP64 1000 steps")
+    return run_gpu(c1, 1000, stats=True)
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("prec", [1, 2, 3], ids=["fp32", "fp16-storage", "bf16-storage"])
+def test_c2_1000_steps_single_precision_against_fp64(c1, c2_fp64, prec):
+    ref, _, _ = c2_fp64
+    wet = configs.wet_mask(c1.mesh)
+    say(f"C2 precision mode {prec}: 1000 steps")
+    got, v0, v1 = run_gpu(c1, 1000, prec=prec, stats=True)
+    l2b, linfb, driftb = C2_BOUNDS[prec]
+    for i in range(c1.ntracers):
+        l2, linf = errors(got[i:i + 1], ref[i:i + 1], wet)
+        drift = abs(v1[i] - v0[i]) / abs(v0[i])
+        say(f"  tracer {i + 1}: rel L2 {l2:.3e}, rel Linf {linf:.3e}, volume-sum drift {drift:.3e}")
+        assert l2 <= l2b and linf <= linfb and drift <= driftb
+
+
+# ---- C3 ------------------------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def c3():
+    say("generating C3 mesh500x64 (500x500x48, 64 tracers)")
+    return configs.mesh500x64(200)
+
+
+@pytest.mark.timeout(900)
+def test_c3_one_step_fp64_matches_oracle_on_four_lanes(c3, oracle_mod):
+    import dataclasses
+    say("C3 FP64 1 step of the 64-tracer block; oracle on tracers 1, 2, 37, 64")
+    got = run_gpu(c3, 1)
+    pick = [0, 1, 36, 63]
+    sub = dataclasses.replace(c3, tracers=c3.tracers[pick], schemes=[c3.schemes[i] for i in pick])
+    ref, _ = oracle_mod.Oracle(sub).run(nsteps=1, save_steps=[1])
+    for j, i in enumerate(pick):
+        err = rel_linf(got[i], ref[1][j])
+        say(f"  tracer {i + 1}: rel Linf {err:.3e} bitwise {np.array_equal(got[i], ref[1][j])}")
+        assert err <= RTOL
+
+
+@pytest.mark.timeout(600)
+def test_c3_200_steps_fp32_within_1e5_of_fp64(c3):
+    wet = configs.wet_mask(c3.mesh)
+    say("C3 200 steps FP64 and FP32 (64-tracer block)")
+    ref, r0, r1 = run_gpu(c3, 200, stats=True)
+    got, v0, v1 = run_gpu(c3, 200, prec=1, stats=True)
+    l2, linf = errors(got, ref, wet)
+    drift = max(abs(b - a) / abs(a) for a, b in zip(v0, v1))
+    drift64 = max(abs(b - a) / abs(a) for a, b in zip(r0, r1))
+    say(f"  FP32: rel L2 {l2:.3e}, rel Linf {linf:.3e}, volume-sum drift {drift:.3e} (FP64 {drift64:.3e})")
+    assert linf <= 1e-5 and l2 <= 2e-6
+    assert drift <= 1e-8 and drift64 <= 1e-13
+
+
+# ---- C4 ------------------------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def c4():
+    say("generating C4 multimesh32 (32 meshes, 4 tracers each)")
+    members = configs.multimesh32(100)
+    return members, configs.union_case(members, "multimesh32")
+
+
+@pytest.mark.timeout(900)
+def test_c4_union_one_step_fp64_matches_members_with_own_dt(c4, oracle_mod):
+    members, u = c4
+    assert u.dt == 1.0 and len({c.dt for c in members}) > 1
+    say("C4 FP64 1 step of the union; oracle on the three smallest members with their own dt")
+    got = run_gpu(u, 1)
+    off = np.cumsum([0] + [c.mesh.nod2D for c in members])
+    order = np.argsort([c.mesh.nod2D for c in members])[:3]
+    for i in order:
+        c = members[i]
+        ref, _ = oracle_mod.Oracle(c).run(nsteps=1, save_steps=[1])
+        part = got[:, off[i]:off[i + 1]]
+        L = c.mesh.nl - 1
+        err = rel_linf(part[:, :, :L], ref[1])
+        say(f"  member {i} ({c.mesh.nod2D} nodes, nl={c.mesh.nl}, dt={c.dt:.1f} s): rel Linf {err:.3e}")
+        assert err <= RTOL
+        assert not np.any(part[:, :, L:])
+
+
+@pytest.mark.timeout(900)
+def test_c4_100_steps_single_precision_against_fp64(c4):
+    _, u = c4
+    wet = configs.wet_mask(u.mesh)
+    say("C4 100 steps of the union: FP64, FP32 and BF16 storage")
+    ref, r0, r1 = run_gpu(u, 100, stats=True)
+    drift64 = max(abs(b - a) / abs(a) for a, b in zip(r0, r1))
+    say(f"  FP64 volume-sum drift {drift64:.3e}")
+    assert drift64 <= 1e-13
+    for prec, (l2b, linfb, driftb) in ((1, (1e-6, 1e-5, 1e-8)), (3, (5e-2, 0.3, 1e-4))):
+        got, v0, v1 = run_gpu(u, 100, prec=prec, stats=True)
+        l2, linf = errors(got, ref, wet)
+        drift = max(abs(b - a) / abs(a) for a, b in zip(v0, v1))
+        say(f"  mode {prec}: rel L2 {l2:.3e}, rel Linf {linf:.3e}, volume-sum drift {drift:.3e}")
+        assert l2 <= l2b and linf <= linfb and drift <= driftb

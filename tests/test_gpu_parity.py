@@ -294,7 +294,7 @@ def test_union_of_meshes_matches_oracle(oracle_mod):
     cs = [configs.bathymetry_case(f"m{i}", n, nl, seed=60 + i, ntracers=4, nsteps=3, lev_lo=3,
                                   lev_hi=nl, umax=u, w_cfl=0.3)
           for i, (n, nl, u) in enumerate([(9, 11, 0.1), (14, 17, 0.3), (7, 8, 0.05)])]
-    u = configs.union_case(cs)
+    u = configs.union_case(cs, own_dt=False)
     say("case union of 3 meshes, 4 tracers, FP64")
     ref, _ = oracle_mod.Oracle(u).run(save_steps=[3])
     got = gpu_run(u, 3, kb=4)

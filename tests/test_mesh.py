@@ -133,9 +133,17 @@ def test_union_case_equals_members_in_oracle(oracle_mod):
     cs = [configs.bathymetry_case(f"m{i}", n, nl, seed=50 + i, ntracers=2, nsteps=2, lev_lo=3,
                                   lev_hi=nl, umax=u, w_cfl=0.3)
           for i, (n, nl, u) in enumerate([(6, 9, 0.1), (8, 13, 0.3), (5, 7, 0.05)])]
-    u = configs.union_case(cs)
+    u = configs.union_case(cs, own_dt=False)
     assert u.mesh.nod2D == sum(c.mesh.nod2D for c in cs) and u.mesh.nl == 13
-    assert u.dt == min(c.dt for c in cs)
+    assert u.dt == min(c.dt for c in cs) and u.dt_node is None
+    # with own_dt every node carries its member's dt, and the union steps with dt = 1
+    v = configs.union_case(cs)
+    assert v.dt == 1.0
+    assert np.array_equal(v.dt_node, np.concatenate([np.full(c.mesh.nod2D, c.dt) for c in cs]))
+    # members with other schemes cannot share a union
+    other = dataclasses.replace(cs[1], schemes=[configs.Scheme("UPW1", "QR4C", "FCT")] * 2)
+    with pytest.raises(ValueError):
+        configs.union_case([cs[0], other])
     ref, _ = oracle_mod.Oracle(u).run(save_steps=[2])
     off = 0
     for c in cs:

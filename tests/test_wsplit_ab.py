@@ -165,3 +165,28 @@ def test_ab2_first_step_and_recurrence(oracle_mod):
     frozen, _ = oracle_mod.Oracle(case).run(save_steps=[2], ab_mode=1)
     plain, _ = oracle_mod.Oracle(case).run(save_steps=[2])
     assert not np.array_equal(out[2], plain[2]) and not np.array_equal(frozen[2], plain[2])
+
+
+def test_ab2_interpolation_pinned_by_exact_cases(oracle_mod):
+    """The AB2 time level is pinned by facts the formula must satisfy, not by restating it:
+    * a field linear in time, T^n = a + b n, gives valuesAB = T at time n + 1/2 + epsilon (the
+      linear extrapolation), for several epsilon;
+    * epsilon = 0.5 makes the coefficients -1 and 2, exact in binary, so valuesold = values
+      returns values bit for bit, and the first AB2 step equals the plain step bit for bit;
+    * epsilon = 0 with a field constant in time returns it (coefficients summing to 1).
+    A swapped pair of coefficients, a wrong sign or a missing epsilon fails one of these."""
+    ab2 = oracle_mod.ab2_interpolate
+    rng = np.random.default_rng(3)
+    a, b = rng.uniform(-5, 5, (50, 7)), rng.uniform(-1, 1, (50, 7))
+    n = 12.0
+    for eps in (0.0, 0.1, 0.25, 0.5):
+        got = ab2(a + b * (n - 1.0), a + b * n, eps)
+        want = a + b * (n + 0.5 + eps)
+        assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want)), eps
+    t = rng.uniform(0.0, 30.0, (50, 7))
+    assert np.array_equal(ab2(t, t, 0.5), t)
+    assert np.max(np.abs(ab2(t, t, 0.0) - t)) <= 2e-15 * np.max(t)
+    case = configs.bathymetry_case("ab2x", 9, 11, seed=48, nsteps=1, lev_lo=4, w_cfl=0.3)
+    two, _ = oracle_mod.Oracle(case).run(save_steps=[1], ab_mode=2, ab_epsilon=0.5)
+    plain, _ = oracle_mod.Oracle(case).run(save_steps=[1])
+    assert np.array_equal(two[1], plain[1])
