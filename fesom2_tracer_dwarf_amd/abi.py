@@ -74,6 +74,7 @@ _SIGNATURES = {
     "tracer_get_step_times": (_c_int, [_pd, _pd, ctypes.POINTER(ctypes.c_long)]),
     "tracer_set_values_ab_device": (_c_int, [_c_int, _vp, _c_int, _c_int]),
     "tracer_get_tendencies_device": (_c_int, [_vp, _vp, _c_int, _c_int]),
+    "tracer_do_oce_adv_tra_noalloc": (_c_int, [_c_double, _c_int, _vp, _vp, _vp, _vp]),
     "tracer_set_precision": (_c_int, [_c_int]),
     "tracer_set_tracers_per_pass": (_c_int, [_c_int]),
     "tracer_get_tracers_per_pass": (_c_int, [_pi]),

@@ -2384,3 +2384,5 @@ const char *tracer_last_error(void) { return g_err.c_str(); }
 const char *tracer_version(void) { return "fesomtra 0.3 (gfx950)"; }
 
 }  // extern "C"
+
+#include "fesomtra_noalloc.inc"

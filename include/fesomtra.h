@@ -165,6 +165,19 @@ int tracer_set_values_ab(int tracer_id, const double *values, int nl, int nod2D)
  * reference order (either tendency pointer may be NULL). */
 int tracer_set_values_ab_device(int tracer_id, const void *values, int nl, int nod2D);
 int tracer_get_tendencies_device(void *dttf_h, void *dttf_v, int nz, int nn);
+/* [new, no counterpart in the reference] tracer_do_oce_adv_tra for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops): values and valuesAB in,
+ * dttf_h / dttf_v accumulated (+=, as do_oce_adv_tra does), all device float64 (nl-1, nod2D) in
+ * reference order; every work array of the pass lives in a fixed 256 MiB device array of the
+ * library (one per GPU, held until the process ends).  Allocates nothing and waits for the
+ * kernels before returning.  Needs FP64 (tracer_set_precision(0)) and the mesh on the device
+ * (tracer_init_for_advection or any step before); returns -1 with a message for a mesh whose
+ * work arrays exceed the fixed array: 7 x (nl-1) x nod2D + nl x nod2D + (nl-1) x edge2D doubles
+ * (2 x (nl-1) x nod2D more with use_wsplit), about 2.5 M node-levels of a triangular mesh.  The
+ * array keeps the last call's contents and nothing in it needs to start at zero; calls that
+ * overlap on one GPU race on it. */
+int tracer_do_oce_adv_tra_noalloc(double dt, int tracer_id, const void *values, const void *values_ab,
+                                  void *dttf_h, void *dttf_v);
 int tracer_get_values_by_id(int tracer_id, double *values, int nz, int nn);
 /* Over the wet layers of tracer_id: min, max, plain sum and sum of T*hnode*areasvol (FP64). */
 int tracer_get_stats_by_id(int tracer_id, double *tmin, double *tmax, double *tsum,
