@@ -87,6 +87,7 @@ _SIGNATURES = {
     "tracer_set_halo_tiles": (_c_int, [_c_int]),
     "tracer_get_halo_tiles": (_c_int, [_pi]),
     "tracer_set_rounding": (_c_int, [_c_int]),
+    "tracer_set_compensated": (_c_int, [_c_int]),
     "tracer_checkpoint": (_c_int, [_c_int]),
     "tracer_set_launch_config": (_c_int, [_c_int, _c_int, _c_int, _c_int]),
     "tracer_get_launch_config": (_c_int, [_pi]),

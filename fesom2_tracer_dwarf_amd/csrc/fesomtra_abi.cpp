@@ -64,6 +64,7 @@ struct State {
     LaunchCfg cfg{};
     bool use_graph = false;
     bool stochastic = true;  // stochastic rounding of the update in the single-precision modes
+    bool compensated = false;  // single precision: keep each stored value's rounding remainder
     uint32_t step_seed = 0;  // advanced once per step
     uint32_t ckpt_seed = 0;  // step_seed at the last tracer_checkpoint(0)
     uint64_t gen = 0;  // bumped whenever captured launch sequences become stale
@@ -101,12 +102,17 @@ struct State {
     std::vector<char> t2_sync;  // dT2 holds the same below-bottom values as dT
     std::vector<void *> allocs;
     std::vector<void *> dT, tabbuf;  // per tracer block, storage type of the precision mode
+    std::vector<void *> dR;          // per block: remainders of the stored values (compensated)
     std::vector<char> tab_on;        // per block: valuesAB held apart from values
     std::vector<void *> ckpt;        // per block: device copy kept by tracer_checkpoint
+    std::vector<void *> ckpt_res;    // per block: the remainders kept with it (compensated storage)
     void *adfh = nullptr, *LO = nullptr, *Rp = nullptr, *Rm = nullptr, *adfv = nullptr;
+    void *LOe = nullptr;  // compensated single precision: remainder of the stored LO increment
     double *acc_h = nullptr, *acc_v = nullptr;
     void *dbg_h = nullptr, *dbg_v = nullptr;
     void *tri = nullptr;      // (c', t') of the implicit vertical solve (allocated with wsplit)
+    void *cqv = nullptr;      // cavity meshes: the limiter's element-bound rows above element tops
+    int cq_rows = 0;          // number of those rows (levels above the slot elements' tops)
     double *stage = nullptr;  // reference-layout staging buffer (N * L doubles)
     int *d_perm_n = nullptr;
     double4 *part = nullptr;
@@ -196,9 +202,11 @@ void release_device(State &s) {
     for (void *p : s.allocs) (void)hipFree(p);
     s.allocs.clear();
     s.dT.clear();
+    s.dR.clear();
     s.tabbuf.clear();
     s.tab_on.clear();
     s.ckpt.clear();
+    s.ckpt_res.clear();
     s.toldbuf.clear();
     s.tabbuf2.clear();
     s.ckpt_ab.clear();
@@ -324,22 +332,40 @@ int validate(State &s) {
     if (!s.have_hnode) return fail("layer thickness not set (tracer_set_vertical_grid / tracer_set_thickness)");
     if (!s.have_vert) return fail("vertical grid Z_3d_n / zbar_3d_n / helem not set (tracer_set_vertical_grid)");
     if (s.myDim >= 0 && s.myDim != s.N) return fail("myDim_nod2D=%d must equal nod2D=%d", s.myDim, s.N);
+    if (s.nl > 255) return fail("nl=%d: at most 255 levels", s.nl);
+    // ulevels > 1 (ice-shelf cavities): a column starts at its top layer; it keeps two layers
     for (int n = 0; n < s.N; ++n) {
-        if (s.ulev_n[n] != 1) return fail("node %d: ulevels_nod2D=%d; cavities (ulevels > 1) are not supported", n + 1, s.ulev_n[n]);
         if (s.nlev_n[n] < 3 || s.nlev_n[n] > s.nl) return fail("node %d: nlevels_nod2D=%d outside [3, nl]", n + 1, s.nlev_n[n]);
+        if (s.ulev_n[n] < 1 || s.nlev_n[n] - s.ulev_n[n] < 2)
+            return fail("node %d: ulevels_nod2D=%d leaves fewer than two layers above nlevels_nod2D=%d", n + 1,
+                        s.ulev_n[n], s.nlev_n[n]);
     }
     for (int t = 0; t < s.T; ++t) {
-        if (s.ulev_e[t] != 1) return fail("element %d: ulevels=%d; cavities are not supported", t + 1, s.ulev_e[t]);
         if (s.nlev_e[t] < 2 || s.nlev_e[t] > s.nl) return fail("element %d: nlevels=%d outside [2, nl]", t + 1, s.nlev_e[t]);
+        if (s.ulev_e[t] < 1 || s.ulev_e[t] >= s.nlev_e[t])
+            return fail("element %d: ulevels=%d leaves no layer above nlevels=%d", t + 1, s.ulev_e[t], s.nlev_e[t]);
         for (int c = 0; c < 3; ++c) {
             const int n = s.enodes[3 * t + c];
             if (s.nlev_n[n] < s.nlev_e[t])
                 return fail("node %d has fewer levels (%d) than element %d (%d)", n + 1, s.nlev_n[n], t + 1, s.nlev_e[t]);
+            if (s.ulev_n[n] > s.ulev_e[t])
+                return fail("node %d starts deeper (ulevels_nod2D=%d) than element %d (ulevels=%d)", n + 1, s.ulev_n[n],
+                            t + 1, s.ulev_e[t]);
         }
     }
-    for (int e = 0; e < s.E; ++e)
-        if (s.etri[2 * e] < 0) return fail("edge %d: edge_tri(1) must be a valid element", e + 1);
+    for (int e = 0; e < s.E; ++e) {
+        const int t1 = s.etri[2 * e], t2 = s.etri[2 * e + 1];
+        if (t1 < 0) return fail("edge %d: edge_tri(1) must be a valid element", e + 1);
+        // the scatter range of an edge is one interval of levels: its elements' wet layers meet
+        if (t2 >= 0 && (s.ulev_e[t2] > s.nlev_e[t1] || s.ulev_e[t1] > s.nlev_e[t2]))
+            return fail("edge %d: the wet layers of its elements %d and %d do not meet", e + 1, t1 + 1, t2 + 1);
+    }
     return 0;
+}
+
+bool has_cavities(const State &s) {
+    return std::any_of(s.ulev_e.begin(), s.ulev_e.end(), [](int u) { return u > 1; }) ||
+           std::any_of(s.ulev_n.begin(), s.ulev_n.end(), [](int u) { return u > 1; });
 }
 
 // nboundary_lay as muscl_adv_init derives it (lib/oce_muscl_adv.F90:96-159).
@@ -593,6 +619,57 @@ int refresh_velocity(State &s) {
     return 0;
 }
 
+// Ice-shelf cavities: the level tops in device order and the tables of the limiter's element
+// bounds above an element's top (MeshCol::cq_*); all null without cavities.
+int upload_cavities(State &s) {
+    MeshCol<double> &m = s.m64;
+    m.ulev_n = m.ulev_max = m.ulev_e = nullptr;
+    m.cq_n = 0;
+    m.cq_edge = m.cq_top = m.cq_off = m.cq_ptr = m.cq_ent = nullptr;
+    s.cq_rows = 0;
+    if (!has_cavities(s)) return 0;
+    const int N = s.N, T = s.T, A = s.max_adj;
+    std::vector<int> un(N), umax(N), ue(T);
+    for (int t = 0; t < T; ++t) ue[t] = s.ulev_e[s.perm_t[t]];
+    for (int n = 0; n < N; ++n) {
+        const int o = s.perm_n[n];
+        un[n] = s.ulev_n[o];
+        int mx = 0;
+        for (int k = 0; k < s.nie_num[o]; ++k) mx = std::max(mx, s.ulev_e[s.nie[(size_t)o * A + k]]);
+        umax[n] = mx;  // ulevels_nod2D_max (lib/oce_mesh.F90:1522-1560)
+    }
+    if (upload(s, &m.ulev_n, un) || upload(s, &m.ulev_max, umax) || upload(s, &m.ulev_e, ue)) return -1;
+    // slots: elements whose top lies below a top of one of their nodes' columns
+    std::vector<int> slot_of(T, -1), edge, top, off, ptr(N + 1, 0), ent;
+    for (int n = 0; n < N; ++n) {
+        const int o = s.perm_n[n];
+        for (int k = 0; k < s.nie_num[o]; ++k) {
+            const int eo = s.nie[(size_t)o * A + k];  // reference element (0-based)
+            if (s.ulev_e[eo] <= s.ulev_n[o]) continue;
+            const int t = s.inv_t[eo];
+            if (slot_of[t] < 0) {
+                // the work array is indexed by the element number as an edge number
+                if (eo >= s.E)
+                    return fail("element %d: the limiter's element bounds above its top read edge %d of %d", eo + 1,
+                                eo + 1, s.E);
+                slot_of[t] = (int)edge.size();
+                edge.push_back(s.inv_e[eo]);
+                top.push_back(s.ulev_e[eo]);
+                off.push_back(s.cq_rows);
+                s.cq_rows += s.ulev_e[eo] - 1;
+            }
+            ent.push_back(slot_of[t]);
+        }
+        ptr[n + 1] = (int)ent.size();
+    }
+    m.cq_n = (int)edge.size();
+    if (m.cq_n == 0) return 0;
+    if (upload(s, &m.cq_edge, edge) || upload(s, &m.cq_top, top) || upload(s, &m.cq_off, off) ||
+        upload(s, &m.cq_ptr, ptr) || upload(s, &m.cq_ent, ent))
+        return -1;
+    return 0;
+}
+
 // The per-node time-step factors in device order (m.dtn = null without them).
 int upload_node_dt(State &s) {
     s.m64.dtn = nullptr;
@@ -687,7 +764,7 @@ int build_device(State &s) {
         s.h_en = en;
         s.h_nle = nle;
     }
-    std::vector<int> top_of(E);
+    std::vector<int> top_of(E), utop_of(E);
     {
         std::vector<int4> ne(E);
         std::vector<int2> ud(E);
@@ -701,8 +778,9 @@ int build_device(State &s) {
             } else {
                 ud[e] = make_int2(-1, -1);
             }
-            // deepest level of the scatters' range (lib/oce_adv_tra_driver.F90:125-136)
+            // deepest and shallowest level of the scatters' range (lib/oce_adv_tra_driver.F90:125-136)
             top_of[o] = std::max(s.nlev_e[t1] - 1, t2 >= 0 ? s.nlev_e[t2] - 1 : 0);
+            utop_of[o] = t2 >= 0 ? std::min(s.ulev_e[t1], s.ulev_e[t2]) : s.ulev_e[t1];
         }
         if (upload(s, &m.edge_ne, ne) || upload(s, &m.updn, ud)) return -1;
         s.h_ud = ud;
@@ -717,21 +795,25 @@ int build_device(State &s) {
         }
         for (int n = 0; n < N; ++n) ptr[n + 1] += ptr[n];
         std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-        std::vector<int> e2(ptr[N]), top(ptr[N]), oth(ptr[N]);
+        std::vector<int> e2(ptr[N]), top(ptr[N]), utop(ptr[N]), oth(ptr[N]);
         for (int o = 0; o < E; ++o) {
             const int a = s.inv_n[s.edges[2 * o]], b = s.inv_n[s.edges[2 * o + 1]];
             int i = fill[a]++;
             e2[i] = s.inv_e[o] * 2 + 0;
             top[i] = top_of[o];
+            utop[i] = utop_of[o];
             oth[i] = b;
             i = fill[b]++;
             e2[i] = s.inv_e[o] * 2 + 1;
             top[i] = top_of[o];
+            utop[i] = utop_of[o];
             oth[i] = a;
         }
         if (upload(s, &m.ne_ptr, ptr) || upload(s, &m.ne_e2, e2) || upload(s, &m.ne_top, top) ||
             upload(s, &m.ne_other, oth))
             return -1;
+        m.ne_utop = nullptr;
+        if (has_cavities(s) && upload(s, &m.ne_utop, utop)) return -1;
         s.h_ptr = ptr;
         s.h_e2 = e2;
         s.h_top = top;
@@ -740,6 +822,7 @@ int build_device(State &s) {
     s.h_nlev = i_nlev;
     if (upload_mesh<double>(s, s.m64, i_nlev, i_nmin, i_nbl)) return -1;
     if (upload_node_dt(s)) return -1;
+    if (upload_cavities(s)) return -1;
     if (dalloc(s, &s.m64.q, (size_t)L * s.Ep)) return -1;
     {
         const int *p;
@@ -757,6 +840,12 @@ int build_device(State &s) {
         return -1;
     if (dalloc(s, &s.part, (size_t)kStatsBlocks) || dalloc(s, &s.stage, (size_t)N * L)) return -1;
     if (dalloc(s, &s.d_seed, 64)) return -1;
+    s.cqv = nullptr;
+    if (s.m64.cq_n > 0) {
+        const size_t bytes = (size_t)s.cq_rows * 2 * kb * cb;
+        if (dalloc_bytes(s, &s.cqv, bytes)) return -1;
+        HIPCHK(hipMemset(s.cqv, 0, std::max<size_t>(bytes, 256)));
+    }
     // work arrays start at zero: the kernels write only the levels a mesh entity has, so the
     // levels below stay zero, as the reference's zeroed work arrays read there
     HIPCHK(hipMemset(s.adfh, 0, ELn * kb * cb));
@@ -771,6 +860,16 @@ int build_device(State &s) {
         if (dalloc_bytes(s, &s.dT[b], NLn * kb * sb)) return -1;
         HIPCHK(hipMemset(s.dT[b], 0, NLn * kb * sb));
     }
+    s.dR.assign(nb, nullptr);
+    s.LOe = nullptr;
+    if (s.compensated && s.prec != PREC_F64) {
+        for (int b = 0; b < nb; ++b) {  // FP32 remainders whatever the storage type
+            if (dalloc_bytes(s, &s.dR[b], NLn * kb * sizeof(float))) return -1;
+            HIPCHK(hipMemset(s.dR[b], 0, NLn * kb * sizeof(float)));
+        }
+        if (dalloc_bytes(s, &s.LOe, NLn * kb * cb)) return -1;
+        HIPCHK(hipMemset(s.LOe, 0, NLn * kb * cb));
+    }
     if (build_halo_tiles(s) || build_edge_tiles(s)) return -1;
     HIPCHK(hipDeviceSynchronize());
     s.on_device = true;
@@ -778,8 +877,17 @@ int build_device(State &s) {
     return 0;
 }
 
-// reference-layout host values -> lane of a device tracer block (through the staging buffer)
+// The remainder buffer of a tracer block's storage (compensated storage), else null.
+void *resid_of(const State &s, const void *buf) {
+    for (size_t b = 0; b < s.dT.size(); ++b)
+        if (s.dT[b] == buf) return b < s.dR.size() ? s.dR[b] : nullptr;
+    return nullptr;
+}
+
+// reference-layout host values -> lane of a device tracer block (through the staging buffer); a
+// block with compensated storage also gets the rounding remainders
 int to_device(State &s, const double *host_or_dev, bool is_device, void *dst, int lane) {
+    void *res = resid_of(s, dst);
     const int L = s.nl - 1;
     const double *src = host_or_dev;
     if (!is_device) {
@@ -789,25 +897,27 @@ int to_device(State &s, const double *host_or_dev, bool is_device, void *dst, in
     hipError_t e;
     switch (s.prec) {
     case PREC_F64: e = launch_relayout<double>(src, (double *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 0, s.stream); break;
-    case PREC_F32: e = launch_relayout<float>(src, (float *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 0, s.stream); break;
-    case PREC_F32_F16: e = launch_relayout<f16_t>(src, (f16_t *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 0, s.stream); break;
-    default: e = launch_relayout<bf16_t>(src, (bf16_t *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 0, s.stream); break;
+    case PREC_F32: e = launch_relayout<float>(src, (float *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 0, s.stream, (float *)res); break;
+    case PREC_F32_F16: e = launch_relayout<f16_t>(src, (f16_t *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 0, s.stream, (float *)res); break;
+    default: e = launch_relayout<bf16_t>(src, (bf16_t *)dst, nullptr, nullptr, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 0, s.stream, (float *)res); break;
     }
     if (e != hipSuccess) return fail("relayout launch failed: %s", hipGetErrorString(e));
     HIPCHK(hipStreamSynchronize(s.stream));
     return 0;
 }
 
-// lane of a device tracer block -> reference layout (host or device destination)
+// lane of a device tracer block -> reference layout (host or device destination), stored value
+// plus remainder with compensated storage
 int from_device(State &s, const void *src, int lane, double *out, bool out_is_device) {
+    void *res = resid_of(s, src);
     const int L = s.nl - 1;
     double *dst = out_is_device ? out : s.stage;
     hipError_t e;
     switch (s.prec) {
     case PREC_F64: e = launch_relayout<double>(nullptr, nullptr, (const double *)src, dst, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 1, s.stream); break;
-    case PREC_F32: e = launch_relayout<float>(nullptr, nullptr, (const float *)src, dst, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 1, s.stream); break;
-    case PREC_F32_F16: e = launch_relayout<f16_t>(nullptr, nullptr, (const f16_t *)src, dst, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 1, s.stream); break;
-    default: e = launch_relayout<bf16_t>(nullptr, nullptr, (const bf16_t *)src, dst, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 1, s.stream); break;
+    case PREC_F32: e = launch_relayout<float>(nullptr, nullptr, (const float *)src, dst, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 1, s.stream, (float *)res); break;
+    case PREC_F32_F16: e = launch_relayout<f16_t>(nullptr, nullptr, (const f16_t *)src, dst, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 1, s.stream, (float *)res); break;
+    default: e = launch_relayout<bf16_t>(nullptr, nullptr, (const bf16_t *)src, dst, s.d_perm_n, s.N, L, s.Np, s.kb, lane, 1, s.stream, (float *)res); break;
     }
     if (e != hipSuccess) return fail("relayout launch failed: %s", hipGetErrorString(e));
     if (!out_is_device)
@@ -916,11 +1026,14 @@ StepPtrs<S_, C_> make_ptrs(State &s, int blk, unsigned long long mask, bool drop
     }
     p.adfh = static_cast<C_ *>(s.adfh);
     p.LO = static_cast<C_ *>(s.LO);
+    p.LOe = static_cast<C_ *>(s.LOe);
     p.Rp = static_cast<C_ *>(s.Rp);
     p.Rm = static_cast<C_ *>(s.Rm);
     p.adfv = static_cast<C_ *>(s.adfv);
     p.mask = mask;
     p.tri = static_cast<double2 *>(s.tri);
+    p.cqv = static_cast<C_ *>(s.cqv);
+    p.resid = dropin ? nullptr : static_cast<float *>(resid_of(s, s.dT[blk]));
     if (dropin) {
         p.acc_h = s.acc_h;
         p.acc_v = s.acc_v;
@@ -932,7 +1045,9 @@ StepPtrs<S_, C_> make_ptrs(State &s, int blk, unsigned long long mask, bool drop
 
 hipError_t launch_block(State &s, int blk, unsigned long long mask, const SchemeSel &sc, double dt, bool dropin,
                         hipEvent_t *ev, uint32_t seed = 0, int ab_parity = 0) {
-    SchemeArgs a{sc.hor, sc.ver, sc.fct, sc.ph, sc.pv, dt, s.wsplit && sc.fct};
+    // cavity meshes: the limiter reads edge gradients filled from this tracer's valuesAB where the
+    // scheme fills them (MUSCL / MFCT) and always for the drop-in, whose caller fills them
+    SchemeArgs a{sc.hor, sc.ver, sc.fct, sc.ph, sc.pv, dt, s.wsplit && sc.fct, sc.hor != HOR_UPW1 || dropin};
     switch (s.prec) {
     case PREC_F64: return launch_step<double, double, double>(s.m64, make_ptrs<double, double>(s, blk, mask, dropin, seed, ab_parity), s.kb, a, s.cfg, s.stream, ev);
     case PREC_F32: return launch_step<float, double, float>(s.m64, make_ptrs<float, float>(s, blk, mask, dropin, seed, ab_parity), s.kb, a, s.cfg, s.stream, ev);
@@ -1090,6 +1205,7 @@ int build_tiles(State &s) {
 // implicit vertical split, with valuesAB := values).
 bool fused_applies(const State &s, const SchemeSel &sc) {
     if (!s.node_dt.empty()) return false;  // the fused kernel takes one time step for all nodes
+    if (s.m64.ulev_n) return false;        // and has no cavity levels
     return s.fused && !s.wsplit && s.ab_mode == 0 && s.prec == PREC_F64 && s.kb == 1 && sc.fct;
 }
 
@@ -1251,6 +1367,18 @@ int advect(State &s, int nsteps, double dt) {
     if (ensure_wsplit(s)) return -1;
     if (s.ab_mode == 0) std::fill(s.tab_on.begin(), s.tab_on.end(), 0);  // valuesAB := values
     const std::vector<Launch> plan = launch_plan(s);
+    if (s.m64.cq_n > 0) {
+        // the limiter of a UPW1 tracer reads the gradients another tracer's step left in the
+        // reference's shared work array: defined here only when no tracer fills them
+        bool fills = false, reads = false;
+        for (const SchemeSel &sc : s.schemes) {
+            fills = fills || sc.hor != HOR_UPW1;
+            reads = reads || (sc.fct && sc.hor == HOR_UPW1);
+        }
+        if (fills && reads)
+            return fail("cavity mesh: a UPW1 tracer with the FCT limiter next to MUSCL / MFCT tracers reads edge "
+                        "gradients of another tracer in the reference (not reproduced); advance them separately");
+    }
     if (s.fused && s.prec == PREC_F64 && s.kb == 1 && ensure_fused(s)) return -1;
     if (!plan.empty()) s.last_fct = plan.back().sc.fct;
     if (nsteps > 0) s.dropin_valid = false;  // the work arrays now hold this call's intermediates
@@ -1332,13 +1460,14 @@ int stats(State &s, int id, int all_levels, double *mn, double *mx, double *sm, 
     if (ensure_device(s)) return -1;
     if (id < 1 || id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", id, s.ntr);
     const void *T = s.dT[(id - 1) / s.kb];
+    const void *R = resid_of(s, T);
     const int kb = s.kb, lane = (id - 1) % s.kb;
     hipError_t e;
     switch (s.prec) {
     case PREC_F64: e = launch_stats<double, double>(s.m64, (const double *)T, kb, lane, all_levels, s.part, kStatsBlocks, s.stream); break;
-    case PREC_F32: e = launch_stats<float, double>(s.m64, (const float *)T, kb, lane, all_levels, s.part, kStatsBlocks, s.stream); break;
-    case PREC_F32_F16: e = launch_stats<f16_t, double>(s.m64, (const f16_t *)T, kb, lane, all_levels, s.part, kStatsBlocks, s.stream); break;
-    default: e = launch_stats<bf16_t, double>(s.m64, (const bf16_t *)T, kb, lane, all_levels, s.part, kStatsBlocks, s.stream); break;
+    case PREC_F32: e = launch_stats<float, double>(s.m64, (const float *)T, kb, lane, all_levels, s.part, kStatsBlocks, s.stream, (const float *)R); break;
+    case PREC_F32_F16: e = launch_stats<f16_t, double>(s.m64, (const f16_t *)T, kb, lane, all_levels, s.part, kStatsBlocks, s.stream, (const float *)R); break;
+    default: e = launch_stats<bf16_t, double>(s.m64, (const bf16_t *)T, kb, lane, all_levels, s.part, kStatsBlocks, s.stream, (const float *)R); break;
     }
     if (e != hipSuccess) return fail("stats launch failed: %s", hipGetErrorString(e));
     std::vector<double4> h(kStatsBlocks);
@@ -1655,6 +1784,7 @@ int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D) {
     hipEvent_t k0 = s.ev0, k1 = s.ev1;
     const int prec = s.prec, kbr = s.kb_req;
     const bool graph = s.use_graph, sr = s.stochastic, fused = s.fused, ht = s.halo_tiles, ws = s.wsplit;
+    const bool comp = s.compensated;
     const int abm = s.ab_mode;
     const double abe = s.ab_eps;
     const uint64_t gen = s.gen;
@@ -1668,6 +1798,7 @@ int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D) {
     s.kb_req = kbr;
     s.use_graph = graph;
     s.stochastic = sr;
+    s.compensated = comp;
     s.fused = fused;
     s.halo_tiles = ht;
     s.wsplit = ws;
@@ -2055,13 +2186,19 @@ int tracer_checkpoint(int restore) {
     State &s = S();
     if (ensure_device(s)) return -1;
     const size_t bytes = (size_t)(s.nl - 1) * s.Np * s.kb * storage_bytes(s.prec);
+    const size_t rbytes = (size_t)(s.nl - 1) * s.Np * s.kb * sizeof(float);  // FP32 remainders
     if (!restore) {
         if (s.ckpt.size() != s.dT.size()) s.ckpt.assign(s.dT.size(), nullptr);
+        if (s.ckpt_res.size() != s.dT.size()) s.ckpt_res.assign(s.dT.size(), nullptr);
         if (s.ckpt_ab.size() != s.dT.size()) s.ckpt_ab.assign(s.dT.size(), nullptr);
         if (s.ckpt_old.size() != s.dT.size()) s.ckpt_old.assign(s.dT.size(), nullptr);
         for (size_t b = 0; b < s.dT.size(); ++b) {
             if (!s.ckpt[b] && dalloc_bytes(s, &s.ckpt[b], bytes)) return -1;
             HIPCHK(hipMemcpyAsync(s.ckpt[b], s.dT[b], bytes, hipMemcpyDeviceToDevice, s.stream));
+            if (b < s.dR.size() && s.dR[b]) {
+                if (!s.ckpt_res[b] && dalloc_bytes(s, &s.ckpt_res[b], rbytes)) return -1;
+                HIPCHK(hipMemcpyAsync(s.ckpt_res[b], s.dR[b], rbytes, hipMemcpyDeviceToDevice, s.stream));
+            }
             if (s.ab_mode != 0) {  // the time levels the next steps read
                 if (!s.ckpt_ab[b] && dalloc_bytes(s, &s.ckpt_ab[b], bytes)) return -1;
                 HIPCHK(hipMemcpyAsync(s.ckpt_ab[b], s.tabbuf[b], bytes, hipMemcpyDeviceToDevice, s.stream));
@@ -2078,6 +2215,11 @@ int tracer_checkpoint(int restore) {
         if (s.ckpt_ab_mode != s.ab_mode) return fail("tracer_checkpoint: the valuesAB mode changed since the checkpoint");
         for (size_t b = 0; b < s.dT.size(); ++b) {
             HIPCHK(hipMemcpyAsync(s.dT[b], s.ckpt[b], bytes, hipMemcpyDeviceToDevice, s.stream));
+            if (b < s.dR.size() && s.dR[b]) {
+                if (b >= s.ckpt_res.size() || !s.ckpt_res[b])
+                    return fail("tracer_checkpoint: the checkpoint holds no remainders (compensated storage changed)");
+                HIPCHK(hipMemcpyAsync(s.dR[b], s.ckpt_res[b], rbytes, hipMemcpyDeviceToDevice, s.stream));
+            }
             if (s.ab_mode != 0)
                 HIPCHK(hipMemcpyAsync(s.tabbuf[b], s.ckpt_ab[b], bytes, hipMemcpyDeviceToDevice, s.stream));
             if (s.ab_mode == 2)
@@ -2156,6 +2298,20 @@ int tracer_set_halo_tiles(int enable) {
 int tracer_get_halo_tiles(int *ntiles) {
     if (!ntiles) return fail("tracer_get_halo_tiles: null pointer");
     *ntiles = S().m64.ht_n;
+    return 0;
+}
+
+int tracer_set_compensated(int enable) {
+    State &s = S();
+    const bool on = enable != 0;
+    if (s.compensated == on) return 0;
+    if (s.on_device) {  // keep the values: read back (with remainders), rebuild with or without them
+        if (pull_values(s)) return -1;
+        release_device(s);
+    }
+    s.compensated = on;
+    s.gen++;
+    if (s.initialized) return ensure_device(s);
     return 0;
 }
 

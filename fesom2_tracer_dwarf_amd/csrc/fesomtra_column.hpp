@@ -2,7 +2,7 @@
 //
 // Every expression keeps the operation order of the reference routine it cites so that the
 // double instantiation reproduces the serial reference bit for bit.  Levels are 1-based (nz),
-// interfaces 1..nlevels_nod2D; ulevels == 1.
+// interfaces ulevels_nod2D..nlevels_nod2D (the top is 1 without ice-shelf cavities).
 #pragma once
 
 #include "fesomtra_types.hpp"
@@ -59,7 +59,7 @@ __device__ __forceinline__ C tr_xy(const C g[3], const LD &t, size_t off, int a,
     }
 }
 
-// Area-weighted mean element gradient around node n at level nz
+// Area-weighted mean element gradient around node n at level nz over the elements wet there
 // (lib/oce_muscl_adv.F90:394-409), accumulated in nod_in_elem2D order.
 template <typename C, typename LD, typename M>
 __device__ __forceinline__ void node_mean_grad(const MeshCol<M> &m, const LD &t, int n, int nz, C &gx,
@@ -69,7 +69,7 @@ __device__ __forceinline__ void node_mean_grad(const MeshCol<M> &m, const LD &t,
     const int cnt = cld(m.nie_num, n);
     for (int i = 0; i < cnt; ++i) {
         const int el = cld(m.nie, (size_t)n * m.max_adj + i);
-        if (cld(m.nlev_e, el) - 1 < nz) continue;
+        if (cld(m.nlev_e, el) - 1 < nz || nz < elem_top(m, el)) continue;
         const C ar = (C)cld(m.elem_area, el);
         const int en[3] = {cld(m.enodes, (size_t)el * 3), cld(m.enodes, (size_t)el * 3 + 1),
                            cld(m.enodes, (size_t)el * 3 + 2)};
@@ -225,15 +225,15 @@ __device__ __forceinline__ C ver_ppm(const Col<C, S, M, KB> &c, int j, C dt, C f
     return tvert - fin;
 }
 
-// QR4C (:394-436) at interface j from the register window tw[i] = T(j-2+i), i = 0..3 and
-// z[i] = Z_3d_n(j-2+i), minus fin.  A 2-layer column updates its middle interface twice, as the
-// reference's statement sequence does.
+// QR4C (:394-436) at interface j of a column with top layer `top` from the register window
+// tw[i] = T(j-2+i), i = 0..3 and z[i] = Z_3d_n(j-2+i), minus fin.  A 2-layer column updates its
+// middle interface twice, as the reference's statement sequence does.
 template <typename C>
-__device__ __forceinline__ C ver_qr4c_w(int j, int nzmax, const C tw[4], const C z[4], C zb, C w,
+__device__ __forceinline__ C ver_qr4c_w(int j, int top, int nzmax, const C tw[4], const C z[4], C zb, C w,
                                         C ar, C num_ord, C fin, const C *qw = nullptr) {
-    if (j == 1) return -tw[2] * w * ar - fin;
+    if (j == top) return -tw[2] * w * ar - fin;
     if (j == nzmax) return C(0) - fin;
-    const bool second = (j == 2), penult = (j == nzmax - 1);
+    const bool second = (j == top + 1), penult = (j == nzmax - 1);
     if (second || penult) {
         const C cv = -C(0.5) * (tw[1] + tw[2]) * w * ar;
         C f = fin;
@@ -252,10 +252,11 @@ __device__ __forceinline__ C ver_qr4c_w(int j, int nzmax, const C tw[4], const C
     return (-C(0.5) * (C(1) - num_ord) * Tmean - num_ord * (C(0.5) * (Tmean1 + Tmean2)) * w) * ar - fin;
 }
 
-// adv_tra_ver_upw1 (:300-327) at interface j with T(j-1) = tm, T(j) = tj, minus fin.
+// adv_tra_ver_upw1 (:300-327) at interface j of a column with top layer `top`, with T(j-1) = tm,
+// T(j) = tj, minus fin.
 template <typename C>
-__device__ __forceinline__ C ver_upw1_w(int j, int nzmax, C tm, C tj, C w, C ar, C fin) {
-    if (j == 1) return -w * tj * ar - fin;
+__device__ __forceinline__ C ver_upw1_w(int j, int top, int nzmax, C tm, C tj, C w, C ar, C fin) {
+    if (j == top) return -w * tj * ar - fin;
     if (j == nzmax) return C(0) - fin;
     return -C(0.5) * (tj * (w + fabs(w)) + tm * (w - fabs(w))) * ar - fin;
 }
@@ -299,15 +300,15 @@ template <int VER, typename C, typename S, typename M, int KB>
 __device__ __forceinline__ C ver_ho_at(int j, int nzmax, const C tw[4], const C z[4], C zbj, C wj, C aj,
                                        C num_ord, C dt, const Col<C, S, M, KB> &col, C fin,
                                        const C *qw = nullptr) {
-    if (VER == VER_QR4C) return ver_qr4c_w<C>(j, nzmax, tw, z, zbj, wj, aj, num_ord, fin, qw);
-    if (VER == VER_UPW1) return ver_upw1_w<C>(j, nzmax, tw[1], tw[2], wj, aj, fin);
+    if (VER == VER_QR4C) return ver_qr4c_w<C>(j, col.nzmin, nzmax, tw, z, zbj, wj, aj, num_ord, fin, qw);
+    if (VER == VER_UPW1) return ver_upw1_w<C>(j, col.nzmin, nzmax, tw[1], tw[2], wj, aj, fin);
     if (VER == VER_CDIFF) return ver_cdiff(col, j, fin);
     return ver_ppm(col, j, dt, fin);
 }
 
 template <typename C, typename S, typename M, int KB>
 __device__ __forceinline__ Col<C, S, M, KB> make_col(const S *t, const M *w, const MeshCol<M> &m, int n,
-                                                     int lane, int nzmax) {
+                                                     int lane, int nzmax, int top = 1) {
     Col<C, S, M, KB> c;
     c.t = t;
     c.w = w;
@@ -317,7 +318,7 @@ __device__ __forceinline__ Col<C, S, M, KB> make_col(const S *t, const M *w, con
     c.Np = m.Np;
     c.n = n;
     c.lane = lane;
-    c.nzmin = 1;
+    c.nzmin = top;
     c.nzmax = nzmax;
     return c;
 }

@@ -193,14 +193,18 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
     const int n1 = q.x, n2 = q.y, el1 = q.z, el2 = q.w;
     const int nl1 = cld(m.nlev_e, el1) - 1;
     const int nl2 = el2 >= 0 ? cld(m.nlev_e, el2) - 1 : 0;
+    // the edge's levels: from the shallower element top (below a cavity roof) to the deeper bottom
+    const int etop = el2 >= 0 ? min(elem_top(m, el1), elem_top(m, el2)) : elem_top(m, el1);
     if constexpr (TILED) {
-        nz0 = et.nz_b;
+        nz0 = max(et.nz_b, etop);
         nz1 = min(et.nz_e, max(nl1, nl2));
         act = act && nz0 <= nz1;
         lv0 = et.nz_b;
         lv1 = et.nz_e;
     } else {
         if (!chunk_layers(max(nl1, nl2), lc, nz0, nz1)) return;
+        nz0 = max(nz0, etop);
+        if (nz0 > nz1) return;
         lv0 = nz0;
         lv1 = nz1;
     }
@@ -208,7 +212,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
     const LaneLd<KB, C, S> ldT{p.T, lane}, ldA{p.TAB, lane};
     const bool same = TILED ? SAME : p.TAB == p.T;
     C dx = 0, dy = 0, a = 0;
-    int shared_end = 0, u[3] = {0, 0, 0}, d[3] = {0, 0, 0}, nb1 = 0, nb2 = 0;
+    int shared_beg = 1, shared_end = 0, u[3] = {0, 0, 0}, d[3] = {0, 0, 0}, nb1 = 0, nb2 = 0;
     C gu[6], gd[6];
     if (HOR != HOR_UPW1) {
         dx = (C)cld(m.edxdy, 2 * (size_t)e);
@@ -217,7 +221,10 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
         if (el2 >= 0) a = C(0.5) * (a + (C)kREarth * (C)cld(m.elem_cos, el2));
         const int2 ud = cld(m.updn, e);
         if (ud.x >= 0 && ud.y >= 0) {
+            // levels where every element around both ends is wet take the up/down-wind
+            // triangles' gradients (lib/oce_muscl_adv.F90:414-441)
             shared_end = min(cld(m.nlev_min, n1), cld(m.nlev_min, n2)) - 1;
+            if (m.ulev_max) shared_beg = max(cld(m.ulev_max, n1), cld(m.ulev_max, n2));
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 u[c] = cld(m.enodes, (size_t)ud.x * 3 + c);
@@ -278,7 +285,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
             f.t[0] = ldT(o + n1);
             f.t[1] = ldT(o + n2);
         }
-        if (HOR != HOR_UPW1 && nz <= shared_end) {
+        if (HOR != HOR_UPW1 && nz >= shared_beg && nz <= shared_end) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 f.a[2 + c] = ldA(o + u[c]);
@@ -289,7 +296,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
     // the edge flux of level nz (staged in LDS when TILED; qcur is its transport; pf: the level's
     // prefetched values in the untiled prefetch form)
     auto compute = [&](int nz, C qcur, const EdgePf<C> *pf) {
-        if (TILED && (!act || nz > nz1)) return;
+        if (TILED && (!act || nz > nz1 || nz < nz0)) return;
         const int buf = nz % NB;
         const int k = nz - 1;
         const C v = TILED ? qcur : (UPF ? pf->q : (C)cld(m.q, (size_t)k * Ep + e));
@@ -306,7 +313,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
         const C a2 = same ? t2 : (TILED ? lsA(slot(1)) : (UPF ? pf->a[1] : ldA(o + n2)));
         C gr[4] = {0, 0, 0, 0}, c1 = 1, c2 = 1;
         if (HOR != HOR_UPW1) {
-            if (nz <= shared_end) {
+            if (nz >= shared_beg && nz <= shared_end) {
                 if constexpr (TILED) {
                     gr[0] = tr_xy<C>(gu, lsA, 0, slot(2), slot(3), slot(4));
                     gr[1] = tr_xy<C>(gd, lsA, 0, slot(5), slot(6), slot(7));
@@ -397,9 +404,9 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
 }
 
 // Edge transport of the current velocity, q(k, e) = sum over the edge's elements wet at level k
-// of (+-) (u dy - v dx) helem (lib/oce_adv_tra_hor.F90:166-253, segments C, D and E), in double,
-// once per step for all tracers.  One thread per edge walks its levels; levels below both
-// elements are not written (never read).
+// of (+-) (u dy - v dx) helem (lib/oce_adv_tra_hor.F90:166-253, segments A to E: below a cavity
+// roof only the element already wet contributes), in double, once per step for all tracers.  One
+// thread per edge walks its levels; levels where neither element is wet are not read.
 template <typename M>
 __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
     const int e = column_index();
@@ -408,6 +415,7 @@ __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
     const int el1 = q.z, el2 = q.w;
     const int nl1 = cld(m.nlev_e, el1) - 1;
     const int nl2 = el2 >= 0 ? cld(m.nlev_e, el2) - 1 : 0;
+    const int nu1 = elem_top(m, el1), nu2 = el2 >= 0 ? elem_top(m, el2) : 1;
     const double cd0 = cld(m.ecd, 4 * (size_t)e), cd1 = cld(m.ecd, 4 * (size_t)e + 1);
     const double cd2 = cld(m.ecd, 4 * (size_t)e + 2), cd3 = cld(m.ecd, 4 * (size_t)e + 3);
     const size_t Tp = m.Tp, Ep = m.Ep;
@@ -417,7 +425,7 @@ __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
     // is unrolled, so the loads of several levels are in flight together.
     const int e2 = el2 >= 0 ? el2 : el1, w1 = max(nl1, 1), w2 = max(nl2, 1);
     auto level = [&](int nz, double u1, double y1, double h1, double u2, double y2, double h2) {
-        const bool use1 = nz <= nl1, use2 = nz <= nl2;
+        const bool use1 = nz <= nl1 && nz >= nu1, use2 = nz <= nl2 && nz >= nu2;
         const double v1 = (-y1 * cd0 + u1 * cd1) * h1;
         const double v2 = (y2 * cd2 - u2 * cd3) * h2;
         m.q[(size_t)(nz - 1) * Ep + e] = (use1 && use2) ? v1 + v2 : (use1 ? v1 : v2);
@@ -463,14 +471,27 @@ __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
 #ifndef FESOMTRA_DEG_CACHE_WIDE
 #define FESOMTRA_DEG_CACHE_WIDE 6
 #endif
+// An entry's level range (the edge's scatter range) is kept as bottom | top << 8 (levels < 256,
+// checked at setup); the top is 1 but below ice-shelf cavities.
 template <bool PACK, int NC>
 struct NodeEdges {
     static constexpr int kN = NC;
     int b, deg;
     int e2[NC], top[NC], oth[PACK ? 1 : NC];
-    __device__ __forceinline__ int top_of(int i) const { return PACK ? (top[i] & 0xffff) : top[i]; }
+    __device__ __forceinline__ int top_of(int i) const { return top[i] & 0xff; }
+    __device__ __forceinline__ int utop_of(int i) const { return (top[i] >> 8) & 0xff; }
+    __device__ __forceinline__ bool on(int i, int nz) const { return nz <= top_of(i) && nz >= utop_of(i); }
     __device__ __forceinline__ int far_of(int i) const { return PACK ? (top[i] >> 16) : oth[i]; }
 };
+template <typename M>
+__device__ __forceinline__ int entry_utop(const MeshCol<M> &m, int i) {
+    return m.ne_utop ? cld(m.ne_utop, i) : 1;
+}
+// Entry i of the CSR lists is active at level nz.
+template <typename M>
+__device__ __forceinline__ bool entry_on(const MeshCol<M> &m, int i, int nz) {
+    return nz <= cld(m.ne_top, i) && nz >= entry_utop(m, i);
+}
 template <bool PACK, int NC, typename M>
 __device__ __forceinline__ void load_edges(const MeshCol<M> &m, int n, NodeEdges<PACK, NC> &ne) {
     ne.b = cld(m.ne_ptr, n);
@@ -479,10 +500,11 @@ __device__ __forceinline__ void load_edges(const MeshCol<M> &m, int n, NodeEdges
     for (int i = 0; i < NC; ++i) {
         const bool on = i < ne.deg;
         ne.e2[i] = on ? cld(m.ne_e2, ne.b + i) : 0;
+        const int lv = on ? cld(m.ne_top, ne.b + i) | (entry_utop(m, ne.b + i) << 8) : 0;
         if constexpr (PACK) {
-            ne.top[i] = on ? (cld(m.ht_loc, ne.b + i) << 16) | cld(m.ne_top, ne.b + i) : 0;
+            ne.top[i] = on ? (cld(m.ht_loc, ne.b + i) << 16) | lv : 0;
         } else {
-            ne.top[i] = on ? cld(m.ne_top, ne.b + i) : 0;
+            ne.top[i] = lv;
             ne.oth[i] = on ? cld(m.ne_other, ne.b + i) : 0;
         }
     }
@@ -500,10 +522,10 @@ template <bool PACK, int NC, typename M, typename F>
 __device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges<PACK, NC> &ne, int nz, F &&f) {
 #pragma unroll
     for (int i = 0; i < NC; ++i)
-        if (i < ne.deg && nz <= ne.top_of(i)) f(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
+        if (i < ne.deg && ne.on(i, nz)) f(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
     for (int i = NC; i < ne.deg; ++i) {
         const int e2 = cld(m.ne_e2, ne.b + i);
-        if (nz <= cld(m.ne_top, ne.b + i))
+        if (entry_on(m, ne.b + i, nz))
             f(e2 >> 1, (e2 & 1) != 0, cld(PACK ? m.ht_loc : m.ne_other, ne.b + i));
     }
 }
@@ -533,17 +555,17 @@ template <bool PACK, int NC, typename LOAD, typename V>
 __device__ __forceinline__ void edges_pre(const NodeEdges<PACK, NC> &ne, int nz, LOAD &&load, V (&v)[NC]) {
 #pragma unroll
     for (int i = 0; i < NC; ++i)
-        if (i < ne.deg && nz <= ne.top_of(i)) v[i] = load(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
+        if (i < ne.deg && ne.on(i, nz)) v[i] = load(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
 }
 template <bool PACK, int NC, typename M, typename V, typename LOAD, typename USE>
 __device__ __forceinline__ void edges_use(const MeshCol<M> &m, const NodeEdges<PACK, NC> &ne, int nz, const V (&v)[NC],
                                           LOAD &&load, USE &&use) {
 #pragma unroll
     for (int i = 0; i < NC; ++i)
-        if (i < ne.deg && nz <= ne.top_of(i)) use(v[i], ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
+        if (i < ne.deg && ne.on(i, nz)) use(v[i], ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
     for (int i = NC; i < ne.deg; ++i) {
         const int e2 = cld(m.ne_e2, ne.b + i);
-        if (nz <= cld(m.ne_top, ne.b + i)) {
+        if (entry_on(m, ne.b + i, nz)) {
             const int far = cld(PACK ? m.ht_loc : m.ne_other, ne.b + i);
             use(load(e2 >> 1, (e2 & 1) != 0, far), e2 >> 1, (e2 & 1) != 0, far);
         }
@@ -575,7 +597,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
     using D = double;
     __shared__ D sT[TILED ? 2 : 1][TILED ? kHtSlots : 1];
     HaloTile<KB, M> ht;
-    int n, lane, nzmax, L, nz0, nz1, lv0, lv1;
+    int n, lane, nzmax, L, top, nz0, nz1, lv0, lv1;
     bool act = true;
     if constexpr (TILED) {
         if (!ht.init(m, lc)) return;
@@ -583,8 +605,9 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         lane = (int)threadIdx.x % KB;
         act = n < m.N && lane_on(p.mask, lane);
         nzmax = act ? cld(m.nlev_n, n) : 1;
+        top = act ? node_top(m, n) : 1;
         L = nzmax - 1;
-        nz0 = ht.nz_b;
+        nz0 = max(ht.nz_b, top);
         nz1 = min(ht.nz_e, L);
         act = act && nz0 <= nz1;
         lv0 = ht.nz_b;
@@ -593,8 +616,11 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         column_lane<KB>(n, lane);
         if (n >= m.N || !lane_on(p.mask, lane)) return;
         nzmax = cld(m.nlev_n, n);
+        top = node_top(m, n);
         L = nzmax - 1;
         if (!chunk_layers(L, lc, nz0, nz1)) return;
+        nz0 = max(nz0, top);
+        if (nz0 > nz1) return;
         lv0 = nz0;
         lv1 = nz1;
     }
@@ -605,7 +631,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
     const int nc_n = act ? n : 0;
     const double dt = node_dt<NDT>(m, nc_n, dt_step);
     const NodeCol<D, S, M, KB> nc{Np, nc_n, lane, L};
-    const Col<D, S, M, KB> col = make_col<D, S, M, KB>(p.TAB, m.w, m, nc_n, lane, nzmax);
+    const Col<D, S, M, KB> col = make_col<D, S, M, KB>(p.TAB, m.w, m, nc_n, lane, nzmax, top);
     const D pv = (D)num_ord;
     D tw[4], z[4], lo_top = 0, adf_top = 0, tcur = 0;
     QuotWin<D> qw;
@@ -619,8 +645,8 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         for (int i = 0; i < 4; ++i) tw[i] = nc.lay(p.TAB, j - 2 + i);
         tcur = nc.lay(p.T, j);
         const D tup = nc.lay(p.T, j - 1);
-        lo_top = ver_upw1_w<D>(j, nzmax, tup, tcur, wej, aj, D(0));
-        const D fin_top = wsplit ? ver_upw1_w<D>(j, nzmax, tup, tcur, wj, aj, D(0)) : lo_top;
+        lo_top = ver_upw1_w<D>(j, top, nzmax, tup, tcur, wej, aj, D(0));
+        const D fin_top = wsplit ? ver_upw1_w<D>(j, top, nzmax, tup, tcur, wj, aj, D(0)) : lo_top;
         if (VER == VER_QR4C) qw.init(tw, z);
         adf_top = ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, fin_top, qw.q);
     }
@@ -633,7 +659,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
     auto stage = [&](int nz) { ht.each([&](int i, int) { sT[nz & 1][threadIdx.x + 256 * i] = st[i]; }); };
     // layer nz of this column (its neighbours' values staged in sT[nz & 1] when TILED)
     auto body = [&](int nz) {
-        if (TILED && (!act || nz > nz1)) return;
+        if (TILED && (!act || nz > nz1 || nz < nz0)) return;
         const int buf = nz & 1;
         const int j = nz + 1;  // bottom interface of layer nz
         if (VER == VER_QR4C) {
@@ -654,8 +680,8 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         tw[3] = nc.lay(p.TAB, j + 1);
         if (VER == VER_QR4C) qw.slide(tw, z);
         const D tnext = nc.lay(p.T, j);
-        const D lo_bot = ver_upw1_w<D>(j, nzmax, tcur, tnext, wej, aj, D(0));
-        const D fin_bot = wsplit ? ver_upw1_w<D>(j, nzmax, tcur, tnext, wj, aj, D(0)) : lo_bot;
+        const D lo_bot = ver_upw1_w<D>(j, top, nzmax, tcur, tnext, wej, aj, D(0));
+        const D fin_bot = wsplit ? ver_upw1_w<D>(j, top, nzmax, tcur, tnext, wj, aj, D(0)) : lo_bot;
         const D adf_bot = ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, fin_bot, qw.q);
         // upwind edge fluxes (lib/oce_adv_tra_hor.F90:178-181) from q and the two end values,
         // summed in reference edge order
@@ -682,6 +708,11 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         // FP64 stores LO; single precision stores the increment LO - T, so that low-order changes
         // far below the tracer's own magnitude survive the rounding to C
         p.LO[cell * KB + lane] = std::is_same<C, double>::value ? (C)lo : (C)(lo - (double)tcur);
+        if constexpr (!std::is_same<C, double>::value)
+            if (p.LOe) {  // compensated storage: what the single-precision increment lost
+                const double inc = lo - (double)tcur;
+                p.LOe[cell * KB + lane] = (C)(inc - (double)(C)inc);
+            }
         p.adfv[cell * KB + lane] = (C)adf_top;
         if (bottom) p.adfv[(cell + Np) * KB + lane] = (C)adf_bot;
         lo_top = lo_bot;
@@ -713,27 +744,33 @@ __global__ __launch_bounds__(256) void k_vimpl(const MeshCol<M> m, const StepPtr
     column_lane<KB>(n, lane);
     if (n >= m.N || !lane_on(p.mask, lane)) return;
     const double dt = node_dt<NDT>(m, n, dt_step);
-    const int L = cld(m.nlev_n, n) - 1;
-    if (L < 2) return;
+    const int L = cld(m.nlev_n, n) - 1, top = node_top(m, n);
+    if (L - top < 1) return;
     const size_t Np = m.Np;
     auto at = [&](int nz) { return ((size_t)(nz - 1) * Np + n) * KB + lane; };
     // LO in double (single precision stores the increment LO - T)
     auto lo_at = [&](int nz) -> double {
         if constexpr (std::is_same<C, double>::value) return p.LO[at(nz)];
-        else return ldc<double>(p.T, at(nz)) + (double)p.LO[at(nz)];
+        else return ldc<double>(p.T, at(nz)) + ((double)p.LO[at(nz)] + (p.LOe ? (double)p.LOe[at(nz)] : 0.0));
     };
     auto lo_put = [&](int nz, double v) {
-        if constexpr (std::is_same<C, double>::value) p.LO[at(nz)] = v;
-        else p.LO[at(nz)] = (C)(v - ldc<double>(p.T, at(nz)));
+        if constexpr (std::is_same<C, double>::value) {
+            p.LO[at(nz)] = v;
+        } else {
+            const double inc = v - ldc<double>(p.T, at(nz));
+            p.LO[at(nz)] = (C)inc;
+            if (p.LOe) p.LOe[at(nz)] = (C)(inc - (double)(C)inc);
+        }
     };
     const double zinv = 1.0 * dt;
-    double t0 = 0.0, t1 = lo_at(1), t2 = lo_at(2), cpp = 0.0, tpp = 0.0;
-    double w1 = (double)cld(m.wi, (size_t)n), a1 = (double)cld(m.area, (size_t)n);
-    for (int nz = 1; nz <= L; ++nz) {
+    double t0 = 0.0, t1 = lo_at(top), t2 = lo_at(top + 1), cpp = 0.0, tpp = 0.0;
+    const size_t c0 = (size_t)(top - 1) * Np + n;  // the column's top interface
+    double w1 = (double)cld(m.wi, c0), a1 = (double)cld(m.area, c0);
+    for (int nz = top; nz <= L; ++nz) {
         const size_t cell = (size_t)(nz - 1) * Np + n;
         const double hn = (double)cld(m.hnode_new, cell), V = (double)cld(m.areasvol, cell);
         double a, b, c, v = zinv * a1 / V;
-        if (nz == 1) {
+        if (nz == top) {
             a = 0.0;
             b = hn + w1 * v;
         } else {
@@ -751,11 +788,11 @@ __global__ __launch_bounds__(256) void k_vimpl(const MeshCol<M> m, const StepPtr
             c = 0.0;
         }
         double tr;
-        if (nz == 1) tr = -(b - hn) * t1 - c * t2;
+        if (nz == top) tr = -(b - hn) * t1 - c * t2;
         else if (nz < L) tr = -a * t0 - (b - hn) * t1 - c * t2;
         else tr = -a * t0 - (b - hn) * t1;
         double cp, tp;
-        if (nz == 1) {
+        if (nz == top) {
             cp = c / b;
             tp = tr / b;
         } else {
@@ -774,7 +811,7 @@ __global__ __launch_bounds__(256) void k_vimpl(const MeshCol<M> m, const StepPtr
     }
     double x = tpp;
     lo_put(L, t0 + x);
-    for (int nz = L - 1; nz >= 1; --nz) {
+    for (int nz = L - 1; nz >= top; --nz) {
         const double2 ct = p.tri[at(nz)];
         x = ct.y - ct.x * x;
         lo_put(nz, lo_at(nz) + x);
@@ -790,12 +827,14 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_vert(const MeshCol<M> m,
     column_lane<KB>(n, lane);
     if (n >= m.N || !lane_on(p.mask, lane)) return;
     const double dt = node_dt<NDT>(m, n, dt_step);
-    const int nzmax = cld(m.nlev_n, n), L = nzmax - 1;
+    const int nzmax = cld(m.nlev_n, n), L = nzmax - 1, top = node_top(m, n);
     int nz0, nz1;
     if (!chunk_layers(L, lc, nz0, nz1)) return;
+    nz0 = max(nz0, top);
+    if (nz0 > nz1) return;
     const size_t Np = m.Np;
     const NodeCol<D, S, M, KB> nc{Np, n, lane, L};
-    const Col<D, S, M, KB> col = make_col<D, S, M, KB>(p.TAB, m.we, m, n, lane, nzmax);
+    const Col<D, S, M, KB> col = make_col<D, S, M, KB>(p.TAB, m.we, m, n, lane, nzmax, top);
     const D pv = (D)num_ord;
     D tw[4], z[4];
 #pragma unroll
@@ -829,6 +868,66 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_vert(const MeshCol<M> m,
 }
 
 // ---------------------------------------------------------------------------------------------
+// Cavity meshes: the values the limiter's cluster bound finds above an element's top.  The
+// reference keeps the element bounds in the edge-gradient work array (lib/oce_adv_tra_fct.F90:77,
+// 129-147) and writes them only from the element's top down, while the cluster bound of a node
+// (:164-184) reads every element around it at every level of the node: above the element's top it
+// finds the gradient fill (lib/oce_muscl_adv.F90:361-530) of the edge that has the element's
+// number, components 1 and 2 (the x gradients at the edge's first and second node), where that
+// fill wrote the level, and zero elsewhere.  One thread per (slot, lane) forms them from valuesAB
+// for the levels above the slot element's top, with the fill's operation order; fill = 0 (a pass
+// whose gradients were never filled) stores zeros.
+template <int KB, typename S, typename M, typename C>
+__global__ __launch_bounds__(256) void k_cavity_aux(const MeshCol<M> m, const StepPtrs<S, C> p, int fill) {
+    const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int sl = g / KB, lane = g % KB;
+    if (sl >= m.cq_n || !lane_on(p.mask, lane)) return;
+    const int rows = cld(m.cq_top, sl) - 1;
+    const size_t off = (size_t)cld(m.cq_off, sl);
+    auto put = [&](int nz, int comp, C v) { p.cqv[((off + (size_t)(nz - 1)) * 2 + comp) * KB + lane] = v; };
+    if (!fill) {
+        for (int nz = 1; nz <= rows; ++nz) {
+            put(nz, 0, C(0));
+            put(nz, 1, C(0));
+        }
+        return;
+    }
+    const int e = cld(m.cq_edge, sl);
+    const int4 q = cld(m.edge_ne, e);
+    const int2 ud = cld(m.updn, e);
+    const bool both = ud.x >= 0 && ud.y >= 0;
+    int nzmin = 1, nzmax = 0;  // the levels that take the up/down-wind triangles' gradients
+    if (both) {
+        nzmin = max(cld(m.ulev_max, q.x), cld(m.ulev_max, q.y));
+        nzmax = min(cld(m.nlev_min, q.x), cld(m.nlev_min, q.y));
+    }
+    const LaneLd<KB, C, S> ldA{p.TAB, lane};
+    for (int nz = 1; nz <= rows; ++nz) {
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const int node = side ? q.y : q.x, tri = side ? ud.y : ud.x;
+            const int nb = cld(m.nlev_n, node) - 1;
+            const int last = both ? max(nzmin - 1, nb) : nb;  // the fill's loops write up to here
+            C v = C(0);
+            if (nz >= node_top(m, node) && nz <= last) {
+                if (both && nz >= nzmin && nz <= nzmax - 1) {
+                    C gc[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) gc[c] = (C)cld(m.gsca, (size_t)tri * 6 + c);
+                    v = tr_xy<C>(gc, ldA, (size_t)(nz - 1) * m.Np, cld(m.enodes, (size_t)tri * 3),
+                                 cld(m.enodes, (size_t)tri * 3 + 1), cld(m.enodes, (size_t)tri * 3 + 2));
+                } else {
+                    C gx, gy;
+                    node_mean_grad<C>(m, ldA, node, nz, gx, gy);
+                    v = gx;
+                }
+            }
+            put(nz, side, v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // FCT bounds and limiter ratios (lib/oce_adv_tra_fct.F90:96-374).  The cluster of node n at
 // level nz is the set of nodes of the elements around n that are wet at nz; with complete
 // nod_in_elem2D that is n plus every neighbour m whose edge (n, m) reaches level nz, and an
@@ -847,7 +946,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
     using D = double;
     __shared__ D shi[TILED ? 2 : 1][TILED ? kHtSlots : 1], slo[TILED ? 2 : 1][TILED ? kHtSlots : 1];
     HaloTile<KB, M> ht;
-    int n, lane, L, nz0, nz1, jb, je, js;  // sweep j = jb..je; levels up to js are staged
+    int n, lane, L, top, nz0, nz1, jb, je, js;  // sweep j = jb..je; levels up to js are staged
     bool act = true;
     if constexpr (TILED) {
         if (!ht.init(m, lc)) return;
@@ -855,7 +954,8 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         lane = (int)threadIdx.x % KB;
         act = n < m.N && lane_on(p.mask, lane);
         L = act ? cld(m.nlev_n, n) - 1 : 0;
-        nz0 = ht.nz_b;
+        top = act ? node_top(m, n) : 1;
+        nz0 = max(ht.nz_b, top);
         nz1 = min(ht.nz_e, L);
         act = act && nz0 <= nz1;
         jb = max(1, ht.nz_b - 1);
@@ -865,8 +965,11 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         column_lane<KB>(n, lane);
         if (n >= m.N || !lane_on(p.mask, lane)) return;
         L = cld(m.nlev_n, n) - 1;
+        top = node_top(m, n);
         if (!chunk_layers(L, lc, nz0, nz1)) return;
-        jb = max(1, nz0 - 1);
+        nz0 = max(nz0, top);
+        if (nz0 > nz1) return;
+        jb = max(top, nz0 - 1);
         je = nz1 + 1;
         js = 0;
     }
@@ -886,6 +989,19 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
     // hardware max / min (one instruction; equal but for the choice between signed zeros)
     auto bmax = [](D a, D b) { return std::is_same<C, double>::value ? cmax(a, b) : fmax(a, b); };
     auto bmin = [](D a, D b) { return std::is_same<C, double>::value ? cmin(a, b) : fmin(a, b); };
+    // Elements around n whose top lies below level nz contribute the values the reference finds in
+    // its shared work array there (m.cq_*, p.cqv, written by k_cavity_aux before this kernel).
+    auto cavity_bounds = [&](int nz, D &vmax, D &vmin) {
+        if (!m.cq_ptr) return;
+        const int qe = cld(m.cq_ptr, n + 1);
+        for (int i = cld(m.cq_ptr, n); i < qe; ++i) {
+            const int sl = cld(m.cq_ent, i);
+            if (nz >= cld(m.cq_top, sl)) continue;
+            const size_t r = ((size_t)cld(m.cq_off, sl) + (size_t)(nz - 1)) * 2;
+            vmax = bmax(vmax, (D)p.cqv[r * KB + lane]);
+            vmin = bmin(vmin, (D)p.cqv[(r + 1) * KB + lane]);
+        }
+    };
     auto cluster = [&](int nz, int buf, D &vmax, D &vmin, D &own_lo) {
         if constexpr (TILED) {  // own_lo: taken from the staging fetch by the caller
             vmax = shi[buf][threadIdx.x];
@@ -915,6 +1031,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
             vmax = bmax(vmax, D(-1.0e3));
             vmin = bmin(vmin, D(1.0e3));
         }
+        cavity_bounds(nz, vmax, vmin);
     };
     // 64-wide untiled form: the next level's cluster values (raw storage values of the node and
     // of its cached neighbours) are loaded while a level is finished, in two register sets
@@ -956,6 +1073,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
             vmax = bmax(vmax, D(-1.0e3));
             vmin = bmin(vmin, D(1.0e3));
         }
+        cavity_bounds(nz, vmax, vmin);
     };
     // P+/P-, Q+/Q- and the ratios of layer nz with cluster bounds (hi, lw) and low-order
     // value lo; the vertical flux at the layer's bottom is kept for the next layer (av_bot)
@@ -1023,9 +1141,9 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         const bool fin = nz >= nz0 && nz <= nz1;
         Pre pre;
         if (fin) preload(nz, pre);
-        if (j <= L) cluster_pf(j, *cp, mx0, mn0, lo0);
+        if (j <= L && j >= top) cluster_pf(j, *cp, mx0, mn0, lo0);
         if (fin) {
-            const bool own = nz == 1 || nz == L;
+            const bool own = nz == top || nz == L;
             finish(nz, own ? mx1 : bmax(bmax(mx2, mx1), mx0), own ? mn1 : bmin(bmin(mn2, mn1), mn0), lo1, pre);
         }
     };
@@ -1067,12 +1185,12 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         const bool fin = nz >= nz0 && nz <= nz1;
         Pre pre;
         if (fin) preload(nz, pre);
-        if (j <= L) {
+        if (j <= L && j >= top) {
             cluster(j, buf, mx0, mn0, lo0);
             if constexpr (TILED) lo0 = lo_j;
         }
         if (fin) {
-            const bool own = nz == 1 || nz == L;
+            const bool own = nz == top || nz == L;
             finish(nz, own ? mx1 : bmax(bmax(mx2, mx1), mx0), own ? mn1 : bmin(bmin(mn2, mn1), mn0), lo1, pre);
         }
     }
@@ -1107,7 +1225,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
     constexpr bool PRE = kFinalPreFor<TILED, KB, C>;
     __shared__ C sRp[TILED ? 2 : 1][TILED ? kHtSlots : 1], sRm[TILED ? 2 : 1][TILED ? kHtSlots : 1];
     HaloTile<KB, M> ht;
-    int n, lane, L, nz0, nz1, lv0, lv1;
+    int n, lane, L, ctop, nz0, nz1, lv0, lv1;  // ctop: the column's top layer
     bool act = true;
     if constexpr (TILED) {
         if (!ht.init(m, lc)) return;
@@ -1115,7 +1233,8 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
         lane = (int)threadIdx.x % KB;
         act = n < m.N && lane_on(p.mask, lane);
         L = act ? cld(m.nlev_n, n) - 1 : 0;
-        nz0 = ht.nz_b;
+        ctop = act ? node_top(m, n) : 1;
+        nz0 = max(ht.nz_b, ctop);
         nz1 = min(ht.nz_e, L);
         act = act && nz0 <= nz1;
         lv0 = ht.nz_b;
@@ -1124,7 +1243,10 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
         column_lane<KB>(n, lane);
         if (n >= m.N || !lane_on(p.mask, lane)) return;
         L = cld(m.nlev_n, n) - 1;
+        ctop = node_top(m, n);
         if (!chunk_layers(L, lc, nz0, nz1)) return;
+        nz0 = max(nz0, ctop);
+        if (nz0 > nz1) return;
         lv0 = nz0;
         lv1 = nz1;
     }
@@ -1136,7 +1258,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
     // limited flux at interface j given R at layers j-1 (rp0, rm0) and j (rp1, rm1)
     auto limit = [&](C a, int j, C rp0, C rm0, C rp1, C rm1) -> C {
         C ae = 1;
-        if (j == 1) ae = a >= C(0) ? cmin(ae, rp1) : cmin(ae, rm1);
+        if (j == ctop) ae = a >= C(0) ? cmin(ae, rp1) : cmin(ae, rm1);
         else if (a >= C(0)) {
             ae = cmin(ae, rm0);
             ae = cmin(ae, rp1);
@@ -1154,8 +1276,8 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
         if (FCT) {
             rp1 = p.Rp[X(c0)];
             rm1 = p.Rm[X(c0)];
-            const C rpa = nz0 > 1 ? p.Rp[X(c0 - Np)] : C(0);
-            const C rma = nz0 > 1 ? p.Rm[X(c0 - Np)] : C(0);
+            const C rpa = nz0 > ctop ? p.Rp[X(c0 - Np)] : C(0);
+            const C rma = nz0 > ctop ? p.Rm[X(c0 - Np)] : C(0);
             top = limit(a0, nz0, rpa, rma, rp1, rm1);
             if (p.dbg_v) p.dbg_v[c0] = top;
         } else {
@@ -1185,7 +1307,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
                     sm[i] = p.Rm[X(c)];
                 });
             }
-            if (!act || nz > nz1) continue;
+            if (!act || nz > nz1 || nz < nz0) continue;
         }
         const size_t cell = (size_t)(nz - 1) * Np + n;
         const size_t eo = (size_t)(nz - 1) * Ep, no = (size_t)(nz - 1) * Np;
@@ -1215,7 +1337,11 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
         C bot;
         double dv = p.acc_v ? p.acc_v[cell] : 0.0;
         if (FCT) {
-            const double lo = std::is_same<C, double>::value ? (double)p.LO[X(cell)] : tval + (double)p.LO[X(cell)];
+            // single precision: LO = T + increment (+ the increment's remainder with compensated
+            // storage, so that the low-order change of the content is the fluxes' to FP64 round-off)
+            const double lo = std::is_same<C, double>::value
+                                  ? (double)p.LO[X(cell)]
+                                  : tval + ((double)p.LO[X(cell)] + (p.LOe ? (double)p.LOe[X(cell)] : 0.0));
             dv = dv - tval * h + lo * hn;
             rp0 = rp1;
             rm0 = rm1;
@@ -1278,6 +1404,13 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
         if (p.acc_h) {
             p.acc_h[cell] = dh;
             p.acc_v[cell] = dv;
+        } else if (p.resid) {
+            // compensated storage: the value is stored + remainder; the step's increment is added
+            // to both (in double), the sum rounded to storage and the new remainder kept, so the
+            // content sum(T h V) changes by the fluxes only (no storage rounding accumulates)
+            const double full = tval + (double)p.resid[X(cell)] + (dh + dv) / hn;
+            std_(p.T, X(cell), full);
+            p.resid[X(cell)] = (float)(full - ldc<double>(p.T, X(cell)));
         } else if (p.seed != 0u && !std::is_same<S, double>::value) {
             const uint32_t step = p.seed_dev ? *p.seed_dev : p.seed;
             st_stochastic(p.T, X(cell), tval + (dh + dv) / hn,
@@ -1305,14 +1438,21 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
 template <typename S>
 __global__ void k_relayout(const double *__restrict__ ref_in, S *__restrict__ dev_out,
                            const S *__restrict__ dev_in, double *__restrict__ ref_out,
-                           const int *__restrict__ perm, int N, int L, size_t Np, int kb, int lane, int dir) {
+                           const int *__restrict__ perm, int N, int L, size_t Np, int kb, int lane, int dir,
+                           float *__restrict__ res) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)N * L) return;
     const int k = (int)(i / N), n = (int)(i - (size_t)k * N);
     const int o = perm[n];  // reference number of device node n
     const size_t d = ((size_t)k * Np + n) * kb + lane;
-    if (dir == 0) std_(dev_out, d, ref_in[(size_t)o * L + k]);
-    else ref_out[(size_t)o * L + k] = ldc<double>(dev_in, d);
+    // res (compensated storage): the rounding remainder v - stored, in FP32 beside the stored value
+    if (dir == 0) {
+        const double v = ref_in[(size_t)o * L + k];
+        std_(dev_out, d, v);
+        if (res) res[d] = (float)(v - ldc<double>(dev_out, d));
+    } else {
+        ref_out[(size_t)o * L + k] = ldc<double>(dev_in, d) + (res ? (double)res[d] : 0.0);
+    }
 }
 
 // FESOM2's second-order Adams-Bashforth interpolation of a tracer block before a step:
@@ -1333,18 +1473,18 @@ __global__ __launch_bounds__(256) void k_ab2(const S *__restrict__ T, S *__restr
 // Stats over one tracer (lane of a block): per-block partial min / max / sum / volume sum over
 // wet layers (or all layers).
 template <typename S, typename M>
-__global__ __launch_bounds__(256) void k_stats_partial(const MeshCol<M> m, const S *__restrict__ T, int kb,
-                                                       int lane, int all_levels, double4 *__restrict__ part) {
+__global__ __launch_bounds__(256) void k_stats_partial(const MeshCol<M> m, const S *__restrict__ T, const float *__restrict__ res,
+                                                       int kb, int lane, int all_levels, double4 *__restrict__ part) {
     __shared__ double smin[256], smax[256], ssum[256], svol[256];
     double mn = __builtin_huge_val(), mx = -__builtin_huge_val(), sm = 0.0, sv = 0.0;
     const size_t total = (size_t)m.L * m.N;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
         const int k = (int)(i / m.N), n = (int)(i - (size_t)k * m.N);
-        const bool wet = k + 1 <= cld(m.nlev_n, n) - 1;
+        const bool wet = k + 1 <= cld(m.nlev_n, n) - 1 && k + 1 >= node_top(m, n);
         if (!all_levels && !wet) continue;
         const size_t ix = (size_t)k * m.Np + n;
-        const double v = ldc<double>(T, ix * kb + lane);
+        const double v = ldc<double>(T, ix * kb + lane) + (res ? (double)res[ix * kb + lane] : 0.0);
         mn = v < mn ? v : mn;
         mx = v > mx ? v : mx;
         sm += v;
@@ -1466,6 +1606,10 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
                     k_vimpl<KB, S, M, C, NDT><<<dim3(grid_for((size_t)m.N * KB, 256)), 256, 0, s>>>(m, p, sc.dt);
                     LAUNCH_CHECK();
                 }
+                if (m.cq_n > 0) {
+                    k_cavity_aux<KB><<<dim3(grid_for((size_t)m.cq_n * KB, 256)), 256, 0, s>>>(m, p, sc.cavity_fill ? 1 : 0);
+                    LAUNCH_CHECK();
+                }
                 MARK(2);
                 k_bounds<true, KB, S, M, C, NDT><<<tgrid(cfg.lc_bounds), 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
                 LAUNCH_CHECK();
@@ -1485,6 +1629,10 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
             if (sc.wsplit) {
                 if (!p.tri) return hipErrorInvalidValue;
                 k_vimpl<KB, S, M, C, NDT><<<dim3(grid_for((size_t)m.N * KB, 256)), 256, 0, s>>>(m, p, sc.dt);
+                LAUNCH_CHECK();
+            }
+            if (m.cq_n > 0) {
+                k_cavity_aux<KB><<<dim3(grid_for((size_t)m.cq_n * KB, 256)), 256, 0, s>>>(m, p, sc.cavity_fill ? 1 : 0);
                 LAUNCH_CHECK();
             }
             MARK(2);
@@ -1538,6 +1686,7 @@ hipError_t launch_step(const MeshCol<M> &m, const StepPtrs<S, C> &p, int kb, con
                        const LaunchCfg &cfg, hipStream_t s, hipEvent_t *ev) {
     if (cfg.lc_edge < 1 || cfg.lc_lo < 1 || cfg.lc_bounds < 1 || cfg.lc_final < 1) return hipErrorInvalidValue;
     if (kb < 64 && (p.mask >> kb) != 0ull) return hipErrorInvalidValue;  // lanes beyond the block
+    if (m.cq_n > 0 && sc.fct && !p.cqv) return hipErrorInvalidValue;    // cavity bounds need their rows
     // a per-node time step (m.dtn) selects kernels that load it; the others keep dt scalar
     const bool ndt = m.dtn != nullptr;
     switch (kb) {
@@ -1554,12 +1703,13 @@ hipError_t launch_step(const MeshCol<M> &m, const StepPtrs<S, C> &p, int kb, con
 
 template <typename S>
 hipError_t launch_relayout(const double *ref_in, S *dev_out, const S *dev_in, double *ref_out,
-                           const int *perm, int N, int L, size_t Np, int kb, int lane, int dir, hipStream_t s) {
+                           const int *perm, int N, int L, size_t Np, int kb, int lane, int dir, hipStream_t s,
+                           float *res) {
     const size_t total = (size_t)N * L;
     if (total == 0) return hipSuccess;
     if (lane < 0 || lane >= kb) return hipErrorInvalidValue;
     k_relayout<S><<<dim3((unsigned)((total + 255) / 256)), 256, 0, s>>>(ref_in, dev_out, dev_in, ref_out,
-                                                                        perm, N, L, Np, kb, lane, dir);
+                                                                        perm, N, L, Np, kb, lane, dir, res);
     return hipGetLastError();
 }
 
@@ -1573,9 +1723,9 @@ hipError_t launch_ab2(const S *T, S *AB, S *old, size_t n, double eps, hipStream
 
 template <typename S, typename M>
 hipError_t launch_stats(const MeshCol<M> &m, const S *T, int kb, int lane, int all_levels, double4 *part,
-                        int nblocks, hipStream_t s) {
+                        int nblocks, hipStream_t s, const float *res) {
     if (lane < 0 || lane >= kb) return hipErrorInvalidValue;
-    k_stats_partial<S, M><<<nblocks, 256, 0, s>>>(m, T, kb, lane, all_levels, part);
+    k_stats_partial<S, M><<<nblocks, 256, 0, s>>>(m, T, res, kb, lane, all_levels, part);
     return hipGetLastError();
 }
 
@@ -1591,9 +1741,9 @@ hipError_t launch_stats(const MeshCol<M> &m, const S *T, int kb, int lane, int a
                                                 const SchemeArgs &, const LaunchCfg &, hipStream_t,         \
                                                 hipEvent_t *);                                              \
     template hipError_t launch_relayout<S_>(const double *, S_ *, const S_ *, double *, const int *, int, \
-                                            int, size_t, int, int, int, hipStream_t);                      \
+                                            int, size_t, int, int, int, hipStream_t, float *);             \
     template hipError_t launch_stats<S_, M_>(const MeshCol<M_> &, const S_ *, int, int, int, double4 *,    \
-                                             int, hipStream_t);                                              \
+                                             int, hipStream_t, const float *);                               \
     template hipError_t launch_ab2<S_>(const S_ *, S_ *, S_ *, size_t, double, hipStream_t);
 #if FESOMTRA_INST & 1
 FESOMTRA_INSTANTIATE(double, double, double)

@@ -10,6 +10,7 @@ struct SchemeArgs {
     bool fct;             // limiter "FCT" (else "NONE")
     double ph, pv, dt;    // tra_adv_ph, tra_adv_pv, time step
     bool wsplit = false;  // dynamics%use_wsplit (FCT only): implicit low-order part with w_i
+    bool cavity_fill = false;  // cavity meshes: the edge gradients were filled for this tracer
 };
 
 // Levels per thread of each kernel (the column of a node / edge is split into chunks of lc
@@ -34,9 +35,12 @@ hipError_t launch_step(const MeshCol<M> &m, const StepPtrs<S, C> &p, int kb, con
                        const LaunchCfg &cfg, hipStream_t s, hipEvent_t *ev);
 
 // dir 0: reference layout (nl-1, nod2D) float64 -> lane of a device tracer block; dir 1: back.
+// res (compensated storage, FP32, or null): dir 0 also stores the rounding remainder there, dir 1
+// adds it.
 template <typename S>
 hipError_t launch_relayout(const double *ref_in, S *dev_out, const S *dev_in, double *ref_out,
-                           const int *perm, int N, int L, size_t Np, int kb, int lane, int dir, hipStream_t s);
+                           const int *perm, int N, int L, size_t Np, int kb, int lane, int dir, hipStream_t s,
+                           float *res = nullptr);
 
 // valuesAB = -(0.5+eps)*valuesold + (1.5+eps)*values and valuesold := values over n entries of a
 // tracer block (FESOM2's AB2 interpolation).
@@ -45,6 +49,6 @@ hipError_t launch_ab2(const S *T, S *AB, S *old, size_t n, double eps, hipStream
 
 template <typename S, typename M>
 hipError_t launch_stats(const MeshCol<M> &m, const S *T, int kb, int lane, int all_levels, double4 *part,
-                        int nblocks, hipStream_t s);
+                        int nblocks, hipStream_t s, const float *res = nullptr);
 
 }  // namespace fesomtra

@@ -139,12 +139,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(const MeshCol<double> m, 
                     const D zb1 = (VER == VER_QR4C) ? nc.itf(m.zbar, 1) : D(0);
                     const D t1w[4] = {tw[0], tw[1], tw[2], tw[3]};
                     const D z1w[4] = {D(0), z[0], z[1], z[2]};
-                    lo_top = ver_upw1_w<D>(1, nzmax, tw[1], tw[2], we1, a1, D(0));
+                    lo_top = ver_upw1_w<D>(1, 1, nzmax, tw[1], tw[2], we1, a1, D(0));
                     av[2] = ver_ho_at<VER>(1, nzmax, t1w, z1w, zb1, w1, a1, pv, dt, col, lo_top);
                 }
                 // window of interface j: T(j-2 .. j+1) = T(a .. a+3) = tw[1..4]
                 const D tj[4] = {tw[1], tw[2], tw[3], tw[4]};
-                const D lo_bot = ver_upw1_w<D>(j, nzmax, tw[2], tw[3], wej, aj, D(0));
+                const D lo_bot = ver_upw1_w<D>(j, 1, nzmax, tw[2], tw[3], wej, aj, D(0));
                 av[3] = ver_ho_at<VER>(j, nzmax, tj, z, zbj, wj, aj, pv, dt, col, lo_bot);
                 const size_t cell = (size_t)a * Np + n;  // layer a+1
                 const double h = cld(m.hnode, cell), hn = cld(m.hnode_new, cell), V = cld(m.areasvol, cell);

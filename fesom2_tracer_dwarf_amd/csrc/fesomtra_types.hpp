@@ -135,7 +135,6 @@ template <typename C>
 __device__ __forceinline__ C csign1(C x) { return signbit(x) ? C(-1) : C(1); }
 
 // Mesh arrays on the device (device numbering; levels are 1-based in the formulas).
-// ulevels == 1 everywhere (cavities are rejected at setup).
 template <typename M>
 struct MeshCol {
     int N, T, E, nl, L, Np, Tp, Ep, max_adj;
@@ -172,7 +171,32 @@ struct MeshCol {
     const M *edxdy;                       // (E, 2) edge_dxdy
     double *q;                            // [L][Ep] edge transport of the current velocity
     const double *dtn;                    // (N) time-step factor per node, or null (all 1)
+    // Ice-shelf cavities (layers above a top are not wet); all null without them (tops are 1)
+    const int *ulev_n;                    // (N) ulevels_nod2D, the node's top layer
+    const int *ulev_max;                  // (N) ulevels_nod2D_max, deepest element top around the node
+    const int *ulev_e;                    // (T) ulevels, the element's top layer
+    const int *ne_utop;                   // CSR entries: top level of the edge's scatter range
+    // The limiter's cluster bound reads, for an element whose top lies below the level, the work
+    // array it shares with the edge gradients (lib/oce_adv_tra_fct.F90:77,164-184): at element
+    // number el that holds the gradient fill of the edge with the same reference number.  Slots
+    // are those elements: the device edge with the element's reference number (cq_edge), the
+    // element's top (cq_top) and the offset of its rows, one per level above the top, in the
+    // per-pass values StepPtrs::cqv (cq_off).  Per node, the slots of the elements around it
+    // whose top lies below the node's (cq_ptr, cq_ent).
+    int cq_n;
+    const int *cq_edge, *cq_top, *cq_off, *cq_ptr, *cq_ent;
 };
+
+// Top layer of node n's column (1 without cavities).
+template <typename M>
+__device__ __forceinline__ int node_top(const MeshCol<M> &m, int n) {
+    return m.ulev_n ? cld(m.ulev_n, n) : 1;
+}
+// Top layer of element el.
+template <typename M>
+__device__ __forceinline__ int elem_top(const MeshCol<M> &m, int el) {
+    return m.ulev_e ? cld(m.ulev_e, el) : 1;
+}
 
 // Time step of node n's cells: dt, or dt * dtn[n] where the mesh carries a factor per node (a
 // union of independent meshes, each advanced with its own time step).  NDT: kernels compiled for
@@ -189,14 +213,18 @@ struct StepPtrs {
     S *T;                  // values (updated in place unless accumulating)
     const S *TAB;          // valuesAB (may equal T)
     C *adfh;               // [L][Ep][KB] antidiffusive (or high-order) edge flux
-    C *LO;                 // [L][Np][KB] low-order solution
+    C *LO;                 // [L][Np][KB] low-order solution (single precision: its increment LO - T)
+    C *LOe;                // single precision with compensated storage: the increment's remainder, or null
     C *Rp, *Rm;            // [L][Np][KB] limiter ratios
     C *adfv;               // [nl][Np][KB] antidiffusive (or high-order) vertical flux
     double *acc_h, *acc_v; // do_oce_adv_tra accumulators [L][Np] of the single selected lane, or null
     C *dbg_h, *dbg_v;      // limited fluxes [L][Ep], [nl][Np] of the single selected lane, or null
     double2 *tri;          // [L][Np][KB] (c', t') of the implicit vertical solve, or null
     S *ab_next, *ab_old;   // AB2: valuesAB of the next step and valuesold, written by the update, or null
+    float *resid;          // compensated storage: [L][Np][KB] FP32 remainder of the stored values, or null
     double ab_c0, ab_c1;   // 0.5 + epsilon, 1.5 + epsilon
+    C *cqv;                // cavity meshes: [rows][2][KB] the limiter's element-bound values above an
+                           // element's top (MeshCol::cq_*), written by k_cavity_aux
     unsigned long long mask;  // lanes advanced by this launch (tracers sharing its scheme)
     int tid0;                 // tracer number of lane 0 (seeds the stochastic rounding)
     uint32_t seed;            // nonzero: stochastic rounding of the update; 0: round to nearest

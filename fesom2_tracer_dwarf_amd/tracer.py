@@ -125,6 +125,10 @@ class FESOMTracerAdvection:
         """0 FP64, 1 FP32, 2 FP32 arithmetic + FP16 storage, 3 FP32 arithmetic + BF16 storage."""
         abi.check(self.lib.tracer_set_precision(mode), "tracer_set_precision")
 
+    def set_compensated(self, enable: bool) -> None:
+        """Single precision: keep each stored value's rounding remainder (tracer_set_compensated)."""
+        abi.check(self.lib.tracer_set_compensated(int(bool(enable))), "tracer_set_compensated")
+
     def set_rounding(self, stochastic: bool) -> None:
         """Stochastic (default) or round-to-nearest storage of the update in modes 1-3."""
         abi.check(self.lib.tracer_set_rounding(int(bool(stochastic))), "tracer_set_rounding")

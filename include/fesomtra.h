@@ -219,6 +219,15 @@ int tracer_set_precision(int mode);
  * width), 0 = round to nearest.  Without it, increments smaller than half a unit in the last place
  * are lost at every step and slowly varying fields stagnate. */
 int tracer_set_rounding(int stochastic);
+/* Compensated storage (modes 1-3; enable = 1): next to every stored value the library keeps its
+ * rounding remainder (value - stored, in FP32), and next to the stored low-order increment its own
+ * remainder; the update adds the step's increment to value + remainder in double and keeps the new
+ * remainder, so no storage rounding accumulates: the tracer content sum(T * hnode * areasvol)
+ * changes only by the (conservative) fluxes, to FP64 round-off, and values, statistics and
+ * read-backs are stored value + remainder.  The fluxes are computed from the stored values.  Costs
+ * one more FP32 read and write per node-level and step for the values and one for the increment.
+ * Default 0; values are carried over when it changes. */
+int tracer_set_compensated(int enable);
 /* restore = 0: keeps a device copy of every tracer (in HBM, storage precision) and the step
  * counter of the stochastic rounding; restore = 1: puts both back (valuesAB := values).  Lets a
  * caller time or tune steps without changing later results. */

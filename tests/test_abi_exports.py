@@ -68,21 +68,28 @@ def test_error_paths_without_gpu():
 
 
 def test_setup_validation_without_gpu():
-    """Mesh checks made at tracer_init_complete (before any device work): cavities, node levels
-    below element levels, missing edge_tri(1); option setters reject bad values."""
+    """Mesh checks made at tracer_init_complete (before any device work): inconsistent cavity tops,
+    node levels below element levels, missing edge_tri(1); option setters reject bad values."""
     from fesom2_tracer_dwarf_amd import configs
     from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
     lib = abi.load()
     case = configs.bathymetry_case("v", 6, 8, seed=3, nsteps=1, lev_lo=3)
     m = case.mesh
     t = FESOMTracerAdvection()
-    # cavity: ulevels_nod2D > 1
+    # a node column starting below the tops of its elements (ulevels_nod2D > ulevels)
     t.setup_case(case, upload=False)
     ul = m.ulevels_nod2D.astype(np.int32).copy()
     ul[3] = 2
     assert lib.tracer_set_levels(abi.iptr(m.nlevels_nod2D.astype(np.int32)), abi.iptr(ul), m.nod2D) == 0
     assert lib.tracer_init_complete() == -1
-    assert b"cavities" in lib.tracer_last_error()
+    assert b"starts deeper" in lib.tracer_last_error()
+    # an element whose top leaves it no layer
+    t.setup_case(case, upload=False)
+    eu = m.ulevels.astype(np.int32).copy()
+    eu[2] = m.nlevels[2]
+    assert lib.tracer_set_elem_levels(abi.iptr(m.nlevels.astype(np.int32)), abi.iptr(eu), m.elem2D) == 0
+    assert lib.tracer_init_complete() == -1
+    assert b"no layer" in lib.tracer_last_error()
     # a node shallower than one of its elements
     t.setup_case(case, upload=False)
     nl = m.nlevels_nod2D.astype(np.int32).copy()

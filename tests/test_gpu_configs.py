@@ -31,13 +31,14 @@ def rel_linf(got, ref):
     return float(np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-300))
 
 
-def run_gpu(case, nsteps, prec=0, values=None, stats=False, kb=0):
+def run_gpu(case, nsteps, prec=0, values=None, stats=False, kb=0, compensated=False):
     """nsteps on the GPU; returns the final values (ntr, N, nl-1) and, with stats, the
     volume-weighted sums before and after."""
     from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
     t = FESOMTracerAdvection()
     t.set_precision(prec)
     t.set_block_width(kb)
+    t.set_compensated(compensated)
     t.setup_case(case)
     if values is not None:
         for i in range(case.ntracers):
@@ -167,6 +168,12 @@ def test_c3_200_steps_fp32_within_1e5_of_fp64(c3):
     say(f"  FP32: rel L2 {l2:.3e}, rel Linf {linf:.3e}, volume-sum drift {drift:.3e} (FP64 {drift64:.3e})")
     assert linf <= 1e-5 and l2 <= 2e-6
     assert drift <= 1e-8 and drift64 <= 1e-13
+    # compensated storage (the rounding remainders kept): the sum is conserved to FP64 round-off
+    got, v0, v1 = run_gpu(c3, 200, prec=1, stats=True, compensated=True)
+    l2, linf = errors(got, ref, wet)
+    drift = max(abs(b - a) / abs(a) for a, b in zip(v0, v1))
+    say(f"  FP32 compensated: rel L2 {l2:.3e}, rel Linf {linf:.3e}, volume-sum drift {drift:.3e}")
+    assert linf <= 1e-5 and l2 <= 2e-6 and drift <= 1e-13
 
 
 # ---- C4 ------------------------------------------------------------------------------------------

@@ -149,3 +149,36 @@ def test_union_members_keep_their_own_dt(oracle_mod):
             say(f"  member {c.name} dt={c.dt:.1f}: rel Linf {err:.3e}")
             assert err <= RTOL
             off += c.mesh.nod2D
+
+
+@pytest.mark.parametrize("prec,bound,drift_bound", [(1, 1e-5, 1e-14), (2, 5e-3, 1e-10), (3, 2e-2, 1e-10)])
+def test_compensated_storage_conserves_to_fp64_roundoff(oracle_mod, prec, bound, drift_bound):
+    """Single-precision storage with the rounding remainders kept beside the values and the
+    low-order increments (tracer_set_compensated): with FP32 storage the volume-weighted sum is
+    conserved to FP64 round-off over 100 steps (stochastic rounding drifts by ~1e-9 here); with
+    16-bit storage the FP32 remainder itself rounds at 2^-35 of a value, so the drift stays near
+    1e-11 (stochastic rounding: ~1e-7 to 1e-5).  Values stay close to the FP64 oracle, and a
+    checkpoint / restore and a block-width change keep the remainders."""
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    case = configs.bathymetry_case("bathy14c", 14, 16, seed=61, ntracers=3, nsteps=100, lev_lo=4, w_cfl=0.3)
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[100])
+    t = FESOMTracerAdvection()
+    t.set_precision(prec)
+    t.set_compensated(True)
+    t.setup_case(case)
+    v0 = [t.stats(i + 1)["volume_sum"] for i in range(case.ntracers)]
+    t.advect_steps(40, case.dt)
+    t.checkpoint()
+    t.advect_steps(10, case.dt)
+    t.checkpoint(restore=True)
+    t.set_block_width(1)
+    t.advect_steps(60, case.dt)
+    got = t.get_all_values()
+    v1 = [t.stats(i + 1)["volume_sum"] for i in range(case.ntracers)]
+    t.set_block_width(0)
+    t.cleanup()
+    drift = max(abs(b - a) / abs(a) for a, b in zip(v0, v1))
+    err = rel_linf(got, ref[100])
+    say(f"case bathy14c precision {prec} compensated: volume-sum drift {drift:.3e}, rel Linf vs FP64 {err:.3e}")
+    assert drift <= drift_bound
+    assert err <= bound
