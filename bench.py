@@ -190,7 +190,7 @@ def make_case(name: str, nsteps=None):
 
 def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, lc=None,
         stage_steps: int = 10, compare=None, keep: bool = False, stochastic: bool = True,
-        tune: bool = True, fused: bool = False):
+        tune: bool = True, fused: bool = False, compensated: bool = False):
     """Advance `case` on the GPU; returns throughput, device time, per-stage times, drift.
 
     keep: also return the final values (list per tracer); compare: final values of a reference
@@ -203,17 +203,22 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
     t.set_block_width(kb)
     t.set_graph(graph)
     t.set_rounding(stochastic)
+    t.set_compensated(compensated)
     if lc:
         t.set_launch_config(lc)
     t.setup_case(case)
-    width = t.block_width()
     if fused:
         t.set_fused(True)
-    tuned = None
+    tuned, layout = None, None
     if tune and not lc and not fused:
         from fesom2_tracer_dwarf_amd import autotune
+        if kb == 0:
+            layout = autotune.tune_layout(t, case)
+            log(f"autotuned layout: block width {layout['block_width']}, tiles {layout['tiles']} "
+                f"(ms/step {layout['step_ms']})")
         tuned = autotune.tune(t, case, prec, use_cache=False)
         log(f"autotuned levels per thread {tuned['launch_config']} (stage ms {[round(x, 3) for x in tuned['stage_ms']]})")
+    width = t.block_width()
     v0 = [t.stats(i + 1)["volume_sum"] for i in range(case.ntracers)]
     if warmup:
         t.advect_steps(warmup, case.dt)
@@ -257,7 +262,7 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
     updates = case.ntracers * m.n_node_levels * steps
     return {"value": updates / wall, "wall_s": wall, "dev_ms": dev_ms, "stage_ms": stage_ms,
             "passes_timed": passes, "block_width": width, "drift": drift, "launch_config": cfg,
-            "values": kept, "error": err, "autotune": tuned, "qflux_ms": qflux_ms,
+            "values": kept, "error": err, "autotune": tuned, "layout": layout, "qflux_ms": qflux_ms,
             "step_ms": step_ms, "steps_timed": nsteps_timed}
 
 
@@ -286,7 +291,7 @@ def roofline(case, prec: int, r: dict, name: str):
             "passes_per_step": passes}
 
 
-PROFILE_ARGS = "--steps 5 --warmup 1 --no-cpu-baseline --no-batched --no-autotune"
+PROFILE_ARGS = "--steps 5 --warmup 1 --no-cpu-baseline --no-batched --no-dropin --no-multimesh --no-autotune"
 
 
 def _rocprof_rows(d: Path, suffix: str):
@@ -301,8 +306,9 @@ def _rocprof_rows(d: Path, suffix: str):
 
 
 def _short(name: str) -> str:
-    """fesomtra::k_edge<true, 2, ...>(...) -> fesomtra::k_edge"""
-    return name.split("<")[0].split("(")[0].strip()
+    """void fesomtra::k_edge<true, 2, ...>(...) -> fesomtra::k_edge"""
+    n = name.split("<")[0].split("(")[0].strip()
+    return n[5:] if n.startswith("void ") else n
 
 
 def _counter_means(d: Path, counter: str):
@@ -382,6 +388,45 @@ def rocprof_summary(args):
     print("\n".join(lines))
 
 
+def dropin_timing(case, kb: int, steps: int = 10):
+    """The reference's per-tracer call sequence on device arrays (INTEGRATION.md section 1 with
+    the device-pointer entry points), per step and tracer: tracer_set_values_device (the caller's
+    current values, reference order), tracer_zero_tendencies, tracer_do_oce_adv_tra and
+    tracer_get_tendencies_device; the caller's own update of its arrays is not included.  Returns
+    ms per step (all tracers) over `steps` steps after one warm-up step."""
+    import ctypes
+    import torch
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    t = FESOMTracerAdvection()
+    t.set_block_width(kb)
+    t.setup_case(case)
+    vals = [torch.from_numpy(np.ascontiguousarray(case.tracers[i])).cuda() for i in range(case.ntracers)]
+    dh = torch.empty_like(vals[0])
+    dv = torch.empty_like(vals[0])
+    L = t.lib
+    nl, N = case.mesh.nl, case.mesh.nod2D
+
+    def step():
+        for i in range(case.ntracers):
+            L.tracer_set_values_device(i + 1, ctypes.c_void_p(vals[i].data_ptr()), nl, N)
+            L.tracer_zero_tendencies()
+            L.tracer_do_oce_adv_tra(case.dt, i + 1)
+            L.tracer_get_tendencies_device(ctypes.c_void_p(dh.data_ptr()), ctypes.c_void_p(dv.data_ptr()), nl - 1, N)
+
+    torch.cuda.synchronize()
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    w = t.block_width()
+    t.set_block_width(0)
+    t.cleanup()
+    return ms, w
+
+
 def describe(case):
     m = case.mesh
     sc = case.schemes[0]
@@ -400,10 +445,13 @@ def precision_study(args):
     log(f"precision study on {describe(case)}, {args.study_steps} steps")
     results = {}
     ref = None
-    for prec, sr in ((0, True), (1, True), (1, False), (2, True), (2, False), (3, True), (3, False)):
+    for prec, sr, comp in ((0, True, False), (1, True, False), (1, False, False), (1, False, True),
+                           (2, True, False), (2, False, False), (2, False, True), (3, True, False),
+                           (3, False, False), (3, False, True)):
         t = FESOMTracerAdvection()
         t.set_precision(prec)
         t.set_rounding(sr)
+        t.set_compensated(comp)
         t.setup_case(case)
         v0 = [t.stats(i + 1)["volume_sum"] for i in range(case.ntracers)]
         ms = t.advect_steps(args.study_steps, case.dt)
@@ -421,7 +469,8 @@ def precision_study(args):
                         "rel_l2": float(np.sqrt(np.sum(d * d) / np.sum(rr * rr))),
                         "rel_linf": float(np.max(np.abs(d)) / np.max(np.abs(rr))),
                         "volume_sum_rel_drift": abs(v1[i] - v0[i]) / abs(v0[i])})
-        key = PREC_NAMES[prec] + ("" if prec == 0 else (", stochastic rounding" if sr else ", nearest"))
+        key = PREC_NAMES[prec] + ("" if prec == 0 else (", compensated" if comp else
+                                                          (", stochastic rounding" if sr else ", nearest")))
         results[key] = {"device_ms": ms, "tracers": per}
         log(f"{key}: {ms:.1f} ms, {per}")
     print(json.dumps({"study": "precision", "workload": describe(case), "steps": args.study_steps,
@@ -448,6 +497,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the mixed-precision batched companion run")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the timing of the per-tracer do_oce_adv_tra call sequence")
+    ap.add_argument("--no-multimesh", action="store_true",
+                    help="skip the 32-mesh union companion run")
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--rounding", default="stochastic", choices=["stochastic", "nearest"],
                     help="rounding of the tracer update in the single-precision modes")
@@ -514,20 +567,58 @@ def main():
             r64 = run(bcase, 0, 64, args.batched_steps, 0, keep=True, tune=not args.no_autotune)
             ref64 = r64["values"]
             log(f"batched FP64 reference: {r64['value']:.3e} updates/s")
+        # compensated FP32 storage (remainders kept: the sum conserved to FP64 round-off), and the
+        # plain FP32 storage with the chosen rounding beside it
         rb = run(bcase, 1, 64, args.batched_steps, 0, compare=ref64, tune=not args.no_autotune,
+                 compensated=True)
+        rs = run(bcase, 1, 64, args.batched_steps, 0, compare=ref64, tune=not args.no_autotune,
                  stochastic=args.rounding == "stochastic")
         ref64 = None
-        batched = {"workload": describe(bcase), "precision": "FP32 arithmetic and storage, FP64 "
-                   "accumulation / update / limiter ratios / sums", "rounding": args.rounding,
-                   "block_width": rb["block_width"],
+        batched = {"workload": describe(bcase), "precision": "FP32 arithmetic and storage with the "
+                   "rounding remainders kept (compensated), FP64 accumulation / update / limiter "
+                   "ratios / sums", "block_width": rb["block_width"],
                    "steps": args.batched_steps, "value": rb["value"], "unit": "node-level updates/s",
                    "ms_per_step": rb["wall_s"] * 1e3 / args.batched_steps,
                    "vs_fp64_value": rb["value"] / r["value"],
                    "volume_sum_rel_drift": rb["drift"],
                    "error_vs_fp64": rb["error"],
                    "fp64_same_workload_value": None if args.no_batched_accuracy else r64["value"],
-                   "roofline": roofline(bcase, 1, rb, "mesh500x64")}
-        log(f"batched: {rb['value']:.3e} updates/s ({rb['value'] / r['value']:.2f}x FP64)")
+                   "vs_fp64_same_workload": None if args.no_batched_accuracy else rb["value"] / r64["value"],
+                   "roofline": roofline(bcase, 1, rb, "mesh500x64"),
+                   "plain_storage": {"rounding": args.rounding, "value": rs["value"],
+                                     "ms_per_step": rs["wall_s"] * 1e3 / args.batched_steps,
+                                     "volume_sum_rel_drift": rs["drift"], "error_vs_fp64": rs["error"]}}
+        log(f"batched: {rb['value']:.3e} updates/s ({rb['value'] / r['value']:.2f}x FP64), drift {rb['drift']:.2e}; "
+            f"plain storage {rs['value']:.3e}, drift {rs['drift']:.2e}")
+
+    multimesh = None
+    if args.workload == "mesh1000" and not args.no_multimesh:
+        t0 = time.perf_counter()
+        mcase = make_case("multimesh32")
+        log(f"multi-mesh companion {describe(mcase)} (generated in {time.perf_counter() - t0:.1f} s)")
+        rm = run(mcase, 1, 4, 100, 5, graph=True, tune=not args.no_autotune, compensated=True)
+        dts = sorted(set(np.round(mcase.dt_node, 3).tolist()))
+        multimesh = {"workload": describe(mcase), "meshes": 32, "nodes": mcase.mesh.nod2D,
+                     "node_levels_per_tracer": mcase.mesh.n_node_levels, "tracers": mcase.ntracers,
+                     "member_dt_s": [dts[0], dts[-1]], "precision": "FP32, compensated storage",
+                     "launch": "the 32 meshes as one disjoint union, every member with its own dt; "
+                               "100 steps replayed as one HIP graph",
+                     "steps": 100, "value": rm["value"], "unit": "node-level updates/s",
+                     "ms_per_step": rm["wall_s"] * 1e3 / 100, "volume_sum_rel_drift": rm["drift"],
+                     "step_ms_kernels": rm["step_ms"], "block_width": rm["block_width"]}
+        log(f"multi-mesh: {rm['value']:.3e} updates/s")
+
+    dropin = None
+    if args.workload == "mesh1000" and not args.no_dropin:
+        dropin = {"sequence": "per tracer: tracer_set_values_device, tracer_zero_tendencies, "
+                              "tracer_do_oce_adv_tra, tracer_get_tendencies_device (device arrays in "
+                              "reference order)", "tracers": case.ntracers}
+        for kbw in (1, 0):
+            ms, w = dropin_timing(case, kbw)
+            dropin[f"ms_per_step_block_width_{w}"] = ms
+            log(f"drop-in sequence, block width {w}: {ms:.3f} ms per step")
+        best = min(v for k, v in dropin.items() if k.startswith("ms_per_step"))
+        dropin["vs_advect_steps"] = best / (r["wall_s"] * 1e3 / steps)
 
     cpu = None
     if not args.no_cpu_baseline and args.workload in ("mesh1000", "analytic20"):
@@ -554,7 +645,10 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "volume_sum_rel_drift": r["drift"],
+        "layout_autotune": r["layout"],
         "batched": batched,
+        "multimesh": multimesh,
+        "dropin": dropin,
     }
     print(json.dumps(out), flush=True)
 

@@ -1,4 +1,11 @@
-"""Autotuner of the launch configuration (levels per thread of each of the four kernels).
+"""Autotuner of the launch configuration: the tracer layout (block width, LDS tiles) and the levels
+per thread of each of the four kernels.
+
+``tune_layout`` times whole steps (k_qflux and every pass, HIP events) for each candidate tracer
+block width -- how many tracers one kernel pass advances, interleaved in HBM -- and, for the
+narrow blocks (width <= 4), with and without the LDS tiles of the edge and node kernels, and keeps
+the fastest; the tracers are carried over exactly when the layout changes and the trial steps are
+undone with a device checkpoint at every candidate.
 
 Each column is split into chunks of ``lc`` layers, one thread per chunk and tracer lane.  Short
 chunks expose more parallelism; long chunks avoid recomputing the vertical stencils at chunk
@@ -32,6 +39,51 @@ def signature(case, precision: int, block_width: int) -> str:
     sc = case.schemes[0]
     return (f"N{m.nod2D}-E{m.edge2D}-nl{m.nl}-NL{m.n_node_levels}-tr{case.ntracers}-kb{block_width}"
             f"-p{precision}-{sc.hor}-{sc.ver}-{sc.lim}")
+
+
+def width_candidates(ntracers: int) -> list:
+    """Block widths worth timing: powers of two from max(1, ntr/8) up to min(64, next power of
+    two of ntr)."""
+    top = 1
+    while top < min(ntracers, 64):
+        top *= 2
+    lo = max(1, top // 8)
+    out, w = [], lo
+    while w <= top:
+        out.append(w)
+        w *= 2
+    return out
+
+
+def tune_layout(tracer, case, widths: Optional[Sequence[int]] = None, steps: int = 3,
+                verbose: bool = False) -> Dict:
+    """Chooses and applies the fastest tracer block width and tile use for the case set up in
+    ``tracer``.  Returns {"block_width", "tiles", "step_ms": {"<width>/<tiles>": ms}}."""
+    lib = tracer.lib
+    tracer.set_graph(False)
+    widths = list(widths or width_candidates(case.ntracers))
+    table = {}
+    for w in widths:
+        for tiles in ((True, False) if w <= 4 else (True,)):
+            tracer.set_block_width(w)
+            tracer.set_halo_tiles(tiles)
+            if lib.tracer_checkpoint(0) != 0:
+                raise RuntimeError("tracer_checkpoint failed")
+            tracer.advect_steps(1, case.dt)  # warm-up (and tile set-up)
+            lib.tracer_set_kernel_timing(1)
+            tracer.advect_steps(steps, case.dt)
+            _, step_ms, _ = tracer.step_times()
+            lib.tracer_set_kernel_timing(0)
+            if lib.tracer_checkpoint(1) != 0:
+                raise RuntimeError("tracer_checkpoint restore failed")
+            table[(w, tiles)] = step_ms
+            if verbose:
+                print(f"[autotune] width {w} tiles {tiles}: {step_ms:.4f} ms/step", flush=True)
+    (w, tiles) = min(table, key=table.get)
+    tracer.set_block_width(w)
+    tracer.set_halo_tiles(tiles)
+    return {"block_width": w, "tiles": tiles,
+            "step_ms": {f"{k[0]}/{'tiles' if k[1] else 'untiled'}": v for k, v in table.items()}}
 
 
 def tune(tracer, case, precision: int, candidates: Sequence[int] = CANDIDATES, steps: int = 3,

@@ -472,15 +472,19 @@ __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
 #define FESOMTRA_DEG_CACHE_WIDE 6
 #endif
 // An entry's level range (the edge's scatter range) is kept as bottom | top << 8 (levels < 256,
-// checked at setup); the top is 1 but below ice-shelf cavities.
-template <bool PACK, int NC>
+// checked at setup).  GEN: kernels for meshes with cavities (the top is 1 otherwise, and the
+// other kernels test the bottom only).
+template <bool PACK, int NC, bool GEN = false>
 struct NodeEdges {
     static constexpr int kN = NC;
     int b, deg;
     int e2[NC], top[NC], oth[PACK ? 1 : NC];
     __device__ __forceinline__ int top_of(int i) const { return top[i] & 0xff; }
     __device__ __forceinline__ int utop_of(int i) const { return (top[i] >> 8) & 0xff; }
-    __device__ __forceinline__ bool on(int i, int nz) const { return nz <= top_of(i) && nz >= utop_of(i); }
+    __device__ __forceinline__ bool on(int i, int nz) const {
+        if constexpr (GEN) return nz <= top_of(i) && nz >= utop_of(i);
+        else return nz <= top_of(i);
+    }
     __device__ __forceinline__ int far_of(int i) const { return PACK ? (top[i] >> 16) : oth[i]; }
 };
 template <typename M>
@@ -488,19 +492,20 @@ __device__ __forceinline__ int entry_utop(const MeshCol<M> &m, int i) {
     return m.ne_utop ? cld(m.ne_utop, i) : 1;
 }
 // Entry i of the CSR lists is active at level nz.
-template <typename M>
+template <bool GEN, typename M>
 __device__ __forceinline__ bool entry_on(const MeshCol<M> &m, int i, int nz) {
-    return nz <= cld(m.ne_top, i) && nz >= entry_utop(m, i);
+    if constexpr (GEN) return nz <= cld(m.ne_top, i) && nz >= entry_utop(m, i);
+    else return nz <= cld(m.ne_top, i);
 }
-template <bool PACK, int NC, typename M>
-__device__ __forceinline__ void load_edges(const MeshCol<M> &m, int n, NodeEdges<PACK, NC> &ne) {
+template <bool PACK, int NC, bool GEN, typename M>
+__device__ __forceinline__ void load_edges(const MeshCol<M> &m, int n, NodeEdges<PACK, NC, GEN> &ne) {
     ne.b = cld(m.ne_ptr, n);
     ne.deg = cld(m.ne_ptr, n + 1) - ne.b;
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         const bool on = i < ne.deg;
         ne.e2[i] = on ? cld(m.ne_e2, ne.b + i) : 0;
-        const int lv = on ? cld(m.ne_top, ne.b + i) | (entry_utop(m, ne.b + i) << 8) : 0;
+        const int lv = on ? cld(m.ne_top, ne.b + i) | ((GEN ? entry_utop(m, ne.b + i) : 1) << 8) : 0;
         if constexpr (PACK) {
             ne.top[i] = on ? (cld(m.ht_loc, ne.b + i) << 16) | lv : 0;
         } else {
@@ -509,8 +514,8 @@ __device__ __forceinline__ void load_edges(const MeshCol<M> &m, int n, NodeEdges
         }
     }
 }
-template <bool PACK, int NC>
-__device__ __forceinline__ void no_edges(NodeEdges<PACK, NC> &ne) {
+template <bool PACK, int NC, bool GEN>
+__device__ __forceinline__ void no_edges(NodeEdges<PACK, NC, GEN> &ne) {
     ne.b = ne.deg = 0;
 #pragma unroll
     for (int i = 0; i < NC; ++i) ne.e2[i] = ne.top[i] = 0;
@@ -518,14 +523,14 @@ __device__ __forceinline__ void no_edges(NodeEdges<PACK, NC> &ne) {
     for (int i = 0; i < (PACK ? 1 : NC); ++i) ne.oth[i] = 0;
 }
 // Calls f(edge, second, far) for the edges of n active at level nz, in reference order.
-template <bool PACK, int NC, typename M, typename F>
-__device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges<PACK, NC> &ne, int nz, F &&f) {
+template <bool PACK, int NC, bool GEN, typename M, typename F>
+__device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges<PACK, NC, GEN> &ne, int nz, F &&f) {
 #pragma unroll
     for (int i = 0; i < NC; ++i)
         if (i < ne.deg && ne.on(i, nz)) f(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
     for (int i = NC; i < ne.deg; ++i) {
         const int e2 = cld(m.ne_e2, ne.b + i);
-        if (entry_on(m, ne.b + i, nz))
+        if (entry_on<GEN>(m, ne.b + i, nz))
             f(e2 >> 1, (e2 & 1) != 0, cld(PACK ? m.ht_loc : m.ne_other, ne.b + i));
     }
 }
@@ -551,28 +556,28 @@ constexpr int kDegCacheFor = KB == 64 ? FESOMTRA_DEG_CACHE_WIDE : FESOMTRA_DEG_C
 // The load phase of the cached entries active at level nz, into v; edges_use consumes them
 // (and loads and consumes the entries past the cache).  Split so that other loads can be issued
 // between the two.
-template <bool PACK, int NC, typename LOAD, typename V>
-__device__ __forceinline__ void edges_pre(const NodeEdges<PACK, NC> &ne, int nz, LOAD &&load, V (&v)[NC]) {
+template <bool PACK, int NC, bool GEN, typename LOAD, typename V>
+__device__ __forceinline__ void edges_pre(const NodeEdges<PACK, NC, GEN> &ne, int nz, LOAD &&load, V (&v)[NC]) {
 #pragma unroll
     for (int i = 0; i < NC; ++i)
         if (i < ne.deg && ne.on(i, nz)) v[i] = load(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
 }
-template <bool PACK, int NC, typename M, typename V, typename LOAD, typename USE>
-__device__ __forceinline__ void edges_use(const MeshCol<M> &m, const NodeEdges<PACK, NC> &ne, int nz, const V (&v)[NC],
+template <bool PACK, int NC, bool GEN, typename M, typename V, typename LOAD, typename USE>
+__device__ __forceinline__ void edges_use(const MeshCol<M> &m, const NodeEdges<PACK, NC, GEN> &ne, int nz, const V (&v)[NC],
                                           LOAD &&load, USE &&use) {
 #pragma unroll
     for (int i = 0; i < NC; ++i)
         if (i < ne.deg && ne.on(i, nz)) use(v[i], ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
     for (int i = NC; i < ne.deg; ++i) {
         const int e2 = cld(m.ne_e2, ne.b + i);
-        if (entry_on(m, ne.b + i, nz)) {
+        if (entry_on<GEN>(m, ne.b + i, nz)) {
             const int far = cld(PACK ? m.ht_loc : m.ne_other, ne.b + i);
             use(load(e2 >> 1, (e2 & 1) != 0, far), e2 >> 1, (e2 & 1) != 0, far);
         }
     }
 }
-template <bool TWO, bool PACK, int NC, typename M, typename LOAD, typename USE>
-__device__ __forceinline__ void for_edges_ld(const MeshCol<M> &m, const NodeEdges<PACK, NC> &ne, int nz, LOAD &&load,
+template <bool TWO, bool PACK, int NC, bool GEN, typename M, typename LOAD, typename USE>
+__device__ __forceinline__ void for_edges_ld(const MeshCol<M> &m, const NodeEdges<PACK, NC, GEN> &ne, int nz, LOAD &&load,
                                              USE &&use) {
     if constexpr (!TWO) {
         for_edges(m, ne, nz, [&](int e, bool second, int far) { use(load(e, second, far), e, second, far); });
@@ -591,7 +596,7 @@ __device__ __forceinline__ void for_edges_ld(const MeshCol<M> &m, const NodeEdge
 // TILED stages the neighbours' values per level in LDS.  The low-order vertical flux uses w_e;
 // with wsplit (:221-231) the flux subtracted from the high-order one is the upwind flux of the
 // full w instead, and k_vimpl then adds the implicit part to the stored LO.
-template <bool TILED, int VER, int KB, typename S, typename M, typename C, bool NDT = false>
+template <bool TILED, int VER, int KB, typename S, typename M, typename C, bool GEN = false>
 __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt_step, int lc,
                                                                  bool wsplit) {
     using D = double;
@@ -605,7 +610,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         lane = (int)threadIdx.x % KB;
         act = n < m.N && lane_on(p.mask, lane);
         nzmax = act ? cld(m.nlev_n, n) : 1;
-        top = act ? node_top(m, n) : 1;
+        top = act ? node_top<GEN>(m, n) : 1;
         L = nzmax - 1;
         nz0 = max(ht.nz_b, top);
         nz1 = min(ht.nz_e, L);
@@ -616,7 +621,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         column_lane<KB>(n, lane);
         if (n >= m.N || !lane_on(p.mask, lane)) return;
         nzmax = cld(m.nlev_n, n);
-        top = node_top(m, n);
+        top = node_top<GEN>(m, n);
         L = nzmax - 1;
         if (!chunk_layers(L, lc, nz0, nz1)) return;
         nz0 = max(nz0, top);
@@ -625,11 +630,11 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         lv1 = nz1;
     }
     const size_t Np = m.Np, Ep = m.Ep;
-    NodeEdges<TILED, kDegCacheFor<KB>> ne;
+    NodeEdges<TILED, kDegCacheFor<KB>, GEN> ne;
     if (act) load_edges(m, n, ne);
     else no_edges(ne);
     const int nc_n = act ? n : 0;
-    const double dt = node_dt<NDT>(m, nc_n, dt_step);
+    const double dt = node_dt<GEN>(m, nc_n, dt_step);
     const NodeCol<D, S, M, KB> nc{Np, nc_n, lane, L};
     const Col<D, S, M, KB> col = make_col<D, S, M, KB>(p.TAB, m.w, m, nc_n, lane, nzmax, top);
     const D pv = (D)num_ord;
@@ -659,7 +664,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
     auto stage = [&](int nz) { ht.each([&](int i, int) { sT[nz & 1][threadIdx.x + 256 * i] = st[i]; }); };
     // layer nz of this column (its neighbours' values staged in sT[nz & 1] when TILED)
     auto body = [&](int nz) {
-        if (TILED && (!act || nz > nz1 || nz < nz0)) return;
+        if (TILED && (!act || nz > nz1 || (GEN && nz < nz0))) return;
         const int buf = nz & 1;
         const int j = nz + 1;  // bottom interface of layer nz
         if (VER == VER_QR4C) {
@@ -738,13 +743,13 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
 // tridiagonal system on hnode_new is built and eliminated top-down in double, storing (c', t')
 // in p.tri, then back-substituted bottom-up and added to LO.  Operation order as the reference
 // (FP64 results are bit-identical).  Columns have at least two wet layers (checked at setup).
-template <int KB, typename S, typename M, typename C, bool NDT = false>
+template <int KB, typename S, typename M, typename C, bool GEN = false>
 __global__ __launch_bounds__(256) void k_vimpl(const MeshCol<M> m, const StepPtrs<S, C> p, double dt_step) {
     int n, lane;
     column_lane<KB>(n, lane);
     if (n >= m.N || !lane_on(p.mask, lane)) return;
-    const double dt = node_dt<NDT>(m, n, dt_step);
-    const int L = cld(m.nlev_n, n) - 1, top = node_top(m, n);
+    const double dt = node_dt<GEN>(m, n, dt_step);
+    const int L = cld(m.nlev_n, n) - 1, top = node_top<GEN>(m, n);
     if (L - top < 1) return;
     const size_t Np = m.Np;
     auto at = [&](int nz) { return ((size_t)(nz - 1) * Np + n) * KB + lane; };
@@ -820,14 +825,14 @@ __global__ __launch_bounds__(256) void k_vimpl(const MeshCol<M> m, const StepPtr
 
 // Vertical high-order flux without the limiter (pwvel = we, o_init_zero=.true.), in double:
 // interfaces nz0..nz1 of the chunk, plus the bottom interface in the deepest chunk.
-template <int VER, int KB, typename S, typename M, typename C, bool NDT = false>
+template <int VER, int KB, typename S, typename M, typename C, bool GEN = false>
 __global__ __launch_bounds__(256, kMinWaves<KB>) void k_vert(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, double dt_step, int lc) {
     using D = double;
     int n, lane;
     column_lane<KB>(n, lane);
     if (n >= m.N || !lane_on(p.mask, lane)) return;
-    const double dt = node_dt<NDT>(m, n, dt_step);
-    const int nzmax = cld(m.nlev_n, n), L = nzmax - 1, top = node_top(m, n);
+    const double dt = node_dt<GEN>(m, n, dt_step);
+    const int nzmax = cld(m.nlev_n, n), L = nzmax - 1, top = node_top<GEN>(m, n);
     int nz0, nz1;
     if (!chunk_layers(L, lc, nz0, nz1)) return;
     nz0 = max(nz0, top);
@@ -941,7 +946,7 @@ __global__ __launch_bounds__(256) void k_cavity_aux(const MeshCol<M> m, const St
 #endif
 template <bool TILED, int KB>
 constexpr bool kBoundsPfFor = FESOMTRA_BOUNDS_PF != 0 && !TILED && KB == 64;
-template <bool TILED, int KB, typename S, typename M, typename C, bool NDT = false>
+template <bool TILED, int KB, typename S, typename M, typename C, bool GEN = false>
 __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshCol<M> m, const StepPtrs<S, C> p, double dt_step, int lc) {
     using D = double;
     __shared__ D shi[TILED ? 2 : 1][TILED ? kHtSlots : 1], slo[TILED ? 2 : 1][TILED ? kHtSlots : 1];
@@ -954,7 +959,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         lane = (int)threadIdx.x % KB;
         act = n < m.N && lane_on(p.mask, lane);
         L = act ? cld(m.nlev_n, n) - 1 : 0;
-        top = act ? node_top(m, n) : 1;
+        top = act ? node_top<GEN>(m, n) : 1;
         nz0 = max(ht.nz_b, top);
         nz1 = min(ht.nz_e, L);
         act = act && nz0 <= nz1;
@@ -965,7 +970,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         column_lane<KB>(n, lane);
         if (n >= m.N || !lane_on(p.mask, lane)) return;
         L = cld(m.nlev_n, n) - 1;
-        top = node_top(m, n);
+        top = node_top<GEN>(m, n);
         if (!chunk_layers(L, lc, nz0, nz1)) return;
         nz0 = max(nz0, top);
         if (nz0 > nz1) return;
@@ -975,8 +980,8 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
     }
     const size_t Np = m.Np, Ep = m.Ep;
     const int dry_from = act ? cld(m.nlev_min, n) : 0;  // levels >= this have a dry element around n
-    const double dt = node_dt<NDT>(m, act ? n : 0, dt_step);
-    NodeEdges<TILED, kDegCacheFor<KB>> ne;
+    const double dt = node_dt<GEN>(m, act ? n : 0, dt_step);
+    NodeEdges<TILED, kDegCacheFor<KB>, GEN> ne;
     if (act) load_edges(m, n, ne);
     else no_edges(ne);
     const LaneLd<KB, D, S> ldT{p.T, lane};
@@ -992,6 +997,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
     // Elements around n whose top lies below level nz contribute the values the reference finds in
     // its shared work array there (m.cq_*, p.cqv, written by k_cavity_aux before this kernel).
     auto cavity_bounds = [&](int nz, D &vmax, D &vmin) {
+        if constexpr (!GEN) return;
         if (!m.cq_ptr) return;
         const int qe = cld(m.cq_ptr, n + 1);
         for (int i = cld(m.cq_ptr, n); i < qe; ++i) {
@@ -1219,7 +1225,7 @@ template <bool TILED, int KB, typename C>
 constexpr bool kFinalPreFor = kFinalPre && !TILED && (KB == 64 || !std::is_same<C, double>::value);
 // AB2: also writes the next step's valuesAB and valuesold (a separate instantiation, so that the
 // other runs keep their register budget).
-template <bool TILED, bool FCT, int KB, typename S, typename M, typename C, bool AB2 = false, bool NDT = false>
+template <bool TILED, bool FCT, int KB, typename S, typename M, typename C, bool AB2 = false, bool GEN = false>
 __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCol<M> m, const StepPtrs<S, C> p, double dt_step, int lc) {
     static_assert(!TILED || FCT, "tiles stage the limiter ratios");
     constexpr bool PRE = kFinalPreFor<TILED, KB, C>;
@@ -1233,7 +1239,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
         lane = (int)threadIdx.x % KB;
         act = n < m.N && lane_on(p.mask, lane);
         L = act ? cld(m.nlev_n, n) - 1 : 0;
-        ctop = act ? node_top(m, n) : 1;
+        ctop = act ? node_top<GEN>(m, n) : 1;
         nz0 = max(ht.nz_b, ctop);
         nz1 = min(ht.nz_e, L);
         act = act && nz0 <= nz1;
@@ -1243,16 +1249,16 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
         column_lane<KB>(n, lane);
         if (n >= m.N || !lane_on(p.mask, lane)) return;
         L = cld(m.nlev_n, n) - 1;
-        ctop = node_top(m, n);
+        ctop = node_top<GEN>(m, n);
         if (!chunk_layers(L, lc, nz0, nz1)) return;
         nz0 = max(nz0, ctop);
         if (nz0 > nz1) return;
         lv0 = nz0;
         lv1 = nz1;
     }
-    const double dt = node_dt<NDT>(m, act ? n : 0, dt_step);
+    const double dt = node_dt<GEN>(m, act ? n : 0, dt_step);
     const size_t Np = m.Np, Ep = m.Ep;
-    NodeEdges<TILED, kDegCacheFor<KB>> ne;
+    NodeEdges<TILED, kDegCacheFor<KB>, GEN> ne;
     if (act) load_edges(m, n, ne);
     else no_edges(ne);
     // limited flux at interface j given R at layers j-1 (rp0, rm0) and j (rp1, rm1)
@@ -1307,7 +1313,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
                     sm[i] = p.Rm[X(c)];
                 });
             }
-            if (!act || nz > nz1 || nz < nz0) continue;
+            if (!act || nz > nz1 || (GEN && nz < nz0)) continue;
         }
         const size_t cell = (size_t)(nz - 1) * Np + n;
         const size_t eo = (size_t)(nz - 1) * Ep, no = (size_t)(nz - 1) * Np;
@@ -1435,23 +1441,47 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
 // Renumbering / transposition between the reference layout (nl-1, nod2D) in reference order
 // and lane `lane` of a device tracer block [k][Np][kb] in device order.
 // dir 0: ref -> dev, dir 1: dev -> ref.
+// The reference layout is node-major (a node's layers contiguous, nodes in reference order), the
+// device layout level-major (a level's nodes contiguous, in device order): a transpose with a
+// permutation.  A workgroup takes 64 device nodes x 32 layers through LDS, so both sides are
+// read and written in contiguous runs (a node's 32 layers on the reference side, 64 nodes of a
+// level on the device side) whatever the permutation.
+constexpr int kRelayNodes = 64, kRelayLevels = 32;
 template <typename S>
-__global__ void k_relayout(const double *__restrict__ ref_in, S *__restrict__ dev_out,
-                           const S *__restrict__ dev_in, double *__restrict__ ref_out,
-                           const int *__restrict__ perm, int N, int L, size_t Np, int kb, int lane, int dir,
-                           float *__restrict__ res) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)N * L) return;
-    const int k = (int)(i / N), n = (int)(i - (size_t)k * N);
-    const int o = perm[n];  // reference number of device node n
-    const size_t d = ((size_t)k * Np + n) * kb + lane;
+__global__ __launch_bounds__(256) void k_relayout(const double *__restrict__ ref_in, S *__restrict__ dev_out,
+                                                  const S *__restrict__ dev_in, double *__restrict__ ref_out,
+                                                  const int *__restrict__ perm, int N, int L, size_t Np, int kb,
+                                                  int lane, int dir, float *__restrict__ res) {
+    __shared__ double tile[kRelayNodes][kRelayLevels + 1];
+    const int n0 = (int)blockIdx.x * kRelayNodes, k0 = (int)blockIdx.y * kRelayLevels;
+    constexpr int kCells = kRelayNodes * kRelayLevels;
     // res (compensated storage): the rounding remainder v - stored, in FP32 beside the stored value
     if (dir == 0) {
-        const double v = ref_in[(size_t)o * L + k];
-        std_(dev_out, d, v);
-        if (res) res[d] = (float)(v - ldc<double>(dev_out, d));
+        for (int i = (int)threadIdx.x; i < kCells; i += (int)blockDim.x) {  // node-major reads
+            const int nn = i / kRelayLevels, kk = i % kRelayLevels, n = n0 + nn, k = k0 + kk;
+            if (n < N && k < L) tile[nn][kk] = ref_in[(size_t)perm[n] * L + k];
+        }
+        __syncthreads();
+        for (int i = (int)threadIdx.x; i < kCells; i += (int)blockDim.x) {  // level-major writes
+            const int kk = i / kRelayNodes, nn = i % kRelayNodes, n = n0 + nn, k = k0 + kk;
+            if (n >= N || k >= L) continue;
+            const size_t d = ((size_t)k * Np + n) * kb + lane;
+            const double v = tile[nn][kk];
+            std_(dev_out, d, v);
+            if (res) res[d] = (float)(v - ldc<double>(dev_out, d));
+        }
     } else {
-        ref_out[(size_t)o * L + k] = ldc<double>(dev_in, d) + (res ? (double)res[d] : 0.0);
+        for (int i = (int)threadIdx.x; i < kCells; i += (int)blockDim.x) {  // level-major reads
+            const int kk = i / kRelayNodes, nn = i % kRelayNodes, n = n0 + nn, k = k0 + kk;
+            if (n >= N || k >= L) continue;
+            const size_t d = ((size_t)k * Np + n) * kb + lane;
+            tile[nn][kk] = ldc<double>(dev_in, d) + (res ? (double)res[d] : 0.0);
+        }
+        __syncthreads();
+        for (int i = (int)threadIdx.x; i < kCells; i += (int)blockDim.x) {  // node-major writes
+            const int nn = i / kRelayLevels, kk = i % kRelayLevels, n = n0 + nn, k = k0 + kk;
+            if (n < N && k < L) ref_out[(size_t)perm[n] * L + k] = tile[nn][kk];
+        }
     }
 }
 
@@ -1562,7 +1592,7 @@ static void edge_kernel(dim3 g, hipStream_t s, const MeshCol<M> &m, const StepPt
     k_edge<false, HOR, FCT, KB, S, M, C><<<g, 256, 0, s>>>(m, p, ph, lc);
 }
 
-template <int KB, bool NDT, typename S, typename M, typename C>
+template <int KB, bool GEN, typename S, typename M, typename C>
 static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const SchemeArgs &sc,
                           const LaunchCfg &cfg, hipStream_t s, hipEvent_t *ev) {
     const int L = m.L;
@@ -1594,16 +1624,16 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
             if constexpr (KB <= 4) {
                 const dim3 tl = tgrid(cfg.lc_lo);
                 switch (sc.ver) {
-                case VER_UPW1: k_lo<true, VER_UPW1, KB, S, M, C, NDT><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-                case VER_QR4C: k_lo<true, VER_QR4C, KB, S, M, C, NDT><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-                case VER_CDIFF: k_lo<true, VER_CDIFF, KB, S, M, C, NDT><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-                case VER_PPM: k_lo<true, VER_PPM, KB, S, M, C, NDT><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+                case VER_UPW1: k_lo<true, VER_UPW1, KB, S, M, C, GEN><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+                case VER_QR4C: k_lo<true, VER_QR4C, KB, S, M, C, GEN><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+                case VER_CDIFF: k_lo<true, VER_CDIFF, KB, S, M, C, GEN><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+                case VER_PPM: k_lo<true, VER_PPM, KB, S, M, C, GEN><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
                 default: return hipErrorInvalidValue;
                 }
                 LAUNCH_CHECK();
                 if (sc.wsplit) {
                     if (!p.tri) return hipErrorInvalidValue;
-                    k_vimpl<KB, S, M, C, NDT><<<dim3(grid_for((size_t)m.N * KB, 256)), 256, 0, s>>>(m, p, sc.dt);
+                    k_vimpl<KB, S, M, C, GEN><<<dim3(grid_for((size_t)m.N * KB, 256)), 256, 0, s>>>(m, p, sc.dt);
                     LAUNCH_CHECK();
                 }
                 if (m.cq_n > 0) {
@@ -1611,24 +1641,24 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
                     LAUNCH_CHECK();
                 }
                 MARK(2);
-                k_bounds<true, KB, S, M, C, NDT><<<tgrid(cfg.lc_bounds), 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
+                k_bounds<true, KB, S, M, C, GEN><<<tgrid(cfg.lc_bounds), 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
                 LAUNCH_CHECK();
                 MARK(3);
-                if (p.ab_next) k_final<true, true, KB, S, M, C, true, NDT><<<tgrid(cfg.lc_final), 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
-                else k_final<true, true, KB, S, M, C, false, NDT><<<tgrid(cfg.lc_final), 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+                if (p.ab_next) k_final<true, true, KB, S, M, C, true, GEN><<<tgrid(cfg.lc_final), 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+                else k_final<true, true, KB, S, M, C, false, GEN><<<tgrid(cfg.lc_final), 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
             }
         } else {
             switch (sc.ver) {
-            case VER_UPW1: k_lo<false, VER_UPW1, KB, S, M, C, NDT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-            case VER_QR4C: k_lo<false, VER_QR4C, KB, S, M, C, NDT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-            case VER_CDIFF: k_lo<false, VER_CDIFF, KB, S, M, C, NDT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-            case VER_PPM: k_lo<false, VER_PPM, KB, S, M, C, NDT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+            case VER_UPW1: k_lo<false, VER_UPW1, KB, S, M, C, GEN><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+            case VER_QR4C: k_lo<false, VER_QR4C, KB, S, M, C, GEN><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+            case VER_CDIFF: k_lo<false, VER_CDIFF, KB, S, M, C, GEN><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+            case VER_PPM: k_lo<false, VER_PPM, KB, S, M, C, GEN><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
             default: return hipErrorInvalidValue;
             }
             LAUNCH_CHECK();
             if (sc.wsplit) {
                 if (!p.tri) return hipErrorInvalidValue;
-                k_vimpl<KB, S, M, C, NDT><<<dim3(grid_for((size_t)m.N * KB, 256)), 256, 0, s>>>(m, p, sc.dt);
+                k_vimpl<KB, S, M, C, GEN><<<dim3(grid_for((size_t)m.N * KB, 256)), 256, 0, s>>>(m, p, sc.dt);
                 LAUNCH_CHECK();
             }
             if (m.cq_n > 0) {
@@ -1636,18 +1666,18 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
                 LAUNCH_CHECK();
             }
             MARK(2);
-            k_bounds<false, KB, S, M, C, NDT><<<gb, 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
+            k_bounds<false, KB, S, M, C, GEN><<<gb, 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
             LAUNCH_CHECK();
             MARK(3);
-            if (p.ab_next) k_final<false, true, KB, S, M, C, true, NDT><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
-            else k_final<false, true, KB, S, M, C, false, NDT><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+            if (p.ab_next) k_final<false, true, KB, S, M, C, true, GEN><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+            else k_final<false, true, KB, S, M, C, false, GEN><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
         }
     } else {
         switch (sc.ver) {
-        case VER_UPW1: k_vert<VER_UPW1, KB, S, M, C, NDT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-        case VER_QR4C: k_vert<VER_QR4C, KB, S, M, C, NDT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-        case VER_CDIFF: k_vert<VER_CDIFF, KB, S, M, C, NDT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
-        case VER_PPM: k_vert<VER_PPM, KB, S, M, C, NDT><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_UPW1: k_vert<VER_UPW1, KB, S, M, C, GEN><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_QR4C: k_vert<VER_QR4C, KB, S, M, C, GEN><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_CDIFF: k_vert<VER_CDIFF, KB, S, M, C, GEN><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
+        case VER_PPM: k_vert<VER_PPM, KB, S, M, C, GEN><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo); break;
         default: return hipErrorInvalidValue;
         }
         LAUNCH_CHECK();
@@ -1666,8 +1696,8 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
                 pf.TAB = reinterpret_cast<const S *>(p.LO);
             }
         }
-        if (pf.ab_next) k_final<false, false, KB, S, M, C, true, NDT><<<gf, 256, 0, s>>>(m, pf, sc.dt, cfg.lc_final);
-        else k_final<false, false, KB, S, M, C, false, NDT><<<gf, 256, 0, s>>>(m, pf, sc.dt, cfg.lc_final);
+        if (pf.ab_next) k_final<false, false, KB, S, M, C, true, GEN><<<gf, 256, 0, s>>>(m, pf, sc.dt, cfg.lc_final);
+        else k_final<false, false, KB, S, M, C, false, GEN><<<gf, 256, 0, s>>>(m, pf, sc.dt, cfg.lc_final);
     }
     LAUNCH_CHECK();
     MARK(4);
@@ -1687,8 +1717,9 @@ hipError_t launch_step(const MeshCol<M> &m, const StepPtrs<S, C> &p, int kb, con
     if (cfg.lc_edge < 1 || cfg.lc_lo < 1 || cfg.lc_bounds < 1 || cfg.lc_final < 1) return hipErrorInvalidValue;
     if (kb < 64 && (p.mask >> kb) != 0ull) return hipErrorInvalidValue;  // lanes beyond the block
     if (m.cq_n > 0 && sc.fct && !p.cqv) return hipErrorInvalidValue;    // cavity bounds need their rows
-    // a per-node time step (m.dtn) selects kernels that load it; the others keep dt scalar
-    const bool ndt = m.dtn != nullptr;
+    // per-node time steps (m.dtn) or cavities (m.ulev_n) select the general kernels; the others
+    // keep dt a scalar and every top at level 1
+    const bool ndt = m.dtn != nullptr || m.ulev_n != nullptr;
     switch (kb) {
     case 1: return ndt ? step_kb<1, true>(m, p, sc, cfg, s, ev) : step_kb<1, false>(m, p, sc, cfg, s, ev);
     case 2: return ndt ? step_kb<2, true>(m, p, sc, cfg, s, ev) : step_kb<2, false>(m, p, sc, cfg, s, ev);
@@ -1708,8 +1739,8 @@ hipError_t launch_relayout(const double *ref_in, S *dev_out, const S *dev_in, do
     const size_t total = (size_t)N * L;
     if (total == 0) return hipSuccess;
     if (lane < 0 || lane >= kb) return hipErrorInvalidValue;
-    k_relayout<S><<<dim3((unsigned)((total + 255) / 256)), 256, 0, s>>>(ref_in, dev_out, dev_in, ref_out,
-                                                                        perm, N, L, Np, kb, lane, dir, res);
+    const dim3 g((unsigned)((N + kRelayNodes - 1) / kRelayNodes), (unsigned)((L + kRelayLevels - 1) / kRelayLevels));
+    k_relayout<S><<<g, 256, 0, s>>>(ref_in, dev_out, dev_in, ref_out, perm, N, L, Np, kb, lane, dir, res);
     return hipGetLastError();
 }
 

@@ -187,10 +187,11 @@ struct MeshCol {
     const int *cq_edge, *cq_top, *cq_off, *cq_ptr, *cq_ent;
 };
 
-// Top layer of node n's column (1 without cavities).
-template <typename M>
+// Top layer of node n's column (1 without cavities; GEN: kernels for cavity meshes).
+template <bool GEN = true, typename M>
 __device__ __forceinline__ int node_top(const MeshCol<M> &m, int n) {
-    return m.ulev_n ? cld(m.ulev_n, n) : 1;
+    if constexpr (GEN) return m.ulev_n ? cld(m.ulev_n, n) : 1;
+    else return 1;
 }
 // Top layer of element el.
 template <typename M>
@@ -199,11 +200,11 @@ __device__ __forceinline__ int elem_top(const MeshCol<M> &m, int el) {
 }
 
 // Time step of node n's cells: dt, or dt * dtn[n] where the mesh carries a factor per node (a
-// union of independent meshes, each advanced with its own time step).  NDT: kernels compiled for
-// such meshes (the others keep dt a scalar kernel argument).
-template <bool NDT, typename M>
+// union of independent meshes, each advanced with its own time step).  GEN: kernels compiled for
+// meshes with such factors or with cavities (the others keep dt a scalar kernel argument).
+template <bool GEN, typename M>
 __device__ __forceinline__ double node_dt(const MeshCol<M> &m, int n, double dt) {
-    if constexpr (NDT) return dt * (double)cld(m.dtn, n);
+    if constexpr (GEN) return m.dtn ? dt * (double)cld(m.dtn, n) : dt;
     else return dt;
 }
 
