@@ -57,6 +57,10 @@ _SIGNATURES = {
     "tracer_set_muscl_support": (_c_int, [_pi, _pi, _c_int, _c_int]),
     "tracer_set_velocity_elem": (_c_int, [_pd, _pd, _pd, _pd, _c_int, _c_int, _c_int]),
     "tracer_set_node_dt": (_c_int, [_pd, _c_int]),
+    "tracer_set_ensemble": (_c_int, [_c_int]),
+    "tracer_get_ensemble": (_c_int, [_pi]),
+    "tracer_set_velocity_member": (_c_int, [_c_int, _pd, _pd, _pd, _pd, _c_int, _c_int, _c_int]),
+    "tracer_set_tracer_member": (_c_int, [_c_int, _c_int]),
     "tracer_set_scheme": (_c_int, [_c_int, _c_char_p, _c_char_p, _c_char_p, _c_double, _c_double]),
     "tracer_advect_steps": (_c_int, [_c_int, _c_double]),
     "tracer_do_oce_adv_tra": (_c_int, [_c_double, _c_int]),

@@ -309,6 +309,44 @@ def multimesh32(nsteps: int = 100, count: int = 32, seed: int = 4,
     return cases
 
 
+def ensemble_cases(base: Case, members: int, seed: int = 0, wsplit_cap: Optional[float] = None) -> List[Case]:
+    """``members`` cases on base's mesh object with base's time step and schemes, each with its own
+    velocity and tracers (an ensemble; member 0 is base).  Member k > 0: the streamfunction
+    sin(2 pi x/Lx + a) sin(2 pi y/Ly + b) with seeded phases, its speed scaled by a seeded factor
+    in [0.5, 1] of base's largest speed (so base's dt stays within its CFL), the layering of
+    :func:`streamfunction_velocity`; a vertical velocity where base has one, with seeded phases
+    and amplitude up to base's largest |w|, on the same interior interfaces; base's tracers scaled
+    by (1 + k / 10).  ``wsplit_cap`` splits every member's w as :func:`split_vertical_velocity`."""
+    mesh = base.mesh
+    rng = np.random.default_rng(np.random.SeedSequence(seed).spawn(1)[0])
+    umax = float(np.max(np.hypot(base.uv[..., 0], base.uv[..., 1])))
+    wmax = float(np.abs(base.w).max())
+    xy = mesh.xy_m
+    out = [base]
+    for k in range(1, members):
+        a, b = rng.uniform(0.0, 2.0 * math.pi, 2)
+        psi = np.sin(2 * math.pi * xy[:, 0] / mesh.Lx + a) * np.sin(2 * math.pi * xy[:, 1] / mesh.Ly + b)
+        scale = rng.uniform(0.5, 1.0)
+        uv = streamfunction_velocity(mesh, umax * scale, 500.0, psi=psi)
+        if mesh.ulevels.max() > 1:  # no transport above an element's top (a cavity roof)
+            dry = np.arange(1, mesh.nl)[None, :] < mesh.ulevels[:, None]
+            uv = np.where(dry[..., None], 0.0, uv)
+        w = np.zeros_like(base.w)
+        if wmax > 0.0:
+            c, d = rng.uniform(0.0, 2.0 * math.pi, 2)
+            horiz = np.sin(2 * math.pi * xy[:, 0] / mesh.Lx + c) * np.cos(2 * math.pi * xy[:, 1] / mesh.Ly + d)
+            lev = np.arange(1, mesh.nl + 1)[None, :]
+            interior = (lev > mesh.ulevels_nod2D[:, None]) & (lev < mesh.nlevels_nod2D[:, None])
+            prof = np.sin(math.pi * (lev - 1) / (mesh.nl - 1))
+            w = np.where(interior, wmax * rng.uniform(0.5, 1.0) * horiz[:, None] * prof, 0.0)
+        c_k = Case(f"{base.name}_m{k}", mesh, uv, w, w.copy(), w.copy(), base.tracers * (1.0 + 0.1 * k),
+                   [dataclasses.replace(sc) for sc in base.schemes], base.dt, base.nsteps)
+        out.append(c_k)
+    if wsplit_cap is not None:
+        out = [split_vertical_velocity(c, wsplit_cap) for c in out]
+    return out
+
+
 def union_case(cases: Sequence[Case], name: str = "union", own_dt: bool = True) -> Case:
     """Independent cases with the same tracer count and schemes as one case on the disjoint union
     of their meshes.  own_dt: every member keeps its own time step (``dt_node`` holds the member's

@@ -49,6 +49,11 @@ struct State {
     std::vector<double> elem_area, area, areasvol, edxdy, ecd, elem_cos, gsca;
     std::vector<double> hnode, hnode_new, helem, Z, zbar3, zbar;
     std::vector<double> uv, w, wi, we;
+    // Ensemble: nmem velocities on this mesh; member 0 is uv / w / wi / we, members 1.. are kept
+    // here in the same layouts, and each tracer belongs to one member (tr_mem, 0-based)
+    int nmem = 1;
+    std::vector<std::vector<double>> muv, mw, mwi, mwe;
+    std::vector<int> tr_mem;
     std::vector<double> node_dt;  // time-step factor per node (empty: 1 everywhere)
     bool have_coord = false, have_conn = false, have_nlev = false, have_elev = false;
     bool have_geom = false, have_vert = false, have_hnode = false, have_muscl = false;
@@ -132,6 +137,7 @@ struct State {
     double qflux_ms = 0.0, step_ms = 0.0;  // summed k_qflux and whole-step times (kernel timing)
     long step_count = 0;
     uint32_t *d_seed = nullptr;  // device copy of step_seed, advanced by a kernel per step
+    int *d_lane_mem = nullptr;   // ensembles: member of each block lane, [blocks][kb]
     bool dropin_valid = false;   // the work arrays hold the last tracer_do_oce_adv_tra call's
     bool lo_dirty = false;       // the LO buffer held a values snapshot (single precision, no limiter)
 };
@@ -415,6 +421,55 @@ std::vector<M> rows_of(const std::vector<double> &src, int ld, const std::vector
     return h;
 }
 
+// Velocities of ensemble member k (0: the shared / first member).
+const std::vector<double> &mem_uv(const State &s, int k) { return k == 0 ? s.uv : s.muv[k - 1]; }
+const std::vector<double> &mem_w(const State &s, int k) { return k == 0 ? s.w : s.mw[k - 1]; }
+const std::vector<double> &mem_wi(const State &s, int k) { return k == 0 ? s.wi : s.mwi[k - 1]; }
+const std::vector<double> &mem_we(const State &s, int k) { return k == 0 ? s.we : s.mwe[k - 1]; }
+
+// Device layouts of every member's velocity: uv member-major [nmem][L][Tp][2]; w, w_e, w_i
+// member-interleaved [nl][Np][nmem] (one member: the plain level-major layouts).  w_e and w_i
+// are left empty where they equal w for every member (the device then reads w for them).
+struct VelHost {
+    std::vector<double> uv, w, we, wi;
+};
+VelHost velocity_layout(const State &s) {
+    const int L = s.nl - 1, nl = s.nl, M = s.nmem;
+    VelHost v;
+    v.uv.assign((size_t)M * L * s.Tp * 2, 0.0);
+    for (int k = 0; k < M; ++k) {
+        const std::vector<double> &src = mem_uv(s, k);
+        double *dst = &v.uv[(size_t)k * L * s.Tp * 2];
+        for (int t = 0; t < s.T; ++t) {
+            const double *row = &src[(size_t)s.perm_t[t] * L * 2];
+            for (int z = 0; z < L; ++z) {
+                dst[((size_t)z * s.Tp + t) * 2] = row[2 * z];
+                dst[((size_t)z * s.Tp + t) * 2 + 1] = row[2 * z + 1];
+            }
+        }
+    }
+    auto inter = [&](const std::vector<double> &(*get)(const State &, int)) {
+        std::vector<double> out((size_t)nl * s.Np * M, 0.0);
+        for (int k = 0; k < M; ++k) {
+            const std::vector<double> &src = get(s, k);
+            for (int n = 0; n < s.N; ++n) {
+                const double *col = &src[(size_t)s.perm_n[n] * nl];
+                for (int j = 0; j < nl; ++j) out[((size_t)j * s.Np + n) * M + k] = col[j];
+            }
+        }
+        return out;
+    };
+    bool we_w = true, wi_w = true;
+    for (int k = 0; k < M; ++k) {
+        we_w = we_w && mem_we(s, k) == mem_w(s, k);
+        wi_w = wi_w && mem_wi(s, k) == mem_w(s, k);
+    }
+    v.w = inter(mem_w);
+    if (!we_w) v.we = inter(mem_we);
+    if (!wi_w) v.wi = inter(mem_wi);
+    return v;
+}
+
 template <class M>
 int upload_mesh(State &s, MeshCol<M> &m, const std::vector<int> &ints_n1, const std::vector<int> &ints_n2,
                 const std::vector<int> &ints_n3) {
@@ -442,11 +497,13 @@ int upload_mesh(State &s, MeshCol<M> &m, const std::vector<int> &ints_n1, const 
     if (same_bits(s.areasvol, s.area)) m.areasvol = m.area;
     else if (upload(s, &m.areasvol, level_major<M>(s.areasvol, nl, s.perm_n, s.Np))) return -1;
     if (upload(s, &m.zbar, level_major<M>(s.zbar3, nl, s.perm_n, s.Np))) return -1;
-    if (upload(s, &m.w, level_major<M>(s.w, nl, s.perm_n, s.Np))) return -1;
+    const VelHost vh = velocity_layout(s);
+    m.nmem = s.nmem;
+    if (upload(s, &m.w, std::vector<M>(vh.w.begin(), vh.w.end()))) return -1;
     m.we = m.w;
-    if (s.we != s.w && upload(s, &m.we, level_major<M>(s.we, nl, s.perm_n, s.Np))) return -1;
+    if (!vh.we.empty() && upload(s, &m.we, std::vector<M>(vh.we.begin(), vh.we.end()))) return -1;
     m.wi = m.w;
-    if (s.wi != s.w && upload(s, &m.wi, level_major<M>(s.wi, nl, s.perm_n, s.Np))) return -1;
+    if (!vh.wi.empty() && upload(s, &m.wi, std::vector<M>(vh.wi.begin(), vh.wi.end()))) return -1;
     // elements
     std::vector<double> ea(s.T), ec(s.T);
     for (int t = 0; t < s.T; ++t) {
@@ -457,17 +514,7 @@ int upload_mesh(State &s, MeshCol<M> &m, const std::vector<int> &ints_n1, const 
     if (upload(s, &m.elem_cos, std::vector<M>(ec.begin(), ec.end()))) return -1;
     if (upload(s, &m.gsca, rows_of<M>(s.gsca, 6, s.perm_t))) return -1;
     if (upload(s, &m.helem, level_major<M>(s.helem, L, s.perm_t, s.Tp))) return -1;
-    {
-        std::vector<M> h((size_t)L * s.Tp * 2, M(0));
-        for (int t = 0; t < s.T; ++t) {
-            const double *row = &s.uv[(size_t)s.perm_t[t] * L * 2];
-            for (int k = 0; k < L; ++k) {
-                h[((size_t)k * s.Tp + t) * 2] = (M)row[2 * k];
-                h[((size_t)k * s.Tp + t) * 2 + 1] = (M)row[2 * k + 1];
-            }
-        }
-        if (upload(s, &m.uv, h)) return -1;
-    }
+    if (upload(s, &m.uv, std::vector<M>(vh.uv.begin(), vh.uv.end()))) return -1;
     if (upload(s, &m.ecd, rows_of<M>(s.ecd, 4, s.perm_e))) return -1;
     if (upload(s, &m.edxdy, rows_of<M>(s.edxdy, 2, s.perm_e))) return -1;
     return 0;
@@ -586,26 +633,16 @@ int refresh_velocity(State &s) {
     s.vel_gen++;
     if (!s.on_device) return 0;
     MeshCol<double> &m = s.m64;
-    const int L = s.nl - 1, nl = s.nl;
-    std::vector<double> h((size_t)L * s.Tp * 2, 0.0);
-    for (int t = 0; t < s.T; ++t) {
-        const double *row = &s.uv[(size_t)s.perm_t[t] * L * 2];
-        for (int k = 0; k < L; ++k) {
-            h[((size_t)k * s.Tp + t) * 2] = row[2 * k];
-            h[((size_t)k * s.Tp + t) * 2 + 1] = row[2 * k + 1];
-        }
-    }
+    const VelHost vh = velocity_layout(s);
     HIPCHK(hipStreamSynchronize(s.stream));
-    HIPCHK(hipMemcpy(const_cast<double *>(m.uv), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
-    const std::vector<double> w = level_major<double>(s.w, nl, s.perm_n, s.Np);
-    HIPCHK(hipMemcpy(const_cast<double *>(m.w), w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
-    auto part = [&](const std::vector<double> &src, const double *&dst) -> int {
-        if (src == s.w) {
+    HIPCHK(hipMemcpy(const_cast<double *>(m.uv), vh.uv.data(), vh.uv.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(const_cast<double *>(m.w), vh.w.data(), vh.w.size() * sizeof(double), hipMemcpyHostToDevice));
+    auto part = [&](const std::vector<double> &d, const double *&dst) -> int {
+        if (d.empty()) {  // equal to w for every member
             if (dst != m.w) s.gen++;  // kernels and graphs take the pointer
             dst = m.w;
             return 0;
         }
-        const std::vector<double> d = level_major<double>(src, nl, s.perm_n, s.Np);
         if (dst == m.w) {
             double *q;
             if (dalloc(s, &q, d.size())) return -1;
@@ -615,7 +652,18 @@ int refresh_velocity(State &s) {
         HIPCHK(hipMemcpy(const_cast<double *>(dst), d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
         return 0;
     };
-    if (part(s.we, m.we) || part(s.wi, m.wi)) return -1;
+    if (part(vh.we, m.we) || part(vh.wi, m.wi)) return -1;
+    return 0;
+}
+
+// Ensembles: the member of every lane of every tracer block (unused lanes: member 0).
+int upload_lane_members(State &s) {
+    if (!s.d_lane_mem) return 0;
+    const int nb = n_blocks(s);
+    std::vector<int> h((size_t)nb * s.kb, 0);
+    for (int t = 0; t < s.ntr && t < (int)h.size(); ++t) h[t] = t < (int)s.tr_mem.size() ? s.tr_mem[t] : 0;
+    HIPCHK(hipStreamSynchronize(s.stream));
+    HIPCHK(hipMemcpy(s.d_lane_mem, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -823,7 +871,7 @@ int build_device(State &s) {
     if (upload_mesh<double>(s, s.m64, i_nlev, i_nmin, i_nbl)) return -1;
     if (upload_node_dt(s)) return -1;
     if (upload_cavities(s)) return -1;
-    if (dalloc(s, &s.m64.q, (size_t)L * s.Ep)) return -1;
+    if (dalloc(s, &s.m64.q, (size_t)L * s.Ep * s.nmem)) return -1;
     {
         const int *p;
         if (upload(s, &p, s.perm_n)) return -1;
@@ -840,6 +888,11 @@ int build_device(State &s) {
         return -1;
     if (dalloc(s, &s.part, (size_t)kStatsBlocks) || dalloc(s, &s.stage, (size_t)N * L)) return -1;
     if (dalloc(s, &s.d_seed, 64)) return -1;
+    s.d_lane_mem = nullptr;
+    if (s.nmem > 1) {
+        if (dalloc(s, &s.d_lane_mem, (size_t)n_blocks(s) * s.kb)) return -1;
+        if (upload_lane_members(s)) return -1;
+    }
     s.cqv = nullptr;
     if (s.m64.cq_n > 0) {
         const size_t bytes = (size_t)s.cq_rows * 2 * kb * cb;
@@ -1034,6 +1087,7 @@ StepPtrs<S_, C_> make_ptrs(State &s, int blk, unsigned long long mask, bool drop
     p.tri = static_cast<double2 *>(s.tri);
     p.cqv = static_cast<C_ *>(s.cqv);
     p.resid = dropin ? nullptr : static_cast<float *>(resid_of(s, s.dT[blk]));
+    p.lane_mem = s.nmem > 1 ? s.d_lane_mem + (size_t)blk * s.kb : nullptr;
     if (dropin) {
         p.acc_h = s.acc_h;
         p.acc_v = s.acc_v;
@@ -1205,6 +1259,7 @@ int build_tiles(State &s) {
 // implicit vertical split, with valuesAB := values).
 bool fused_applies(const State &s, const SchemeSel &sc) {
     if (!s.node_dt.empty()) return false;  // the fused kernel takes one time step for all nodes
+    if (s.nmem > 1) return false;          // and one velocity
     if (s.m64.ulev_n) return false;        // and has no cavity levels
     return s.fused && !s.wsplit && s.ab_mode == 0 && s.prec == PREC_F64 && s.kb == 1 && sc.fct;
 }
@@ -1739,6 +1794,9 @@ int tracer_finalize(const char *output_dir) {
     State &s = S();
     if (!s.initialized) return fail("tracer not initialized.");
     if (!output_dir) return fail("tracer_finalize: null path");
+    if (s.nmem > 1)
+        return fail("tracer_finalize: the restart files hold one velocity; an ensemble of %d members is not written",
+                    s.nmem);
     if (s.on_device && pull_values(s)) return -1;
     RestartFiles out;
     restart_of_state(s, out);
@@ -1861,6 +1919,7 @@ int tracer_allocate_tracers(int num_tracers, int nl, int nod2D, int AB_order) {
     s.ntr = num_tracers;
     s.values.assign(num_tracers, std::vector<double>((size_t)(nl - 1) * nod2D, 0.0));
     s.schemes.assign(num_tracers, SchemeSel());
+    s.tr_mem.assign(num_tracers, 0);
     s.on_device = false;
     return 0;
 }
@@ -2502,6 +2561,67 @@ int tracer_set_node_dt(const double *factor, int nod2D) {
         s.gen++;  // kernels and captured graphs take the pointer
     }
     return 0;
+}
+
+int tracer_set_ensemble(int nmembers) {
+    State &s = S();
+    if (!s.dims) return fail("tracer_set_ensemble: call tracer_set_mesh_dims first");
+    if (nmembers < 1 || nmembers > kMaxKB)
+        return fail("tracer_set_ensemble: nmembers=%d (1 to %d)", nmembers, kMaxKB);
+    if (nmembers == s.nmem) return 0;
+    for (int t = 0; t < (int)s.tr_mem.size(); ++t)
+        if (s.tr_mem[t] >= nmembers)
+            return fail("tracer_set_ensemble: tracer %d belongs to member %d", t + 1, s.tr_mem[t] + 1);
+    if (invalidate(s)) return -1;  // the device arrays change size
+    const size_t nuv = (size_t)2 * (s.nl - 1) * s.T, nw = (size_t)s.nl * s.N;
+    auto sized = [](const std::vector<double> &v, size_t n) { return v.size() == n ? v : std::vector<double>(n, 0.0); };
+    // new members start from the first member's velocity
+    while ((int)s.muv.size() < nmembers - 1) {
+        s.muv.push_back(sized(s.uv, nuv));
+        s.mw.push_back(sized(s.w, nw));
+        s.mwi.push_back(sized(s.wi, nw));
+        s.mwe.push_back(sized(s.we, nw));
+    }
+    s.muv.resize(nmembers - 1);
+    s.mw.resize(nmembers - 1);
+    s.mwi.resize(nmembers - 1);
+    s.mwe.resize(nmembers - 1);
+    s.nmem = nmembers;
+    s.vel_gen++;
+    s.gen++;
+    return 0;
+}
+
+int tracer_get_ensemble(int *nmembers) {
+    if (!nmembers) return fail("tracer_get_ensemble: null pointer");
+    *nmembers = S().nmem;
+    return 0;
+}
+
+int tracer_set_velocity_member(int member, const double *uv, const double *w, const double *w_i,
+                               const double *w_e, int nl, int nod2D, int elem2D) {
+    State &s = S();
+    if (member < 1 || member > s.nmem)
+        return fail("tracer_set_velocity_member: member=%d, nmembers=%d (tracer_set_ensemble)", member, s.nmem);
+    if (member == 1) return tracer_set_velocity_elem(uv, w, w_i, w_e, nl, nod2D, elem2D);
+    if (check_dims(s, nl, nod2D, "tracer_set_velocity_member")) return -1;
+    if (elem2D != s.T) return fail("tracer_set_velocity_member: elem2D=%d != %d", elem2D, s.T);
+    if (!uv || !w) return fail("tracer_set_velocity_member: null pointer");
+    const int k = member - 2;
+    s.muv[k].assign(uv, uv + (size_t)2 * (nl - 1) * elem2D);
+    s.mw[k].assign(w, w + (size_t)nl * nod2D);
+    s.mwi[k] = w_i ? std::vector<double>(w_i, w_i + (size_t)nl * nod2D) : s.mw[k];
+    s.mwe[k] = w_e ? std::vector<double>(w_e, w_e + (size_t)nl * nod2D) : s.mw[k];
+    return refresh_velocity(s);
+}
+
+int tracer_set_tracer_member(int tracer_id, int member) {
+    State &s = S();
+    if (tracer_id < 1 || tracer_id > s.ntr) return fail("invalid tracer_id %d, num_tracers=%d", tracer_id, s.ntr);
+    if (member < 1 || member > s.nmem)
+        return fail("tracer_set_tracer_member: member=%d, nmembers=%d (tracer_set_ensemble)", member, s.nmem);
+    s.tr_mem[tracer_id - 1] = member - 1;
+    return s.on_device ? upload_lane_members(s) : 0;
 }
 
 int tracer_set_kernel_timing(int enable) {

@@ -134,8 +134,9 @@ struct Col {
     const M *w, *ar, *hn, *ho;
     size_t Np;
     int n, lane, nzmin, nzmax;
+    Member mb;  // the lane's velocity member (w is member-interleaved)
     __device__ __forceinline__ C T(int nz) const { return ldc<C>(t, ((size_t)(nz - 1) * Np + n) * KB + lane); }
-    __device__ __forceinline__ C W(int j) const { return (C)cld(w, (size_t)(j - 1) * Np + n); }
+    __device__ __forceinline__ C W(int j) const { return (C)cld(w, mb.at((size_t)(j - 1) * Np + n)); }
     __device__ __forceinline__ C A(int j) const { return (C)cld(ar, (size_t)(j - 1) * Np + n); }
     __device__ __forceinline__ C HN(int nz) const { return (C)cld(hn, (size_t)(nz - 1) * Np + n); }
     __device__ __forceinline__ C HO(int nz) const { return (C)cld(ho, (size_t)(nz - 1) * Np + n); }
@@ -285,6 +286,9 @@ template <typename C, typename S, typename M, int KB>
 struct NodeCol {
     size_t Np;
     int n, lane, L;
+    Member mb;  // the lane's velocity member
+    // velocity-dependent node-interface array (w, w_e, w_i) of the lane's member at interface j
+    __device__ __forceinline__ C vel(const M *a, int j) const { return (C)cld(a, mb.at((size_t)(j - 1) * Np + n)); }
     __device__ __forceinline__ C lay(const S *a, int nz) const {
         return (nz >= 1 && nz <= L) ? ldc<C>(a, ((size_t)(nz - 1) * Np + n) * KB + lane) : C(0);
     }
@@ -308,8 +312,9 @@ __device__ __forceinline__ C ver_ho_at(int j, int nzmax, const C tw[4], const C 
 
 template <typename C, typename S, typename M, int KB>
 __device__ __forceinline__ Col<C, S, M, KB> make_col(const S *t, const M *w, const MeshCol<M> &m, int n,
-                                                     int lane, int nzmax, int top = 1) {
+                                                     int lane, int nzmax, int top = 1, Member mb = Member{0, 1}) {
     Col<C, S, M, KB> c;
+    c.mb = mb;
     c.t = t;
     c.w = w;
     c.ar = m.area;

@@ -168,8 +168,9 @@ struct ArrLd {
     __device__ __forceinline__ C operator()(size_t i) const { return v[i]; }
 };
 
-// SAME (tiled form only): valuesAB is values, so only one field is staged.
-template <bool TILED, int HOR, bool FCT, int KB, typename S, typename M, typename C, bool SAME = false>
+// SAME (tiled form only): valuesAB is values, so only one field is staged.  GEN: ensembles (each
+// lane reads its member's transport) and cavity meshes.
+template <bool TILED, int HOR, bool FCT, int KB, typename S, typename M, typename C, bool SAME = false, bool GEN = false>
 __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, int lc) {
     // two levels in flight only where the registers allow it (one staged field)
     constexpr int P = SAME ? kEdgePipe : 1;
@@ -189,6 +190,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
         column_lane<KB>(e, lane);
         if (e >= m.E || !lane_on(p.mask, lane)) return;
     }
+    const Member mb = lane_member<GEN>(m, p, lane);
     const int4 q = cld(m.edge_ne, e);
     const int n1 = q.x, n2 = q.y, el1 = q.z, el2 = q.w;
     const int nl1 = cld(m.nlev_e, el1) - 1;
@@ -259,7 +261,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
             if constexpr (!SAME)
                 if (!same) stA[r][i] = ldA(o + node);
         });
-        qn[r] = (act && nz <= nz1) ? (C)cld(m.q, (size_t)(nz - 1) * Ep + e) : C(0);
+        qn[r] = (act && nz <= nz1) ? (C)cld(m.q, mb.at((size_t)(nz - 1) * Ep + e)) : C(0);
     };
     if constexpr (TILED) {
         fetch(lv0, std::integral_constant<int, 0>{});
@@ -276,7 +278,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
     constexpr bool UPF = kEdgeUntiledPf<TILED, KB>;
     auto pf_load = [&](int nz, EdgePf<C> &f) {
         const size_t o = (size_t)(nz - 1) * Np;
-        f.q = (C)cld(m.q, (size_t)(nz - 1) * Ep + e);
+        f.q = (C)cld(m.q, mb.at((size_t)(nz - 1) * Ep + e));
         f.a[0] = ldA(o + n1);
         f.a[1] = ldA(o + n2);
         f.t[0] = f.a[0];
@@ -299,7 +301,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
         if (TILED && (!act || nz > nz1 || nz < nz0)) return;
         const int buf = nz % NB;
         const int k = nz - 1;
-        const C v = TILED ? qcur : (UPF ? pf->q : (C)cld(m.q, (size_t)k * Ep + e));
+        const C v = TILED ? qcur : (UPF ? pf->q : (C)cld(m.q, mb.at((size_t)k * Ep + e)));
         const size_t o = (size_t)k * Np;
         const size_t ie = ((size_t)k * Ep + e) * KB + lane;
         const SlotLd<KB, C> lsT{sT[TILED ? buf : 0], lane},
@@ -406,11 +408,14 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
 // Edge transport of the current velocity, q(k, e) = sum over the edge's elements wet at level k
 // of (+-) (u dy - v dx) helem (lib/oce_adv_tra_hor.F90:166-253, segments A to E: below a cavity
 // roof only the element already wet contributes), in double, once per step for all tracers.  One
-// thread per edge walks its levels; levels where neither element is wet are not read.
-template <typename M>
+// thread per edge walks its levels; levels where neither element is wet are not read.  An
+// ensemble's (ENS) member is blockIdx.y: it reads that member's uv and writes its interleaved q.
+template <typename M, bool ENS = false>
 __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
     const int e = column_index();
     if (e >= m.E) return;
+    const Member mb = ENS ? Member{(int)blockIdx.y, m.nmem} : Member{0, 1};
+    const M *__restrict__ uv = ENS ? m.uv + (size_t)mb.m * (size_t)m.L * m.Tp * 2 : m.uv;
     const int4 q = cld(m.edge_ne, e);
     const int el1 = q.z, el2 = q.w;
     const int nl1 = cld(m.nlev_e, el1) - 1;
@@ -428,7 +433,7 @@ __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
         const bool use1 = nz <= nl1 && nz >= nu1, use2 = nz <= nl2 && nz >= nu2;
         const double v1 = (-y1 * cd0 + u1 * cd1) * h1;
         const double v2 = (y2 * cd2 - u2 * cd3) * h2;
-        m.q[(size_t)(nz - 1) * Ep + e] = (use1 && use2) ? v1 + v2 : (use1 ? v1 : v2);
+        m.q[mb.at((size_t)(nz - 1) * Ep + e)] = (use1 && use2) ? v1 + v2 : (use1 ? v1 : v2);
     };
     auto idx = [&](int nz, int w, int el) { return (size_t)(min(nz, w) - 1) * Tp + el; };
     constexpr int B = 4;  // levels per batch: all their loads are issued before the first store
@@ -439,11 +444,11 @@ __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
 #pragma unroll
             for (int j = 0; j < B; ++j) {
                 const size_t i1 = idx(n0 + j, w1, el1), i2 = idx(n0 + j, w2, e2);
-                u1[j] = (double)cld(m.uv, 2 * i1);
-                y1[j] = (double)cld(m.uv, 2 * i1 + 1);
+                u1[j] = (double)cld(uv, 2 * i1);
+                y1[j] = (double)cld(uv, 2 * i1 + 1);
                 h1[j] = (double)cld(m.helem, i1);
-                u2[j] = (double)cld(m.uv, 2 * i2);
-                y2[j] = (double)cld(m.uv, 2 * i2 + 1);
+                u2[j] = (double)cld(uv, 2 * i2);
+                y2[j] = (double)cld(uv, 2 * i2 + 1);
                 h2[j] = (double)cld(m.helem, i2);
             }
 #pragma unroll
@@ -452,8 +457,8 @@ __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
     } else {
         for (int nz = 1; nz <= top; ++nz) {
             const size_t i1 = idx(nz, w1, el1), i2 = idx(nz, w2, e2);
-            level(nz, (double)cld(m.uv, 2 * i1), (double)cld(m.uv, 2 * i1 + 1), (double)cld(m.helem, i1),
-                  (double)cld(m.uv, 2 * i2), (double)cld(m.uv, 2 * i2 + 1), (double)cld(m.helem, i2));
+            level(nz, (double)cld(uv, 2 * i1), (double)cld(uv, 2 * i1 + 1), (double)cld(m.helem, i1),
+                  (double)cld(uv, 2 * i2), (double)cld(uv, 2 * i2 + 1), (double)cld(m.helem, i2));
         }
     }
 }
@@ -635,8 +640,9 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
     else no_edges(ne);
     const int nc_n = act ? n : 0;
     const double dt = node_dt<GEN>(m, nc_n, dt_step);
-    const NodeCol<D, S, M, KB> nc{Np, nc_n, lane, L};
-    const Col<D, S, M, KB> col = make_col<D, S, M, KB>(p.TAB, m.w, m, nc_n, lane, nzmax, top);
+    const Member mb = lane_member<GEN>(m, p, lane);
+    const NodeCol<D, S, M, KB> nc{Np, nc_n, lane, L, mb};
+    const Col<D, S, M, KB> col = make_col<D, S, M, KB>(p.TAB, m.w, m, nc_n, lane, nzmax, top, mb);
     const D pv = (D)num_ord;
     D tw[4], z[4], lo_top = 0, adf_top = 0, tcur = 0;
     QuotWin<D> qw;
@@ -644,7 +650,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         const int j = nz0;  // top interface of the chunk
 #pragma unroll
         for (int i = 0; i < 4; ++i) z[i] = (VER == VER_QR4C) ? nc.mlay(m.Z, j - 2 + i) : D(0);
-        const D wej = nc.itf(m.we, j), wj = nc.itf(m.w, j), aj = nc.itf(m.area, j);
+        const D wej = nc.vel(m.we, j), wj = nc.vel(m.w, j), aj = nc.itf(m.area, j);
         const D zbj = (VER == VER_QR4C) ? nc.itf(m.zbar, j) : D(0);
 #pragma unroll
         for (int i = 0; i < 4; ++i) tw[i] = nc.lay(p.TAB, j - 2 + i);
@@ -674,7 +680,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
             z[3] = nc.mlay(m.Z, j + 1);
         }
         const bool bottom = (j == nzmax);
-        const D wej = nc.itf(m.we, j), wj = nc.itf(m.w, j), aj = nc.itf(m.area, j);
+        const D wej = nc.vel(m.we, j), wj = nc.vel(m.w, j), aj = nc.itf(m.area, j);
         const D zbj = (VER == VER_QR4C && !bottom) ? nc.itf(m.zbar, j) : D(0);
         const size_t cell = (size_t)(nz - 1) * Np + n;
         const double h = (double)cld(m.hnode, cell), hn = (double)cld(m.hnode_new, cell);
@@ -699,13 +705,13 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         if constexpr (TILED) {
             // transports from memory, issued together; neighbour values from the tile
             for_edges_ld<true>(
-                m, ne, nz, [&](int e, bool, int) { return (double)cld(m.q, eo + e); },
+                m, ne, nz, [&](int e, bool, int) { return (double)cld(m.q, mb.at(eo + e)); },
                 [&](double qe, int, bool second, int oth) { add_edge(sT[buf][oth * KB + lane], qe, second); });
         } else {
             for_edges_ld<kTwoPhase<KB>>(
                 m, ne, nz,
                 [&](int e, bool, int oth) {
-                    return make_double2(ldc<double>(p.T, (no + oth) * KB + lane), (double)cld(m.q, eo + e));
+                    return make_double2(ldc<double>(p.T, (no + oth) * KB + lane), (double)cld(m.q, mb.at(eo + e)));
                 },
                 [&](double2 v, int, bool second, int) { add_edge(v.x, v.y, second); });
         }
@@ -770,7 +776,8 @@ __global__ __launch_bounds__(256) void k_vimpl(const MeshCol<M> m, const StepPtr
     const double zinv = 1.0 * dt;
     double t0 = 0.0, t1 = lo_at(top), t2 = lo_at(top + 1), cpp = 0.0, tpp = 0.0;
     const size_t c0 = (size_t)(top - 1) * Np + n;  // the column's top interface
-    double w1 = (double)cld(m.wi, c0), a1 = (double)cld(m.area, c0);
+    const Member mb = lane_member<GEN>(m, p, lane);
+    double w1 = (double)cld(m.wi, mb.at(c0)), a1 = (double)cld(m.area, c0);
     for (int nz = top; nz <= L; ++nz) {
         const size_t cell = (size_t)(nz - 1) * Np + n;
         const double hn = (double)cld(m.hnode_new, cell), V = (double)cld(m.areasvol, cell);
@@ -784,7 +791,7 @@ __global__ __launch_bounds__(256) void k_vimpl(const MeshCol<M> m, const StepPtr
         }
         double w2 = 0.0, a2 = 0.0;
         if (nz < L) {
-            w2 = (double)cld(m.wi, cell + Np);
+            w2 = (double)cld(m.wi, mb.at(cell + Np));
             a2 = (double)cld(m.area, cell + Np);
             v = zinv * a2 / V;
             b = b - cmin(0.0, w2) * v;
@@ -838,8 +845,9 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_vert(const MeshCol<M> m,
     nz0 = max(nz0, top);
     if (nz0 > nz1) return;
     const size_t Np = m.Np;
-    const NodeCol<D, S, M, KB> nc{Np, n, lane, L};
-    const Col<D, S, M, KB> col = make_col<D, S, M, KB>(p.TAB, m.we, m, n, lane, nzmax, top);
+    const Member mb = lane_member<GEN>(m, p, lane);
+    const NodeCol<D, S, M, KB> nc{Np, n, lane, L, mb};
+    const Col<D, S, M, KB> col = make_col<D, S, M, KB>(p.TAB, m.we, m, n, lane, nzmax, top, mb);
     const D pv = (D)num_ord;
     D tw[4], z[4];
 #pragma unroll
@@ -860,7 +868,7 @@ __global__ __launch_bounds__(256, kMinWaves<KB>) void k_vert(const MeshCol<M> m,
             z[3] = nc.mlay(m.Z, j + 1);
         }
         const bool bottom = (j == nzmax);
-        const D wj = nc.itf(m.we, j), aj = nc.itf(m.area, j);
+        const D wj = nc.vel(m.we, j), aj = nc.itf(m.area, j);
         const D zbj = (VER == VER_QR4C && !bottom) ? nc.itf(m.zbar, j) : D(0);
         tw[0] = tw[1];
         tw[1] = tw[2];
@@ -1329,7 +1337,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
             }
             if (kAddUpw) {
                 v.ao = ldc<double>(p.TAB, X(no + oth));
-                v.q = cld(m.q, eo + e);
+                v.q = cld(m.q, lane_member<GEN>(m, p, lane).at(eo + e));
             }
             return v;
         };
@@ -1579,17 +1587,17 @@ __global__ __launch_bounds__(256) void k_copy(S *__restrict__ dst, const S *__re
     if (i < n) dst[i] = src[i];
 }
 
-template <int HOR, bool FCT, int KB, typename S, typename M, typename C>
+template <int HOR, bool FCT, int KB, bool GEN, typename S, typename M, typename C>
 static void edge_kernel(dim3 g, hipStream_t s, const MeshCol<M> &m, const StepPtrs<S, C> &p, C ph, int lc) {
     if constexpr (KB <= 4) {
         if (m.et_n > 0 && m.et_kb == KB) {  // edge tiles built for this block width
             const dim3 gt(grid_for((size_t)m.et_n, 1), g.y);
-            if (p.TAB == p.T) k_edge<true, HOR, FCT, KB, S, M, C, true><<<gt, 256, 0, s>>>(m, p, ph, lc);
-            else k_edge<true, HOR, FCT, KB, S, M, C, false><<<gt, 256, 0, s>>>(m, p, ph, lc);
+            if (p.TAB == p.T) k_edge<true, HOR, FCT, KB, S, M, C, true, GEN><<<gt, 256, 0, s>>>(m, p, ph, lc);
+            else k_edge<true, HOR, FCT, KB, S, M, C, false, GEN><<<gt, 256, 0, s>>>(m, p, ph, lc);
             return;
         }
     }
-    k_edge<false, HOR, FCT, KB, S, M, C><<<g, 256, 0, s>>>(m, p, ph, lc);
+    k_edge<false, HOR, FCT, KB, S, M, C, false, GEN><<<g, 256, 0, s>>>(m, p, ph, lc);
 }
 
 template <int KB, bool GEN, typename S, typename M, typename C>
@@ -1604,12 +1612,12 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
     const C ph = (C)sc.ph, pv = (C)sc.pv;
     MARK(0);
     switch (sc.hor * 2 + (sc.fct ? 1 : 0)) {
-    case HOR_UPW1 * 2 + 0: edge_kernel<HOR_UPW1, false, KB>(ge, s, m, p, ph, cfg.lc_edge); break;
-    case HOR_UPW1 * 2 + 1: edge_kernel<HOR_UPW1, true, KB>(ge, s, m, p, ph, cfg.lc_edge); break;
-    case HOR_MUSCL * 2 + 0: edge_kernel<HOR_MUSCL, false, KB>(ge, s, m, p, ph, cfg.lc_edge); break;
-    case HOR_MUSCL * 2 + 1: edge_kernel<HOR_MUSCL, true, KB>(ge, s, m, p, ph, cfg.lc_edge); break;
-    case HOR_MFCT * 2 + 0: edge_kernel<HOR_MFCT, false, KB>(ge, s, m, p, ph, cfg.lc_edge); break;
-    case HOR_MFCT * 2 + 1: edge_kernel<HOR_MFCT, true, KB>(ge, s, m, p, ph, cfg.lc_edge); break;
+    case HOR_UPW1 * 2 + 0: edge_kernel<HOR_UPW1, false, KB, GEN>(ge, s, m, p, ph, cfg.lc_edge); break;
+    case HOR_UPW1 * 2 + 1: edge_kernel<HOR_UPW1, true, KB, GEN>(ge, s, m, p, ph, cfg.lc_edge); break;
+    case HOR_MUSCL * 2 + 0: edge_kernel<HOR_MUSCL, false, KB, GEN>(ge, s, m, p, ph, cfg.lc_edge); break;
+    case HOR_MUSCL * 2 + 1: edge_kernel<HOR_MUSCL, true, KB, GEN>(ge, s, m, p, ph, cfg.lc_edge); break;
+    case HOR_MFCT * 2 + 0: edge_kernel<HOR_MFCT, false, KB, GEN>(ge, s, m, p, ph, cfg.lc_edge); break;
+    case HOR_MFCT * 2 + 1: edge_kernel<HOR_MFCT, true, KB, GEN>(ge, s, m, p, ph, cfg.lc_edge); break;
     default: return hipErrorInvalidValue;
     }
     LAUNCH_CHECK();
@@ -1707,7 +1715,8 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
 template <typename M>
 hipError_t launch_qflux(const MeshCol<M> &m, hipStream_t s) {
     if (m.L < 1 || m.E < 1) return hipSuccess;
-    k_qflux<M><<<dim3(grid_for((size_t)m.E, 256)), 256, 0, s>>>(m);
+    if (m.nmem > 1) k_qflux<M, true><<<dim3(grid_for((size_t)m.E, 256), (unsigned)m.nmem), 256, 0, s>>>(m);
+    else k_qflux<M><<<dim3(grid_for((size_t)m.E, 256)), 256, 0, s>>>(m);
     return hipGetLastError();
 }
 
@@ -1717,9 +1726,10 @@ hipError_t launch_step(const MeshCol<M> &m, const StepPtrs<S, C> &p, int kb, con
     if (cfg.lc_edge < 1 || cfg.lc_lo < 1 || cfg.lc_bounds < 1 || cfg.lc_final < 1) return hipErrorInvalidValue;
     if (kb < 64 && (p.mask >> kb) != 0ull) return hipErrorInvalidValue;  // lanes beyond the block
     if (m.cq_n > 0 && sc.fct && !p.cqv) return hipErrorInvalidValue;    // cavity bounds need their rows
-    // per-node time steps (m.dtn) or cavities (m.ulev_n) select the general kernels; the others
-    // keep dt a scalar and every top at level 1
-    const bool ndt = m.dtn != nullptr || m.ulev_n != nullptr;
+    // per-node time steps (m.dtn), cavities (m.ulev_n) or ensemble members select the general
+    // kernels; the others keep dt a scalar, every top at level 1 and one velocity
+    const bool ndt = m.dtn != nullptr || m.ulev_n != nullptr || m.nmem > 1;
+    if (m.nmem > 1 && !p.lane_mem) return hipErrorInvalidValue;  // members need the lane table
     switch (kb) {
     case 1: return ndt ? step_kb<1, true>(m, p, sc, cfg, s, ev) : step_kb<1, false>(m, p, sc, cfg, s, ev);
     case 2: return ndt ? step_kb<2, true>(m, p, sc, cfg, s, ev) : step_kb<2, false>(m, p, sc, cfg, s, ev);

@@ -170,6 +170,10 @@ struct MeshCol {
     const M *ecd;                         // (E, 4) edge_cross_dxdy
     const M *edxdy;                       // (E, 2) edge_dxdy
     double *q;                            // [L][Ep] edge transport of the current velocity
+    // Ensemble members (several velocities on this mesh; nmem <= 1: one): the velocity arrays hold
+    // nmem members, uv member-major ([nmem][L][Tp][2]), w / we / wi and q member-interleaved
+    // ([nl][Np][nmem], [L][Ep][nmem]); each lane of a tracer block reads its own member's values.
+    int nmem;
     const double *dtn;                    // (N) time-step factor per node, or null (all 1)
     // Ice-shelf cavities (layers above a top are not wet); all null without them (tops are 1)
     const int *ulev_n;                    // (N) ulevels_nod2D, the node's top layer
@@ -208,6 +212,13 @@ __device__ __forceinline__ double node_dt(const MeshCol<M> &m, int n, double dt)
     else return dt;
 }
 
+// Member of a lane and the member count, the index and stride into the member-interleaved
+// velocity arrays: a[i * n + m] for entry i of the single-velocity layout.
+struct Member {
+    int m = 0, n = 1;
+    __device__ __forceinline__ size_t at(size_t i) const { return i * (size_t)n + (size_t)m; }
+};
+
 // Pointers of one step over one tracer block (arrays interleaved by the block width KB).
 template <typename S, typename C>
 struct StepPtrs {
@@ -232,6 +243,15 @@ struct StepPtrs {
     const uint32_t *seed_dev; // the step counter that seeds the stochastic rounding, in device
                               // memory (advanced by a kernel per step, so captured graphs replay
                               // with new seeds); null: `seed` itself is the counter
+    const int *lane_mem;      // ensembles: the member of each lane of this block (KB entries), or null
 };
+
+// Velocity member of `lane` (GEN: kernels for ensembles, per-node time steps or cavities; the
+// others read the single-velocity layout).
+template <bool GEN, typename M, typename S, typename C>
+__device__ __forceinline__ Member lane_member(const MeshCol<M> &m, const StepPtrs<S, C> &p, int lane) {
+    if constexpr (GEN) return p.lane_mem ? Member{cld(p.lane_mem, (size_t)lane), m.nmem} : Member{0, 1};
+    else return Member{0, 1};
+}
 
 }  // namespace fesomtra

@@ -220,6 +220,53 @@ class FESOMTracerAdvection:
         abi.check(self.lib.tracer_set_velocity_elem(ptr[0], ptr[1], ptr[2], ptr[3], self.nl, self.nod2D,
                                                     self.elem2D), "tracer_set_velocity_elem")
 
+    # --- ensembles ---------------------------------------------------------------------------------
+    def set_ensemble(self, nmembers: int) -> None:
+        """Number of velocity members on the mesh (tracer_set_ensemble)."""
+        abi.check(self.lib.tracer_set_ensemble(int(nmembers)), "tracer_set_ensemble")
+
+    def ensemble(self) -> int:
+        n = ctypes.c_int()
+        abi.check(self.lib.tracer_get_ensemble(ctypes.byref(n)), "tracer_get_ensemble")
+        return n.value
+
+    def set_velocity_member(self, member: int, uv: np.ndarray, w: np.ndarray,
+                            wi: Optional[np.ndarray] = None, we: Optional[np.ndarray] = None) -> None:
+        """Velocity of ensemble member `member` (1-based), layouts as set_velocity."""
+        arrs = [_f64(uv), _f64(w)] + [None if a is None else _f64(a) for a in (wi, we)]
+        ptr = [None if a is None else abi.dptr(a) for a in arrs]
+        abi.check(self.lib.tracer_set_velocity_member(int(member), ptr[0], ptr[1], ptr[2], ptr[3], self.nl,
+                                                      self.nod2D, self.elem2D), "tracer_set_velocity_member")
+
+    def set_tracer_member(self, tracer_id: int, member: int) -> None:
+        abi.check(self.lib.tracer_set_tracer_member(int(tracer_id), int(member)), "tracer_set_tracer_member")
+
+    def setup_ensemble(self, cases, order=None) -> None:
+        """Members that share a mesh and a time step, each with its own velocity and tracers
+        (configs.ensemble_cases).  The library's tracer j + 1 is slot order[j] of the member-major
+        list of (member, tracer) pairs (default: member by member); ``self.member_tracers[k][i]``
+        is the library tracer id of tracer i of member k."""
+        import dataclasses
+        base = cases[0]
+        for c in cases[1:]:
+            if c.mesh is not base.mesh or c.dt != base.dt:
+                raise ValueError("ensemble members share one mesh object and one time step")
+        slots = [(k, i) for k, c in enumerate(cases) for i in range(c.ntracers)]
+        if order is not None:
+            slots = [slots[j] for j in order]
+        tracers = np.stack([cases[k].tracers[i] for k, i in slots])
+        schemes = [cases[k].schemes[i] for k, i in slots]
+        allc = dataclasses.replace(base, tracers=tracers, schemes=schemes)
+        self.setup_case(allc, upload=False)
+        self.set_ensemble(len(cases))
+        for k, c in enumerate(cases[1:], start=2):
+            self.set_velocity_member(k, c.uv, c.w, c.wi, c.we)
+        self.member_tracers = [[0] * c.ntracers for c in cases]
+        for tid, (k, i) in enumerate(slots, start=1):
+            self.set_tracer_member(tid, k + 1)
+            self.member_tracers[k][i] = tid
+        abi.check(self.lib.tracer_init_complete(), "tracer_init_complete")
+
     def checkpoint(self, restore: bool = False) -> None:
         """Keep (restore=False) or put back (restore=True) the tracers and their time levels."""
         abi.check(self.lib.tracer_checkpoint(int(bool(restore))), "tracer_checkpoint")

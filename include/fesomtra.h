@@ -137,6 +137,21 @@ int tracer_set_velocity_elem(const double *uv, const double *w, const double *w_
  * and step with dt = 1.  A factor that differs between the two ends of an edge is not
  * conservative; the library does not check it. */
 int tracer_set_node_dt(const double *factor, int nod2D);
+/* [new] Ensemble members: nmembers (1 to 64, default 1) velocity fields on the one mesh, each
+ * tracer advected by the velocity of its member.  The reference runs one velocity per model
+ * instance (do_oce_adv_tra takes vel, w, w_i, w_e per call, lib/oce_adv_tra_driver.F90:23-45);
+ * here the members of a tracer block share every mesh read and each lane reads its member's
+ * transport and vertical velocity.  Member 1 is the velocity of tracer_set_velocity_elem; new
+ * members start as a copy of it.  Changing the count rebuilds the device copy (tracer values are
+ * kept); restart files (tracer_finalize) are refused for more than one member. */
+int tracer_set_ensemble(int nmembers);
+int tracer_get_ensemble(int *nmembers);
+/* [new] Velocity of member `member` (1-based, layouts of tracer_set_velocity_elem; member 1 is
+ * tracer_set_velocity_elem itself). */
+int tracer_set_velocity_member(int member, const double *uv, const double *w, const double *w_i,
+                               const double *w_e, int nl, int nod2D, int elem2D);
+/* [new] Member (1-based) whose velocity advects tracer `tracer_id` (default 1). */
+int tracer_set_tracer_member(int tracer_id, int member);
 /* Scheme of one tracer with the reference's names: hor "UPW1" | "MUSCL" | "MFCT",
  * ver "UPW1" | "QR4C" | "CDIFF" | "PPM", lim "NONE" | "FCT"; ph, pv as tra_adv_ph / tra_adv_pv
  * (lib/MOD_TRACER.F90:18-22).  Unknown names return -1 (the reference aborts). */

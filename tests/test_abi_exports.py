@@ -120,3 +120,63 @@ def test_setup_validation_without_gpu():
     assert b"Unknown vertical" in lib.tracer_last_error()
     assert lib.tracer_set_scheme(9, b"UPW1", b"UPW1", b"FCT", 1.0, 1.0) == -1
     lib.tracer_cleanup()
+
+
+def test_ensemble_setters_without_gpu():
+    """tracer_set_ensemble / tracer_set_velocity_member / tracer_set_tracer_member argument checks
+    (before the device copy exists), and the restart writer's refusal of several members."""
+    from fesom2_tracer_dwarf_amd import configs
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    lib = abi.load()
+    lib.tracer_cleanup()
+    assert lib.tracer_set_ensemble(2) == -1                          # before the mesh dims
+    assert b"tracer_set_mesh_dims" in lib.tracer_last_error()
+    case = configs.bathymetry_case("e", 6, 8, seed=3, nsteps=1, lev_lo=3)
+    m = case.mesh
+    t = FESOMTracerAdvection()
+    t.setup_case(case, upload=False)
+    n = ctypes.c_int()
+    assert lib.tracer_get_ensemble(ctypes.byref(n)) == 0 and n.value == 1
+    assert lib.tracer_set_ensemble(0) == -1
+    assert lib.tracer_set_ensemble(65) == -1
+    assert lib.tracer_set_ensemble(3) == 0
+    assert lib.tracer_get_ensemble(ctypes.byref(n)) == 0 and n.value == 3
+    uv, w = np.ascontiguousarray(case.uv), np.ascontiguousarray(case.w)
+    assert lib.tracer_set_velocity_member(4, abi.dptr(uv), abi.dptr(w), None, None, m.nl, m.nod2D, m.elem2D) == -1
+    assert lib.tracer_set_velocity_member(0, abi.dptr(uv), abi.dptr(w), None, None, m.nl, m.nod2D, m.elem2D) == -1
+    assert lib.tracer_set_velocity_member(3, abi.dptr(uv), abi.dptr(w), None, None, m.nl, m.nod2D, m.elem2D + 1) == -1
+    assert lib.tracer_set_velocity_member(3, abi.dptr(uv), abi.dptr(w), None, None, m.nl, m.nod2D, m.elem2D) == 0
+    assert lib.tracer_set_tracer_member(1, 4) == -1
+    assert lib.tracer_set_tracer_member(3, 1) == -1                  # two tracers only
+    assert lib.tracer_set_tracer_member(2, 3) == 0
+    assert lib.tracer_set_ensemble(2) == -1                          # tracer 2 is in member 3
+    assert b"member 3" in lib.tracer_last_error()
+    assert lib.tracer_set_tracer_member(2, 2) == 0
+    assert lib.tracer_set_ensemble(2) == 0
+    lib.tracer_cleanup()
+
+
+def test_ensemble_cases_share_the_mesh_and_stay_stable():
+    """configs.ensemble_cases: one mesh object and time step; member velocities differ, stay within
+    the base's largest speed (so its CFL bound holds) and are zero on dry layers; w only on
+    interior interfaces."""
+    from fesom2_tracer_dwarf_amd import configs
+    base = configs.bathymetry_case("e", 10, 12, seed=4, nsteps=1, lev_lo=3, w_cfl=0.3)
+    cases = configs.ensemble_cases(base, 4, seed=1)
+    umax = np.max(np.hypot(base.uv[..., 0], base.uv[..., 1]))
+    m = base.mesh
+    dry = np.arange(1, m.nl)[None, :] > (m.nlevels - 1)[:, None]
+    lev = np.arange(1, m.nl + 1)[None, :]
+    interior = (lev > m.ulevels_nod2D[:, None]) & (lev < m.nlevels_nod2D[:, None])
+    for k, c in enumerate(cases):
+        assert c.mesh is m and c.dt == base.dt
+        sp = np.hypot(c.uv[..., 0], c.uv[..., 1])
+        assert sp.max() <= umax * (1 + 1e-12)
+        assert np.all(sp[dry] == 0.0)
+        assert np.all(c.w[~interior] == 0.0) and np.abs(c.w).max() <= np.abs(base.w).max()
+        if k:
+            assert not np.array_equal(c.uv, base.uv)
+            assert np.array_equal(c.tracers, base.tracers * (1.0 + 0.1 * k))
+    split = configs.ensemble_cases(base, 2, seed=1, wsplit_cap=0.5)
+    assert all(c.use_wsplit for c in split)
+    assert np.allclose(split[1].we + split[1].wi, cases[1].w, rtol=0, atol=0)
