@@ -28,6 +28,7 @@ from __future__ import annotations
 
 import argparse
 import csv
+import dataclasses
 import hashlib
 import json
 import os
@@ -190,11 +191,13 @@ def make_case(name: str, nsteps=None):
 
 def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, lc=None,
         stage_steps: int = 10, compare=None, keep: bool = False, stochastic: bool = True,
-        tune: bool = True, fused: bool = False, compensated: bool = False):
+        tune: bool = True, fused: bool = False, compensated: bool = False, members=None):
     """Advance `case` on the GPU; returns throughput, device time, per-stage times, drift.
 
     keep: also return the final values (list per tracer); compare: final values of a reference
-    run of the same steps, against which relative L-inf / L2 over the wet layers are reported."""
+    run of the same steps, against which relative L-inf / L2 over the wet layers are reported.
+    members: the cases of an ensemble on case's mesh (case holds all their tracers, member by
+    member), set up with tracer_set_ensemble."""
     import torch
     from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
     m = case.mesh
@@ -206,7 +209,10 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
     t.set_compensated(compensated)
     if lc:
         t.set_launch_config(lc)
-    t.setup_case(case)
+    if members:
+        t.setup_ensemble(members)
+    else:
+        t.setup_case(case)
     if fused:
         t.set_fused(True)
     tuned, layout = None, None
@@ -499,6 +505,8 @@ def main():
                     help="skip the mixed-precision batched companion run")
     ap.add_argument("--no-dropin", action="store_true",
                     help="skip the timing of the per-tracer do_oce_adv_tra call sequence")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the ensemble companion run (16 velocity members on the batched mesh)")
     ap.add_argument("--no-multimesh", action="store_true",
                     help="skip the 32-mesh union companion run")
     ap.add_argument("--batched-steps", type=int, default=200)
@@ -591,6 +599,29 @@ def main():
         log(f"batched: {rb['value']:.3e} updates/s ({rb['value'] / r['value']:.2f}x FP64), drift {rb['drift']:.2e}; "
             f"plain storage {rs['value']:.3e}, drift {rs['drift']:.2e}")
 
+    ensemble = None
+    if args.workload == "mesh1000" and not args.no_batched and not args.no_ensemble:
+        # the batched workload's mesh with 16 members of 4 tracers, each member its own velocity
+        from fesom2_tracer_dwarf_amd import configs
+        base = dataclasses.replace(bcase, tracers=bcase.tracers[:4], schemes=bcase.schemes[:4])
+        members = configs.ensemble_cases(base, 16, seed=7)
+        ecase = dataclasses.replace(base, name="mesh500_ensemble16x4",
+                                    tracers=np.concatenate([c.tracers for c in members]),
+                                    schemes=[sc for c in members for sc in c.schemes])
+        re_ = run(ecase, 1, 64, args.batched_steps, 0, tune=not args.no_autotune, compensated=True,
+                  members=members)
+        ensemble = {"workload": "the batched workload's mesh and schemes, 16 ensemble members x 4 "
+                                "tracers in one block of 64 lanes, each member with its own velocity "
+                                "(tracer_set_ensemble)",
+                    "members": 16, "tracers_per_member": 4, "precision": batched["precision"],
+                    "steps": args.batched_steps, "value": re_["value"], "unit": "node-level updates/s",
+                    "ms_per_step": re_["wall_s"] * 1e3 / args.batched_steps,
+                    "vs_shared_velocity": re_["value"] / batched["value"],
+                    "volume_sum_rel_drift": re_["drift"], "step_ms_kernels": re_["step_ms"],
+                    "qflux_ms": re_["qflux_ms"]}
+        log(f"ensemble: {re_['value']:.3e} updates/s ({ensemble['vs_shared_velocity']:.2f}x the shared "
+            f"velocity), drift {re_['drift']:.2e}")
+
     multimesh = None
     if args.workload == "mesh1000" and not args.no_multimesh:
         t0 = time.perf_counter()
@@ -648,6 +679,7 @@ def main():
         "layout_autotune": r["layout"],
         "batched": batched,
         "multimesh": multimesh,
+        "ensemble": ensemble,
         "dropin": dropin,
     }
     print(json.dumps(out), flush=True)

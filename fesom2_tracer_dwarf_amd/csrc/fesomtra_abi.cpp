@@ -113,6 +113,7 @@ struct State {
     std::vector<void *> ckpt_res;    // per block: the remainders kept with it (compensated storage)
     void *adfh = nullptr, *LO = nullptr, *Rp = nullptr, *Rm = nullptr, *adfv = nullptr;
     void *LOe = nullptr;  // compensated single precision: remainder of the stored LO increment
+    void *mg = nullptr;   // node-mean gradients of a pass (MeshCol::mg_* entries x 2 x kb)
     double *acc_h = nullptr, *acc_v = nullptr;
     void *dbg_h = nullptr, *dbg_v = nullptr;
     void *tri = nullptr;      // (c', t') of the implicit vertical solve (allocated with wsplit)
@@ -718,6 +719,68 @@ int upload_cavities(State &s) {
     return 0;
 }
 
+// Node-mean gradient entries (MeshCol::mg_*): for every edge with up/down triangles, the levels
+// of its range [element top, deepest element bottom] outside the triangles' shared wet range
+// [ulevels_nod2D_max, nlevels_nod2D_min - 1] of both ends (all its levels without both triangles)
+// take the mean gradients of both end nodes; each node gets one contiguous range of entries
+// covering the levels its edges ask for.  nmin: nlevels_nod2D_min in device order.
+int upload_mean_grad_levels(State &s, const std::vector<int> &nmin) {
+    MeshCol<double> &m = s.m64;
+    m.mg_n = 0;
+    m.mg_node = m.mg_lev = m.mg_base = nullptr;
+    const int N = s.N, E = s.E, A = s.max_adj;
+    const bool cav = has_cavities(s);
+    std::vector<int> umax(N, 1);
+    if (cav)
+        for (int n = 0; n < N; ++n) {
+            const int o = s.perm_n[n];
+            int mx = 0;
+            for (int k = 0; k < s.nie_num[o]; ++k) mx = std::max(mx, s.ulev_e[s.nie[(size_t)o * A + k]]);
+            umax[n] = mx;
+        }
+    auto etop = [&](int t) { return cav ? s.ulev_e[s.perm_t[t]] : 1; };
+    std::vector<int> lo(N, 1 << 30), hi(N, 0);
+    auto want = [&](int n, int a, int b) {
+        if (a > b) return;
+        lo[n] = std::min(lo[n], a);
+        hi[n] = std::max(hi[n], b);
+    };
+    for (int e = 0; e < E; ++e) {
+        const int4 ne = s.h_ne[e];
+        const int2 ud = s.h_ud[e];
+        const int nl1 = s.h_nle[ne.z] - 1, nl2 = ne.w >= 0 ? s.h_nle[ne.w] - 1 : 0;
+        const int top = std::max(nl1, nl2);
+        const int et = ne.w >= 0 ? std::min(etop(ne.z), etop(ne.w)) : etop(ne.z);
+        int sb = 1, se = 0;
+        if (ud.x >= 0 && ud.y >= 0) {
+            se = std::min(nmin[ne.x], nmin[ne.y]) - 1;
+            if (cav) sb = std::max(umax[ne.x], umax[ne.y]);
+        }
+        for (int n : {ne.x, ne.y}) {
+            if (se < sb) {  // no shared range: every level
+                want(n, et, top);
+            } else {
+                want(n, et, std::min(top, sb - 1));
+                want(n, std::max(et, se + 1), top);
+            }
+        }
+    }
+    std::vector<int> node, lev, base(s.Np, 0);
+    for (int n = 0; n < N; ++n) {
+        if (lo[n] > hi[n]) continue;
+        base[n] = (int)node.size() - lo[n];
+        for (int z = lo[n]; z <= hi[n]; ++z) {
+            node.push_back(n);
+            lev.push_back(z);
+        }
+    }
+    if (node.size() > (size_t)INT32_MAX / 2) return fail("too many node-mean gradient entries (%zu)", node.size());
+    m.mg_n = (int)node.size();
+    if (m.mg_n == 0) return 0;
+    if (upload(s, &m.mg_node, node) || upload(s, &m.mg_lev, lev) || upload(s, &m.mg_base, base)) return -1;
+    return 0;
+}
+
 // The per-node time-step factors in device order (m.dtn = null without them).
 int upload_node_dt(State &s) {
     s.m64.dtn = nullptr;
@@ -871,6 +934,7 @@ int build_device(State &s) {
     if (upload_mesh<double>(s, s.m64, i_nlev, i_nmin, i_nbl)) return -1;
     if (upload_node_dt(s)) return -1;
     if (upload_cavities(s)) return -1;
+    if (upload_mean_grad_levels(s, i_nmin)) return -1;
     if (dalloc(s, &s.m64.q, (size_t)L * s.Ep * s.nmem)) return -1;
     {
         const int *p;
@@ -887,6 +951,8 @@ int build_device(State &s) {
         dalloc_bytes(s, &s.Rm, NLn * kb * cb) || dalloc_bytes(s, &s.adfv, NIn * kb * cb))
         return -1;
     if (dalloc(s, &s.part, (size_t)kStatsBlocks) || dalloc(s, &s.stage, (size_t)N * L)) return -1;
+    s.mg = nullptr;
+    if (s.m64.mg_n > 0 && dalloc_bytes(s, &s.mg, (size_t)s.m64.mg_n * 2 * kb * cb)) return -1;
     if (dalloc(s, &s.d_seed, 64)) return -1;
     s.d_lane_mem = nullptr;
     if (s.nmem > 1) {
@@ -1088,6 +1154,7 @@ StepPtrs<S_, C_> make_ptrs(State &s, int blk, unsigned long long mask, bool drop
     p.cqv = static_cast<C_ *>(s.cqv);
     p.resid = dropin ? nullptr : static_cast<float *>(resid_of(s, s.dT[blk]));
     p.lane_mem = s.nmem > 1 ? s.d_lane_mem + (size_t)blk * s.kb : nullptr;
+    p.mg = static_cast<C_ *>(s.mg);
     if (dropin) {
         p.acc_h = s.acc_h;
         p.acc_v = s.acc_v;

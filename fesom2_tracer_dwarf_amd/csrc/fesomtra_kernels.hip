@@ -36,6 +36,12 @@ constexpr int kMinWaves = KB == 64 ? 1 : FESOMTRA_MIN_WAVES;
 #endif
 template <bool TILED, int KB>
 constexpr int kWaves = TILED ? FESOMTRA_MIN_WAVES_TILED : kMinWaves<KB>;
+// The tiled edge kernel's own target (it holds a group of levels in flight).
+#ifndef FESOMTRA_MIN_WAVES_EDGE
+#define FESOMTRA_MIN_WAVES_EDGE FESOMTRA_MIN_WAVES_TILED
+#endif
+template <bool TILED, int KB>
+constexpr int kEdgeWaves = TILED ? FESOMTRA_MIN_WAVES_EDGE : kMinWaves<KB>;
 
 // ---------------------------------------------------------------------------------------------
 // Work decomposition: one thread owns `lc` consecutive levels of one column for one lane of the
@@ -148,6 +154,13 @@ static_assert(kEdgePipe == 1 || kEdgePipe == 2, "edge pipeline depth 1 or 2");
 #endif
 // With two levels in flight, stage both in LDS behind one barrier (four LDS buffers).
 constexpr bool kEdgePair = FESOMTRA_EDGE_PAIR != 0;
+#ifndef FESOMTRA_EDGE_GROUP
+#define FESOMTRA_EDGE_GROUP 2
+#endif
+// Levels staged behind one barrier in the paired form (2 or 4); the next group's loads are in
+// flight while a group is computed (2 * group LDS buffers).
+constexpr int kEdgeGroup = FESOMTRA_EDGE_GROUP;
+static_assert(kEdgeGroup == 2 || kEdgeGroup == 4, "edge level group of 2 or 4");
 
 #ifndef FESOMTRA_EDGE_UNTILED_PF
 #define FESOMTRA_EDGE_UNTILED_PF 1
@@ -168,14 +181,26 @@ struct ArrLd {
     __device__ __forceinline__ C operator()(size_t i) const { return v[i]; }
 };
 
+// f(std::integral_constant<int, r>) for r = 0 .. N-1 (register sets addressed statically).
+template <int N, typename F, int... R>
+__device__ __forceinline__ void each_of_impl(F &&f, std::integer_sequence<int, R...>) {
+    (f(std::integral_constant<int, R>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void each_of(F &&f) {
+    each_of_impl<N>(f, std::make_integer_sequence<int, N>{});
+}
+
 // SAME (tiled form only): valuesAB is values, so only one field is staged.  GEN: ensembles (each
 // lane reads its member's transport) and cavity meshes.
 template <bool TILED, int HOR, bool FCT, int KB, typename S, typename M, typename C, bool SAME = false, bool GEN = false>
-__global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, int lc) {
-    // two levels in flight only where the registers allow it (one staged field)
-    constexpr int P = SAME ? kEdgePipe : 1;
-    constexpr bool PAIR = TILED && P == 2 && kEdgePair;
-    constexpr int NB = PAIR ? 4 : 2;  // LDS buffers of the staged field, level nz in nz % NB
+__global__ __launch_bounds__(256, (kEdgeWaves<TILED, KB>)) void k_edge(const MeshCol<M> m, const StepPtrs<S, C> p, C num_ord, int lc) {
+    // two levels in flight only where the registers allow it (one staged field); the paired
+    // form stages a group of G levels behind one barrier
+    constexpr bool PAIR = TILED && (SAME ? kEdgePipe : 1) == 2 && kEdgePair;
+    constexpr int G = PAIR ? kEdgeGroup : 1;
+    constexpr int P = PAIR ? G : (SAME ? kEdgePipe : 1);
+    constexpr int NB = PAIR ? 2 * G : 2;  // LDS buffers of the staged field, level nz in nz % NB
     __shared__ C sT[TILED ? NB : 1][TILED ? kHtSlots : 1], sA[TILED && !SAME ? 2 : 1][TILED && !SAME ? kHtSlots : 1];
     EdgeTile<KB, M> et;
     int e, lane, nz0, nz1, lv0, lv1;
@@ -214,7 +239,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
     const LaneLd<KB, C, S> ldT{p.T, lane}, ldA{p.TAB, lane};
     const bool same = TILED ? SAME : p.TAB == p.T;
     C dx = 0, dy = 0, a = 0;
-    int shared_beg = 1, shared_end = 0, u[3] = {0, 0, 0}, d[3] = {0, 0, 0}, nb1 = 0, nb2 = 0;
+    int shared_beg = 1, shared_end = 0, u[3] = {0, 0, 0}, d[3] = {0, 0, 0}, nb1 = 0, nb2 = 0, mgb1 = 0, mgb2 = 0;
     C gu[6], gd[6];
     if (HOR != HOR_UPW1) {
         dx = (C)cld(m.edxdy, 2 * (size_t)e);
@@ -242,6 +267,10 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
             nb1 = cld(m.nbl, n1);
             nb2 = cld(m.nbl, n2);
         }
+        if (p.mg) {
+            mgb1 = cld(m.mg_base, n1);
+            mgb2 = cld(m.mg_base, n2);
+        }
     }
     // tile slots of n1, n2, u0..u2, d0..d2 (two 16-bit slots per word)
     int sl[4] = {0, 0, 0, 0};
@@ -263,7 +292,11 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
         });
         qn[r] = (act && nz <= nz1) ? (C)cld(m.q, mb.at((size_t)(nz - 1) * Ep + e)) : C(0);
     };
-    if constexpr (TILED) {
+    if constexpr (PAIR) {
+        each_of<G>([&](auto r) {
+            if (lv0 + r <= lv1) fetch(lv0 + r, r);
+        });
+    } else if constexpr (TILED) {
         fetch(lv0, std::integral_constant<int, 0>{});
         if (P > 1 && lv0 < lv1) fetch(lv0 + 1, std::integral_constant<int, P - 1>{});
     }
@@ -295,28 +328,32 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
             }
         }
     };
-    // the edge flux of level nz (staged in LDS when TILED; qcur is its transport; pf: the level's
-    // prefetched values in the untiled prefetch form)
-    auto compute = [&](int nz, C qcur, const EdgePf<C> *pf) {
+    // the edge flux of level nz (LDS: from the values staged for the tiled level loop, qcur its
+    // transport; otherwise from memory, or pf: the level's prefetched values in the untiled
+    // prefetch form).  The tiled level loop leaves the node-mean levels to a pass after it.
+    auto compute = [&](int nz, C qcur, const EdgePf<C> *pf, auto lds_tag) {
+        constexpr bool LDS = decltype(lds_tag)::value;
         if (TILED && (!act || nz > nz1 || nz < nz0)) return;
+        const bool tri = HOR == HOR_UPW1 || (nz >= shared_beg && nz <= shared_end);
+        if (LDS && !tri) return;
         const int buf = nz % NB;
         const int k = nz - 1;
-        const C v = TILED ? qcur : (UPF ? pf->q : (C)cld(m.q, mb.at((size_t)k * Ep + e)));
+        const C v = LDS ? qcur : (UPF ? pf->q : (C)cld(m.q, mb.at((size_t)k * Ep + e)));
         const size_t o = (size_t)k * Np;
         const size_t ie = ((size_t)k * Ep + e) * KB + lane;
         const SlotLd<KB, C> lsT{sT[TILED ? buf : 0], lane},
             lsA{same ? sT[TILED ? buf : 0] : sA[TILED && !SAME ? (nz & 1) : 0], lane};
         C t1 = 0, t2 = 0;
         if (FCT || same) {
-            t1 = TILED ? lsT(slot(0)) : (UPF ? pf->t[0] : ldT(o + n1));
-            t2 = TILED ? lsT(slot(1)) : (UPF ? pf->t[1] : ldT(o + n2));
+            t1 = LDS ? lsT(slot(0)) : (UPF ? pf->t[0] : ldT(o + n1));
+            t2 = LDS ? lsT(slot(1)) : (UPF ? pf->t[1] : ldT(o + n2));
         }
-        const C a1 = same ? t1 : (TILED ? lsA(slot(0)) : (UPF ? pf->a[0] : ldA(o + n1)));
-        const C a2 = same ? t2 : (TILED ? lsA(slot(1)) : (UPF ? pf->a[1] : ldA(o + n2)));
+        const C a1 = same ? t1 : (LDS ? lsA(slot(0)) : (UPF ? pf->a[0] : ldA(o + n1)));
+        const C a2 = same ? t2 : (LDS ? lsA(slot(1)) : (UPF ? pf->a[1] : ldA(o + n2)));
         C gr[4] = {0, 0, 0, 0}, c1 = 1, c2 = 1;
         if (HOR != HOR_UPW1) {
-            if (nz >= shared_beg && nz <= shared_end) {
-                if constexpr (TILED) {
+            if (tri) {
+                if constexpr (LDS) {
                     gr[0] = tr_xy<C>(gu, lsA, 0, slot(2), slot(3), slot(4));
                     gr[1] = tr_xy<C>(gd, lsA, 0, slot(5), slot(6), slot(7));
                     gr[2] = tr_xy<C>(gu + 3, lsA, 0, slot(2), slot(3), slot(4));
@@ -327,12 +364,18 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
                     gr[1] = tr_xy<C>(gd, la, 0, 5, 6, 7);
                     gr[2] = tr_xy<C>(gu + 3, la, 0, 2, 3, 4);
                     gr[3] = tr_xy<C>(gd + 3, la, 0, 5, 6, 7);
-                } else {
+                } else if constexpr (!TILED) {  // (the tiled form reads its triangles from LDS only)
                     gr[0] = tr_xy<C>(gu, ldA, o, u[0], u[1], u[2]);
                     gr[1] = tr_xy<C>(gd, ldA, o, d[0], d[1], d[2]);
                     gr[2] = tr_xy<C>(gu + 3, ldA, o, u[0], u[1], u[2]);
                     gr[3] = tr_xy<C>(gd + 3, ldA, o, d[0], d[1], d[2]);
                 }
+            } else if (p.mg) {  // evaluated by k_mgrad for this pass
+                const size_t i1 = (size_t)(mgb1 + nz) * 2 * KB + lane, i2 = (size_t)(mgb2 + nz) * 2 * KB + lane;
+                gr[0] = p.mg[i1];
+                gr[2] = p.mg[i1 + KB];
+                gr[1] = p.mg[i2];
+                gr[3] = p.mg[i2 + KB];
             } else {
                 node_mean_grad<C>(m, ldA, n1, nz, gr[0], gr[2]);
                 node_mean_grad<C>(m, ldA, n2, nz, gr[1], gr[3]);
@@ -366,21 +409,24 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
             __syncthreads();
             if (nz + P <= lv1) fetch(nz + P, set);
         }
-        compute(nz, qcur, nullptr);
+        compute(nz, qcur, nullptr, std::integral_constant<bool, TILED>{});
     };
     if constexpr (PAIR) {
-        // levels nz and nz + 1 staged together behind one barrier; the buffers of the pair
+        // levels nz .. nz + G - 1 staged together behind one barrier; the buffers of the group
         // before are free once every thread has passed it
-        for (int nz = lv0; nz <= lv1; nz += 2) {
-            const bool two = nz + 1 <= lv1;
-            stage(nz, std::integral_constant<int, 0>{});
-            if (two) stage(nz + 1, std::integral_constant<int, 1>{});
-            const C q0 = qn[0], q1 = qn[1];
+        for (int nz = lv0; nz <= lv1; nz += G) {
+            C qc[G];
+            each_of<G>([&](auto r) {
+                if (nz + r <= lv1) stage(nz + r, r);
+                qc[r] = qn[r];
+            });
             __syncthreads();
-            if (nz + 2 <= lv1) fetch(nz + 2, std::integral_constant<int, 0>{});
-            if (nz + 3 <= lv1) fetch(nz + 3, std::integral_constant<int, 1>{});
-            compute(nz, q0, nullptr);
-            if (two) compute(nz + 1, q1, nullptr);
+            each_of<G>([&](auto r) {
+                if (nz + G + r <= lv1) fetch(nz + G + r, r);
+            });
+            each_of<G>([&](auto r) {
+                if (nz + r <= lv1) compute(nz + r, qc[r], nullptr, std::true_type{});
+            });
         }
     } else if constexpr (UPF) {
         // two register sets, alternating: level nz + 1 loads while level nz computes
@@ -389,10 +435,10 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
         for (int nz = lv0; nz <= lv1; nz += 2) {
             const bool two = nz + 1 <= lv1;
             if (two) pf_load(nz + 1, f1);
-            compute(nz, C(0), &f0);
+            compute(nz, C(0), &f0, std::false_type{});
             if (two) {
                 if (nz + 2 <= lv1) pf_load(nz + 2, f0);
-                compute(nz + 1, C(0), &f1);
+                compute(nz + 1, C(0), &f1, std::false_type{});
             }
         }
     } else {
@@ -403,6 +449,36 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_edge(const MeshCol
                 if (nz + 1 <= lv1) level(nz + 1, std::integral_constant<int, P - 1>{});
         }
     }
+    if constexpr (TILED && HOR != HOR_UPW1) {
+        // the node-mean levels (outside the triangles' shared wet range), from memory and free of
+        // the tile's barriers: the top levels below a cavity roof and the bottom levels
+        if (act) {
+            const std::false_type direct{};
+            if (shared_end < shared_beg) {
+                for (int nz = nz0; nz <= nz1; ++nz) compute(nz, C(0), nullptr, direct);
+            } else {
+                for (int nz = nz0; nz <= min(nz1, shared_beg - 1); ++nz) compute(nz, C(0), nullptr, direct);
+                for (int nz = max(nz0, shared_end + 1); nz <= nz1; ++nz) compute(nz, C(0), nullptr, direct);
+            }
+        }
+    }
+}
+
+// Node-mean gradients of valuesAB for the entries MeshCol::mg_* lists (lib/oce_muscl_adv.F90:394-409,
+// the same evaluation as node_mean_grad in the edge kernel, so the stored values are those it
+// would compute): one thread per entry and lane.  Separate from the edge kernel, whose tiles
+// would otherwise wait at their next barrier for these few serial gathers.
+template <int KB, typename S, typename M, typename C>
+__global__ __launch_bounds__(256) void k_mgrad(const MeshCol<M> m, const StepPtrs<S, C> p) {
+    int i, lane;
+    column_lane<KB>(i, lane);
+    if (i >= m.mg_n || !lane_on(p.mask, lane)) return;
+    const int n = cld(m.mg_node, i), nz = cld(m.mg_lev, i);
+    const LaneLd<KB, C, S> ldA{p.TAB, lane};
+    C gx, gy;
+    node_mean_grad<C>(m, ldA, n, nz, gx, gy);
+    p.mg[(size_t)i * 2 * KB + lane] = gx;
+    p.mg[((size_t)i * 2 + 1) * KB + lane] = gy;
 }
 
 // Edge transport of the current velocity, q(k, e) = sum over the edge's elements wet at level k
@@ -1611,6 +1687,10 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
                gf = grid(m.N, cfg.lc_final);
     const C ph = (C)sc.ph, pv = (C)sc.pv;
     MARK(0);
+    if (sc.hor != HOR_UPW1 && p.mg && m.mg_n > 0) {
+        k_mgrad<KB><<<dim3(grid_for((size_t)m.mg_n * KB, 256)), 256, 0, s>>>(m, p);
+        LAUNCH_CHECK();
+    }
     switch (sc.hor * 2 + (sc.fct ? 1 : 0)) {
     case HOR_UPW1 * 2 + 0: edge_kernel<HOR_UPW1, false, KB, GEN>(ge, s, m, p, ph, cfg.lc_edge); break;
     case HOR_UPW1 * 2 + 1: edge_kernel<HOR_UPW1, true, KB, GEN>(ge, s, m, p, ph, cfg.lc_edge); break;

@@ -30,6 +30,7 @@
 #include <stdint.h>
 
 #include <type_traits>
+#include <utility>
 
 namespace fesomtra {
 
@@ -174,6 +175,12 @@ struct MeshCol {
     // nmem members, uv member-major ([nmem][L][Tp][2]), w / we / wi and q member-interleaved
     // ([nl][Np][nmem], [L][Ep][nmem]); each lane of a tracer block reads its own member's values.
     int nmem;
+    // Node-mean gradient entries: a MUSCL / MFCT edge at a level outside its up/down triangles'
+    // shared wet range takes the area-weighted mean gradients of both end nodes.  Entry
+    // mg_base[n] + nz is node n at level nz, for the levels any of its edges asks for; mg_node /
+    // mg_lev list the mg_n entries.  Evaluated once per pass by k_mgrad (StepPtrs::mg).
+    int mg_n;
+    const int *mg_node, *mg_lev, *mg_base;
     const double *dtn;                    // (N) time-step factor per node, or null (all 1)
     // Ice-shelf cavities (layers above a top are not wet); all null without them (tops are 1)
     const int *ulev_n;                    // (N) ulevels_nod2D, the node's top layer
@@ -244,6 +251,8 @@ struct StepPtrs {
                               // memory (advanced by a kernel per step, so captured graphs replay
                               // with new seeds); null: `seed` itself is the counter
     const int *lane_mem;      // ensembles: the member of each lane of this block (KB entries), or null
+    C *mg;                    // [mg_n][2][KB] node-mean gradients (x, y) of valuesAB, or null
+                              // (then the edge kernel evaluates them where it needs them)
 };
 
 // Velocity member of `lane` (GEN: kernels for ensembles, per-node time steps or cavities; the
