@@ -9,7 +9,7 @@ accumulation, update, limiter ratios and sums.
 
     python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--workload mesh1000]
     python bench.py --workload multimesh32          # 32 uneven meshes, one HIP graph per run
-    python bench.py --precision-study               # 1000-step FP32/FP16/BF16 errors vs FP64
+    python bench.py --precision-study               # 1000-step FP32/FP16/BF16 errors vs FP64, all modes
 
 Prints one JSON line.  ``value`` = tracers x wet node-levels x steps / wall time of the timed
 steps (inputs resident in HBM).  ``roofline`` relates the algorithmic bytes of one step (every
@@ -48,12 +48,13 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(mesh, k_tracers: int = 1, s_tr: int = 8, s_mesh: int = 8) -> int:
+def algorithmic_bytes(mesh, k_tracers: int = 1, s_tr: int = 8, s_mesh: int = 8, members: int = 1) -> int:
     """Bytes one pass must move at least: every distinct input once, every output once.
 
     K tracers in and out, per node-layer mesh arrays (hnode, hnode_new, areasvol, Z_3d_n),
     per node-interface arrays (area, w, zbar_3d_n), per element-layer uv(2) and helem, and the
-    per-edge / per-element / per-node connectivity, geometry and level integers.
+    per-edge / per-element / per-node connectivity, geometry and level integers.  An ensemble of
+    `members` velocities adds each further member's uv and w.
     """
     N, T, E = mesh.nod2D, mesh.elem2D, mesh.edge2D
     NL = mesh.n_node_levels
@@ -61,6 +62,7 @@ def algorithmic_bytes(mesh, k_tracers: int = 1, s_tr: int = 8, s_mesh: int = 8) 
     adj = float(np.mean(mesh.nod_in_elem2D_num))
     return int(k_tracers * 2 * s_tr * NL
                + s_mesh * (4 * NL + 3 * (NL + N) + 3 * TL)
+               + (members - 1) * s_mesh * (2 * TL + NL + N)
                + E * (16 + 6 * s_mesh + 8)
                + T * (20 + 8 * s_mesh)
                + N * (20 + 4 * adj))
@@ -191,7 +193,8 @@ def make_case(name: str, nsteps=None):
 
 def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, lc=None,
         stage_steps: int = 10, compare=None, keep: bool = False, stochastic: bool = True,
-        tune: bool = True, fused: bool = False, compensated: bool = False, members=None):
+        tune: bool = True, fused: bool = False, compensated: bool = False, members=None,
+        fused_limiter: bool = False):
     """Advance `case` on the GPU; returns throughput, device time, per-stage times, drift.
 
     keep: also return the final values (list per tracer); compare: final values of a reference
@@ -207,6 +210,7 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
     t.set_graph(graph)
     t.set_rounding(stochastic)
     t.set_compensated(compensated)
+    t.set_fused_limiter(fused_limiter)
     if lc:
         t.set_launch_config(lc)
     if members:
@@ -225,6 +229,7 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
         tuned = autotune.tune(t, case, prec, use_cache=False)
         log(f"autotuned levels per thread {tuned['launch_config']} (stage ms {[round(x, 3) for x in tuned['stage_ms']]})")
     width = t.block_width()
+    limiter = t.fused_limiter_active()
     v0 = [t.stats(i + 1)["volume_sum"] for i in range(case.ntracers)]
     if warmup:
         t.advect_steps(warmup, case.dt)
@@ -269,35 +274,43 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
     return {"value": updates / wall, "wall_s": wall, "dev_ms": dev_ms, "stage_ms": stage_ms,
             "passes_timed": passes, "block_width": width, "drift": drift, "launch_config": cfg,
             "values": kept, "error": err, "autotune": tuned, "layout": layout, "qflux_ms": qflux_ms,
-            "step_ms": step_ms, "steps_timed": nsteps_timed}
+            "step_ms": step_ms, "steps_timed": nsteps_timed, "fused_limiter": limiter}
 
 
-def roofline(case, prec: int, r: dict, name: str):
+def roofline(case, prec: int, r: dict, name: str, members: int = 1):
     """The step as the unit: algorithmic bytes of one step of every tracer over the mean HIP-event
     time of the whole step (k_qflux and every pass of the step)."""
     s_tr, s_m = PREC_BYTES[prec]
     k = case.ntracers
-    B = algorithmic_bytes(case.mesh, k, s_tr, s_m)
+    B = algorithmic_bytes(case.mesh, k, s_tr, s_m, members)
     step_ms = r["step_ms"]
     achieved = B / (step_ms * 1e-3) / 1e9
     st = r["stage_ms"]
     passes = max(1, round(r["passes_timed"] / max(r["steps_timed"], 1)))
     traffic, source = traffic_from_profiles(name)
     fused = not (st[0] or st[1] or st[2])
+    lim = r.get("fused_limiter", False)
+    if fused:
+        label, stages = "fused pass (k_tile)", {"k_tile": round(st[3], 4)}
+    elif lim:
+        label = "pass (k_edge + k_lo + k_limit)"
+        stages = {"k_edge": round(st[0], 4), "k_lo": round(st[1], 4), "k_limit": round(st[2] + st[3], 4)}
+    else:
+        label = "pass (k_edge + k_lo + k_bounds + k_final)"
+        stages = {"k_edge": round(st[0], 4), "k_lo": round(st[1], 4), "k_bounds": round(st[2], 4),
+                  "k_final": round(st[3], 4)}
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": source,
-            "kernel": ("one step: k_qflux + " + (f"{passes} x " if passes > 1 else "") +
-                       ("fused pass (k_tile)" if fused else "pass (k_edge + k_lo + k_bounds + k_final)") +
+            "kernel": ("one step: k_qflux + " + (f"{passes} x " if passes > 1 else "") + label +
                        f" over {k} tracer(s) in blocks of {r['block_width']}, {PREC_NAMES[prec]}"),
             "algorithmic_bytes_per_step": B, "tracers_per_step": k, "step_ms": round(step_ms, 4),
-            "stage_ms": ({"k_qflux": round(r["qflux_ms"], 4), "k_tile": round(st[3], 4)} if fused else
-                         {"k_qflux": round(r["qflux_ms"], 4), "k_edge": round(st[0], 4), "k_lo": round(st[1], 4),
-                          "k_bounds": round(st[2], 4), "k_final": round(st[3], 4)}),
+            "stage_ms": {"k_qflux": round(r["qflux_ms"], 4), **stages},
             "passes_per_step": passes}
 
 
-PROFILE_ARGS = "--steps 5 --warmup 1 --no-cpu-baseline --no-batched --no-dropin --no-multimesh --no-autotune"
+PROFILE_ARGS = ("--steps 5 --warmup 1 --no-cpu-baseline --no-batched --no-dropin --no-multimesh --no-precision --no-autotune "
+                "--lc 48,48,48,48")
 
 
 def _rocprof_rows(d: Path, suffix: str):
@@ -350,7 +363,11 @@ def rocprof_summary(args):
     ck = max(cf, key=cf.get)  # the copy kernel dominates the calibration run
     fscale = copy_bytes / (cf[ck] * 1024.0)
     wscale = copy_bytes / (cw[max(cw, key=cw.get)] * 1024.0)
-    step = ["fesomtra::k_qflux", "fesomtra::k_edge", "fesomtra::k_lo", "fesomtra::k_bounds", "fesomtra::k_final"]
+    # every kernel the step launches (the transport, the gradient pre-pass, the pass kernels, the
+    # rounding counter); relayouts and statistics belong to set-up and read-back
+    step = ["fesomtra::k_qflux", "fesomtra::k_mgrad", "fesomtra::k_edge", "fesomtra::k_lo", "fesomtra::k_vimpl",
+            "fesomtra::k_cavity_aux", "fesomtra::k_bounds", "fesomtra::k_final", "fesomtra::k_limit", "fesomtra::k_vert",
+            "fesomtra::k_copy", "fesomtra::k_seed", "fesomtra::k_ab2"]
     lines = [f"# {w}: rocprofv3 kernel trace and HBM counters of bench.py ({PROFILE_ARGS})", "",
              "Commands (each a run of its own; the copy calibration moves 1 GiB):",
              f"    rocprofv3 --kernel-trace --stats -f csv -d <dir>/k -o k -- python -u bench.py --workload {w} {PROFILE_ARGS}",
@@ -394,30 +411,33 @@ def rocprof_summary(args):
     print("\n".join(lines))
 
 
-def dropin_timing(case, kb: int, steps: int = 10):
-    """The reference's per-tracer call sequence on device arrays (INTEGRATION.md section 1 with
-    the device-pointer entry points), per step and tracer: tracer_set_values_device (the caller's
-    current values, reference order), tracer_zero_tendencies, tracer_do_oce_adv_tra and
-    tracer_get_tendencies_device; the caller's own update of its arrays is not included.  Returns
-    ms per step (all tracers) over `steps` steps after one warm-up step."""
+def dropin_timing(case, kb: int, steps: int = 10, keep_work: bool = False):
+    """The reference's per-tracer do_oce_adv_tra call sequence on device arrays in reference order
+    (the Fortran call sequence of the integration notes with the device-pointer entry points), per
+    step and tracer: tracer_set_values_device (the caller's current values), tracer_zero_tendencies,
+    tracer_do_oce_adv_tra and tracer_get_tendencies_device; the caller's own update of its arrays
+    is not included.  keep_work: the call also stores the limited fluxes for tracer_get_work.
+    Returns ms per step (all tracers) over `steps` steps after one warm-up step."""
     import ctypes
     import torch
     from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
     t = FESOMTracerAdvection()
     t.set_block_width(kb)
     t.setup_case(case)
+    t.set_keep_work(keep_work)
     vals = [torch.from_numpy(np.ascontiguousarray(case.tracers[i])).cuda() for i in range(case.ntracers)]
     dh = torch.empty_like(vals[0])
     dv = torch.empty_like(vals[0])
     L = t.lib
     nl, N = case.mesh.nl, case.mesh.nod2D
+    P = ctypes.c_void_p
 
     def step():
         for i in range(case.ntracers):
-            L.tracer_set_values_device(i + 1, ctypes.c_void_p(vals[i].data_ptr()), nl, N)
+            L.tracer_set_values_device(i + 1, P(vals[i].data_ptr()), nl, N)
             L.tracer_zero_tendencies()
             L.tracer_do_oce_adv_tra(case.dt, i + 1)
-            L.tracer_get_tendencies_device(ctypes.c_void_p(dh.data_ptr()), ctypes.c_void_p(dv.data_ptr()), nl - 1, N)
+            L.tracer_get_tendencies_device(P(dh.data_ptr()), P(dv.data_ptr()), nl - 1, N)
 
     torch.cuda.synchronize()
     step()
@@ -428,6 +448,7 @@ def dropin_timing(case, kb: int, steps: int = 10):
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
     w = t.block_width()
+    t.set_keep_work(True)
     t.set_block_width(0)
     t.cleanup()
     return ms, w
@@ -441,29 +462,37 @@ def describe(case):
             f"{sc.hor}/{sc.ver}/{sc.lim}")
 
 
-def precision_study(args):
-    """FP32 arithmetic with FP32 / FP16 / BF16 storage for many steps against the FP64 GPU path
-    (itself bit-identical to the reference restatement) on the 1000x1000x48 workload."""
+STUDY_ALL = ((1, True, False), (1, False, False), (1, False, True), (2, True, False), (2, False, False),
+             (2, False, True), (3, True, False), (3, False, False), (3, False, True))
+# the bench's own C2 line: each storage with the remainders kept (compensated) and with stochastic
+# rounding
+STUDY_BENCH = ((1, False, True), (2, False, True), (3, False, True), (1, True, False), (2, True, False),
+               (3, True, False))
+
+
+def precision_study(case, steps: int, modes=STUDY_ALL):
+    """FP32 arithmetic with FP32 / FP16 / BF16 storage for `steps` steps against the FP64 GPU path
+    (itself bit-identical to the reference restatement) on `case`: per mode (precision,
+    stochastic rounding, compensated storage) and tracer the rel L2 / Linf over the wet layers and
+    the drift of the volume-weighted sum."""
     from fesom2_tracer_dwarf_amd import configs
     from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
-    case = configs.mesh1000(args.study_steps)
     wet = configs.wet_mask(case.mesh)
-    log(f"precision study on {describe(case)}, {args.study_steps} steps")
+    log(f"precision study on {describe(case)}, {steps} steps")
     results = {}
     ref = None
-    for prec, sr, comp in ((0, True, False), (1, True, False), (1, False, False), (1, False, True),
-                           (2, True, False), (2, False, False), (2, False, True), (3, True, False),
-                           (3, False, False), (3, False, True)):
+    for prec, sr, comp in ((0, True, False),) + tuple(modes):
         t = FESOMTracerAdvection()
         t.set_precision(prec)
         t.set_rounding(sr)
         t.set_compensated(comp)
         t.setup_case(case)
         v0 = [t.stats(i + 1)["volume_sum"] for i in range(case.ntracers)]
-        ms = t.advect_steps(args.study_steps, case.dt)
+        ms = t.advect_steps(steps, case.dt)
         vals = t.get_all_values()
         v1 = [t.stats(i + 1)["volume_sum"] for i in range(case.ntracers)]
         t.set_precision(0)
+        t.set_compensated(False)
         t.cleanup()
         if prec == 0:
             ref = vals
@@ -479,8 +508,8 @@ def precision_study(args):
                                                           (", stochastic rounding" if sr else ", nearest")))
         results[key] = {"device_ms": ms, "tracers": per}
         log(f"{key}: {ms:.1f} ms, {per}")
-    print(json.dumps({"study": "precision", "workload": describe(case), "steps": args.study_steps,
-                      "reference": "FP64 GPU path", "results": results}), flush=True)
+    return {"study": "precision", "workload": describe(case), "steps": steps,
+            "reference": "FP64 GPU path", "results": results}
 
 
 def main():
@@ -497,6 +526,8 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay the timed steps as one HIP graph")
     ap.add_argument("--fused", type=int, default=0, choices=[0, 1],
                     help="fused single-pass kernel for FP64 FCT tracers, one tracer per block")
+    ap.add_argument("--fused-limiter", type=int, default=0, choices=[0, 1],
+                    help="limiter bounds, ratios and update in one kernel where it applies (FP64, width <= 4)")
     ap.add_argument("--lc", default="", help="levels per thread: edge,lo,bounds,final")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch configuration instead of autotuning it")
@@ -505,6 +536,8 @@ def main():
                     help="skip the mixed-precision batched companion run")
     ap.add_argument("--no-dropin", action="store_true",
                     help="skip the timing of the per-tracer do_oce_adv_tra call sequence")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the 1000-step FP32/FP16/BF16 storage companion (C2) on the main mesh")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the ensemble companion run (16 velocity members on the batched mesh)")
     ap.add_argument("--no-multimesh", action="store_true",
@@ -542,7 +575,8 @@ def main():
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.precision_study:
-        precision_study(args)
+        from fesom2_tracer_dwarf_amd import configs
+        print(json.dumps(precision_study(configs.mesh1000(args.study_steps), args.study_steps)), flush=True)
         return
 
     defaults = {"mesh1000": (100, 0, 0, False), "mesh500x64": (200, 1, 64, False),
@@ -561,7 +595,7 @@ def main():
     log(f"workload {describe(case)}: N={m.nod2D} T={m.elem2D} E={m.edge2D} N_L={m.n_node_levels} "
         f"(generated in {time.perf_counter() - t0:.1f} s)")
     r = run(case, prec, kb, steps, args.warmup, graph, lc, tune=not args.no_autotune, fused=fused,
-            stochastic=args.rounding == "stochastic")
+            stochastic=args.rounding == "stochastic", fused_limiter=bool(args.fused_limiter))
     log(f"timed {steps} steps: wall {r['wall_s'] * 1e3:.2f} ms, device {r['dev_ms']:.2f} ms, "
         f"{r['value']:.3e} updates/s, stages {r['stage_ms']}")
     roof = roofline(case, prec, r, args.workload)
@@ -618,7 +652,8 @@ def main():
                     "ms_per_step": re_["wall_s"] * 1e3 / args.batched_steps,
                     "vs_shared_velocity": re_["value"] / batched["value"],
                     "volume_sum_rel_drift": re_["drift"], "step_ms_kernels": re_["step_ms"],
-                    "qflux_ms": re_["qflux_ms"]}
+                    "qflux_ms": re_["qflux_ms"],
+                    "roofline": roofline(ecase, 1, re_, "mesh500_ensemble16x4", members=16)}
         log(f"ensemble: {re_['value']:.3e} updates/s ({ensemble['vs_shared_velocity']:.2f}x the shared "
             f"velocity), drift {re_['drift']:.2e}")
 
@@ -636,19 +671,28 @@ def main():
                                "100 steps replayed as one HIP graph",
                      "steps": 100, "value": rm["value"], "unit": "node-level updates/s",
                      "ms_per_step": rm["wall_s"] * 1e3 / 100, "volume_sum_rel_drift": rm["drift"],
-                     "step_ms_kernels": rm["step_ms"], "block_width": rm["block_width"]}
+                     "step_ms_kernels": rm["step_ms"], "block_width": rm["block_width"],
+                     "roofline": roofline(mcase, 1, rm, "multimesh32")}
         log(f"multi-mesh: {rm['value']:.3e} updates/s")
+
+    precision = None
+    if args.workload == "mesh1000" and not args.no_precision:
+        precision = precision_study(case, args.study_steps, STUDY_BENCH)
 
     dropin = None
     if args.workload == "mesh1000" and not args.no_dropin:
         dropin = {"sequence": "per tracer: tracer_set_values_device, tracer_zero_tendencies, "
                               "tracer_do_oce_adv_tra, tracer_get_tendencies_device (device arrays in "
-                              "reference order)", "tracers": case.ntracers}
+                              "reference order); tendencies only (tracer_set_keep_work(0)) unless "
+                              "the key says keep_work", "tracers": case.ntracers}
         for kbw in (1, 0):
             ms, w = dropin_timing(case, kbw)
             dropin[f"ms_per_step_block_width_{w}"] = ms
             log(f"drop-in sequence, block width {w}: {ms:.3f} ms per step")
         best = min(v for k, v in dropin.items() if k.startswith("ms_per_step"))
+        ms, w = dropin_timing(case, 1, keep_work=True)
+        dropin[f"keep_work_ms_per_step_block_width_{w}"] = ms
+        log(f"drop-in sequence keeping the limited fluxes, block width {w}: {ms:.3f} ms per step")
         dropin["vs_advect_steps"] = best / (r["wall_s"] * 1e3 / steps)
 
     cpu = None
@@ -672,7 +716,7 @@ def main():
                    "edges": m.edge2D, "node_levels_per_tracer": m.n_node_levels,
                    "tracers": case.ntracers, "block_width": r["block_width"],
                    "launch_config": list(r["launch_config"]), "hip_graph": graph, "fused": fused,
-                   "dt_s": case.dt, "parallelism": "1 GPU"},
+                   "fused_limiter": r["fused_limiter"], "dt_s": case.dt, "parallelism": "1 GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "volume_sum_rel_drift": r["drift"],
@@ -680,6 +724,7 @@ def main():
         "batched": batched,
         "multimesh": multimesh,
         "ensemble": ensemble,
+        "precision": precision,
         "dropin": dropin,
     }
     print(json.dumps(out), flush=True)

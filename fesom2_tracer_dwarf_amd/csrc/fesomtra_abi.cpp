@@ -102,6 +102,8 @@ struct State {
     bool halo_tiles = true;
     int ht_built = 0;  // number of node tiles built for the current block width (0: none)
     int et_built = 0;  // number of edge tiles built for the current block width (0: none)
+    int rt_built = 0;  // node tiles with two-ring halos (0: none, or they did not fit)
+    bool fused_limiter = false;  // limiter bounds and update in one kernel where the tiles allow it
     TileMesh tm{};
     std::vector<void *> dT2;    // second tracer buffer per block (output of a fused step)
     std::vector<char> t2_sync;  // dT2 holds the same below-bottom values as dT
@@ -116,6 +118,8 @@ struct State {
     void *mg = nullptr;   // node-mean gradients of a pass (MeshCol::mg_* entries x 2 x kb)
     double *acc_h = nullptr, *acc_v = nullptr;
     void *dbg_h = nullptr, *dbg_v = nullptr;
+    bool keep_work = true;  // the drop-in stores the limited fluxes for tracer_get_work
+    bool work_kept = false;  // the last drop-in call stored them
     void *tri = nullptr;      // (c', t') of the implicit vertical solve (allocated with wsplit)
     void *cqv = nullptr;      // cavity meshes: the limiter's element-bound rows above element tops
     int cq_rows = 0;          // number of those rows (levels above the slot elements' tops)
@@ -227,6 +231,7 @@ void release_device(State &s) {
     s.tm = TileMesh{};
     s.ht_built = 0;
     s.et_built = 0;
+    s.rt_built = 0;
     s.adfh = s.LO = s.Rp = s.Rm = s.adfv = nullptr;
     s.acc_h = s.acc_v = nullptr;
     s.dbg_h = s.dbg_v = nullptr;
@@ -526,10 +531,66 @@ int upload_mesh(State &s, MeshCol<M> &m, const std::vector<int> &ints_n1, const 
 // curve); its halo is the ascending list of far ends of its nodes' edges outside the tile.  A
 // tile's workgroup stages two slots per thread (its column and one halo node), so the halo may
 // hold at most tc nodes; meshes with a larger halo anywhere run the untiled kernels.
+// Second halo ring of the node tiles, for the fused limiter kernel: per tile the one-ring halo
+// (hptr / halo of build_halo_tiles, same order) followed by the nodes adjacent to it outside the
+// tile and the first ring, sorted; per one-ring node the slots of its CSR entries' far nodes.
+// Tiles whose two rings exceed 256 / kb nodes leave the fused kernel off (rt_built = 0).
+int build_ring2_tiles(State &s, const std::vector<int> &hptr, const std::vector<int> &halo) {
+    MeshCol<double> &m = s.m64;
+    m.rt_n = 0;
+    s.rt_built = 0;
+    const int tc = 256 / s.kb, N = s.N, nt = s.ht_built;
+    std::vector<int> rptr(nt + 1, 0), rhalo, eoff, rloc, rlmax(nt, 0), r2;
+    for (int t = 0; t < nt; ++t) {
+        const int a0 = t * tc, a1 = std::min(N, a0 + tc);
+        const int *r1b = halo.data() + hptr[t], *r1e = halo.data() + hptr[t + 1];
+        const int nr1 = (int)(r1e - r1b);
+        auto in_r1 = [&](int o) { return std::binary_search(r1b, r1e, o); };
+        r2.clear();
+        for (const int *h = r1b; h != r1e; ++h)
+            for (int i = s.h_ptr[*h]; i < s.h_ptr[*h + 1]; ++i) {
+                const int o = s.h_oth[i];
+                if ((o < a0 || o >= a1) && !in_r1(o)) r2.push_back(o);
+            }
+        std::sort(r2.begin(), r2.end());
+        r2.erase(std::unique(r2.begin(), r2.end()), r2.end());
+        if (nr1 + (int)r2.size() > tc) return 0;  // does not fit: the unfused kernels run
+        int lm = 0;
+        for (int n = a0; n < a1; ++n) lm = std::max(lm, s.h_nlev[n] - 1);
+        for (const int *h = r1b; h != r1e; ++h) {
+            lm = std::max(lm, s.h_nlev[*h] - 1);
+            eoff.push_back((int)rloc.size());
+            for (int i = s.h_ptr[*h]; i < s.h_ptr[*h + 1]; ++i) {
+                const int o = s.h_oth[i];
+                int sl;
+                if (o >= a0 && o < a1) sl = o - a0;
+                else if (in_r1(o)) sl = tc + (int)(std::lower_bound(r1b, r1e, o) - r1b);
+                else sl = tc + nr1 + (int)(std::lower_bound(r2.begin(), r2.end(), o) - r2.begin());
+                rloc.push_back(sl);
+            }
+        }
+        rlmax[t] = lm;
+        rhalo.insert(rhalo.end(), r1b, r1e);
+        rhalo.insert(rhalo.end(), r2.begin(), r2.end());
+        rptr[t + 1] = (int)rhalo.size();
+    }
+    if (rhalo.empty()) rhalo.push_back(0);
+    if (eoff.empty()) eoff.push_back(0);
+    if (rloc.empty()) rloc.push_back(0);
+    if (upload(s, &m.rt_hptr, rptr) || upload(s, &m.rt_halo, rhalo) || upload(s, &m.rt_eoff, eoff) ||
+        upload(s, &m.rt_loc, rloc) || upload(s, &m.rt_lmax, rlmax))
+        return -1;
+    s.rt_built = nt;
+    m.rt_n = (s.halo_tiles && s.fused_limiter) ? nt : 0;
+    return 0;
+}
+
 int build_halo_tiles(State &s) {
     MeshCol<double> &m = s.m64;
     m.ht_n = m.ht_kb = 0;
+    m.rt_n = 0;
     s.ht_built = 0;
+    s.rt_built = 0;
     if (s.kb > 4 || s.N < 1) return 0;
     const int tc = 256 / s.kb, N = s.N, nt = (N + tc - 1) / tc;
     std::vector<int> hptr(nt + 1, 0), halo, loc(s.h_oth.size()), lmax(nt, 0), hl;
@@ -559,7 +620,7 @@ int build_halo_tiles(State &s) {
     s.ht_built = nt;
     m.ht_kb = s.kb;
     m.ht_n = s.halo_tiles ? nt : 0;
-    return 0;
+    return build_ring2_tiles(s, hptr, halo);
 }
 
 // Edge tiles of the edge-flux kernel for a tracer block of width kb <= 4: tile t owns the
@@ -1158,17 +1219,30 @@ StepPtrs<S_, C_> make_ptrs(State &s, int blk, unsigned long long mask, bool drop
     if (dropin) {
         p.acc_h = s.acc_h;
         p.acc_v = s.acc_v;
-        p.dbg_h = static_cast<C_ *>(s.dbg_h);
-        p.dbg_v = static_cast<C_ *>(s.dbg_v);
+        if (s.keep_work) {
+            p.dbg_h = static_cast<C_ *>(s.dbg_h);
+            p.dbg_v = static_cast<C_ *>(s.dbg_v);
+        }
     }
     return p;
 }
 
+// tin / tout (fused limiter steps): the block's values are read from tin and the update written to
+// tout (the two buffers of the block, alternating by step); null: the primary buffer in place.
 hipError_t launch_block(State &s, int blk, unsigned long long mask, const SchemeSel &sc, double dt, bool dropin,
-                        hipEvent_t *ev, uint32_t seed = 0, int ab_parity = 0) {
+                        hipEvent_t *ev, uint32_t seed = 0, int ab_parity = 0, void *tin = nullptr,
+                        void *tout = nullptr) {
     // cavity meshes: the limiter reads edge gradients filled from this tracer's valuesAB where the
     // scheme fills them (MUSCL / MFCT) and always for the drop-in, whose caller fills them
     SchemeArgs a{sc.hor, sc.ver, sc.fct, sc.ph, sc.pv, dt, s.wsplit && sc.fct, sc.hor != HOR_UPW1 || dropin};
+    if (tin) {
+        if (s.prec != PREC_F64) return hipErrorInvalidValue;
+        StepPtrs<double, double> p = make_ptrs<double, double>(s, blk, mask, dropin, seed, ab_parity);
+        if (!s.tab_on[blk]) p.TAB = static_cast<const double *>(tin);
+        p.T = static_cast<double *>(tin);
+        p.Tout = static_cast<double *>(tout);
+        return launch_step<double, double, double>(s.m64, p, s.kb, a, s.cfg, s.stream, ev);
+    }
     switch (s.prec) {
     case PREC_F64: return launch_step<double, double, double>(s.m64, make_ptrs<double, double>(s, blk, mask, dropin, seed, ab_parity), s.kb, a, s.cfg, s.stream, ev);
     case PREC_F32: return launch_step<float, double, float>(s.m64, make_ptrs<float, float>(s, blk, mask, dropin, seed, ab_parity), s.kb, a, s.cfg, s.stream, ev);
@@ -1322,6 +1396,25 @@ int build_tiles(State &s) {
     return 0;
 }
 
+// One launch sequence per (block, scheme): the lanes of a block that share a scheme.
+struct Launch {
+    int blk;
+    unsigned long long mask;
+    SchemeSel sc;
+};
+
+// The fused limiter (k_limit) takes the bounds, ratios and update of every pass of a step: FP64,
+// node tiles with two-ring halos built for the block width, the FCT limiter on every pass, no
+// cavities, per-node time steps, ensembles or AB2 time levels.  The update then goes to the
+// block's second buffer (steps alternate between the two).
+bool limiter_applies(const State &s, const std::vector<Launch> &plan) {
+    if (!s.fused_limiter || s.m64.rt_n < 1 || s.m64.ht_kb != s.kb || s.kb > 4 || s.prec != PREC_F64) return false;
+    if (!s.node_dt.empty() || s.nmem > 1 || s.m64.ulev_n || s.ab_mode == 2 || s.fused) return false;
+    for (const Launch &l : plan)
+        if (!l.sc.fct) return false;
+    return !plan.empty();
+}
+
 // The fused path applies to FP64 blocks of one tracer with the FCT limiter (without the
 // implicit vertical split, with valuesAB := values).
 bool fused_applies(const State &s, const SchemeSel &sc) {
@@ -1337,8 +1430,9 @@ int ensure_wsplit(State &s) {
     return dalloc_bytes(s, &s.tri, (size_t)(s.nl - 1) * s.Np * s.kb * sizeof(double2));
 }
 
-int ensure_fused(State &s) {
-    if (!s.tiles_built && build_tiles(s)) return -1;
+// Second value buffers of the blocks (output of fused steps), with the values below the bottom
+// copied once from the primary ones.
+int ensure_t2(State &s) {
     const size_t bytes = (size_t)(s.nl - 1) * s.Np * s.kb * storage_bytes(s.prec);
     if (s.dT2.size() != s.dT.size()) {
         s.dT2.assign(s.dT.size(), nullptr);
@@ -1346,12 +1440,17 @@ int ensure_fused(State &s) {
     }
     for (size_t b = 0; b < s.dT.size(); ++b) {
         if (!s.dT2[b] && dalloc_bytes(s, &s.dT2[b], bytes)) return -1;
-        if (!s.t2_sync[b]) {  // values below the bottom are never written by a step: copy them once
+        if (!s.t2_sync[b]) {
             HIPCHK(hipMemcpyAsync(s.dT2[b], s.dT[b], bytes, hipMemcpyDeviceToDevice, s.stream));
             s.t2_sync[b] = 1;
         }
     }
     return 0;
+}
+
+int ensure_fused(State &s) {
+    if (!s.tiles_built && build_tiles(s)) return -1;
+    return ensure_t2(s);
 }
 
 // Marks the second buffers stale after the primary storage was written from outside a step.
@@ -1375,12 +1474,6 @@ hipError_t launch_ab2_block(State &s, int b) {
     }
 }
 
-// One launch sequence per (block, scheme): the lanes of a block that share a scheme.
-struct Launch {
-    int blk;
-    unsigned long long mask;
-    SchemeSel sc;
-};
 std::vector<Launch> launch_plan(const State &s) {
     std::vector<Launch> plan;
     for (int b = 0; b < n_blocks(s); ++b) {
@@ -1414,6 +1507,7 @@ hipError_t run_steps(State &s, const std::vector<Launch> &plan, int nsteps, doub
                      bool capture = false) {
     std::vector<char> where(s.dT.size(), 0);
     const bool ab2 = s.ab_mode == 2;
+    const bool lim = limiter_applies(s, plan);  // every pass through k_limit, values alternating
     const size_t nev = step_events(plan.size());
     for (int st = 0; st < nsteps; ++st) {
         hipEvent_t *sev = evs ? &evs[(size_t)st * nev] : nullptr;
@@ -1439,11 +1533,18 @@ hipError_t run_steps(State &s, const std::vector<Launch> &plan, int nsteps, doub
                 e = launch_tile_step(s.m64, s.tm, in, out, l.sc.hor, l.sc.ver, l.sc.ph, l.sc.pv, dt, s.stream);
                 if (e == hipSuccess && ev) e = hipEventRecord(ev[4], s.stream);
                 where[l.blk] ^= 1;
+            } else if (lim) {
+                // every pass of the block in this step reads one buffer and writes the other
+                void *in = where[l.blk] ? s.dT2[l.blk] : s.dT[l.blk];
+                void *out = where[l.blk] ? s.dT[l.blk] : s.dT2[l.blk];
+                e = launch_block(s, l.blk, l.mask, l.sc, dt, false, ev, seed, 0, in, out);
             } else {
                 e = launch_block(s, l.blk, l.mask, l.sc, dt, false, ev, seed, ab2 ? (st & 1) : 0);
             }
             if (e != hipSuccess) return e;
         }
+        if (lim)
+            for (size_t b = 0; b < where.size(); ++b) where[b] ^= 1;
     }
     const size_t bytes = (size_t)(s.nl - 1) * s.Np * s.kb * storage_bytes(s.prec);
     for (size_t b = 0; b < where.size(); ++b)
@@ -1502,6 +1603,7 @@ int advect(State &s, int nsteps, double dt) {
                         "gradients of another tracer in the reference (not reproduced); advance them separately");
     }
     if (s.fused && s.prec == PREC_F64 && s.kb == 1 && ensure_fused(s)) return -1;
+    if (limiter_applies(s, plan) && ensure_t2(s)) return -1;
     if (!plan.empty()) s.last_fct = plan.back().sc.fct;
     if (nsteps > 0) s.dropin_valid = false;  // the work arrays now hold this call's intermediates
     for (const Launch &l : plan)
@@ -1909,7 +2011,7 @@ int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D) {
     hipEvent_t k0 = s.ev0, k1 = s.ev1;
     const int prec = s.prec, kbr = s.kb_req;
     const bool graph = s.use_graph, sr = s.stochastic, fused = s.fused, ht = s.halo_tiles, ws = s.wsplit;
-    const bool comp = s.compensated;
+    const bool comp = s.compensated, flim = s.fused_limiter, keepw = s.keep_work;
     const int abm = s.ab_mode;
     const double abe = s.ab_eps;
     const uint64_t gen = s.gen;
@@ -1924,6 +2026,8 @@ int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D) {
     s.use_graph = graph;
     s.stochastic = sr;
     s.compensated = comp;
+    s.fused_limiter = flim;
+    s.keep_work = keepw;
     s.fused = fused;
     s.halo_tiles = ht;
     s.wsplit = ws;
@@ -2417,6 +2521,7 @@ int tracer_set_halo_tiles(int enable) {
     s.halo_tiles = enable != 0;
     s.m64.ht_n = s.halo_tiles ? s.ht_built : 0;
     s.m64.et_n = s.halo_tiles ? s.et_built : 0;
+    s.m64.rt_n = (s.halo_tiles && s.fused_limiter) ? s.rt_built : 0;
     s.gen++;
     return 0;
 }
@@ -2438,6 +2543,26 @@ int tracer_set_compensated(int enable) {
     s.compensated = on;
     s.gen++;
     if (s.initialized) return ensure_device(s);
+    return 0;
+}
+
+int tracer_set_fused_limiter(int enable) {
+    State &s = S();
+    s.fused_limiter = enable != 0;
+    s.m64.rt_n = (s.halo_tiles && s.fused_limiter) ? s.rt_built : 0;
+    s.gen++;
+    return 0;
+}
+
+int tracer_get_fused_limiter(int *active) {
+    if (!active) return fail("tracer_get_fused_limiter: null pointer");
+    State &s = S();
+    *active = (s.initialized && s.on_device && limiter_applies(s, launch_plan(s))) ? 1 : 0;
+    return 0;
+}
+
+int tracer_set_keep_work(int keep) {
+    S().keep_work = keep != 0;
     return 0;
 }
 
@@ -2515,6 +2640,7 @@ int tracer_do_oce_adv_tra(double dt, int tracer_id) {
     s.last_fct = s.schemes[id].fct;
     s.last_lane = lane;
     s.dropin_valid = true;
+    s.work_kept = s.keep_work;
     return 0;
 }
 
@@ -2539,6 +2665,9 @@ int tracer_get_work(const char *name, double *out, size_t count) {
         if (count != L * s.N) return fail("tracer_get_work(fct_LO): count must be %zu", L * s.N);
         return download_work(s, s.LO, (int)L, s.Np, s.perm_n, out, cb, kb, lane);
     }
+    if (s.last_fct && !s.work_kept && (!strcmp(name, "adv_flux_hor") || !strcmp(name, "adv_flux_ver")))
+        return fail("tracer_get_work(%s): the last tracer_do_oce_adv_tra call did not keep the limited fluxes "
+                    "(tracer_set_keep_work(0))", name);
     if (!strcmp(name, "adv_flux_hor")) {
         if (count != L * s.E) return fail("tracer_get_work(adv_flux_hor): count must be %zu", L * s.E);
         if (s.last_fct && s.dbg_h) return download_work(s, s.dbg_h, (int)L, s.Ep, s.perm_e, out, cb, 1, 0);

@@ -484,13 +484,18 @@ __global__ __launch_bounds__(256) void k_mgrad(const MeshCol<M> m, const StepPtr
 // Edge transport of the current velocity, q(k, e) = sum over the edge's elements wet at level k
 // of (+-) (u dy - v dx) helem (lib/oce_adv_tra_hor.F90:166-253, segments A to E: below a cavity
 // roof only the element already wet contributes), in double, once per step for all tracers.  One
-// thread per edge walks its levels; levels where neither element is wet are not read.  An
-// ensemble's (ENS) member is blockIdx.y: it reads that member's uv and writes its interleaved q.
+// thread per edge walks its levels; levels where neither element is wet are not read.  In an
+// ensemble (ENS) consecutive threads take the members of one edge: each reads its member's uv
+// and a wavefront stores contiguous runs of the member-interleaved q.
 template <typename M, bool ENS = false>
 __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
-    const int e = column_index();
+    int e = column_index();
+    Member mb{0, 1};
+    if constexpr (ENS) {
+        mb = Member{e % m.nmem, m.nmem};
+        e /= m.nmem;
+    }
     if (e >= m.E) return;
-    const Member mb = ENS ? Member{(int)blockIdx.y, m.nmem} : Member{0, 1};
     const M *__restrict__ uv = ENS ? m.uv + (size_t)mb.m * (size_t)m.L * m.Tp * 2 : m.uv;
     const int4 q = cld(m.edge_ne, e);
     const int el1 = q.z, el2 = q.w;
@@ -1522,49 +1527,398 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
 }
 
 // ---------------------------------------------------------------------------------------------
+// Fused limiter: bounds, ratios, limiting, flux divergence and update in one kernel
+// (lib/oce_adv_tra_fct.F90:96-471, lib/oce_adv_tra_driver.F90:298-450, src/fesom_analytic.F90:419-437)
+// for tracer blocks of width KB <= 4 on node tiles with two-ring halos, FP64, meshes without
+// cavities, per-node time steps or ensembles, one level chunk per column.
+//
+// The update of a tile node needs the limiter ratios R of its one-ring neighbours, which need
+// the cluster values max(LO, T) / min(LO, T) of their own neighbours: the tile's second ring.  The
+// kernel stages those values for the tile and both rings, forms R for the tile nodes and for the
+// one-ring nodes (the latter a second time, as the neighbouring tiles do too), stages R in LDS and
+// limits and updates the tile nodes, so R never goes through memory and the antidiffusive fluxes,
+// the vertical fluxes, LO and the volumes are read once for both phases.  Sweep step j: stage the
+// cluster values of level j and the ratios of layer j - 2; one barrier; fetch level j + 1; form
+// the ratios of layer j - 1 (clusters j - 2 .. j, as k_bounds); limit and update layer j - 2 (R of
+// layers j - 3 .. j - 1 of the column, R of layer j - 2 at the neighbours, as k_final).  Every
+// expression and summation order is that of k_bounds and k_final, so results are bit-identical.
+#ifndef FESOMTRA_MIN_WAVES_LIMIT
+#define FESOMTRA_MIN_WAVES_LIMIT 2
+#endif
+template <int KB, typename M>
+struct Ring2Tile {
+    static constexpr int TC = 256 / KB;
+    int t, n0, lmax, lstage, nr1, hb1;
+    int snode[2];
+    bool son[2];
+    __device__ __forceinline__ bool init(const MeshCol<M> &m) {
+        t = xcd_block();
+        if (t >= m.rt_n) return false;
+        n0 = t * TC;
+        lmax = cld(m.ht_lmax, t);
+        lstage = min(lmax + 1, cld(m.rt_lmax, t));
+        hb1 = cld(m.ht_hptr, t);
+        nr1 = cld(m.ht_hptr, t + 1) - hb1;
+        const int hb = cld(m.rt_hptr, t), H = cld(m.rt_hptr, t + 1) - hb;
+        const int col = (int)threadIdx.x / KB;
+        snode[0] = min(n0 + col, m.N - 1);
+        son[0] = true;
+        son[1] = col < H;
+        snode[1] = son[1] ? cld(m.rt_halo, hb + col) : 0;
+        return true;
+    }
+    template <typename F>
+    __device__ __forceinline__ void each(F &&f) const {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            if (son[i]) f(i, snode[i]);
+    }
+};
+
+template <int KB, typename S, typename M, typename C>
+__global__ __launch_bounds__(256, FESOMTRA_MIN_WAVES_LIMIT) void k_limit(const MeshCol<M> m, const StepPtrs<S, C> p, double dt) {
+    static_assert(std::is_same<C, double>::value && std::is_same<S, double>::value, "FP64 only");
+    using D = double;
+    constexpr int TC = 256 / KB, NC = kDegCacheFor<KB>;
+    __shared__ D shi[2][kHtSlots], slo[2][kHtSlots];  // cluster values of a level, by level parity
+    __shared__ C sRp[2][kHtSlots], sRm[2][kHtSlots];  // R of a layer (tile and first ring), by layer parity
+    // the first-ring node's sweep window, one slot per thread (registers are short): cluster
+    // bounds of levels j - 2 .. j by j % 3, its LO at j - 1, j by j % 2
+    __shared__ D whx[3][256], whn[3][256], whl[2][256];
+    Ring2Tile<KB, M> rt;
+    if (!rt.init(m)) return;
+    const int col = (int)threadIdx.x / KB, lane = (int)threadIdx.x % KB;
+    const size_t Np = m.Np, Ep = m.Ep;
+    auto X = [&](size_t cell) { return cell * KB + lane; };
+    // this thread's tile column
+    const int n = rt.n0 + col;
+    bool act = n < m.N && lane_on(p.mask, lane);
+    const int L = act ? cld(m.nlev_n, n) - 1 : 0;
+    act = act && L >= 1;
+    const int dry_from = act ? cld(m.nlev_min, n) : 0;
+    NodeEdges<true, NC, false> ne;
+    if (act) load_edges(m, n, ne);
+    else no_edges(ne);
+    // this thread's first-ring node (halo slot col), whose ratios the tile's updates need for the
+    // layers of the tile (its entries' far nodes are tile slots, rt_loc)
+    const int h = rt.snode[1];
+    bool acth = rt.son[1] && col < rt.nr1 && lane_on(p.mask, lane);
+    const int Lh = acth ? cld(m.nlev_n, h) - 1 : 0;
+    const int Lhr = min(Lh, rt.lmax);  // layers of h whose ratios are formed
+    acth = acth && Lhr >= 1;
+    const int hdry = acth ? cld(m.nlev_min, h) : 0;
+    const int heb = acth ? cld(m.ne_ptr, h) : 0;
+    const int hdeg = acth ? cld(m.ne_ptr, h + 1) - heb : 0;
+    const int hoff = acth ? cld(m.rt_eoff, rt.hb1 + col) : 0;
+    // h's entries: the first NC in registers (edge, and deepest level | slot << 16), the rest
+    // from memory; h_edges(nz, f) calls f(entry, edge, second, slot) in reference order
+    int he2[NC], htop[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        const bool on = i < hdeg;
+        he2[i] = on ? cld(m.ne_e2, heb + i) : 0;
+        htop[i] = on ? (cld(m.rt_loc, hoff + i) << 16) | cld(m.ne_top, heb + i) : 0;
+    }
+    auto h_edges = [&](int nz, auto &&f) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i)
+            if (i < hdeg && nz <= (htop[i] & 0xffff)) f(i, he2[i] >> 1, (he2[i] & 1) != 0, htop[i] >> 16);
+        for (int i = NC; i < hdeg; ++i) {
+            if (nz > cld(m.ne_top, heb + i)) continue;
+            const int e2 = cld(m.ne_e2, heb + i);
+            f(i, e2 >> 1, (e2 & 1) != 0, cld(m.rt_loc, hoff + i));
+        }
+    };
+    const LaneLd<KB, D, S> ldT{p.T, lane};
+    // cluster bound of a column at level j from the staged values (k_bounds' cluster)
+    auto clamp_dry = [&](int j, int dfrom, D &vmax, D &vmin) {
+        if (j >= dfrom) {
+            vmax = cmax(vmax, D(-1.0e3));
+            vmin = cmin(vmin, D(1.0e3));
+        }
+    };
+    // ratios of layer nz with cluster bounds (hi, lw), low-order value lo, vertical fluxes av0 /
+    // av1 and volume V; the column's antidiffusive edge fluxes come through use_edges (k_bounds'
+    // finish)
+    auto ratios = [&](D hi, D lw, D lo, C av0, C av1, double V, auto &&use_edges, C &rp, C &rm) {
+        const D qmax = hi - lo, qmin = lw - lo;
+        double pp = 0.0 + ((double)cmax(C(0), av0) + (double)cmax(C(0), -av1));
+        double pm = 0.0 + ((double)cmin(C(0), av0) + (double)cmin(C(0), -av1));
+        use_edges([&](C a, bool second) {
+            if (second) a = -a;
+            pp = pp + (double)cmax(C(0), a);
+            pm = pm + (double)cmin(C(0), a);
+        });
+        const double fp = pp * dt / V + 1e-16;
+        const double fm = pm * dt / V - 1e-16;
+        const double r_p = (double)qmax / fp, r_m = (double)qmin / fm;
+        rp = (C)(r_p < 1.0 ? r_p : 1.0);
+        rm = (C)(r_m < 1.0 ? r_m : 1.0);
+    };
+    // limited flux at interface j from R at layers j - 1 (rp0, rm0) and j (rp1, rm1) (k_final)
+    auto limit = [&](C a, int j, C rp0, C rm0, C rp1, C rm1) -> C {
+        C ae = 1;
+        if (j == 1) ae = a >= C(0) ? cmin(ae, rp1) : cmin(ae, rm1);
+        else if (a >= C(0)) {
+            ae = cmin(ae, rm0);
+            ae = cmin(ae, rp1);
+        } else {
+            ae = cmin(ae, rp0);
+            ae = cmin(ae, rm1);
+        }
+        return ae * a;
+    };
+    // staging registers of the next level: cluster values and LO of both slots
+    D hi_r[2] = {0, 0}, lo_r[2] = {0, 0}, lo_st[2] = {0, 0};
+    auto fetch = [&](int j) {
+        rt.each([&](int i, int node) {
+            const size_t c = (size_t)(j - 1) * Np + node;
+            const D t = ldT(c), l = p.LO[X(c)];
+            hi_r[i] = cmax(l, t);
+            lo_r[i] = cmin(l, t);
+            lo_st[i] = l;
+        });
+    };
+    // sweep state: clusters of levels j - 2, j - 1, j and LO at j - 1, j (tile column, first-ring
+    // node); the column's inputs of layer j - 2 kept from its ratio step for the update (vertical
+    // fluxes, volume, LO; its antidiffusive fluxes are read again, from the cache)
+    D mx2 = 0, mn2 = 0, mx1 = 0, mn1 = 0, mx0 = 0, mn0 = 0, lo1 = 0, lo0 = 0;
+    C rpB = 0, rmB = 0;  // R of layer j - 2 of the column (formed in the previous step)
+    C hrp = 0, hrm = 0;                                    // first-ring node's R of layer j - 1
+    C f_k[NC], av0_k = 0, av1_k = 0, av_bot = 0, hav_bot = 0, topf = 0;
+    double V_k = 0.0;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) f_k[i] = 0;
+    D lo_k = 0;
+    if (rt.lstage >= 1) fetch(1);
+    for (int j = 1; j <= rt.lmax + 2; ++j) {
+        const int buf = j & 1, rbuf = j & 1;  // R of layer j - 2 goes to parity (j - 2) & 1 = j & 1
+        D lo_j = 0, hlo_j = 0;
+        if (j <= rt.lstage) {
+            rt.each([&](int i, int) {
+                shi[buf][threadIdx.x + 256 * i] = hi_r[i];
+                slo[buf][threadIdx.x + 256 * i] = lo_r[i];
+            });
+            lo_j = lo_st[0];
+            hlo_j = lo_st[1];
+        }
+        if (j >= 3) {
+            // R of layer j - 2: the tile column's (slot col) and the first-ring node's (slot TC + col)
+            sRp[rbuf][threadIdx.x] = rpB;
+            sRm[rbuf][threadIdx.x] = rmB;
+            if (col < rt.nr1) {
+                sRp[rbuf][threadIdx.x + 256] = hrp;
+                sRm[rbuf][threadIdx.x + 256] = hrm;
+            }
+        }
+        __syncthreads();
+        if (j < rt.lstage) fetch(j + 1);
+        // ---- ratios of layer j - 1 ----
+        const int nz = j - 1;
+        mx2 = mx1; mn2 = mn1; mx1 = mx0; mn1 = mn0; lo1 = lo0;
+        C f_n[NC], av0_n = 0, av1_n = 0, rpN = 0, rmN = 0;
+        double V_n = 0.0;
+        const bool fin = act && nz >= 1 && nz <= L;
+        if (fin) {
+            const size_t cell = (size_t)(nz - 1) * Np + n;
+            V_n = (double)cld(m.areasvol, cell);
+            av0_n = nz == 1 ? p.adfv[X(cell)] : av_bot;
+            av1_n = p.adfv[X(cell + Np)];
+            const size_t eo = (size_t)(nz - 1) * Ep;
+            edges_pre(ne, nz, [&](int e, bool, int) { return p.adfh[X(eo + e)]; }, f_n);
+        }
+        // the first-ring node's inputs of layer nz and the update's of layer j - 2, loaded
+        // together with the column's (one memory round per level)
+        const bool hfin = acth && nz >= 1 && nz <= Lhr;
+        C hf[NC], ha0 = 0, ha1 = 0;
+        double hV = 1.0;
+        if (hfin) {
+            const size_t cell = (size_t)(nz - 1) * Np + h;
+            const size_t eo = (size_t)(nz - 1) * Ep;
+#pragma unroll
+            for (int i = 0; i < NC; ++i)
+                hf[i] = (i < hdeg && nz <= (htop[i] & 0xffff)) ? p.adfh[X(eo + (he2[i] >> 1))] : C(0);
+            hV = (double)cld(m.areasvol, cell);
+            ha0 = nz == 1 ? p.adfv[X(cell)] : hav_bot;
+            ha1 = p.adfv[X(cell + Np)];
+        }
+        const int nf = j - 2;
+        const bool upd = act && nf >= 1 && nf <= L;
+        double u_hn = 1.0, u_hh = 0.0, u_t = 0.0, u_dv = 0.0, u_dh = 0.0;
+        if (upd) {
+            const size_t cell = (size_t)(nf - 1) * Np + n;
+            u_hn = (double)cld(m.hnode_new, cell);
+            u_hh = (double)cld(m.hnode, cell);
+            u_t = (double)ldc<C>(p.T, X(cell));
+            if (p.acc_h) {
+                u_dv = p.acc_v[cell];
+                u_dh = p.acc_h[cell];
+            }
+        }
+        if (act && j <= L) {
+            mx0 = shi[buf][threadIdx.x];
+            mn0 = slo[buf][threadIdx.x];
+            for_edges(m, ne, j, [&](int, bool, int oth) {
+                mx0 = cmax(mx0, shi[buf][oth * KB + lane]);
+                mn0 = cmin(mn0, slo[buf][oth * KB + lane]);
+            });
+            clamp_dry(j, dry_from, mx0, mn0);
+            lo0 = lo_j;
+        }
+        if (fin) {
+            const bool own = nz == 1 || nz == L;
+            const size_t eo = (size_t)(nz - 1) * Ep;
+            ratios(own ? mx1 : cmax(cmax(mx2, mx1), mx0), own ? mn1 : cmin(cmin(mn2, mn1), mn0), lo1, av0_n, av1_n,
+                   V_n,
+                   [&](auto &&g) {
+                       edges_use(m, ne, nz, f_n, [&](int e, bool, int) { return p.adfh[X(eo + e)]; },
+                                 [&](C a, int, bool second, int) { g(a, second); });
+                   },
+                   rpN, rmN);
+            av_bot = av1_n;
+        }
+        // the first-ring node's ratios of layer nz (its fluxes and volume straight from memory)
+        if (acth) {
+            const int w0 = j % 3, w1 = (j + 2) % 3, w2 = (j + 1) % 3;  // levels j, j - 1, j - 2
+            if (j <= Lh && j <= rt.lstage) {
+                D hx0 = shi[buf][threadIdx.x + 256], hn0 = slo[buf][threadIdx.x + 256];
+                h_edges(j, [&](int, int, bool, int sl) {
+                    hx0 = cmax(hx0, shi[buf][sl * KB + lane]);
+                    hn0 = cmin(hn0, slo[buf][sl * KB + lane]);
+                });
+                clamp_dry(j, hdry, hx0, hn0);
+                whx[w0][threadIdx.x] = hx0;
+                whn[w0][threadIdx.x] = hn0;
+                whl[j & 1][threadIdx.x] = hlo_j;
+            }
+            if (hfin) {
+                const size_t eo = (size_t)(nz - 1) * Ep;
+                const double V = hV;
+                const C a0 = ha0, a1 = ha1;
+                const bool own = nz == 1 || nz == Lh;
+                const D hx1 = whx[w1][threadIdx.x], hn1 = whn[w1][threadIdx.x];
+                const D hi = own ? hx1 : cmax(cmax(whx[w2][threadIdx.x], hx1), whx[w0][threadIdx.x]);
+                const D lw = own ? hn1 : cmin(cmin(whn[w2][threadIdx.x], hn1), whn[w0][threadIdx.x]);
+                ratios(hi, lw, whl[(j - 1) & 1][threadIdx.x], a0, a1, V,
+                       [&](auto &&g) {
+                           h_edges(nz, [&](int i, int e, bool second, int) {
+                               g(i < NC ? hf[i < NC ? i : 0] : p.adfh[X(eo + e)], second);
+                           });
+                       },
+                       hrp, hrm);
+                hav_bot = a1;
+            }
+        }
+        // ---- limiting and update of layer j - 2 ----
+        if (upd) {
+            const size_t cell = (size_t)(nf - 1) * Np + n;
+            const size_t eo = (size_t)(nf - 1) * Ep;
+            const int fb = nf & 1;
+            // R of this column: layer nf (rpB), nf + 1 (rpN, formed above); the top interface's
+            // limited flux was formed with R of nf - 1 by the previous update (topf)
+            if (nf == 1) {
+                topf = limit(av0_k, 1, C(0), C(0), rpB, rmB);
+                if (p.dbg_v) p.dbg_v[cell] = topf;
+            }
+            C bot;
+            if (nf + 1 <= L) {
+                bot = limit(av1_k, nf + 1, rpB, rmB, rpN, rmN);
+            } else {
+                bot = av1_k;  // bottom interface: never limited (always zero)
+            }
+            if (p.dbg_v) p.dbg_v[cell + Np] = bot;
+            const double V = V_k;
+            const double hn = u_hn, hh = u_hh, tval = u_t;
+            double dv = u_dv;
+            dv = dv - tval * hh + (double)lo_k * hn;
+            dv = dv + ((double)topf - (double)bot) * dt / V;
+            double dh = u_dh;
+            const C rpn = rpB, rmn = rmB;
+            edges_use(m, ne, nf, f_k, [&](int e, bool, int) { return p.adfh[X(eo + e)]; },
+                      [&](C f, int e, bool second, int oth) {
+                          const C rpo = sRp[fb][oth * KB + lane], rmo = sRm[fb][oth * KB + lane];
+                          const C rp_1 = second ? rpo : rpn, rm_1 = second ? rmo : rmn;  // edges(1)
+                          const C rp_2 = second ? rpn : rpo, rm_2 = second ? rmn : rmo;  // edges(2)
+                          C ae = 1;
+                          if (f >= C(0)) {
+                              ae = cmin(ae, rp_1);
+                              ae = cmin(ae, rm_2);
+                          } else {
+                              ae = cmin(ae, rm_1);
+                              ae = cmin(ae, rp_2);
+                          }
+                          f = ae * f;
+                          if (p.dbg_h && !second) p.dbg_h[eo + e] = f;
+                          const double x = (double)f * dt / V;
+                          dh = second ? dh - x : dh + x;
+                      });
+            if (p.acc_h) {
+                p.acc_h[cell] = dh;
+                p.acc_v[cell] = dv;
+            } else {
+                std_(p.Tout, X(cell), tval + (dh + dv) / hn);
+            }
+            topf = bot;
+        }
+        // shift: the ratio step's inputs of layer j - 1 become the update's of the next step
+        if (fin) {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) f_k[i] = f_n[i];
+            av0_k = av0_n;
+            av1_k = av1_n;
+            V_k = V_n;
+            lo_k = lo1;
+        }
+        rpB = rpN;
+        rmB = rmN;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Renumbering / transposition between the reference layout (nl-1, nod2D) in reference order
 // and lane `lane` of a device tracer block [k][Np][kb] in device order.
 // dir 0: ref -> dev, dir 1: dev -> ref.
 // The reference layout is node-major (a node's layers contiguous, nodes in reference order), the
 // device layout level-major (a level's nodes contiguous, in device order): a transpose with a
-// permutation.  A workgroup takes 64 device nodes x 32 layers through LDS, so both sides are
-// read and written in contiguous runs (a node's 32 layers on the reference side, 64 nodes of a
-// level on the device side) whatever the permutation.
-constexpr int kRelayNodes = 64, kRelayLevels = 32;
+// permutation.  A workgroup takes 64 device nodes x up to 64 layers (one chunk covers the columns
+// of most meshes) through LDS, so both sides are read and written in contiguous runs (a node's
+// layers on the reference side, 64 nodes of a level on the device side) whatever the
+// permutation.  The cells of a chunk are numbered densely (no idle threads for a chunk shorter
+// than 64 layers).
+constexpr int kRelayNodes = 64, kRelayLevels = 64;
 template <typename S>
 __global__ __launch_bounds__(256) void k_relayout(const double *__restrict__ ref_in, S *__restrict__ dev_out,
                                                   const S *__restrict__ dev_in, double *__restrict__ ref_out,
                                                   const int *__restrict__ perm, int N, int L, size_t Np, int kb,
                                                   int lane, int dir, float *__restrict__ res) {
-    __shared__ double tile[kRelayNodes][kRelayLevels + 1];
+    __shared__ double tile[kRelayNodes * (kRelayLevels + 1)];
     const int n0 = (int)blockIdx.x * kRelayNodes, k0 = (int)blockIdx.y * kRelayLevels;
-    constexpr int kCells = kRelayNodes * kRelayLevels;
+    const int nn_n = min(kRelayNodes, N - n0), nk = min(kRelayLevels, L - k0);
+    const int cells = nn_n * nk, ld = nk + 1;  // tile row of a node: its nk layers (+1 against bank conflicts)
     // res (compensated storage): the rounding remainder v - stored, in FP32 beside the stored value
     if (dir == 0) {
-        for (int i = (int)threadIdx.x; i < kCells; i += (int)blockDim.x) {  // node-major reads
-            const int nn = i / kRelayLevels, kk = i % kRelayLevels, n = n0 + nn, k = k0 + kk;
-            if (n < N && k < L) tile[nn][kk] = ref_in[(size_t)perm[n] * L + k];
+        for (int i = (int)threadIdx.x; i < cells; i += (int)blockDim.x) {  // node-major reads
+            const int nn = i / nk, kk = i - nn * nk;
+            tile[nn * ld + kk] = ref_in[(size_t)perm[n0 + nn] * L + k0 + kk];
         }
         __syncthreads();
-        for (int i = (int)threadIdx.x; i < kCells; i += (int)blockDim.x) {  // level-major writes
-            const int kk = i / kRelayNodes, nn = i % kRelayNodes, n = n0 + nn, k = k0 + kk;
-            if (n >= N || k >= L) continue;
-            const size_t d = ((size_t)k * Np + n) * kb + lane;
-            const double v = tile[nn][kk];
+        for (int i = (int)threadIdx.x; i < cells; i += (int)blockDim.x) {  // level-major writes
+            const int kk = i / nn_n, nn = i - kk * nn_n;
+            const size_t d = ((size_t)(k0 + kk) * Np + n0 + nn) * kb + lane;
+            const double v = tile[nn * ld + kk];
             std_(dev_out, d, v);
             if (res) res[d] = (float)(v - ldc<double>(dev_out, d));
         }
     } else {
-        for (int i = (int)threadIdx.x; i < kCells; i += (int)blockDim.x) {  // level-major reads
-            const int kk = i / kRelayNodes, nn = i % kRelayNodes, n = n0 + nn, k = k0 + kk;
-            if (n >= N || k >= L) continue;
-            const size_t d = ((size_t)k * Np + n) * kb + lane;
-            tile[nn][kk] = ldc<double>(dev_in, d) + (res ? (double)res[d] : 0.0);
+        for (int i = (int)threadIdx.x; i < cells; i += (int)blockDim.x) {  // level-major reads
+            const int kk = i / nn_n, nn = i - kk * nn_n;
+            const size_t d = ((size_t)(k0 + kk) * Np + n0 + nn) * kb + lane;
+            tile[nn * ld + kk] = ldc<double>(dev_in, d) + (res ? (double)res[d] : 0.0);
         }
         __syncthreads();
-        for (int i = (int)threadIdx.x; i < kCells; i += (int)blockDim.x) {  // node-major writes
-            const int nn = i / kRelayLevels, kk = i % kRelayLevels, n = n0 + nn, k = k0 + kk;
-            if (n < N && k < L) ref_out[(size_t)perm[n] * L + k] = tile[nn][kk];
+        for (int i = (int)threadIdx.x; i < cells; i += (int)blockDim.x) {  // node-major writes
+            const int nn = i / nk, kk = i - nn * nk;
+            ref_out[(size_t)perm[n0 + nn] * L + k0 + kk] = tile[nn * ld + kk];
         }
     }
 }
@@ -1728,12 +2082,25 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
                     k_cavity_aux<KB><<<dim3(grid_for((size_t)m.cq_n * KB, 256)), 256, 0, s>>>(m, p, sc.cavity_fill ? 1 : 0);
                     LAUNCH_CHECK();
                 }
+                // the fused limiter where the caller asked for it: FP64 with the two-ring tiles, an
+                // output buffer apart from the values (or the drop-in's accumulators)
+                bool fused_lim = false;
+                if constexpr (std::is_same<C, double>::value && std::is_same<S, double>::value && !GEN)
+                    fused_lim = m.rt_n > 0 && !p.ab_next && (p.Tout || p.acc_h);
+                if (p.Tout && !fused_lim) return hipErrorInvalidValue;  // the update must reach Tout
                 MARK(2);
-                k_bounds<true, KB, S, M, C, GEN><<<tgrid(cfg.lc_bounds), 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
-                LAUNCH_CHECK();
-                MARK(3);
-                if (p.ab_next) k_final<true, true, KB, S, M, C, true, GEN><<<tgrid(cfg.lc_final), 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
-                else k_final<true, true, KB, S, M, C, false, GEN><<<tgrid(cfg.lc_final), 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+                if (fused_lim) {
+                    if constexpr (std::is_same<C, double>::value && std::is_same<S, double>::value && !GEN) {
+                        MARK(3);
+                        k_limit<KB, S, M, C><<<dim3(gt), 256, 0, s>>>(m, p, sc.dt);
+                    }
+                } else {
+                    k_bounds<true, KB, S, M, C, GEN><<<tgrid(cfg.lc_bounds), 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
+                    LAUNCH_CHECK();
+                    MARK(3);
+                    if (p.ab_next) k_final<true, true, KB, S, M, C, true, GEN><<<tgrid(cfg.lc_final), 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+                    else k_final<true, true, KB, S, M, C, false, GEN><<<tgrid(cfg.lc_final), 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+                }
             }
         } else {
             switch (sc.ver) {
@@ -1795,7 +2162,7 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
 template <typename M>
 hipError_t launch_qflux(const MeshCol<M> &m, hipStream_t s) {
     if (m.L < 1 || m.E < 1) return hipSuccess;
-    if (m.nmem > 1) k_qflux<M, true><<<dim3(grid_for((size_t)m.E, 256), (unsigned)m.nmem), 256, 0, s>>>(m);
+    if (m.nmem > 1) k_qflux<M, true><<<dim3(grid_for((size_t)m.E * m.nmem, 256)), 256, 0, s>>>(m);
     else k_qflux<M><<<dim3(grid_for((size_t)m.E, 256)), 256, 0, s>>>(m);
     return hipGetLastError();
 }
@@ -1810,6 +2177,7 @@ hipError_t launch_step(const MeshCol<M> &m, const StepPtrs<S, C> &p, int kb, con
     // kernels; the others keep dt a scalar, every top at level 1 and one velocity
     const bool ndt = m.dtn != nullptr || m.ulev_n != nullptr || m.nmem > 1;
     if (m.nmem > 1 && !p.lane_mem) return hipErrorInvalidValue;  // members need the lane table
+    if (p.Tout && (!sc.fct || kb > 4 || m.ht_kb != kb || m.rt_n < 1)) return hipErrorInvalidValue;
     switch (kb) {
     case 1: return ndt ? step_kb<1, true>(m, p, sc, cfg, s, ev) : step_kb<1, false>(m, p, sc, cfg, s, ev);
     case 2: return ndt ? step_kb<2, true>(m, p, sc, cfg, s, ev) : step_kb<2, false>(m, p, sc, cfg, s, ev);

@@ -150,6 +150,13 @@ struct MeshCol {
     // corners); et_loc holds per edge the slots of n1, n2, u0, u1, u2, d0, d1, d2 (16 bits each).
     int et_n, et_kb;
     const int *et_loc, *et_nptr, *et_nodes, *et_lmax;  // (E, 4), (et_n + 1), (nodes), (et_n) layers
+    // Two-ring halos of the same node tiles (rt_n = ht_n when every tile's two rings fit its
+    // staging slots, else 0): per tile the one-ring halo (the list of ht_halo, in the same order)
+    // followed by the second ring, at most 256 / ht_kb nodes (rt_hptr, rt_halo); per one-ring
+    // node its CSR entries' far nodes as slots of the tile (rt_eoff: offset of the node's entries in
+    // rt_loc).  The fused limiter kernel forms the limiter ratios of the one-ring nodes itself.
+    int rt_n;
+    const int *rt_hptr, *rt_halo, *rt_eoff, *rt_loc, *rt_lmax;  // (ht_n + 1), (halo), (ring-1 nodes), (entries), (ht_n)
     // nodes
     const int *nlev_n, *nlev_min, *nbl;   // nlevels_nod2D, nlevels_nod2D_min, nboundary_lay
     const int *ne_ptr;                    // CSR node -> edges, reference edge order
@@ -229,7 +236,9 @@ struct Member {
 // Pointers of one step over one tracer block (arrays interleaved by the block width KB).
 template <typename S, typename C>
 struct StepPtrs {
-    S *T;                  // values (updated in place unless accumulating)
+    S *T;                  // values (updated in place unless accumulating, or unless Tout is set)
+    S *Tout;               // fused limiter (k_limit): the updated values go here, T is left as read
+                           // (neighbouring tiles read T of their halos while the update runs), or null
     const S *TAB;          // valuesAB (may equal T)
     C *adfh;               // [L][Ep][KB] antidiffusive (or high-order) edge flux
     C *LO;                 // [L][Np][KB] low-order solution (single precision: its increment LO - T)

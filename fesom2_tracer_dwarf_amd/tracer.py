@@ -290,6 +290,19 @@ class FESOMTracerAdvection:
                   "tracer_get_tendencies")
         return h, v
 
+    def set_fused_limiter(self, enable: bool) -> None:
+        """Limiter bounds, ratios and update in one kernel where it applies (default True)."""
+        abi.check(self.lib.tracer_set_fused_limiter(int(bool(enable))), "tracer_set_fused_limiter")
+
+    def fused_limiter_active(self) -> bool:
+        v = ctypes.c_int(0)
+        abi.check(self.lib.tracer_get_fused_limiter(ctypes.byref(v)), "tracer_get_fused_limiter")
+        return bool(v.value)
+
+    def set_keep_work(self, keep: bool) -> None:
+        """Whether do_oce_adv_tra keeps the limited fluxes for get_work (default True)."""
+        abi.check(self.lib.tracer_set_keep_work(int(bool(keep))), "tracer_set_keep_work")
+
     def get_work(self, name: str) -> np.ndarray:
         shapes = {"fct_LO": (self.nod2D, self.nl - 1), "adv_flux_hor": (self.edge2D, self.nl - 1),
                   "adv_flux_ver": (self.nod2D, self.nl)}

@@ -205,6 +205,24 @@ int tracer_get_stats_by_id(int tracer_id, double *tmin, double *tmax, double *ts
  * tracer without the limiter is not available (-1: the kernels store the high-order minus the
  * upwind flux). */
 int tracer_get_work(const char *name, double *out, size_t count);
+/* Whether tracer_do_oce_adv_tra keeps the limited fluxes "adv_flux_hor" / "adv_flux_ver" for
+ * tracer_get_work (keep = 1, the default, as the reference leaves them in its work arrays), or
+ * only accumulates the tendencies (keep = 0: 0.9 GB fewer stores per call on a 1000x1000x48 mesh;
+ * tracer_get_work then returns -1 for those two arrays; "fct_LO" is always kept). */
+int tracer_set_keep_work(int keep);
+/* Fused limiter (enable = 1; default 0): the limiter bounds and ratios, the limiting and the
+ * update of FP64 tracer blocks of width <= 4 run as one kernel on node tiles with two-ring halos,
+ * which forms the ratios of the tiles' first ring itself instead of storing and re-reading them;
+ * the values then alternate between two buffers per step (the stored values are always the
+ * primary buffer's after a call).  Results are bit-identical either way.  It applies when every
+ * pass has the FCT limiter, there are no cavities, per-node time steps, ensembles or AB2 time
+ * levels and every tile's two rings fit its staging slots (in practice width 1);
+ * tracer_get_fused_limiter reports whether the next steps use it (1) or the two kernels (0).  On
+ * MI355X it is slower than the two kernels (1.83 against 1.26 ms per tracer-step on a
+ * 1000x1000x48 mesh at width 1: its register footprint leaves two waves per SIMD), so it is off
+ * by default. */
+int tracer_set_fused_limiter(int enable);
+int tracer_get_fused_limiter(int *active);
 /* Device-pointer variants of tracer_set_values / tracer_get_values for callers whose tracers
  * already live in HBM (e.g. PyTorch tensors): values(nl-1, nod2D) float64 in reference order. */
 int tracer_set_values_device(int tracer_id, const void *values, int nl, int nod2D);

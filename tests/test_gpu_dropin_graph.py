@@ -97,6 +97,36 @@ def test_work_arrays_refused_after_advect():
     t.cleanup()
 
 
+def test_dropin_without_kept_work_gives_the_same_tendencies(oracle_mod):
+    """tracer_set_keep_work(0): the drop-in skips the limited-flux stores; its tendencies and
+    fct_LO equal the oracle's as before, and tracer_get_work refuses the flux arrays it did not
+    keep until a call keeps them again."""
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    case = dropin_case()
+    o = oracle_mod.Oracle(case)
+    t = FESOMTracerAdvection()
+    t.setup_case(case)
+    say("case bathy16d drop-in with tracer_set_keep_work(0), tracers 1 and 2")
+    for tid in (1, 2):
+        vals = case.tracers[tid - 1]
+        ref = oracle_mod.DropIn(o).run(vals, vals, case.schemes[tid - 1], case.dt)
+        for keep in (False, True):
+            t.set_keep_work(keep)
+            t.zero_tendencies()
+            t.do_oce_adv_tra(case.dt, tid)
+            h, v = t.get_tendencies()
+            assert rel_linf(h, ref["dttf_h"]) <= RTOL and rel_linf(v, ref["dttf_v"]) <= RTOL, (tid, keep)
+            assert rel_linf(t.get_work("fct_LO"), ref["fct_LO"]) <= RTOL
+            if keep:
+                assert rel_linf(t.get_work("adv_flux_hor"), ref["adv_flux_hor"]) <= RTOL
+                assert rel_linf(t.get_work("adv_flux_ver"), ref["adv_flux_ver"]) <= RTOL
+            else:
+                for name in ("adv_flux_hor", "adv_flux_ver"):
+                    with pytest.raises(abi.TracerError):
+                        t.get_work(name)
+    t.cleanup()
+
+
 @pytest.mark.parametrize("prec", [1, 2])
 def test_stochastic_rounding_graph_replays_equal_eager_steps(prec):
     """Single-precision storage with stochastic rounding: 1-step calls replayed from a captured
