@@ -117,6 +117,11 @@ struct State {
     void *LOe = nullptr;  // compensated single precision: remainder of the stored LO increment
     void *mg = nullptr;   // node-mean gradients of a pass (MeshCol::mg_* entries x 2 x kb)
     double *acc_h = nullptr, *acc_v = nullptr;
+    // tracer_zero_tendencies was called and no drop-in call has accumulated since: the next call's
+    // update writes its sums instead of adding to the arrays (the cells it does not write, below
+    // the bottom and above a cavity's top, were zeroed when the arrays were allocated and are never
+    // written)
+    bool acc_zero = false;
     void *dbg_h = nullptr, *dbg_v = nullptr;
     bool keep_work = true;  // the drop-in stores the limited fluxes for tracer_get_work
     bool work_kept = false;  // the last drop-in call stored them
@@ -1219,6 +1224,7 @@ StepPtrs<S_, C_> make_ptrs(State &s, int blk, unsigned long long mask, bool drop
     if (dropin) {
         p.acc_h = s.acc_h;
         p.acc_v = s.acc_v;
+        p.acc_init = s.acc_zero ? 1 : 0;
         if (s.keep_work) {
             p.dbg_h = static_cast<C_ *>(s.dbg_h);
             p.dbg_v = static_cast<C_ *>(s.dbg_v);
@@ -2604,10 +2610,19 @@ int tracer_get_launch_config(int *cfg4) {
 int tracer_zero_tendencies(void) {
     State &s = S();
     if (ensure_device(s) || ensure_dropin_buffers(s)) return -1;
+    // no memset: the next drop-in call starts its sums from zero (acc_zero), and a read-back
+    // before it zeroes the arrays then
+    s.acc_zero = true;
+    return 0;
+}
+
+// The accumulators' contents for a read-back: zeroed now if tracer_zero_tendencies came last.
+static int settle_tendencies(State &s) {
+    if (!s.acc_zero) return 0;
     const size_t NLn = (size_t)(s.nl - 1) * s.Np;
-    // queued on the library's stream ahead of the next drop-in call; no host wait
     HIPCHK(hipMemsetAsync(s.acc_h, 0, NLn * sizeof(double), s.stream));
     HIPCHK(hipMemsetAsync(s.acc_v, 0, NLn * sizeof(double), s.stream));
+    s.acc_zero = false;
     return 0;
 }
 
@@ -2641,6 +2656,7 @@ int tracer_do_oce_adv_tra(double dt, int tracer_id) {
     s.last_lane = lane;
     s.dropin_valid = true;
     s.work_kept = s.keep_work;
+    s.acc_zero = false;  // this call's update wrote (or added to) every wet cell
     return 0;
 }
 
@@ -2648,6 +2664,7 @@ int tracer_get_tendencies(double *dttf_h, double *dttf_v, int nz, int nn) {
     State &s = S();
     if (!s.on_device || !s.acc_h) return fail("tracer_get_tendencies: no tendencies computed yet");
     if (nz != s.nl - 1 || nn != s.N) return fail("tracer_get_tendencies: expected (%d, %d)", s.nl - 1, s.N);
+    if (settle_tendencies(s)) return -1;
     int rc = 0;  // accumulators are double, one tracer
     if (dttf_h) rc |= download_work(s, s.acc_h, nz, s.Np, s.perm_n, dttf_h, 8, 1, 0);
     if (dttf_v) rc |= download_work(s, s.acc_v, nz, s.Np, s.perm_n, dttf_v, 8, 1, 0);
@@ -2727,6 +2744,7 @@ int tracer_get_tendencies_device(void *dttf_h, void *dttf_v, int nz, int nn) {
     State &s = S();
     if (!s.on_device || !s.acc_h) return fail("tracer_get_tendencies_device: no tendencies computed yet");
     if (nz != s.nl - 1 || nn != s.N) return fail("tracer_get_tendencies_device: expected (%d, %d)", s.nl - 1, s.N);
+    if (settle_tendencies(s)) return -1;
     const double *src[2] = {s.acc_h, s.acc_v};
     void *dst[2] = {dttf_h, dttf_v};
     for (int i = 0; i < 2; ++i) {

@@ -1430,7 +1430,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
         const double tval = (double)ldc<C>(p.T, X(cell));
         const C ab = p.adfv[X(cell + Np)];
         C bot;
-        double dv = p.acc_v ? p.acc_v[cell] : 0.0;
+        double dv = (p.acc_v && !p.acc_init) ? p.acc_v[cell] : 0.0;
         if (FCT) {
             // single precision: LO = T + increment (+ the increment's remainder with compensated
             // storage, so that the low-order change of the content is the fluxes' to FP64 round-off)
@@ -1458,7 +1458,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
         }
         const double yV = kDivReuse<KB> ? 1.0 / V : 0.0;
         dv = dv + (kDivReuse<KB> ? div_reuse(((double)top - (double)bot) * dt, V, yV) : ((double)top - (double)bot) * dt / V);
-        double dh = p.acc_h ? p.acc_h[cell] : 0.0;
+        double dh = (p.acc_h && !p.acc_init) ? p.acc_h[cell] : 0.0;
         const C rpn = FCT ? rp0 : C(0), rmn = FCT ? rm0 : C(0);
         const double an = kAddUpw ? ldc<double>(p.TAB, X(cell)) : 0.0;
         auto add_edge = [&](const EdgeVals<C> &v, int e, bool second) {
@@ -1750,7 +1750,7 @@ __global__ __launch_bounds__(256, FESOMTRA_MIN_WAVES_LIMIT) void k_limit(const M
             u_hn = (double)cld(m.hnode_new, cell);
             u_hh = (double)cld(m.hnode, cell);
             u_t = (double)ldc<C>(p.T, X(cell));
-            if (p.acc_h) {
+            if (p.acc_h && !p.acc_init) {
                 u_dv = p.acc_v[cell];
                 u_dh = p.acc_h[cell];
             }

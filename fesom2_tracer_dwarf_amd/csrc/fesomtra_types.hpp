@@ -246,6 +246,8 @@ struct StepPtrs {
     C *Rp, *Rm;            // [L][Np][KB] limiter ratios
     C *adfv;               // [nl][Np][KB] antidiffusive (or high-order) vertical flux
     double *acc_h, *acc_v; // do_oce_adv_tra accumulators [L][Np] of the single selected lane, or null
+    int acc_init;          // the accumulators are logically zero (zeroed since the last call): the
+                           // update starts its sums from 0 instead of reading them
     C *dbg_h, *dbg_v;      // limited fluxes [L][Ep], [nl][Np] of the single selected lane, or null
     double2 *tri;          // [L][Np][KB] (c', t') of the implicit vertical solve, or null
     S *ab_next, *ab_old;   // AB2: valuesAB of the next step and valuesold, written by the update, or null

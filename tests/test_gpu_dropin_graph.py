@@ -55,6 +55,38 @@ def test_dropin_device_pointers_match_oracle(oracle_mod):
     t.cleanup()
 
 
+def test_zeroed_tendencies_and_accumulation(oracle_mod):
+    """tracer_zero_tendencies defers the zeroing to the next call (whose update writes its sums):
+    a read-back right after it gives zeros, a call after it gives the oracle's tendencies, and a
+    second call without zeroing adds its tendencies to the first."""
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    case = dropin_case()
+    o = oracle_mod.Oracle(case)
+    t = FESOMTracerAdvection()
+    t.set_block_width(1)
+    t.setup_case(case)
+    say("case bathy16d zeroed tendencies, then one and two accumulating calls")
+    vals = case.tracers[0]
+    ref = oracle_mod.DropIn(o).run(vals, vals, case.schemes[0], case.dt)
+    t.zero_tendencies()
+    t.do_oce_adv_tra(case.dt, 1)
+    h, v = t.get_tendencies()
+    assert rel_linf(h, ref["dttf_h"]) <= RTOL and rel_linf(v, ref["dttf_v"]) <= RTOL
+    t.zero_tendencies()
+    h0, v0 = t.get_tendencies()
+    assert not np.any(h0) and not np.any(v0)
+    t.do_oce_adv_tra(case.dt, 1)
+    t.do_oce_adv_tra(case.dt, 1)
+    h2, v2 = t.get_tendencies()
+    assert rel_linf(h2, 2 * ref["dttf_h"]) <= RTOL and rel_linf(v2, 2 * ref["dttf_v"]) <= RTOL
+    t.zero_tendencies()
+    t.do_oce_adv_tra(case.dt, 1)
+    h1, v1 = t.get_tendencies()
+    assert np.array_equal(h1, h) and np.array_equal(v1, v)
+    t.set_block_width(0)
+    t.cleanup()
+
+
 def test_dropin_follows_a_new_velocity(oracle_mod):
     """The drop-in reuses the edge transport between calls with the same velocity and recomputes
     it after tracer_set_velocity_elem: a call after a velocity change gives the oracle's result for
