@@ -194,7 +194,7 @@ def make_case(name: str, nsteps=None):
 def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, lc=None,
         stage_steps: int = 10, compare=None, keep: bool = False, stochastic: bool = True,
         tune: bool = True, fused: bool = False, compensated: bool = False, members=None,
-        fused_limiter: bool = False):
+        fused_limiter: bool = False, wsplit: bool = False):
     """Advance `case` on the GPU; returns throughput, device time, per-stage times, drift.
 
     keep: also return the final values (list per tracer); compare: final values of a reference
@@ -217,6 +217,8 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
         t.setup_ensemble(members)
     else:
         t.setup_case(case)
+    if wsplit:
+        t.set_wsplit(True)
     if fused:
         t.set_fused(True)
     tuned, layout = None, None
@@ -267,6 +269,7 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
     cfg = t.launch_config()
     t.set_graph(False)
     t.set_fused(False)
+    t.set_wsplit(False)
     t.set_precision(0)
     t.set_block_width(0)
     t.cleanup()
@@ -309,7 +312,8 @@ def roofline(case, prec: int, r: dict, name: str, members: int = 1):
             "passes_per_step": passes}
 
 
-PROFILE_ARGS = ("--steps 5 --warmup 1 --no-cpu-baseline --no-batched --no-dropin --no-multimesh --no-precision --no-autotune "
+PROFILE_ARGS = ("--steps 5 --warmup 1 --no-cpu-baseline --no-batched --no-dropin --no-multimesh --no-precision "
+                "--no-vertical --no-autotune "
                 "--lc 48,48,48,48")
 
 
@@ -538,6 +542,8 @@ def main():
                     help="skip the timing of the per-tracer do_oce_adv_tra call sequence")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the 1000-step FP32/FP16/BF16 storage companion (C2) on the main mesh")
+    ap.add_argument("--no-vertical", action="store_true",
+                    help="skip the companion run of the main mesh with a vertical velocity")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the ensemble companion run (16 velocity members on the batched mesh)")
     ap.add_argument("--no-multimesh", action="store_true",
@@ -679,6 +685,30 @@ def main():
     if args.workload == "mesh1000" and not args.no_precision:
         precision = precision_study(case, args.study_steps, STUDY_BENCH)
 
+    vertical = None
+    if args.workload == "mesh1000" and not args.no_vertical:
+        # the main workload's mesh with an interior vertical velocity (largest vertical Courant
+        # number 0.3), so the vertical schemes and the vertical limiter do real work
+        from fesom2_tracer_dwarf_amd import configs
+        from fesom2_tracer_dwarf_amd.configs import Scheme
+        t0 = time.perf_counter()
+        vcase = configs.bathymetry_case("mesh1000w", 1000, 48, seed=1, ntracers=2, nsteps=20, w_cfl=0.3)
+        log(f"vertical companion: the main mesh with w (generated in {time.perf_counter() - t0:.1f} s)")
+        vertical = {"workload": "the main workload's mesh and tracers with an interior vertical velocity "
+                                "(largest vertical Courant number 0.3), FP64, block of 2, the main run's "
+                                "launch configuration", "steps": 20}
+        lc_main = list(r["launch_config"])
+        for label, sc, split in (("MFCT/QR4C/FCT", Scheme(), False),
+                                 ("MUSCL/PPM/FCT", Scheme("MUSCL", "PPM", "FCT", 0.9, 0.6), False),
+                                 ("MFCT/QR4C/FCT use_wsplit", Scheme(), True)):
+            c = dataclasses.replace(vcase, schemes=[sc, sc])
+            if split:
+                c = configs.split_vertical_velocity(c, 0.5)
+            rv = run(c, 0, 0, 20, 2, lc=lc_main, tune=False, wsplit=split)
+            vertical[label] = {"ms_per_step": rv["wall_s"] * 1e3 / 20, "value": rv["value"],
+                               "volume_sum_rel_drift": rv["drift"]}
+            log(f"vertical {label}: {rv['wall_s'] * 1e3 / 20:.3f} ms per step, drift {rv['drift']:.2e}")
+
     dropin = None
     if args.workload == "mesh1000" and not args.no_dropin:
         dropin = {"sequence": "per tracer: tracer_set_values_device, tracer_zero_tendencies, "
@@ -725,6 +755,7 @@ def main():
         "multimesh": multimesh,
         "ensemble": ensemble,
         "precision": precision,
+        "vertical": vertical,
         "dropin": dropin,
     }
     print(json.dumps(out), flush=True)

@@ -1880,12 +1880,17 @@ __global__ __launch_bounds__(256, FESOMTRA_MIN_WAVES_LIMIT) void k_limit(const M
 // dir 0: ref -> dev, dir 1: dev -> ref.
 // The reference layout is node-major (a node's layers contiguous, nodes in reference order), the
 // device layout level-major (a level's nodes contiguous, in device order): a transpose with a
-// permutation.  A workgroup takes 64 device nodes x up to 64 layers (one chunk covers the columns
+// permutation.  A workgroup takes 32 device nodes x up to 64 layers (one chunk covers the columns
 // of most meshes) through LDS, so both sides are read and written in contiguous runs (a node's
-// layers on the reference side, 64 nodes of a level on the device side) whatever the
+// layers on the reference side, 32 nodes of a level on the device side) whatever the
 // permutation.  The cells of a chunk are numbered densely (no idle threads for a chunk shorter
-// than 64 layers).
-constexpr int kRelayNodes = 64, kRelayLevels = 64;
+// than 64 layers).  With a random node numbering one side is 376-byte runs at random places:
+// 32-node tiles (17 KB of LDS, 9 workgroups per CU) keep more of them in flight than 64-node tiles
+// (0.245 against 0.311 ms per 225 MB tracer of the 1000x1000x48 mesh).
+#ifndef FESOMTRA_RELAY_NODES
+#define FESOMTRA_RELAY_NODES 32
+#endif
+constexpr int kRelayNodes = FESOMTRA_RELAY_NODES, kRelayLevels = 64;
 template <typename S>
 __global__ __launch_bounds__(256) void k_relayout(const double *__restrict__ ref_in, S *__restrict__ dev_out,
                                                   const S *__restrict__ dev_in, double *__restrict__ ref_out,
