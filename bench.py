@@ -194,7 +194,7 @@ def make_case(name: str, nsteps=None):
 def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, lc=None,
         stage_steps: int = 10, compare=None, keep: bool = False, stochastic: bool = True,
         tune: bool = True, fused: bool = False, compensated: bool = False, members=None,
-        fused_limiter: bool = False, wsplit: bool = False):
+        fused_limiter: bool = False, wsplit: bool = False, concurrent: bool = True):
     """Advance `case` on the GPU; returns throughput, device time, per-stage times, drift.
 
     keep: also return the final values (list per tracer); compare: final values of a reference
@@ -211,6 +211,7 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
     t.set_rounding(stochastic)
     t.set_compensated(compensated)
     t.set_fused_limiter(fused_limiter)
+    t.set_concurrent(concurrent)
     if lc:
         t.set_launch_config(lc)
     if members:
@@ -277,16 +278,19 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
     return {"value": updates / wall, "wall_s": wall, "dev_ms": dev_ms, "stage_ms": stage_ms,
             "passes_timed": passes, "block_width": width, "drift": drift, "launch_config": cfg,
             "values": kept, "error": err, "autotune": tuned, "layout": layout, "qflux_ms": qflux_ms,
-            "step_ms": step_ms, "steps_timed": nsteps_timed, "fused_limiter": limiter}
+            "step_ms": step_ms, "steps_timed": nsteps_timed, "fused_limiter": limiter, "steps": steps}
 
 
 def roofline(case, prec: int, r: dict, name: str, members: int = 1):
-    """The step as the unit: algorithmic bytes of one step of every tracer over the mean HIP-event
-    time of the whole step (k_qflux and every pass of the step)."""
+    """The step as the unit: algorithmic bytes of one step of every tracer over the mean time of a
+    step of the timed run, from the HIP events the library records around it on its own stream
+    (tracer_last_elapsed_ms; k_qflux and every pass, the low-order kernels on the second stream
+    joined into it).  stage_ms are per-kernel HIP-event times of a separate run with the stages
+    timed one after another."""
     s_tr, s_m = PREC_BYTES[prec]
     k = case.ntracers
     B = algorithmic_bytes(case.mesh, k, s_tr, s_m, members)
-    step_ms = r["step_ms"]
+    step_ms = r["dev_ms"] / r["steps"]
     achieved = B / (step_ms * 1e-3) / 1e9
     st = r["stage_ms"]
     passes = max(1, round(r["passes_timed"] / max(r["steps_timed"], 1)))
@@ -308,6 +312,7 @@ def roofline(case, prec: int, r: dict, name: str, members: int = 1):
             "kernel": ("one step: k_qflux + " + (f"{passes} x " if passes > 1 else "") + label +
                        f" over {k} tracer(s) in blocks of {r['block_width']}, {PREC_NAMES[prec]}"),
             "algorithmic_bytes_per_step": B, "tracers_per_step": k, "step_ms": round(step_ms, 4),
+            "step_ms_stages_serial": round(r["step_ms"], 4),
             "stage_ms": {"k_qflux": round(r["qflux_ms"], 4), **stages},
             "passes_per_step": passes}
 
@@ -532,6 +537,8 @@ def main():
                     help="fused single-pass kernel for FP64 FCT tracers, one tracer per block")
     ap.add_argument("--fused-limiter", type=int, default=0, choices=[0, 1],
                     help="limiter bounds, ratios and update in one kernel where it applies (FP64, width <= 4)")
+    ap.add_argument("--concurrent", type=int, default=1, choices=[0, 1],
+                    help="low-order kernels on a second stream beside the edge-flux kernels")
     ap.add_argument("--lc", default="", help="levels per thread: edge,lo,bounds,final")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch configuration instead of autotuning it")
@@ -601,7 +608,8 @@ def main():
     log(f"workload {describe(case)}: N={m.nod2D} T={m.elem2D} E={m.edge2D} N_L={m.n_node_levels} "
         f"(generated in {time.perf_counter() - t0:.1f} s)")
     r = run(case, prec, kb, steps, args.warmup, graph, lc, tune=not args.no_autotune, fused=fused,
-            stochastic=args.rounding == "stochastic", fused_limiter=bool(args.fused_limiter))
+            stochastic=args.rounding == "stochastic", fused_limiter=bool(args.fused_limiter),
+            concurrent=bool(args.concurrent))
     log(f"timed {steps} steps: wall {r['wall_s'] * 1e3:.2f} ms, device {r['dev_ms']:.2f} ms, "
         f"{r['value']:.3e} updates/s, stages {r['stage_ms']}")
     roof = roofline(case, prec, r, args.workload)
@@ -746,7 +754,8 @@ def main():
                    "edges": m.edge2D, "node_levels_per_tracer": m.n_node_levels,
                    "tracers": case.ntracers, "block_width": r["block_width"],
                    "launch_config": list(r["launch_config"]), "hip_graph": graph, "fused": fused,
-                   "fused_limiter": r["fused_limiter"], "dt_s": case.dt, "parallelism": "1 GPU"},
+                   "fused_limiter": r["fused_limiter"], "concurrent_low_order": bool(args.concurrent),
+                   "dt_s": case.dt, "parallelism": "1 GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "volume_sum_rel_drift": r["drift"],

@@ -72,6 +72,7 @@ _SIGNATURES = {
     "tracer_get_work": (_c_int, [_c_char_p, _pd, ctypes.c_size_t]),
     "tracer_set_keep_work": (_c_int, [_c_int]),
     "tracer_set_fused_limiter": (_c_int, [_c_int]),
+    "tracer_set_concurrent": (_c_int, [_c_int]),
     "tracer_get_fused_limiter": (_c_int, [_pi]),
     "tracer_set_values_device": (_c_int, [_c_int, _vp, _c_int, _c_int]),
     "tracer_get_values_device": (_c_int, [_c_int, _vp, _c_int, _c_int]),

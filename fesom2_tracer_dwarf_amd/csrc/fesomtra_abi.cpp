@@ -104,6 +104,7 @@ struct State {
     int et_built = 0;  // number of edge tiles built for the current block width (0: none)
     int rt_built = 0;  // node tiles with two-ring halos (0: none, or they did not fit)
     bool fused_limiter = false;  // limiter bounds and update in one kernel where the tiles allow it
+    bool concurrent = true;  // low-order kernels on an aux stream beside the edge-flux kernels
     TileMesh tm{};
     std::vector<void *> dT2;    // second tracer buffer per block (output of a fused step)
     std::vector<char> t2_sync;  // dT2 holds the same below-bottom values as dT
@@ -873,6 +874,12 @@ int build_device(State &s) {
     if (!s.stream) HIPCHK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
     if (!s.ev0) HIPCHK(hipEventCreate(&s.ev0));
     if (!s.ev1) HIPCHK(hipEventCreate(&s.ev1));
+    // the aux stream of the low-order kernels (default priority, like the main stream)
+    if (s.concurrent && !s.cfg.aux) {
+        HIPCHK(hipStreamCreateWithFlags(&s.cfg.aux, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&s.cfg.fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&s.cfg.join, hipEventDisableTiming));
+    }
     const int N = s.N, T = s.T, E = s.E, nl = s.nl, L = nl - 1, A = s.max_adj;
     s.Np = pad64(N);
     s.Tp = pad64(T);
@@ -1233,6 +1240,13 @@ StepPtrs<S_, C_> make_ptrs(State &s, int blk, unsigned long long mask, bool drop
     return p;
 }
 
+// The launch configuration of a pass: the aux stream only while concurrency is on.
+LaunchCfg launch_cfg(const State &s) {
+    LaunchCfg c = s.cfg;
+    if (!s.concurrent) c.aux = nullptr;
+    return c;
+}
+
 // tin / tout (fused limiter steps): the block's values are read from tin and the update written to
 // tout (the two buffers of the block, alternating by step); null: the primary buffer in place.
 hipError_t launch_block(State &s, int blk, unsigned long long mask, const SchemeSel &sc, double dt, bool dropin,
@@ -1247,13 +1261,13 @@ hipError_t launch_block(State &s, int blk, unsigned long long mask, const Scheme
         if (!s.tab_on[blk]) p.TAB = static_cast<const double *>(tin);
         p.T = static_cast<double *>(tin);
         p.Tout = static_cast<double *>(tout);
-        return launch_step<double, double, double>(s.m64, p, s.kb, a, s.cfg, s.stream, ev);
+        return launch_step<double, double, double>(s.m64, p, s.kb, a, launch_cfg(s), s.stream, ev);
     }
     switch (s.prec) {
-    case PREC_F64: return launch_step<double, double, double>(s.m64, make_ptrs<double, double>(s, blk, mask, dropin, seed, ab_parity), s.kb, a, s.cfg, s.stream, ev);
-    case PREC_F32: return launch_step<float, double, float>(s.m64, make_ptrs<float, float>(s, blk, mask, dropin, seed, ab_parity), s.kb, a, s.cfg, s.stream, ev);
-    case PREC_F32_F16: return launch_step<f16_t, double, float>(s.m64, make_ptrs<f16_t, float>(s, blk, mask, dropin, seed, ab_parity), s.kb, a, s.cfg, s.stream, ev);
-    default: return launch_step<bf16_t, double, float>(s.m64, make_ptrs<bf16_t, float>(s, blk, mask, dropin, seed, ab_parity), s.kb, a, s.cfg, s.stream, ev);
+    case PREC_F64: return launch_step<double, double, double>(s.m64, make_ptrs<double, double>(s, blk, mask, dropin, seed, ab_parity), s.kb, a, launch_cfg(s), s.stream, ev);
+    case PREC_F32: return launch_step<float, double, float>(s.m64, make_ptrs<float, float>(s, blk, mask, dropin, seed, ab_parity), s.kb, a, launch_cfg(s), s.stream, ev);
+    case PREC_F32_F16: return launch_step<f16_t, double, float>(s.m64, make_ptrs<f16_t, float>(s, blk, mask, dropin, seed, ab_parity), s.kb, a, launch_cfg(s), s.stream, ev);
+    default: return launch_step<bf16_t, double, float>(s.m64, make_ptrs<bf16_t, float>(s, blk, mask, dropin, seed, ab_parity), s.kb, a, launch_cfg(s), s.stream, ev);
     }
 }
 
@@ -2000,6 +2014,9 @@ int tracer_cleanup(void) {
         release_device(*g);
         if (g->ev0) (void)hipEventDestroy(g->ev0);
         if (g->ev1) (void)hipEventDestroy(g->ev1);
+        if (g->cfg.fork) (void)hipEventDestroy(g->cfg.fork);
+        if (g->cfg.join) (void)hipEventDestroy(g->cfg.join);
+        if (g->cfg.aux) (void)hipStreamDestroy(g->cfg.aux);
         if (g->stream) (void)hipStreamDestroy(g->stream);
         delete g;
         g = nullptr;
@@ -2017,7 +2034,7 @@ int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D) {
     hipEvent_t k0 = s.ev0, k1 = s.ev1;
     const int prec = s.prec, kbr = s.kb_req;
     const bool graph = s.use_graph, sr = s.stochastic, fused = s.fused, ht = s.halo_tiles, ws = s.wsplit;
-    const bool comp = s.compensated, flim = s.fused_limiter, keepw = s.keep_work;
+    const bool comp = s.compensated, flim = s.fused_limiter, keepw = s.keep_work, conc = s.concurrent;
     const int abm = s.ab_mode;
     const double abe = s.ab_eps;
     const uint64_t gen = s.gen;
@@ -2034,6 +2051,7 @@ int tracer_set_mesh_dims(int nl, int nod2D, int edge2D, int elem2D) {
     s.compensated = comp;
     s.fused_limiter = flim;
     s.keep_work = keepw;
+    s.concurrent = conc;
     s.fused = fused;
     s.halo_tiles = ht;
     s.wsplit = ws;
@@ -2549,6 +2567,18 @@ int tracer_set_compensated(int enable) {
     s.compensated = on;
     s.gen++;
     if (s.initialized) return ensure_device(s);
+    return 0;
+}
+
+int tracer_set_concurrent(int enable) {
+    State &s = S();
+    s.concurrent = enable != 0;
+    if (s.concurrent && s.stream && !s.cfg.aux) {
+        HIPCHK(hipStreamCreateWithFlags(&s.cfg.aux, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&s.cfg.fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&s.cfg.join, hipEventDisableTiming));
+    }
+    s.gen++;
     return 0;
 }
 

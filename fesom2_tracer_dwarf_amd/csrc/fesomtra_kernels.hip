@@ -2045,6 +2045,56 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
     const dim3 ge = grid(m.E, cfg.lc_edge), gl = grid(m.N, cfg.lc_lo), gb = grid(m.N, cfg.lc_bounds),
                gf = grid(m.N, cfg.lc_final);
     const C ph = (C)sc.ph, pv = (C)sc.pv;
+    // halo-tiled node kernels when the tiles were built for this block width
+    bool tiled = false;
+    if constexpr (KB <= 4) tiled = m.ht_n > 0 && m.ht_kb == KB;
+    const unsigned gt = grid_for((size_t)m.ht_n, 1);
+    auto tgrid = [&](int lc) { return dim3(gt, (unsigned)((L + lc - 1) / lc)); };
+    // The low-order solution (with the implicit split and the cavity rows) reads only the
+    // transport and the values, not the antidiffusive fluxes: with the limiter it runs on the aux
+    // stream beside the gradient and edge-flux kernels, joined before the bounds kernel (not while
+    // the stages are timed one by one).
+    auto lo_group = [&](hipStream_t st) -> hipError_t {
+        if (tiled) {
+            if constexpr (KB <= 4) {
+                const dim3 tl = tgrid(cfg.lc_lo);
+                switch (sc.ver) {
+                case VER_UPW1: k_lo<true, VER_UPW1, KB, S, M, C, GEN><<<tl, 256, 0, st>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+                case VER_QR4C: k_lo<true, VER_QR4C, KB, S, M, C, GEN><<<tl, 256, 0, st>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+                case VER_CDIFF: k_lo<true, VER_CDIFF, KB, S, M, C, GEN><<<tl, 256, 0, st>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+                case VER_PPM: k_lo<true, VER_PPM, KB, S, M, C, GEN><<<tl, 256, 0, st>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+                default: return hipErrorInvalidValue;
+                }
+            }
+        } else {
+            switch (sc.ver) {
+            case VER_UPW1: k_lo<false, VER_UPW1, KB, S, M, C, GEN><<<gl, 256, 0, st>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+            case VER_QR4C: k_lo<false, VER_QR4C, KB, S, M, C, GEN><<<gl, 256, 0, st>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+            case VER_CDIFF: k_lo<false, VER_CDIFF, KB, S, M, C, GEN><<<gl, 256, 0, st>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+            case VER_PPM: k_lo<false, VER_PPM, KB, S, M, C, GEN><<<gl, 256, 0, st>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
+            default: return hipErrorInvalidValue;
+            }
+        }
+        LAUNCH_CHECK();
+        if (sc.wsplit) {
+            if (!p.tri) return hipErrorInvalidValue;
+            k_vimpl<KB, S, M, C, GEN><<<dim3(grid_for((size_t)m.N * KB, 256)), 256, 0, st>>>(m, p, sc.dt);
+            LAUNCH_CHECK();
+        }
+        if (m.cq_n > 0) {
+            k_cavity_aux<KB><<<dim3(grid_for((size_t)m.cq_n * KB, 256)), 256, 0, st>>>(m, p, sc.cavity_fill ? 1 : 0);
+            LAUNCH_CHECK();
+        }
+        return hipSuccess;
+    };
+    const bool conc = sc.fct && cfg.aux && cfg.fork && cfg.join && !ev;
+    if (conc) {
+        hipError_t e = hipEventRecord(cfg.fork, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(cfg.aux, cfg.fork, 0);
+        if (e == hipSuccess) e = lo_group(cfg.aux);
+        if (e == hipSuccess) e = hipEventRecord(cfg.join, cfg.aux);
+        if (e != hipSuccess) return e;
+    }
     MARK(0);
     if (sc.hor != HOR_UPW1 && p.mg && m.mg_n > 0) {
         k_mgrad<KB><<<dim3(grid_for((size_t)m.mg_n * KB, 256)), 256, 0, s>>>(m, p);
@@ -2062,31 +2112,15 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
     LAUNCH_CHECK();
     MARK(1);
     if (sc.fct) {
-        // halo-tiled node kernels when the tiles were built for this block width
-        bool tiled = false;
-        if constexpr (KB <= 4) tiled = m.ht_n > 0 && m.ht_kb == KB;
-        const unsigned gt = grid_for((size_t)m.ht_n, 1);
-        auto tgrid = [&](int lc) { return dim3(gt, (unsigned)((L + lc - 1) / lc)); };
+        if (conc) {
+            const hipError_t e = hipStreamWaitEvent(s, cfg.join, 0);
+            if (e != hipSuccess) return e;
+        } else {
+            const hipError_t e = lo_group(s);
+            if (e != hipSuccess) return e;
+        }
         if (tiled) {
             if constexpr (KB <= 4) {
-                const dim3 tl = tgrid(cfg.lc_lo);
-                switch (sc.ver) {
-                case VER_UPW1: k_lo<true, VER_UPW1, KB, S, M, C, GEN><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-                case VER_QR4C: k_lo<true, VER_QR4C, KB, S, M, C, GEN><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-                case VER_CDIFF: k_lo<true, VER_CDIFF, KB, S, M, C, GEN><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-                case VER_PPM: k_lo<true, VER_PPM, KB, S, M, C, GEN><<<tl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-                default: return hipErrorInvalidValue;
-                }
-                LAUNCH_CHECK();
-                if (sc.wsplit) {
-                    if (!p.tri) return hipErrorInvalidValue;
-                    k_vimpl<KB, S, M, C, GEN><<<dim3(grid_for((size_t)m.N * KB, 256)), 256, 0, s>>>(m, p, sc.dt);
-                    LAUNCH_CHECK();
-                }
-                if (m.cq_n > 0) {
-                    k_cavity_aux<KB><<<dim3(grid_for((size_t)m.cq_n * KB, 256)), 256, 0, s>>>(m, p, sc.cavity_fill ? 1 : 0);
-                    LAUNCH_CHECK();
-                }
                 // the fused limiter where the caller asked for it: FP64 with the two-ring tiles, an
                 // output buffer apart from the values (or the drop-in's accumulators)
                 bool fused_lim = false;
@@ -2108,23 +2142,6 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
                 }
             }
         } else {
-            switch (sc.ver) {
-            case VER_UPW1: k_lo<false, VER_UPW1, KB, S, M, C, GEN><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-            case VER_QR4C: k_lo<false, VER_QR4C, KB, S, M, C, GEN><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-            case VER_CDIFF: k_lo<false, VER_CDIFF, KB, S, M, C, GEN><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-            case VER_PPM: k_lo<false, VER_PPM, KB, S, M, C, GEN><<<gl, 256, 0, s>>>(m, p, pv, sc.dt, cfg.lc_lo, sc.wsplit); break;
-            default: return hipErrorInvalidValue;
-            }
-            LAUNCH_CHECK();
-            if (sc.wsplit) {
-                if (!p.tri) return hipErrorInvalidValue;
-                k_vimpl<KB, S, M, C, GEN><<<dim3(grid_for((size_t)m.N * KB, 256)), 256, 0, s>>>(m, p, sc.dt);
-                LAUNCH_CHECK();
-            }
-            if (m.cq_n > 0) {
-                k_cavity_aux<KB><<<dim3(grid_for((size_t)m.cq_n * KB, 256)), 256, 0, s>>>(m, p, sc.cavity_fill ? 1 : 0);
-                LAUNCH_CHECK();
-            }
             MARK(2);
             k_bounds<false, KB, S, M, C, GEN><<<gb, 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);
             LAUNCH_CHECK();

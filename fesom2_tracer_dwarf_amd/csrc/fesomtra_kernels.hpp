@@ -17,6 +17,10 @@ struct SchemeArgs {
 // layers, one thread per chunk).
 struct LaunchCfg {
     int lc_edge = 16, lc_lo = 16, lc_bounds = 16, lc_final = 16;
+    // With the limiter the low-order kernels run on `aux` beside the edge-flux kernels: `fork` is
+    // recorded on the main stream before them and `join` on aux after them (null: one stream).
+    hipStream_t aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 // One step of the lanes p.mask of a tracer block of width kb (1, 2, 4, ..., 64); the selected

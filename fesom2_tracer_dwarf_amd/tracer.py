@@ -290,6 +290,10 @@ class FESOMTracerAdvection:
                   "tracer_get_tendencies")
         return h, v
 
+    def set_concurrent(self, enable: bool) -> None:
+        """Low-order kernels on a second stream beside the edge-flux kernels (default True)."""
+        abi.check(self.lib.tracer_set_concurrent(int(bool(enable))), "tracer_set_concurrent")
+
     def set_fused_limiter(self, enable: bool) -> None:
         """Limiter bounds, ratios and update in one kernel where it applies (default True)."""
         abi.check(self.lib.tracer_set_fused_limiter(int(bool(enable))), "tracer_set_fused_limiter")

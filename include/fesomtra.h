@@ -221,6 +221,11 @@ int tracer_set_keep_work(int keep);
  * MI355X it is slower than the two kernels (1.83 against 1.26 ms per tracer-step on a
  * 1000x1000x48 mesh at width 1: its register footprint leaves two waves per SIMD), so it is off
  * by default. */
+/* Concurrent low-order kernels (enable = 1, the default): with the FCT limiter the low-order solution
+ * (and the implicit split) runs on a second stream of the library beside the gradient and
+ * edge-flux kernels, which it does not depend on, joined before the limiter bounds; 0 runs every
+ * kernel on one stream.  Results are identical either way; graphs capture both streams. */
+int tracer_set_concurrent(int enable);
 int tracer_set_fused_limiter(int enable);
 int tracer_get_fused_limiter(int *active);
 /* Device-pointer variants of tracer_set_values / tracer_get_values for callers whose tracers
