@@ -227,7 +227,8 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
         from fesom2_tracer_dwarf_amd import autotune
         if kb == 0:
             layout = autotune.tune_layout(t, case)
-            log(f"autotuned layout: block width {layout['block_width']}, tiles {layout['tiles']} "
+            log(f"autotuned layout: block width {layout['block_width']}, tiles {layout['tiles']}, "
+                f"fused limiter {layout['fused_limiter']}, two streams {layout['concurrent']} "
                 f"(ms/step {layout['step_ms']})")
         tuned = autotune.tune(t, case, prec, use_cache=False)
         log(f"autotuned levels per thread {tuned['launch_config']} (stage ms {[round(x, 3) for x in tuned['stage_ms']]})")

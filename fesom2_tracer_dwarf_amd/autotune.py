@@ -1,10 +1,12 @@
 """Autotuner of the launch configuration: the tracer layout (block width, LDS tiles) and the levels
 per thread of each of the four kernels.
 
-``tune_layout`` times whole steps (k_qflux and every pass, HIP events) for each candidate tracer
-block width -- how many tracers one kernel pass advances, interleaved in HBM -- and, for the
-narrow blocks (width <= 4), with and without the LDS tiles of the edge and node kernels, and keeps
-the fastest; the tracers are carried over exactly when the layout changes and the trial steps are
+``tune_layout`` times whole steps (the device time of a few steps, HIP events on the library's
+stream) for each candidate tracer block width -- how many tracers one kernel pass advances,
+interleaved in HBM -- and, for the narrow blocks (width <= 4), with and without the LDS tiles of
+the edge and node kernels and, on tiles, with the limiter as two kernels or as the fused kernel
+where it applies, each with the low-order kernels on a second stream or on the main one; it keeps
+the fastest.  The tracers are carried over exactly when the layout changes and the trial steps are
 undone with a device checkpoint at every candidate.
 
 Each column is split into chunks of ``lc`` layers, one thread per chunk and tracer lane.  Short
@@ -57,33 +59,41 @@ def width_candidates(ntracers: int) -> list:
 
 def tune_layout(tracer, case, widths: Optional[Sequence[int]] = None, steps: int = 3,
                 verbose: bool = False) -> Dict:
-    """Chooses and applies the fastest tracer block width and tile use for the case set up in
-    ``tracer``.  Returns {"block_width", "tiles", "step_ms": {"<width>/<tiles>": ms}}."""
+    """Chooses and applies the fastest tracer layout for the case set up in ``tracer``: block
+    width, tiles, limiter form and stream use.  Returns {"block_width", "tiles", "fused_limiter",
+    "concurrent", "step_ms": {"<width>/<tiles>/<limiter>/<streams>": ms}}."""
     lib = tracer.lib
     tracer.set_graph(False)
     widths = list(widths or width_candidates(case.ntracers))
     table = {}
     for w in widths:
         for tiles in ((True, False) if w <= 4 else (True,)):
-            tracer.set_block_width(w)
-            tracer.set_halo_tiles(tiles)
-            if lib.tracer_checkpoint(0) != 0:
-                raise RuntimeError("tracer_checkpoint failed")
-            tracer.advect_steps(1, case.dt)  # warm-up (and tile set-up)
-            lib.tracer_set_kernel_timing(1)
-            tracer.advect_steps(steps, case.dt)
-            _, step_ms, _ = tracer.step_times()
-            lib.tracer_set_kernel_timing(0)
-            if lib.tracer_checkpoint(1) != 0:
-                raise RuntimeError("tracer_checkpoint restore failed")
-            table[(w, tiles)] = step_ms
-            if verbose:
-                print(f"[autotune] width {w} tiles {tiles}: {step_ms:.4f} ms/step", flush=True)
-    (w, tiles) = min(table, key=table.get)
+            for fused in ((False, True) if (w <= 4 and tiles) else (False,)):
+                tracer.set_block_width(w)
+                tracer.set_halo_tiles(tiles)
+                tracer.set_fused_limiter(fused)
+                if fused and not tracer.fused_limiter_active():
+                    continue  # the two rings of some tile do not fit, or a pass has no limiter
+                for conc in (True, False):
+                    tracer.set_concurrent(conc)
+                    if lib.tracer_checkpoint(0) != 0:
+                        raise RuntimeError("tracer_checkpoint failed")
+                    tracer.advect_steps(1, case.dt)  # warm-up (and tile set-up)
+                    ms = tracer.advect_steps(steps, case.dt) / steps
+                    if lib.tracer_checkpoint(1) != 0:
+                        raise RuntimeError("tracer_checkpoint restore failed")
+                    table[(w, tiles, fused, conc)] = ms
+                    if verbose:
+                        print(f"[autotune] width {w} tiles {tiles} fused limiter {fused} two streams {conc}: "
+                              f"{ms:.4f} ms/step", flush=True)
+    (w, tiles, fused, conc) = min(table, key=table.get)
     tracer.set_block_width(w)
     tracer.set_halo_tiles(tiles)
-    return {"block_width": w, "tiles": tiles,
-            "step_ms": {f"{k[0]}/{'tiles' if k[1] else 'untiled'}": v for k, v in table.items()}}
+    tracer.set_fused_limiter(fused)
+    tracer.set_concurrent(conc)
+    return {"block_width": w, "tiles": tiles, "fused_limiter": fused, "concurrent": conc,
+            "step_ms": {f"{k[0]}/{'tiles' if k[1] else 'untiled'}/{'fused' if k[2] else 'two-kernel'}"
+                        f"/{'two-streams' if k[3] else 'one-stream'}": v for k, v in table.items()}}
 
 
 def tune(tracer, case, precision: int, candidates: Sequence[int] = CANDIDATES, steps: int = 3,

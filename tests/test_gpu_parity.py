@@ -326,6 +326,33 @@ def test_autotune_and_checkpoint_keep_state(oracle_mod, tmp_path, monkeypatch):
     assert rel_linf(got, ref[3]) <= RTOL
 
 
+def test_layout_autotune_keeps_state(oracle_mod):
+    """The layout autotuner (block width, tiles, limiter form, streams) runs trial steps between
+    a device checkpoint and its restore: afterwards the values are the initial ones and the chosen
+    layout gives the reference result."""
+    from fesom2_tracer_dwarf_amd import autotune
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    case = configs.bathymetry_case("bathy40l", 40, 30, seed=47, ntracers=2, nsteps=3, lev_lo=4, w_cfl=0.3)
+    say("case bathy40l layout autotune")
+    ref, _ = oracle_mod.Oracle(case).run(save_steps=[3])
+    t = FESOMTracerAdvection()
+    t.setup_case(case)
+    res = autotune.tune_layout(t, case, steps=2)
+    say(f"  chose {res['block_width']}, tiles {res['tiles']}, fused limiter {res['fused_limiter']}, "
+        f"two streams {res['concurrent']} of {sorted(res['step_ms'])}")
+    assert any("/fused/" in k for k in res["step_ms"]) and any("one-stream" in k for k in res["step_ms"])
+    assert t.block_width() == res["block_width"]
+    assert np.array_equal(t.get_all_values(), case.tracers)
+    t.advect_steps(3, case.dt)
+    got = t.get_all_values()
+    t.set_fused_limiter(False)
+    t.set_concurrent(True)
+    t.set_halo_tiles(True)
+    t.set_block_width(0)
+    t.cleanup()
+    assert rel_linf(got, ref[3]) <= RTOL
+
+
 FUSED_SCHEMES = [Scheme("MFCT", "QR4C", "FCT", 0.85, 0.75), Scheme("MUSCL", "PPM", "FCT", 0.9, 0.6),
                  Scheme("UPW1", "CDIFF", "FCT"), Scheme("MFCT", "UPW1", "FCT", 1.0, 1.0)]
 
