@@ -537,54 +537,70 @@ int upload_mesh(State &s, MeshCol<M> &m, const std::vector<int> &ints_n1, const 
 // curve); its halo is the ascending list of far ends of its nodes' edges outside the tile.  A
 // tile's workgroup stages two slots per thread (its column and one halo node), so the halo may
 // hold at most tc nodes; meshes with a larger halo anywhere run the untiled kernels.
-// Second halo ring of the node tiles, for the fused limiter kernel: per tile the one-ring halo
-// (hptr / halo of build_halo_tiles, same order) followed by the nodes adjacent to it outside the
-// tile and the first ring, sorted; per one-ring node the slots of its CSR entries' far nodes.
-// Tiles whose two rings exceed 256 / kb nodes leave the fused kernel off (rt_built = 0).
-int build_ring2_tiles(State &s, const std::vector<int> &hptr, const std::vector<int> &halo) {
+// Tiles of the fused limiter kernel: tile t owns the tcs = 128 / kb consecutive device nodes
+// from t * tcs; its first ring (the far ends of its nodes' edges outside it, sorted) takes the
+// node threads after the tile's (at most tcs of them, 256 / kb threads in all) and its second ring
+// (the far ends of the first ring's edges outside the tile and the first ring, sorted) the second
+// staging slots (at most 256 / kb).  Per node thread its CSR entries' far nodes as staging slots.
+// Tiles whose rings do not fit leave the fused kernel off (rt_built = 0).
+int build_limiter_tiles(State &s) {
     MeshCol<double> &m = s.m64;
     m.rt_n = 0;
     s.rt_built = 0;
-    const int tc = 256 / s.kb, N = s.N, nt = s.ht_built;
-    std::vector<int> rptr(nt + 1, 0), rhalo, eoff, rloc, rlmax(nt, 0), r2;
+    const int nt_thr = 256 / s.kb, tcs = 128 / s.kb, N = s.N, nt = (N + tcs - 1) / tcs;
+    std::vector<int> rptr(nt + 1, 0), rhalo, nr1v(nt, 0), eoff((size_t)nt * nt_thr, 0), rloc, rlmax(nt, 0),
+        rsmax(nt, 0), r1, r2;
     for (int t = 0; t < nt; ++t) {
-        const int a0 = t * tc, a1 = std::min(N, a0 + tc);
-        const int *r1b = halo.data() + hptr[t], *r1e = halo.data() + hptr[t + 1];
-        const int nr1 = (int)(r1e - r1b);
-        auto in_r1 = [&](int o) { return std::binary_search(r1b, r1e, o); };
+        const int a0 = t * tcs, a1 = std::min(N, a0 + tcs);
+        auto in_tile = [&](int o) { return o >= a0 && o < a1; };
+        r1.clear();
+        for (int n = a0; n < a1; ++n)
+            for (int i = s.h_ptr[n]; i < s.h_ptr[n + 1]; ++i)
+                if (!in_tile(s.h_oth[i])) r1.push_back(s.h_oth[i]);
+        std::sort(r1.begin(), r1.end());
+        r1.erase(std::unique(r1.begin(), r1.end()), r1.end());
+        if ((int)r1.size() > nt_thr - tcs) return 0;  // does not fit: the unfused kernels run
+        auto in_r1 = [&](int o) { return std::binary_search(r1.begin(), r1.end(), o); };
         r2.clear();
-        for (const int *h = r1b; h != r1e; ++h)
-            for (int i = s.h_ptr[*h]; i < s.h_ptr[*h + 1]; ++i) {
+        for (int h : r1)
+            for (int i = s.h_ptr[h]; i < s.h_ptr[h + 1]; ++i) {
                 const int o = s.h_oth[i];
-                if ((o < a0 || o >= a1) && !in_r1(o)) r2.push_back(o);
+                if (!in_tile(o) && !in_r1(o)) r2.push_back(o);
             }
         std::sort(r2.begin(), r2.end());
         r2.erase(std::unique(r2.begin(), r2.end()), r2.end());
-        if (nr1 + (int)r2.size() > tc) return 0;  // does not fit: the unfused kernels run
-        int lm = 0;
-        for (int n = a0; n < a1; ++n) lm = std::max(lm, s.h_nlev[n] - 1);
-        for (const int *h = r1b; h != r1e; ++h) {
-            lm = std::max(lm, s.h_nlev[*h] - 1);
-            eoff.push_back((int)rloc.size());
-            for (int i = s.h_ptr[*h]; i < s.h_ptr[*h + 1]; ++i) {
-                const int o = s.h_oth[i];
-                int sl;
-                if (o >= a0 && o < a1) sl = o - a0;
-                else if (in_r1(o)) sl = tc + (int)(std::lower_bound(r1b, r1e, o) - r1b);
-                else sl = tc + nr1 + (int)(std::lower_bound(r2.begin(), r2.end(), o) - r2.begin());
-                rloc.push_back(sl);
-            }
+        if ((int)r2.size() > nt_thr) return 0;
+        auto slot = [&](int o) {
+            if (in_tile(o)) return o - a0;
+            if (in_r1(o)) return tcs + (int)(std::lower_bound(r1.begin(), r1.end(), o) - r1.begin());
+            return nt_thr + (int)(std::lower_bound(r2.begin(), r2.end(), o) - r2.begin());
+        };
+        int lt = 0, ls = 0;
+        auto entries = [&](int i, int n) {
+            eoff[(size_t)t * nt_thr + i] = (int)rloc.size();
+            for (int k = s.h_ptr[n]; k < s.h_ptr[n + 1]; ++k) rloc.push_back(slot(s.h_oth[k]));
+        };
+        for (int n = a0; n < a1; ++n) {
+            lt = std::max(lt, s.h_nlev[n] - 1);
+            entries(n - a0, n);
         }
-        rlmax[t] = lm;
-        rhalo.insert(rhalo.end(), r1b, r1e);
+        ls = lt;
+        for (int k = 0; k < (int)r1.size(); ++k) {
+            ls = std::max(ls, s.h_nlev[r1[k]] - 1);
+            entries(tcs + k, r1[k]);
+        }
+        rlmax[t] = lt;
+        rsmax[t] = ls;
+        nr1v[t] = (int)r1.size();
+        rhalo.insert(rhalo.end(), r1.begin(), r1.end());
         rhalo.insert(rhalo.end(), r2.begin(), r2.end());
         rptr[t + 1] = (int)rhalo.size();
     }
     if (rhalo.empty()) rhalo.push_back(0);
-    if (eoff.empty()) eoff.push_back(0);
     if (rloc.empty()) rloc.push_back(0);
-    if (upload(s, &m.rt_hptr, rptr) || upload(s, &m.rt_halo, rhalo) || upload(s, &m.rt_eoff, eoff) ||
-        upload(s, &m.rt_loc, rloc) || upload(s, &m.rt_lmax, rlmax))
+    if (upload(s, &m.rt_hptr, rptr) || upload(s, &m.rt_halo, rhalo) || upload(s, &m.rt_nr1, nr1v) ||
+        upload(s, &m.rt_eoff, eoff) || upload(s, &m.rt_loc, rloc) || upload(s, &m.rt_lmax, rlmax) ||
+        upload(s, &m.rt_smax, rsmax))
         return -1;
     s.rt_built = nt;
     m.rt_n = (s.halo_tiles && s.fused_limiter) ? nt : 0;
@@ -626,7 +642,7 @@ int build_halo_tiles(State &s) {
     s.ht_built = nt;
     m.ht_kb = s.kb;
     m.ht_n = s.halo_tiles ? nt : 0;
-    return build_ring2_tiles(s, hptr, halo);
+    return build_limiter_tiles(s);
 }
 
 // Edge tiles of the edge-flux kernel for a tracer block of width kb <= 4: tile t owns the

@@ -648,6 +648,14 @@ __device__ __forceinline__ void edges_pre(const NodeEdges<PACK, NC, GEN> &ne, in
     for (int i = 0; i < NC; ++i)
         if (i < ne.deg && ne.on(i, nz)) v[i] = load(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
 }
+// As edges_pre, but loading every cached slot whatever the level (no branch around a load, so
+// the loads are issued back to back); a slot past the node's degree names edge 0 and node 0.
+// For loads whose index stays in bounds for any cached entry at any level of the column.
+template <bool PACK, int NC, bool GEN, typename LOAD, typename V>
+__device__ __forceinline__ void edges_pre_all(const NodeEdges<PACK, NC, GEN> &ne, LOAD &&load, V (&v)[NC]) {
+#pragma unroll
+    for (int i = 0; i < NC; ++i) v[i] = load(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
+}
 template <bool PACK, int NC, bool GEN, typename M, typename V, typename LOAD, typename USE>
 __device__ __forceinline__ void edges_use(const MeshCol<M> &m, const NodeEdges<PACK, NC, GEN> &ne, int nz, const V (&v)[NC],
                                           LOAD &&load, USE &&use) {
@@ -749,11 +757,31 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         ht.each([&](int i, int node) { st[i] = ldc<D>(p.T, ((size_t)(nz - 1) * Np + node) * KB + lane); });
     };
     auto stage = [&](int nz) { ht.each([&](int i, int) { sT[nz & 1][threadIdx.x + 256 * i] = st[i]; }); };
+    // The per-lane loads of a layer (the window's next value, the next layer's value and, untiled,
+    // the far values of the cached edges with their transports) are issued together before any
+    // is used: unconditionally, at a level clamped into the column, the value then selected, so
+    // the compiler does not wait for each load inside its own branch.
+    auto lay_raw = [&](const S *a, int lz) -> S {
+        const int c = min(max(lz, 1), max(L, 1));
+        return a[((size_t)(c - 1) * Np + n) * KB + lane];
+    };
+    auto lay_of = [&](S raw, int lz) -> D { return (lz >= 1 && lz <= L) ? ldc<D>(&raw, 0) : D(0); };
+    struct EdgeRaw {
+        S t;
+        double q;
+    };
     // layer nz of this column (its neighbours' values staged in sT[nz & 1] when TILED)
     auto body = [&](int nz) {
         if (TILED && (!act || nz > nz1 || (GEN && nz < nz0))) return;
         const int buf = nz & 1;
         const int j = nz + 1;  // bottom interface of layer nz
+        const size_t eo = (size_t)(nz - 1) * Ep, no = (size_t)(nz - 1) * Np;
+        const S raw3 = lay_raw(p.TAB, j + 1), rawn = lay_raw(p.T, j);
+        auto edge_raw = [&](int e, bool, int oth) {
+            return EdgeRaw{p.T[(no + oth) * KB + lane], (double)cld(m.q, mb.at(eo + e))};
+        };
+        EdgeRaw er[TILED ? 1 : decltype(ne)::kN];
+        if constexpr (!TILED) edges_pre_all(ne, edge_raw, er);
         if (VER == VER_QR4C) {
             z[0] = z[1];
             z[1] = z[2];
@@ -769,18 +797,25 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         tw[0] = tw[1];
         tw[1] = tw[2];
         tw[2] = tw[3];
-        tw[3] = nc.lay(p.TAB, j + 1);
+        tw[3] = lay_of(raw3, j + 1);
         if (VER == VER_QR4C) qw.slide(tw, z);
-        const D tnext = nc.lay(p.T, j);
+        const D tnext = lay_of(rawn, j);
         const D lo_bot = ver_upw1_w<D>(j, top, nzmax, tcur, tnext, wej, aj, D(0));
         const D fin_bot = wsplit ? ver_upw1_w<D>(j, top, nzmax, tcur, tnext, wj, aj, D(0)) : lo_bot;
         const D adf_bot = ver_ho_at<VER>(j, nzmax, tw, z, zbj, wj, aj, pv, dt, col, fin_bot, qw.q);
         // upwind edge fluxes (lib/oce_adv_tra_hor.F90:178-181) from q and the two end values,
         // summed in reference edge order
         double s = 0.0;
-        const size_t eo = (size_t)(nz - 1) * Ep, no = (size_t)(nz - 1) * Np;
         auto add_edge = [&](double to, double qe, bool second) {
-            const double f = upw1_edge<double>(second ? to : tcur, second ? tcur : to, qe);
+            double f;
+            if constexpr (std::is_same<C, double>::value) {
+                f = upw1_edge<double>(second ? to : tcur, second ? tcur : to, qe);
+            } else {
+                // single precision: -0.5 (t1 (v + |v|) + t2 (v - |v|)) is -v t1 for v >= 0 and
+                // -v t2 otherwise (one product instead of four operations; both ends of an edge
+                // form the same value, so the fluxes stay antisymmetric)
+                f = -qe * (qe >= 0.0 ? (second ? to : tcur) : (second ? tcur : to));
+            }
             s = second ? s - f : s + f;
         };
         if constexpr (TILED) {
@@ -789,12 +824,9 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
                 m, ne, nz, [&](int e, bool, int) { return (double)cld(m.q, mb.at(eo + e)); },
                 [&](double qe, int, bool second, int oth) { add_edge(sT[buf][oth * KB + lane], qe, second); });
         } else {
-            for_edges_ld<kTwoPhase<KB>>(
-                m, ne, nz,
-                [&](int e, bool, int oth) {
-                    return make_double2(ldc<double>(p.T, (no + oth) * KB + lane), (double)cld(m.q, mb.at(eo + e)));
-                },
-                [&](double2 v, int, bool second, int) { add_edge(v.x, v.y, second); });
+            edges_use(m, ne, nz, er, edge_raw, [&](const EdgeRaw &v, int, bool second, int) {
+                add_edge(ldc<double>(&v.t, 0), v.q, second);
+            });
         }
         const double lo = ((double)tcur * h + (s + ((double)lo_top - (double)lo_bot)) * dt / V) / hn;
         // FP64 stores LO; single precision stores the increment LO - T, so that low-order changes
@@ -1181,13 +1213,14 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         C av0, av1;
         C f[decltype(ne)::kN];
     };
-    auto preload = [&](int nz, Pre &pre) {
-        const size_t cell = (size_t)(nz - 1) * Np + n;
+    auto preload = [&](int nz, Pre &pre, int node) {
+        const size_t cell = (size_t)(nz - 1) * Np + node;
         pre.V = (double)cld(m.areasvol, cell);
-        pre.av0 = nz == nz0 ? p.adfv[cell * KB + lane] : av_bot;
+        const C a0 = p.adfv[cell * KB + lane];  // read unconditionally: no load inside a branch
+        pre.av0 = nz == nz0 ? a0 : av_bot;
         pre.av1 = p.adfv[(cell + Np) * KB + lane];
         const size_t eo = (size_t)(nz - 1) * Ep;
-        edges_pre(ne, nz, [&](int e, bool, int) { return p.adfh[(eo + e) * KB + lane]; }, pre.f);
+        edges_pre_all(ne, [&](int e, bool, int) { return p.adfh[(eo + e) * KB + lane]; }, pre.f);
     };
     auto finish = [&](int nz, D hi, D lw, D lo, const Pre &pre) {
         const size_t cell = (size_t)(nz - 1) * Np + n;
@@ -1211,14 +1244,15 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         p.Rp[cell * KB + lane] = (C)(rp < 1.0 ? rp : 1.0);
         p.Rm[cell * KB + lane] = (C)(rm < 1.0 ? rm : 1.0);
     };
-    D hi_r[2], lo_r[2], lo_next = 0;  // staging registers; LO of this column's fetched level
+    // staging registers: the raw values and LO of the fetched level (combined into the cluster
+    // values when staged, so that the fetch does not wait for its own loads)
+    S t_r[2] = {S(0), S(0)};
+    C l_r[2] = {C(0), C(0)};
     auto fetch = [&](int j) {
         ht.each([&](int i, int node) {
-            const size_t c = (size_t)(j - 1) * Np + node;
-            const D t = ldT(c), l = lo_at(c, t);
-            hi_r[i] = bmax(l, t);
-            lo_r[i] = bmin(l, t);
-            if (i == 0) lo_next = l;
+            const size_t c = ((size_t)(j - 1) * Np + node) * KB + lane;
+            t_r[i] = p.T[c];
+            l_r[i] = p.LO[c];
         });
     };
     if constexpr (TILED) fetch(jb);
@@ -1235,7 +1269,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         const int nz = j - 1;
         const bool fin = nz >= nz0 && nz <= nz1;
         Pre pre;
-        if (fin) preload(nz, pre);
+        if (fin) preload(nz, pre, n);
         if (j <= L && j >= top) cluster_pf(j, *cp, mx0, mn0, lo0);
         if (fin) {
             const bool own = nz == top || nz == L;
@@ -1258,14 +1292,24 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
     }
     for (int j = jb; j <= je; ++j) {
         const int buf = j & 1;
+        const int nz = j - 1;
+        const bool fin = act && nz >= nz0 && nz <= nz1;
+        Pre pre;
         D lo_j = 0;
         if constexpr (TILED) {
+            // the layer's own inputs are requested before the next level's fetch: loads complete
+            // in issue order, so requested after it they would make the layer wait for the fetch.
+            // Requested by every thread (a column with nothing to finish reads a cell in bounds
+            // and ignores it): a load inside a branch would make the staging below wait for it.
+            preload(fin ? nz : 1, pre, fin ? n : ht.snode[0]);
             if (j <= js) {
                 ht.each([&](int i, int) {
-                    shi[buf][threadIdx.x + 256 * i] = hi_r[i];
-                    slo[buf][threadIdx.x + 256 * i] = lo_r[i];
+                    const D t = ldc<D>(&t_r[i], 0);
+                    const D l = std::is_same<C, double>::value ? (D)l_r[i] : t + (D)l_r[i];
+                    shi[buf][threadIdx.x + 256 * i] = bmax(l, t);
+                    slo[buf][threadIdx.x + 256 * i] = bmin(l, t);
+                    if (i == 0) lo_j = l;
                 });
-                lo_j = lo_next;
                 __syncthreads();
                 if (j < js) fetch(j + 1);
             }
@@ -1276,10 +1320,8 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
         mx1 = mx0;
         mn1 = mn0;
         lo1 = lo0;
-        const int nz = j - 1;
-        const bool fin = nz >= nz0 && nz <= nz1;
-        Pre pre;
-        if (fin) preload(nz, pre);
+        if constexpr (!TILED)
+            if (fin) preload(nz, pre, n);
         if (j <= L && j >= top) {
             cluster(j, buf, mx0, mn0, lo0);
             if constexpr (TILED) lo0 = lo_j;
@@ -1387,14 +1429,53 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
             sm[i] = p.Rm[X(c)];
         });
     }
+    // Tiled form: a layer's own inputs are requested before the next layer's staging fetch (its
+    // cached edges' fluxes and its R one layer down even before the staging barrier).  Loads
+    // complete in issue order, so requested after that fetch they would make the layer wait for
+    // the fetch too; requested before, the fetch stays in flight while the layer is computed.
+    struct OwnIn {
+        double V, hn, h, tval, lo;
+        C ab, rpd, rmd;
+        C f[decltype(ne)::kN];
+    };
+    OwnIn own;
+    auto own_edges = [&](int nz, int node) {
+        const size_t cell = (size_t)(nz - 1) * Np + node, eo = (size_t)(nz - 1) * Ep;
+        if constexpr (FCT) {
+            const size_t cd = nz + 1 <= L ? cell + Np : cell;  // R one layer down (unused at the bottom)
+            own.rpd = p.Rp[X(cd)];
+            own.rmd = p.Rm[X(cd)];
+        }
+        edges_pre_all(ne, [&](int e, bool, int) { return p.adfh[X(eo + e)]; }, own.f);
+    };
+    auto own_col = [&](int nz, int node) {
+        const size_t cell = (size_t)(nz - 1) * Np + node;
+        own.V = (double)cld(m.areasvol, cell);
+        own.hn = (double)cld(m.hnode_new, cell);
+        own.h = FCT ? (double)cld(m.hnode, cell) : 0.0;
+        own.tval = (double)ldc<C>(p.T, X(cell));
+        own.ab = p.adfv[X(cell + Np)];
+        if constexpr (FCT) {
+            // the stored LO (single precision: the increment and its remainder, combined at use)
+            own.lo = std::is_same<C, double>::value
+                         ? (double)p.LO[X(cell)]
+                         : (double)p.LO[X(cell)] + (p.LOe ? (double)p.LOe[X(cell)] : 0.0);
+        }
+    };
     for (int nz = lv0; nz <= lv1; ++nz) {
         const int buf = nz & 1;
         if constexpr (TILED) {
+            // requested by every thread (one with nothing to update reads a cell in bounds and
+            // ignores it): a load inside a branch would make the staging below wait for it
+            const bool live = act && nz <= nz1 && (!GEN || nz >= nz0);
+            const int onz = live ? nz : 1, onode = live ? n : ht.snode[0];
+            own_edges(onz, onode);
             ht.each([&](int i, int) {
                 sRp[buf][threadIdx.x + 256 * i] = sp[i];
                 sRm[buf][threadIdx.x + 256 * i] = sm[i];
             });
             __syncthreads();
+            own_col(onz, onode);
             if (nz < lv1) {
                 ht.each([&](int i, int node) {
                     const size_t c = (size_t)nz * Np + node;
@@ -1402,7 +1483,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
                     sm[i] = p.Rm[X(c)];
                 });
             }
-            if (!act || nz > nz1 || (GEN && nz < nz0)) continue;
+            if (!live) continue;
         }
         const size_t cell = (size_t)(nz - 1) * Np + n;
         const size_t eo = (size_t)(nz - 1) * Ep, no = (size_t)(nz - 1) * Np;
@@ -1424,26 +1505,27 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
         };
         EdgeVals<C> pre[PRE ? decltype(ne)::kN : 1];
         if constexpr (PRE) edges_pre(ne, nz, edge_load, pre);
-        const double V = (double)cld(m.areasvol, cell);
-        const double hn = (double)cld(m.hnode_new, cell);
-        const double h = FCT ? (double)cld(m.hnode, cell) : 0.0;
-        const double tval = (double)ldc<C>(p.T, X(cell));
-        const C ab = p.adfv[X(cell + Np)];
+        const double V = TILED ? own.V : (double)cld(m.areasvol, cell);
+        const double hn = TILED ? own.hn : (double)cld(m.hnode_new, cell);
+        const double h = TILED ? own.h : FCT ? (double)cld(m.hnode, cell) : 0.0;
+        const double tval = TILED ? own.tval : (double)ldc<C>(p.T, X(cell));
+        const C ab = TILED ? own.ab : p.adfv[X(cell + Np)];
         C bot;
         double dv = (p.acc_v && !p.acc_init) ? p.acc_v[cell] : 0.0;
         if (FCT) {
             // single precision: LO = T + increment (+ the increment's remainder with compensated
             // storage, so that the low-order change of the content is the fluxes' to FP64 round-off)
             const double lo = std::is_same<C, double>::value
-                                  ? (double)p.LO[X(cell)]
-                                  : tval + ((double)p.LO[X(cell)] + (p.LOe ? (double)p.LOe[X(cell)] : 0.0));
+                                  ? (TILED ? own.lo : (double)p.LO[X(cell)])
+                                  : tval + (TILED ? own.lo
+                                                  : ((double)p.LO[X(cell)] + (p.LOe ? (double)p.LOe[X(cell)] : 0.0)));
             dv = dv - tval * h + lo * hn;
             rp0 = rp1;
             rm0 = rm1;
             if (nz + 1 <= L) {
-                if (TILED && nz < lv1) {  // staging slot 0 holds this column's next level
-                    rp1 = sp[0];
-                    rm1 = sm[0];
+                if (TILED) {
+                    rp1 = own.rpd;
+                    rm1 = own.rmd;
                 } else {
                     rp1 = p.Rp[X(cell + Np)];
                     rm1 = p.Rm[X(cell + Np)];
@@ -1484,12 +1566,12 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
             dh = second ? dh - x : dh + x;
         };
         if constexpr (TILED) {
-            // antidiffusive fluxes from memory, issued together; limiter ratios from the tile
-            for_edges_ld<true>(
-                m, ne, nz, [&](int e, bool, int) { return p.adfh[X(eo + e)]; },
-                [&](C f, int e, bool second, int oth) {
-                    add_edge(EdgeVals<C>{f, sRp[buf][oth * KB + lane], sRm[buf][oth * KB + lane], 0.0, 0.0}, e, second);
-                });
+            // antidiffusive fluxes requested before the barrier; limiter ratios from the tile
+            edges_use(m, ne, nz, own.f, [&](int e, bool, int) { return p.adfh[X(eo + e)]; },
+                      [&](C f, int e, bool second, int oth) {
+                          add_edge(EdgeVals<C>{f, sRp[buf][oth * KB + lane], sRm[buf][oth * KB + lane], 0.0, 0.0}, e,
+                                   second);
+                      });
         } else if constexpr (PRE) {
             edges_use(m, ne, nz, pre, edge_load, [&](const EdgeVals<C> &v, int e, bool second, int) { add_edge(v, e, second); });
         } else {
@@ -1529,49 +1611,48 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
 // ---------------------------------------------------------------------------------------------
 // Fused limiter: bounds, ratios, limiting, flux divergence and update in one kernel
 // (lib/oce_adv_tra_fct.F90:96-471, lib/oce_adv_tra_driver.F90:298-450, src/fesom_analytic.F90:419-437)
-// for tracer blocks of width KB <= 4 on node tiles with two-ring halos, FP64, meshes without
-// cavities, per-node time steps or ensembles, one level chunk per column.
+// for tracer blocks of width KB <= 4, FP64, meshes without cavities, per-node time steps or
+// ensembles, one level chunk per column.
 //
 // The update of a tile node needs the limiter ratios R of its one-ring neighbours, which need
-// the cluster values max(LO, T) / min(LO, T) of their own neighbours: the tile's second ring.  The
-// kernel stages those values for the tile and both rings, forms R for the tile nodes and for the
-// one-ring nodes (the latter a second time, as the neighbouring tiles do too), stages R in LDS and
-// limits and updates the tile nodes, so R never goes through memory and the antidiffusive fluxes,
-// the vertical fluxes, LO and the volumes are read once for both phases.  Sweep step j: stage the
-// cluster values of level j and the ratios of layer j - 2; one barrier; fetch level j + 1; form
-// the ratios of layer j - 1 (clusters j - 2 .. j, as k_bounds); limit and update layer j - 2 (R of
-// layers j - 3 .. j - 1 of the column, R of layer j - 2 at the neighbours, as k_final).  Every
-// expression and summation order is that of k_bounds and k_final, so results are bit-identical.
+// the cluster values max(LO, T) / min(LO, T) of their own neighbours: the tile's second ring.  A
+// workgroup owns a tile of 128 / KB consecutive nodes; its 256 / KB node threads hold the tile's
+// nodes and then its first ring, so every thread forms the ratios of one column (the first-ring
+// columns a second time, as the neighbouring tiles do too) and only the tile's threads go on to
+// limit and update.  Each thread stages two slots: its own column and one second-ring node.  R
+// never goes through memory and the antidiffusive fluxes, the vertical fluxes, LO and the volumes
+// are read once for both phases.  Sweep step j: stage the cluster values of level j and the
+// ratios of layer j - 2; one barrier; fetch level j + 1; form the ratios of layer j - 1 (clusters
+// j - 2 .. j, as k_bounds); limit and update layer j - 2 (R of layers j - 3 .. j - 1 of the
+// column, R of layer j - 2 at the neighbours, as k_final).  Every expression and summation order
+// is that of k_bounds and k_final, so results are bit-identical.  The updated values go to a
+// second buffer (neighbouring tiles read the old values of their rings while a tile updates).
 #ifndef FESOMTRA_MIN_WAVES_LIMIT
-#define FESOMTRA_MIN_WAVES_LIMIT 2
+#define FESOMTRA_MIN_WAVES_LIMIT 3
 #endif
 template <int KB, typename M>
-struct Ring2Tile {
-    static constexpr int TC = 256 / KB;
-    int t, n0, lmax, lstage, nr1, hb1;
-    int snode[2];
-    bool son[2];
-    __device__ __forceinline__ bool init(const MeshCol<M> &m) {
+struct SplitTile {
+    static constexpr int NT = 256 / KB, TCS = 128 / KB;
+    int t, n0, lmax, lstage, hb, nr1;
+    int node;  // this thread's column: a tile node (tile), a first-ring node, or none (-1)
+    bool tile;
+    int r2node;  // the second-ring node staged in this thread's second slot
+    bool r2on;
+    __device__ __forceinline__ bool init(const MeshCol<M> &m, int col) {
         t = xcd_block();
         if (t >= m.rt_n) return false;
-        n0 = t * TC;
-        lmax = cld(m.ht_lmax, t);
-        lstage = min(lmax + 1, cld(m.rt_lmax, t));
-        hb1 = cld(m.ht_hptr, t);
-        nr1 = cld(m.ht_hptr, t + 1) - hb1;
-        const int hb = cld(m.rt_hptr, t), H = cld(m.rt_hptr, t + 1) - hb;
-        const int col = (int)threadIdx.x / KB;
-        snode[0] = min(n0 + col, m.N - 1);
-        son[0] = true;
-        son[1] = col < H;
-        snode[1] = son[1] ? cld(m.rt_halo, hb + col) : 0;
+        n0 = t * TCS;
+        lmax = cld(m.rt_lmax, t);
+        lstage = min(lmax + 1, cld(m.rt_smax, t));
+        hb = cld(m.rt_hptr, t);
+        const int H = cld(m.rt_hptr, t + 1) - hb;
+        nr1 = cld(m.rt_nr1, t);
+        tile = col < TCS;
+        if (tile) node = n0 + col < m.N ? n0 + col : -1;
+        else node = col - TCS < nr1 ? cld(m.rt_halo, hb + col - TCS) : -1;
+        r2on = col < H - nr1;
+        r2node = r2on ? cld(m.rt_halo, hb + nr1 + col) : 0;
         return true;
-    }
-    template <typename F>
-    __device__ __forceinline__ void each(F &&f) const {
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-            if (son[i]) f(i, snode[i]);
     }
 };
 
@@ -1579,62 +1660,42 @@ template <int KB, typename S, typename M, typename C>
 __global__ __launch_bounds__(256, FESOMTRA_MIN_WAVES_LIMIT) void k_limit(const MeshCol<M> m, const StepPtrs<S, C> p, double dt) {
     static_assert(std::is_same<C, double>::value && std::is_same<S, double>::value, "FP64 only");
     using D = double;
-    constexpr int TC = 256 / KB, NC = kDegCacheFor<KB>;
-    __shared__ D shi[2][kHtSlots], slo[2][kHtSlots];  // cluster values of a level, by level parity
-    __shared__ C sRp[2][kHtSlots], sRm[2][kHtSlots];  // R of a layer (tile and first ring), by layer parity
-    // the first-ring node's sweep window, one slot per thread (registers are short): cluster
-    // bounds of levels j - 2 .. j by j % 3, its LO at j - 1, j by j % 2
-    __shared__ D whx[3][256], whn[3][256], whl[2][256];
-    Ring2Tile<KB, M> rt;
-    if (!rt.init(m)) return;
+    constexpr int NT = SplitTile<KB, M>::NT, NC = kDegCacheFor<KB>;
+    __shared__ D shi[2][2 * 256], slo[2][2 * 256];  // cluster values of a level, by level parity
+    __shared__ C sRp[2][256], sRm[2][256];           // R of a layer (tile and first ring), by layer parity
+    SplitTile<KB, M> st;
     const int col = (int)threadIdx.x / KB, lane = (int)threadIdx.x % KB;
+    if (!st.init(m, col)) return;
     const size_t Np = m.Np, Ep = m.Ep;
     auto X = [&](size_t cell) { return cell * KB + lane; };
-    // this thread's tile column
-    const int n = rt.n0 + col;
-    bool act = n < m.N && lane_on(p.mask, lane);
-    const int L = act ? cld(m.nlev_n, n) - 1 : 0;
+    const int n = st.node >= 0 ? st.node : 0;  // a column in bounds for threads without one
+    bool act = st.node >= 0 && lane_on(p.mask, lane);
+    const int Lc = act ? cld(m.nlev_n, n) - 1 : 0;
+    const int L = st.tile ? Lc : min(Lc, st.lmax);  // layers whose ratios this thread forms
     act = act && L >= 1;
+    const bool upd_col = act && st.tile;
     const int dry_from = act ? cld(m.nlev_min, n) : 0;
-    NodeEdges<true, NC, false> ne;
-    if (act) load_edges(m, n, ne);
-    else no_edges(ne);
-    // this thread's first-ring node (halo slot col), whose ratios the tile's updates need for the
-    // layers of the tile (its entries' far nodes are tile slots, rt_loc)
-    const int h = rt.snode[1];
-    bool acth = rt.son[1] && col < rt.nr1 && lane_on(p.mask, lane);
-    const int Lh = acth ? cld(m.nlev_n, h) - 1 : 0;
-    const int Lhr = min(Lh, rt.lmax);  // layers of h whose ratios are formed
-    acth = acth && Lhr >= 1;
-    const int hdry = acth ? cld(m.nlev_min, h) : 0;
-    const int heb = acth ? cld(m.ne_ptr, h) : 0;
-    const int hdeg = acth ? cld(m.ne_ptr, h + 1) - heb : 0;
-    const int hoff = acth ? cld(m.rt_eoff, rt.hb1 + col) : 0;
-    // h's entries: the first NC in registers (edge, and deepest level | slot << 16), the rest
-    // from memory; h_edges(nz, f) calls f(entry, edge, second, slot) in reference order
-    int he2[NC], htop[NC];
+    // the column's entries: the first NC in registers (edge * 2 + second, and deepest level |
+    // far slot << 16), the rest from memory; edges(nz, f) calls f(entry, edge, second, slot) in
+    // reference order
+    const int eb = act ? cld(m.ne_ptr, n) : 0;
+    const int deg = act ? cld(m.ne_ptr, n + 1) - eb : 0;
+    const int eoff = act ? cld(m.rt_eoff, st.t * NT + col) : 0;
+    int ce2[NC], ctop[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
-        const bool on = i < hdeg;
-        he2[i] = on ? cld(m.ne_e2, heb + i) : 0;
-        htop[i] = on ? (cld(m.rt_loc, hoff + i) << 16) | cld(m.ne_top, heb + i) : 0;
+        const bool on = i < deg;
+        ce2[i] = on ? cld(m.ne_e2, eb + i) : 0;
+        ctop[i] = on ? (cld(m.rt_loc, eoff + i) << 16) | cld(m.ne_top, eb + i) : 0;
     }
-    auto h_edges = [&](int nz, auto &&f) {
+    auto edges = [&](int nz, auto &&f) {
 #pragma unroll
         for (int i = 0; i < NC; ++i)
-            if (i < hdeg && nz <= (htop[i] & 0xffff)) f(i, he2[i] >> 1, (he2[i] & 1) != 0, htop[i] >> 16);
-        for (int i = NC; i < hdeg; ++i) {
-            if (nz > cld(m.ne_top, heb + i)) continue;
-            const int e2 = cld(m.ne_e2, heb + i);
-            f(i, e2 >> 1, (e2 & 1) != 0, cld(m.rt_loc, hoff + i));
-        }
-    };
-    const LaneLd<KB, D, S> ldT{p.T, lane};
-    // cluster bound of a column at level j from the staged values (k_bounds' cluster)
-    auto clamp_dry = [&](int j, int dfrom, D &vmax, D &vmin) {
-        if (j >= dfrom) {
-            vmax = cmax(vmax, D(-1.0e3));
-            vmin = cmin(vmin, D(1.0e3));
+            if (i < deg && nz <= (ctop[i] & 0xffff)) f(i, ce2[i] >> 1, (ce2[i] & 1) != 0, ctop[i] >> 16);
+        for (int i = NC; i < deg; ++i) {
+            if (nz > cld(m.ne_top, eb + i)) continue;
+            const int e2 = cld(m.ne_e2, eb + i);
+            f(i, e2 >> 1, (e2 & 1) != 0, cld(m.rt_loc, eoff + i));
         }
     };
     // ratios of layer nz with cluster bounds (hi, lw), low-order value lo, vertical fluxes av0 /
@@ -1668,85 +1729,54 @@ __global__ __launch_bounds__(256, FESOMTRA_MIN_WAVES_LIMIT) void k_limit(const M
         }
         return ae * a;
     };
-    // staging registers of the next level: cluster values and LO of both slots
-    D hi_r[2] = {0, 0}, lo_r[2] = {0, 0}, lo_st[2] = {0, 0};
+    // staging registers of the next level: raw values and LO of both slots (combined when staged,
+    // so that the fetch does not wait for its own loads)
+    D t_r[2] = {0, 0}, l_r[2] = {0, 0};
+    const int snode[2] = {n, st.r2node};
+    const bool son[2] = {st.node >= 0, st.r2on};
     auto fetch = [&](int j) {
-        rt.each([&](int i, int node) {
-            const size_t c = (size_t)(j - 1) * Np + node;
-            const D t = ldT(c), l = p.LO[X(c)];
-            hi_r[i] = cmax(l, t);
-            lo_r[i] = cmin(l, t);
-            lo_st[i] = l;
-        });
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const size_t c = (size_t)(j - 1) * Np + snode[i];
+            t_r[i] = p.T[X(c)];
+            l_r[i] = p.LO[X(c)];
+        }
     };
-    // sweep state: clusters of levels j - 2, j - 1, j and LO at j - 1, j (tile column, first-ring
-    // node); the column's inputs of layer j - 2 kept from its ratio step for the update (vertical
-    // fluxes, volume, LO; its antidiffusive fluxes are read again, from the cache)
+    // sweep state: clusters of levels j - 2, j - 1, j and LO at j - 1, j; the column's inputs of
+    // layer j - 2 kept from its ratio step for the update (antidiffusive and vertical fluxes,
+    // volume, LO)
     D mx2 = 0, mn2 = 0, mx1 = 0, mn1 = 0, mx0 = 0, mn0 = 0, lo1 = 0, lo0 = 0;
     C rpB = 0, rmB = 0;  // R of layer j - 2 of the column (formed in the previous step)
-    C hrp = 0, hrm = 0;                                    // first-ring node's R of layer j - 1
-    C f_k[NC], av0_k = 0, av1_k = 0, av_bot = 0, hav_bot = 0, topf = 0;
+    C f_k[NC], av0_k = 0, av1_k = 0, av_bot = 0, topf = 0;
     double V_k = 0.0;
 #pragma unroll
     for (int i = 0; i < NC; ++i) f_k[i] = 0;
     D lo_k = 0;
-    if (rt.lstage >= 1) fetch(1);
-    for (int j = 1; j <= rt.lmax + 2; ++j) {
+    if (st.lstage >= 1) fetch(1);
+    for (int j = 1; j <= st.lmax + 2; ++j) {
         const int buf = j & 1, rbuf = j & 1;  // R of layer j - 2 goes to parity (j - 2) & 1 = j & 1
-        D lo_j = 0, hlo_j = 0;
-        if (j <= rt.lstage) {
-            rt.each([&](int i, int) {
-                shi[buf][threadIdx.x + 256 * i] = hi_r[i];
-                slo[buf][threadIdx.x + 256 * i] = lo_r[i];
-            });
-            lo_j = lo_st[0];
-            hlo_j = lo_st[1];
-        }
-        if (j >= 3) {
-            // R of layer j - 2: the tile column's (slot col) and the first-ring node's (slot TC + col)
-            sRp[rbuf][threadIdx.x] = rpB;
-            sRm[rbuf][threadIdx.x] = rmB;
-            if (col < rt.nr1) {
-                sRp[rbuf][threadIdx.x + 256] = hrp;
-                sRm[rbuf][threadIdx.x + 256] = hrm;
-            }
-        }
-        __syncthreads();
-        if (j < rt.lstage) fetch(j + 1);
-        // ---- ratios of layer j - 1 ----
-        const int nz = j - 1;
-        mx2 = mx1; mn2 = mn1; mx1 = mx0; mn1 = mn0; lo1 = lo0;
-        C f_n[NC], av0_n = 0, av1_n = 0, rpN = 0, rmN = 0;
-        double V_n = 0.0;
+        // this step's own inputs, requested before the next level's fetch (loads complete in
+        // issue order) and by every thread (a thread with nothing to do reads cells in bounds and
+        // ignores them, so no load sits inside a branch): the ratio step's (layer j - 1) and the
+        // update's (layer j - 2)
+        const int nz = j - 1, nf = j - 2;
         const bool fin = act && nz >= 1 && nz <= L;
-        if (fin) {
-            const size_t cell = (size_t)(nz - 1) * Np + n;
+        const bool upd = upd_col && nf >= 1 && nf <= L;
+        C f_n[NC], av0_n, av1_n;
+        double V_n;
+        {
+            const int z = fin ? nz : 1;
+            const size_t cell = (size_t)(z - 1) * Np + n, eo = (size_t)(z - 1) * Ep;
             V_n = (double)cld(m.areasvol, cell);
-            av0_n = nz == 1 ? p.adfv[X(cell)] : av_bot;
+            const C a0 = p.adfv[X(cell)];
+            av0_n = z == 1 ? a0 : av_bot;
             av1_n = p.adfv[X(cell + Np)];
-            const size_t eo = (size_t)(nz - 1) * Ep;
-            edges_pre(ne, nz, [&](int e, bool, int) { return p.adfh[X(eo + e)]; }, f_n);
-        }
-        // the first-ring node's inputs of layer nz and the update's of layer j - 2, loaded
-        // together with the column's (one memory round per level)
-        const bool hfin = acth && nz >= 1 && nz <= Lhr;
-        C hf[NC], ha0 = 0, ha1 = 0;
-        double hV = 1.0;
-        if (hfin) {
-            const size_t cell = (size_t)(nz - 1) * Np + h;
-            const size_t eo = (size_t)(nz - 1) * Ep;
 #pragma unroll
-            for (int i = 0; i < NC; ++i)
-                hf[i] = (i < hdeg && nz <= (htop[i] & 0xffff)) ? p.adfh[X(eo + (he2[i] >> 1))] : C(0);
-            hV = (double)cld(m.areasvol, cell);
-            ha0 = nz == 1 ? p.adfv[X(cell)] : hav_bot;
-            ha1 = p.adfv[X(cell + Np)];
+            for (int i = 0; i < NC; ++i) f_n[i] = p.adfh[X(eo + (ce2[i] >> 1))];
         }
-        const int nf = j - 2;
-        const bool upd = act && nf >= 1 && nf <= L;
-        double u_hn = 1.0, u_hh = 0.0, u_t = 0.0, u_dv = 0.0, u_dh = 0.0;
-        if (upd) {
-            const size_t cell = (size_t)(nf - 1) * Np + n;
+        double u_hn, u_hh, u_t, u_dv = 0.0, u_dh = 0.0;
+        {
+            const size_t cell = (size_t)((upd ? nf : 1) - 1) * Np + n;
             u_hn = (double)cld(m.hnode_new, cell);
             u_hh = (double)cld(m.hnode, cell);
             u_t = (double)ldc<C>(p.T, X(cell));
@@ -1755,59 +1785,52 @@ __global__ __launch_bounds__(256, FESOMTRA_MIN_WAVES_LIMIT) void k_limit(const M
                 u_dh = p.acc_h[cell];
             }
         }
-        if (act && j <= L) {
+        D lo_j = 0;
+        if (j <= st.lstage) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                if (!son[i]) continue;
+                const D t = t_r[i], l = l_r[i];
+                shi[buf][threadIdx.x + 256 * i] = cmax(l, t);
+                slo[buf][threadIdx.x + 256 * i] = cmin(l, t);
+            }
+            lo_j = l_r[0];
+        }
+        if (j >= 3) {
+            // R of layer j - 2 of this thread's column (slot col)
+            sRp[rbuf][threadIdx.x] = rpB;
+            sRm[rbuf][threadIdx.x] = rmB;
+        }
+        __syncthreads();
+        if (j < st.lstage) fetch(j + 1);
+        // ---- ratios of layer j - 1 ----
+        mx2 = mx1; mn2 = mn1; mx1 = mx0; mn1 = mn0; lo1 = lo0;
+        C rpN = 0, rmN = 0;
+        if (act && j <= Lc && j <= st.lstage) {
             mx0 = shi[buf][threadIdx.x];
             mn0 = slo[buf][threadIdx.x];
-            for_edges(m, ne, j, [&](int, bool, int oth) {
-                mx0 = cmax(mx0, shi[buf][oth * KB + lane]);
-                mn0 = cmin(mn0, slo[buf][oth * KB + lane]);
+            edges(j, [&](int, int, bool, int sl) {
+                mx0 = cmax(mx0, shi[buf][sl * KB + lane]);
+                mn0 = cmin(mn0, slo[buf][sl * KB + lane]);
             });
-            clamp_dry(j, dry_from, mx0, mn0);
+            if (j >= dry_from) {
+                mx0 = cmax(mx0, D(-1.0e3));
+                mn0 = cmin(mn0, D(1.0e3));
+            }
             lo0 = lo_j;
         }
         if (fin) {
-            const bool own = nz == 1 || nz == L;
+            const bool own = nz == 1 || nz == Lc;
             const size_t eo = (size_t)(nz - 1) * Ep;
             ratios(own ? mx1 : cmax(cmax(mx2, mx1), mx0), own ? mn1 : cmin(cmin(mn2, mn1), mn0), lo1, av0_n, av1_n,
                    V_n,
                    [&](auto &&g) {
-                       edges_use(m, ne, nz, f_n, [&](int e, bool, int) { return p.adfh[X(eo + e)]; },
-                                 [&](C a, int, bool second, int) { g(a, second); });
+                       edges(nz, [&](int i, int e, bool second, int) {
+                           g(i < NC ? f_n[i < NC ? i : 0] : p.adfh[X(eo + e)], second);
+                       });
                    },
                    rpN, rmN);
             av_bot = av1_n;
-        }
-        // the first-ring node's ratios of layer nz (its fluxes and volume straight from memory)
-        if (acth) {
-            const int w0 = j % 3, w1 = (j + 2) % 3, w2 = (j + 1) % 3;  // levels j, j - 1, j - 2
-            if (j <= Lh && j <= rt.lstage) {
-                D hx0 = shi[buf][threadIdx.x + 256], hn0 = slo[buf][threadIdx.x + 256];
-                h_edges(j, [&](int, int, bool, int sl) {
-                    hx0 = cmax(hx0, shi[buf][sl * KB + lane]);
-                    hn0 = cmin(hn0, slo[buf][sl * KB + lane]);
-                });
-                clamp_dry(j, hdry, hx0, hn0);
-                whx[w0][threadIdx.x] = hx0;
-                whn[w0][threadIdx.x] = hn0;
-                whl[j & 1][threadIdx.x] = hlo_j;
-            }
-            if (hfin) {
-                const size_t eo = (size_t)(nz - 1) * Ep;
-                const double V = hV;
-                const C a0 = ha0, a1 = ha1;
-                const bool own = nz == 1 || nz == Lh;
-                const D hx1 = whx[w1][threadIdx.x], hn1 = whn[w1][threadIdx.x];
-                const D hi = own ? hx1 : cmax(cmax(whx[w2][threadIdx.x], hx1), whx[w0][threadIdx.x]);
-                const D lw = own ? hn1 : cmin(cmin(whn[w2][threadIdx.x], hn1), whn[w0][threadIdx.x]);
-                ratios(hi, lw, whl[(j - 1) & 1][threadIdx.x], a0, a1, V,
-                       [&](auto &&g) {
-                           h_edges(nz, [&](int i, int e, bool second, int) {
-                               g(i < NC ? hf[i < NC ? i : 0] : p.adfh[X(eo + e)], second);
-                           });
-                       },
-                       hrp, hrm);
-                hav_bot = a1;
-            }
         }
         // ---- limiting and update of layer j - 2 ----
         if (upd) {
@@ -1834,24 +1857,24 @@ __global__ __launch_bounds__(256, FESOMTRA_MIN_WAVES_LIMIT) void k_limit(const M
             dv = dv + ((double)topf - (double)bot) * dt / V;
             double dh = u_dh;
             const C rpn = rpB, rmn = rmB;
-            edges_use(m, ne, nf, f_k, [&](int e, bool, int) { return p.adfh[X(eo + e)]; },
-                      [&](C f, int e, bool second, int oth) {
-                          const C rpo = sRp[fb][oth * KB + lane], rmo = sRm[fb][oth * KB + lane];
-                          const C rp_1 = second ? rpo : rpn, rm_1 = second ? rmo : rmn;  // edges(1)
-                          const C rp_2 = second ? rpn : rpo, rm_2 = second ? rmn : rmo;  // edges(2)
-                          C ae = 1;
-                          if (f >= C(0)) {
-                              ae = cmin(ae, rp_1);
-                              ae = cmin(ae, rm_2);
-                          } else {
-                              ae = cmin(ae, rm_1);
-                              ae = cmin(ae, rp_2);
-                          }
-                          f = ae * f;
-                          if (p.dbg_h && !second) p.dbg_h[eo + e] = f;
-                          const double x = (double)f * dt / V;
-                          dh = second ? dh - x : dh + x;
-                      });
+            edges(nf, [&](int i, int e, bool second, int oth) {
+                C f = i < NC ? f_k[i < NC ? i : 0] : p.adfh[X(eo + e)];
+                const C rpo = sRp[fb][oth * KB + lane], rmo = sRm[fb][oth * KB + lane];
+                const C rp_1 = second ? rpo : rpn, rm_1 = second ? rmo : rmn;  // edges(1)
+                const C rp_2 = second ? rpn : rpo, rm_2 = second ? rmn : rmo;  // edges(2)
+                C ae = 1;
+                if (f >= C(0)) {
+                    ae = cmin(ae, rp_1);
+                    ae = cmin(ae, rm_2);
+                } else {
+                    ae = cmin(ae, rm_1);
+                    ae = cmin(ae, rp_2);
+                }
+                f = ae * f;
+                if (p.dbg_h && !second) p.dbg_h[eo + e] = f;
+                const double x = (double)f * dt / V;
+                dh = second ? dh - x : dh + x;
+            });
             if (p.acc_h) {
                 p.acc_h[cell] = dh;
                 p.acc_v[cell] = dv;
@@ -2131,7 +2154,7 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
                 if (fused_lim) {
                     if constexpr (std::is_same<C, double>::value && std::is_same<S, double>::value && !GEN) {
                         MARK(3);
-                        k_limit<KB, S, M, C><<<dim3(gt), 256, 0, s>>>(m, p, sc.dt);
+                        k_limit<KB, S, M, C><<<dim3(grid_for((size_t)m.rt_n, 1)), 256, 0, s>>>(m, p, sc.dt);
                     }
                 } else {
                     k_bounds<true, KB, S, M, C, GEN><<<tgrid(cfg.lc_bounds), 256, 0, s>>>(m, p, sc.dt, cfg.lc_bounds);

@@ -150,13 +150,15 @@ struct MeshCol {
     // corners); et_loc holds per edge the slots of n1, n2, u0, u1, u2, d0, d1, d2 (16 bits each).
     int et_n, et_kb;
     const int *et_loc, *et_nptr, *et_nodes, *et_lmax;  // (E, 4), (et_n + 1), (nodes), (et_n) layers
-    // Two-ring halos of the same node tiles (rt_n = ht_n when every tile's two rings fit its
-    // staging slots, else 0): per tile the one-ring halo (the list of ht_halo, in the same order)
-    // followed by the second ring, at most 256 / ht_kb nodes (rt_hptr, rt_halo); per one-ring
-    // node its CSR entries' far nodes as slots of the tile (rt_eoff: offset of the node's entries in
-    // rt_loc).  The fused limiter kernel forms the limiter ratios of the one-ring nodes itself.
+    // Tiles of the fused limiter kernel for the same widths (rt_n = 0: none, or a tile's rings
+    // did not fit): tile t owns the 128 / ht_kb consecutive nodes from t * 128 / ht_kb; rt_halo
+    // lists its first ring (rt_nr1[t] nodes, at most 128 / ht_kb) and then its second ring (at
+    // most 256 / ht_kb); per node thread i of the tile (tile node, then first-ring node) rt_eoff[t *
+    // 256 / ht_kb + i] is the offset in rt_loc of its CSR entries' far nodes as staging slots
+    // (tile node k: k, first-ring node k: 128 / ht_kb + k, second-ring node k: 256 / ht_kb + k).
+    // rt_lmax: the tile nodes' deepest layer; rt_smax: that of the tile and its first ring.
     int rt_n;
-    const int *rt_hptr, *rt_halo, *rt_eoff, *rt_loc, *rt_lmax;  // (ht_n + 1), (halo), (ring-1 nodes), (entries), (ht_n)
+    const int *rt_hptr, *rt_halo, *rt_nr1, *rt_eoff, *rt_loc, *rt_lmax, *rt_smax;
     // nodes
     const int *nlev_n, *nlev_min, *nbl;   // nlevels_nod2D, nlevels_nod2D_min, nboundary_lay
     const int *ne_ptr;                    // CSR node -> edges, reference edge order
