@@ -608,6 +608,17 @@ __device__ __forceinline__ void no_edges(NodeEdges<PACK, NC, GEN> &ne) {
 #pragma unroll
     for (int i = 0; i < (PACK ? 1 : NC); ++i) ne.oth[i] = 0;
 }
+// One column per wavefront (KB = 64): the cached entries live in scalar registers.  Marking them
+// as redefined at the top of every level keeps the compiler from hoisting each entry's 64-bit
+// addresses out of the level loop: held across the loop they exceed the scalar register file and
+// are spilled to vector lanes and read back at every level.
+template <bool PACK, int NC, bool GEN>
+__device__ __forceinline__ void scalar_fence(NodeEdges<PACK, NC, GEN> &ne) {
+#pragma unroll
+    for (int i = 0; i < NC; ++i) asm volatile("" : "+s"(ne.e2[i]));
+#pragma unroll
+    for (int i = 0; i < (PACK ? 1 : NC); ++i) asm volatile("" : "+s"(ne.oth[i]));
+}
 // Calls f(edge, second, far) for the edges of n active at level nz, in reference order.
 template <bool PACK, int NC, bool GEN, typename M, typename F>
 __device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges<PACK, NC, GEN> &ne, int nz, F &&f) {
@@ -776,6 +787,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
         const int buf = nz & 1;
         const int j = nz + 1;  // bottom interface of layer nz
         const size_t eo = (size_t)(nz - 1) * Ep, no = (size_t)(nz - 1) * Np;
+        if constexpr (KB == 64) scalar_fence(ne);
         const S raw3 = lay_raw(p.TAB, j + 1), rawn = lay_raw(p.T, j);
         auto edge_raw = [&](int e, bool, int oth) {
             return EdgeRaw{p.T[(no + oth) * KB + lane], (double)cld(m.q, mb.at(eo + e))};
@@ -1261,6 +1273,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_bounds(const MeshC
     // level j of the sweep with the prefetched cluster values cp (CPF; the other forms run the
     // same sequence inline below)
     auto step = [&](int j, const CPre *cp) {
+        if constexpr (KB == 64) scalar_fence(ne);
         mx2 = mx1;
         mn2 = mn1;
         mx1 = mx0;
@@ -1464,6 +1477,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
     };
     for (int nz = lv0; nz <= lv1; ++nz) {
         const int buf = nz & 1;
+        if constexpr (KB == 64) scalar_fence(ne);
         if constexpr (TILED) {
             // requested by every thread (one with nothing to update reads a cell in bounds and
             // ignores it): a load inside a branch would make the staging below wait for it
