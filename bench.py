@@ -319,8 +319,15 @@ def roofline(case, prec: int, r: dict, name: str, members: int = 1):
 
 
 PROFILE_ARGS = ("--steps 5 --warmup 1 --no-cpu-baseline --no-batched --no-dropin --no-multimesh --no-precision "
-                "--no-vertical --no-autotune "
+                "--no-vertical --no-autotune --concurrent 0 "
                 "--lc 48,48,48,48")
+
+
+def profile_args(workload: str) -> str:
+    """The bench options of the profiled runs: one stream (the kernels' own durations, as the
+    timed runs' autotuned layout), and for the 64-tracer workload the batched line's compensated
+    storage."""
+    return PROFILE_ARGS + (" --compensated" if workload == "mesh500x64" else "")
 
 
 def _rocprof_rows(d: Path, suffix: str):
@@ -378,11 +385,12 @@ def rocprof_summary(args):
     step = ["fesomtra::k_qflux", "fesomtra::k_mgrad", "fesomtra::k_edge", "fesomtra::k_lo", "fesomtra::k_vimpl",
             "fesomtra::k_cavity_aux", "fesomtra::k_bounds", "fesomtra::k_final", "fesomtra::k_limit", "fesomtra::k_vert",
             "fesomtra::k_copy", "fesomtra::k_seed", "fesomtra::k_ab2"]
-    lines = [f"# {w}: rocprofv3 kernel trace and HBM counters of bench.py ({PROFILE_ARGS})", "",
+    pa = profile_args(w)
+    lines = [f"# {w}: rocprofv3 kernel trace and HBM counters of bench.py ({pa})", "",
              "Commands (each a run of its own; the copy calibration moves 1 GiB):",
-             f"    rocprofv3 --kernel-trace --stats -f csv -d <dir>/k -o k -- python -u bench.py --workload {w} {PROFILE_ARGS}",
-             f"    rocprofv3 --pmc FETCH_SIZE -f csv -d <dir>/f -o f -- python -u bench.py --workload {w} {PROFILE_ARGS}",
-             f"    rocprofv3 --pmc WRITE_SIZE -f csv -d <dir>/w -o w -- python -u bench.py --workload {w} {PROFILE_ARGS}",
+             f"    rocprofv3 --kernel-trace --stats -f csv -d <dir>/k -o k -- python -u bench.py --workload {w} {pa}",
+             f"    rocprofv3 --pmc FETCH_SIZE -f csv -d <dir>/f -o f -- python -u bench.py --workload {w} {pa}",
+             f"    rocprofv3 --pmc WRITE_SIZE -f csv -d <dir>/w -o w -- python -u bench.py --workload {w} {pa}",
              "    rocprofv3 --pmc FETCH_SIZE -f csv -d <dir>/cf -o cf -- python -u bench.py --copy-calibration",
              "    rocprofv3 --pmc WRITE_SIZE -f csv -d <dir>/cw -o cw -- python -u bench.py --copy-calibration",
              f"    python bench.py --rocprof-summary <dir> --workload {w}",
@@ -557,6 +565,8 @@ def main():
     ap.add_argument("--no-multimesh", action="store_true",
                     help="skip the 32-mesh union companion run")
     ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--compensated", action="store_true",
+                    help="single precision: keep each stored value's rounding remainder (tracer_set_compensated)")
     ap.add_argument("--rounding", default="stochastic", choices=["stochastic", "nearest"],
                     help="rounding of the tracer update in the single-precision modes")
     ap.add_argument("--no-batched-accuracy", action="store_true",
@@ -610,7 +620,7 @@ def main():
         f"(generated in {time.perf_counter() - t0:.1f} s)")
     r = run(case, prec, kb, steps, args.warmup, graph, lc, tune=not args.no_autotune, fused=fused,
             stochastic=args.rounding == "stochastic", fused_limiter=bool(args.fused_limiter),
-            concurrent=bool(args.concurrent))
+            concurrent=bool(args.concurrent), compensated=args.compensated and prec != 0)
     log(f"timed {steps} steps: wall {r['wall_s'] * 1e3:.2f} ms, device {r['dev_ms']:.2f} ms, "
         f"{r['value']:.3e} updates/s, stages {r['stage_ms']}")
     roof = roofline(case, prec, r, args.workload)

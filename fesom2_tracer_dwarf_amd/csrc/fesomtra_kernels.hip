@@ -558,7 +558,8 @@ __global__ __launch_bounds__(256) void k_qflux(const MeshCol<M> m) {
 #define FESOMTRA_DEG_CACHE_WIDE 6
 #endif
 // An entry's level range (the edge's scatter range) is kept as bottom | top << 8 (levels < 256,
-// checked at setup).  GEN: kernels for meshes with cavities (the top is 1 otherwise, and the
+// checked at setup); a slot past the node's degree holds 0, so it is active at no level (no
+// separate test of the degree, whose per-slot masks would take scalar registers).  GEN: kernels for meshes with cavities (the top is 1 otherwise, and the
 // other kernels test the bottom only).
 template <bool PACK, int NC, bool GEN = false>
 struct NodeEdges {
@@ -624,7 +625,7 @@ template <bool PACK, int NC, bool GEN, typename M, typename F>
 __device__ __forceinline__ void for_edges(const MeshCol<M> &m, const NodeEdges<PACK, NC, GEN> &ne, int nz, F &&f) {
 #pragma unroll
     for (int i = 0; i < NC; ++i)
-        if (i < ne.deg && ne.on(i, nz)) f(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
+        if (ne.on(i, nz)) f(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
     for (int i = NC; i < ne.deg; ++i) {
         const int e2 = cld(m.ne_e2, ne.b + i);
         if (entry_on<GEN>(m, ne.b + i, nz))
@@ -657,7 +658,7 @@ template <bool PACK, int NC, bool GEN, typename LOAD, typename V>
 __device__ __forceinline__ void edges_pre(const NodeEdges<PACK, NC, GEN> &ne, int nz, LOAD &&load, V (&v)[NC]) {
 #pragma unroll
     for (int i = 0; i < NC; ++i)
-        if (i < ne.deg && ne.on(i, nz)) v[i] = load(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
+        if (ne.on(i, nz)) v[i] = load(ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
 }
 // As edges_pre, but loading every cached slot whatever the level (no branch around a load, so
 // the loads are issued back to back); a slot past the node's degree names edge 0 and node 0.
@@ -672,7 +673,7 @@ __device__ __forceinline__ void edges_use(const MeshCol<M> &m, const NodeEdges<P
                                           LOAD &&load, USE &&use) {
 #pragma unroll
     for (int i = 0; i < NC; ++i)
-        if (i < ne.deg && ne.on(i, nz)) use(v[i], ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
+        if (ne.on(i, nz)) use(v[i], ne.e2[i] >> 1, (ne.e2[i] & 1) != 0, ne.far_of(i));
     for (int i = NC; i < ne.deg; ++i) {
         const int e2 = cld(m.ne_e2, ne.b + i);
         if (entry_on<GEN>(m, ne.b + i, nz)) {
@@ -1705,7 +1706,7 @@ __global__ __launch_bounds__(256, FESOMTRA_MIN_WAVES_LIMIT) void k_limit(const M
     auto edges = [&](int nz, auto &&f) {
 #pragma unroll
         for (int i = 0; i < NC; ++i)
-            if (i < deg && nz <= (ctop[i] & 0xffff)) f(i, ce2[i] >> 1, (ce2[i] & 1) != 0, ctop[i] >> 16);
+            if (nz <= (ctop[i] & 0xffff)) f(i, ce2[i] >> 1, (ce2[i] & 1) != 0, ctop[i] >> 16);
         for (int i = NC; i < deg; ++i) {
             if (nz > cld(m.ne_top, eb + i)) continue;
             const int e2 = cld(m.ne_e2, eb + i);
