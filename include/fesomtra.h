@@ -211,16 +211,17 @@ int tracer_get_work(const char *name, double *out, size_t count);
  * tracer_get_work then returns -1 for those two arrays; "fct_LO" is always kept). */
 int tracer_set_keep_work(int keep);
 /* Fused limiter (enable = 1; default 0): the limiter bounds and ratios, the limiting and the
- * update of FP64 tracer blocks of width <= 4 run as one kernel on node tiles with two-ring halos,
- * which forms the ratios of the tiles' first ring itself instead of storing and re-reading them;
- * the values then alternate between two buffers per step (the stored values are always the
- * primary buffer's after a call).  Results are bit-identical either way.  It applies when every
- * pass has the FCT limiter, there are no cavities, per-node time steps, ensembles or AB2 time
- * levels and every tile's two rings fit its staging slots (in practice width 1);
+ * update of FP64 tracer blocks of width <= 4 run as one kernel on tiles of 128 / width nodes whose
+ * first ring has threads of its own, which form that ring's ratios themselves instead of storing
+ * and re-reading them; the values then alternate between two buffers per step (the stored values
+ * are always the primary buffer's after a call).  Results are bit-identical either way.  It
+ * applies when every pass has the FCT limiter, there are no cavities, per-node time steps,
+ * ensembles or AB2 time levels and every tile's two rings fit its threads and staging slots;
  * tracer_get_fused_limiter reports whether the next steps use it (1) or the two kernels (0).  On
- * MI355X it is slower than the two kernels (1.83 against 1.26 ms per tracer-step on a
- * 1000x1000x48 mesh at width 1: its register footprint leaves two waves per SIMD), so it is off
- * by default. */
+ * MI355X it is slower than the two kernels (1000x1000x48 mesh: 1.36 against 1.26 ms per
+ * tracer-step at width 1, 2.62 against 2.17 ms for a block of two: the first ring adds about half
+ * again to the ratio work, and the fused state leaves three waves per SIMD), so it is off by
+ * default. */
 /* Concurrent low-order kernels (enable = 1, the default): with the FCT limiter the low-order solution
  * (and the implicit split) runs on a second stream of the library beside the gradient and
  * edge-flux kernels, which it does not depend on, joined before the limiter bounds; 0 runs every
