@@ -639,6 +639,32 @@ int build_halo_tiles(State &s) {
     if (upload(s, &m.ht_loc, loc) || upload(s, &m.ht_hptr, hptr) || upload(s, &m.ht_halo, halo) ||
         upload(s, &m.ht_lmax, lmax))
         return -1;
+    // the edges of each tile's nodes (sorted), whose transports the low-order kernel stages per
+    // level (at most 1024 per tile at width 1, 512 at widths 2 and 4; a mesh where a tile has more
+    // runs untiled), and per CSR entry its edge's slot in that list (* 2 + whether the node is the
+    // edge's second end)
+    {
+        const int qs = s.kb == 1 ? 4 : 2;
+        std::vector<int> eptr(nt + 1, 0), edges, eloc(s.h_e2.size()), el;
+        bool fits = true;
+        for (int t = 0; t < nt && fits; ++t) {
+            const int a0 = t * tc, a1 = std::min(N, a0 + tc);
+            el.clear();
+            for (int i = s.h_ptr[a0]; i < s.h_ptr[a1]; ++i) el.push_back(s.h_e2[i] >> 1);
+            std::sort(el.begin(), el.end());
+            el.erase(std::unique(el.begin(), el.end()), el.end());
+            if ((int)el.size() > qs * 256) fits = false;
+            for (int i = s.h_ptr[a0]; i < s.h_ptr[a1]; ++i)
+                eloc[i] = ((int)(std::lower_bound(el.begin(), el.end(), s.h_e2[i] >> 1) - el.begin()) << 1) |
+                          (s.h_e2[i] & 1);
+            edges.insert(edges.end(), el.begin(), el.end());
+            eptr[t + 1] = (int)edges.size();
+        }
+        m.ht_eptr = m.ht_edges = m.ht_eloc = nullptr;
+        if (!fits) return 0;
+        if (edges.empty()) edges.push_back(0);
+        if (upload(s, &m.ht_eptr, eptr) || upload(s, &m.ht_edges, edges) || upload(s, &m.ht_eloc, eloc)) return -1;
+    }
     s.ht_built = nt;
     m.ht_kb = s.kb;
     m.ht_n = s.halo_tiles ? nt : 0;

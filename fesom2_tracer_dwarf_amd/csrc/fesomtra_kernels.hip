@@ -707,6 +707,11 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
                                                                  bool wsplit) {
     using D = double;
     __shared__ D sT[TILED ? 2 : 1][TILED ? kHtSlots : 1];
+    // tiled form with one velocity member: the transports of the tile's edges are staged per
+    // level too (QS per thread), fetched a level ahead with the neighbour values
+    constexpr bool QST = TILED && !GEN;
+    constexpr int QS = KB == 1 ? 4 : 2;
+    __shared__ double sQ[QST ? 2 : 1][QST ? QS * 256 : 1];
     HaloTile<KB, M> ht;
     int n, lane, nzmax, L, top, nz0, nz1, lv0, lv1;
     bool act = true;
@@ -765,10 +770,34 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
     // staging registers of the next level (tiled form; the neighbour values of one level in
     // flight while a level is computed: staging two levels behind one barrier measured slower)
     D st[2];
+    // staged transports: this thread's edges of the tile (slot j * 256 + thread; a slot past the
+    // tile's edges loads edge 0 and is never read), and each cached entry's slot * 2 + second
+    int qe[QS], ce[decltype(ne)::kN];
+    double sq[QS];
+    if constexpr (QST) {  // the host builds tiles only where every tile's edges fit the slots
+        const int t = ht.n0 / HaloTile<KB, M>::TC, eb = cld(m.ht_eptr, t), ec = cld(m.ht_eptr, t + 1) - eb;
+#pragma unroll
+        for (int j = 0; j < QS; ++j) {
+            const int sl = j * 256 + (int)threadIdx.x;
+            qe[j] = sl < ec ? cld(m.ht_edges, eb + sl) : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < decltype(ne)::kN; ++i) ce[i] = (act && i < ne.deg) ? cld(m.ht_eloc, ne.b + i) : 0;
+    }
     auto fetch = [&](int nz) {
         ht.each([&](int i, int node) { st[i] = ldc<D>(p.T, ((size_t)(nz - 1) * Np + node) * KB + lane); });
+        if constexpr (QST) {
+#pragma unroll
+            for (int j = 0; j < QS; ++j) sq[j] = (double)cld(m.q, (size_t)(nz - 1) * Ep + qe[j]);
+        }
     };
-    auto stage = [&](int nz) { ht.each([&](int i, int) { sT[nz & 1][threadIdx.x + 256 * i] = st[i]; }); };
+    auto stage = [&](int nz) {
+        ht.each([&](int i, int) { sT[nz & 1][threadIdx.x + 256 * i] = st[i]; });
+        if constexpr (QST) {
+#pragma unroll
+            for (int j = 0; j < QS; ++j) sQ[nz & 1][j * 256 + threadIdx.x] = sq[j];
+        }
+    };
     // The per-lane loads of a layer (the window's next value, the next layer's value and, untiled,
     // the far values of the cached edges with their transports) are issued together before any
     // is used: unconditionally, at a level clamped into the column, the value then selected, so
@@ -831,7 +860,17 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_lo(const MeshCol<M
             }
             s = second ? s - f : s + f;
         };
-        if constexpr (TILED) {
+        if constexpr (QST) {
+            // transports and neighbour values from the tile, in reference edge order
+#pragma unroll
+            for (int i = 0; i < decltype(ne)::kN; ++i)
+                if (ne.on(i, nz)) add_edge(sT[buf][ne.far_of(i) * KB + lane], sQ[buf][ce[i] >> 1], (ce[i] & 1) != 0);
+            for (int i = decltype(ne)::kN; i < ne.deg; ++i) {
+                if (!entry_on<GEN>(m, ne.b + i, nz)) continue;
+                const int c = cld(m.ht_eloc, ne.b + i);
+                add_edge(sT[buf][cld(m.ht_loc, ne.b + i) * KB + lane], sQ[buf][c >> 1], (c & 1) != 0);
+            }
+        } else if constexpr (TILED) {
             // transports from memory, issued together; neighbour values from the tile
             for_edges_ld<true>(
                 m, ne, nz, [&](int e, bool, int) { return (double)cld(m.q, mb.at(eo + e)); },

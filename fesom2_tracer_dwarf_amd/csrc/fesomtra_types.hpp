@@ -145,6 +145,11 @@ struct MeshCol {
     // or 256 / ht_kb + halo position).
     int ht_n, ht_kb;
     const int *ht_loc, *ht_hptr, *ht_halo, *ht_lmax;  // (entries), (ht_n + 1), (halo), (ht_n) wet layers
+    // The edges of each tile's nodes, sorted (ht_eptr, ht_edges; at most 1024 per tile at width 1,
+    // 512 at widths 2 and 4, else no tiles are built), and per CSR entry its edge's position in
+    // that list * 2 + whether the node is the edge's second end (ht_eloc): the low-order kernel
+    // stages the tile's transports level by level.
+    const int *ht_eptr, *ht_edges, *ht_eloc;
     // Edge tiles of the edge-flux kernel (same widths): tile t owns the 256 / et_kb consecutive
     // edges from t * 256 / et_kb and lists the nodes they read (ends and up/down-wind triangle
     // corners); et_loc holds per edge the slots of n1, n2, u0, u1, u2, d0, d1, d2 (16 bits each).
