@@ -1408,8 +1408,11 @@ constexpr bool kFinalPre = FESOMTRA_FINAL_PRE != 0;
 template <bool TILED, int KB, typename C>
 constexpr bool kFinalPreFor = kFinalPre && !TILED && (KB == 64 || !std::is_same<C, double>::value);
 // AB2: also writes the next step's valuesAB and valuesold (a separate instantiation, so that the
-// other runs keep their register budget).
-template <bool TILED, bool FCT, int KB, typename S, typename M, typename C, bool AB2 = false, bool GEN = false>
+// other runs keep their register budget).  LEAN (one column per wavefront): a pass without the
+// drop-in's accumulators or kept fluxes, compiled without them, whose pointer tests would
+// otherwise hold scalar registers through the level loop.
+template <bool TILED, bool FCT, int KB, typename S, typename M, typename C, bool AB2 = false, bool GEN = false,
+          bool LEAN = false>
 __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCol<M> m, const StepPtrs<S, C> p, double dt_step, int lc) {
     static_assert(!TILED || FCT, "tiles stage the limiter ratios");
     constexpr bool PRE = kFinalPreFor<TILED, KB, C>;
@@ -1469,7 +1472,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
             const C rpa = nz0 > ctop ? p.Rp[X(c0 - Np)] : C(0);
             const C rma = nz0 > ctop ? p.Rm[X(c0 - Np)] : C(0);
             top = limit(a0, nz0, rpa, rma, rp1, rm1);
-            if (p.dbg_v) p.dbg_v[c0] = top;
+            if (!LEAN && p.dbg_v) p.dbg_v[c0] = top;
         } else {
             top = a0;
         }
@@ -1565,7 +1568,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
         const double tval = TILED ? own.tval : (double)ldc<C>(p.T, X(cell));
         const C ab = TILED ? own.ab : p.adfv[X(cell + Np)];
         C bot;
-        double dv = (p.acc_v && !p.acc_init) ? p.acc_v[cell] : 0.0;
+        double dv = (!LEAN && p.acc_v && !p.acc_init) ? p.acc_v[cell] : 0.0;
         if (FCT) {
             // single precision: LO = T + increment (+ the increment's remainder with compensated
             // storage, so that the low-order change of the content is the fluxes' to FP64 round-off)
@@ -1588,13 +1591,13 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
             } else {
                 bot = ab;  // bottom interface: never limited (always zero)
             }
-            if (p.dbg_v) p.dbg_v[cell + Np] = bot;
+            if (!LEAN && p.dbg_v) p.dbg_v[cell + Np] = bot;
         } else {
             bot = ab;
         }
         const double yV = kDivReuse<KB> ? 1.0 / V : 0.0;
         dv = dv + (kDivReuse<KB> ? div_reuse(((double)top - (double)bot) * dt, V, yV) : ((double)top - (double)bot) * dt / V);
-        double dh = (p.acc_h && !p.acc_init) ? p.acc_h[cell] : 0.0;
+        double dh = (!LEAN && p.acc_h && !p.acc_init) ? p.acc_h[cell] : 0.0;
         const C rpn = FCT ? rp0 : C(0), rmn = FCT ? rm0 : C(0);
         const double an = kAddUpw ? ldc<double>(p.TAB, X(cell)) : 0.0;
         auto add_edge = [&](const EdgeVals<C> &v, int e, bool second) {
@@ -1612,7 +1615,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
                     ae = cmin(ae, rp_2);
                 }
                 f = ae * f;
-                if (p.dbg_h && !second) p.dbg_h[eo + e] = f;
+                if (!LEAN && p.dbg_h && !second) p.dbg_h[eo + e] = f;
             }
             double fd = (double)f;
             if (kAddUpw) fd = fd + upw1_edge<double>(second ? v.ao : an, second ? an : v.ao, v.q);
@@ -1632,7 +1635,7 @@ __global__ __launch_bounds__(256, (kWaves<TILED, KB>)) void k_final(const MeshCo
             for_edges_ld<kTwoPhase<KB>>(m, ne, nz, edge_load,
                                         [&](const EdgeVals<C> &v, int e, bool second, int) { add_edge(v, e, second); });
         }
-        if (p.acc_h) {
+        if (!LEAN && p.acc_h) {
             p.acc_h[cell] = dh;
             p.acc_v[cell] = dv;
         } else if (p.resid) {
@@ -2224,6 +2227,8 @@ static hipError_t step_kb(const MeshCol<M> &m, const StepPtrs<S, C> &p, const Sc
             LAUNCH_CHECK();
             MARK(3);
             if (p.ab_next) k_final<false, true, KB, S, M, C, true, GEN><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
+            else if (KB == 64 && !p.acc_h && !p.dbg_h && !p.dbg_v)  // no accumulators or kept fluxes
+                k_final<false, true, KB, S, M, C, false, GEN, true><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
             else k_final<false, true, KB, S, M, C, false, GEN><<<gf, 256, 0, s>>>(m, p, sc.dt, cfg.lc_final);
         }
     } else {
