@@ -358,6 +358,55 @@ def _counter_means(d: Path, counter: str):
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
+SQ_PASSES = ("SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_VMEM "
+             "SQ_ACTIVE_INST_LDS SQ_INSTS_VALU",
+             "SQ_WAVES SQ_INSTS_VMEM_RD SQ_INSTS_LDS SQ_INSTS_VALU_FMA_F64 SQ_INSTS_VALU_ADD_F64 SQ_INSTS_VALU_MUL_F64 "
+             "SQ_INSTS_VALU_TRANS_F64 SQ_INSTS_SALU")
+
+
+def sq_summary(args):
+    """profiles/sq_<workload>.txt from the two rocprofv3 SQ counter passes DIR/s1 and DIR/s2 of
+    `bench.py --workload W PROFILE_ARGS` (counters summed over the GPU, averaged over launches)."""
+    d = Path(args.sq_summary)
+    w = args.workload
+    acc = {}
+    for sub in ("s1", "s2"):
+        for r in _rocprof_rows(d / sub, "counter_collection.csv"):
+            k = _short(r["Kernel_Name"])
+            if not k.startswith("fesomtra::"):
+                continue
+            acc.setdefault((k, r["Kernel_Name"]), {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    pa = profile_args(w)
+    lines = [f"# {w}: SQ counters of bench.py ({pa})", "",
+             "Commands (two counter passes of the same program):"]
+    for i, c in enumerate(SQ_PASSES, 1):
+        lines.append(f"    rocprofv3 --pmc {c} -f csv -d <dir>/s{i} -o s{i} -- python -u bench.py --workload {w} {pa}")
+    lines += [f"    python bench.py --sq-summary <dir> --workload {w}",
+              f"Kernel source hash {kernel_source_hash()}", "",
+              "Per kernel instantiation: fraction of wavefront cycles spent waiting (any dependency / for an",
+              "instruction to issue), cycles with a VALU, vector-memory or LDS instruction active per wavefront",
+              "cycle, and instructions per wavefront (VALU, of which FP64 add/mul/fma/transcendental; vector",
+              "loads; LDS; scalar).", "",
+              f"{'kernel':<60}{'wait':>6}{'waitI':>6}{'valu':>6}{'vmem':>6}{'lds':>6}{'VALU/w':>8}{'F64%':>6}"
+              f"{'VMRD/w':>8}{'LDS/w':>7}{'SALU/w':>8}"]
+    f64n = ("SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_TRANS_F64")
+    for (k, full), c in sorted(acc.items()):
+        m = {n: sum(v) / len(v) for n, v in c.items()}
+        wc, nw = m.get("SQ_WAVE_CYCLES", 0.0), m.get("SQ_WAVES", 0.0)
+        if wc <= 0 or nw <= 0:
+            continue
+        valu = m.get("SQ_INSTS_VALU", 0.0)
+        name = full.split("(")[0].replace("void ", "")[:59]
+        lines.append(f"{name:<60}{m.get('SQ_WAIT_ANY', 0) / wc:>6.2f}{m.get('SQ_WAIT_INST_ANY', 0) / wc:>6.2f}"
+                     f"{m.get('SQ_ACTIVE_INST_VALU', 0) / wc:>6.2f}{m.get('SQ_ACTIVE_INST_VMEM', 0) / wc:>6.2f}"
+                     f"{m.get('SQ_ACTIVE_INST_LDS', 0) / wc:>6.2f}{valu / nw:>8.0f}"
+                     f"{100 * sum(m.get(x, 0) for x in f64n) / max(valu, 1):>6.1f}{m.get('SQ_INSTS_VMEM_RD', 0) / nw:>8.0f}"
+                     f"{m.get('SQ_INSTS_LDS', 0) / nw:>7.0f}{m.get('SQ_INSTS_SALU', 0) / nw:>8.0f}")
+    out = REPO / "profiles" / f"sq_{w}.txt"
+    out.write_text("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def rocprof_summary(args):
     """profiles/kernels_<workload>.txt and profiles/traffic_<workload>.json from rocprofv3 output
     directories under DIR: k (kernel trace, --stats), f (--pmc FETCH_SIZE), w (--pmc WRITE_SIZE) of
@@ -577,9 +626,14 @@ def main():
                     help="only run a plain 1 GiB device-to-device copy (rocprof counter scale)")
     ap.add_argument("--rocprof-summary", default="",
                     help="summarise rocprofv3 CSV directories DIR/{k,f,w,cf,cw} into profiles/ (no GPU)")
+    ap.add_argument("--sq-summary", default="",
+                    help="summarise the two SQ counter passes DIR/{s1,s2} into profiles/sq_<workload>.txt (no GPU)")
     args = ap.parse_args()
     if args.rocprof_summary:
         rocprof_summary(args)
+        return
+    if args.sq_summary:
+        sq_summary(args)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
