@@ -579,6 +579,25 @@ def precision_study(case, steps: int, modes=STUDY_ALL):
             "reference": "FP64 GPU path", "results": results}
 
 
+def size_sweep(sizes, steps: int, warmup: int) -> None:
+    """FP64 throughput against mesh size: n x n periodic meshes with 48 levels from the generator
+    and seed of the 1000x1000x48 workload (random node numbering, smooth bathymetry), 2 tracers
+    MFCT/QR4C/FCT, 48 levels per thread, the timed steps replayed as one HIP graph.  One JSON
+    line per size on stdout."""
+    from fesom2_tracer_dwarf_amd import configs
+    for n in sizes:
+        t0 = time.perf_counter()
+        case = configs.bathymetry_case(f"mesh{n}", n, 48, seed=1, ntracers=2, nsteps=steps)
+        m = case.mesh
+        log(f"size {n}: N_L={m.n_node_levels} (generated in {time.perf_counter() - t0:.1f} s)")
+        r = run(case, 0, 0, steps, warmup, graph=True, lc=[48, 48, 48, 48], tune=False)
+        row = {"n": n, "node_levels": m.n_node_levels, "ms_per_step": r["wall_s"] * 1e3 / steps,
+               "updates_per_s": r["value"], "stage_ms": [round(x, 3) for x in r["stage_ms"]],
+               "qflux_ms": r["qflux_ms"], "block_width": r["block_width"]}
+        log(f"size {n}: {row['ms_per_step']:.3f} ms/step, {r['value']:.3e} updates/s, stages {row['stage_ms']}")
+        print(json.dumps(row), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -622,6 +641,9 @@ def main():
                     help="skip the FP64 run of the batched workload used for its error")
     ap.add_argument("--precision-study", action="store_true")
     ap.add_argument("--study-steps", type=int, default=1000)
+    ap.add_argument("--size-sweep", default="",
+                    help="comma-separated n: FP64 throughput on n x n x 48 meshes of the 1000x1000x48 "
+                         "workload's generator (2 tracers, 48 levels per thread, HIP graph of 20 steps)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (rocprof counter scale)")
     ap.add_argument("--rocprof-summary", default="",
@@ -655,6 +677,9 @@ def main():
     if args.precision_study:
         from fesom2_tracer_dwarf_amd import configs
         print(json.dumps(precision_study(configs.mesh1000(args.study_steps), args.study_steps)), flush=True)
+        return
+    if args.size_sweep:
+        size_sweep([int(x) for x in args.size_sweep.split(",")], args.steps or 20, args.warmup)
         return
 
     defaults = {"mesh1000": (100, 0, 0, False), "mesh500x64": (200, 1, 64, False),
