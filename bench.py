@@ -478,6 +478,57 @@ def rocprof_summary(args):
     print("\n".join(lines))
 
 
+def dropin_summary(d: Path, steps: int) -> None:
+    """profiles/kernels_dropin_mesh1000.txt from the kernel trace DIR/k of `bench.py --dropin-only
+    --steps S`: the kernels' durations, and over the timed steps (from the end of the warm-up step's
+    last update kernel to the end of the last one) the elapsed time and the time the GPU was busy
+    (union of the kernel intervals: the low-order kernel runs on a second stream beside the edge
+    kernels, so their durations overlap and do not add up)."""
+    stats = {}
+    for r in _rocprof_rows(d / "k", "kernel_stats.csv"):
+        k = _short(r["Name"])
+        calls, tot = int(r["Calls"]), float(r["TotalDurationNs"])
+        c0 = stats.get(k, (0, 0.0, float("inf"), 0.0))
+        stats[k] = (c0[0] + calls, c0[1] + tot, min(c0[2], float(r["MinNs"])), max(c0[3], float(r["MaxNs"])))
+    trace = sorted(_rocprof_rows(d / "k", "kernel_trace.csv"), key=lambda r: int(r["Start_Timestamp"]))
+    fin = [r for r in trace if _short(r["Kernel_Name"]) == "fesomtra::k_final"]
+    ntr = (len(fin) // (steps + 1)) or 1  # tracers per step (one warm-up step before the timed ones)
+    t0, t1 = int(fin[ntr - 1]["End_Timestamp"]), int(fin[-1]["End_Timestamp"])
+    win = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"]), _short(r["Kernel_Name"])) for r in trace
+           if int(r["Start_Timestamp"]) >= t0 and int(r["End_Timestamp"]) <= t1]
+    busy, cs, ce = 0, None, None
+    for st, en, _ in sorted(win):
+        if ce is None or st > ce:
+            busy += 0 if ce is None else ce - cs
+            cs, ce = st, en
+        else:
+            ce = max(ce, en)
+    busy += 0 if ce is None else ce - cs
+    nsteps = (len(fin) - ntr) // ntr
+    per_kernel = {}
+    for st, en, k in win:
+        per_kernel.setdefault(k, []).append(en - st)
+    cmd = f"python -u bench.py --dropin-only --steps {steps}"
+    lines = [f"# do_oce_adv_tra drop-in sequence for T and S, 1000x1000x48 FP64, block width 1 ({cmd})", "",
+             "Commands:",
+             f"    rocprofv3 --kernel-trace --stats -f csv -d <dir>/k -o k -- {cmd}",
+             f"    python bench.py --dropin-summary <dir> --steps {steps}",
+             f"Kernel source hash {kernel_source_hash()}", "",
+             "Whole run (set-up and teardown copies included):",
+             f"{'kernel':<34}{'calls':>7}{'avg_us':>10}{'min_us':>10}{'max_us':>10}"]
+    for k, (calls, tot, mn, mx) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        lines.append(f"{k:<34}{calls:>7}{tot / calls / 1e3:>10.1f}{mn / 1e3:>10.1f}{mx / 1e3:>10.1f}")
+    lines += ["", f"Timed steps ({nsteps} steps of {ntr} tracers, one do_oce_adv_tra call per tracer):",
+              f"{'kernel':<34}{'calls':>7}{'avg_us':>10}"]
+    for k, v in sorted(per_kernel.items(), key=lambda kv: -sum(kv[1])):
+        lines.append(f"{k:<34}{len(v):>7}{sum(v) / len(v) / 1e3:>10.1f}")
+    lines += ["", f"elapsed {(t1 - t0) / 1e6 / max(nsteps, 1):.3f} ms per step, GPU busy "
+              f"{busy / 1e6 / max(nsteps, 1):.3f} ms per step (union of kernel intervals); the velocity is "
+              "unchanged, so k_qflux ran once, before the timed steps"]
+    (REPO / "profiles" / "kernels_dropin_mesh1000.txt").write_text("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def dropin_timing(case, kb: int, steps: int = 10, keep_work: bool = False):
     """The reference's per-tracer do_oce_adv_tra call sequence on device arrays in reference order
     (the Fortran call sequence of the integration notes with the device-pointer entry points), per
@@ -644,6 +695,12 @@ def main():
     ap.add_argument("--size-sweep", default="",
                     help="comma-separated n: FP64 throughput on n x n x 48 meshes of the 1000x1000x48 "
                          "workload's generator (2 tracers, 48 levels per thread, HIP graph of 20 steps)")
+    ap.add_argument("--dropin-only", action="store_true",
+                    help="time only the do_oce_adv_tra drop-in sequence for T and S on the 1000x1000x48 "
+                         "workload (block width 1, device arrays)")
+    ap.add_argument("--dropin-summary", default="",
+                    help="summarise the rocprofv3 kernel trace DIR/k of --dropin-only into "
+                         "profiles/kernels_dropin_mesh1000.txt (no GPU)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (rocprof counter scale)")
     ap.add_argument("--rocprof-summary", default="",
@@ -656,6 +713,9 @@ def main():
         return
     if args.sq_summary:
         sq_summary(args)
+        return
+    if args.dropin_summary:
+        dropin_summary(Path(args.dropin_summary), args.steps or 10)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -677,6 +737,14 @@ def main():
     if args.precision_study:
         from fesom2_tracer_dwarf_amd import configs
         print(json.dumps(precision_study(configs.mesh1000(args.study_steps), args.study_steps)), flush=True)
+        return
+    if args.dropin_only:
+        case = make_case("mesh1000")
+        steps = args.steps or 10
+        ms, w = dropin_timing(case, 1, steps=steps)
+        log(f"drop-in sequence, block width {w}: {ms:.3f} ms per step")
+        print(json.dumps({"workload": describe(case), "dropin_ms_per_step": ms, "block_width": w,
+                          "steps": steps, "tracers": case.ntracers}), flush=True)
         return
     if args.size_sweep:
         size_sweep([int(x) for x in args.size_sweep.split(",")], args.steps or 20, args.warmup)
