@@ -166,8 +166,9 @@ def kernel_source_hash() -> str:
     return h.hexdigest()[:16]
 
 
-def traffic_from_profiles(name: str):
-    """HBM bytes of one step from profiles/traffic_<name>.json, if measured on these sources."""
+def traffic_from_profiles(name: str, passes: int = None):
+    """HBM bytes of one step from profiles/traffic_<name>.json, if measured on these sources and,
+    when `passes` is given, with as many passes per step (tracers / block width) as this run."""
     f = REPO / "profiles" / f"traffic_{name}.json"
     if not f.exists():
         return None, f"no {f.relative_to(REPO)}"
@@ -175,6 +176,9 @@ def traffic_from_profiles(name: str):
     if d.get("source_hash") != kernel_source_hash():
         return None, (f"{f.relative_to(REPO)} was measured on other kernel sources "
                       f"({d.get('source_hash')} != {kernel_source_hash()})")
+    if passes is not None and d.get("passes_per_step") not in (None, passes):
+        return None, (f"{f.relative_to(REPO)} was measured with {d['passes_per_step']} passes per step, "
+                      f"this run makes {passes}")
     return d.get("hbm_bytes_per_step"), f"{f.relative_to(REPO)} (source hash {d['source_hash']})"
 
 
@@ -269,6 +273,7 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
     qflux_ms, step_ms, nsteps_timed = t.step_times()
     t.lib.tracer_set_kernel_timing(0)
     cfg = t.launch_config()
+    tiles = t.halo_tiles()
     t.set_graph(False)
     t.set_fused(False)
     t.set_wsplit(False)
@@ -279,7 +284,8 @@ def run(case, prec: int, kb: int, steps: int, warmup: int, graph: bool = False, 
     return {"value": updates / wall, "wall_s": wall, "dev_ms": dev_ms, "stage_ms": stage_ms,
             "passes_timed": passes, "block_width": width, "drift": drift, "launch_config": cfg,
             "values": kept, "error": err, "autotune": tuned, "layout": layout, "qflux_ms": qflux_ms,
-            "step_ms": step_ms, "steps_timed": nsteps_timed, "fused_limiter": limiter, "steps": steps}
+            "step_ms": step_ms, "steps_timed": nsteps_timed, "fused_limiter": limiter, "steps": steps,
+            "tiles": tiles}
 
 
 def roofline(case, prec: int, r: dict, name: str, members: int = 1):
@@ -295,7 +301,7 @@ def roofline(case, prec: int, r: dict, name: str, members: int = 1):
     achieved = B / (step_ms * 1e-3) / 1e9
     st = r["stage_ms"]
     passes = max(1, round(r["passes_timed"] / max(r["steps_timed"], 1)))
-    traffic, source = traffic_from_profiles(name)
+    traffic, source = traffic_from_profiles(name, passes)
     fused = not (st[0] or st[1] or st[2])
     lim = r.get("fused_limiter", False)
     if fused:
@@ -318,6 +324,12 @@ def roofline(case, prec: int, r: dict, name: str, members: int = 1):
             "passes_per_step": passes}
 
 
+# Block width of the 32-mesh union: at width 1 every tile of the union fits the tiled kernels'
+# staging space (256 nodes, their halo and edges); at width 4 the 64-node tiles cut along the
+# members' edges into slivers whose halos overflow it, and the union runs untiled (24.4 against
+# 20.8 ms per step of 4 tracers).
+MULTIMESH_WIDTH = 1
+
 PROFILE_ARGS = ("--steps 5 --warmup 1 --no-cpu-baseline --no-batched --no-dropin --no-multimesh --no-precision "
                 "--no-vertical --no-autotune --concurrent 0 "
                 "--lc 48,48,48,48")
@@ -325,9 +337,14 @@ PROFILE_ARGS = ("--steps 5 --warmup 1 --no-cpu-baseline --no-batched --no-dropin
 
 def profile_args(workload: str) -> str:
     """The bench options of the profiled runs: one stream (the kernels' own durations, as the
-    timed runs' autotuned layout), and for the 64-tracer workload the batched line's compensated
-    storage."""
-    return PROFILE_ARGS + (" --compensated" if workload == "mesh500x64" else "")
+    timed runs' autotuned layout); for the 64-tracer and the multi-mesh workloads the compensated
+    storage of their bench lines; the multi-mesh steps launched directly rather than replayed as
+    a graph (the same kernels, each dispatch counted on its own)."""
+    if workload == "mesh500x64":
+        return PROFILE_ARGS + " --compensated"
+    if workload == "multimesh32":
+        return PROFILE_ARGS + " --compensated --no-graph"
+    return PROFILE_ARGS
 
 
 def _rocprof_rows(d: Path, suffix: str):
@@ -473,6 +490,7 @@ def rocprof_summary(args):
     (out / f"traffic_{w}.json").write_text(json.dumps({
         "workload": w, "hbm_bytes_per_step": int(rd + wr), "hbm_read_bytes_per_step": int(rd),
         "hbm_write_bytes_per_step": int(wr), "step_kernels": present, "step_us": us,
+        "passes_per_step": round(stats["fesomtra::k_edge"][0] / nq) if "fesomtra::k_edge" in stats else None,
         "fetch_scale": fscale, "write_scale": wscale, "source_hash": kernel_source_hash(),
         "commit_before_measurement": commit, "source": f"profiles/kernels_{w}.txt"}, indent=1) + "\n")
     print("\n".join(lines))
@@ -661,6 +679,9 @@ def main():
     ap.add_argument("--block-width", type=int, default=0,
                     help="tracers interleaved per block and advanced per kernel pass (0: all)")
     ap.add_argument("--graph", action="store_true", help="replay the timed steps as one HIP graph")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every step's kernels directly, also for the workloads replayed as a graph "
+                         "by default (profiled counter runs)")
     ap.add_argument("--fused", type=int, default=0, choices=[0, 1],
                     help="fused single-pass kernel for FP64 FCT tracers, one tracer per block")
     ap.add_argument("--fused-limiter", type=int, default=0, choices=[0, 1],
@@ -751,13 +772,13 @@ def main():
         return
 
     defaults = {"mesh1000": (100, 0, 0, False), "mesh500x64": (200, 1, 64, False),
-                "multimesh32": (100, 1, 4, True), "analytic20": (10, 0, 0, False)}
+                "multimesh32": (100, 1, MULTIMESH_WIDTH, True), "analytic20": (10, 0, 0, False)}
     d_steps, d_prec, d_kb, d_graph = defaults[args.workload]
     steps = args.steps or d_steps
     prec = d_prec if args.precision is None else args.precision
     fused = bool(args.fused) and prec == 0
     kb = 1 if fused else (args.block_width or d_kb)
-    graph = args.graph or d_graph
+    graph = (args.graph or d_graph) and not args.no_graph
     lc = [int(x) for x in args.lc.split(",")] if args.lc else None
 
     t0 = time.perf_counter()
@@ -769,7 +790,8 @@ def main():
             stochastic=args.rounding == "stochastic", fused_limiter=bool(args.fused_limiter),
             concurrent=bool(args.concurrent), compensated=args.compensated and prec != 0)
     log(f"timed {steps} steps: wall {r['wall_s'] * 1e3:.2f} ms, device {r['dev_ms']:.2f} ms, "
-        f"{r['value']:.3e} updates/s, stages {r['stage_ms']}")
+        f"{r['value']:.3e} updates/s, stages {r['stage_ms']}, block width {r['block_width']}, "
+        f"halo tiles {r['tiles']}")
     roof = roofline(case, prec, r, args.workload)
 
     batched = None
@@ -834,7 +856,7 @@ def main():
         t0 = time.perf_counter()
         mcase = make_case("multimesh32")
         log(f"multi-mesh companion {describe(mcase)} (generated in {time.perf_counter() - t0:.1f} s)")
-        rm = run(mcase, 1, 4, 100, 5, graph=True, tune=not args.no_autotune, compensated=True)
+        rm = run(mcase, 1, MULTIMESH_WIDTH, 100, 5, graph=True, tune=not args.no_autotune, compensated=True)
         dts = sorted(set(np.round(mcase.dt_node, 3).tolist()))
         multimesh = {"workload": describe(mcase), "meshes": 32, "nodes": mcase.mesh.nod2D,
                      "node_levels_per_tracer": mcase.mesh.n_node_levels, "tracers": mcase.ntracers,
@@ -843,7 +865,7 @@ def main():
                                "100 steps replayed as one HIP graph",
                      "steps": 100, "value": rm["value"], "unit": "node-level updates/s",
                      "ms_per_step": rm["wall_s"] * 1e3 / 100, "volume_sum_rel_drift": rm["drift"],
-                     "step_ms_kernels": rm["step_ms"], "block_width": rm["block_width"],
+                     "step_ms_kernels": rm["step_ms"], "block_width": rm["block_width"], "halo_tiles": rm["tiles"],
                      "roofline": roofline(mcase, 1, rm, "multimesh32")}
         log(f"multi-mesh: {rm['value']:.3e} updates/s")
 

@@ -439,15 +439,22 @@ def union_mesh(meshes) -> Mesh:
     """Disjoint union of independent meshes as one mesh (no edge or element joins two of them).
 
     Node, element and edge numbers of mesh i are offset by the counts of meshes 0..i-1; the level
-    axis is padded to the largest nl; coordinates are shifted along x so that the meshes occupy
-    disjoint strips (a space-filling-curve ordering keeps each mesh contiguous).  A step of the
-    union is a step of every member mesh: the stencils never cross between components.
+    axis is padded to the largest nl.  The coordinates place mesh i in cell (i mod G, i div G) of
+    a G x G grid of equal cells (G = ceil(sqrt(count)), a cell 1.5 times the largest member's
+    extent), so the union's bounding box is about square: the library orders nodes along a
+    Hilbert curve scaled to that box per axis, and the 256 / width consecutive nodes of one tile
+    then form compact patches whose halos fit the tiled kernels (a strip of 32 meshes along x,
+    squeezed 32:1 by the scaling, gave long thin tiles whose halos overflowed nearly every tile at
+    width 4, and the whole union ran untiled).  A step of the union is a step of every member mesh:
+    the stencils never cross between components.
     """
     meshes = list(meshes)
     nl = max(m.nl for m in meshes)
     max_adj = max(m.max_adj for m in meshes)
     no = eo = to = 0
-    x_off = 0.0
+    ext = np.array([np.ptp(m.coord_nod2D, axis=0) if m.nod2D > 1 else np.zeros(2) for m in meshes])
+    cell = ext.max(axis=0) * 1.5 + 1e-3
+    cols = int(np.ceil(np.sqrt(len(meshes))))
     parts = {k: [] for k in ("coord", "xy", "enodes", "edges", "etri", "nie", "nie_num", "nlev",
                              "ulev", "nlev_n", "ulev_n", "nlev_min", "ulev_max", "elem_area",
                              "area", "areasvol", "edxdy", "ecd", "ecos", "gsca", "hnode",
@@ -457,12 +464,11 @@ def union_mesh(meshes) -> Mesh:
         a = np.asarray(a, dtype=np.int64)
         return np.where(a > 0, a + off, 0).astype(np.int32)
 
-    for m in meshes:
-        c = m.coord_nod2D.copy()
-        c[:, 0] += x_off
+    for i, m in enumerate(meshes):
+        c = m.coord_nod2D - m.coord_nod2D.min(axis=0)
+        c[:, 0] += (i % cols) * cell[0]
+        c[:, 1] += (i // cols) * cell[1]
         parts["coord"].append(c)
-        width = float(np.ptp(m.coord_nod2D[:, 0])) if m.nod2D > 1 else 0.0
-        x_off += width * 1.5 + 1e-3
         if m.xy_m is not None:
             parts["xy"].append(m.xy_m)
         parts["enodes"].append(shift(m.elem2D_nodes, no))

@@ -154,6 +154,28 @@ def test_union_case_equals_members_in_oracle(oracle_mod):
         off += c.mesh.nod2D
 
 
+def test_union_members_on_a_square_grid_of_disjoint_cells():
+    """The union's coordinates place the members on a near-square grid of disjoint cells (the
+    library orders nodes along a Hilbert curve scaled to the bounding box per axis, so a long strip
+    of meshes would give long thin tiles), and each member keeps its own shape: its coordinates
+    are those of the member shifted."""
+    from fesom2_tracer_dwarf_amd.mesh import union_mesh
+    ms = [periodic_mesh(n, n, 2.0e6, 2.0e6, stretched_layers(4, 5.0, 50.0)) for n in (4, 6, 5, 7, 4, 8, 5, 6, 9, 4)]
+    u = union_mesh(ms)
+    span = np.ptp(u.coord_nod2D, axis=0)
+    assert 0.67 <= span[0] / span[1] <= 1.5
+    boxes, off = [], 0
+    for m in ms:
+        c = u.coord_nod2D[off:off + m.nod2D]
+        d = c - m.coord_nod2D
+        assert np.allclose(d, d[0], rtol=0, atol=1e-12)
+        boxes.append((c.min(axis=0), c.max(axis=0)))
+        off += m.nod2D
+    for i, (lo1, hi1) in enumerate(boxes):
+        for lo2, hi2 in boxes[i + 1:]:
+            assert np.any(hi1 < lo2) or np.any(hi2 < lo1)
+
+
 def test_bathymetry_velocity_keeps_tracers_bounded(oracle_mod):
     """The layer-wise tapered streamfunction is divergence free on the cut dual cells: a uniform
     tracer stays uniform to round-off and T / S stay inside their initial range (FCT with a
