@@ -205,6 +205,27 @@ def test_c4_union_one_step_fp64_matches_members_with_own_dt(c4, oracle_mod):
 
 
 @pytest.mark.timeout(900)
+def test_c4_union_tiled_at_width_1_equals_default_width(c4):
+    """At block width 1 every tile of the union fits the tiled kernels (the members sit on a
+    near-square grid of the union's coordinates, which the node order follows), and one FP64 step
+    through them equals, bit for bit, the step with all four tracers in one block."""
+    from fesom2_tracer_dwarf_amd.tracer import FESOMTracerAdvection
+    _, u = c4
+    say("C4 FP64 1 step of the union at block width 1 and with one block of 4")
+    t = FESOMTracerAdvection()
+    t.set_block_width(1)
+    t.setup_case(u)
+    t.advect_steps(1, u.dt)
+    tiles = t.halo_tiles()
+    got = t.get_all_values()
+    t.set_block_width(0)
+    t.cleanup()
+    say(f"  width 1: {tiles} halo tiles")
+    assert tiles > 0
+    assert np.array_equal(got, run_gpu(u, 1))
+
+
+@pytest.mark.timeout(900)
 def test_c4_100_steps_single_precision_against_fp64(c4):
     _, u = c4
     wet = configs.wet_mask(u.mesh)
